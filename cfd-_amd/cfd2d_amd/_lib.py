@@ -1,0 +1,147 @@
+"""ctypes binding of libcfd2d_hip.so (declarations in include/cfd2d_hip.h).
+
+The library is built in-tree by ``cfd-_amd/build.py``.  There is no CPU
+fallback: importing the simulation classes works without a GPU, but creating a
+device context raises ``RuntimeError`` when the library or a HIP device is
+missing.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import warnings
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libcfd2d_hip.so"
+
+CFD_OK = 0
+CFD_ERR_ARG = -1
+CFD_ERR_HIP = -2
+CFD_ERR_STATE = -3
+CFD_ERR_LIMIT = -4
+
+BC_KINDS = {"noslip": 0, "slip": 1, "inflow": 2, "outflow": 3, "periodic": 4}
+FIELDS = {"u": 0, "v": 1, "p": 2, "F": 3, "G": 4, "hu": 5, "hv": 6, "hu_old": 7, "hv_old": 8,
+          "x": 9}
+SOLVERS = {"pcg64": 0, "mixed": 1}
+
+_d = ctypes.c_double
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_vp = ctypes.c_void_p
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [("nx", _i32), ("ny", _i32), ("dx", _d), ("dy", _d), ("periodic", _i32),
+                ("fluid", _vp), ("bc_kind", _i32 * 4), ("bc_velocity", _d * 4),
+                ("inflow", _vp * 4), ("open", _vp * 4), ("pcode", _vp), ("pcomp", _vp),
+                ("ncomp", _i32), ("comp_count", _vp), ("diag_table", _vp), ("inv_table", _vp)]
+
+
+class ScalarDesc(ctypes.Structure):
+    _fields_ = [("diffusivity", _d), ("limiter", _i32), ("obstacle_fixed", _i32),
+                ("obstacle_value", _d), ("bt", _vp * 4), ("bv", _vp * 4), ("wall_flux", _i32),
+                ("boussinesq", _i32), ("gx", _d), ("gy", _d), ("t_ref", _d)]
+
+
+class StepParams(ctypes.Structure):
+    _fields_ = [("nu", _d), ("tau", _d), ("scheme_ab2", _i32), ("gamma_auto", _i32),
+                ("gamma_fixed", _d), ("dt_fixed", _d), ("dt_model", _d), ("t_end", _d),
+                ("clamp_t_end", _i32), ("solver", _i32), ("fx", _d), ("fy", _d), ("rtol", _d),
+                ("maxiter", _i32), ("check_t_end", _i32), ("reliable_delta", _d), ("steady_tol", _d),
+                ("max_steps", _i64)]
+
+
+class State(ctypes.Structure):
+    _fields_ = [("t", _d), ("dt", _d), ("dt_old", _d), ("gamma", _d), ("flow_force", _d),
+                ("n", _i64), ("has_old", _i32), ("done", _i32), ("error", _i32), ("pad", _i32)]
+
+
+class Record(ctypes.Structure):
+    _fields_ = [("t", _d), ("dt", _d), ("gamma", _d), ("residual", _d), ("divergence", _d),
+                ("kinetic_energy", _d), ("cg_relres", _d), ("mon", _d * 24), ("n", _i64),
+                ("cg_iters", _i32), ("cg_info", _i32), ("cg_outer", _i32), ("nmon", _i32)]
+
+
+_LIB = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load the HIP library; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python cfd-_amd/build.py` "
+                           "(there is no CPU fallback)")
+    L = ctypes.CDLL(str(LIB_PATH))
+    P = ctypes.POINTER
+    sig = {
+        "cfd_last_error": ([], ctypes.c_char_p),
+        "cfd_device_count": ([P(_i32)], _i32),
+        "cfd_ctx_create": ([P(GridDesc), P(_vp)], _i32),
+        "cfd_ctx_destroy": ([_vp], _i32),
+        "cfd_ctx_set_stream": ([_vp, _vp], _i32),
+        "cfd_add_scalar": ([_vp, P(ScalarDesc), P(_i32)], _i32),
+        "cfd_upload": ([_vp, _i32, _vp], _i32),
+        "cfd_download": ([_vp, _i32, _vp], _i32),
+        "cfd_upload_scalar": ([_vp, _i32, _vp], _i32),
+        "cfd_download_scalar": ([_vp, _i32, _vp], _i32),
+        "cfd_set_state": ([_vp, P(State)], _i32),
+        "cfd_get_state": ([_vp, P(State)], _i32),
+        "cfd_apply_bc": ([_vp], _i32),
+        "cfd_project_initial": ([_vp, P(StepParams)], _i32),
+        "cfd_step": ([_vp, P(StepParams), _i32], _i32),
+        "cfd_sync": ([_vp], _i32),
+        "cfd_read_records": ([_vp, _i64, _i32, P(Record)], _i32),
+        "cfd_last_elapsed_ms": ([_vp, P(_d)], _i32),
+        "cfd_pressure_solve": ([_vp, _vp, _vp, _i32, _d, _i32, P(_i32), P(_d)], _i32),
+        "cfd_momentum_tendencies": ([_vp, _vp, _vp, _i32, _i32, _d, _d, _d, _d, _i32, _vp, _vp,
+                                     _vp], _i32),
+        "cfd_obstacle_bc": ([_vp, _vp, _vp, _i32, _i32, _i32, _vp], _i32),
+        "cfd_scalar_transport": ([_vp, _vp, _vp, _vp, _i32, _i32, _d, _d, _vp, _d, _i32, _i32]
+                                 + [_vp] * 8 + [_i32, _d, _vp, _vp], _i32),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _LIB = L
+    return L
+
+
+def exported_symbols() -> list[str]:
+    return [n for n in ("cfd_last_error", "cfd_device_count", "cfd_ctx_create",
+                        "cfd_ctx_destroy", "cfd_ctx_set_stream", "cfd_add_scalar", "cfd_upload",
+                        "cfd_download", "cfd_upload_scalar", "cfd_download_scalar",
+                        "cfd_set_state", "cfd_get_state", "cfd_apply_bc",
+                        "cfd_project_initial", "cfd_step", "cfd_sync", "cfd_read_records",
+                        "cfd_last_elapsed_ms", "cfd_pressure_solve", "cfd_momentum_tendencies",
+                        "cfd_obstacle_bc", "cfd_scalar_transport")]
+
+
+def check(rc: int, what: str = "") -> int:
+    """Map a status code to the reference's conventions: < 0 raises
+    (ValueError for argument/limit errors, RuntimeError for HIP errors);
+    > 0 (solver not converged) warns, as pressure.py:174-176 does."""
+    if rc == CFD_OK:
+        return rc
+    msg = lib().cfd_last_error().decode(errors="replace")
+    if rc > 0:
+        warnings.warn(f"CG-Drucklöser nicht konvergiert ({rc} Iterationen)", stacklevel=3)
+        return rc
+    if rc in (CFD_ERR_ARG, CFD_ERR_LIMIT):
+        raise ValueError(f"{what}: {msg}")
+    raise RuntimeError(f"{what}: {msg}")
+
+
+def device_count() -> int:
+    n = _i32(0)
+    lib().cfd_device_count(ctypes.byref(n))
+    return int(n.value)
+
+
+def ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(_vp)

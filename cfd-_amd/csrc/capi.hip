@@ -1,0 +1,800 @@
+// C ABI of the cfd2d MI355X library (include/cfd2d_hip.h): device context,
+// reference-layout uploads/downloads, and the launch sequence of one
+// projection step.  One translation unit holds the whole library.
+#include "cfd_kernels.h"
+#include "../../include/cfd2d_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "stencil.hip"
+#include "pcg.hip"
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPC(expr)                                                                     \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return set_err(CFD_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int RING = 8192;
+constexpr int BS2D = 256;
+
+struct ScalarDev {
+    cfd_scalar_desc desc;
+    double* phi[2];
+    int64_t* bt[4];
+    double* bv[4];
+};
+
+}  // namespace
+
+struct cfd_ctx {
+    int nx, ny, P;
+    double dx, dy;
+    int periodic;
+    int has_solid;
+    int kind[4];
+    double vel[4];
+    int ncomp;
+    Geo geo;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    int num_cu = 0;
+    int proj_blocks = 0, proj_bs = 0;
+    int red_blocks = 0;
+    // device arrays (pitch P, nx+2 rows)
+    double *u = nullptr, *v = nullptr, *p = nullptr, *F = nullptr, *G = nullptr;
+    double* H[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // [u/v][buffer]
+    double *x = nullptr, *b = nullptr, *r = nullptr, *q = nullptr, *d0 = nullptr, *d1 = nullptr;
+    double *tmp0 = nullptr, *tmp1 = nullptr;
+    float *r32 = nullptr, *q32 = nullptr, *e0 = nullptr, *e1 = nullptr;
+    uint8_t *fl = nullptr, *code = nullptr, *comp = nullptr;
+    double *lut = nullptr, *comp_count = nullptr;
+    double* inflow[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint8_t* open[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *red = nullptr, *part = nullptr;
+    unsigned *bar = nullptr, *ticket = nullptr;
+    unsigned long long* acc = nullptr;
+    DevState* st = nullptr;
+    StepRecord* ring = nullptr;
+    std::vector<ScalarDev> scalars;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(cfd_ctx* c, T** p, size_t n) {
+    void* q = nullptr;
+    HIPC(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
+    HIPC(hipMemset(q, 0, std::max<size_t>(n, 1) * sizeof(T)));
+    c->allocs.push_back(q);
+    *p = (T*)q;
+    return 0;
+}
+
+#define TRY(expr)                 \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_ != 0) return rc_; \
+    } while (0)
+
+size_t cells(const cfd_ctx* c) { return (size_t)(c->nx + 2) * c->P; }
+
+dim3 grid2d(const cfd_ctx* c) { return dim3((c->ny + 2 + BS2D - 1) / BS2D, c->nx + 2); }
+
+int shape_of(const cfd_ctx* c, int field, int* rows, int* cols, int* off_i, int* off_j) {
+    *off_i = 0;
+    *off_j = 0;
+    switch (field) {
+    case CFD_FIELD_U: case CFD_FIELD_F: case CFD_FIELD_HU: case CFD_FIELD_HU_OLD:
+        *rows = c->nx + 1; *cols = c->ny + 2; return 0;
+    case CFD_FIELD_V: case CFD_FIELD_G: case CFD_FIELD_HV: case CFD_FIELD_HV_OLD:
+        *rows = c->nx + 2; *cols = c->ny + 1; return 0;
+    case CFD_FIELD_P:
+        *rows = c->nx + 2; *cols = c->ny + 2; return 0;
+    case CFD_FIELD_X:
+        *rows = c->nx; *cols = c->ny; *off_i = 1; *off_j = 1; return 0;
+    default:
+        return set_err(CFD_ERR_ARG, "unknown field id %d", field);
+    }
+}
+
+int read_state(cfd_ctx* c, DevState* h) {
+    HIPC(hipMemcpyAsync(h, c->st, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+double* field_ptr(cfd_ctx* c, int field, int par) {
+    switch (field) {
+    case CFD_FIELD_U: return c->u;
+    case CFD_FIELD_V: return c->v;
+    case CFD_FIELD_P: return c->p;
+    case CFD_FIELD_F: return c->F;
+    case CFD_FIELD_G: return c->G;
+    // parity par: H[.][par] is written by the next step, H[.][par^1] holds the last step's
+    case CFD_FIELD_HU: return c->H[0][par ^ 1];
+    case CFD_FIELD_HV: return c->H[1][par ^ 1];
+    case CFD_FIELD_HU_OLD: return c->H[0][par ^ 1];
+    case CFD_FIELD_HV_OLD: return c->H[1][par ^ 1];
+    case CFD_FIELD_X: return c->x;
+    default: return nullptr;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+BCArgs bc_args(cfd_ctx* c, bool check_done) {
+    BCArgs a;
+    a.u = c->u;
+    a.v = c->v;
+    a.st = check_done ? c->st : nullptr;
+    for (int s = 0; s < 4; ++s) {
+        a.kind[s] = c->kind[s];
+        a.vel[s] = c->vel[s];
+        a.inflow[s] = c->inflow[s];
+        a.open[s] = c->open[s];
+    }
+    return a;
+}
+
+int launch_bc(cfd_ctx* c, bool check_done) {
+    hipLaunchKernelGGL(k_bc_domain, dim3(1), dim3(1024), 0, c->stream, bc_args(c, check_done),
+                       c->geo);
+    HIPC(hipGetLastError());
+    if (c->has_solid || c->periodic) {
+        hipLaunchKernelGGL(k_obstacle_bc, grid2d(c), dim3(BS2D), 0, c->stream, c->u, c->v, c->fl,
+                           c->geo, check_done ? c->st : (const DevState*)nullptr);
+        HIPC(hipGetLastError());
+    }
+    return 0;
+}
+
+PredArgs pred_args(cfd_ctx* c, const cfd_step_params* prm, int par) {
+    PredArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.u = c->u;
+    a.v = c->v;
+    a.fl = c->fl;
+    a.hu = c->H[0][par];
+    a.hv = c->H[1][par];
+    a.hu_old = c->H[0][par ^ 1];
+    a.hv_old = c->H[1][par ^ 1];
+    a.F = c->F;
+    a.G = c->G;
+    a.T = nullptr;
+    for (auto& s : c->scalars) {
+        if (s.desc.boussinesq) {
+            a.T = s.phi[par];
+            a.gx = s.desc.gx;
+            a.gy = s.desc.gy;
+            a.Tref = s.desc.t_ref;
+        }
+    }
+    a.nu0 = prm ? prm->nu : 0.0;
+    a.fx = prm ? prm->fx : 0.0;
+    a.fy = prm ? prm->fy : 0.0;
+    a.st = c->st;
+    a.scheme_ab2 = prm ? prm->scheme_ab2 : 0;
+    for (int s = 0; s < 4; ++s) {
+        a.ext[s] = c->kind[s] == CFD_BC_OUTFLOW;
+        a.open[s] = c->open[s];
+    }
+    return a;
+}
+
+ProjArgs proj_args(cfd_ctx* c, int method, double rtol, int maxiter) {
+    ProjArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.F = c->F;
+    a.G = c->G;
+    a.u = c->u;
+    a.v = c->v;
+    a.p = c->p;
+    a.code = c->code;
+    a.comp = c->comp;
+    a.lut = c->lut;
+    a.comp_count = c->comp_count;
+    a.ncomp = c->ncomp;
+    a.x = c->x;
+    a.b = c->b;
+    a.r = c->r;
+    a.q = c->q;
+    a.d0 = c->d0;
+    a.d1 = c->d1;
+    a.r32 = c->r32;
+    a.q32 = c->q32;
+    a.d32_0 = c->e0;
+    a.d32_1 = c->e1;
+    a.red = c->red;
+    a.bar = c->bar;
+    a.st = c->st;
+    a.rtol = rtol;
+    a.maxiter = maxiter;
+    a.method = method;
+    a.correct = 1;
+    a.reliable_delta = 0.1;
+    return a;
+}
+
+int launch_project(cfd_ctx* c, ProjArgs a) {
+    if (a.method != PROJ_PCG64)
+        return set_err(CFD_ERR_ARG, "pressure solver %d is not available", a.method);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->bar, 1);
+    HIPC(hipGetLastError());
+    Geo g = c->geo;
+    void* args[] = {(void*)&a, (void*)&g};
+    HIPC(hipLaunchCooperativeKernel((const void*)k_project, dim3(c->proj_blocks),
+                                    dim3(c->proj_bs), args, 0, c->stream));
+    return 0;
+}
+
+ScalarArgs scalar_args(cfd_ctx* c, ScalarDev& s, int par) {
+    ScalarArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.phi_in = s.phi[par];
+    a.phi_out = s.phi[par ^ 1];
+    a.u = c->u;
+    a.v = c->v;
+    a.fl = c->fl;
+    a.diff = nullptr;
+    a.D0 = s.desc.diffusivity;
+    a.limiter = s.desc.limiter;
+    a.obs_t = s.desc.obstacle_fixed ? 1 : 0;
+    a.obs_v = s.desc.obstacle_fixed ? s.desc.obstacle_value : 0.0;
+    a.has_solid = c->has_solid;
+    for (int k = 0; k < 4; ++k) {
+        a.bt[k] = s.bt[k];
+        a.bv[k] = s.bv[k];
+    }
+    a.st = c->st;
+    return a;
+}
+
+int host_parity(cfd_ctx* c, int* par) {
+    DevState h;
+    TRY(read_state(c, &h));
+    *par = (int)(h.n & 1);
+    return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* cfd_last_error(void) { return g_err.c_str(); }
+
+int cfd_device_count(int32_t* count) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return 0;
+}
+
+int cfd_ctx_create(const cfd_grid_desc* d, cfd_ctx** out) {
+    if (!d || !out) return set_err(CFD_ERR_ARG, "null argument");
+    if (d->nx < 4 || d->ny < 4) return set_err(CFD_ERR_ARG, "grid needs at least 4 cells per side");
+    if (!d->fluid || !d->pcode || !d->pcomp || !d->diag_table || !d->inv_table)
+        return set_err(CFD_ERR_ARG, "grid description lacks masks or tables");
+    if (d->ncomp < 0 || d->ncomp > CFD_MAX_COMP)
+        return set_err(CFD_ERR_LIMIT, "%d pure-Neumann pressure components (at most %d)", d->ncomp,
+                       CFD_MAX_COMP);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return set_err(CFD_ERR_HIP, "no HIP device available");
+    cfd_ctx* c = new cfd_ctx();
+    c->nx = d->nx;
+    c->ny = d->ny;
+    c->dx = d->dx;
+    c->dy = d->dy;
+    c->periodic = d->periodic ? 1 : 0;
+    c->P = ((d->ny + 2 + 31) / 32) * 32;
+    c->ncomp = d->ncomp;
+    for (int s = 0; s < 4; ++s) {
+        c->kind[s] = d->bc_kind[s];
+        c->vel[s] = d->bc_velocity[s];
+    }
+    c->geo.nx = c->nx;
+    c->geo.ny = c->ny;
+    c->geo.pu = c->geo.pv = c->geo.pc = c->P;
+    c->geo.periodic = c->periodic;
+    c->geo.dx = c->dx;
+    c->geo.dy = c->dy;
+    *out = nullptr;
+    auto fail = [&](int rc) {
+        cfd_ctx_destroy(c);
+        return rc;
+    };
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return fail(set_err(CFD_ERR_HIP, "cannot query the HIP device"));
+    c->num_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_err(CFD_ERR_HIP, "stream creation failed"));
+    c->stream = c->own_stream;
+    hipEventCreate(&c->ev0);
+    hipEventCreate(&c->ev1);
+
+    const size_t N = cells(c);
+    int rc = 0;
+    if ((rc = dalloc(c, &c->u, N)) || (rc = dalloc(c, &c->v, N)) || (rc = dalloc(c, &c->p, N)) ||
+        (rc = dalloc(c, &c->F, N)) || (rc = dalloc(c, &c->G, N)) ||
+        (rc = dalloc(c, &c->H[0][0], N)) || (rc = dalloc(c, &c->H[0][1], N)) ||
+        (rc = dalloc(c, &c->H[1][0], N)) || (rc = dalloc(c, &c->H[1][1], N)) ||
+        (rc = dalloc(c, &c->x, N)) || (rc = dalloc(c, &c->b, N)) || (rc = dalloc(c, &c->r, N)) ||
+        (rc = dalloc(c, &c->q, N)) || (rc = dalloc(c, &c->d0, N)) || (rc = dalloc(c, &c->d1, N)) ||
+        (rc = dalloc(c, &c->tmp0, N)) || (rc = dalloc(c, &c->tmp1, N)) ||
+        (rc = dalloc(c, &c->r32, N)) || (rc = dalloc(c, &c->q32, N)) ||
+        (rc = dalloc(c, &c->e0, N)) || (rc = dalloc(c, &c->e1, N)) ||
+        (rc = dalloc(c, &c->fl, N)) || (rc = dalloc(c, &c->code, N)) ||
+        (rc = dalloc(c, &c->comp, N)) || (rc = dalloc(c, &c->lut, 128)) ||
+        (rc = dalloc(c, &c->comp_count, CFD_MAX_COMP)) || (rc = dalloc(c, &c->st, 1)) ||
+        (rc = dalloc(c, &c->ring, RING)) || (rc = dalloc(c, &c->bar, 4)) ||
+        (rc = dalloc(c, &c->ticket, 4)) || (rc = dalloc(c, &c->acc, 4)))
+        return fail(rc);
+    // persistent projection launch shape: one block per CU (or per row on small grids)
+    c->proj_bs = std::min(PROJ_BS, ((c->ny + 63) / 64) * 64);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_project, c->proj_bs, 0) !=
+            hipSuccess || per_cu < 1)
+        return fail(set_err(CFD_ERR_HIP, "projection kernel does not fit on a CU"));
+    c->proj_blocks = std::max(1, std::min(c->num_cu, c->nx));
+    c->red_blocks = std::max(c->proj_blocks, 4 * c->num_cu);
+    if ((rc = dalloc(c, &c->red, (size_t)2 * CFD_MAX_COMP * c->proj_blocks)) ||
+        (rc = dalloc(c, &c->part, (size_t)REC_NV * c->red_blocks)))
+        return fail(rc);
+
+    // masks and tables
+    const int nx = c->nx, ny = c->ny, P = c->P;
+    std::vector<uint8_t> flh(N, 0), codeh(N, 0), comph(N, 0);
+    for (int i = 0; i < nx + 2; ++i)
+        for (int j = 0; j < ny + 2; ++j) flh[(size_t)i * P + j] = d->fluid[(size_t)i * (ny + 2) + j] ? 1 : 0;
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+            codeh[(size_t)(i + 1) * P + j + 1] = d->pcode[(size_t)i * ny + j];
+            const int cm = d->pcomp[(size_t)i * ny + j];
+            if (cm > d->ncomp) return fail(set_err(CFD_ERR_ARG, "component label %d > ncomp", cm));
+            comph[(size_t)(i + 1) * P + j + 1] = (uint8_t)cm;
+        }
+    c->has_solid = 0;
+    for (int i = 1; i <= nx && !c->has_solid; ++i)
+        for (int j = 1; j <= ny; ++j)
+            if (!flh[(size_t)i * P + j]) { c->has_solid = 1; break; }
+    std::vector<double> luth(128, 0.0);
+    for (int k = 0; k < 64; ++k) {
+        luth[k] = d->diag_table[k];
+        luth[64 + k] = d->inv_table[k];
+    }
+    luth[0] = 0.0;
+    luth[64] = 0.0;
+    std::vector<double> cc(CFD_MAX_COMP, 1.0);
+    for (int k = 0; k < d->ncomp; ++k) cc[k] = d->comp_count[k];
+    if (hipMemcpy(c->fl, flh.data(), N, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->code, codeh.data(), N, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->comp, comph.data(), N, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->lut, luth.data(), 128 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->comp_count, cc.data(), CFD_MAX_COMP * sizeof(double),
+                  hipMemcpyHostToDevice) != hipSuccess)
+        return fail(set_err(CFD_ERR_HIP, "mask upload failed"));
+    for (int s = 0; s < 4; ++s) {
+        const int len = s < 2 ? ny : nx;
+        if ((rc = dalloc(c, &c->inflow[s], len)) || (rc = dalloc(c, &c->open[s], len)))
+            return fail(rc);
+        if (!d->open[s]) return fail(set_err(CFD_ERR_ARG, "open mask of side %d missing", s));
+        if (hipMemcpy(c->open[s], d->open[s], len, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_err(CFD_ERR_HIP, "open mask upload failed"));
+        if (d->inflow[s] && hipMemcpy(c->inflow[s], d->inflow[s], len * sizeof(double),
+                                      hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_err(CFD_ERR_HIP, "inflow upload failed"));
+    }
+    if (hipDeviceSynchronize() != hipSuccess) return fail(set_err(CFD_ERR_HIP, "setup sync failed"));
+    *out = c;
+    return 0;
+}
+
+int cfd_ctx_destroy(cfd_ctx* c) {
+    if (!c) return 0;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) hipFree(p);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->own_stream) hipStreamDestroy(c->own_stream);
+    delete c;
+    return 0;
+}
+
+int cfd_ctx_set_stream(cfd_ctx* c, void* stream) {
+    if (!c) return set_err(CFD_ERR_ARG, "null context");
+    HIPC(hipStreamSynchronize(c->stream));
+    c->stream = stream ? (hipStream_t)stream : c->own_stream;
+    return 0;
+}
+
+int cfd_add_scalar(cfd_ctx* c, const cfd_scalar_desc* d, int32_t* id) {
+    if (!c || !d || !id) return set_err(CFD_ERR_ARG, "null argument");
+    if ((int)c->scalars.size() >= CFD_MAX_SCALARS)
+        return set_err(CFD_ERR_LIMIT, "at most %d transported fields", CFD_MAX_SCALARS);
+    if (d->limiter < 0 || d->limiter > 5) return set_err(CFD_ERR_ARG, "bad limiter %d", d->limiter);
+    ScalarDev s;
+    std::memset(&s, 0, sizeof(s));
+    s.desc = *d;
+    const size_t N = cells(c);
+    TRY(dalloc(c, &s.phi[0], N));
+    TRY(dalloc(c, &s.phi[1], N));
+    for (int k = 0; k < 4; ++k) {
+        const int len = k < 2 ? c->ny : c->nx;
+        TRY(dalloc(c, &s.bt[k], len));
+        TRY(dalloc(c, &s.bv[k], len));
+        if (!d->bt[k] || !d->bv[k]) return set_err(CFD_ERR_ARG, "scalar BC arrays missing");
+        for (int q = 0; q < len; ++q)
+            if (d->bt[k][q] < 0 || d->bt[k][q] > 2)
+                return set_err(CFD_ERR_ARG, "scalar BC code %lld", (long long)d->bt[k][q]);
+        HIPC(hipMemcpy(s.bt[k], d->bt[k], len * sizeof(int64_t), hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(s.bv[k], d->bv[k], len * sizeof(double), hipMemcpyHostToDevice));
+    }
+    for (int k = 0; k < 4; ++k) s.desc.bt[k] = nullptr, s.desc.bv[k] = nullptr;
+    c->scalars.push_back(s);
+    *id = (int32_t)c->scalars.size() - 1;
+    return 0;
+}
+
+int cfd_upload(cfd_ctx* c, int32_t field, const double* host) {
+    if (!c || !host) return set_err(CFD_ERR_ARG, "null argument");
+    int rows, cols, oi, oj;
+    TRY(shape_of(c, field, &rows, &cols, &oi, &oj));
+    int par = 0;
+    TRY(host_parity(c, &par));
+    double* dst = field_ptr(c, field, par);
+    HIPC(hipMemcpy2DAsync(dst + (size_t)oi * c->P + oj, c->P * sizeof(double), host,
+                          cols * sizeof(double), cols * sizeof(double), rows,
+                          hipMemcpyHostToDevice, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int cfd_download(cfd_ctx* c, int32_t field, double* host) {
+    if (!c || !host) return set_err(CFD_ERR_ARG, "null argument");
+    int rows, cols, oi, oj;
+    TRY(shape_of(c, field, &rows, &cols, &oi, &oj));
+    int par = 0;
+    TRY(host_parity(c, &par));
+    const double* src = field_ptr(c, field, par);
+    HIPC(hipMemcpy2DAsync(host, cols * sizeof(double), src + (size_t)oi * c->P + oj,
+                          c->P * sizeof(double), cols * sizeof(double), rows,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int cfd_upload_scalar(cfd_ctx* c, int32_t id, const double* host) {
+    if (!c || !host || id < 0 || id >= (int)c->scalars.size())
+        return set_err(CFD_ERR_ARG, "bad scalar id");
+    int par = 0;
+    TRY(host_parity(c, &par));
+    ScalarDev& s = c->scalars[id];
+    for (int k = 0; k < 2; ++k)
+        HIPC(hipMemcpy2DAsync(s.phi[k], c->P * sizeof(double), host, (c->ny + 2) * sizeof(double),
+                              (c->ny + 2) * sizeof(double), c->nx + 2, hipMemcpyHostToDevice,
+                              c->stream));
+    // fill_ghosts on the uploaded field (functions.py:366-369), result in phi[par]
+    ScalarArgs a = scalar_args(c, s, par ^ 1);
+    a.st = nullptr;
+    hipLaunchKernelGGL(k_fill_ghosts, grid2d(c), dim3(BS2D), 0, c->stream, a, c->geo);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(s.phi[par ^ 1], s.phi[par], cells(c) * sizeof(double),
+                        hipMemcpyDeviceToDevice, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int cfd_download_scalar(cfd_ctx* c, int32_t id, double* host) {
+    if (!c || !host || id < 0 || id >= (int)c->scalars.size())
+        return set_err(CFD_ERR_ARG, "bad scalar id");
+    int par = 0;
+    TRY(host_parity(c, &par));
+    ScalarDev& s = c->scalars[id];
+    HIPC(hipMemcpy2DAsync(host, (c->ny + 2) * sizeof(double), s.phi[par], c->P * sizeof(double),
+                          (c->ny + 2) * sizeof(double), c->nx + 2, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int cfd_set_state(cfd_ctx* c, const cfd_state* s) {
+    if (!c || !s) return set_err(CFD_ERR_ARG, "null argument");
+    DevState h;
+    TRY(read_state(c, &h));
+    const int old_par = (int)(h.n & 1), new_par = (int)(s->n & 1);
+    if (old_par != new_par) {
+        // keep the double-buffered fields where the new step count expects them
+        for (int k = 0; k < 2; ++k)
+            HIPC(hipMemcpyAsync(c->H[k][new_par ^ 1], c->H[k][old_par ^ 1],
+                                cells(c) * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        for (auto& sc : c->scalars)
+            HIPC(hipMemcpyAsync(sc.phi[new_par], sc.phi[old_par], cells(c) * sizeof(double),
+                                hipMemcpyDeviceToDevice, c->stream));
+    }
+    h.t = s->t;
+    h.dt = s->dt;
+    h.dt_old = s->dt_old;
+    h.gamma = s->gamma;
+    h.flow_force = s->flow_force;
+    h.n = s->n;
+    h.has_old = s->has_old;
+    h.done = s->done;
+    h.error = s->error;
+    HIPC(hipMemcpyAsync(c->st, &h, sizeof(DevState), hipMemcpyHostToDevice, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int cfd_get_state(cfd_ctx* c, cfd_state* s) {
+    if (!c || !s) return set_err(CFD_ERR_ARG, "null argument");
+    DevState h;
+    TRY(read_state(c, &h));
+    s->t = h.t;
+    s->dt = h.dt;
+    s->dt_old = h.dt_old;
+    s->gamma = h.gamma;
+    s->flow_force = h.flow_force;
+    s->n = h.n;
+    s->has_old = h.has_old;
+    s->done = h.done;
+    s->error = h.error;
+    s->pad = 0;
+    return 0;
+}
+
+int cfd_apply_bc(cfd_ctx* c) {
+    if (!c) return set_err(CFD_ERR_ARG, "null context");
+    TRY(launch_bc(c, false));
+    return 0;
+}
+
+int cfd_project_initial(cfd_ctx* c, const cfd_step_params* prm) {
+    if (!c || !prm) return set_err(CFD_ERR_ARG, "null argument");
+    PredArgs pa = pred_args(c, prm, 0);
+    hipLaunchKernelGGL(k_copy_predictor, grid2d(c), dim3(BS2D), 0, c->stream, pa, c->geo);
+    HIPC(hipGetLastError());
+    ProjArgs a = proj_args(c, prm->solver, prm->rtol, prm->maxiter);
+    a.dt_override = 1.0;
+    a.zero_p_after = 1;
+    TRY(launch_project(c, a));
+    TRY(launch_bc(c, false));
+    HIPC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int cfd_step(cfd_ctx* c, const cfd_step_params* prm, int32_t count) {
+    if (!c || !prm) return set_err(CFD_ERR_ARG, "null argument");
+    if (prm->nu <= 0.0) return set_err(CFD_ERR_ARG, "viscosity must be positive");
+    int par = 0;
+    TRY(host_parity(c, &par));
+    StepParams sp;
+    std::memset(&sp, 0, sizeof(sp));
+    sp.nu = prm->nu;
+    sp.tau = prm->tau;
+    sp.scheme_ab2 = prm->scheme_ab2;
+    sp.gamma_auto = prm->gamma_auto;
+    sp.gamma_fixed = prm->gamma_fixed;
+    sp.dt_fixed = prm->dt_fixed;
+    sp.dt_model = prm->dt_model;
+    sp.t_end = prm->t_end;
+    sp.clamp_t_end = prm->clamp_t_end;
+    sp.fx = prm->fx;
+    sp.fy = prm->fy;
+    HIPC(hipEventRecord(c->ev0, c->stream));
+    for (int k = 0; k < count; ++k) {
+        // steps that the device skips (run finished) keep the parity: the host
+        // re-reads it only at the next synchronising call, so every step of a
+        // batch is issued with the parity it will have if all steps run.
+        const int pk = par ^ (k & 1);
+        TRY(launch_bc(c, true));
+        hipLaunchKernelGGL(k_dt_gamma, dim3(std::min(4 * c->num_cu, 1024)), dim3(256), 0,
+                           c->stream, c->u, c->v, c->geo, sp, c->st, c->acc, c->ticket);
+        HIPC(hipGetLastError());
+        PredArgs pa = pred_args(c, prm, pk);
+        hipLaunchKernelGGL(k_predictor, grid2d(c), dim3(BS2D), 0, c->stream, pa, c->geo);
+        HIPC(hipGetLastError());
+        TRY(launch_project(c, proj_args(c, prm->solver, prm->rtol, prm->maxiter)));
+        TRY(launch_bc(c, true));
+        for (auto& s : c->scalars) {
+            ScalarArgs a = scalar_args(c, s, pk);
+            hipLaunchKernelGGL((k_scalar_transport<true, 1>), grid2d(c), dim3(BS2D), 0, c->stream,
+                               a, c->geo);
+            HIPC(hipGetLastError());
+            hipLaunchKernelGGL(k_fill_ghosts, grid2d(c), dim3(BS2D), 0, c->stream, a, c->geo);
+            HIPC(hipGetLastError());
+        }
+        RecordArgs ra;
+        std::memset(&ra, 0, sizeof(ra));
+        ra.u = c->u;
+        ra.v = c->v;
+        ra.fl = c->fl;
+        ra.p = c->p;
+        ra.has_solid = c->has_solid;
+        ra.nscalar = (int)c->scalars.size();
+        for (int s = 0; s < ra.nscalar; ++s) {
+            ScalarDev& sd = c->scalars[s];
+            ra.phi[s] = sd.phi[pk ^ 1];
+            ra.diffs[s] = nullptr;
+            ra.D0[s] = sd.desc.diffusivity;
+            ra.wallflux[s] = sd.desc.wall_flux;
+            for (int k2 = 0; k2 < 4; ++k2) {
+                ra.bt[s][k2] = sd.bt[k2];
+                ra.bv[s][k2] = sd.bv[k2];
+            }
+        }
+        ra.st = c->st;
+        ra.part = c->part;
+        ra.ticket = c->ticket + 1;
+        ra.ring = c->ring;
+        ra.ring_len = RING;
+        ra.scheme_ab2 = prm->scheme_ab2;
+        ra.check_t_end = prm->check_t_end;
+        ra.t_end = prm->t_end;
+        ra.steady_tol = prm->steady_tol;
+        ra.max_steps = prm->max_steps;
+        hipLaunchKernelGGL(k_record, dim3(c->red_blocks), dim3(256), 0, c->stream, ra, c->geo);
+        HIPC(hipGetLastError());
+    }
+    HIPC(hipEventRecord(c->ev1, c->stream));
+    return 0;
+}
+
+int cfd_sync(cfd_ctx* c) {
+    if (!c) return set_err(CFD_ERR_ARG, "null context");
+    HIPC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int cfd_last_elapsed_ms(cfd_ctx* c, double* ms) {
+    if (!c || !ms) return set_err(CFD_ERR_ARG, "null argument");
+    HIPC(hipEventSynchronize(c->ev1));
+    float f = 0.0f;
+    HIPC(hipEventElapsedTime(&f, c->ev0, c->ev1));
+    *ms = f;
+    return 0;
+}
+
+int cfd_read_records(cfd_ctx* c, int64_t first_n, int32_t count, cfd_record* out) {
+    if (!c || !out || count < 0) return set_err(CFD_ERR_ARG, "bad arguments");
+    static_assert(sizeof(cfd_record) == sizeof(StepRecord), "record layout");
+    DevState h;
+    TRY(read_state(c, &h));
+    if (first_n < 1 || first_n + count - 1 > h.n || h.n - first_n >= RING)
+        return set_err(CFD_ERR_ARG, "records %lld..%lld not available (step %lld, ring %d)",
+                       (long long)first_n, (long long)(first_n + count - 1), (long long)h.n, RING);
+    for (int k = 0; k < count; ++k) {
+        const long long n = first_n + k;
+        HIPC(hipMemcpyAsync(&out[k], &c->ring[(n - 1) % RING], sizeof(StepRecord),
+                            hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int cfd_pressure_solve(cfd_ctx* c, const double* rhs, double* p_out, int32_t solver, double rtol,
+                       int32_t maxiter, int32_t* iters, double* relres) {
+    if (!c || !rhs || !p_out) return set_err(CFD_ERR_ARG, "null argument");
+    HIPC(hipMemcpy2DAsync(c->tmp0 + c->P + 1, c->P * sizeof(double), rhs, c->ny * sizeof(double),
+                          c->ny * sizeof(double), c->nx, hipMemcpyHostToDevice, c->stream));
+    DevState h;
+    TRY(read_state(c, &h));
+    const int done = h.done;
+    if (done) {
+        h.done = 0;
+        HIPC(hipMemcpyAsync(c->st, &h, sizeof(DevState), hipMemcpyHostToDevice, c->stream));
+    }
+    ProjArgs a = proj_args(c, solver, rtol, maxiter);
+    a.rhs_in = c->tmp0;
+    a.p = c->tmp1;
+    a.correct = 0;
+    a.dt_override = 1.0;
+    TRY(launch_project(c, a));
+    HIPC(hipMemcpy2DAsync(p_out, c->ny * sizeof(double), c->tmp1 + c->P + 1,
+                          c->P * sizeof(double), c->ny * sizeof(double), c->nx,
+                          hipMemcpyDeviceToHost, c->stream));
+    DevState h2;
+    TRY(read_state(c, &h2));
+    if (iters) *iters = h2.cg_iters;
+    if (relres) *relres = h2.cg_relres;
+    if (done) {
+        h2.done = 1;
+        HIPC(hipMemcpyAsync(c->st, &h2, sizeof(DevState), hipMemcpyHostToDevice, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+    }
+    return h2.cg_info > 0 ? h2.cg_info : 0;
+}
+
+// ---- stateless kernels on reference-layout device arrays ----
+int cfd_momentum_tendencies(const double* u, const double* v, const uint8_t* fluid, int32_t nx,
+                            int32_t ny, double dx, double dy, double nu0, double gamma,
+                            int32_t periodic, double* hu, double* hv, void* stream) {
+    if (!u || !v || !fluid || !hu || !hv) return set_err(CFD_ERR_ARG, "null array");
+    if (nx < 4 || ny < 4) return set_err(CFD_ERR_ARG, "grid too small");
+    Geo g{nx, ny, ny + 2, ny + 1, ny + 2, periodic ? 1 : 0, dx, dy};
+    hipLaunchKernelGGL(k_momentum_tendencies, dim3((ny + 2 + BS2D - 1) / BS2D, nx + 2),
+                       dim3(BS2D), 0, (hipStream_t)stream, u, v, fluid, g, nu0, gamma, hu, hv);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+int cfd_obstacle_bc(double* u, double* v, const uint8_t* fluid, int32_t nx, int32_t ny,
+                    int32_t periodic, void* stream) {
+    if (!u || !v || !fluid) return set_err(CFD_ERR_ARG, "null array");
+    if (nx < 4 || ny < 4) return set_err(CFD_ERR_ARG, "grid too small");
+    Geo g{nx, ny, ny + 2, ny + 1, ny + 2, periodic ? 1 : 0, 1.0, 1.0};
+    hipLaunchKernelGGL(k_obstacle_bc, dim3((ny + 2 + BS2D - 1) / BS2D, nx + 2), dim3(BS2D), 0,
+                       (hipStream_t)stream, u, v, fluid, g, (const DevState*)nullptr);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+int cfd_scalar_transport(const double* phi, const double* u, const double* v,
+                         const uint8_t* fluid, int32_t nx, int32_t ny, double dx, double dy,
+                         const double* diff, double d0, int32_t limiter, int32_t periodic,
+                         const int64_t* bt_l, const double* bv_l, const int64_t* bt_r,
+                         const double* bv_r, const int64_t* bt_b, const double* bv_b,
+                         const int64_t* bt_t, const double* bv_t, int32_t obs_t, double obs_v,
+                         double* out, void* stream) {
+    if (!phi || !u || !v || !fluid || !out || !bt_l || !bt_r || !bt_b || !bt_t || !bv_l ||
+        !bv_r || !bv_b || !bv_t)
+        return set_err(CFD_ERR_ARG, "null array");
+    if (nx < 4 || ny < 4) return set_err(CFD_ERR_ARG, "grid too small");
+    if (limiter < 0 || limiter > 5) return set_err(CFD_ERR_ARG, "bad limiter %d", limiter);
+    Geo g{nx, ny, ny + 2, ny + 1, ny + 2, periodic ? 1 : 0, dx, dy};
+    ScalarArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.phi_in = phi;
+    a.out = out;
+    a.u = u;
+    a.v = v;
+    a.fl = fluid;
+    a.diff = diff;
+    a.D0 = d0;
+    a.limiter = limiter;
+    a.obs_t = obs_t;
+    a.obs_v = obs_v;
+    a.bt[0] = bt_l; a.bt[1] = bt_r; a.bt[2] = bt_b; a.bt[3] = bt_t;
+    a.bv[0] = bv_l; a.bv[1] = bv_r; a.bv[2] = bv_b; a.bv[3] = bv_t;
+    const dim3 grid((ny + 2 + BS2D - 1) / BS2D, nx + 2);
+    if (diff)
+        hipLaunchKernelGGL((k_scalar_transport<false, 0>), grid, dim3(BS2D), 0,
+                           (hipStream_t)stream, a, g);
+    else
+        hipLaunchKernelGGL((k_scalar_transport<true, 0>), grid, dim3(BS2D), 0,
+                           (hipStream_t)stream, a, g);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+}  // extern "C"

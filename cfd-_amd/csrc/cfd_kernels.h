@@ -1,0 +1,152 @@
+// Kernel argument blocks and launch declarations of the cfd2d HIP library.
+#pragma once
+#include "common.h"
+#include "state.h"
+
+// Boundary-condition kinds (boundary.py:19-25)
+enum { BC_NOSLIP = 0, BC_SLIP = 1, BC_INFLOW = 2, BC_OUTFLOW = 3, BC_PERIODIC = 4 };
+
+#define CFD_MAX_SCALARS 2
+#define REC_NV (10 + 5 * CFD_MAX_SCALARS)
+#define CFD_MAX_COMP 8
+#define PROJ_BS 1024
+
+// Geometry of one grid.  Row pitches are given separately for u-like
+// ((nx+1) rows), v-like ((nx+2) rows) and cell ((nx+2) rows) arrays, so the
+// same kernels run on the reference's own layout (pitches ny+2, ny+1, ny+2)
+// and on the padded layout of a device context (one pitch for all).
+struct Geo {
+    int nx, ny;
+    int pu, pv, pc;
+    int periodic;
+    double dx, dy;
+};
+
+struct PredArgs {
+    const double* u;
+    const double* v;
+    const uint8_t* fl;       // cell flags, bit 0 = fluid (reference `fluid`, ghosts incl.)
+    double* hu;
+    double* hv;
+    const double* hu_old;
+    const double* hv_old;
+    double* F;
+    double* G;
+    const double* T;         // Boussinesq temperature or nullptr
+    double gx, gy, Tref;
+    double nu0;
+    double fx, fy;
+    const DevState* st;
+    int scheme_ab2;
+    int ext[4];              // outflow extrapolation per side (left, right, bottom, top)
+    const uint8_t* open[4];  // open boundary faces per side
+};
+
+struct BCArgs {
+    double* u;
+    double* v;
+    const DevState* st;
+    int kind[4];
+    double vel[4];
+    const double* inflow[4];
+    const uint8_t* open[4];
+};
+
+struct ScalarArgs {
+    const double* phi_in;
+    double* phi_out;
+    double* out;             // standalone transport: right-hand side
+    const double* u;
+    const double* v;
+    const uint8_t* fl;
+    const double* diff;      // nullptr: constant D0
+    double D0;
+    int limiter;
+    int obs_t;
+    double obs_v;
+    int has_solid;
+    const int64_t* bt[4];
+    const double* bv[4];
+    const DevState* st;
+};
+
+struct RecordArgs {
+    const double* u;
+    const double* v;
+    const uint8_t* fl;
+    const double* p;
+    int has_solid;
+    int nscalar;
+    const double* phi[CFD_MAX_SCALARS];
+    const double* diffs[CFD_MAX_SCALARS];
+    double D0[CFD_MAX_SCALARS];
+    int wallflux[CFD_MAX_SCALARS];
+    const int64_t* bt[CFD_MAX_SCALARS][4];
+    const double* bv[CFD_MAX_SCALARS][4];
+    DevState* st;
+    double* part;
+    unsigned* ticket;
+    StepRecord* ring;
+    int ring_len;
+    int scheme_ab2;
+    int check_t_end;
+    double t_end;
+    double steady_tol;
+    long long max_steps;
+};
+
+// Pressure-projection arguments.  Cell arrays (x, b, r, q, d0, d1, p, code,
+// comp) use pitch g.pc and are zero outside the fluid interior.
+struct ProjArgs {
+    const double* F;
+    const double* G;
+    double* u;
+    double* v;
+    double* p;
+    const uint8_t* code;     // bits 0-5: index into the diagonal table (0 = not fluid),
+                             // bit 6: Dirichlet x-face, bit 7: Dirichlet y-face
+    const uint8_t* comp;     // Neumann component 1..ncomp, 0 = none
+    const double* lut;       // [0..63] diagonal, [64..127] its reciprocal
+    const double* comp_count;
+    int ncomp;
+    double* x;
+    double* b;
+    double* r;
+    double* q;
+    double* d0;
+    double* d1;
+    float* r32;              // mixed precision work vectors
+    float* q32;
+    float* d32_0;
+    float* d32_1;
+    double* red;
+    unsigned* bar;
+    DevState* st;
+    double dt_override;      // > 0: use this dt (initial projection with dt = 1)
+    const double* rhs_in;    // standalone solve: right-hand side (cell layout) instead of div(F,G)/dt
+    int correct;             // 0: solve only (no pressure write-back correction of u, v)
+    double rtol;
+    int maxiter;
+    int zero_p_after;        // initial projection: p := 0 afterwards
+    int method;              // PROJ_PCG64, PROJ_MIXED
+    double reliable_delta;   // mixed: true-residual replacement when ||r|| drops by this factor
+};
+
+enum { PROJ_PCG64 = 0, PROJ_MIXED = 1 };
+
+// kernels (stencil.hip)
+__global__ void k_momentum_tendencies(const double* u, const double* v, const uint8_t* fl, Geo g,
+                                      double nu0, double gamma, double* hu, double* hv);
+__global__ void k_predictor(PredArgs a, Geo g);
+__global__ void k_copy_predictor(PredArgs a, Geo g);
+__global__ void k_bc_domain(BCArgs a, Geo g);
+__global__ void k_obstacle_bc(double* u, double* v, const uint8_t* fl, Geo g, const DevState* st);
+__global__ void k_dt_gamma(const double* u, const double* v, Geo g, StepParams prm, DevState* st,
+                           unsigned long long* acc, unsigned* ticket);
+template <bool CONST_D, int MODE>
+__global__ void k_scalar_transport(ScalarArgs a, Geo g);
+__global__ void k_fill_ghosts(ScalarArgs a, Geo g);
+__global__ void k_record(RecordArgs a, Geo g);
+__global__ void k_zero_u32(unsigned* p, int n);
+// pcg.hip
+__global__ void k_project(ProjArgs a, Geo g);

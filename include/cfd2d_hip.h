@@ -1,0 +1,194 @@
+/*
+ * cfd2d_hip.h - C ABI of the MI355X (gfx950) implementation of the cfd2d
+ * projection time step (kaeptnzursee1910/CFD-, package cfd2d).
+ *
+ * Plain C: integers, doubles and pointers only.  Host arrays use the
+ * reference's NumPy layout (C order, index [i, j], j contiguous):
+ *     u, F, hu, hu_old : (nx+1) x (ny+2)     v, G, hv, hv_old : (nx+2) x (ny+1)
+ *     p, phi, fluid    : (nx+2) x (ny+2)     pressure cell arrays : nx x ny
+ * Every entry point returns 0 on success, > 0 for "solver not converged after
+ * k iterations" (as scipy's `info`), < 0 on error; cfd_last_error() gives the
+ * message of the last error of the calling thread.
+ *
+ * Ownership: a context owns all device memory it allocates (hipMalloc) and
+ * runs on one HIP stream (its own, or one set by cfd_ctx_set_stream).  Calls
+ * that only enqueue work return before the GPU has finished it; calls that
+ * return host data synchronise the stream.  A context is used by one host
+ * thread at a time.
+ */
+#ifndef CFD2D_HIP_H
+#define CFD2D_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFD_OK 0
+#define CFD_ERR_ARG (-1)
+#define CFD_ERR_HIP (-2)
+#define CFD_ERR_STATE (-3)
+#define CFD_ERR_LIMIT (-4)
+#define CFD_ERR_UNSTABLE (-5)
+
+/* boundary.py:19-25 */
+enum { CFD_BC_NOSLIP = 0, CFD_BC_SLIP = 1, CFD_BC_INFLOW = 2, CFD_BC_OUTFLOW = 3,
+       CFD_BC_PERIODIC = 4 };
+/* kernels.py:33 */
+enum { CFD_LIM_UPWIND = 0, CFD_LIM_VANLEER = 1, CFD_LIM_MINMOD = 2, CFD_LIM_SUPERBEE = 3,
+       CFD_LIM_LINEAR = 4, CFD_LIM_MC = 5 };
+/* fields of a context, reference shapes as listed above; X is the pressure
+   solver's warm start (pressure.py:140, 177) as an nx x ny array */
+enum { CFD_FIELD_U = 0, CFD_FIELD_V = 1, CFD_FIELD_P = 2, CFD_FIELD_F = 3, CFD_FIELD_G = 4,
+       CFD_FIELD_HU = 5, CFD_FIELD_HV = 6, CFD_FIELD_HU_OLD = 7, CFD_FIELD_HV_OLD = 8,
+       CFD_FIELD_X = 9 };
+/* pressure solvers: Jacobi-PCG in FP64 (the reference's "cg", pressure.py:171),
+   and Jacobi-PCG with FP32 vectors, FP64 dots and FP64 true-residual replacement */
+enum { CFD_SOLVER_PCG64 = 0, CFD_SOLVER_MIXED = 1 };
+
+typedef struct cfd_ctx cfd_ctx;
+
+/* Grid, boundary conditions and the assembled pressure-system description.
+   Mirrors Grid (grid.py:26-152), DomainBoundary (boundary.py:92-141) and the
+   setup half of PressureSolver (pressure.py:32-141). */
+typedef struct cfd_grid_desc {
+    int32_t nx, ny;
+    double dx, dy;
+    int32_t periodic;            /* periodic in x (grid.set_periodic_x applied to fluid) */
+    const uint8_t* fluid;        /* (nx+2) x (ny+2), grid.fluid after finalize() */
+    int32_t bc_kind[4];          /* left, right, bottom, top: CFD_BC_* */
+    double bc_velocity[4];       /* wall tangential velocity (noslip) */
+    const double* inflow[4];     /* signed inflow normal velocity per boundary face, or NULL */
+    const uint8_t* open[4];      /* boundary face open (adjacent inner cell is fluid) */
+    const uint8_t* pcode;        /* nx x ny: index 1..63 into diag_table (0 = not fluid)
+                                    | 64 if the cell has a Dirichlet x-face
+                                    | 128 if it has a Dirichlet y-face */
+    const uint8_t* pcomp;        /* nx x ny: pure-Neumann component 1..ncomp, 0 = none */
+    int32_t ncomp;               /* <= 8 */
+    const double* comp_count;    /* cells per component */
+    const double* diag_table;    /* 64 diagonal entries of A = -Laplacian */
+    const double* inv_table;     /* 64 reciprocals (Jacobi preconditioner, pressure.py:139) */
+} cfd_grid_desc;
+
+/* One transported cell field (models.py:73-136 ScalarField.attach). */
+typedef struct cfd_scalar_desc {
+    double diffusivity;          /* D0: constant diffusivity field */
+    int32_t limiter;             /* CFD_LIM_* */
+    int32_t obstacle_fixed;      /* obstacle BC: 0 zero-gradient, 1 fixed value */
+    double obstacle_value;
+    const int64_t* bt[4];        /* per boundary face: 0 zero-gradient, 1 fixed, 2 flux */
+    const double* bv[4];
+    int32_t wall_flux;           /* accumulate wall fluxes for Nusselt monitors */
+    int32_t boussinesq;          /* add the buoyancy source of this field (models.py:386-396) */
+    double gx, gy, t_ref;
+} cfd_scalar_desc;
+
+/* Time-step settings (solver.py:95-173, 209-253, 448-492). */
+typedef struct cfd_step_params {
+    double nu, tau;
+    int32_t scheme_ab2;          /* 1 AB2, 0 explicit Euler */
+    int32_t gamma_auto;
+    double gamma_fixed;
+    double dt_fixed;             /* > 0: fixed dt */
+    double dt_model;             /* min of the models' dt_limit(), or >= 1e300 */
+    double t_end;                /* run(): shorten the last step to end at t_end */
+    int32_t clamp_t_end;
+    int32_t solver;              /* CFD_SOLVER_* */
+    double fx, fy;               /* body force */
+    double rtol;                 /* pressure solve relative tolerance */
+    int32_t maxiter;
+    int32_t check_t_end;         /* run(): stop once t >= t_end - 1e-12 */
+    double reliable_delta;       /* mixed solver: residual-replacement trigger factor */
+    double steady_tol;           /* run(): stop once n > 10 and residual < steady_tol (0: off) */
+    int64_t max_steps;           /* run(): stop at this step count (0: off) */
+} cfd_step_params;
+
+/* Scalars of a running simulation (Simulation attributes t, n, dt, dt_old,
+   flow_force and whether AB2 history exists). */
+typedef struct cfd_state {
+    double t, dt, dt_old, gamma, flow_force;
+    int64_t n;
+    int32_t has_old;
+    int32_t done;                /* a run reached t_end / max_steps / steady_tol */
+    int32_t error;               /* instability detected at step n (solver.py:395-399) */
+    int32_t pad;
+} cfd_state;
+
+/* One history record (solver.py:427-443) plus solver statistics.  mon[] holds
+   raw sums over the grid: [0] max |div u|, [1] sum of uc^2 + vc^2 over fluid
+   cells, [2..5] sums of p over fluid cells with a solid east / west / north /
+   south neighbour, [6..7] uc summed over the north / south ones, [8..9] vc
+   over the east / west ones (obstacle forces, solver.py:542-567), then per
+   scalar s (5 values from index 10+5s): sum of phi over fluid cells and the
+   wall-flux sums of the left, right, bottom, top sides (models.py:228-250). */
+typedef struct cfd_record {
+    double t, dt, gamma, residual, divergence, kinetic_energy;
+    double cg_relres;
+    double mon[24];
+    int64_t n;
+    int32_t cg_iters, cg_info, cg_outer, nmon;
+} cfd_record;
+
+const char* cfd_last_error(void);
+int cfd_device_count(int32_t* count);
+
+/* Simulation.__init__ device half (solver.py:110-147): allocate and upload. */
+int cfd_ctx_create(const cfd_grid_desc* desc, cfd_ctx** out);
+int cfd_ctx_destroy(cfd_ctx* ctx);
+/* Run on a caller-owned hipStream_t (NULL restores the context's own stream). */
+int cfd_ctx_set_stream(cfd_ctx* ctx, void* stream);
+/* Add a transported field (PassiveScalar / BoussinesqTemperature attach,
+   models.py:102-136, 296-333, 339-384); *id receives its index. */
+int cfd_add_scalar(cfd_ctx* ctx, const cfd_scalar_desc* desc, int32_t* id);
+
+/* Host <-> device copies in the reference layout (checkpoint save/load,
+   functions.py:326-379; probes and reports read fields this way). */
+int cfd_upload(cfd_ctx* ctx, int32_t field, const double* host);
+int cfd_download(cfd_ctx* ctx, int32_t field, double* host);
+int cfd_upload_scalar(cfd_ctx* ctx, int32_t id, const double* host);   /* + fill_ghosts */
+int cfd_download_scalar(cfd_ctx* ctx, int32_t id, double* host);
+int cfd_set_state(cfd_ctx* ctx, const cfd_state* st);
+int cfd_get_state(cfd_ctx* ctx, cfd_state* st);
+
+/* Simulation._apply_bc (solver.py:190-193): DomainBoundary.apply + obstacle_bc. */
+int cfd_apply_bc(cfd_ctx* ctx);
+/* Simulation._project_initial (solver.py:195-204). */
+int cfd_project_initial(cfd_ctx* ctx, const cfd_step_params* prm);
+/* Simulation.step (solver.py:326-403): enqueue `count` steps, no host sync. */
+int cfd_step(cfd_ctx* ctx, const cfd_step_params* prm, int32_t count);
+/* Wait for enqueued work. */
+int cfd_sync(cfd_ctx* ctx);
+/* Copy the records of steps first_n .. first_n+count-1 (1-based step numbers,
+   at most the ring length back from the current step). */
+int cfd_read_records(cfd_ctx* ctx, int64_t first_n, int32_t count, cfd_record* out);
+/* Time of the last cfd_step batch on the device, in ms (HIP events). */
+int cfd_last_elapsed_ms(cfd_ctx* ctx, double* ms);
+
+/* PressureSolver.solve (pressure.py:143-185) on a host right-hand side
+   (nx x ny) -> host pressure (nx x ny); warm start from the context's X. */
+int cfd_pressure_solve(cfd_ctx* ctx, const double* rhs, double* p_out, int32_t solver,
+                       double rtol, int32_t maxiter, int32_t* iters, double* relres);
+
+/* ---- stateless kernels on caller device arrays in the reference layout ---- */
+/* kernels.momentum_tendencies (kernels.py:44-155), constant viscosity nu0 */
+int cfd_momentum_tendencies(const double* u, const double* v, const uint8_t* fluid, int32_t nx,
+                            int32_t ny, double dx, double dy, double nu0, double gamma,
+                            int32_t periodic, double* hu, double* hv, void* stream);
+/* kernels.obstacle_bc (kernels.py:158-209) */
+int cfd_obstacle_bc(double* u, double* v, const uint8_t* fluid, int32_t nx, int32_t ny,
+                    int32_t periodic, void* stream);
+/* kernels.scalar_transport (kernels.py:254-391); diff may be NULL for a
+   constant diffusivity d0 */
+int cfd_scalar_transport(const double* phi, const double* u, const double* v,
+                         const uint8_t* fluid, int32_t nx, int32_t ny, double dx, double dy,
+                         const double* diff, double d0, int32_t limiter, int32_t periodic,
+                         const int64_t* bt_l, const double* bv_l, const int64_t* bt_r,
+                         const double* bv_r, const int64_t* bt_b, const double* bv_b,
+                         const int64_t* bt_t, const double* bv_t, int32_t obs_t, double obs_v,
+                         double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,230 @@
+/*
+ * CPU FP64 oracle: serial C restatement of the reference's Numba kernels.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product links or calls this file;
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * the shared library built from it (oracle/Makefile), as the checker.
+ *
+ * Array layout is the reference's: C order, index [i, j] with j contiguous,
+ *   u, F, hu : (nx+1) x (ny+2)      v, G, hv : (nx+2) x (ny+1)
+ *   p, phi, fluid, diff : (nx+2) x (ny+2)
+ * Built with -ffp-contract=off so every product and sum rounds exactly as the
+ * reference's IEEE double arithmetic does (Python floats / Numba without
+ * fastmath).
+ *
+ * Follows cfd2d/kernels.py:
+ *   oracle_momentum_tendencies  kernels.py:44-155 (constant-viscosity branch,
+ *                               lines 104-105; no variable density)
+ *   oracle_obstacle_bc          kernels.py:158-209
+ *   limiter / face_value        kernels.py:215-251
+ *   oracle_scalar_transport     kernels.py:254-391
+ */
+#include <math.h>
+#include <stdint.h>
+
+#define U_(i, j) u[(long)(i) * (ny + 2) + (j)]
+#define V_(i, j) v[(long)(i) * (ny + 1) + (j)]
+#define C_(a, i, j) a[(long)(i) * (ny + 2) + (j)]
+
+void oracle_momentum_tendencies(const double *u, const double *v, const uint8_t *fluid,
+                                int nx, int ny, double dx, double dy, double nu0, double gamma,
+                                int periodic, double *hu, double *hv)
+{
+    const double idx2 = 1.0 / (dx * dx), idy2 = 1.0 / (dy * dy);
+    const double idx = 1.0 / dx, idy = 1.0 / dy;
+    for (long k = 0; k < (long)(nx + 1) * (ny + 2); ++k) hu[k] = 0.0;
+    for (long k = 0; k < (long)(nx + 2) * (ny + 1); ++k) hv[k] = 0.0;
+    const int imax = periodic ? nx : nx - 1;
+
+    /* u faces: between cells i and i+1 */
+    for (int i = 1; i <= imax; ++i) {
+        const int ip = i < nx ? i + 1 : 1;
+        for (int j = 1; j <= ny; ++j) {
+            if (!(C_(fluid, i, j) && C_(fluid, i + 1, j))) continue;
+            const double uc = U_(i, j), ue = U_(ip, j), uw = U_(i - 1, j);
+            const double un = U_(i, j + 1), us = U_(i, j - 1);
+            const double diff = nu0 * ((ue - 2.0 * uc + uw) * idx2 + (un - 2.0 * uc + us) * idy2);
+            const double a_e = 0.5 * (uc + ue), a_w = 0.5 * (uw + uc);
+            const double du2dx = idx * (a_e * a_e - a_w * a_w)
+                + gamma * idx * (fabs(a_e) * 0.5 * (uc - ue) - fabs(a_w) * 0.5 * (uw - uc));
+            const double v_n = 0.5 * (V_(i, j) + V_(i + 1, j));
+            const double v_s = 0.5 * (V_(i, j - 1) + V_(i + 1, j - 1));
+            const double duvdy = idy * (v_n * 0.5 * (uc + un) - v_s * 0.5 * (us + uc))
+                + gamma * idy * (fabs(v_n) * 0.5 * (uc - un) - fabs(v_s) * 0.5 * (us - uc));
+            hu[(long)i * (ny + 2) + j] = diff - du2dx - duvdy;
+        }
+    }
+    /* v faces: between cells j and j+1 */
+    for (int i = 1; i <= nx; ++i) {
+        for (int j = 1; j < ny; ++j) {
+            if (!(C_(fluid, i, j) && C_(fluid, i, j + 1))) continue;
+            const double vc = V_(i, j), ve = V_(i + 1, j), vw = V_(i - 1, j);
+            const double vn = V_(i, j + 1), vs = V_(i, j - 1);
+            const double diff = nu0 * ((ve - 2.0 * vc + vw) * idx2 + (vn - 2.0 * vc + vs) * idy2);
+            const double b_n = 0.5 * (vc + vn), b_s = 0.5 * (vs + vc);
+            const double dv2dy = idy * (b_n * b_n - b_s * b_s)
+                + gamma * idy * (fabs(b_n) * 0.5 * (vc - vn) - fabs(b_s) * 0.5 * (vs - vc));
+            const double u_e = 0.5 * (U_(i, j) + U_(i, j + 1));
+            const double u_w = 0.5 * (U_(i - 1, j) + U_(i - 1, j + 1));
+            const double duvdx = idx * (u_e * 0.5 * (vc + ve) - u_w * 0.5 * (vw + vc))
+                + gamma * idx * (fabs(u_e) * 0.5 * (vc - ve) - fabs(u_w) * 0.5 * (vw - vc));
+            hv[(long)i * (ny + 1) + j] = diff - duvdx - dv2dy;
+        }
+    }
+}
+
+void oracle_obstacle_bc(double *u, double *v, const uint8_t *fluid, int nx, int ny, int periodic)
+{
+    const int imax = periodic ? nx : nx - 1;
+    for (int i = 1; i <= imax; ++i) {
+        for (int j = 1; j <= ny; ++j) {
+            const int fl = C_(fluid, i, j), fr = C_(fluid, i + 1, j);
+            if (fl && fr) continue;
+            if (fl || fr) { U_(i, j) = 0.0; continue; }
+            double s = 0.0;
+            int c = 0;
+            if (C_(fluid, i, j + 1) && C_(fluid, i + 1, j + 1)) { s += U_(i, j + 1); ++c; }
+            if (C_(fluid, i, j - 1) && C_(fluid, i + 1, j - 1)) { s += U_(i, j - 1); ++c; }
+            U_(i, j) = c > 0 ? -s / c : 0.0;
+        }
+    }
+    if (periodic)
+        for (int j = 0; j < ny + 2; ++j) U_(0, j) = U_(nx, j);
+    for (int i = 1; i <= nx; ++i) {
+        for (int j = 1; j < ny; ++j) {
+            const int fb = C_(fluid, i, j), ft = C_(fluid, i, j + 1);
+            if (fb && ft) continue;
+            if (fb || ft) { V_(i, j) = 0.0; continue; }
+            double s = 0.0;
+            int c = 0;
+            if (C_(fluid, i + 1, j) && C_(fluid, i + 1, j + 1)) { s += V_(i + 1, j); ++c; }
+            if (C_(fluid, i - 1, j) && C_(fluid, i - 1, j + 1)) { s += V_(i - 1, j); ++c; }
+            V_(i, j) = c > 0 ? -s / c : 0.0;
+        }
+    }
+}
+
+double oracle_limiter(double r, int kind)
+{
+    switch (kind) {
+    case 0: return 0.0;
+    case 1: return r <= 0.0 ? 0.0 : 2.0 * r / (1.0 + r);
+    case 2: return r <= 0.0 ? 0.0 : fmin(r, 1.0);
+    case 3: return r <= 0.0 ? 0.0 : fmax(fmin(2.0 * r, 1.0), fmin(r, 2.0));
+    case 5: return r <= 0.0 ? 0.0 : fmin(2.0 * r, fmin(0.5 * (1.0 + r), 2.0));
+    default: return 1.0;
+    }
+}
+
+static double face_value(double phi_u, double phi_c, double phi_d, int has_u, int kind)
+{
+    if (kind == 0) return phi_c;
+    if (kind == 4) return 0.5 * (phi_c + phi_d);
+    if (!has_u) return phi_c;
+    const double den = phi_d - phi_c;
+    if (fabs(den) < 1e-300) return phi_c;
+    const double r = (phi_c - phi_u) / den;
+    return phi_c + 0.5 * oracle_limiter(r, kind) * den;
+}
+
+void oracle_scalar_transport(const double *phi, const double *u, const double *v,
+                             const uint8_t *fluid, int nx, int ny, double dx, double dy,
+                             const double *diff, int limiter, int periodic,
+                             const int64_t *bt_l, const double *bv_l,
+                             const int64_t *bt_r, const double *bv_r,
+                             const int64_t *bt_b, const double *bv_b,
+                             const int64_t *bt_t, const double *bv_t,
+                             int obs_t, double obs_v, double *out)
+{
+    for (long k = 0; k < (long)(nx + 2) * (ny + 2); ++k) out[k] = 0.0;
+
+    for (int i = 0; i <= nx; ++i) {
+        for (int j = 1; j <= ny; ++j) {
+            if (i == 0 || (i == nx && !periodic)) {
+                int P, t;
+                double sgn, val;
+                if (i == 0) {
+                    if (periodic) continue;
+                    P = 1; sgn = 1.0; t = (int)bt_l[j - 1]; val = bv_l[j - 1];
+                } else {
+                    P = nx; sgn = -1.0; t = (int)bt_r[j - 1]; val = bv_r[j - 1];
+                }
+                if (!C_(fluid, P, j)) continue;
+                const double un = sgn * U_(i, j);
+                const double phb = (t == 1 && un > 0.0) ? val : C_(phi, P, j);
+                double flux_in = un * phb;
+                if (t == 1) flux_in += C_(diff, P, j) * (val - C_(phi, P, j)) / (0.5 * dx);
+                else if (t == 2) flux_in += val;
+                C_(out, P, j) += flux_in / dx;
+                continue;
+            }
+            const int L = i, R = i < nx ? i + 1 : 1;
+            const int fL = C_(fluid, L, j), fR = C_(fluid, R, j);
+            if (fL && fR) {
+                const double uf = U_(i, j);
+                int Cc, D, Ui;
+                if (uf >= 0.0) {
+                    Cc = L; D = R; Ui = L - 1;
+                    if (Ui < 1) Ui = periodic ? nx : -1;
+                } else {
+                    Cc = R; D = L; Ui = R + 1;
+                    if (Ui > nx) Ui = periodic ? 1 : -1;
+                }
+                const int has_u = Ui >= 1 && C_(fluid, Ui, j);
+                const double phu = has_u ? C_(phi, Ui, j) : 0.0;
+                const double phf = face_value(phu, C_(phi, Cc, j), C_(phi, D, j), has_u, limiter);
+                const double dl = C_(diff, L, j), dr = C_(diff, R, j);
+                const double df = dl + dr > 0.0 ? 2.0 * dl * dr / (dl + dr) : 0.0;
+                const double F = uf * phf - df * (C_(phi, R, j) - C_(phi, L, j)) / dx;
+                C_(out, L, j) -= F / dx;
+                C_(out, R, j) += F / dx;
+            } else if ((fL || fR) && obs_t == 1) {
+                const int P = fL ? L : R;
+                C_(out, P, j) += C_(diff, P, j) * (obs_v - C_(phi, P, j)) / (0.5 * dx) / dx;
+            }
+        }
+    }
+
+    for (int i = 1; i <= nx; ++i) {
+        for (int j = 0; j <= ny; ++j) {
+            if (j == 0 || j == ny) {
+                int P, t;
+                double sgn, val;
+                if (j == 0) { P = 1; sgn = 1.0; t = (int)bt_b[i - 1]; val = bv_b[i - 1]; }
+                else { P = ny; sgn = -1.0; t = (int)bt_t[i - 1]; val = bv_t[i - 1]; }
+                if (!C_(fluid, i, P)) continue;
+                const double vn = sgn * V_(i, j);
+                const double phb = (t == 1 && vn > 0.0) ? val : C_(phi, i, P);
+                double flux_in = vn * phb;
+                if (t == 1) flux_in += C_(diff, i, P) * (val - C_(phi, i, P)) / (0.5 * dy);
+                else if (t == 2) flux_in += val;
+                C_(out, i, P) += flux_in / dy;
+                continue;
+            }
+            const int B = j, T = j + 1;
+            const int fB = C_(fluid, i, B), fT = C_(fluid, i, T);
+            if (fB && fT) {
+                const double vf = V_(i, j);
+                int Cc, D, Uj;
+                if (vf >= 0.0) { Cc = B; D = T; Uj = B - 1; }
+                else { Cc = T; D = B; Uj = T + 1; }
+                const int has_u = 1 <= Uj && Uj <= ny && C_(fluid, i, Uj);
+                const double phu = has_u ? C_(phi, i, Uj) : 0.0;
+                const double phf = face_value(phu, C_(phi, i, Cc), C_(phi, i, D), has_u, limiter);
+                const double db = C_(diff, i, B), dt_ = C_(diff, i, T);
+                const double df = db + dt_ > 0.0 ? 2.0 * db * dt_ / (db + dt_) : 0.0;
+                const double F = vf * phf - df * (C_(phi, i, T) - C_(phi, i, B)) / dy;
+                C_(out, i, B) -= F / dy;
+                C_(out, i, T) += F / dy;
+            } else if ((fB || fT) && obs_t == 1) {
+                const int P = fB ? B : T;
+                C_(out, i, P) += C_(diff, i, P) * (obs_v - C_(phi, i, P)) / (0.5 * dy) / dy;
+            }
+        }
+    }
+
+    for (int i = 0; i < nx + 2; ++i)
+        for (int j = 0; j < ny + 2; ++j)
+            if (!C_(fluid, i, j) || i == 0 || i == nx + 1 || j == 0 || j == ny + 1)
+                C_(out, i, j) = 0.0;
+}
