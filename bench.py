@@ -1,0 +1,225 @@
+"""Benchmark: MAC projection steps per second on a 1024x1024 lid-driven cavity.
+
+Workload (one GPU): lid-driven cavity Re=1000 on 1024x1024, AB2, gamma "auto",
+tau 0.5, with a passive van Leer scalar (D = 1e-3) initialised as 8 Gaussian
+blobs (seed 0).  Every step is the whole projection step (BCs, dt/gamma,
+predictor, pressure solve to relative residual 1e-6, correction, scalar
+transport, monitors), all inputs resident in HBM.  Two solver legs are timed:
+plain FP64 Jacobi-PCG, and the FP32-bulk / FP64-dot mixed-precision PCG
+(the headline ``value``).
+
+Prints one JSON line.  ``roofline`` describes the persistent projection
+kernel (the dominant kernel): algorithmic bytes per launch
+(8 B x N x (3 + 7 + 10 n_it) in FP64, SURVEY-style count: RHS, correction and
+10 vector passes per CG iteration; 4 B per vector element for the FP32 bulk
+of the mixed solver) over its mean launch time, timed with HIP events on the
+launch stream.  ``cpu_baseline`` times the CPU FP64 oracle (SciPy CG) on one
+host core for a bounded number of steps of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+for p in (str(ROOT), str(ROOT / "cfd-_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+N = 1024
+RE = 1000.0
+RTOL = 1e-6
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = ("lid-driven cavity Re=1000, 1024x1024 MAC grid, AB2, gamma auto, tau 0.5, "
+            "passive van Leer scalar (D=1e-3, 8 Gaussian blobs, seed 0), "
+            "pressure solve to relative residual 1e-6")
+
+
+def gaussian_blobs(n: int, seed: int = 0, count: int = 8) -> np.ndarray:
+    """Initial scalar at cell centres: sum of Gaussians with centres U(0.1,0.9)^2,
+    sigma U(0.02,0.08), amplitude U(0,1), drawn in that order."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.1, 0.9, (count, 2))
+    s = rng.uniform(0.02, 0.08, count)
+    a = rng.uniform(0.0, 1.0, count)
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    phi = np.zeros_like(X)
+    for k in range(count):
+        phi += a[k] * np.exp(-((X - c[k, 0]) ** 2 + (Y - c[k, 1]) ** 2) / (2 * s[k] ** 2))
+    return phi
+
+
+def build_case(solver: str, n: int = N):
+    from cfd2d_amd import cases
+
+    case = cases.lid_driven_cavity(re=RE, n=n, pressure_solver=solver, pressure_tol=RTOL,
+                                   pressure_maxiter=50000)
+    cases.add_scalar(case, "c", diffusivity=1e-3, limiter="vanleer", initial=gaussian_blobs(n))
+    return case
+
+
+def run_leg(solver: str, steps: int, warmup: int) -> dict:
+    case = build_case(solver)
+    sim = case.sim
+    ctx = sim._ctx
+    prm = sim._params()
+    t0 = time.perf_counter()
+    ctx.step(prm, warmup)
+    ctx.sync()
+    t_warm = time.perf_counter() - t0
+    ctx.set_profiling(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(prm, steps)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    proj_ms, launches = ctx.projection_times()
+    ctx.set_profiling(False)
+    st = ctx.get_state()
+    recs = ctx.records(st.n - steps + 1, steps)
+    iters = np.array([r.cg_iters for r in recs], dtype=float)
+    relres = np.array([r.cg_relres for r in recs])
+    info = [r.cg_info for r in recs]
+    divmax = max(r.divergence for r in recs)
+    out = {
+        "solver": solver,
+        "steps_per_s": steps / wall,
+        "ms_per_step": 1e3 * wall / steps,
+        "cg_iters_mean": float(iters.mean()),
+        "cg_iters_max": int(iters.max()),
+        "cg_relres_max": float(relres.max()),
+        "not_converged_steps": int(sum(1 for k in info if k != 0)),
+        "max_div": float(divmax),
+        "proj_ms_mean": proj_ms / max(launches, 1),
+        "proj_launches": launches,
+        "warmup_s": t_warm,
+        "t_end": float(st.t),
+        "n_steps_total": int(st.n),
+    }
+    sim.close()
+    return out
+
+
+def roofline_for(leg: dict) -> dict:
+    n_cells = N * N
+    s = 8.0
+    n_it = leg["cg_iters_mean"]
+    if leg["solver"] == "cg_mixed":
+        bytes_per_launch = n_cells * (8.0 * (3 + 7) + 4.0 * 10 * n_it)
+    else:
+        bytes_per_launch = n_cells * s * (3 + 7 + 10 * n_it)
+    t = leg["proj_ms_mean"] * 1e-3
+    achieved = bytes_per_launch / t / 1e9 if t > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_project", "bytes_per_launch": bytes_per_launch,
+            "launch_ms": round(leg["proj_ms_mean"], 4), "cg_iters_per_launch": round(n_it, 2)}
+
+
+# --------------------------------------------------------------------------- #
+# CPU baseline: the FP64 oracle (NumPy/SciPy + serial C kernels) on one core
+# --------------------------------------------------------------------------- #
+def cpu_baseline_child(budget_s: float, max_steps: int) -> dict:
+    from oracle import cfd_oracle as O
+
+    t0 = time.perf_counter()
+    sim = O.cavity(re=RE, n=N, pressure_solver="cg", pressure_tol=RTOL)
+    sim.pressure.maxiter = 50000
+    O.add_scalar(sim, "c", 1e-3, limiter="vanleer", initial=gaussian_blobs(N))
+    setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    k = 0
+    while k < max_steps and time.perf_counter() - t0 < budget_s:
+        sim.step()
+        k += 1
+    wall = time.perf_counter() - t0
+    return {"steps": k, "wall_s": wall, "setup_s": setup, "cg_iters": sim.cg_iters}
+
+
+def cpu_baseline(budget_s: float = 20.0, max_steps: int = 30) -> dict | None:
+    env = dict(os.environ)
+    env.update({"OMP_NUM_THREADS": "1", "OPENBLAS_NUM_THREADS": "1", "MKL_NUM_THREADS": "1"})
+    cmd = [sys.executable, str(Path(__file__).resolve()), "--cpu-child", str(budget_s),
+           str(max_steps)]
+    try:
+        res = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=600)
+        line = [l for l in res.stdout.splitlines() if l.startswith("{")][-1]
+        r = json.loads(line)
+    except Exception as e:  # pragma: no cover - reported, not fatal
+        print(f"cpu baseline failed: {e}", file=sys.stderr)
+        return None
+    return {"value": round(r["steps"] / r["wall_s"], 4), "unit": "steps/s", "cores": 1,
+            "kind": "port",
+            "sample": (f"FP64 oracle (serial C kernels + SciPy Jacobi-CG, rtol 1e-6), "
+                       f"first {r['steps']} steps of the same 1024x1024 workload from rest, "
+                       f"{r['wall_s']:.1f} s, one thread; CG iterations "
+                       f"{r['cg_iters']}")}
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--legs", default="cg,cg_mixed")
+    ap.add_argument("--cpu-child", nargs=2, default=None)
+    args = ap.parse_args()
+    if args.cpu_child:
+        print(json.dumps(cpu_baseline_child(float(args.cpu_child[0]), int(args.cpu_child[1]))))
+        return
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path is "
+              "single-GPU in this build", file=sys.stderr)
+        sys.exit(2)
+
+    legs = {}
+    for solver in args.legs.split(","):
+        try:
+            legs[solver] = run_leg(solver, args.steps, args.warmup)
+        except ValueError as e:
+            legs[solver] = {"solver": solver, "error": str(e)}
+        print(f"leg {solver}: {json.dumps(legs[solver])}", file=sys.stderr, flush=True)
+    ok = [l for l in legs.values() if "error" not in l]
+    head = legs.get("cg_mixed") if "error" not in legs.get("cg_mixed", {"error": 1}) else None
+    head = head or legs.get("cg")
+    base = legs.get("cg")
+    cpu = None if args.no_cpu else cpu_baseline()
+    line = {
+        "metric": "MAC steps/s @1024² cavity: learned-PCG fp32 vs plain fp64 CG "
+                  "(resid 1e-6)",
+        "value": round(head["steps_per_s"], 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if head["solver"] == "cg_mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": WORKLOAD, "grid": [N, N], "solver": head["solver"],
+                   "rtol": RTOL},
+        "roofline": roofline_for(head),
+        "cpu_baseline": cpu,
+        "fp64_cg_steps_per_s": round(base["steps_per_s"], 3) if base and "error" not in base
+        else None,
+        "speedup_vs_fp64_cg": (round(head["steps_per_s"] / base["steps_per_s"], 3)
+                               if base and "error" not in base else None),
+        "legs": legs,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

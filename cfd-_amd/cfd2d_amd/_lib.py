@@ -97,7 +97,12 @@ def lib() -> ctypes.CDLL:
         "cfd_sync": ([_vp], _i32),
         "cfd_read_records": ([_vp, _i64, _i32, P(Record)], _i32),
         "cfd_last_elapsed_ms": ([_vp, P(_d)], _i32),
+        "cfd_set_profiling": ([_vp, _i32], _i32),
+        "cfd_projection_times": ([_vp, P(_d), P(_i64)], _i32),
         "cfd_pressure_solve": ([_vp, _vp, _vp, _i32, _d, _i32, P(_i32), P(_d)], _i32),
+        "cfd_dev_alloc": ([_i64, P(_vp)], _i32),
+        "cfd_dev_free": ([_vp], _i32),
+        "cfd_memcpy": ([_vp, _vp, _i64, _i32], _i32),
         "cfd_momentum_tendencies": ([_vp, _vp, _vp, _i32, _i32, _d, _d, _d, _d, _i32, _vp, _vp,
                                      _vp], _i32),
         "cfd_obstacle_bc": ([_vp, _vp, _vp, _i32, _i32, _i32, _vp], _i32),
@@ -118,7 +123,9 @@ def exported_symbols() -> list[str]:
                         "cfd_download", "cfd_upload_scalar", "cfd_download_scalar",
                         "cfd_set_state", "cfd_get_state", "cfd_apply_bc",
                         "cfd_project_initial", "cfd_step", "cfd_sync", "cfd_read_records",
-                        "cfd_last_elapsed_ms", "cfd_pressure_solve", "cfd_momentum_tendencies",
+                        "cfd_last_elapsed_ms", "cfd_set_profiling", "cfd_projection_times",
+                        "cfd_pressure_solve", "cfd_dev_alloc",
+                        "cfd_dev_free", "cfd_memcpy", "cfd_momentum_tendencies",
                         "cfd_obstacle_bc", "cfd_scalar_transport")]
 
 
@@ -141,6 +148,39 @@ def device_count() -> int:
     n = _i32(0)
     lib().cfd_device_count(ctypes.byref(n))
     return int(n.value)
+
+
+class DeviceArray:
+    """A device buffer holding a copy of a NumPy array (C-order), for the
+    stateless kernel entry points."""
+
+    def __init__(self, host: np.ndarray):
+        self.shape, self.dtype = host.shape, host.dtype
+        self.nbytes = int(host.nbytes)
+        p = _vp()
+        check(lib().cfd_dev_alloc(self.nbytes, ctypes.byref(p)), "device alloc")
+        self.ptr = p
+        self.upload(host)
+
+    def upload(self, host: np.ndarray) -> None:
+        h = np.ascontiguousarray(host, dtype=self.dtype)
+        check(lib().cfd_memcpy(self.ptr, h.ctypes.data_as(_vp), self.nbytes, 1), "copy to device")
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(lib().cfd_memcpy(out.ctypes.data_as(_vp), self.ptr, self.nbytes, 2), "copy to host")
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().cfd_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def ptr(a: np.ndarray | None):

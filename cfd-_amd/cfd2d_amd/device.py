@@ -132,6 +132,16 @@ class DeviceContext:
         _lib.check(_lib.lib().cfd_last_elapsed_ms(self.handle, ctypes.byref(ms)), "elapsed")
         return float(ms.value)
 
+    def set_profiling(self, on: bool) -> None:
+        _lib.check(_lib.lib().cfd_set_profiling(self.handle, 1 if on else 0), "profiling")
+
+    def projection_times(self) -> tuple[float, int]:
+        ms = ctypes.c_double(0.0)
+        n = ctypes.c_int64(0)
+        _lib.check(_lib.lib().cfd_projection_times(self.handle, ctypes.byref(ms), ctypes.byref(n)),
+                   "projection times")
+        return float(ms.value), int(n.value)
+
     def records(self, first_n: int, count: int) -> list:
         if count <= 0:
             return []

@@ -79,6 +79,10 @@ struct cfd_ctx {
     StepRecord* ring = nullptr;
     std::vector<ScalarDev> scalars;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // optional timing of every projection launch (HIP events on the context stream)
+    int prof_on = 0;
+    size_t prof_used = 0;
+    std::vector<hipEvent_t> prof_ev;
     std::vector<void*> allocs;
 };
 
@@ -247,8 +251,22 @@ int launch_project(cfd_ctx* c, ProjArgs a) {
     HIPC(hipGetLastError());
     Geo g = c->geo;
     void* args[] = {(void*)&a, (void*)&g};
+    if (c->prof_on) {
+        if (c->prof_used + 2 > c->prof_ev.size()) {
+            for (int k = 0; k < 256; ++k) {
+                hipEvent_t e;
+                HIPC(hipEventCreate(&e));
+                c->prof_ev.push_back(e);
+            }
+        }
+        HIPC(hipEventRecord(c->prof_ev[c->prof_used], c->stream));
+    }
     HIPC(hipLaunchCooperativeKernel((const void*)k_project, dim3(c->proj_blocks),
                                     dim3(c->proj_bs), args, 0, c->stream));
+    if (c->prof_on) {
+        HIPC(hipEventRecord(c->prof_ev[c->prof_used + 1], c->stream));
+        c->prof_used += 2;
+    }
     return 0;
 }
 
@@ -423,6 +441,7 @@ int cfd_ctx_create(const cfd_grid_desc* d, cfd_ctx** out) {
 int cfd_ctx_destroy(cfd_ctx* c) {
     if (!c) return 0;
     if (c->stream) hipStreamSynchronize(c->stream);
+    for (hipEvent_t e : c->prof_ev) hipEventDestroy(e);
     for (void* p : c->allocs) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -685,6 +704,29 @@ int cfd_last_elapsed_ms(cfd_ctx* c, double* ms) {
     return 0;
 }
 
+int cfd_set_profiling(cfd_ctx* c, int32_t on) {
+    if (!c) return set_err(CFD_ERR_ARG, "null context");
+    HIPC(hipStreamSynchronize(c->stream));
+    c->prof_on = on ? 1 : 0;
+    c->prof_used = 0;
+    return 0;
+}
+
+int cfd_projection_times(cfd_ctx* c, double* total_ms, int64_t* launches) {
+    if (!c || !total_ms || !launches) return set_err(CFD_ERR_ARG, "null argument");
+    HIPC(hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    for (size_t k = 0; k + 1 < c->prof_used; k += 2) {
+        float ms = 0.0f;
+        HIPC(hipEventElapsedTime(&ms, c->prof_ev[k], c->prof_ev[k + 1]));
+        tot += ms;
+    }
+    *total_ms = tot;
+    *launches = (int64_t)(c->prof_used / 2);
+    c->prof_used = 0;
+    return 0;
+}
+
 int cfd_read_records(cfd_ctx* c, int64_t first_n, int32_t count, cfd_record* out) {
     if (!c || !out || count < 0) return set_err(CFD_ERR_ARG, "bad arguments");
     static_assert(sizeof(cfd_record) == sizeof(StepRecord), "record layout");
@@ -733,6 +775,33 @@ int cfd_pressure_solve(cfd_ctx* c, const double* rhs, double* p_out, int32_t sol
         HIPC(hipStreamSynchronize(c->stream));
     }
     return h2.cg_info > 0 ? h2.cg_info : 0;
+}
+
+// ---- device memory helpers ----
+int cfd_dev_alloc(int64_t bytes, void** out) {
+    if (!out || bytes < 0) return set_err(CFD_ERR_ARG, "bad arguments");
+    *out = nullptr;
+    HIPC(hipMalloc(out, std::max<int64_t>(bytes, 1)));
+    return 0;
+}
+
+int cfd_dev_free(void* ptr) {
+    if (ptr) HIPC(hipFree(ptr));
+    return 0;
+}
+
+int cfd_memcpy(void* dst, const void* src, int64_t bytes, int32_t kind) {
+    if ((!dst || !src) && bytes > 0) return set_err(CFD_ERR_ARG, "null pointer");
+    hipMemcpyKind k;
+    switch (kind) {
+    case 1: k = hipMemcpyHostToDevice; break;
+    case 2: k = hipMemcpyDeviceToHost; break;
+    case 3: k = hipMemcpyDeviceToDevice; break;
+    default: return set_err(CFD_ERR_ARG, "bad copy kind %d", kind);
+    }
+    if (bytes > 0) HIPC(hipMemcpy(dst, src, (size_t)bytes, k));
+    HIPC(hipDeviceSynchronize());
+    return 0;
 }
 
 // ---- stateless kernels on reference-layout device arrays ----

@@ -164,11 +164,24 @@ int cfd_sync(cfd_ctx* ctx);
 int cfd_read_records(cfd_ctx* ctx, int64_t first_n, int32_t count, cfd_record* out);
 /* Time of the last cfd_step batch on the device, in ms (HIP events). */
 int cfd_last_elapsed_ms(cfd_ctx* ctx, double* ms);
+/* Time every pressure-projection launch with HIP events on the context's
+   stream (measurement aid, no reference counterpart): switch on, run steps,
+   then read the summed kernel time and launch count (which resets them). */
+int cfd_set_profiling(cfd_ctx* ctx, int32_t on);
+int cfd_projection_times(cfd_ctx* ctx, double* total_ms, int64_t* launches);
 
 /* PressureSolver.solve (pressure.py:143-185) on a host right-hand side
    (nx x ny) -> host pressure (nx x ny); warm start from the context's X. */
 int cfd_pressure_solve(cfd_ctx* ctx, const double* rhs, double* p_out, int32_t solver,
                        double rtol, int32_t maxiter, int32_t* iters, double* relres);
+
+/* ---- device memory for callers of the stateless kernels (no counterpart in
+   the reference, which has no device; these replace framework tensors, since a
+   second HIP runtime in the same process cannot open the GPU) ---- */
+int cfd_dev_alloc(int64_t bytes, void** out);
+int cfd_dev_free(void* ptr);
+/* kind: 1 host->device, 2 device->host, 3 device->device; synchronous */
+int cfd_memcpy(void* dst, const void* src, int64_t bytes, int32_t kind);
 
 /* ---- stateless kernels on caller device arrays in the reference layout ---- */
 /* kernels.momentum_tendencies (kernels.py:44-155), constant viscosity nu0 */

@@ -18,17 +18,13 @@ pytestmark = pytest.mark.gpu
 
 
 def _dev(a, dtype=None):
-    import torch
+    from cfd2d_amd._lib import DeviceArray
 
-    t = torch.from_numpy(np.ascontiguousarray(a if dtype is None else a.astype(dtype)))
-    return t.to("cuda")
+    return DeviceArray(np.ascontiguousarray(a if dtype is None else a.astype(dtype)))
 
 
-def _host(t):
-    import torch
-
-    torch.cuda.synchronize()
-    return t.cpu().numpy()
+def _host(d):
+    return d.get()
 
 
 @pytest.fixture(scope="module")
@@ -46,9 +42,9 @@ def test_momentum_tendencies_bitexact(gpu, gold, tag, periodic, gi, gamma):
     fl = _dev(gold[f"fluid_{tag}"], np.uint8)
     hu = _dev(np.full((nx + 1, ny + 2), 7.0))
     hv = _dev(np.full((nx + 2, ny + 1), 7.0))
-    rc = _lib.lib().cfd_momentum_tendencies(u.data_ptr(), v.data_ptr(), fl.data_ptr(), nx, ny,
+    rc = _lib.lib().cfd_momentum_tendencies(u.ptr, v.ptr, fl.ptr, nx, ny,
                                             float(gold["dx"]), float(gold["dy"]), 0.0137, gamma,
-                                            int(periodic), hu.data_ptr(), hv.data_ptr(), None)
+                                            int(periodic), hu.ptr, hv.ptr, None)
     _lib.check(rc, "momentum")
     assert np.array_equal(_host(hu), gold[f"mom_hu_{tag}_g{gi}"])
     assert np.array_equal(_host(hv), gold[f"mom_hv_{tag}_g{gi}"])
@@ -61,7 +57,7 @@ def test_obstacle_bc_bitexact(gpu, gold, tag, periodic):
     nx, ny = int(gold["nx"]), int(gold["ny"])
     u, v = _dev(gold[f"u_{tag}"]), _dev(gold[f"v_{tag}"])
     fl = _dev(gold[f"fluid_{tag}"], np.uint8)
-    _lib.check(_lib.lib().cfd_obstacle_bc(u.data_ptr(), v.data_ptr(), fl.data_ptr(), nx, ny,
+    _lib.check(_lib.lib().cfd_obstacle_bc(u.ptr, v.ptr, fl.ptr, nx, ny,
                                           int(periodic), None), "obstacle_bc")
     assert np.array_equal(_host(u), gold[f"obc_u_{tag}"])
     assert np.array_equal(_host(v), gold[f"obc_v_{tag}"])
@@ -82,11 +78,11 @@ def test_scalar_transport_bitexact(gpu, gold, tag, periodic, obs, lim):
     out = _dev(np.full((nx + 2, ny + 2), 3.0))
     obs_v = 0.37 if obs else 0.0
     rc = _lib.lib().cfd_scalar_transport(
-        args[0].data_ptr(), args[1].data_ptr(), args[2].data_ptr(), args[3].data_ptr(), nx, ny,
-        float(gold["dx"]), float(gold["dy"]), diff.data_ptr(), 0.0, lim, int(periodic),
-        bts[0].data_ptr(), bvs[0].data_ptr(), bts[1].data_ptr(), bvs[1].data_ptr(),
-        bts[2].data_ptr(), bvs[2].data_ptr(), bts[3].data_ptr(), bvs[3].data_ptr(),
-        obs, obs_v, out.data_ptr(), None)
+        args[0].ptr, args[1].ptr, args[2].ptr, args[3].ptr, nx, ny,
+        float(gold["dx"]), float(gold["dy"]), diff.ptr, 0.0, lim, int(periodic),
+        bts[0].ptr, bvs[0].ptr, bts[1].ptr, bvs[1].ptr,
+        bts[2].ptr, bvs[2].ptr, bts[3].ptr, bvs[3].ptr,
+        obs, obs_v, out.ptr, None)
     _lib.check(rc, "scalar_transport")
     assert np.array_equal(_host(out), gold[f"tvd_{tag}_o{obs}_l{lim}"])
 

@@ -1,0 +1,158 @@
+"""Host-side logic of cfd2d_amd and the C ABI library, without a GPU."""
+
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from oracle import cfd_oracle as O
+
+
+def test_library_exports_every_declared_symbol():
+    from cfd2d_amd import _lib
+
+    header = (ROOT / "include" / "cfd2d_hip.h").read_text()
+    declared = sorted(set(re.findall(r"^(?:int|const char\*)\s+(cfd_\w+)\(", header, re.M)))
+    assert len(declared) >= 20
+    out = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)],
+                         capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r" T (cfd_\w+)$", out, re.M))
+    assert set(declared) <= exported, set(declared) - exported
+    assert set(declared) == set(_lib.exported_symbols())
+    L = _lib.lib()
+    for name in declared:
+        assert getattr(L, name) is not None
+
+
+def test_no_device_means_no_simulation():
+    from cfd2d_amd import _lib, cases
+
+    if _lib.device_count() > 0:
+        pytest.skip("a HIP device is present")
+    with pytest.raises(RuntimeError, match="GPU"):
+        cases.lid_driven_cavity(n=16)
+
+
+def _pressure_pair(kind):
+    from cfd2d_amd.boundary import BoundaryCondition as BC
+    from cfd2d_amd.boundary import DomainBoundary
+    from cfd2d_amd.grid import Grid
+    from cfd2d_amd import geometry
+    from cfd2d_amd.pressure import PressureSolver
+
+    if kind == "cavity":
+        g = Grid(20, 16, 1.0, 0.8)
+        og = O.OGrid(20, 16, 1.0, 0.8)
+        bcs = {s: BC("noslip") for s in ("left", "right", "bottom", "top")}
+        obcs = {s: O.OBC("noslip") for s in ("left", "right", "bottom", "top")}
+    elif kind == "channel":
+        g = Grid(40, 12, 4.0, 1.0)
+        og = O.OGrid(40, 12, 4.0, 1.0)
+        for gg in (g,):
+            gg.add_obstacle(geometry.circle(gg, 1.0, 0.5, 0.2))
+            gg.add_obstacle(geometry.rectangle(gg, 2.5, 0.0, 2.8, 1.0))  # splits the domain
+        og.add_obstacle(og.circle(1.0, 0.5, 0.2))
+        X, Y = og.cell_centers()
+        og.add_obstacle((X >= 2.5) & (X <= 2.8) & (Y >= 0.0) & (Y <= 1.0))
+        bcs = {"left": BC("inflow", 1.0, "parabolic"), "right": BC("outflow"),
+               "bottom": BC("slip"), "top": BC("slip")}
+        obcs = {"left": O.OBC("inflow", 1.0, "parabolic"), "right": O.OBC("outflow"),
+                "bottom": O.OBC("slip"), "top": O.OBC("slip")}
+    else:
+        g = Grid(16, 10, 1.0, 0.5)
+        og = O.OGrid(16, 10, 1.0, 0.5)
+        bcs = {"left": BC("periodic"), "right": BC("periodic"), "bottom": BC("noslip"),
+               "top": BC("outflow")}
+        obcs = {"left": O.OBC("periodic"), "right": O.OBC("periodic"),
+                "bottom": O.OBC("noslip"), "top": O.OBC("outflow")}
+    g.finalize()
+    og.finalize()
+    ps = PressureSolver(g, DomainBoundary(g, bcs), method="cg")
+    ops = O.OPressure(og, O.OBoundary(og, obcs), method="cg")
+    return ps, ops
+
+
+@pytest.mark.parametrize("kind", ["cavity", "channel", "periodic"])
+def test_pressure_description_matches_oracle_assembly(kind):
+    ps, ops = _pressure_pair(kind)
+    fl = ps.fluid
+    idx = ps.code & 63
+    assert np.array_equal(idx > 0, fl)
+    diag = ps.diag_table[idx]
+    assert np.array_equal(diag[fl], ops.diag)                 # exact, reference order
+    assert np.array_equal(ps.inv_table[idx][fl], 1.0 / ops.diag)
+    assert ps._pinned == ops.pinned
+    assert len(ps.neumann_components) == len(ops.neumann)
+    # matrix-free stencil with these codes reproduces the assembled matrix
+    nx, ny = fl.shape
+    rng = np.random.default_rng(0)
+    x = np.where(fl, rng.standard_normal(fl.shape), 0.0)
+    xp = np.zeros((nx + 2, ny + 2))
+    xp[1:-1, 1:-1] = x
+    if ps.periodic:
+        xp[0, 1:-1], xp[-1, 1:-1] = x[-1], x[0]
+    ax, ay = 1 / ps.grid.dx ** 2, 1 / ps.grid.dy ** 2
+    Ax = diag * x - ax * (xp[2:, 1:-1] + xp[:-2, 1:-1]) - ay * (xp[1:-1, 2:] + xp[1:-1, :-2])
+    assert np.allclose(Ax[fl], ops.A @ x[fl], rtol=1e-13, atol=1e-9 * ax)
+    dirx = (ps.code & 64) > 0
+    assert np.array_equal(dirx[-1, :], ops.dirichlet["right"] & fl[-1, :])
+
+
+def test_inflow_profile_matches_oracle():
+    from cfd2d_amd.boundary import inflow_profile
+
+    y = (np.arange(40) + 0.5) / 40
+    open_mask = np.ones(40, bool)
+    open_mask[10:14] = False
+    open_mask[30] = False
+    for prof in ("uniform", "parabolic"):
+        a = inflow_profile(y, open_mask, 1 / 40, 1.3, prof)
+        b = O.inflow_profile(y, open_mask, 1 / 40, 1.3, prof)
+        assert np.array_equal(a, b)
+    p = inflow_profile(y, np.ones(40, bool), 1 / 40, 1.0, "parabolic")
+    assert p.sum() / 40 == pytest.approx(1.0, rel=1e-3)
+
+
+def test_grid_finalize_removes_enclosed_fluid():
+    from cfd2d_amd import geometry
+    from cfd2d_amd.grid import Grid
+
+    g = Grid(40, 40, 1.0, 1.0)
+    ring = geometry.circle(g, 0.5, 0.5, 0.3) & ~geometry.circle(g, 0.5, 0.5, 0.2)
+    g.add_obstacle(ring)
+    assert g.finalize() > 0
+    assert g.solid[20, 20]
+
+
+def test_validation_errors():
+    from cfd2d_amd.boundary import BoundaryCondition as BC
+    from cfd2d_amd.grid import Grid
+    from cfd2d_amd.solver import Simulation
+
+    with pytest.raises(ValueError):
+        BC("wand")
+    with pytest.raises(ValueError):
+        Grid(3, 8, 1, 1)
+    g = Grid(8, 8, 1, 1)
+    bcs = {s: BC("noslip") for s in ("left", "right", "bottom", "top")}
+    for kw in ({"nu": 0.0}, {"nu": 1.0, "scheme": "rk4"}, {"nu": 1.0, "tau": 1.5},
+               {"nu": 1.0, "gamma": 2.0}, {"nu": 1.0, "pressure_solver": "lu"}):
+        with pytest.raises(ValueError):
+            Simulation(g, bcs, **kw)
+
+
+def test_case_json_roundtrip(tmp_path):
+    from cfd2d_amd import cases
+
+    spec = {"fall": "custom", "parameter": {"obstacles": [{"typ": "kreis", "x": 1, "y": 0.5,
+                                                           "r": 0.1}], "nx": 64, "ny": 16},
+            "numerik": {"scheme": "ab2", "pressure_solver": "cg_mixed", "pressure_tol": 1e-6},
+            "skalar": {"name": "c", "diffusivity": 1e-3, "inflow_band": [0.25, 0.75],
+                       "limiter": "superbee"}}
+    cases.save_case(spec, tmp_path / "case.json")
+    assert cases.load_case(tmp_path / "case.json") == spec
+    with pytest.raises(ValueError):
+        cases.build_case({"fall": "warp_drive"})
