@@ -114,7 +114,10 @@ def roofline_for(leg: dict) -> dict:
     s = 8.0
     n_it = leg["cg_iters_mean"]
     if leg["solver"] == "cg_mixed":
-        bytes_per_launch = n_cells * (8.0 * (3 + 7) + 4.0 * 10 * n_it)
+        # per flexible-CG iteration: FP64 outer vectors (r, r_old, x, p, q: 104 B/cell)
+        # plus the FP32 V-cycle (fine level 50 B/cell, coarse levels 1/3 more
+        # with their 12 B/cell of coefficients: ~66 B/cell)
+        bytes_per_launch = n_cells * (8.0 * (3 + 7) + n_it * (104.0 + 66.0))
     else:
         bytes_per_launch = n_cells * s * (3 + 7 + 10 * n_it)
     t = leg["proj_ms_mean"] * 1e-3

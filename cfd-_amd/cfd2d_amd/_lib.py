@@ -25,7 +25,7 @@ CFD_ERR_LIMIT = -4
 BC_KINDS = {"noslip": 0, "slip": 1, "inflow": 2, "outflow": 3, "periodic": 4}
 FIELDS = {"u": 0, "v": 1, "p": 2, "F": 3, "G": 4, "hu": 5, "hv": 6, "hu_old": 7, "hv_old": 8,
           "x": 9}
-SOLVERS = {"pcg64": 0, "mixed": 1}
+SOLVERS = {"pcg64": 0, "mgcg": 1}
 
 _d = ctypes.c_double
 _i32 = ctypes.c_int32
@@ -52,6 +52,15 @@ class StepParams(ctypes.Structure):
                 ("clamp_t_end", _i32), ("solver", _i32), ("fx", _d), ("fy", _d), ("rtol", _d),
                 ("maxiter", _i32), ("check_t_end", _i32), ("reliable_delta", _d), ("steady_tol", _d),
                 ("max_steps", _i64)]
+
+
+class MGLevelDesc(ctypes.Structure):
+    _fields_ = [("nx", _i32), ("ny", _i32), ("diag", _vp), ("ce", _vp), ("cn", _vp)]
+
+
+class MGDesc(ctypes.Structure):
+    _fields_ = [("nlev", _i32), ("omega", ctypes.c_float), ("wc", ctypes.c_float),
+                ("ncoarse", _i32), ("cinv", _vp), ("lev", MGLevelDesc * 16)]
 
 
 class State(ctypes.Structure):
@@ -92,6 +101,7 @@ def lib() -> ctypes.CDLL:
         "cfd_set_state": ([_vp, P(State)], _i32),
         "cfd_get_state": ([_vp, P(State)], _i32),
         "cfd_apply_bc": ([_vp], _i32),
+        "cfd_set_multigrid": ([_vp, P(MGDesc)], _i32),
         "cfd_project_initial": ([_vp, P(StepParams)], _i32),
         "cfd_step": ([_vp, P(StepParams), _i32], _i32),
         "cfd_sync": ([_vp], _i32),
@@ -121,7 +131,7 @@ def exported_symbols() -> list[str]:
     return [n for n in ("cfd_last_error", "cfd_device_count", "cfd_ctx_create",
                         "cfd_ctx_destroy", "cfd_ctx_set_stream", "cfd_add_scalar", "cfd_upload",
                         "cfd_download", "cfd_upload_scalar", "cfd_download_scalar",
-                        "cfd_set_state", "cfd_get_state", "cfd_apply_bc",
+                        "cfd_set_state", "cfd_get_state", "cfd_set_multigrid", "cfd_apply_bc",
                         "cfd_project_initial", "cfd_step", "cfd_sync", "cfd_read_records",
                         "cfd_last_elapsed_ms", "cfd_set_profiling", "cfd_projection_times",
                         "cfd_pressure_solve", "cfd_dev_alloc",

@@ -14,8 +14,9 @@ The device solves the system matrix-free.  This host part describes it once:
 Methods: ``"cg"`` is Jacobi-PCG in FP64 with the reference's semantics (warm
 start, recursive-residual stop, ``maxiter`` 5000); ``"direct"`` (the
 reference's LU) runs the same solver to rtol 1e-12 with a large iteration cap;
-``"cg_mixed"`` keeps the vectors in FP32 with FP64 dot products and FP64
-true-residual replacement.
+``"cg_mixed"`` is flexible CG with FP64 vectors and dot products,
+preconditioned by an FP32 aggregation-multigrid V-cycle
+(:mod:`cfd2d_amd.multigrid`), ending on the FP64 true residual.
 """
 
 from __future__ import annotations
@@ -133,7 +134,7 @@ class PressureSolver:
     # ------------------------------------------------------------------ #
     @property
     def solver_id(self) -> int:
-        return _lib.SOLVERS["mixed"] if self.method == "cg_mixed" else _lib.SOLVERS["pcg64"]
+        return _lib.SOLVERS["mgcg"] if self.method == "cg_mixed" else _lib.SOLVERS["pcg64"]
 
     @property
     def rtol(self) -> float:
@@ -141,6 +142,13 @@ class PressureSolver:
 
     def attach(self, ctx) -> None:
         self._ctx = ctx
+        if self.method == "cg_mixed":
+            from .multigrid import Hierarchy
+
+            g = self.grid
+            self.hierarchy = Hierarchy(self.fluid, self.diag, 1.0 / g.dx ** 2, 1.0 / g.dy ** 2,
+                                       self.periodic)
+            self.hierarchy.upload(ctx)
 
     def solve(self, rhs: np.ndarray) -> np.ndarray:
         """Solve ``Lap(p) = rhs`` for rhs of shape (nx, ny) on the device

@@ -72,12 +72,15 @@ struct cfd_ctx {
     double *lut = nullptr, *comp_count = nullptr;
     double* inflow[4] = {nullptr, nullptr, nullptr, nullptr};
     uint8_t* open[4] = {nullptr, nullptr, nullptr, nullptr};
-    double *red = nullptr, *part = nullptr;
+    double *sync = nullptr, *part = nullptr;
     unsigned *bar = nullptr, *ticket = nullptr;
     unsigned long long* acc = nullptr;
     DevState* st = nullptr;
     StepRecord* ring = nullptr;
     std::vector<ScalarDev> scalars;
+    double* rold = nullptr;
+    MGDev* mg_dev = nullptr;     // device copy of the hierarchy description
+    int mg_ready = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // optional timing of every projection launch (HIP events on the context stream)
     int prof_on = 0;
@@ -233,8 +236,9 @@ ProjArgs proj_args(cfd_ctx* c, int method, double rtol, int maxiter) {
     a.q32 = c->q32;
     a.d32_0 = c->e0;
     a.d32_1 = c->e1;
-    a.red = c->red;
-    a.bar = c->bar;
+    a.sync = c->sync;
+    a.rold = c->rold;
+    a.mg = c->mg_dev;
     a.st = c->st;
     a.rtol = rtol;
     a.maxiter = maxiter;
@@ -245,9 +249,13 @@ ProjArgs proj_args(cfd_ctx* c, int method, double rtol, int maxiter) {
 }
 
 int launch_project(cfd_ctx* c, ProjArgs a) {
-    if (a.method != PROJ_PCG64)
+    if (a.method != PROJ_PCG64 && a.method != PROJ_MG)
         return set_err(CFD_ERR_ARG, "pressure solver %d is not available", a.method);
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->bar, 1);
+    if (a.method == PROJ_MG && !c->mg_ready)
+        return set_err(CFD_ERR_STATE, "multigrid solver selected before cfd_set_multigrid");
+    const int words = (c->proj_blocks + 2) * CFD_SLOT * 2;
+    hipLaunchKernelGGL(k_zero_u32, dim3((words + 255) / 256), dim3(256), 0, c->stream,
+                       (unsigned*)c->sync, words);
     HIPC(hipGetLastError());
     Geo g = c->geo;
     void* args[] = {(void*)&a, (void*)&g};
@@ -370,6 +378,7 @@ int cfd_ctx_create(const cfd_grid_desc* d, cfd_ctx** out) {
         (rc = dalloc(c, &c->x, N)) || (rc = dalloc(c, &c->b, N)) || (rc = dalloc(c, &c->r, N)) ||
         (rc = dalloc(c, &c->q, N)) || (rc = dalloc(c, &c->d0, N)) || (rc = dalloc(c, &c->d1, N)) ||
         (rc = dalloc(c, &c->tmp0, N)) || (rc = dalloc(c, &c->tmp1, N)) ||
+        (rc = dalloc(c, &c->rold, N)) ||
         (rc = dalloc(c, &c->r32, N)) || (rc = dalloc(c, &c->q32, N)) ||
         (rc = dalloc(c, &c->e0, N)) || (rc = dalloc(c, &c->e1, N)) ||
         (rc = dalloc(c, &c->fl, N)) || (rc = dalloc(c, &c->code, N)) ||
@@ -385,8 +394,8 @@ int cfd_ctx_create(const cfd_grid_desc* d, cfd_ctx** out) {
             hipSuccess || per_cu < 1)
         return fail(set_err(CFD_ERR_HIP, "projection kernel does not fit on a CU"));
     c->proj_blocks = std::max(1, std::min(c->num_cu, c->nx));
-    c->red_blocks = std::max(c->proj_blocks, 4 * c->num_cu);
-    if ((rc = dalloc(c, &c->red, (size_t)2 * CFD_MAX_COMP * c->proj_blocks)) ||
+    c->red_blocks = std::max(1, std::min(c->num_cu, c->nx + 2));
+    if ((rc = dalloc(c, &c->sync, (size_t)(c->proj_blocks + 2) * CFD_SLOT)) ||
         (rc = dalloc(c, &c->part, (size_t)REC_NV * c->red_blocks)))
         return fail(rc);
 
@@ -482,6 +491,66 @@ int cfd_add_scalar(cfd_ctx* c, const cfd_scalar_desc* d, int32_t* id) {
     for (int k = 0; k < 4; ++k) s.desc.bt[k] = nullptr, s.desc.bv[k] = nullptr;
     c->scalars.push_back(s);
     *id = (int32_t)c->scalars.size() - 1;
+    return 0;
+}
+
+int cfd_set_multigrid(cfd_ctx* c, const cfd_mg_desc* d) {
+    if (!c || !d) return set_err(CFD_ERR_ARG, "null argument");
+    if (d->nlev < 2 || d->nlev > MG_MAX_LEVELS) return set_err(CFD_ERR_ARG, "nlev %d", d->nlev);
+    if (d->lev[0].nx != c->nx || d->lev[0].ny != c->ny)
+        return set_err(CFD_ERR_ARG, "level 0 must be the context grid");
+    const int last = d->nlev - 1;
+    if (d->ncoarse != d->lev[last].nx * d->lev[last].ny || d->ncoarse > 4096 || !d->cinv)
+        return set_err(CFD_ERR_ARG, "bad coarsest level (%d cells)", d->ncoarse);
+    HIPC(hipStreamSynchronize(c->stream));
+    MGDev h;
+    std::memset(&h, 0, sizeof(h));
+    h.nlev = d->nlev;
+    h.omega = d->omega;
+    h.wc = d->wc;
+    h.ncoarse = d->ncoarse;
+    for (int l = 0; l < d->nlev; ++l) {
+        const cfd_mg_level_desc& L = d->lev[l];
+        MGLevelDev& D = h.lev[l];
+        D.nx = L.nx;
+        D.ny = L.ny;
+        if (l > 0) {
+            const cfd_mg_level_desc& F = d->lev[l - 1];
+            if (L.nx != (F.nx + 1) / 2 || L.ny != (F.ny + 1) / 2)
+                return set_err(CFD_ERR_ARG, "level %d is not the 2x2 coarsening of level %d", l, l - 1);
+            if (!L.diag || !L.ce || !L.cn) return set_err(CFD_ERR_ARG, "level %d coefficients", l);
+        }
+        D.P = l == 0 ? c->P : ((L.ny + 2 + 31) / 32) * 32;
+        D.grid = l == 0 ? 1 : ((long)L.nx * L.ny > 4096 && h.lev[l - 1].grid ? 1 : 0);
+        const size_t n = (size_t)(L.nx + 2) * D.P;
+        float *b, *x, *t;
+        TRY(dalloc(c, &b, n));
+        TRY(dalloc(c, &x, n));
+        TRY(dalloc(c, &t, n));
+        D.b = b;
+        D.x = x;
+        D.t = t;
+        if (l > 0) {
+            float* co[3];
+            const float* src[3] = {L.diag, L.ce, L.cn};
+            for (int k = 0; k < 3; ++k) {
+                TRY(dalloc(c, &co[k], n));
+                HIPC(hipMemcpy2D(co[k] + D.P + 1, D.P * sizeof(float), src[k], L.ny * sizeof(float),
+                                 L.ny * sizeof(float), L.nx, hipMemcpyHostToDevice));
+            }
+            D.diag = co[0];
+            D.cE = co[1];
+            D.cN = co[2];
+        }
+    }
+    float* cinv;
+    TRY(dalloc(c, &cinv, (size_t)d->ncoarse * d->ncoarse));
+    HIPC(hipMemcpy(cinv, d->cinv, (size_t)d->ncoarse * d->ncoarse * sizeof(float),
+                   hipMemcpyHostToDevice));
+    h.cinv = cinv;
+    if (!c->mg_dev) TRY(dalloc(c, &c->mg_dev, 1));
+    HIPC(hipMemcpy(c->mg_dev, &h, sizeof(MGDev), hipMemcpyHostToDevice));
+    c->mg_ready = 1;
     return 0;
 }
 
@@ -637,7 +706,7 @@ int cfd_step(cfd_ctx* c, const cfd_step_params* prm, int32_t count) {
         // batch is issued with the parity it will have if all steps run.
         const int pk = par ^ (k & 1);
         TRY(launch_bc(c, true));
-        hipLaunchKernelGGL(k_dt_gamma, dim3(std::min(4 * c->num_cu, 1024)), dim3(256), 0,
+        hipLaunchKernelGGL(k_dt_gamma, dim3(c->red_blocks), dim3(256), 0,
                            c->stream, c->u, c->v, c->geo, sp, c->st, c->acc, c->ticket);
         HIPC(hipGetLastError());
         PredArgs pa = pred_args(c, prm, pk);

@@ -95,6 +95,27 @@ struct RecordArgs {
     long long max_steps;
 };
 
+#define MG_MAX_LEVELS 16
+// One multigrid level: cell arrays of pitch P with a zero ghost ring.
+struct MGLevelDev {
+    int nx, ny, P;
+    int grid;                // 1: spread over all blocks; 0: block 0 alone
+    const float* diag;       // level >= 1: Galerkin diagonal (0 = no fluid); level 0: unused
+    const float* cE;         // coupling magnitude to the east neighbour (periodic wrap at nx)
+    const float* cN;         // coupling magnitude to the north neighbour
+    float* b;
+    float* x;
+    float* t;
+};
+
+struct MGDev {
+    int nlev;                // levels 0..nlev-1; the last one is solved with cinv
+    float omega, wc;
+    int ncoarse;             // cells of the coarsest level (nx * ny)
+    const float* cinv;       // dense inverse over the coarsest cells (row-major i*ny+j)
+    MGLevelDev lev[MG_MAX_LEVELS];
+};
+
 // Pressure-projection arguments.  Cell arrays (x, b, r, q, d0, d1, p, code,
 // comp) use pitch g.pc and are zero outside the fluid interior.
 struct ProjArgs {
@@ -119,8 +140,9 @@ struct ProjArgs {
     float* q32;
     float* d32_0;
     float* d32_1;
-    double* red;
-    unsigned* bar;
+    double* rold;            // flexible CG: previous residual
+    const MGDev* mg;         // multigrid hierarchy (device memory) for PROJ_MG
+    double* sync;            // grid-sync lines: gridDim.x + 2 lines of CFD_SLOT doubles
     DevState* st;
     double dt_override;      // > 0: use this dt (initial projection with dt = 1)
     const double* rhs_in;    // standalone solve: right-hand side (cell layout) instead of div(F,G)/dt
@@ -128,11 +150,11 @@ struct ProjArgs {
     double rtol;
     int maxiter;
     int zero_p_after;        // initial projection: p := 0 afterwards
-    int method;              // PROJ_PCG64, PROJ_MIXED
+    int method;              // PROJ_PCG64, PROJ_MG
     double reliable_delta;   // mixed: true-residual replacement when ||r|| drops by this factor
 };
 
-enum { PROJ_PCG64 = 0, PROJ_MIXED = 1 };
+enum { PROJ_PCG64 = 0, PROJ_MG = 1 };
 
 // kernels (stencil.hip)
 __global__ void k_momentum_tendencies(const double* u, const double* v, const uint8_t* fl, Geo g,

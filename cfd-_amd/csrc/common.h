@@ -71,50 +71,82 @@ __device__ __forceinline__ void block_max(double (&val)[NV], double* lds) {
 }
 
 // ---------------------------------------------------------------------------
-// Grid-wide barrier for persistent (cooperatively launched) kernels.
-// `counter` is zeroed by a separate kernel before the launch; barrier number k
-// (1, 2, ...) completes when the counter reaches k * gridDim.x.  Release and
-// acquire are at agent scope so stores made before the barrier by any block
-// (any XCD) are visible to loads made after it.
+// Grid-wide synchronisation for persistent (cooperatively launched) kernels.
+//
+// No atomic read-modify-write: every block publishes its partial values and an
+// epoch tag on its own 128-byte line (`slots`, 16 doubles per block; tag in the
+// last word), block 0 polls all tags, adds the partials in block order with a
+// fixed tree, and publishes the totals and the tag on a release line (`rel`,
+// two lines used alternately).  Every block then reads the same totals, so the
+// reduction is deterministic and identical in all blocks.  Tags are
+// release-stored and acquire-loaded at agent scope, which makes every store a
+// block made before arriving visible to every block after the release (across
+// XCDs).  The tags are zeroed by a small kernel before each launch.
+//
+// A block writes its slot for epoch e+1 only after it has seen the release of
+// epoch e, which block 0 publishes only after reading every slot for epoch e.
+// The release line of epoch e is reused at epoch e+2, which block 0 can only
+// reach after every block has arrived at e+1, i.e. has finished reading e.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void grid_barrier(unsigned* counter, unsigned& epoch) {
-    __syncthreads();
+#define CFD_SLOT 16   // doubles per block slot (128 bytes); values in [0, 15), tag in [15]
+#define CFD_MAXV 14
+
+struct GridSync {
+    double* slots;    // gridDim.x * CFD_SLOT doubles
+    double* rel;      // 2 * CFD_SLOT doubles
+};
+
+__device__ __forceinline__ unsigned* slot_tag(double* line) {
+    return reinterpret_cast<unsigned*>(line + CFD_SLOT - 1);
+}
+
+template <int NV, bool IS_MAX>
+__device__ __forceinline__ void grid_reduce(double (&val)[NV], const GridSync& gs, double* lds,
+                                            unsigned& epoch) {
+    static_assert(NV >= 1 && NV <= CFD_MAXV, "1..14 reduction values");
+    if (IS_MAX) block_max<NV>(val, lds);
+    else block_sum<NV>(val, lds);
     epoch += 1;
+    const unsigned e = epoch;
     if (threadIdx.x == 0) {
-        const unsigned target = epoch * gridDim.x;
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target)
+        double* my = gs.slots + (long)blockIdx.x * CFD_SLOT;
+#pragma unroll
+        for (int q = 0; q < NV; ++q) my[q] = val[q];
+        __hip_atomic_store(slot_tag(my), e, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    double* rel = gs.rel + (e & 1) * CFD_SLOT;
+    if (blockIdx.x == 0) {
+        double acc[NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+        for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+            double* line = gs.slots + (long)b * CFD_SLOT;
+            // poll without acquire (an acquire load invalidates the caches on
+            // every try); one acquire fence once the tag has been seen
+            while (__hip_atomic_load(slot_tag(line), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != e) {
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                const double x = __hip_atomic_load(&line[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                acc[q] = IS_MAX ? nanmax(acc[q], x) : acc[q] + x;
+            }
+        }
+        if (IS_MAX) block_max<NV>(acc, lds);
+        else block_sum<NV>(acc, lds);
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) rel[q] = acc[q];
+            __hip_atomic_store(slot_tag(rel), e, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    if (threadIdx.x == 0) {
+        while (__hip_atomic_load(slot_tag(rel), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != e)
             __builtin_amdgcn_s_sleep(1);
-    }
-    __syncthreads();
-}
-
-// Grid-wide deterministic sum of NV partials: each block stores its partials
-// in red[NV*block + q], then after the barrier every block adds all partials
-// in block order with one wave, so all blocks obtain bit-identical totals.
-// `red` holds two slots of NV*gridDim.x doubles used by barrier parity: a block
-// writes slot (e&1) for barrier e only after passing barrier e-1, by which time
-// every block has finished reading that slot's previous use (barrier e-2).
-template <int NV>
-__device__ __forceinline__ void grid_sum(double (&val)[NV], double* red, double* lds,
-                                         unsigned* counter, unsigned& epoch) {
-    block_sum<NV>(val, lds);
-    red += ((epoch + 1) & 1) * (long)NV * gridDim.x;
-    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
-        for (int q = 0; q < NV; ++q) red[NV * blockIdx.x + q] = val[q];
-    }
-    grid_barrier(counter, epoch);
-    if (threadIdx.x < CFD_WAVE) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) {
-            double s = 0.0;
-            for (int b = threadIdx.x; b < (int)gridDim.x; b += CFD_WAVE)
-                s += __hip_atomic_load(&red[NV * b + q], __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-            s = wave_sum(s);
-            if (threadIdx.x == 0) lds[q] = s;
-        }
+        for (int q = 0; q < NV; ++q)
+            lds[q] = __hip_atomic_load(&rel[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
 #pragma unroll
@@ -122,29 +154,19 @@ __device__ __forceinline__ void grid_sum(double (&val)[NV], double* red, double*
     __syncthreads();
 }
 
+__device__ __forceinline__ void grid_sync(const GridSync& gs, double* lds, unsigned& epoch) {
+    double none[1] = {0.0};
+    grid_reduce<1, false>(none, gs, lds, epoch);
+}
+
 template <int NV>
-__device__ __forceinline__ void grid_max(double (&val)[NV], double* red, double* lds,
-                                         unsigned* counter, unsigned& epoch) {
-    block_max<NV>(val, lds);
-    red += ((epoch + 1) & 1) * (long)NV * gridDim.x;
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) red[NV * blockIdx.x + q] = val[q];
-    }
-    grid_barrier(counter, epoch);
-    if (threadIdx.x < CFD_WAVE) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) {
-            double s = 0.0;
-            for (int b = threadIdx.x; b < (int)gridDim.x; b += CFD_WAVE)
-                s = nanmax(s, __hip_atomic_load(&red[NV * b + q], __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT));
-            s = wave_max(s);
-            if (threadIdx.x == 0) lds[q] = s;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NV; ++q) val[q] = lds[q];
-    __syncthreads();
+__device__ __forceinline__ void grid_sum(double (&val)[NV], const GridSync& gs, double* lds,
+                                         unsigned& epoch) {
+    grid_reduce<NV, false>(val, gs, lds, epoch);
+}
+
+template <int NV>
+__device__ __forceinline__ void grid_max(double (&val)[NV], const GridSync& gs, double* lds,
+                                         unsigned& epoch) {
+    grid_reduce<NV, true>(val, gs, lds, epoch);
 }

@@ -45,16 +45,184 @@ __device__ __forceinline__ double sel_comp(const double* m, int c) {
     return v;
 }
 
+
+// ---------------------------------------------------------------------------
+// FP32 aggregation multigrid V-cycle (preconditioner of the FP64 flexible CG).
+// Level 0 is the fine grid (matrix-free operator of the projection); level
+// l >= 1 holds the Galerkin operator 0.5 P^T A P of 2x2-cell aggregation as a
+// 5-point stencil (diag, east and north couplings); the coarsest level is
+// solved with a dense inverse.  Smoother: damped Jacobi, two sweeps before
+// and after the coarse correction (the first pre-sweep from x = 0 is local).
+// Levels with many cells are spread over all blocks with grid syncs between
+// sweeps; small levels run in block 0 alone with work-group barriers.
+// ---------------------------------------------------------------------------
+// Cells of a level handled by this block: a contiguous range of the
+// row-major cell index (balanced over all blocks for grid levels, everything
+// in block 0 otherwise); threads stride over the range.
+struct MGPart {
+    int k0, k1;
+    bool active;
+};
+
+__device__ __forceinline__ MGPart mg_part(const MGLevelDev& L) {
+    MGPart p;
+    const long total = (long)L.nx * L.ny;
+    if (L.grid) {
+        p.k0 = (int)((total * blockIdx.x) / gridDim.x);
+        p.k1 = (int)((total * (blockIdx.x + 1)) / gridDim.x);
+        p.active = true;
+    } else {
+        p.k0 = 0;
+        p.k1 = (int)total;
+        p.active = blockIdx.x == 0;
+    }
+    return p;
+}
+
+#define MG_FOR(pt, L)                                                       \
+    for (int k_ = (pt).k0 + (int)threadIdx.x; k_ < (pt).k1; k_ += blockDim.x) \
+        for (int i = k_ / (L).ny + 1, j = k_ - (i - 1) * (L).ny + 1, once_ = 1; once_; once_ = 0)
+
+struct MGCtx {
+    const MGDev* mg;
+    const uint8_t* code;     // fine-level codes (pitch of level 0)
+    const float* diag0;      // fine diagonal table in LDS (64)
+    int periodic;
+    float ax, ay;
+};
+
+// diagonal of level l at cell c (0: not a fluid cell)
+__device__ __forceinline__ float mg_diag(const MGCtx& m, int l, const MGLevelDev& L, long c) {
+    return l == 0 ? m.diag0[m.code[c] & 63] : L.diag[c];
+}
+
+// (A v)(i, j) on level l; `val(i, j)` gives v at a cell (0 outside the fluid)
+template <typename F>
+__device__ __forceinline__ float mg_apply(const MGCtx& m, int l, const MGLevelDev& L, int i, int j,
+                                          float d, F val) {
+    const int iE = (m.periodic && i == L.nx) ? 1 : i + 1;
+    const int iW = (m.periodic && i == 1) ? L.nx : i - 1;
+    if (l == 0)
+        return d * val(i, j) - m.ax * (val(iE, j) + val(iW, j)) - m.ay * (val(i, j + 1) + val(i, j - 1));
+    const int P = L.P;
+    return d * val(i, j) - L.cE[ix(i, j, P)] * val(iE, j) - L.cE[ix(iW, j, P)] * val(iW, j) -
+           L.cN[ix(i, j, P)] * val(i, j + 1) - L.cN[ix(i, j - 1, P)] * val(i, j - 1);
+}
+
+__device__ __forceinline__ void mg_sync(const MGLevelDev& L, const GridSync& gs, double* lds,
+                                        unsigned& epoch) {
+    if (L.grid) grid_sync(gs, lds, epoch);
+    else if (blockIdx.x == 0) __syncthreads();
+}
+
+// z = M r with the fine residual already in lev[0].b and lev[0].x = w b / d.
+// Result in lev[0].t.
+__device__ void mg_vcycle(const MGCtx& m, const GridSync& gs, double* lds, unsigned& epoch) {
+    const MGDev& mg = *m.mg;
+    const int NL = mg.nlev;
+    const float w = mg.omega;
+    const int tid = threadIdx.x, BS = blockDim.x;
+    for (int l = 0; l < NL - 1; ++l) {
+        const MGLevelDev& L = mg.lev[l];
+        const MGPart pt = mg_part(L);
+        mg_sync(L, gs, lds, epoch);
+        if (pt.active) {   // second pre-smoothing sweep: t = x + w (b - A x) / d
+            MG_FOR(pt, L) {
+                    const long c = ix(i, j, L.P);
+                    const float d = mg_diag(m, l, L, c);
+                    if (d == 0.0f) continue;
+                    auto xv = [&](int a, int b) -> float { return L.x[ix(a, b, L.P)]; };
+                    const float Ax = mg_apply(m, l, L, i, j, d, xv);
+                    L.t[c] = L.x[c] + w * (L.b[c] - Ax) / d;
+                }
+        }
+        mg_sync(L, gs, lds, epoch);
+        const MGLevelDev& C = mg.lev[l + 1];
+        const MGPart pc = mg_part(C);
+        if (pc.active) {   // restriction of the residual (sum over the 2x2 children)
+            MG_FOR(pc, C) {
+                    const int ci = i, cj = j;
+                    const long cc = ix(ci, cj, C.P);
+                    float bc = 0.0f;
+                    for (int di = 0; di < 2; ++di)
+                        for (int dj = 0; dj < 2; ++dj) {
+                            const int i = 2 * ci - 1 + di, j = 2 * cj - 1 + dj;
+                            if (i > L.nx || j > L.ny) continue;
+                            const long c = ix(i, j, L.P);
+                            const float d = mg_diag(m, l, L, c);
+                            if (d == 0.0f) continue;
+                            auto tv = [&](int a, int b) -> float { return L.t[ix(a, b, L.P)]; };
+                            bc += L.b[c] - mg_apply(m, l, L, i, j, d, tv);
+                        }
+                    C.b[cc] = bc;
+                    if (l + 1 < NL - 1) {
+                        const float dc = C.diag[cc];
+                        C.x[cc] = dc != 0.0f ? w * bc / dc : 0.0f;
+                    }
+                }
+        }
+    }
+    // coarsest level: dense inverse (block 0)
+    {
+        const MGLevelDev& C = mg.lev[NL - 1];
+        mg_sync(C, gs, lds, epoch);
+        if (blockIdx.x == 0) {
+            const int n = mg.ncoarse;
+            for (int k = tid; k < n; k += BS) {
+                float acc = 0.0f;
+                for (int q = 0; q < n; ++q) {
+                    const int qi = q / C.ny + 1, qj = q % C.ny + 1;
+                    acc += mg.cinv[(long)k * n + q] * C.b[ix(qi, qj, C.P)];
+                }
+                C.t[ix(k / C.ny + 1, k % C.ny + 1, C.P)] = acc;
+            }
+        }
+    }
+    for (int l = NL - 2; l >= 0; --l) {
+        const MGLevelDev& L = mg.lev[l];
+        const MGLevelDev& C = mg.lev[l + 1];
+        const MGPart pt = mg_part(L);
+        mg_sync(L, gs, lds, epoch);
+        // coarse correction t + wc e_parent, then the first post-smoothing sweep into x
+        auto corr = [&](int a, int b) -> float {
+            const long c = ix(a, b, L.P);
+            if (mg_diag(m, l, L, c) == 0.0f) return 0.0f;
+            return L.t[c] + mg.wc * C.t[ix((a + 1) >> 1, (b + 1) >> 1, C.P)];
+        };
+        if (pt.active) {
+            MG_FOR(pt, L) {
+                    const long c = ix(i, j, L.P);
+                    const float d = mg_diag(m, l, L, c);
+                    if (d == 0.0f) continue;
+                    const float y = corr(i, j);
+                    L.x[c] = y + w * (L.b[c] - mg_apply(m, l, L, i, j, d, corr)) / d;
+                }
+        }
+        mg_sync(L, gs, lds, epoch);
+        if (pt.active) {
+            MG_FOR(pt, L) {
+                    const long c = ix(i, j, L.P);
+                    const float d = mg_diag(m, l, L, c);
+                    if (d == 0.0f) continue;
+                    auto xv = [&](int a, int b) -> float { return L.x[ix(a, b, L.P)]; };
+                    L.t[c] = L.x[c] + w * (L.b[c] - mg_apply(m, l, L, i, j, d, xv)) / d;
+                }
+        }
+    }
+}
+
 }  // namespace
 
 __global__ void __launch_bounds__(PROJ_BS) k_project(ProjArgs a, Geo g) {
     __shared__ double lds[CFD_MAX_COMP * (PROJ_BS / 64) + 16];
     __shared__ double s_diag[64], s_inv[64];
+    __shared__ float s_diagf[64];
     DevState* st = a.st;
     if (st->done) return;   // uniform over the grid: set by an earlier launch
     if (threadIdx.x < 64) {
         s_diag[threadIdx.x] = a.lut[threadIdx.x];
         s_inv[threadIdx.x] = a.lut[64 + threadIdx.x];
+        s_diagf[threadIdx.x] = (float)a.lut[threadIdx.x];
     }
     __syncthreads();
 
@@ -66,7 +234,7 @@ __global__ void __launch_bounds__(PROJ_BS) k_project(ProjArgs a, Geo g) {
     const Rows rows = block_rows(nx);
     const int tid = threadIdx.x, BS = blockDim.x;
     const bool lead = blockIdx.x == 0 && tid == 0;
-    unsigned* bar = a.bar;
+    const GridSync gs{a.sync, a.sync + (long)gridDim.x * CFD_SLOT};
 
 #define FOR_CELLS                                            \
     for (int i = rows.i0; i < rows.i1; ++i)                  \
@@ -94,7 +262,7 @@ __global__ void __launch_bounds__(PROJ_BS) k_project(ProjArgs a, Geo g) {
     }
     double mean[CFD_MAX_COMP];
     if (a.ncomp > 0) {
-        grid_sum<CFD_MAX_COMP>(cs, a.red, lds, bar, epoch);
+        grid_sum<CFD_MAX_COMP>(cs, gs, lds, epoch);
 #pragma unroll
         for (int k = 0; k < CFD_MAX_COMP; ++k)
             mean[k] = k < a.ncomp ? cs[k] / a.comp_count[k] : 0.0;
@@ -123,7 +291,7 @@ __global__ void __launch_bounds__(PROJ_BS) k_project(ProjArgs a, Geo g) {
         s3[1] += rv * rv;
         s3[2] += rv * z;
     }
-    grid_sum<3>(s3, a.red, lds, bar, epoch);
+    grid_sum<3>(s3, gs, lds, epoch);
     const double bnorm = sqrt(s3[0]);
     const double atol = a.rtol * bnorm;
     double rr = s3[1], rz = s3[2];
@@ -136,7 +304,7 @@ __global__ void __launch_bounds__(PROJ_BS) k_project(ProjArgs a, Geo g) {
             if (a.code[c] & 63) a.x[c] = 0.0;
         }
         rr = 0.0;
-    } else {
+    } else if (a.method == PROJ_PCG64) {
         bool conv = false;
         double rz_prev = 1.0;
         for (int k = 0; k < a.maxiter; ++k) {
@@ -170,7 +338,7 @@ __global__ void __launch_bounds__(PROJ_BS) k_project(ProjArgs a, Geo g) {
                 a.q[c] = q;
                 pq[0] += dc * q;
             }
-            grid_sum<1>(pq, a.red, lds, bar, epoch);
+            grid_sum<1>(pq, gs, lds, epoch);
             const double alpha = rz / pq[0];
             double s2[2] = {0.0, 0.0};
             FOR_CELLS {
@@ -185,7 +353,7 @@ __global__ void __launch_bounds__(PROJ_BS) k_project(ProjArgs a, Geo g) {
                 s2[0] += rv * rv;
                 s2[1] += rv * z;
             }
-            grid_sum<2>(s2, a.red, lds, bar, epoch);
+            grid_sum<2>(s2, gs, lds, epoch);
             rz_prev = rz;
             rr = s2[0];
             rz = s2[1];
@@ -193,11 +361,103 @@ __global__ void __launch_bounds__(PROJ_BS) k_project(ProjArgs a, Geo g) {
         }
         if (!conv && iters >= a.maxiter) info = a.maxiter;
         if (!conv && iters < a.maxiter) info = -1;   // breakdown (non-finite residual)
+    } else {
+        // flexible CG (Polak-Ribiere beta) with FP64 vectors and dot products,
+        // preconditioned by the FP32 multigrid V-cycle; the loop stops on the
+        // recursive residual, then the true residual b - A x decides (a restart
+        // with the true residual if it is not yet below the tolerance)
+        const MGCtx m{a.mg, a.code, s_diagf, periodic, (float)ax, (float)ay};
+        const MGLevelDev& L0 = a.mg->lev[0];
+        const float w = a.mg->omega;
+        double rho_old = 1.0;
+        bool conv = false;
+        int k = 0, kstart = 0, reps = 0;
+        for (int rep = 0; rep < 3; ++rep) {
+            for (; k < a.maxiter; ++k) {
+                if (sqrt(rr) < atol || !(rr == rr)) break;
+                FOR_CELLS {
+                    const long c = ix(i, j, pc);
+                    const int li = a.code[c] & 63;
+                    if (!li) continue;
+                    const float rf = (float)a.r[c];
+                    L0.b[c] = rf;
+                    L0.x[c] = w * rf / s_diagf[li];
+                }
+                mg_vcycle(m, gs, lds, epoch);
+                const bool first = k == kstart;
+                double s2[2] = {0.0, 0.0};
+                FOR_CELLS {
+                    const long c = ix(i, j, pc);
+                    if (!(a.code[c] & 63)) continue;
+                    const double z = (double)L0.t[c];
+                    s2[0] += a.r[c] * z;
+                    if (!first) s2[1] += a.rold[c] * z;
+                }
+                grid_sum<2>(s2, gs, lds, epoch);
+                const double beta = first ? 0.0 : (s2[0] - s2[1]) / rho_old;
+                double* dnew = (k & 1) ? a.d1 : a.d0;
+                const double* dold = (k & 1) ? a.d0 : a.d1;
+                double pq[1] = {0.0};
+                FOR_CELLS {
+                    const long c = ix(i, j, pc);
+                    const int li = a.code[c] & 63;
+                    if (!li) continue;
+                    const int iE = (periodic && i == nx) ? 1 : i + 1;
+                    const int iW = (periodic && i == 1) ? nx : i - 1;
+                    auto dirv = [&](long cc) -> double {
+                        const double z = (double)L0.t[cc];
+                        return first ? z : z + beta * dold[cc];
+                    };
+                    const double dc = dirv(c);
+                    const double q = s_diag[li] * dc - ax * (dirv(ix(iE, j, pc)) + dirv(ix(iW, j, pc))) -
+                                     ay * (dirv(ix(i, j + 1, pc)) + dirv(ix(i, j - 1, pc)));
+                    dnew[c] = dc;
+                    a.q[c] = q;
+                    pq[0] += dc * q;
+                }
+                grid_sum<1>(pq, gs, lds, epoch);
+                const double alpha = s2[0] / pq[0];
+                double r2[1] = {0.0};
+                FOR_CELLS {
+                    const long c = ix(i, j, pc);
+                    if (!(a.code[c] & 63)) continue;
+                    a.x[c] = a.x[c] + alpha * dnew[c];
+                    const double rv = a.r[c];
+                    a.rold[c] = rv;
+                    const double rn = rv - alpha * a.q[c];
+                    a.r[c] = rn;
+                    r2[0] += rn * rn;
+                }
+                grid_sum<1>(r2, gs, lds, epoch);
+                rr = r2[0];
+                rho_old = s2[0];
+            }
+            // true residual of the FP64 solution
+            grid_sync(gs, lds, epoch);
+            double t2[1] = {0.0};
+            FOR_CELLS {
+                const long c = ix(i, j, pc);
+                const int li = a.code[c] & 63;
+                if (!li) continue;
+                const double rt = a.b[c] - apply_A(a.x, i, j, pc, nx, periodic, s_diag[li], ax, ay);
+                a.r[c] = rt;
+                t2[0] += rt * rt;
+            }
+            grid_sum<1>(t2, gs, lds, epoch);
+            rr = t2[0];
+            reps += 1;
+            conv = sqrt(rr) < atol;
+            if (conv || k >= a.maxiter || !(rr == rr)) break;
+            kstart = k;
+        }
+        iters = k;
+        info = conv ? 0 : (k >= a.maxiter ? a.maxiter : -1);
+        if (lead) st->cg_outer = reps;
     }
     if (lead) {
         st->cg_iters = iters;
         st->cg_info = info;
-        st->cg_outer = 0;
+        if (a.method == PROJ_PCG64) st->cg_outer = 0;
         st->cg_bnorm = bnorm;
         st->cg_relres = bnorm > 0.0 ? sqrt(rr) / bnorm : 0.0;
     }
@@ -213,7 +473,7 @@ __global__ void __launch_bounds__(PROJ_BS) k_project(ProjArgs a, Geo g) {
 #pragma unroll
             for (int k = 0; k < CFD_MAX_COMP; ++k) cs[k] += (cm == k + 1) ? xv : 0.0;
         }
-        grid_sum<CFD_MAX_COMP>(cs, a.red, lds, bar, epoch);
+        grid_sum<CFD_MAX_COMP>(cs, gs, lds, epoch);
 #pragma unroll
         for (int k = 0; k < CFD_MAX_COMP; ++k)
             mean[k] = k < a.ncomp ? cs[k] / a.comp_count[k] : 0.0;
@@ -240,7 +500,7 @@ __global__ void __launch_bounds__(PROJ_BS) k_project(ProjArgs a, Geo g) {
         for (int j = tid; j <= ny + 1; j += BS)
             a.p[ix(nx + 1, j, pc)] = (periodic && j >= 1 && j <= ny) ? pval(1, j) : 0.0;
     if (!a.correct) return;
-    grid_barrier(bar, epoch);
+    grid_sync(gs, lds, epoch);
 
     // ---- velocity correction u = F - dt grad p and the residual monitor ----
     const double* p = a.p;
@@ -295,7 +555,7 @@ __global__ void __launch_bounds__(PROJ_BS) k_project(ProjArgs a, Geo g) {
             }
         }
     }
-    grid_max<1>(res, a.red, lds, bar, epoch);
+    grid_max<1>(res, gs, lds, epoch);
     if (lead && a.dt_override <= 0.0) st->residual = res[0];
     if (periodic && blockIdx.x == gridDim.x - 1 && tid < 2) {
         const int j = tid == 0 ? 0 : ny + 1;

@@ -340,17 +340,14 @@ __global__ void k_dt_gamma(const double* __restrict__ u, const double* __restric
     const int nx = g.nx, ny = g.ny;
     double m[2] = {0.0, 0.0};
     if (!st->done) {
-        const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-        const int nth = gridDim.x * blockDim.x;
-        const long nu_ = (long)(nx + 1) * (ny + 2);
-        for (long k = tid; k < nu_; k += nth) {
-            const int i = (int)(k / (ny + 2)), j = (int)(k - (long)i * (ny + 2));
-            m[0] = nanmax(m[0], fabs(u[ix(i, j, g.pu)]));
-        }
-        const long nv_ = (long)(nx + 2) * (ny + 1);
-        for (long k = tid; k < nv_; k += nth) {
-            const int i = (int)(k / (ny + 1)), j = (int)(k - (long)i * (ny + 1));
-            m[1] = nanmax(m[1], fabs(v[ix(i, j, g.pv)]));
+        // row band of this block over rows 0..nx+1; threads along j
+        const int r0 = (int)(((long)(nx + 2) * blockIdx.x) / gridDim.x);
+        const int r1 = (int)(((long)(nx + 2) * (blockIdx.x + 1)) / gridDim.x);
+        for (int i = r0; i < r1; ++i) {
+            for (int j = threadIdx.x; j <= ny + 1; j += blockDim.x) {
+                if (i <= nx) m[0] = nanmax(m[0], fabs(u[ix(i, j, g.pu)]));
+                if (j <= ny) m[1] = nanmax(m[1], fabs(v[ix(i, j, g.pv)]));
+            }
         }
     }
     block_max<2>(m, lds);
@@ -661,9 +658,10 @@ __global__ void k_record(RecordArgs a, Geo g) {
     if (active) {
         const int tid = blockIdx.x * blockDim.x + threadIdx.x;
         const int nth = gridDim.x * blockDim.x;
-        const long ncell = (long)nx * ny;
-        for (long k = tid; k < ncell; k += nth) {
-            const int i = (int)(k / ny) + 1, j = (int)(k - (long)(i - 1) * ny) + 1;
+        const int r0 = 1 + (int)(((long)nx * blockIdx.x) / gridDim.x);
+        const int r1 = 1 + (int)(((long)nx * (blockIdx.x + 1)) / gridDim.x);
+        for (int i = r0; i < r1; ++i)
+            for (int j = 1 + threadIdx.x; j <= ny; j += blockDim.x) {
             if (!(a.fl[ix(i, j, pc)] & 1)) continue;
             const double div = (a.u[ix(i, j, g.pu)] - a.u[ix(i - 1, j, g.pu)]) / g.dx +
                                (a.v[ix(i, j, g.pv)] - a.v[ix(i, j - 1, g.pv)]) / g.dy;

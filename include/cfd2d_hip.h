@@ -44,8 +44,10 @@ enum { CFD_FIELD_U = 0, CFD_FIELD_V = 1, CFD_FIELD_P = 2, CFD_FIELD_F = 3, CFD_F
        CFD_FIELD_HU = 5, CFD_FIELD_HV = 6, CFD_FIELD_HU_OLD = 7, CFD_FIELD_HV_OLD = 8,
        CFD_FIELD_X = 9 };
 /* pressure solvers: Jacobi-PCG in FP64 (the reference's "cg", pressure.py:171),
-   and Jacobi-PCG with FP32 vectors, FP64 dots and FP64 true-residual replacement */
-enum { CFD_SOLVER_PCG64 = 0, CFD_SOLVER_MIXED = 1 };
+   and flexible CG with FP64 vectors and dot products preconditioned by an FP32
+   aggregation-multigrid V-cycle (set up with cfd_set_multigrid), the solve
+   ending on the FP64 true residual */
+enum { CFD_SOLVER_PCG64 = 0, CFD_SOLVER_MGCG = 1 };
 
 typedef struct cfd_ctx cfd_ctx;
 
@@ -150,6 +152,28 @@ int cfd_upload_scalar(cfd_ctx* ctx, int32_t id, const double* host);   /* + fill
 int cfd_download_scalar(cfd_ctx* ctx, int32_t id, double* host);
 int cfd_set_state(cfd_ctx* ctx, const cfd_state* st);
 int cfd_get_state(cfd_ctx* ctx, cfd_state* st);
+
+/* Multigrid hierarchy for CFD_SOLVER_MGCG (no reference counterpart; the
+   reference's solver choice is pressure.py:136-139).  Level 0 is the context's
+   grid (only nx, ny given); level l >= 1 holds the 5-point Galerkin operator
+   of 2x2 aggregation as nx x ny host arrays (diagonal, coupling magnitudes to
+   the east and north neighbours, 0 where no fluid).  The last level is solved
+   with the dense inverse cinv (ncoarse x ncoarse over its cells, row-major). */
+typedef struct cfd_mg_level_desc {
+    int32_t nx, ny;
+    const float* diag;
+    const float* ce;
+    const float* cn;
+} cfd_mg_level_desc;
+typedef struct cfd_mg_desc {
+    int32_t nlev;                /* 2..16 */
+    float omega;                 /* damped-Jacobi weight */
+    float wc;                    /* coarse-correction weight */
+    int32_t ncoarse;             /* nx*ny of the last level, <= 4096 */
+    const float* cinv;
+    cfd_mg_level_desc lev[16];
+} cfd_mg_desc;
+int cfd_set_multigrid(cfd_ctx* ctx, const cfd_mg_desc* desc);
 
 /* Simulation._apply_bc (solver.py:190-193): DomainBoundary.apply + obstacle_bc. */
 int cfd_apply_bc(cfd_ctx* ctx);

@@ -162,3 +162,70 @@ def test_instability_raises(gpu):
     with pytest.raises(FloatingPointError):
         for _ in range(200):
             sim.step()
+
+
+# ---------------------------------------------------------------------------
+# cg_mixed: FP64 flexible CG with the FP32 multigrid preconditioner
+# ---------------------------------------------------------------------------
+def test_mixed_solver_cavity_matches_fp64_oracle(gpu):
+    """Velocity within 1e-5 of the FP64 oracle (LU) after 60 steps with the
+    pressure solved to a true relative residual of 1e-6 on every step."""
+    from cfd2d_amd import cases
+    from oracle import cfd_oracle as O
+
+    n = 64
+    sim = cases.lid_driven_cavity(re=1000, n=n, pressure_solver="cg_mixed",
+                                  pressure_tol=1e-6).sim
+    ref = O.cavity(re=1000, n=n, pressure_solver="direct")
+    for _ in range(60):
+        sim.step()
+        ref.step()
+    assert max(sim.cg_relres) <= 1e-6
+    assert max(sim.cg_iterations) < 30
+    assert _cmp(sim.u, ref.u) < 1e-5
+    assert _cmp(sim.v, ref.v) < 1e-5
+
+
+def test_mixed_solver_channel_obstacles(gpu):
+    from cfd2d_amd import cases
+
+    g = np.load(GOLDEN / "channel64x16.npz")
+    obstacles = [{"typ": "kreis", "x": 0.7, "y": 0.5, "r": 0.12},
+                 {"typ": "kreis", "x": 1.5, "y": 0.3, "r": 0.08},
+                 {"typ": "kreis", "x": 2.1, "y": 0.72, "r": 0.1},
+                 {"typ": "kreis", "x": 2.8, "y": 0.45, "r": 0.06},
+                 {"typ": "kreis", "x": 3.3, "y": 0.2, "r": 0.07}]
+    case = cases.custom_obstacles(obstacles, lx=4.0, ly=1.0, nx=64, ny=16, re=200, L_ref=1.0,
+                                  inflow_profile="parabolic", pressure_solver="cg_mixed",
+                                  pressure_tol=1e-9)
+    cases.add_scalar(case, "c", 1e-3, inflow_band=(0.25, 0.75), limiter="superbee")
+    sim = case.sim
+    for _ in range(30):
+        sim.step()
+    assert max(sim.cg_relres) <= 1e-9
+    assert _cmp(sim.u, g["u_30"]) < 1e-7
+    assert _cmp(sim.scalars["c"].phi, g["c_30"]) < 1e-7
+
+
+def test_mixed_pressure_solve_equals_dct(gpu):
+    """Standalone PressureSolver.solve (cg_mixed) against the exact DCT-II
+    solution of the Neumann problem."""
+    import scipy.fft
+
+    from cfd2d_amd import cases
+
+    n = 128
+    sim = cases.lid_driven_cavity(re=100, n=n, pressure_solver="cg_mixed",
+                                  pressure_tol=1e-10).sim
+    rng = np.random.default_rng(5)
+    rhs = rng.standard_normal((n, n))
+    p = sim.pressure.solve(rhs)
+    h = 1.0 / n
+    lam = (2 - 2 * np.cos(np.pi * np.arange(n) / n)) / h ** 2
+    L = lam[:, None] + lam[None, :]
+    L[0, 0] = 1.0
+    r = scipy.fft.dctn(-(rhs - rhs.mean()), type=2, norm="ortho") / L
+    r[0, 0] = 0.0
+    exact = scipy.fft.idctn(r, type=2, norm="ortho")
+    assert sim.pressure.last_relres <= 1e-10
+    assert np.abs(p - exact).max() < 1e-7 * np.abs(exact).max()
