@@ -171,8 +171,8 @@ def cpu_baseline(budget_s: float = 20.0, max_steps: int = 30) -> dict | None:
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--legs", default="cg,cg_mixed")
     ap.add_argument("--cpu-child", nargs=2, default=None)
