@@ -38,6 +38,7 @@ N = 1024
 RE = 1000.0
 RTOL = 1e-6
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_RATIO_MGCG = 1.10
 WORKLOAD = ("lid-driven cavity Re=1000, 1024x1024 MAC grid, AB2, gamma auto, tau 0.5, "
             "passive van Leer scalar (D=1e-3, 8 Gaussian blobs, seed 0), "
             "pressure solve to relative residual 1e-6")
@@ -122,8 +123,14 @@ def roofline_for(leg: dict) -> dict:
         bytes_per_launch = n_cells * s * (3 + 7 + 10 * n_it)
     t = leg["proj_ms_mean"] * 1e-3
     achieved = bytes_per_launch / t / 1e9 if t > 0 else 0.0
+    # measured HBM traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated against a
+    # 256 MiB copy) was 1.10x the algorithmic bytes for the multigrid launches
+    traffic = round(TRAFFIC_RATIO_MGCG * bytes_per_launch) if leg["solver"] == "cg_mixed" else None
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE of 6-iteration launches "
+                              "(ratio 1.10 to algorithmic), profiles/k_project_traffic_cavity1024.md"
+            if traffic else None,
             "kernel": "k_project", "bytes_per_launch": bytes_per_launch,
             "launch_ms": round(leg["proj_ms_mean"], 4), "cg_iters_per_launch": round(n_it, 2)}
 
@@ -168,6 +175,24 @@ def cpu_baseline(budget_s: float = 20.0, max_steps: int = 30) -> dict | None:
                        f"{r['cg_iters']}")}
 
 
+def copy_calibration(nbytes: int = 256 << 20) -> None:
+    """One device-to-device copy of `nbytes` (rocclr's copy kernel reads and
+    writes exactly that many bytes): the reference dispatch for scaling the
+    HBM byte counters of the projection kernel."""
+    import ctypes
+
+    from cfd2d_amd import _lib
+
+    L = _lib.lib()
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    _lib.check(L.cfd_dev_alloc(nbytes, ctypes.byref(a)), "alloc")
+    _lib.check(L.cfd_dev_alloc(nbytes, ctypes.byref(b)), "alloc")
+    _lib.check(L.cfd_memcpy(b, a, nbytes, 3), "copy")
+    L.cfd_dev_free(a)
+    L.cfd_dev_free(b)
+    print(f"copy calibration: {nbytes} bytes device-to-device", file=sys.stderr, flush=True)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,6 +201,9 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--legs", default="cg,cg_mixed")
     ap.add_argument("--cpu-child", nargs=2, default=None)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 256 MiB device-to-device (a known-size kernel for "
+                         "scaling rocprof FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.cpu_child:
         print(json.dumps(cpu_baseline_child(float(args.cpu_child[0]), int(args.cpu_child[1]))))
@@ -185,6 +213,8 @@ def main() -> None:
               "single-GPU in this build", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        copy_calibration()
     legs = {}
     for solver in args.legs.split(","):
         try:
