@@ -118,6 +118,17 @@ def lib() -> ctypes.CDLL:
         "cfd_obstacle_bc": ([_vp, _vp, _vp, _i32, _i32, _i32, _vp], _i32),
         "cfd_scalar_transport": ([_vp, _vp, _vp, _vp, _i32, _i32, _d, _d, _vp, _d, _i32, _i32]
                                  + [_vp] * 8 + [_i32, _d, _vp, _vp], _i32),
+        "cfd_unet_param_count": ([], _i64),
+        "cfd_unet_create": ([_i32, _i32, P(_vp)], _i32),
+        "cfd_unet_destroy": ([_vp], _i32),
+        "cfd_unet_set_params": ([_vp, _vp], _i32),
+        "cfd_unet_get_params": ([_vp, _vp], _i32),
+        "cfd_unet_get_grad": ([_vp, _vp], _i32),
+        "cfd_unet_zero_grad": ([_vp], _i32),
+        "cfd_unet_forward": ([_vp, _vp, _vp], _i32),
+        "cfd_unet_loss_grad": ([_vp, _vp, _vp, _vp, _d, P(_d)], _i32),
+        "cfd_unet_adam": ([_vp, ctypes.c_float, ctypes.c_float], _i32),
+        "cfd_mfma_layout_check": ([_vp, _vp, _vp], _i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -136,7 +147,11 @@ def exported_symbols() -> list[str]:
                         "cfd_last_elapsed_ms", "cfd_set_profiling", "cfd_projection_times",
                         "cfd_pressure_solve", "cfd_dev_alloc",
                         "cfd_dev_free", "cfd_memcpy", "cfd_momentum_tendencies",
-                        "cfd_obstacle_bc", "cfd_scalar_transport")]
+                        "cfd_obstacle_bc", "cfd_scalar_transport", "cfd_unet_param_count",
+                        "cfd_unet_create", "cfd_unet_destroy", "cfd_unet_set_params",
+                        "cfd_unet_get_params", "cfd_unet_get_grad", "cfd_unet_zero_grad",
+                        "cfd_unet_forward", "cfd_unet_loss_grad", "cfd_unet_adam",
+                        "cfd_mfma_layout_check")]
 
 
 def check(rc: int, what: str = "") -> int:

@@ -14,6 +14,7 @@
 
 #include "stencil.hip"
 #include "pcg.hip"
+#include "unet.hip"
 
 namespace {
 
@@ -936,3 +937,5 @@ int cfd_scalar_transport(const double* phi, const double* u, const double* v,
 }
 
 }  // extern "C"
+
+#include "unet_net.hip"

@@ -1,0 +1,412 @@
+// Convolution building blocks of the learned pressure preconditioner (U-Net)
+// on gfx950: 3x3 "same" convolutions in NHWC with BF16 operands and FP32
+// accumulation on the matrix cores (v_mfma_f32_16x16x16_bf16), their
+// backward passes, pooling/upsampling, the loss and the Adam update.
+//
+// MFMA tiling (one wavefront = one output tile of 16 pixels x 16 channels per
+// accumulator): for v_mfma_f32_16x16x16_bf16 lane l holds
+//   A (16 x 16, M x K): row l%16, k = 4*(l/16) .. +3
+//   B (16 x 16, K x N): col l%16, k = 4*(l/16) .. +3
+//   D (16 x 16, M x N): col l%16, rows 4*(l/16) .. +3
+// (checked on the device by cfd_mfma_layout_check against a host product).
+// Here M = 16 consecutive pixels of one image row, K = 16 input channels of
+// one filter tap, N = 16 output channels.  Weights are stored [tap][Cout][Cin]
+// so the 4 K-values a lane needs are contiguous (one 8-byte load).
+#include "cfd_kernels.h"
+#include <hip/hip_bf16.h>
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short f2bf(float f) {
+    // round to nearest even
+    unsigned int u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)(u >> 16) | ((u & 0xffff) ? 0x40 : 0);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
+
+// ---------------------------------------------------------------------------
+// Layout check: D = A(16x16) * B(16x16) from row-major float inputs.
+// ---------------------------------------------------------------------------
+__global__ void k_mfma_layout_check(const float* A, const float* B, float* D) {
+    const int l = threadIdx.x;   // one wavefront
+    bf16x4 a, b;
+    for (int j = 0; j < 4; ++j) {
+        const int k = 4 * (l / 16) + j;
+        a[j] = (short)f2bf(A[(l % 16) * 16 + k]);
+        b[j] = (short)f2bf(B[k * 16 + (l % 16)]);
+    }
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    for (int j = 0; j < 4; ++j) D[(4 * (l / 16) + j) * 16 + (l % 16)] = acc[j];
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 convolution, MFMA path.  Input = concat(X1 [H][W][C1], X2 [H][W][C2]),
+// C1, C2, Cout multiples of 16 (C2 may be 0).  Wt [9][Cout][C1+C2] bf16.
+// mode bit 0: add bias; bit 1: ReLU; bit 2: accumulate into Y (fp32 out).
+// Output either bf16 (Yb) or fp32 (Yf); with fp32 output, channels >= csplit
+// go to Yf2 (channel stride Cout - csplit), the others to Yf (stride csplit):
+// the gradient of a concatenated input lands in its two parts.  One wave per
+// 16-pixel tile; the block's 4 waves take 4 consecutive tiles.
+// ---------------------------------------------------------------------------
+template <int NCO>   // Cout / 16 accumulators per wave
+__global__ void __launch_bounds__(256) k_conv3x3_mfma(
+    const unsigned short* __restrict__ X1, int C1, const unsigned short* __restrict__ X2, int C2,
+    const unsigned short* __restrict__ Wt, const float* __restrict__ bias, int H, int Wd,
+    unsigned short* __restrict__ Yb, float* __restrict__ Yf, float* __restrict__ Yf2, int csplit,
+    int mode) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int tiles_w = (Wd + 15) / 16;
+    const long tile = (long)blockIdx.x * 4 + wave;
+    if (tile >= (long)tiles_w * H) return;
+    const int h = (int)(tile / tiles_w);
+    const int w0 = (int)(tile - (long)h * tiles_w) * 16;
+    const int Cin = C1 + C2;
+    const int Cout = NCO * 16;
+    const int m = lane % 16, kq = 4 * (lane / 16);
+    f32x4 acc[NCO];
+#pragma unroll
+    for (int n = 0; n < NCO; ++n) acc[n] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < 9; ++t) {
+        const int hh = h + t / 3 - 1;
+        const int ww = w0 + m + t % 3 - 1;
+        const bool in = hh >= 0 && hh < H && ww >= 0 && ww < Wd;
+        const long pix = (long)hh * Wd + ww;
+        for (int k0 = 0; k0 < Cin; k0 += 16) {
+            bf16x4 a = {0, 0, 0, 0};
+            if (in) {
+                const unsigned short* src = k0 < C1 ? X1 + pix * C1 + k0 + kq
+                                                    : X2 + pix * C2 + (k0 - C1) + kq;
+                a = *reinterpret_cast<const bf16x4*>(src);
+            }
+#pragma unroll
+            for (int n = 0; n < NCO; ++n) {
+                const bf16x4 b = *reinterpret_cast<const bf16x4*>(
+                    Wt + ((long)t * Cout + n * 16 + m) * Cin + k0 + kq);
+                acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[n], 0, 0, 0);
+            }
+        }
+    }
+    // D: lane holds pixels w0 + kq + j (j = 0..3) of channel n*16 + m
+#pragma unroll
+    for (int n = 0; n < NCO; ++n) {
+        const int co = n * 16 + m;
+        const float bv = (mode & 1) ? bias[co] : 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int w = w0 + kq + j;
+            if (w >= Wd) continue;
+            float v = acc[n][j] + bv;
+            const long px = (long)h * Wd + w;
+            if (Yf) {
+                float* dst = co < csplit ? Yf + px * csplit + co
+                                         : Yf2 + px * (Cout - csplit) + (co - csplit);
+                if (mode & 4) v += *dst;
+                if (mode & 2) v = fmaxf(v, 0.f);
+                *dst = v;
+            } else {
+                if (mode & 2) v = fmaxf(v, 0.f);
+                Yb[px * Cout + co] = f2bf(v);
+            }
+        }
+    }
+}
+
+// Weight-gradient partials: dW[t][co][ci] += sum_p G[p][co] * X[p+off_t][ci]
+// over this block's pixel range (one wave per (t, co-chunk, ci-chunk) tile,
+// K = pixels in steps of 16: A = G^T (co x p), B = X shifted (p x ci)).
+__global__ void __launch_bounds__(256) k_conv3x3_wgrad(
+    const unsigned short* __restrict__ G, int Cout, const unsigned short* __restrict__ X1, int C1,
+    const unsigned short* __restrict__ X2, int C2, int H, int Wd, long pix_per_block,
+    float* __restrict__ part) {
+    const int lane = threadIdx.x & 63;
+    const int Cin = C1 + C2;
+    const int nco = Cout / 16, nci = Cin / 16;
+    const int ntiles = 9 * nco * nci;
+    const long p0 = (long)blockIdx.x * pix_per_block;
+    const long npix = (long)H * Wd;
+    const long p1 = p0 + pix_per_block < npix ? p0 + pix_per_block : npix;
+    const int m = lane % 16, kq = 4 * (lane / 16);
+    for (int tile = threadIdx.x >> 6; tile < ntiles; tile += blockDim.x >> 6) {
+        const int t = tile / (nco * nci);
+        const int rem = tile - t * nco * nci;
+        const int cb = rem / nci, ib = rem - cb * nci;
+        const int dh = t / 3 - 1, dw = t % 3 - 1;
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (long p = p0; p < p1; p += 16) {
+            bf16x4 a, b;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long pp = p + kq + j;
+                unsigned short ga = 0, xb = 0;
+                if (pp < p1) {
+                    ga = G[pp * Cout + cb * 16 + m];
+                    const int hh = (int)(pp / Wd) + dh, ww = (int)(pp % Wd) + dw;
+                    if (hh >= 0 && hh < H && ww >= 0 && ww < Wd) {
+                        const long q = (long)hh * Wd + ww;
+                        const int ci = ib * 16 + m;
+                        xb = ci < C1 ? X1[q * C1 + ci] : X2[q * C2 + ci - C1];
+                    }
+                }
+                a[j] = (short)ga;
+                b[j] = (short)xb;
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        }
+        // D rows = co (4*(lane/16)+j within the chunk), cols = ci (lane%16)
+        float* dst = part + (long)blockIdx.x * 9 * Cout * Cin;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int co = cb * 16 + kq + j, ci = ib * 16 + m;
+            dst[((long)t * Cout + co) * Cin + ci] = acc[j];
+        }
+    }
+}
+
+// sum of partials over blocks (fixed order): out[k] += sum_b part[b * stride + k]
+__global__ void k_sum_partials(const float* __restrict__ part, int nb, long stride, long n,
+                               float* __restrict__ out) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    float s = 0.f;
+    for (int b = 0; b < nb; ++b) s += part[(long)b * stride + k];
+    out[k] += s;
+}
+
+// Backward-data weights: W2[t][ci][co] = W[8 - t][co][ci]
+__global__ void k_flip_transpose(const unsigned short* __restrict__ W, int Cout, int Cin,
+                                 unsigned short* __restrict__ W2) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = 9L * Cout * Cin;
+    if (k >= n) return;
+    const int t = (int)(k / ((long)Cout * Cin));
+    const long r = k - (long)t * Cout * Cin;
+    const int co = (int)(r / Cin), ci = (int)(r - (long)co * Cin);
+    W2[((long)(8 - t) * Cin + ci) * Cout + co] = W[k];
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise / resampling kernels (NHWC, bf16 activations)
+// ---------------------------------------------------------------------------
+// G = dY * (Y > 0)   (ReLU backward), bf16 in/out
+__global__ void k_relu_mask(const float* __restrict__ dY, const unsigned short* __restrict__ Y,
+                            long n, unsigned short* __restrict__ G) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) G[k] = f2bf(bf2f(Y[k]) > 0.f ? dY[k] : 0.f);
+}
+
+// 2x2 average pooling, C channels: Y[h][w] = mean of X[2h..2h+1][2w..2w+1]
+__global__ void k_avgpool2(const unsigned short* __restrict__ X, int H, int Wd, int C,
+                           unsigned short* __restrict__ Y) {
+    const int H2 = H / 2, W2 = Wd / 2;
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long)H2 * W2 * C) return;
+    const int c = (int)(k % C);
+    const long p = k / C;
+    const int h = (int)(p / W2), w = (int)(p % W2);
+    const float s = bf2f(X[((long)(2 * h) * Wd + 2 * w) * C + c]) +
+                    bf2f(X[((long)(2 * h) * Wd + 2 * w + 1) * C + c]) +
+                    bf2f(X[((long)(2 * h + 1) * Wd + 2 * w) * C + c]) +
+                    bf2f(X[((long)(2 * h + 1) * Wd + 2 * w + 1) * C + c]);
+    Y[k] = f2bf(0.25f * s);
+}
+
+// backward of avgpool: dX[fine] += dY[coarse] / 4
+__global__ void k_avgpool2_bwd(const float* __restrict__ dY, int H, int Wd, int C,
+                               float* __restrict__ dX) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long)H * Wd * C) return;
+    const int c = (int)(k % C);
+    const long p = k / C;
+    const int h = (int)(p / Wd), w = (int)(p % Wd);
+    dX[k] += 0.25f * dY[((long)(h / 2) * (Wd / 2) + w / 2) * C + c];
+}
+
+// nearest upsampling x2: Y[h][w] = X[h/2][w/2]
+__global__ void k_upsample2(const unsigned short* __restrict__ X, int H, int Wd, int C,
+                            unsigned short* __restrict__ Y) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long)H * Wd * C) return;
+    const int c = (int)(k % C);
+    const long p = k / C;
+    const int h = (int)(p / Wd), w = (int)(p % Wd);
+    Y[k] = X[((long)(h / 2) * (Wd / 2) + w / 2) * C + c];
+}
+
+// backward of upsampling: dX[coarse] += sum of the four fine dY
+__global__ void k_upsample2_bwd(const float* __restrict__ dY, int H, int Wd, int C,
+                                float* __restrict__ dX) {
+    const int H2 = H / 2, W2 = Wd / 2;
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long)H2 * W2 * C) return;
+    const int c = (int)(k % C);
+    const long p = k / C;
+    const int h = (int)(p / W2), w = (int)(p % W2);
+    dX[k] += dY[((long)(2 * h) * Wd + 2 * w) * C + c] + dY[((long)(2 * h) * Wd + 2 * w + 1) * C + c] +
+             dY[((long)(2 * h + 1) * Wd + 2 * w) * C + c] + dY[((long)(2 * h + 1) * Wd + 2 * w + 1) * C + c];
+}
+
+// first layer (1 -> C channels), scalar: Y = relu(b + sum_t W[t][c] x[p+off_t])
+__global__ void k_conv_in(const float* __restrict__ x, int H, int Wd, const float* __restrict__ W,
+                          const float* __restrict__ b, int C, unsigned short* __restrict__ Y) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long)H * Wd * C) return;
+    const int c = (int)(k % C);
+    const long p = k / C;
+    const int h = (int)(p / Wd), w = (int)(p % Wd);
+    float s = b[c];
+    for (int t = 0; t < 9; ++t) {
+        const int hh = h + t / 3 - 1, ww = w + t % 3 - 1;
+        if (hh >= 0 && hh < H && ww >= 0 && ww < Wd) s += W[t * C + c] * x[(long)hh * Wd + ww];
+    }
+    Y[k] = f2bf(fmaxf(s, 0.f));
+}
+
+// last layer (C -> 1), scalar, linear: y = b + sum_t sum_c W[t][c] X[p+off_t][c]
+__global__ void k_conv_out(const unsigned short* __restrict__ X, int H, int Wd, int C,
+                           const float* __restrict__ W, const float* __restrict__ b,
+                           float* __restrict__ y) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long)H * Wd) return;
+    const int h = (int)(p / Wd), w = (int)(p % Wd);
+    float s = b[0];
+    for (int t = 0; t < 9; ++t) {
+        const int hh = h + t / 3 - 1, ww = w + t % 3 - 1;
+        if (hh < 0 || hh >= H || ww < 0 || ww >= Wd) continue;
+        const unsigned short* xp = X + ((long)hh * Wd + ww) * C;
+        for (int c = 0; c < C; ++c) s += W[t * C + c] * bf2f(xp[c]);
+    }
+    y[p] = s;
+}
+
+// backward of the last layer to its input: dX[p][c] = sum_t W[t][c] dy[p - off_t]
+__global__ void k_conv_out_bwd_data(const float* __restrict__ dy, int H, int Wd, int C,
+                                    const float* __restrict__ W, float* __restrict__ dX) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long)H * Wd * C) return;
+    const int c = (int)(k % C);
+    const long p = k / C;
+    const int h = (int)(p / Wd), w = (int)(p % Wd);
+    float s = 0.f;
+    for (int t = 0; t < 9; ++t) {
+        const int hh = h - (t / 3 - 1), ww = w - (t % 3 - 1);
+        if (hh >= 0 && hh < H && ww >= 0 && ww < Wd) s += W[t * C + c] * dy[(long)hh * Wd + ww];
+    }
+    dX[k] = s;
+}
+
+// weight/bias gradient partials of the scalar layers over a pixel range:
+// in-layer:  gW[t][c] = sum_p G[p][c] x[p+off_t],  gb[c] = sum_p G[p][c]
+// out-layer: gW[t][c] = sum_p dy[p] X[p+off_t][c], gb = sum_p dy[p]
+// Each block writes 10*C floats (9 taps + bias) of partials.
+__global__ void k_scalar_layer_wgrad(int which, const float* __restrict__ xin,
+                                     const unsigned short* __restrict__ Gin,
+                                     const float* __restrict__ dy,
+                                     const unsigned short* __restrict__ Xout, int H, int Wd, int C,
+                                     long pix_per_block, float* __restrict__ part) {
+    __shared__ float acc[10 * 64];
+    for (int k = threadIdx.x; k < 10 * C; k += blockDim.x) acc[k] = 0.f;
+    __syncthreads();
+    const long npix = (long)H * Wd;
+    const long p0 = (long)blockIdx.x * pix_per_block;
+    const long p1 = p0 + pix_per_block < npix ? p0 + pix_per_block : npix;
+    // thread handles channel c for a strided pixel subset; private sums
+    const int c = threadIdx.x % C;
+    const int lanes = blockDim.x / C;
+    const int sub = threadIdx.x / C;
+    if (sub < lanes) {
+        float s[10];
+        for (int q = 0; q < 10; ++q) s[q] = 0.f;
+        for (long p = p0 + sub; p < p1; p += lanes) {
+            const int h = (int)(p / Wd), w = (int)(p % Wd);
+            const float g = which == 0 ? bf2f(Gin[p * C + c]) : dy[p];
+            if (g == 0.f) continue;
+            for (int t = 0; t < 9; ++t) {
+                const int hh = h + t / 3 - 1, ww = w + t % 3 - 1;
+                if (hh < 0 || hh >= H || ww < 0 || ww >= Wd) continue;
+                const long q = (long)hh * Wd + ww;
+                s[t] += g * (which == 0 ? xin[q] : bf2f(Xout[q * C + c]));
+            }
+            s[9] += g;
+        }
+        for (int q = 0; q < 10; ++q) atomicAdd(&acc[q * C + c], s[q]);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 10 * C; k += blockDim.x)
+        part[(long)blockIdx.x * 10 * C + k] = acc[k];
+}
+
+// bias gradient partials of an MFMA layer: gb[c] = sum_p G[p][c]
+__global__ void k_bias_grad(const unsigned short* __restrict__ G, long npix, int C,
+                            long pix_per_block, float* __restrict__ part) {
+    __shared__ float acc[128];
+    for (int k = threadIdx.x; k < C; k += blockDim.x) acc[k] = 0.f;
+    __syncthreads();
+    const long p0 = (long)blockIdx.x * pix_per_block;
+    const long p1 = p0 + pix_per_block < npix ? p0 + pix_per_block : npix;
+    const int c = threadIdx.x % C, lanes = blockDim.x / C, sub = threadIdx.x / C;
+    if (sub < lanes) {
+        float s = 0.f;
+        for (long p = p0 + sub; p < p1; p += lanes) s += bf2f(G[p * C + c]);
+        atomicAdd(&acc[c], s);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < C; k += blockDim.x) part[(long)blockIdx.x * C + k] = acc[k];
+}
+
+// fp32 -> bf16 copy and fp32 fill
+__global__ void k_to_bf16(const float* __restrict__ x, long n, unsigned short* __restrict__ y) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) y[k] = f2bf(x[k]);
+}
+__global__ void k_fill_f32(float* __restrict__ x, long n, float v) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) x[k] = v;
+}
+__global__ void k_scale_f32(float* __restrict__ x, long n, float v) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) x[k] *= v;
+}
+
+// masked squared error and its gradient: loss += sum m (y - t)^2 / norm,
+// dy = 2 m (y - t) / norm; per-block partial sums of the loss
+__global__ void k_mse(const float* __restrict__ y, const float* __restrict__ t,
+                      const float* __restrict__ mask, long n, float inv_norm,
+                      float* __restrict__ dy, float* __restrict__ part) {
+    __shared__ float s_acc[256];
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.f;
+    if (k < n) {
+        const float m = mask ? mask[k] : 1.f;
+        const float d = m * (y[k] - t[k]);
+        dy[k] = 2.f * d * inv_norm;
+        l = d * d * inv_norm;
+    }
+    s_acc[threadIdx.x] = l;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s_acc[threadIdx.x] += s_acc[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s_acc[0];
+}
+
+// Adam on FP32 master weights; refreshes the BF16 copy used by the MFMA layers
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m1,
+                       float* __restrict__ m2, long n, float lr, float b1, float b2, float eps,
+                       float c1, float c2, unsigned short* __restrict__ wb) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float gk = g[k];
+    const float a = b1 * m1[k] + (1.f - b1) * gk;
+    const float b = b2 * m2[k] + (1.f - b2) * gk * gk;
+    m1[k] = a;
+    m2[k] = b;
+    const float v = w[k] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+    w[k] = v;
+    if (wb) wb[k] = f2bf(v);
+}

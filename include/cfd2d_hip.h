@@ -225,6 +225,35 @@ int cfd_scalar_transport(const double* phi, const double* u, const double* v,
                          const int64_t* bt_t, const double* bv_t, int32_t obs_t, double obs_v,
                          double* out, void* stream);
 
+/* ---- learned preconditioner network (no counterpart in the reference: the
+   U-Net of the learned flexible PCG, trained in-process on residual ->
+   correction pairs harvested from the pressure solves) ----
+   4 levels, 3x3 convolutions, channels 16-32-64-64, ReLU, 2x2 average pooling,
+   nearest upsampling with skip concatenation; FP32 master parameters (flat
+   vector, layer by layer: weights [tap][Cout][Cin], then biases), BF16 weights
+   and activations on the matrix cores with FP32 accumulation.  Images are
+   H x W fp32 on the device, H and W multiples of 8. */
+typedef struct cfd_unet cfd_unet;
+int64_t cfd_unet_param_count(void);
+int cfd_unet_create(int32_t H, int32_t W, cfd_unet** out);
+int cfd_unet_destroy(cfd_unet* net);
+/* host <-> device parameter copies; set_params also resets the Adam moments */
+int cfd_unet_set_params(cfd_unet* net, const float* host);
+int cfd_unet_get_params(cfd_unet* net, float* host);
+int cfd_unet_get_grad(cfd_unet* net, float* host);
+int cfd_unet_zero_grad(cfd_unet* net);
+/* y = net(x) */
+int cfd_unet_forward(cfd_unet* net, const float* x_dev, float* y_dev);
+/* loss = sum mask (net(x) - t)^2 / tnorm2 (mask may be NULL); the gradient of
+   the loss is added to the network's gradient vector */
+int cfd_unet_loss_grad(cfd_unet* net, const float* x_dev, const float* t_dev,
+                       const float* mask_dev, double tnorm2, double* loss);
+/* one Adam step (beta 0.9/0.999) on the gradient times `scale`, then zero it */
+int cfd_unet_adam(cfd_unet* net, float lr, float scale);
+/* D = A B for 16 x 16 row-major matrices, through one bf16 matrix-core
+   instruction (checks the operand lane layout the convolutions rely on) */
+int cfd_mfma_layout_check(const float* A, const float* B, float* D);
+
 #ifdef __cplusplus
 }
 #endif

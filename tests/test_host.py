@@ -15,7 +15,7 @@ def test_library_exports_every_declared_symbol():
     from cfd2d_amd import _lib
 
     header = (ROOT / "include" / "cfd2d_hip.h").read_text()
-    declared = sorted(set(re.findall(r"^(?:int|const char\*)\s+(cfd_\w+)\(", header, re.M)))
+    declared = sorted(set(re.findall(r"^(?:int|int64_t|const char\*)\s+(cfd_\w+)\(", header, re.M)))
     assert len(declared) >= 20
     out = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)],
                          capture_output=True, text=True, check=True).stdout
