@@ -25,7 +25,8 @@ CFD_ERR_LIMIT = -4
 BC_KINDS = {"noslip": 0, "slip": 1, "inflow": 2, "outflow": 3, "periodic": 4}
 FIELDS = {"u": 0, "v": 1, "p": 2, "F": 3, "G": 4, "hu": 5, "hv": 6, "hu_old": 7, "hv_old": 8,
           "x": 9}
-SOLVERS = {"pcg64": 0, "mgcg": 1}
+SOLVERS = {"pcg64": 0, "mgcg": 1, "learned": 2}
+LEARNED_MODES = {"net": 0, "none": 1, "teacher": 2}
 
 _d = ctypes.c_double
 _i32 = ctypes.c_int32
@@ -61,6 +62,18 @@ class MGLevelDesc(ctypes.Structure):
 class MGDesc(ctypes.Structure):
     _fields_ = [("nlev", _i32), ("omega", ctypes.c_float), ("wc", ctypes.c_float),
                 ("ncoarse", _i32), ("cinv", _vp), ("lev", MGLevelDesc * 16)]
+
+
+class LearnedDesc(ctypes.Structure):
+    _fields_ = [("net", _vp), ("level", _i32), ("mode", _i32), ("teacher_cycles", _i32),
+                ("max_iter", _i32), ("fallback", _i32), ("use_graph", _i32),
+                ("sigma", ctypes.c_float), ("one_a_one", _d), ("harvest_in", _vp),
+                ("harvest_out", _vp), ("harvest_cap", _i64)]
+
+
+class LearnedStats(ctypes.Structure):
+    _fields_ = [("harvested", _i64), ("solves", _i64), ("iterations", _i64), ("fallbacks", _i64),
+                ("last_iters", _i32), ("last_info", _i32), ("last_relres", _d)]
 
 
 class State(ctypes.Structure):
@@ -118,6 +131,9 @@ def lib() -> ctypes.CDLL:
         "cfd_obstacle_bc": ([_vp, _vp, _vp, _i32, _i32, _i32, _vp], _i32),
         "cfd_scalar_transport": ([_vp, _vp, _vp, _vp, _i32, _i32, _d, _d, _vp, _d, _i32, _i32]
                                  + [_vp] * 8 + [_i32, _d, _vp, _vp], _i32),
+        "cfd_set_learned": ([_vp, P(LearnedDesc)], _i32),
+        "cfd_learned_stats": ([_vp, P(LearnedStats)], _i32),
+        "cfd_learned_reset": ([_vp], _i32),
         "cfd_unet_param_count": ([], _i64),
         "cfd_unet_create": ([_i32, _i32, P(_vp)], _i32),
         "cfd_unet_destroy": ([_vp], _i32),
@@ -128,6 +144,7 @@ def lib() -> ctypes.CDLL:
         "cfd_unet_forward": ([_vp, _vp, _vp], _i32),
         "cfd_unet_loss_grad": ([_vp, _vp, _vp, _vp, _d, P(_d)], _i32),
         "cfd_unet_adam": ([_vp, ctypes.c_float, ctypes.c_float], _i32),
+        "cfd_unet_set_loss": ([_vp, _i32, _i32, _i32, ctypes.c_float], _i32),
         "cfd_mfma_layout_check": ([_vp, _vp, _vp], _i32),
     }
     for name, (args, res) in sig.items():
@@ -147,10 +164,12 @@ def exported_symbols() -> list[str]:
                         "cfd_last_elapsed_ms", "cfd_set_profiling", "cfd_projection_times",
                         "cfd_pressure_solve", "cfd_dev_alloc",
                         "cfd_dev_free", "cfd_memcpy", "cfd_momentum_tendencies",
-                        "cfd_obstacle_bc", "cfd_scalar_transport", "cfd_unet_param_count",
+                        "cfd_obstacle_bc", "cfd_scalar_transport", "cfd_set_learned",
+                        "cfd_learned_stats", "cfd_learned_reset", "cfd_unet_param_count",
                         "cfd_unet_create", "cfd_unet_destroy", "cfd_unet_set_params",
                         "cfd_unet_get_params", "cfd_unet_get_grad", "cfd_unet_zero_grad",
                         "cfd_unet_forward", "cfd_unet_loss_grad", "cfd_unet_adam",
+                        "cfd_unet_set_loss",
                         "cfd_mfma_layout_check")]
 
 
@@ -196,6 +215,13 @@ class DeviceArray:
         check(lib().cfd_memcpy(out.ctypes.data_as(_vp), self.ptr, self.nbytes, 2), "copy to host")
         return out
 
+    def image(self, k: int) -> "DeviceImage":
+        """View of image k of a (N, H, W) array (no copy)."""
+        if len(self.shape) != 3 or not 0 <= k < self.shape[0]:
+            raise IndexError(k)
+        h, w = self.shape[1], self.shape[2]
+        return DeviceImage(self, k * h * w * self.dtype.itemsize, (h, w))
+
     def free(self) -> None:
         if self.ptr:
             lib().cfd_dev_free(self.ptr)
@@ -206,6 +232,19 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+class DeviceImage:
+    """A (H, W) window into a DeviceArray (keeps the owner alive)."""
+
+    def __init__(self, owner: DeviceArray, offset: int, shape):
+        self.owner, self.shape, self.dtype = owner, tuple(shape), owner.dtype
+        self.ptr = _vp(owner.ptr.value + offset)
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(lib().cfd_memcpy(out.ctypes.data_as(_vp), self.ptr, out.nbytes, 2), "copy to host")
+        return out
 
 
 def ptr(a: np.ndarray | None):

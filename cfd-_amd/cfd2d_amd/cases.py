@@ -27,7 +27,7 @@ from .models import BoussinesqTemperature, PassiveScalar, ScalarBC, default_scal
 from .solver import Simulation
 
 NUMERIK_KEYS = ("scheme", "gamma", "tau", "dt", "pressure_solver", "pressure_tol",
-                "pressure_maxiter")
+                "pressure_maxiter", "learned")
 
 
 @dataclass

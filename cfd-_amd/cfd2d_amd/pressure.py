@@ -16,7 +16,9 @@ start, recursive-residual stop, ``maxiter`` 5000); ``"direct"`` (the
 reference's LU) runs the same solver to rtol 1e-12 with a large iteration cap;
 ``"cg_mixed"`` is flexible CG with FP64 vectors and dot products,
 preconditioned by an FP32 aggregation-multigrid V-cycle
-(:mod:`cfd2d_amd.multigrid`), ending on the FP64 true residual.
+(:mod:`cfd2d_amd.multigrid`), ending on the FP64 true residual;
+``"cg_learned"`` is the same flexible CG with the learned preconditioner of
+:mod:`cfd2d_amd.learned` (a V-cycle whose coarse correction is a U-Net).
 """
 
 from __future__ import annotations
@@ -36,10 +38,10 @@ MAX_DIAG_VALUES = 63
 
 
 class PressureSolver:
-    METHODS = ("direct", "cg", "cg_mixed")
+    METHODS = ("direct", "cg", "cg_mixed", "cg_learned")
 
     def __init__(self, grid: Grid, boundary: DomainBoundary, method: str = "direct",
-                 tol: float = 1e-10, maxiter: int | None = None):
+                 tol: float = 1e-10, maxiter: int | None = None, learned=None):
         if method not in self.METHODS:
             raise ValueError(f"Unbekannter Drucklöser {method!r}, erlaubt: {self.METHODS}")
         self.grid, self.method = grid, method
@@ -130,11 +132,14 @@ class PressureSolver:
         self.last_iterations = 0
         self.last_relres = 0.0
         self._ctx = None
+        self.learned_spec = learned
+        self.learned = None
 
     # ------------------------------------------------------------------ #
     @property
     def solver_id(self) -> int:
-        return _lib.SOLVERS["mgcg"] if self.method == "cg_mixed" else _lib.SOLVERS["pcg64"]
+        return {"cg_mixed": _lib.SOLVERS["mgcg"],
+                "cg_learned": _lib.SOLVERS["learned"]}.get(self.method, _lib.SOLVERS["pcg64"])
 
     @property
     def rtol(self) -> float:
@@ -142,13 +147,17 @@ class PressureSolver:
 
     def attach(self, ctx) -> None:
         self._ctx = ctx
-        if self.method == "cg_mixed":
+        if self.method in ("cg_mixed", "cg_learned"):
             from .multigrid import Hierarchy
 
             g = self.grid
             self.hierarchy = Hierarchy(self.fluid, self.diag, 1.0 / g.dx ** 2, 1.0 / g.dy ** 2,
                                        self.periodic)
             self.hierarchy.upload(ctx)
+        if self.method == "cg_learned":
+            from .learned import Attached, LearnedConfig
+
+            self.learned = Attached(ctx, self.hierarchy, LearnedConfig.from_spec(self.learned_spec))
 
     def solve(self, rhs: np.ndarray) -> np.ndarray:
         """Solve ``Lap(p) = rhs`` for rhs of shape (nx, ny) on the device

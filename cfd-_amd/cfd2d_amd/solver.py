@@ -73,7 +73,7 @@ class Simulation:
                  compute_forces: bool = False, initial_velocity: tuple = (0.0, 0.0),
                  models: list | None = None, body_force: tuple = (0.0, 0.0),
                  flow_rate: float | None = None, pressure_tol: float | None = None,
-                 pressure_maxiter: int | None = None, sync_every: int = 1):
+                 pressure_maxiter: int | None = None, sync_every: int = 1, learned=None):
         if nu <= 0:
             raise ValueError("Die Viskosität muss positiv sein.")
         if scheme not in ("ab2", "euler"):
@@ -98,7 +98,7 @@ class Simulation:
         self.bcs = bcs
         self.pressure = PressureSolver(grid, self.boundary, method=pressure_solver,
                                        tol=1e-10 if pressure_tol is None else pressure_tol,
-                                       maxiter=pressure_maxiter)
+                                       maxiter=pressure_maxiter, learned=learned)
         self._ctx = DeviceContext(grid, self.boundary, self.pressure)
         self.pressure.attach(self._ctx)
 
@@ -512,7 +512,8 @@ class Simulation:
     def summary(self) -> str:
         g = self.grid
         solver = {"direct": "CG (Jacobi, FP64, rtol 1e-12)", "cg": "CG (Jacobi, FP64)",
-                  "cg_mixed": "CG (Jacobi, FP32/FP64)"}[self.pressure.method]
+                  "cg_mixed": "FCG (multigrid, FP32/FP64)",
+                  "cg_learned": "FCG (learned U-Net V-cycle, FP32/FP64)"}[self.pressure.method]
         lines = [
             f"Gitter: {g.nx} x {g.ny} Zellen, Gebiet {g.lx:g} x {g.ly:g}, "
             f"dx = {g.dx:.4g}, dy = {g.dy:.4g}",

@@ -100,6 +100,16 @@ class UNet:
                    "U-Net loss_grad")
         return float(loss.value)
 
+    def set_loss(self, kind: str = "mse", region: tuple[int, int] | None = None,
+                 eps: float = 0.0) -> None:
+        """'mse': masked squared error; 'energy': (y - t).A(y - t) on the
+        region (rows, cols) with A the unit 5-point Laplacian (zero-flux
+        sides) + eps I."""
+        k = {"mse": 0, "energy": 1}[kind]
+        nx, ny = region if region is not None else (self.H, self.W)
+        _lib.check(_lib.lib().cfd_unet_set_loss(self._h, k, int(nx), int(ny), float(eps)),
+                   "U-Net set_loss")
+
     def adam(self, lr: float, scale: float = 1.0) -> None:
         _lib.check(_lib.lib().cfd_unet_adam(self._h, float(lr), float(scale)), "U-Net adam")
 

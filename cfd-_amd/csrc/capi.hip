@@ -15,6 +15,7 @@
 #include "stencil.hip"
 #include "pcg.hip"
 #include "unet.hip"
+#include "lpcg.hip"
 
 namespace {
 
@@ -81,7 +82,27 @@ struct cfd_ctx {
     std::vector<ScalarDev> scalars;
     double* rold = nullptr;
     MGDev* mg_dev = nullptr;     // device copy of the hierarchy description
+    MGDev mg_host;               // host copy (level sizes and array pointers)
     int mg_ready = 0;
+    // learned flexible CG (lpcg.hip)
+    int l_ready = 0;
+    int l_level = 0, l_mode = 0, l_cycles = 0, l_fallback = 1, l_graph = 1, l_maxiter = 200;
+    int l_H = 0, l_W = 0;
+    float l_sigma = 1.0f, l_D = 1.0f;
+    double l_1a1 = 0.0;
+    cfd_unet* l_net = nullptr;
+    TeachVec l_tv = {};
+    int l_tv_level = -1;
+    float *l_hin = nullptr, *l_hout = nullptr;
+    long long l_hcap = 0;
+    float* diag0f = nullptr;
+    FcgState* fcg = nullptr;
+    FcgState* fcg_host = nullptr;   // pinned mirror
+    double* l_part = nullptr;
+    hipGraph_t l_graph_g = nullptr;
+    hipGraphExec_t l_graph_x = nullptr;
+    int l_prev_iters = 0;
+    long long l_fallbacks = 0, l_solves = 0, l_iters_total = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // optional timing of every projection launch (HIP events on the context stream)
     int prof_on = 0;
@@ -249,18 +270,18 @@ ProjArgs proj_args(cfd_ctx* c, int method, double rtol, int maxiter) {
     return a;
 }
 
-int launch_project(cfd_ctx* c, ProjArgs a) {
-    if (a.method != PROJ_PCG64 && a.method != PROJ_MG)
-        return set_err(CFD_ERR_ARG, "pressure solver %d is not available", a.method);
-    if (a.method == PROJ_MG && !c->mg_ready)
-        return set_err(CFD_ERR_STATE, "multigrid solver selected before cfd_set_multigrid");
+int run_learned(cfd_ctx* c, ProjArgs a);
+
+// one cooperative launch of k_project (timed when profiling is on and prof)
+int launch_coop(cfd_ctx* c, ProjArgs a, bool prof = true) {
     const int words = (c->proj_blocks + 2) * CFD_SLOT * 2;
     hipLaunchKernelGGL(k_zero_u32, dim3((words + 255) / 256), dim3(256), 0, c->stream,
                        (unsigned*)c->sync, words);
     HIPC(hipGetLastError());
     Geo g = c->geo;
     void* args[] = {(void*)&a, (void*)&g};
-    if (c->prof_on) {
+    prof = prof && c->prof_on;
+    if (prof) {
         if (c->prof_used + 2 > c->prof_ev.size()) {
             for (int k = 0; k < 256; ++k) {
                 hipEvent_t e;
@@ -272,11 +293,24 @@ int launch_project(cfd_ctx* c, ProjArgs a) {
     }
     HIPC(hipLaunchCooperativeKernel((const void*)k_project, dim3(c->proj_blocks),
                                     dim3(c->proj_bs), args, 0, c->stream));
-    if (c->prof_on) {
+    if (prof) {
         HIPC(hipEventRecord(c->prof_ev[c->prof_used + 1], c->stream));
         c->prof_used += 2;
     }
     return 0;
+}
+
+int launch_project(cfd_ctx* c, ProjArgs a) {
+    if (a.method != PROJ_PCG64 && a.method != PROJ_MG && a.method != PROJ_LEARNED)
+        return set_err(CFD_ERR_ARG, "pressure solver %d is not available", a.method);
+    if (a.method == PROJ_MG && !c->mg_ready)
+        return set_err(CFD_ERR_STATE, "multigrid solver selected before cfd_set_multigrid");
+    if (a.method == PROJ_LEARNED) {
+        if (!c->l_ready)
+            return set_err(CFD_ERR_STATE, "learned solver selected before cfd_set_learned");
+        return run_learned(c, a);
+    }
+    return launch_coop(c, a);
 }
 
 ScalarArgs scalar_args(cfd_ctx* c, ScalarDev& s, int par) {
@@ -305,6 +339,196 @@ int host_parity(cfd_ctx* c, int* par) {
     DevState h;
     TRY(read_state(c, &h));
     *par = (int)(h.n & 1);
+    return 0;
+}
+
+}  // namespace
+
+#include "unet_net.hip"
+
+namespace {
+
+LvArgs lv_args(cfd_ctx* c) {
+    LvArgs a;
+    a.mg = c->mg_dev;
+    a.code = c->code;
+    a.diag0 = c->diag0f;
+    a.periodic = c->periodic;
+    a.ax = (float)(1.0 / (c->dx * c->dx));
+    a.ay = (float)(1.0 / (c->dy * c->dy));
+    a.fs = c->fcg;
+    a.part = c->l_part;
+    return a;
+}
+
+FcgVec fcg_vec(cfd_ctx* c) {
+    FcgVec v;
+    v.x = c->x;
+    v.b = c->b;
+    v.r = c->r;
+    v.rold = c->rold;
+    v.q = c->q;
+    v.d0 = c->d0;
+    v.d1 = c->d1;
+    v.lut = c->lut;
+    v.nx = c->nx;
+    v.pc = c->P;
+    v.ax = 1.0 / (c->dx * c->dx);
+    v.ay = 1.0 / (c->dy * c->dy);
+    return v;
+}
+
+#define LV_LAUNCH(kern, blocks, ...)                                                       \
+    do {                                                                                   \
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, c->stream, __VA_ARGS__);       \
+        HIPC(hipGetLastError());                                                           \
+    } while (0)
+
+// the kernels of one flexible-CG iteration, in stream order
+int learned_iteration(cfd_ctx* c) {
+    const LvArgs a = lv_args(c);
+    const FcgVec v = fcg_vec(c);
+    const MGDev& mg = c->mg_host;
+    const int K = c->l_level, NL = mg.nlev;
+    auto blocks = [&](int l) { return (unsigned)std::min(mg.lev[l].nx, 1024); };
+    const unsigned nimg = (unsigned)(((long)c->l_H * c->l_W + 255) / 256);
+    for (int l = 0; l < K; ++l) {
+        LV_LAUNCH(k_lv_smooth, blocks(l), a, l, 0);
+        LV_LAUNCH(k_lv_restrict, blocks(l + 1), a, l);
+    }
+    if (c->l_mode != LEARNED_TEACHER) {
+        const bool net = c->l_mode == LEARNED_NET;
+        LV_LAUNCH(k_lv_cmode, blocks(K), a, K, c->l_1a1, 0LL);
+        if (net) {
+            LV_LAUNCH(k_lv_net_in, nimg, a, K, c->l_net->xin, c->l_H, c->l_W);
+            // the network runs on this context's stream and idles once the solve is done
+            cfd_unet* u = c->l_net;
+            const hipStream_t s0 = u->stream;
+            const int* k0 = u->skip;
+            u->stream = c->stream;
+            u->skip = &c->fcg->done;
+            const int rc = unet_forward(u);
+            u->stream = s0;
+            u->skip = k0;
+            TRY(rc);
+        }
+        LV_LAUNCH(k_lv_net_out, blocks(K), a, K, net ? (const float*)c->l_net->y : nullptr, c->l_W,
+                  c->l_sigma / c->l_D, net ? 1 : 0);
+        LV_LAUNCH(k_lv_smooth, blocks(K), a, K, 1);
+        LV_LAUNCH(k_lv_smooth, blocks(K), a, K, 0);
+    } else {
+        const TeachVec tv = c->l_tv;
+        LV_LAUNCH(k_lv_cmode, blocks(K), a, K, c->l_1a1, c->l_hcap);
+        LV_LAUNCH(k_tch_edge, blocks(K), a, K, tv, 0);
+        for (int it = 0; it < c->l_cycles; ++it) {
+            for (int l = K; l < NL - 1; ++l) {
+                LV_LAUNCH(k_lv_smooth, blocks(l), a, l, 0);
+                LV_LAUNCH(k_lv_restrict, blocks(l + 1), a, l);
+            }
+            LV_LAUNCH(k_lv_dense, 1, a);
+            for (int l = NL - 2; l >= K; --l) {
+                LV_LAUNCH(k_lv_post, blocks(l), a, l);
+                LV_LAUNCH(k_lv_smooth, blocks(l), a, l, 0);
+            }
+            LV_LAUNCH(k_tch_dots, blocks(K), a, K, tv, it);
+            LV_LAUNCH(k_tch_dir, blocks(K), a, K, tv, it);
+            LV_LAUNCH(k_tch_update, blocks(K), a, K, tv, it);
+        }
+        LV_LAUNCH(k_tch_edge, blocks(K), a, K, tv, 1);
+        if (c->l_hcap > 0)
+            LV_LAUNCH(k_lv_harvest, nimg, a, K, c->l_hin, c->l_hout, c->l_H, c->l_W, c->l_D);
+    }
+    for (int l = K - 1; l >= 0; --l) {
+        LV_LAUNCH(k_lv_post, blocks(l), a, l);
+        LV_LAUNCH(k_lv_smooth, blocks(l), a, l, 0);
+    }
+    LV_LAUNCH(k_fcg_dots, blocks(0), a, v);
+    LV_LAUNCH(k_fcg_dir, blocks(0), a, v);
+    LV_LAUNCH(k_fcg_update, blocks(0), a, v);
+    LV_LAUNCH(k_fcg_true, blocks(0), a, v);
+    return 0;
+}
+
+void learned_drop_graph(cfd_ctx* c) {
+    if (c->l_graph_x) hipGraphExecDestroy(c->l_graph_x);
+    if (c->l_graph_g) hipGraphDestroy(c->l_graph_g);
+    c->l_graph_x = nullptr;
+    c->l_graph_g = nullptr;
+}
+
+int learned_capture(cfd_ctx* c) {
+    HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    const int rc = learned_iteration(c);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    HIPC(e);
+    c->l_graph_g = g;
+    HIPC(hipGraphInstantiate(&c->l_graph_x, g, nullptr, nullptr, 0));
+    return 0;
+}
+
+// A projection with the learned flexible CG: k_project up to the initial
+// residual, iterations until the device reports the solve finished (the host
+// polls the state after the predicted number of iterations, then after each
+// one), then k_project from the pressure write-back on; a solve that does not
+// converge is continued by the FP64 Jacobi-PCG of k_project when enabled.
+int run_learned(cfd_ctx* c, ProjArgs a) {
+    const int maxiter_all = a.maxiter;
+    a.fcg = c->fcg;
+    a.maxiter = std::min(a.maxiter, c->l_maxiter);
+    a.phase = 1;
+    if (c->l_graph && !c->l_graph_x) TRY(learned_capture(c));
+    const bool prof = c->prof_on;
+    if (prof) {   // the whole projection is timed as one unit
+        if (c->prof_used + 2 > c->prof_ev.size()) {
+            for (int k = 0; k < 256; ++k) {
+                hipEvent_t e;
+                HIPC(hipEventCreate(&e));
+                c->prof_ev.push_back(e);
+            }
+        }
+        HIPC(hipEventRecord(c->prof_ev[c->prof_used], c->stream));
+    }
+    TRY(launch_coop(c, a, false));
+    int batch = std::max(1, c->l_prev_iters - 1);
+    int launched = 0;
+    for (;;) {
+        for (int q = 0; q < batch; ++q) {
+            if (c->l_graph_x) HIPC(hipGraphLaunch(c->l_graph_x, c->stream));
+            else TRY(learned_iteration(c));
+        }
+        launched += batch;
+        HIPC(hipMemcpyAsync(c->fcg_host, c->fcg, sizeof(FcgState), hipMemcpyDeviceToHost,
+                            c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+        if (c->fcg_host->done) break;
+        if (launched > a.maxiter + 2)
+            return set_err(CFD_ERR_STATE, "learned solver did not finish in %d iterations", launched);
+        batch = 1;
+    }
+    const FcgState h = *c->fcg_host;
+    c->l_solves += 1;
+    c->l_iters_total += h.iters;
+    if (h.info == 0) c->l_prev_iters = h.iters;
+    if (h.info != 0 && c->l_fallback) {
+        c->l_fallbacks += 1;
+        ProjArgs f = a;
+        f.method = PROJ_PCG64;
+        f.phase = 0;
+        f.maxiter = maxiter_all;
+        TRY(launch_coop(c, f, false));
+    } else {
+        a.phase = 2;
+        TRY(launch_coop(c, a, false));
+    }
+    if (prof) {
+        HIPC(hipEventRecord(c->prof_ev[c->prof_used + 1], c->stream));
+        c->prof_used += 2;
+    }
     return 0;
 }
 
@@ -452,6 +676,9 @@ int cfd_ctx_destroy(cfd_ctx* c) {
     if (!c) return 0;
     if (c->stream) hipStreamSynchronize(c->stream);
     for (hipEvent_t e : c->prof_ev) hipEventDestroy(e);
+    if (c->l_graph_x) hipGraphExecDestroy(c->l_graph_x);
+    if (c->l_graph_g) hipGraphDestroy(c->l_graph_g);
+    if (c->fcg_host) hipHostFree(c->fcg_host);
     for (void* p : c->allocs) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -551,7 +778,107 @@ int cfd_set_multigrid(cfd_ctx* c, const cfd_mg_desc* d) {
     h.cinv = cinv;
     if (!c->mg_dev) TRY(dalloc(c, &c->mg_dev, 1));
     HIPC(hipMemcpy(c->mg_dev, &h, sizeof(MGDev), hipMemcpyHostToDevice));
+    c->mg_host = h;
     c->mg_ready = 1;
+    c->l_ready = 0;   // a learned solver set up on the previous hierarchy is stale
+    if (c->l_graph_x) {
+        hipGraphExecDestroy(c->l_graph_x);
+        hipGraphDestroy(c->l_graph_g);
+        c->l_graph_x = nullptr;
+        c->l_graph_g = nullptr;
+    }
+    return 0;
+}
+
+int cfd_set_learned(cfd_ctx* c, const cfd_learned_desc* d) {
+    if (!c || !d) return set_err(CFD_ERR_ARG, "null argument");
+    if (!c->mg_ready) return set_err(CFD_ERR_STATE, "cfd_set_multigrid must come first");
+    const MGDev& mg = c->mg_host;
+    if (d->level < 0 || d->level >= mg.nlev)
+        return set_err(CFD_ERR_ARG, "level %d outside the hierarchy (0..%d)", d->level, mg.nlev - 1);
+    if (d->mode < 0 || d->mode > 2) return set_err(CFD_ERR_ARG, "mode %d", d->mode);
+    const MGLevelDev& L = mg.lev[d->level];
+    const int H = ((L.nx + 7) / 8) * 8, W = ((L.ny + 7) / 8) * 8;
+    if (d->mode == LEARNED_NET) {
+        if (!d->net) return set_err(CFD_ERR_ARG, "network mode without a network");
+        if (d->net->H != H || d->net->W != W)
+            return set_err(CFD_ERR_ARG, "network image %d x %d, level %d needs %d x %d", d->net->H,
+                           d->net->W, d->level, H, W);
+    }
+    if (d->mode == LEARNED_TEACHER && (d->teacher_cycles < 1 || d->teacher_cycles > 64))
+        return set_err(CFD_ERR_ARG, "teacher_cycles %d", d->teacher_cycles);
+    if (d->harvest_cap < 0 || (d->harvest_cap > 0 && (!d->harvest_in || !d->harvest_out)))
+        return set_err(CFD_ERR_ARG, "harvest buffers");
+    if (d->max_iter < 1) return set_err(CFD_ERR_ARG, "max_iter %d", d->max_iter);
+    HIPC(hipStreamSynchronize(c->stream));
+    if (!c->fcg) {
+        TRY(dalloc(c, &c->fcg, 1));
+        TRY(dalloc(c, &c->l_part, 1024 * 8));
+        TRY(dalloc(c, &c->diag0f, 64));
+        HIPC(hipHostMalloc((void**)&c->fcg_host, sizeof(FcgState), 0));
+        std::vector<double> lut(128);
+        HIPC(hipMemcpy(lut.data(), c->lut, 128 * sizeof(double), hipMemcpyDeviceToHost));
+        float df[64];
+        for (int k = 0; k < 64; ++k) df[k] = (float)lut[k];
+        HIPC(hipMemcpy(c->diag0f, df, sizeof(df), hipMemcpyHostToDevice));
+    }
+    FcgState h;
+    std::memset(&h, 0, sizeof(h));
+    h.done = 1;   // nothing to iterate before the first solve
+    HIPC(hipMemcpy(c->fcg, &h, sizeof(h), hipMemcpyHostToDevice));
+    const size_t n = (size_t)(L.nx + 2) * L.P;
+    if (d->mode == LEARNED_TEACHER && c->l_tv_level != d->level) {
+        float** v[7] = {&c->l_tv.e, &c->l_tv.r, &c->l_tv.ro, &c->l_tv.d0, &c->l_tv.d1, &c->l_tv.q,
+                        &c->l_tv.bk};
+        for (float** p : v) TRY(dalloc(c, p, n));
+        c->l_tv_level = d->level;
+    }
+    learned_drop_graph(c);
+    c->l_level = d->level;
+    c->l_mode = d->mode;
+    c->l_cycles = d->teacher_cycles;
+    c->l_sigma = d->sigma;
+    c->l_D = (float)(2.0 / (c->dx * c->dx) + 2.0 / (c->dy * c->dy));
+    c->l_1a1 = d->one_a_one;
+    c->l_fallback = d->fallback ? 1 : 0;
+    c->l_graph = d->use_graph ? 1 : 0;
+    c->l_maxiter = d->max_iter;
+    c->l_H = H;
+    c->l_W = W;
+    c->l_net = d->mode == LEARNED_NET ? d->net : nullptr;
+    c->l_hin = d->harvest_in;
+    c->l_hout = d->harvest_out;
+    c->l_hcap = d->harvest_cap;
+    c->l_prev_iters = 0;
+    c->l_ready = 1;
+    return 0;
+}
+
+int cfd_learned_stats(cfd_ctx* c, cfd_learned_stats_t* out) {
+    if (!c || !out) return set_err(CFD_ERR_ARG, "null argument");
+    if (!c->fcg) return set_err(CFD_ERR_STATE, "no learned solver");
+    HIPC(hipStreamSynchronize(c->stream));
+    FcgState h;
+    HIPC(hipMemcpy(&h, c->fcg, sizeof(h), hipMemcpyDeviceToHost));
+    out->harvested = h.harvested;
+    out->solves = c->l_solves;
+    out->iterations = c->l_iters_total;
+    out->fallbacks = c->l_fallbacks;
+    out->last_iters = h.iters;
+    out->last_info = h.info;
+    out->last_relres = h.bnorm > 0.0 ? sqrt(h.rr) / h.bnorm : 0.0;
+    return 0;
+}
+
+int cfd_learned_reset(cfd_ctx* c) {
+    if (!c) return set_err(CFD_ERR_ARG, "null argument");
+    if (!c->fcg) return set_err(CFD_ERR_STATE, "no learned solver");
+    HIPC(hipStreamSynchronize(c->stream));
+    FcgState h;
+    HIPC(hipMemcpy(&h, c->fcg, sizeof(h), hipMemcpyDeviceToHost));
+    h.harvested = 0;
+    HIPC(hipMemcpy(c->fcg, &h, sizeof(h), hipMemcpyHostToDevice));
+    c->l_solves = c->l_iters_total = c->l_fallbacks = 0;
     return 0;
 }
 
@@ -937,5 +1264,3 @@ int cfd_scalar_transport(const double* phi, const double* u, const double* v,
 }
 
 }  // extern "C"
-
-#include "unet_net.hip"

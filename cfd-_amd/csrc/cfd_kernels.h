@@ -150,11 +150,26 @@ struct ProjArgs {
     double rtol;
     int maxiter;
     int zero_p_after;        // initial projection: p := 0 afterwards
-    int method;              // PROJ_PCG64, PROJ_MG
+    int method;              // PROJ_PCG64, PROJ_MG, PROJ_LEARNED
     double reliable_delta;   // mixed: true-residual replacement when ||r|| drops by this factor
+    int phase;               // 0 whole projection; PROJ_LEARNED: 1 = up to the initial
+                             // residual, 2 = from the pressure write-back on
+    struct FcgState* fcg;    // PROJ_LEARNED: state of the host-driven flexible CG
 };
 
-enum { PROJ_PCG64 = 0, PROJ_MG = 1 };
+enum { PROJ_PCG64 = 0, PROJ_MG = 1, PROJ_LEARNED = 2 };
+
+// State of the learned flexible CG, whose iterations are separate kernel
+// launches (device memory; the reductions finish in the last block to arrive).
+struct FcgState {
+    double bnorm, atol, rr, rho, rho_old, beta, alpha;
+    double cm;               // constant-mode coefficient of the current coarse solve
+    double s_in;             // RMS of the current coarse right-hand side (network input scale)
+    double irho, irho_old, ibeta, ialpha;   // teacher: flexible CG on the coarse level
+    int k, first, done, pending, iters, info, reps, maxiter;
+    long long harvested, hidx;
+    unsigned ticket[4];
+};
 
 // kernels (stencil.hip)
 __global__ void k_momentum_tendencies(const double* u, const double* v, const uint8_t* fl, Geo g,

@@ -1,0 +1,519 @@
+// Learned flexible CG for the pressure system (pressure_solver "cg_learned").
+//
+// Outer solver: flexible CG (Polak-Ribiere beta) with FP64 vectors and dot
+// products, ending on the FP64 true residual ||b - A x|| < rtol ||b|| (with a
+// restart from the true residual otherwise), as in the multigrid-preconditioned
+// branch of k_project.  Its iterations are ordinary kernel launches, captured
+// once into a HIP graph and replayed; every kernel returns at once after the
+// solve has finished, and reductions finish in the last work-group to arrive
+// (fixed-order sums, so results are reproducible).
+//
+// Preconditioner: an FP32 V-cycle over the aggregation hierarchy (damped
+// Jacobi, two sweeps down and up, as mg_vcycle) that stops at a coarse level K,
+// where the coarse correction is
+//     e_K = c 1 + (sigma / D) s NN((b_K - c A_K 1) / s)
+// followed by two Jacobi sweeps on level K.  c = 1^T b_K / 1^T A_K 1 is the
+// Galerkin correction in the constant vector (the near-null space of the
+// pinned Neumann problem), s = RMS of the deflated right-hand side, D the
+// interior diagonal 2 (1/dx^2 + 1/dy^2), sigma the network's output scale, and
+// NN the U-Net of unet.hip (an image of the level-K cells padded to multiples
+// of 8).  Without the network (mode LEARNED_NONE) the coarse correction is
+// c 1 alone; the teacher mode (LEARNED_TEACHER) solves level K to high accuracy
+// (flexible CG preconditioned by V-cycles down to the dense coarsest level)
+// and can store the pair (network input, normalised correction) for training.
+#include "cfd_kernels.h"
+
+enum { LEARNED_NET = 0, LEARNED_NONE = 1, LEARNED_TEACHER = 2 };
+
+struct LvArgs {
+    const MGDev* mg;         // hierarchy (device memory)
+    const uint8_t* code;     // fine-level codes
+    const float* diag0;      // fine diagonal table (64 floats)
+    int periodic;
+    float ax, ay;
+    FcgState* fs;
+    double* part;            // reduction partials
+};
+
+struct FcgVec {
+    double *x, *b, *r, *rold, *q, *d0, *d1;
+    const double* lut;       // fine diagonal table (double)
+    int nx, pc;
+    double ax, ay;
+};
+
+namespace {
+
+__device__ __forceinline__ MGCtx lv_ctx(const LvArgs& a) {
+    return MGCtx{a.mg, a.code, a.diag0, a.periodic, a.ax, a.ay};
+}
+
+// f(i, j) over the level's interior cells: a band of rows per work-group,
+// threads along the contiguous j direction
+template <typename F>
+__device__ __forceinline__ void lv_cells(int nx, int ny, F f) {
+    const int i0 = 1 + (int)(((long)nx * blockIdx.x) / gridDim.x);
+    const int i1 = 1 + (int)(((long)nx * (blockIdx.x + 1)) / gridDim.x);
+    for (int i = i0; i < i1; ++i)
+        for (int j = 1 + (int)threadIdx.x; j <= ny; j += blockDim.x) f(i, j);
+}
+
+// Block sums of v written to part; the last work-group to arrive returns true
+// with the totals (summed in block order) in v.
+template <int NV>
+__device__ bool lv_reduce(double (&v)[NV], double* part, unsigned* ticket) {
+    __shared__ double lds[NV * 16];
+    __shared__ int s_last;
+    block_sum<NV>(v, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) part[(long)blockIdx.x * NV + q] = v[q];
+        __threadfence();
+        const unsigned t =
+            __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = t == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    __threadfence();
+    double acc[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) acc[q] += part[(long)b * NV + q];
+    block_sum<NV>(acc, lds);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = acc[q];
+    if (threadIdx.x == 0) *ticket = 0u;
+    return true;
+}
+
+// (A 1)(i, j) on level l: the row sum over fluid neighbours
+__device__ __forceinline__ float lv_a1(const MGCtx& m, int l, const MGLevelDev& L, int i, int j,
+                                       float d) {
+    auto one = [&](int a, int b) -> float {
+        return mg_diag(m, l, L, ix(a, b, L.P)) != 0.0f ? 1.0f : 0.0f;
+    };
+    return mg_apply(m, l, L, i, j, d, one);
+}
+
+}  // namespace
+
+// dst = src + w (b - A src) / d on level l (from_t: src = t, dst = x; else x -> t)
+__global__ void __launch_bounds__(256) k_lv_smooth(LvArgs a, int l, int from_t) {
+    if (a.fs->done) return;
+    const MGCtx m = lv_ctx(a);
+    const MGLevelDev L = a.mg->lev[l];
+    const float w = a.mg->omega;
+    const float* src = from_t ? L.t : L.x;
+    float* dst = from_t ? L.x : L.t;
+    lv_cells(L.nx, L.ny, [&](int i, int j) {
+        const long c = ix(i, j, L.P);
+        const float d = mg_diag(m, l, L, c);
+        if (d == 0.0f) return;
+        auto sv = [&](int p, int q) -> float { return src[ix(p, q, L.P)]; };
+        dst[c] = src[c] + w * (L.b[c] - mg_apply(m, l, L, i, j, d, sv)) / d;
+    });
+}
+
+// level l+1: b = sum over the 2x2 children of (b - A t); x = w b / d
+__global__ void __launch_bounds__(256) k_lv_restrict(LvArgs a, int l) {
+    if (a.fs->done) return;
+    const MGCtx m = lv_ctx(a);
+    const MGLevelDev L = a.mg->lev[l];
+    const MGLevelDev C = a.mg->lev[l + 1];
+    const float w = a.mg->omega;
+    lv_cells(C.nx, C.ny, [&](int ci, int cj) {
+        const long cc = ix(ci, cj, C.P);
+        float bc = 0.0f;
+        for (int di = 0; di < 2; ++di)
+            for (int dj = 0; dj < 2; ++dj) {
+                const int i = 2 * ci - 1 + di, j = 2 * cj - 1 + dj;
+                if (i > L.nx || j > L.ny) continue;
+                const long c = ix(i, j, L.P);
+                const float d = mg_diag(m, l, L, c);
+                if (d == 0.0f) continue;
+                auto tv = [&](int p, int q) -> float { return L.t[ix(p, q, L.P)]; };
+                bc += L.b[c] - mg_apply(m, l, L, i, j, d, tv);
+            }
+        C.b[cc] = bc;
+        const float dc = C.diag[cc];
+        C.x[cc] = dc != 0.0f ? w * bc / dc : 0.0f;
+    });
+}
+
+// coarsest level: t = cinv b (one work-group)
+__global__ void __launch_bounds__(256) k_lv_dense(LvArgs a) {
+    if (a.fs->done) return;
+    const MGDev& mg = *a.mg;
+    const MGLevelDev C = mg.lev[mg.nlev - 1];
+    const int n = mg.ncoarse;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        float acc = 0.0f;
+        for (int q = 0; q < n; ++q)
+            acc += mg.cinv[(long)k * n + q] * C.b[ix(q / C.ny + 1, q % C.ny + 1, C.P)];
+        C.t[ix(k / C.ny + 1, k % C.ny + 1, C.P)] = acc;
+    }
+}
+
+// level l: y = t + wc e_parent, then x = y + w (b - A y) / d
+__global__ void __launch_bounds__(256) k_lv_post(LvArgs a, int l) {
+    if (a.fs->done) return;
+    const MGCtx m = lv_ctx(a);
+    const MGLevelDev L = a.mg->lev[l];
+    const MGLevelDev C = a.mg->lev[l + 1];
+    const float w = a.mg->omega, wc = a.mg->wc;
+    auto corr = [&](int p, int q) -> float {
+        const long c = ix(p, q, L.P);
+        if (mg_diag(m, l, L, c) == 0.0f) return 0.0f;
+        return L.t[c] + wc * C.t[ix((p + 1) >> 1, (q + 1) >> 1, C.P)];
+    };
+    lv_cells(L.nx, L.ny, [&](int i, int j) {
+        const long c = ix(i, j, L.P);
+        const float d = mg_diag(m, l, L, c);
+        if (d == 0.0f) return;
+        const float y = corr(i, j);
+        L.x[c] = y + w * (L.b[c] - mg_apply(m, l, L, i, j, d, corr)) / d;
+    });
+}
+
+// Level K statistics: c = 1^T b / 1^T A 1 and s = RMS(b - c A 1) over the
+// fluid cells; reserves a harvest slot when harvest_cap > 0.
+__global__ void __launch_bounds__(256) k_lv_cmode(LvArgs a, int l, double one_a_one,
+                                                  long long harvest_cap) {
+    FcgState* fs = a.fs;
+    if (fs->done) return;
+    const MGCtx m = lv_ctx(a);
+    const MGLevelDev L = a.mg->lev[l];
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    lv_cells(L.nx, L.ny, [&](int i, int j) {
+        const long c = ix(i, j, L.P);
+        const float d = mg_diag(m, l, L, c);
+        if (d == 0.0f) return;
+        const double a1 = lv_a1(m, l, L, i, j, d);
+        const double bv = L.b[c];
+        s[0] += bv;
+        s[1] += bv * bv;
+        s[2] += bv * a1;
+        s[3] += a1 * a1;
+        s[4] += 1.0;
+    });
+    if (lv_reduce<5>(s, a.part, &fs->ticket[0]) && threadIdx.x == 0) {
+        const double cm = one_a_one > 0.0 ? s[0] / one_a_one : 0.0;
+        const double bb = s[1] - 2.0 * cm * s[2] + cm * cm * s[3];
+        fs->cm = cm;
+        fs->s_in = s[4] > 0.0 ? sqrt(fmax(bb, 0.0) / s[4]) : 0.0;
+        if (harvest_cap > 0 && fs->harvested < harvest_cap) {
+            fs->hidx = fs->harvested;
+            fs->harvested += 1;
+        } else {
+            fs->hidx = -1;
+        }
+    }
+}
+
+// network input: the deflated, normalised level-K right-hand side on the
+// padded H x W image (zero outside the fluid cells)
+__global__ void __launch_bounds__(256) k_lv_net_in(LvArgs a, int l, float* img, int H, int W) {
+    const FcgState* fs = a.fs;
+    if (fs->done) return;
+    const MGCtx m = lv_ctx(a);
+    const MGLevelDev L = a.mg->lev[l];
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long)H * W) return;
+    const int i = (int)(k / W) + 1, j = (int)(k % W) + 1;
+    float v = 0.0f;
+    if (i <= L.nx && j <= L.ny) {
+        const long c = ix(i, j, L.P);
+        const float d = mg_diag(m, l, L, c);
+        const double s = fs->s_in;
+        if (d != 0.0f && s > 0.0)
+            v = (float)(((double)L.b[c] - fs->cm * (double)lv_a1(m, l, L, i, j, d)) / s);
+    }
+    img[k] = v;
+}
+
+// level-K correction t = c + gain s y (gain = sigma / D; use_net = 0: t = c)
+__global__ void __launch_bounds__(256) k_lv_net_out(LvArgs a, int l, const float* y, int W,
+                                                    float gain, int use_net) {
+    const FcgState* fs = a.fs;
+    if (fs->done) return;
+    const MGCtx m = lv_ctx(a);
+    const MGLevelDev L = a.mg->lev[l];
+    const double cm = fs->cm, s = fs->s_in;
+    lv_cells(L.nx, L.ny, [&](int i, int j) {
+        const long c = ix(i, j, L.P);
+        if (mg_diag(m, l, L, c) == 0.0f) return;
+        double t = cm;
+        if (use_net) t += (double)gain * s * (double)y[(long)(i - 1) * W + (j - 1)];
+        L.t[c] = (float)t;
+    });
+}
+
+// Teacher solve of level K: flexible CG in FP32 vectors with FP64 dot
+// products, preconditioned by V-cycles from level K down to the dense
+// coarsest level, a fixed number of iterations.  Vectors on level K (pitch of
+// the level): e solution, r residual, ro previous residual, d0/d1 directions,
+// q = A d, bk the right-hand side.
+struct TeachVec {
+    float *e, *r, *ro, *d0, *d1, *q, *bk;
+};
+
+// begin: bk = b, e = 0, r = b, first V-cycle input (b, x = w b / d);
+// end: t = e, b = bk
+__global__ void __launch_bounds__(256) k_tch_edge(LvArgs a, int l, TeachVec tv, int end) {
+    if (a.fs->done) return;
+    const MGCtx m = lv_ctx(a);
+    const MGLevelDev L = a.mg->lev[l];
+    const float w = a.mg->omega;
+    lv_cells(L.nx, L.ny, [&](int i, int j) {
+        const long c = ix(i, j, L.P);
+        const float d = mg_diag(m, l, L, c);
+        if (d == 0.0f) return;
+        if (!end) {
+            const float b = L.b[c];
+            tv.bk[c] = b;
+            tv.e[c] = 0.0f;
+            tv.r[c] = b;
+            L.x[c] = w * b / d;
+        } else {
+            L.t[c] = tv.e[c];
+            L.b[c] = tv.bk[c];
+        }
+    });
+}
+
+// beta = (r.z - ro.z) / rho_old (0 at it = 0), rho = r.z, z = t
+__global__ void __launch_bounds__(256) k_tch_dots(LvArgs a, int l, TeachVec tv, int it) {
+    FcgState* fs = a.fs;
+    if (fs->done) return;
+    const MGCtx m = lv_ctx(a);
+    const MGLevelDev L = a.mg->lev[l];
+    double s[2] = {0.0, 0.0};
+    lv_cells(L.nx, L.ny, [&](int i, int j) {
+        const long c = ix(i, j, L.P);
+        if (mg_diag(m, l, L, c) == 0.0f) return;
+        const double z = L.t[c];
+        s[0] += (double)tv.r[c] * z;
+        if (it > 0) s[1] += (double)tv.ro[c] * z;
+    });
+    if (lv_reduce<2>(s, a.part, &fs->ticket[1]) && threadIdx.x == 0) {
+        fs->ibeta = it == 0 ? 0.0 : (s[0] - s[1]) / fs->irho_old;
+        fs->irho = s[0];
+    }
+}
+
+// d = z + beta d_old, q = A d, alpha = rho / d.q
+__global__ void __launch_bounds__(256) k_tch_dir(LvArgs a, int l, TeachVec tv, int it) {
+    FcgState* fs = a.fs;
+    if (fs->done) return;
+    const MGCtx m = lv_ctx(a);
+    const MGLevelDev L = a.mg->lev[l];
+    const float beta = (float)fs->ibeta;
+    float* dn = (it & 1) ? tv.d1 : tv.d0;
+    const float* dd = (it & 1) ? tv.d0 : tv.d1;
+    auto dirv = [&](int p, int q) -> float {
+        const long c = ix(p, q, L.P);
+        return it == 0 ? L.t[c] : L.t[c] + beta * dd[c];
+    };
+    double pq[1] = {0.0};
+    lv_cells(L.nx, L.ny, [&](int i, int j) {
+        const long c = ix(i, j, L.P);
+        const float d = mg_diag(m, l, L, c);
+        if (d == 0.0f) return;
+        const float dc = dirv(i, j);
+        const float q = mg_apply(m, l, L, i, j, d, dirv);
+        dn[c] = dc;
+        tv.q[c] = q;
+        pq[0] += (double)dc * (double)q;
+    });
+    if (lv_reduce<1>(pq, a.part, &fs->ticket[2]) && threadIdx.x == 0)
+        fs->ialpha = pq[0] > 0.0 ? fs->irho / pq[0] : 0.0;
+}
+
+// e += alpha d, ro = r, r -= alpha q; next V-cycle input b = r, x = w r / d
+__global__ void __launch_bounds__(256) k_tch_update(LvArgs a, int l, TeachVec tv, int it) {
+    FcgState* fs = a.fs;
+    if (fs->done) return;
+    const MGCtx m = lv_ctx(a);
+    const MGLevelDev L = a.mg->lev[l];
+    const float w = a.mg->omega;
+    const float alpha = (float)fs->ialpha;
+    const float* dn = (it & 1) ? tv.d1 : tv.d0;
+    lv_cells(L.nx, L.ny, [&](int i, int j) {
+        const long c = ix(i, j, L.P);
+        const float d = mg_diag(m, l, L, c);
+        if (d == 0.0f) return;
+        tv.e[c] += alpha * dn[c];
+        const float rv = tv.r[c];
+        tv.ro[c] = rv;
+        const float rn = rv - alpha * tv.q[c];
+        tv.r[c] = rn;
+        L.b[c] = rn;
+        L.x[c] = w * rn / d;
+    });
+    if (blockIdx.x == 0 && threadIdx.x == 0) fs->irho_old = fs->irho;
+}
+
+// store the pair (network input, (t - c) D / s) in harvest slot fs->hidx
+__global__ void __launch_bounds__(256) k_lv_harvest(LvArgs a, int l, float* hin, float* hout,
+                                                    int H, int W, float D) {
+    const FcgState* fs = a.fs;
+    if (fs->done || fs->hidx < 0) return;
+    const MGCtx m = lv_ctx(a);
+    const MGLevelDev L = a.mg->lev[l];
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long)H * W) return;
+    const int i = (int)(k / W) + 1, j = (int)(k % W) + 1;
+    float vin = 0.0f, vout = 0.0f;
+    const double s = fs->s_in;
+    if (i <= L.nx && j <= L.ny && s > 0.0) {
+        const long c = ix(i, j, L.P);
+        const float d = mg_diag(m, l, L, c);
+        if (d != 0.0f) {
+            vin = (float)(((double)L.b[c] - fs->cm * (double)lv_a1(m, l, L, i, j, d)) / s);
+            vout = (float)(((double)L.t[c] - fs->cm) * D / s);
+        }
+    }
+    const long base = fs->hidx * (long)H * W;
+    hin[base + k] = vin;
+    hout[base + k] = vout;
+}
+
+// ---- flexible CG steps on the fine level (z = lev[0].t) ----
+// beta = (r.z - r_old.z) / rho_old (0 at a (re)start), rho = r.z
+__global__ void __launch_bounds__(256) k_fcg_dots(LvArgs a, FcgVec v) {
+    FcgState* fs = a.fs;
+    if (fs->done) return;
+    const float* z = a.mg->lev[0].t;
+    const bool first = fs->first;
+    double s[2] = {0.0, 0.0};
+    lv_cells(v.nx, a.mg->lev[0].ny, [&](int i, int j) {
+        const long c = ix(i, j, v.pc);
+        if (!(a.code[c] & 63)) return;
+        const double zc = z[c];
+        s[0] += v.r[c] * zc;
+        if (!first) s[1] += v.rold[c] * zc;
+    });
+    if (lv_reduce<2>(s, a.part, &fs->ticket[1]) && threadIdx.x == 0) {
+        fs->beta = first ? 0.0 : (s[0] - s[1]) / fs->rho_old;
+        fs->rho = s[0];
+    }
+}
+
+// d = z + beta d_old, q = A d (neighbours' d rebuilt from z and d_old), alpha = rho / d.q
+__global__ void __launch_bounds__(256) k_fcg_dir(LvArgs a, FcgVec v) {
+    FcgState* fs = a.fs;
+    if (fs->done) return;
+    const float* z = a.mg->lev[0].t;
+    const int ny = a.mg->lev[0].ny;
+    const int k = fs->k;
+    const bool first = fs->first;
+    const double beta = fs->beta;
+    double* dnew = (k & 1) ? v.d1 : v.d0;
+    const double* dold = (k & 1) ? v.d0 : v.d1;
+    double pq[1] = {0.0};
+    lv_cells(v.nx, ny, [&](int i, int j) {
+        const long c = ix(i, j, v.pc);
+        const int li = a.code[c] & 63;
+        if (!li) return;
+        const int iE = (a.periodic && i == v.nx) ? 1 : i + 1;
+        const int iW = (a.periodic && i == 1) ? v.nx : i - 1;
+        auto dirv = [&](long cc) -> double {
+            const double zc = z[cc];
+            return first ? zc : zc + beta * dold[cc];
+        };
+        const double dc = dirv(c);
+        const double q = v.lut[li] * dc - v.ax * (dirv(ix(iE, j, v.pc)) + dirv(ix(iW, j, v.pc))) -
+                         v.ay * (dirv(ix(i, j + 1, v.pc)) + dirv(ix(i, j - 1, v.pc)));
+        dnew[c] = dc;
+        v.q[c] = q;
+        pq[0] += dc * q;
+    });
+    if (lv_reduce<1>(pq, a.part, &fs->ticket[2]) && threadIdx.x == 0) {
+        const double alpha = fs->rho / pq[0];
+        fs->alpha = alpha;
+        if (!(pq[0] > 0.0) || !(fabs(alpha) < 1e300)) {   // breakdown or non-finite
+            fs->done = 1;
+            fs->info = -1;
+        }
+    }
+}
+
+// x += alpha d, r_old = r, r -= alpha q; next V-cycle input b0 = r, x0 = w r / d
+__global__ void __launch_bounds__(256) k_fcg_update(LvArgs a, FcgVec v) {
+    FcgState* fs = a.fs;
+    if (fs->done) return;
+    const MGLevelDev L0 = a.mg->lev[0];
+    const float w = a.mg->omega;
+    const int k = fs->k;
+    const double alpha = fs->alpha;
+    const double* dnew = (k & 1) ? v.d1 : v.d0;
+    double s[1] = {0.0};
+    lv_cells(v.nx, L0.ny, [&](int i, int j) {
+        const long c = ix(i, j, v.pc);
+        const int li = a.code[c] & 63;
+        if (!li) return;
+        v.x[c] = v.x[c] + alpha * dnew[c];
+        const double rv = v.r[c];
+        v.rold[c] = rv;
+        const double rn = rv - alpha * v.q[c];
+        v.r[c] = rn;
+        s[0] += rn * rn;
+        const float rf = (float)rn;
+        L0.b[c] = rf;
+        L0.x[c] = w * rf / a.diag0[li];
+    });
+    if (lv_reduce<1>(s, a.part, &fs->ticket[3]) && threadIdx.x == 0) {
+        fs->rr = s[0];
+        fs->rho_old = fs->rho;
+        fs->k = k + 1;
+        fs->iters = k + 1;
+        fs->first = 0;
+        if (!(s[0] == s[0])) {
+            fs->done = 1;
+            fs->info = -1;
+        } else if (sqrt(s[0]) < fs->atol || k + 1 >= fs->maxiter) {
+            fs->pending = 1;
+        }
+    }
+}
+
+// true residual r = b - A x once the recursive one has converged (or at
+// maxiter): finish, or restart the flexible CG from it
+__global__ void __launch_bounds__(256) k_fcg_true(LvArgs a, FcgVec v) {
+    FcgState* fs = a.fs;
+    if (fs->done || !fs->pending) return;
+    const MGLevelDev L0 = a.mg->lev[0];
+    const float w = a.mg->omega;
+    double s[1] = {0.0};
+    lv_cells(v.nx, L0.ny, [&](int i, int j) {
+        const long c = ix(i, j, v.pc);
+        const int li = a.code[c] & 63;
+        if (!li) return;
+        const double rt = v.b[c] - apply_A(v.x, i, j, v.pc, v.nx, a.periodic, v.lut[li], v.ax, v.ay);
+        v.r[c] = rt;
+        s[0] += rt * rt;
+        const float rf = (float)rt;
+        L0.b[c] = rf;
+        L0.x[c] = w * rf / a.diag0[li];
+    });
+    if (lv_reduce<1>(s, a.part, &fs->ticket[0]) && threadIdx.x == 0) {
+        fs->rr = s[0];
+        fs->reps += 1;
+        fs->pending = 0;
+        if (sqrt(s[0]) < fs->atol) {
+            fs->done = 1;
+            fs->info = 0;
+        } else if (!(s[0] == s[0])) {
+            fs->done = 1;
+            fs->info = -1;
+        } else if (fs->k >= fs->maxiter) {
+            fs->done = 1;
+            fs->info = fs->maxiter;
+        } else {
+            fs->first = 1;
+        }
+    }
+}

@@ -47,6 +47,8 @@ __global__ void k_mfma_layout_check(const float* A, const float* B, float* D) {
 // 3x3 convolution, MFMA path.  Input = concat(X1 [H][W][C1], X2 [H][W][C2]),
 // C1, C2, Cout multiples of 16 (C2 may be 0).  Wt [9][Cout][C1+C2] bf16.
 // mode bit 0: add bias; bit 1: ReLU; bit 2: accumulate into Y (fp32 out).
+// The forward kernels return at once when *skip is set (a solver that has
+// converged leaves the rest of its captured iteration idle); skip may be NULL.
 // Output either bf16 (Yb) or fp32 (Yf); with fp32 output, channels >= csplit
 // go to Yf2 (channel stride Cout - csplit), the others to Yf (stride csplit):
 // the gradient of a concatenated input lands in its two parts.  One wave per
@@ -57,7 +59,8 @@ __global__ void __launch_bounds__(256) k_conv3x3_mfma(
     const unsigned short* __restrict__ X1, int C1, const unsigned short* __restrict__ X2, int C2,
     const unsigned short* __restrict__ Wt, const float* __restrict__ bias, int H, int Wd,
     unsigned short* __restrict__ Yb, float* __restrict__ Yf, float* __restrict__ Yf2, int csplit,
-    int mode) {
+    int mode, const int* skip) {
+    if (skip && *skip) return;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int tiles_w = (Wd + 15) / 16;
@@ -116,9 +119,12 @@ __global__ void __launch_bounds__(256) k_conv3x3_mfma(
     }
 }
 
-// Weight-gradient partials: dW[t][co][ci] += sum_p G[p][co] * X[p+off_t][ci]
-// over this block's pixel range (one wave per (t, co-chunk, ci-chunk) tile,
-// K = pixels in steps of 16: A = G^T (co x p), B = X shifted (p x ci)).
+// Weight-gradient partials: dW[t][co][ci] = sum_p G[p][co] * X[p+off_t][ci]
+// over the pixel chunk blockIdx.x (pix_per_block pixels).  One wave per (tap,
+// 16-channel output chunk) computes all input-channel chunks (K = pixels in
+// steps of 16: A = G^T (co x p) shared by the chunks, B = X shifted (p x ci));
+// blockIdx.y * 4 + wave selects the (tap, output chunk).  Partials of chunk b
+// at part[b][9][Cout][Cin].
 __global__ void __launch_bounds__(256) k_conv3x3_wgrad(
     const unsigned short* __restrict__ G, int Cout, const unsigned short* __restrict__ X1, int C1,
     const unsigned short* __restrict__ X2, int C2, int H, int Wd, long pix_per_block,
@@ -126,43 +132,55 @@ __global__ void __launch_bounds__(256) k_conv3x3_wgrad(
     const int lane = threadIdx.x & 63;
     const int Cin = C1 + C2;
     const int nco = Cout / 16, nci = Cin / 16;
-    const int ntiles = 9 * nco * nci;
+    const int task = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (task >= 9 * nco) return;
+    const int t = task / nco, cb = task - t * nco;
+    const int dh = t / 3 - 1, dw = t % 3 - 1;
     const long p0 = (long)blockIdx.x * pix_per_block;
     const long npix = (long)H * Wd;
     const long p1 = p0 + pix_per_block < npix ? p0 + pix_per_block : npix;
     const int m = lane % 16, kq = 4 * (lane / 16);
-    for (int tile = threadIdx.x >> 6; tile < ntiles; tile += blockDim.x >> 6) {
-        const int t = tile / (nco * nci);
-        const int rem = tile - t * nco * nci;
-        const int cb = rem / nci, ib = rem - cb * nci;
-        const int dh = t / 3 - 1, dw = t % 3 - 1;
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        for (long p = p0; p < p1; p += 16) {
-            bf16x4 a, b;
+    f32x4 acc[8];
+#pragma unroll
+    for (int ib = 0; ib < 8; ++ib) acc[ib] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (long p = p0; p < p1; p += 16) {
+        bf16x4 a;
+        long q[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long pp = p + kq + j;
+            unsigned short ga = 0;
+            q[j] = -1;
+            if (pp < p1) {
+                ga = G[pp * Cout + cb * 16 + m];
+                const int hh = (int)(pp / Wd) + dh, ww = (int)(pp % Wd) + dw;
+                if (hh >= 0 && hh < H && ww >= 0 && ww < Wd) q[j] = (long)hh * Wd + ww;
+            }
+            a[j] = (short)ga;
+        }
+#pragma unroll
+        for (int ib = 0; ib < 8; ++ib) {
+            if (ib >= nci) break;
+            const int ci = ib * 16 + m;
+            bf16x4 b;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const long pp = p + kq + j;
-                unsigned short ga = 0, xb = 0;
-                if (pp < p1) {
-                    ga = G[pp * Cout + cb * 16 + m];
-                    const int hh = (int)(pp / Wd) + dh, ww = (int)(pp % Wd) + dw;
-                    if (hh >= 0 && hh < H && ww >= 0 && ww < Wd) {
-                        const long q = (long)hh * Wd + ww;
-                        const int ci = ib * 16 + m;
-                        xb = ci < C1 ? X1[q * C1 + ci] : X2[q * C2 + ci - C1];
-                    }
-                }
-                a[j] = (short)ga;
+                unsigned short xb = 0;
+                if (q[j] >= 0) xb = ci < C1 ? X1[q[j] * C1 + ci] : X2[q[j] * C2 + ci - C1];
                 b[j] = (short)xb;
             }
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+            acc[ib] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[ib], 0, 0, 0);
         }
-        // D rows = co (4*(lane/16)+j within the chunk), cols = ci (lane%16)
-        float* dst = part + (long)blockIdx.x * 9 * Cout * Cin;
+    }
+    // D rows = co (4*(lane/16)+j within the chunk), cols = ci (lane%16)
+    float* dst = part + (long)blockIdx.x * 9 * Cout * Cin;
+#pragma unroll
+    for (int ib = 0; ib < 8; ++ib) {
+        if (ib >= nci) break;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int co = cb * 16 + kq + j, ci = ib * 16 + m;
-            dst[((long)t * Cout + co) * Cin + ci] = acc[j];
+            dst[((long)t * Cout + co) * Cin + ci] = acc[ib][j];
         }
     }
 }
@@ -201,7 +219,8 @@ __global__ void k_relu_mask(const float* __restrict__ dY, const unsigned short* 
 
 // 2x2 average pooling, C channels: Y[h][w] = mean of X[2h..2h+1][2w..2w+1]
 __global__ void k_avgpool2(const unsigned short* __restrict__ X, int H, int Wd, int C,
-                           unsigned short* __restrict__ Y) {
+                           unsigned short* __restrict__ Y, const int* skip) {
+    if (skip && *skip) return;
     const int H2 = H / 2, W2 = Wd / 2;
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (long)H2 * W2 * C) return;
@@ -228,7 +247,8 @@ __global__ void k_avgpool2_bwd(const float* __restrict__ dY, int H, int Wd, int 
 
 // nearest upsampling x2: Y[h][w] = X[h/2][w/2]
 __global__ void k_upsample2(const unsigned short* __restrict__ X, int H, int Wd, int C,
-                            unsigned short* __restrict__ Y) {
+                            unsigned short* __restrict__ Y, const int* skip) {
+    if (skip && *skip) return;
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (long)H * Wd * C) return;
     const int c = (int)(k % C);
@@ -252,7 +272,9 @@ __global__ void k_upsample2_bwd(const float* __restrict__ dY, int H, int Wd, int
 
 // first layer (1 -> C channels), scalar: Y = relu(b + sum_t W[t][c] x[p+off_t])
 __global__ void k_conv_in(const float* __restrict__ x, int H, int Wd, const float* __restrict__ W,
-                          const float* __restrict__ b, int C, unsigned short* __restrict__ Y) {
+                          const float* __restrict__ b, int C, unsigned short* __restrict__ Y,
+                          const int* skip) {
+    if (skip && *skip) return;
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (long)H * Wd * C) return;
     const int c = (int)(k % C);
@@ -269,7 +291,8 @@ __global__ void k_conv_in(const float* __restrict__ x, int H, int Wd, const floa
 // last layer (C -> 1), scalar, linear: y = b + sum_t sum_c W[t][c] X[p+off_t][c]
 __global__ void k_conv_out(const unsigned short* __restrict__ X, int H, int Wd, int C,
                            const float* __restrict__ W, const float* __restrict__ b,
-                           float* __restrict__ y) {
+                           float* __restrict__ y, const int* skip) {
+    if (skip && *skip) return;
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (long)H * Wd) return;
     const int h = (int)(p / Wd), w = (int)(p % Wd);
@@ -385,6 +408,39 @@ __global__ void k_mse(const float* __restrict__ y, const float* __restrict__ t,
         const float d = m * (y[k] - t[k]);
         dy[k] = 2.f * d * inv_norm;
         l = d * d * inv_norm;
+    }
+    s_acc[threadIdx.x] = l;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s_acc[threadIdx.x] += s_acc[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s_acc[0];
+}
+
+// energy-norm error and its gradient: e = y - t on the valid nx x ny corner
+// of the H x W image, A = the 5-point Laplacian with zero-flux sides (unit
+// couplings) plus eps I;  loss += e.A e / norm, dy = 2 A e / norm (0 outside)
+__global__ void k_energy(const float* __restrict__ y, const float* __restrict__ t, int H, int W,
+                         int nxv, int nyv, float eps, float inv_norm, float* __restrict__ dy,
+                         float* __restrict__ part) {
+    __shared__ float s_acc[256];
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.f;
+    if (k < (long)H * W) {
+        const int h = (int)(k / W), w = (int)(k % W);
+        float g = 0.f;
+        if (h < nxv && w < nyv) {
+            const float e = y[k] - t[k];
+            float ae = eps * e;
+            if (h > 0) ae += e - (y[k - W] - t[k - W]);
+            if (h < nxv - 1) ae += e - (y[k + W] - t[k + W]);
+            if (w > 0) ae += e - (y[k - 1] - t[k - 1]);
+            if (w < nyv - 1) ae += e - (y[k + 1] - t[k + 1]);
+            l = e * ae * inv_norm;
+            g = 2.f * ae * inv_norm;
+        }
+        dy[k] = g;
     }
     s_acc[threadIdx.x] = l;
     __syncthreads();

@@ -44,6 +44,10 @@ static const LayerIO LIO[NL] = {
     {A2A, -1, A2B}, {P3, -1, A3A}, {A3A, -1, A3B}, {U2, A2B, B2A}, {B2A, -1, B2B},
     {U1, A1B, B1A}, {B1A, -1, B1B}, {U0, A0B, B0A}, {B0A, -1, B0B}, {B0B, -1, -1}};
 
+// pixels per work-group in the weight-gradient partial sums
+constexpr long WG_PPB = 256;    // matrix-core layers
+constexpr long SC_PPB = 1024;   // scalar in/out layers and biases
+
 inline long param_count() {
     long n = 0;
     for (int l = 0; l < NL; ++l) n += 9L * (LDEF[l][1] + LDEF[l][2]) * LDEF[l][3] + LDEF[l][3];
@@ -70,7 +74,11 @@ struct cfd_unet {
     float* part = nullptr;
     long part_n = 0;
     int steps = 0;
+    int loss_kind = 0;               // 0 masked squared error, 1 energy norm
+    int loss_nx = 0, loss_ny = 0;
+    float loss_eps = 0.f;
     hipStream_t stream = nullptr;
+    const int* skip = nullptr;       // forward kernels return at once while *skip != 0
     std::vector<void*> allocs;
 };
 
@@ -90,33 +98,34 @@ inline unsigned nblk(long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs
 
 int conv_mfma(cfd_unet* u, const unsigned short* x1, int c1, const unsigned short* x2, int c2,
               const unsigned short* wt, const float* bias, int H, int W, int cout,
-              unsigned short* yb, float* yf, float* yf2, int csplit, int mode) {
+              unsigned short* yb, float* yf, float* yf2, int csplit, int mode,
+              const int* skip) {
     const long tiles = (long)H * ((W + 15) / 16);
     const dim3 grid((unsigned)((tiles + 3) / 4));
     switch (cout) {
     case 16:
         hipLaunchKernelGGL(k_conv3x3_mfma<1>, grid, dim3(256), 0, u->stream, x1, c1, x2, c2, wt,
-                           bias, H, W, yb, yf, yf2, csplit, mode);
+                           bias, H, W, yb, yf, yf2, csplit, mode, skip);
         break;
     case 32:
         hipLaunchKernelGGL(k_conv3x3_mfma<2>, grid, dim3(256), 0, u->stream, x1, c1, x2, c2, wt,
-                           bias, H, W, yb, yf, yf2, csplit, mode);
+                           bias, H, W, yb, yf, yf2, csplit, mode, skip);
         break;
     case 48:
         hipLaunchKernelGGL(k_conv3x3_mfma<3>, grid, dim3(256), 0, u->stream, x1, c1, x2, c2, wt,
-                           bias, H, W, yb, yf, yf2, csplit, mode);
+                           bias, H, W, yb, yf, yf2, csplit, mode, skip);
         break;
     case 64:
         hipLaunchKernelGGL(k_conv3x3_mfma<4>, grid, dim3(256), 0, u->stream, x1, c1, x2, c2, wt,
-                           bias, H, W, yb, yf, yf2, csplit, mode);
+                           bias, H, W, yb, yf, yf2, csplit, mode, skip);
         break;
     case 96:
         hipLaunchKernelGGL(k_conv3x3_mfma<6>, grid, dim3(256), 0, u->stream, x1, c1, x2, c2, wt,
-                           bias, H, W, yb, yf, yf2, csplit, mode);
+                           bias, H, W, yb, yf, yf2, csplit, mode, skip);
         break;
     case 128:
         hipLaunchKernelGGL(k_conv3x3_mfma<8>, grid, dim3(256), 0, u->stream, x1, c1, x2, c2, wt,
-                           bias, H, W, yb, yf, yf2, csplit, mode);
+                           bias, H, W, yb, yf, yf2, csplit, mode, skip);
         break;
     default:
         return set_err(CFD_ERR_ARG, "unsupported channel count %d", cout);
@@ -139,27 +148,28 @@ int unet_forward(cfd_unet* u) {
             const int lf = ACT_L[src];
             const long n = (long)H * W * ACT_C[src];
             hipLaunchKernelGGL(k_avgpool2, dim3(nblk(n)), dim3(256), 0, u->stream, u->act[src],
-                               u->hs[lf], u->ws[lf], ACT_C[src], u->act[dst]);
+                               u->hs[lf], u->ws[lf], ACT_C[src], u->act[dst], u->skip);
         }
         if (l == 8 || l == 10 || l == 12) {
             const int src = l == 8 ? A3B : (l == 10 ? B2B : B1B);
             const int dst = l == 8 ? U2 : (l == 10 ? U1 : U0);
             const long n = (long)H * W * ACT_C[src];
             hipLaunchKernelGGL(k_upsample2, dim3(nblk(n)), dim3(256), 0, u->stream, u->act[src],
-                               H, W, ACT_C[src], u->act[dst]);
+                               H, W, ACT_C[src], u->act[dst], u->skip);
         }
         if (Ly.type == L_IN) {
             const long n = (long)H * W * Ly.cout;
             hipLaunchKernelGGL(k_conv_in, dim3(nblk(n)), dim3(256), 0, u->stream, u->xin, H, W,
-                               u->w + Ly.woff, u->w + Ly.boff, Ly.cout, u->act[LIO[l].y]);
+                               u->w + Ly.woff, u->w + Ly.boff, Ly.cout, u->act[LIO[l].y],
+                               u->skip);
         } else if (Ly.type == L_OUT) {
             hipLaunchKernelGGL(k_conv_out, dim3(nblk((long)H * W)), dim3(256), 0, u->stream,
                                u->act[LIO[l].x1], H, W, Ly.cin1, u->w + Ly.woff, u->w + Ly.boff,
-                               u->y);
+                               u->y, u->skip);
         } else {
             TRY(conv_mfma(u, u->act[LIO[l].x1], Ly.cin1, act_in(u, LIO[l].x2), Ly.cin2,
                           u->wb + Ly.woff, u->w + Ly.boff, H, W, Ly.cout, u->act[LIO[l].y],
-                          nullptr, nullptr, Ly.cout, 3));
+                          nullptr, nullptr, Ly.cout, 3, u->skip));
         }
         HIPC(hipGetLastError());
     }
@@ -185,7 +195,7 @@ int unet_backward(cfd_unet* u) {
         const Layer& Ly = u->L[l];
         const int H = u->hs[Ly.level], W = u->ws[Ly.level];
         const long npix = (long)H * W;
-        const long ppb = 4096;
+        const long ppb = unet::SC_PPB;
         const int nb = (int)((npix + ppb - 1) / ppb);
         if (Ly.type == L_OUT) {
             hipLaunchKernelGGL(k_scalar_layer_wgrad, dim3(nb), dim3(256), 0, u->stream, 1,
@@ -218,12 +228,14 @@ int unet_backward(cfd_unet* u) {
             continue;
         }
         const int cin = Ly.cin1 + Ly.cin2;
-        // weight gradient
-        hipLaunchKernelGGL(k_conv3x3_wgrad, dim3(nb), dim3(256), 0, u->stream, u->gbuf, Ly.cout,
-                           u->act[LIO[l].x1], Ly.cin1, act_in(u, LIO[l].x2), Ly.cin2, H, W, ppb,
-                           u->part);
+        // weight gradient: pixel chunks x (tap, output chunk) groups of 4 waves
+        const int nbw = (int)((npix + unet::WG_PPB - 1) / unet::WG_PPB);
+        const dim3 gw(nbw, (9 * (Ly.cout / 16) + 3) / 4);
+        hipLaunchKernelGGL(k_conv3x3_wgrad, gw, dim3(256), 0, u->stream, u->gbuf, Ly.cout,
+                           u->act[LIO[l].x1], Ly.cin1, act_in(u, LIO[l].x2), Ly.cin2, H, W,
+                           (long)unet::WG_PPB, u->part);
         HIPC(hipGetLastError());
-        TRY(sum_into(u, nb, 9L * Ly.cout * cin, 0, 9L * Ly.cout * cin, Ly.woff));
+        TRY(sum_into(u, nbw, 9L * Ly.cout * cin, 0, 9L * Ly.cout * cin, Ly.woff));
         hipLaunchKernelGGL(k_bias_grad, dim3(nb), dim3(256), 0, u->stream, u->gbuf, npix,
                            Ly.cout, ppb, u->part);
         HIPC(hipGetLastError());
@@ -236,7 +248,7 @@ int unet_backward(cfd_unet* u) {
         float* d1 = u->dact[LIO[l].x1];
         float* d2 = LIO[l].x2 >= 0 ? u->dact[LIO[l].x2] : nullptr;
         TRY(conv_mfma(u, u->gbuf, Ly.cout, nullptr, 0, u->wflip, nullptr, H, W, cin, nullptr, d1,
-                      d2, Ly.cin1, 4));
+                      d2, Ly.cin1, 4, nullptr));
         // resampling that fed this layer, backwards
         if (l == 2 || l == 4 || l == 6) {
             const int src = l == 2 ? A0B : (l == 4 ? A1B : A2B);
@@ -302,7 +314,17 @@ int cfd_unet_create(int32_t H, int32_t W, cfd_unet** out) {
         maxact = std::max(maxact, n);
         if ((rc = ualloc(u, &u->act[a], n)) || (rc = ualloc(u, &u->dact[a], n))) return fail(rc);
     }
-    u->part_n = ((n0 + 4095) / 4096) * 9L * 128 * 64 + 1024;
+    // partial-sum buffer: the largest of the per-layer needs
+    long pn = (n0 + 255) / 256 + 1024;
+    for (int l = 0; l < unet::NL; ++l) {
+        const unet::Layer& Ly = u->L[l];
+        const long np = (long)u->hs[Ly.level] * u->ws[Ly.level];
+        const long cw = (np + unet::WG_PPB - 1) / unet::WG_PPB;
+        const long cs = (np + unet::SC_PPB - 1) / unet::SC_PPB;
+        pn = std::max(pn, cw * 9L * Ly.cout * (Ly.cin1 + Ly.cin2));
+        pn = std::max(pn, cs * 10L * std::max(Ly.cout, Ly.cin1));
+    }
+    u->part_n = pn;
     if ((rc = ualloc(u, &u->w, u->npar)) || (rc = ualloc(u, &u->wb, u->npar)) ||
         (rc = ualloc(u, &u->wflip, 9L * 128 * 64)) || (rc = ualloc(u, &u->g, u->npar)) ||
         (rc = ualloc(u, &u->m1, u->npar)) || (rc = ualloc(u, &u->m2, u->npar)) ||
@@ -376,8 +398,13 @@ int cfd_unet_loss_grad(cfd_unet* u, const float* x_dev, const float* t_dev, cons
     HIPC(hipMemcpyAsync(u->xin, x_dev, n * sizeof(float), hipMemcpyDeviceToDevice, u->stream));
     TRY(unet_forward(u));
     const unsigned nb = nblk(n);
-    hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, u->stream, u->y, t_dev, mask_dev, n,
-                       (float)(1.0 / tnorm2), u->dy, u->part);
+    if (u->loss_kind == 1)
+        hipLaunchKernelGGL(k_energy, dim3(nb), dim3(256), 0, u->stream, u->y, t_dev, u->H, u->W,
+                           u->loss_nx, u->loss_ny, u->loss_eps, (float)(1.0 / tnorm2), u->dy,
+                           u->part);
+    else
+        hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, u->stream, u->y, t_dev, mask_dev, n,
+                           (float)(1.0 / tnorm2), u->dy, u->part);
     HIPC(hipGetLastError());
     std::vector<float> lp(nb);
     HIPC(hipMemcpyAsync(lp.data(), u->part, nb * sizeof(float), hipMemcpyDeviceToHost, u->stream));
@@ -386,6 +413,18 @@ int cfd_unet_loss_grad(cfd_unet* u, const float* x_dev, const float* t_dev, cons
     for (float v : lp) s += v;
     *loss = s;
     TRY(unet_backward(u));
+    return 0;
+}
+
+int cfd_unet_set_loss(cfd_unet* u, int32_t kind, int32_t nx, int32_t ny, float eps) {
+    if (!u || kind < 0 || kind > 1) return set_err(CFD_ERR_ARG, "bad loss kind");
+    if (kind == 1 && (nx < 1 || ny < 1 || nx > u->H || ny > u->W || !(eps >= 0.f)))
+        return set_err(CFD_ERR_ARG, "energy loss region %d x %d in a %d x %d image", nx, ny, u->H,
+                       u->W);
+    u->loss_kind = kind;
+    u->loss_nx = nx;
+    u->loss_ny = ny;
+    u->loss_eps = eps;
     return 0;
 }
 

@@ -47,7 +47,7 @@ enum { CFD_FIELD_U = 0, CFD_FIELD_V = 1, CFD_FIELD_P = 2, CFD_FIELD_F = 3, CFD_F
    and flexible CG with FP64 vectors and dot products preconditioned by an FP32
    aggregation-multigrid V-cycle (set up with cfd_set_multigrid), the solve
    ending on the FP64 true residual */
-enum { CFD_SOLVER_PCG64 = 0, CFD_SOLVER_MGCG = 1 };
+enum { CFD_SOLVER_PCG64 = 0, CFD_SOLVER_MGCG = 1, CFD_SOLVER_LEARNED = 2 };
 
 typedef struct cfd_ctx cfd_ctx;
 
@@ -225,6 +225,46 @@ int cfd_scalar_transport(const double* phi, const double* u, const double* v,
                          const int64_t* bt_t, const double* bv_t, int32_t obs_t, double obs_v,
                          double* out, void* stream);
 
+/* ---- learned flexible CG (pressure solver CFD_SOLVER_LEARNED; no counterpart
+   in the reference, whose solvers are splu and Jacobi-PCG, pressure.py:136-139).
+   FP64 flexible CG, ending on the FP64 true residual, preconditioned by an FP32
+   V-cycle over the multigrid hierarchy (cfd_set_multigrid) that stops at
+   `level`, where the U-Net supplies the coarse correction after an exact
+   correction in the constant vector.  Modes: 0 network, 1 no network (the
+   constant-vector correction alone), 2 teacher (level solved by
+   `teacher_cycles` iterations of flexible CG preconditioned by V-cycles down
+   to the dense coarsest level; with harvest
+   buffers, each iteration stores the pair (network input, normalised
+   correction) as two H x W float images, H and W the level's sizes rounded up
+   to multiples of 8).  A solve that does not converge within max_iter
+   iterations is continued by the FP64 Jacobi-PCG when `fallback` is set.
+   The context keeps `net` (and the harvest buffers) until cfd_set_learned is
+   called again or the context is destroyed: the caller keeps them alive. ---- */
+typedef struct cfd_unet cfd_unet;
+typedef struct cfd_learned_desc {
+    cfd_unet* net;
+    int32_t level;
+    int32_t mode;
+    int32_t teacher_cycles;
+    int32_t max_iter;
+    int32_t fallback;
+    int32_t use_graph;        /* replay each iteration as a captured HIP graph */
+    float sigma;              /* network output scale */
+    double one_a_one;         /* 1^T A_level 1 (0: no constant-vector correction) */
+    float* harvest_in;        /* device, harvest_cap x H x W floats (mode 2) */
+    float* harvest_out;
+    int64_t harvest_cap;
+} cfd_learned_desc;
+typedef struct cfd_learned_stats_t {
+    int64_t harvested, solves, iterations, fallbacks;
+    int32_t last_iters, last_info;
+    double last_relres;
+} cfd_learned_stats_t;
+int cfd_set_learned(cfd_ctx* ctx, const cfd_learned_desc* desc);
+int cfd_learned_stats(cfd_ctx* ctx, cfd_learned_stats_t* out);
+/* zero the harvest counter and the solve statistics */
+int cfd_learned_reset(cfd_ctx* ctx);
+
 /* ---- learned preconditioner network (no counterpart in the reference: the
    U-Net of the learned flexible PCG, trained in-process on residual ->
    correction pairs harvested from the pressure solves) ----
@@ -233,7 +273,6 @@ int cfd_scalar_transport(const double* phi, const double* u, const double* v,
    vector, layer by layer: weights [tap][Cout][Cin], then biases), BF16 weights
    and activations on the matrix cores with FP32 accumulation.  Images are
    H x W fp32 on the device, H and W multiples of 8. */
-typedef struct cfd_unet cfd_unet;
 int64_t cfd_unet_param_count(void);
 int cfd_unet_create(int32_t H, int32_t W, cfd_unet** out);
 int cfd_unet_destroy(cfd_unet* net);
@@ -248,6 +287,11 @@ int cfd_unet_forward(cfd_unet* net, const float* x_dev, float* y_dev);
    the loss is added to the network's gradient vector */
 int cfd_unet_loss_grad(cfd_unet* net, const float* x_dev, const float* t_dev,
                        const float* mask_dev, double tnorm2, double* loss);
+/* loss of cfd_unet_loss_grad: kind 0 the masked squared error above; kind 1
+   the energy norm (y - t).A(y - t) / tnorm2 on the nx x ny corner of the
+   image, A = 5-point Laplacian with zero-flux sides and unit couplings + eps I
+   (mask unused) */
+int cfd_unet_set_loss(cfd_unet* net, int32_t kind, int32_t nx, int32_t ny, float eps);
 /* one Adam step (beta 0.9/0.999) on the gradient times `scale`, then zero it */
 int cfd_unet_adam(cfd_unet* net, float lr, float scale);
 /* D = A B for 16 x 16 row-major matrices, through one bf16 matrix-core
