@@ -4,17 +4,22 @@ Workload (one GPU): lid-driven cavity Re=1000 on 1024x1024, AB2, gamma "auto",
 tau 0.5, with a passive van Leer scalar (D = 1e-3) initialised as 8 Gaussian
 blobs (seed 0).  Every step is the whole projection step (BCs, dt/gamma,
 predictor, pressure solve to relative residual 1e-6, correction, scalar
-transport, monitors), all inputs resident in HBM.  Two solver legs are timed:
-plain FP64 Jacobi-PCG, and the FP32-bulk / FP64-dot mixed-precision PCG
-(the headline ``value``).
+transport, monitors), all inputs resident in HBM.  Solver legs:
 
-Prints one JSON line.  ``roofline`` describes the persistent projection
-kernel (the dominant kernel): algorithmic bytes per launch
-(8 B x N x (3 + 7 + 10 n_it) in FP64, SURVEY-style count: RHS, correction and
-10 vector passes per CG iteration; 4 B per vector element for the FP32 bulk
-of the mixed solver) over its mean launch time, timed with HIP events on the
-launch stream.  ``cpu_baseline`` times the CPU FP64 oracle (SciPy CG) on one
-host core for a bounded number of steps of the same workload.
+* ``cg``               plain FP64 Jacobi-PCG (the reference's method, the baseline);
+* ``cg_learned``       the learned flexible CG (FP64 outer, FP32 V-cycle, BF16
+                       U-Net coarse correction; the shipped trained weights): the
+                       headline ``value``;
+* ``cg_learned_none``  the same solver without the network (the gain it brings);
+* ``cg_mixed``         FP64 flexible CG with the FP32 multigrid V-cycle.
+
+Prints one JSON line.  ``roofline`` describes the pressure projection of the
+headline leg (it takes nearly all of the step): algorithmic bytes per
+projection (per cell: 80 B for the right-hand side, pressure write-back and
+correction, plus per iteration the FP64 outer vectors and the FP32 V-cycle,
+counted per kernel below) over its mean duration, timed with HIP events on
+the launch stream.  ``cpu_baseline`` times the CPU FP64 oracle (SciPy CG) on
+one host core for a bounded number of steps of the same workload.
 """
 
 from __future__ import annotations
@@ -59,11 +64,17 @@ def gaussian_blobs(n: int, seed: int = 0, count: int = 8) -> np.ndarray:
     return phi
 
 
-def build_case(solver: str, n: int = N):
+LEGS = {"cg": ("cg", None), "cg_mixed": ("cg_mixed", None),
+        "cg_learned": ("cg_learned", {"mode": "net"}),
+        "cg_learned_none": ("cg_learned", {"mode": "none"})}
+
+
+def build_case(leg: str, n: int = N):
     from cfd2d_amd import cases
 
+    solver, learned = LEGS[leg]
     case = cases.lid_driven_cavity(re=RE, n=n, pressure_solver=solver, pressure_tol=RTOL,
-                                   pressure_maxiter=50000)
+                                   pressure_maxiter=50000, learned=learned)
     cases.add_scalar(case, "c", diffusivity=1e-3, limiter="vanleer", initial=gaussian_blobs(n))
     return case
 
@@ -114,7 +125,15 @@ def roofline_for(leg: dict) -> dict:
     n_cells = N * N
     s = 8.0
     n_it = leg["cg_iters_mean"]
-    if leg["solver"] == "cg_mixed":
+    if leg["solver"].startswith("cg_learned"):
+        # per learned-FCG iteration and fine cell: V-cycle sweeps on level 0
+        # (down sweep 13 B, restriction 11 B, prolongation sweep 14 B, last sweep
+        # with the r.z and r_old.z partials 29 B), FP64 direction + A d (29 B),
+        # FP64 update of x, r, r_old plus the next V-cycle input (73 B); coarse
+        # levels add a third of the level-0 float traffic with their
+        # coefficients (21 B); the U-Net on the 32x32 level is negligible
+        bytes_per_launch = n_cells * (8.0 * (3 + 7) + n_it * (169.0 + 21.0))
+    elif leg["solver"] == "cg_mixed":
         # per flexible-CG iteration: FP64 outer vectors (r, r_old, x, p, q: 104 B/cell)
         # plus the FP32 V-cycle (fine level 50 B/cell, coarse levels 1/3 more
         # with their 12 B/cell of coefficients: ~66 B/cell)
@@ -126,12 +145,16 @@ def roofline_for(leg: dict) -> dict:
     # measured HBM traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated against a
     # 256 MiB copy) was 1.10x the algorithmic bytes for the multigrid launches
     traffic = round(TRAFFIC_RATIO_MGCG * bytes_per_launch) if leg["solver"] == "cg_mixed" else None
+    learned = leg["solver"].startswith("cg_learned")
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE of 6-iteration launches "
                               "(ratio 1.10 to algorithmic), profiles/k_project_traffic_cavity1024.md"
             if traffic else None,
-            "kernel": "k_project", "bytes_per_launch": bytes_per_launch,
+            "kernel": ("learned projection: k_project (right-hand side; write-back and "
+                       "correction) + the flexible-CG iteration kernels" if learned
+                       else "k_project"),
+            "bytes_per_launch": bytes_per_launch,
             "launch_ms": round(leg["proj_ms_mean"], 4), "cg_iters_per_launch": round(n_it, 2)}
 
 
@@ -199,7 +222,7 @@ def main() -> None:
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="cg,cg_mixed")
+    ap.add_argument("--legs", default="cg,cg_learned,cg_learned_none,cg_mixed")
     ap.add_argument("--cpu-child", nargs=2, default=None)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device-to-device (a known-size kernel for "
@@ -222,10 +245,16 @@ def main() -> None:
         except ValueError as e:
             legs[solver] = {"solver": solver, "error": str(e)}
         print(f"leg {solver}: {json.dumps(legs[solver])}", file=sys.stderr, flush=True)
-    ok = [l for l in legs.values() if "error" not in l]
-    head = legs.get("cg_mixed") if "error" not in legs.get("cg_mixed", {"error": 1}) else None
-    head = head or legs.get("cg")
-    base = legs.get("cg")
+    def good(name):
+        leg = legs.get(name)
+        return leg if leg is not None and "error" not in leg else None
+
+    head = good("cg_learned") or good("cg_mixed") or good("cg")
+    if head is None:
+        print("no solver leg ran", file=sys.stderr)
+        sys.exit(1)
+    base = good("cg")
+    none = good("cg_learned_none")
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "MAC steps/s @1024² cavity: learned-PCG fp32 vs plain fp64 CG "
@@ -239,7 +268,7 @@ def main() -> None:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if head["solver"] == "cg_mixed" else "f64",
+        "dtype": {"cg_learned": "f64+f32+bf16", "cg_mixed": "f64+f32"}.get(head["solver"], "f64"),
         "data": "synthetic",
         "config": {"workload": WORKLOAD, "grid": [N, N], "solver": head["solver"],
                    "rtol": RTOL},
@@ -249,6 +278,10 @@ def main() -> None:
         else None,
         "speedup_vs_fp64_cg": (round(head["steps_per_s"] / base["steps_per_s"], 3)
                                if base and "error" not in base else None),
+        "network_gain": ({"iters_without": round(none["cg_iters_mean"], 2),
+                          "iters_with": round(head["cg_iters_mean"], 2),
+                          "steps_per_s_without": round(none["steps_per_s"], 3)}
+                         if none and head["solver"] == "cg_learned" else None),
         "legs": legs,
     }
     print(json.dumps(line))

@@ -29,7 +29,7 @@ import numpy as np
 from . import _lib
 from .unet import UNet, init_params, param_count
 
-TARGET_SIDE = 64
+TARGET_SIDE = 32
 WEIGHTS = Path(__file__).resolve().parent / "weights" / "unet_pressure.npz"
 
 
@@ -283,3 +283,39 @@ def harvest_from_cases(cases_spec, steps: int, cap_per_case: int, every: int = 1
     if len(shapes) != 1:
         raise ValueError(f"cases give different coarse image sizes {shapes}")
     return np.concatenate(xs), np.concatenate(ts)
+
+
+# the training set and recipe of the shipped network
+DEFAULT_CASES = ([(n, re) for n in (256, 512, 1024) for re in (100.0, 400.0, 1000.0, 3200.0, 5000.0)]
+                 + [(2048, 1000.0), (2048, 5000.0)])
+
+
+def train_default(out=None, steps: int = 150, seconds: float = 150.0, cases_list=None,
+                  side: int = TARGET_SIDE, seed: int = 0, log=None) -> Path:
+    """Harvest pairs from lid-driven cavities (sizes 256-2048, Re 100-5000,
+    pressure_tol 1e-6, `steps` steps each, teacher mode) and train the network
+    with the squared-error loss (lr 1e-3, batches of 8, augmentation by the
+    grid symmetries) for about `seconds`; writes the weights to `out`."""
+    from . import cases as C
+
+    xs, ts = [], []
+    for n, re in (cases_list or DEFAULT_CASES):
+        cfg = LearnedConfig(mode="teacher", harvest_cap=4 * steps + 64, side=side,
+                            teacher_cycles=10)
+        sim = C.lid_driven_cavity(re=re, n=n, pressure_solver="cg_learned", pressure_tol=1e-6,
+                                  learned=cfg).sim
+        for _ in range(steps):
+            sim.step()
+        sim._flush()
+        x, t = sim.pressure.learned.harvest()
+        xs.append(x)
+        ts.append(t)
+        if log:
+            log(f"cavity n={n} Re={re:g}: {x.shape[0]} pairs")
+        sim.close()
+    x, t = np.concatenate(xs), np.concatenate(ts)
+    params, sigma, hist = train(x, t, epochs=1000, batch=8, lr=1e-3, per_epoch=2000, seed=seed,
+                                loss="mse", log=log, time_limit=seconds)
+    meta = {"pairs": int(x.shape[0]), "image": list(x.shape[1:]), "epochs": len(hist),
+            "final_loss": float(hist[-1]), "side": side, "steps": steps}
+    return save_weights(out or WEIGHTS, params, sigma, meta)

@@ -139,6 +139,7 @@ class Simulation:
         self.history = History()
         self.wall_time = 0.0
         self.cg_iterations: list[int] = []
+        self.cg_info: list[int] = []      # per step: 0 converged, >0 iterations without, <0 breakdown
         self.cg_relres: list[float] = []
         self._recorded = 0
         self._since_sync = 0
@@ -330,6 +331,7 @@ class Simulation:
         h.divergence.append(float(r.divergence))
         h.kinetic_energy.append(float(r.kinetic_energy))
         self.cg_iterations.append(int(r.cg_iters))
+        self.cg_info.append(int(r.cg_info))
         self.cg_relres.append(float(r.cg_relres))
         if self.compute_forces:
             fx, fy = self._forces_from(r)

@@ -390,15 +390,18 @@ int learned_iteration(cfd_ctx* c) {
     const FcgVec v = fcg_vec(c);
     const MGDev& mg = c->mg_host;
     const int K = c->l_level, NL = mg.nlev;
-    auto blocks = [&](int l) { return (unsigned)std::min(mg.lev[l].nx, 1024); };
+    auto blocks = [&](int l) { return std::min(mg.lev[l].nx, 512); };
+    const int nb0 = blocks(0), nbK = blocks(K);
     const unsigned nimg = (unsigned)(((long)c->l_H * c->l_W + 255) / 256);
     for (int l = 0; l < K; ++l) {
         LV_LAUNCH(k_lv_smooth, blocks(l), a, l, 0);
         LV_LAUNCH(k_lv_restrict, blocks(l + 1), a, l);
     }
+    LV_LAUNCH(k_lv_cmode, nbK, a, K);
+    LV_LAUNCH(k_lv_cmode_fin, 1, a, nbK, c->l_1a1,
+              c->l_mode == LEARNED_TEACHER ? c->l_hcap : 0LL);
     if (c->l_mode != LEARNED_TEACHER) {
         const bool net = c->l_mode == LEARNED_NET;
-        LV_LAUNCH(k_lv_cmode, blocks(K), a, K, c->l_1a1, 0LL);
         if (net) {
             LV_LAUNCH(k_lv_net_in, nimg, a, K, c->l_net->xin, c->l_H, c->l_W);
             // the network runs on this context's stream and idles once the solve is done
@@ -412,14 +415,13 @@ int learned_iteration(cfd_ctx* c) {
             u->skip = k0;
             TRY(rc);
         }
-        LV_LAUNCH(k_lv_net_out, blocks(K), a, K, net ? (const float*)c->l_net->y : nullptr, c->l_W,
+        LV_LAUNCH(k_lv_net_out, nbK, a, K, net ? (const float*)c->l_net->y : nullptr, c->l_W,
                   c->l_sigma / c->l_D, net ? 1 : 0);
-        LV_LAUNCH(k_lv_smooth, blocks(K), a, K, 1);
-        LV_LAUNCH(k_lv_smooth, blocks(K), a, K, 0);
+        LV_LAUNCH(k_lv_smooth, nbK, a, K, 1);
+        LV_LAUNCH(k_lv_smooth, nbK, a, K, 0);
     } else {
         const TeachVec tv = c->l_tv;
-        LV_LAUNCH(k_lv_cmode, blocks(K), a, K, c->l_1a1, c->l_hcap);
-        LV_LAUNCH(k_tch_edge, blocks(K), a, K, tv, 0);
+        LV_LAUNCH(k_tch_edge, nbK, a, K, tv, 0);
         for (int it = 0; it < c->l_cycles; ++it) {
             for (int l = K; l < NL - 1; ++l) {
                 LV_LAUNCH(k_lv_smooth, blocks(l), a, l, 0);
@@ -430,22 +432,29 @@ int learned_iteration(cfd_ctx* c) {
                 LV_LAUNCH(k_lv_post, blocks(l), a, l);
                 LV_LAUNCH(k_lv_smooth, blocks(l), a, l, 0);
             }
-            LV_LAUNCH(k_tch_dots, blocks(K), a, K, tv, it);
-            LV_LAUNCH(k_tch_dir, blocks(K), a, K, tv, it);
-            LV_LAUNCH(k_tch_update, blocks(K), a, K, tv, it);
+            LV_LAUNCH(k_tch_dots, nbK, a, K, tv, it);
+            LV_LAUNCH(k_tch_dir, nbK, a, K, tv, it);
+            LV_LAUNCH(k_tch_update, nbK, a, K, tv, it);
         }
-        LV_LAUNCH(k_tch_edge, blocks(K), a, K, tv, 1);
+        LV_LAUNCH(k_tch_edge, nbK, a, K, tv, 1);
         if (c->l_hcap > 0)
             LV_LAUNCH(k_lv_harvest, nimg, a, K, c->l_hin, c->l_hout, c->l_H, c->l_W, c->l_D);
     }
-    for (int l = K - 1; l >= 0; --l) {
+    for (int l = K - 1; l >= 1; --l) {
         LV_LAUNCH(k_lv_post, blocks(l), a, l);
         LV_LAUNCH(k_lv_smooth, blocks(l), a, l, 0);
     }
-    LV_LAUNCH(k_fcg_dots, blocks(0), a, v);
-    LV_LAUNCH(k_fcg_dir, blocks(0), a, v);
-    LV_LAUNCH(k_fcg_update, blocks(0), a, v);
-    LV_LAUNCH(k_fcg_true, blocks(0), a, v);
+    if (K >= 1) {
+        LV_LAUNCH(k_lv_post, nb0, a, 0);
+        LV_LAUNCH(k_lv_smooth_dots, nb0, a, v);
+    } else {
+        LV_LAUNCH(k_fcg_dots, nb0, a, v);
+    }
+    LV_LAUNCH(k_fcg_dir, nb0, a, v, nb0);
+    LV_LAUNCH(k_fcg_update, nb0, a, v, nb0);
+    LV_LAUNCH(k_fcg_fin, 1, a, nb0);
+    LV_LAUNCH(k_fcg_true, nb0, a, v);
+    LV_LAUNCH(k_fcg_true_fin, 1, a, nb0);
     return 0;
 }
 
@@ -813,7 +822,7 @@ int cfd_set_learned(cfd_ctx* c, const cfd_learned_desc* d) {
     HIPC(hipStreamSynchronize(c->stream));
     if (!c->fcg) {
         TRY(dalloc(c, &c->fcg, 1));
-        TRY(dalloc(c, &c->l_part, 1024 * 8));
+        TRY(dalloc(c, &c->l_part, (size_t)LV_NSLOT * LV_SLOT));
         TRY(dalloc(c, &c->diag0f, 64));
         HIPC(hipHostMalloc((void**)&c->fcg_host, sizeof(FcgState), 0));
         std::vector<double> lut(128);

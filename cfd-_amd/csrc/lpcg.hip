@@ -5,8 +5,10 @@
 // restart from the true residual otherwise), as in the multigrid-preconditioned
 // branch of k_project.  Its iterations are ordinary kernel launches, captured
 // once into a HIP graph and replayed; every kernel returns at once after the
-// solve has finished, and reductions finish in the last work-group to arrive
-// (fixed-order sums, so results are reproducible).
+// solve has finished.  Reductions: each work-group stores its partial sums,
+// and the kernel that needs the total adds them in work-group order (so the
+// results are reproducible); one-work-group kernels take the decisions that
+// the host reads (converged, restart, breakdown).
 //
 // Preconditioner: an FP32 V-cycle over the aggregation hierarchy (damped
 // Jacobi, two sweeps down and up, as mg_vcycle) that stops at a coarse level K,
@@ -25,6 +27,11 @@
 
 enum { LEARNED_NET = 0, LEARNED_NONE = 1, LEARNED_TEACHER = 2 };
 
+// partial-sum slots (LV_SLOT doubles each) of LvArgs::part
+enum { LV_P_DOTS = 0, LV_P_PQ = 1, LV_P_RR = 2, LV_P_CM = 3, LV_NSLOT = 4 };
+constexpr int LV_MAXB = 1024;            // work-groups of any reducing kernel
+constexpr int LV_SLOT = LV_MAXB * 8;
+
 struct LvArgs {
     const MGDev* mg;         // hierarchy (device memory)
     const uint8_t* code;     // fine-level codes
@@ -32,7 +39,7 @@ struct LvArgs {
     int periodic;
     float ax, ay;
     FcgState* fs;
-    double* part;            // reduction partials
+    double* part;            // LV_NSLOT slots of partial sums
 };
 
 struct FcgVec {
@@ -48,6 +55,13 @@ __device__ __forceinline__ MGCtx lv_ctx(const LvArgs& a) {
     return MGCtx{a.mg, a.code, a.diag0, a.periodic, a.ax, a.ay};
 }
 
+// the fine diagonal table in LDS (after the uniform early return)
+#define LV_CTX_LDS(a)                                                          \
+    __shared__ float s_d0_[64];                                                \
+    if (threadIdx.x < 64) s_d0_[threadIdx.x] = (a).diag0[threadIdx.x];         \
+    __syncthreads();                                                           \
+    const MGCtx m{(a).mg, (a).code, s_d0_, (a).periodic, (a).ax, (a).ay}
+
 // f(i, j) over the level's interior cells: a band of rows per work-group,
 // threads along the contiguous j direction
 template <typename F>
@@ -58,8 +72,30 @@ __device__ __forceinline__ void lv_cells(int nx, int ny, F f) {
         for (int j = 1 + (int)threadIdx.x; j <= ny; j += blockDim.x) f(i, j);
 }
 
+// this work-group's sums of v into slot[blockIdx.x * NV + q]
+template <int NV>
+__device__ __forceinline__ void lv_partial(double (&v)[NV], double* slot) {
+    __shared__ double lds[NV * 16];
+    block_sum<NV>(v, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) slot[(long)blockIdx.x * NV + q] = v[q];
+}
+
+// totals of the partials of nb work-groups (every thread gets them)
+template <int NV>
+__device__ __forceinline__ void lv_total(const double* slot, int nb, double (&v)[NV]) {
+    __shared__ double lds[NV * 16];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) v[q] += slot[(long)b * NV + q];
+    block_sum<NV>(v, lds);
+}
+
 // Block sums of v written to part; the last work-group to arrive returns true
-// with the totals (summed in block order) in v.
+// with the totals (summed in block order) in v.  (Teacher solve only.)
 template <int NV>
 __device__ bool lv_reduce(double (&v)[NV], double* part, unsigned* ticket) {
     __shared__ double lds[NV * 16];
@@ -103,7 +139,7 @@ __device__ __forceinline__ float lv_a1(const MGCtx& m, int l, const MGLevelDev& 
 // dst = src + w (b - A src) / d on level l (from_t: src = t, dst = x; else x -> t)
 __global__ void __launch_bounds__(256) k_lv_smooth(LvArgs a, int l, int from_t) {
     if (a.fs->done) return;
-    const MGCtx m = lv_ctx(a);
+    LV_CTX_LDS(a);
     const MGLevelDev L = a.mg->lev[l];
     const float w = a.mg->omega;
     const float* src = from_t ? L.t : L.x;
@@ -117,10 +153,50 @@ __global__ void __launch_bounds__(256) k_lv_smooth(LvArgs a, int l, int from_t) 
     });
 }
 
+// last sweep of the V-cycle on the fine level, z = t = x + w (b - A x) / d,
+// with the partial dot products r.z and r_old.z of the flexible CG
+__global__ void __launch_bounds__(256) k_lv_smooth_dots(LvArgs a, FcgVec v) {
+    const FcgState* fs = a.fs;
+    if (fs->done) return;
+    LV_CTX_LDS(a);
+    const MGLevelDev L = a.mg->lev[0];
+    const float w = a.mg->omega;
+    const bool first = fs->first;
+    double s[2] = {0.0, 0.0};
+    lv_cells(L.nx, L.ny, [&](int i, int j) {
+        const long c = ix(i, j, L.P);
+        const float d = mg_diag(m, 0, L, c);
+        if (d == 0.0f) return;
+        auto xv = [&](int p, int q) -> float { return L.x[ix(p, q, L.P)]; };
+        const float z = L.x[c] + w * (L.b[c] - mg_apply(m, 0, L, i, j, d, xv)) / d;
+        L.t[c] = z;
+        s[0] += v.r[c] * (double)z;
+        if (!first) s[1] += v.rold[c] * (double)z;
+    });
+    lv_partial<2>(s, a.part + LV_P_DOTS * LV_SLOT);
+}
+
+// partial dot products r.z and r_old.z with z = lev[0].t already computed
+__global__ void __launch_bounds__(256) k_fcg_dots(LvArgs a, FcgVec v) {
+    const FcgState* fs = a.fs;
+    if (fs->done) return;
+    const float* z = a.mg->lev[0].t;
+    const bool first = fs->first;
+    double s[2] = {0.0, 0.0};
+    lv_cells(v.nx, a.mg->lev[0].ny, [&](int i, int j) {
+        const long c = ix(i, j, v.pc);
+        if (!(a.code[c] & 63)) return;
+        const double zc = z[c];
+        s[0] += v.r[c] * zc;
+        if (!first) s[1] += v.rold[c] * zc;
+    });
+    lv_partial<2>(s, a.part + LV_P_DOTS * LV_SLOT);
+}
+
 // level l+1: b = sum over the 2x2 children of (b - A t); x = w b / d
 __global__ void __launch_bounds__(256) k_lv_restrict(LvArgs a, int l) {
     if (a.fs->done) return;
-    const MGCtx m = lv_ctx(a);
+    LV_CTX_LDS(a);
     const MGLevelDev L = a.mg->lev[l];
     const MGLevelDev C = a.mg->lev[l + 1];
     const float w = a.mg->omega;
@@ -160,7 +236,7 @@ __global__ void __launch_bounds__(256) k_lv_dense(LvArgs a) {
 // level l: y = t + wc e_parent, then x = y + w (b - A y) / d
 __global__ void __launch_bounds__(256) k_lv_post(LvArgs a, int l) {
     if (a.fs->done) return;
-    const MGCtx m = lv_ctx(a);
+    LV_CTX_LDS(a);
     const MGLevelDev L = a.mg->lev[l];
     const MGLevelDev C = a.mg->lev[l + 1];
     const float w = a.mg->omega, wc = a.mg->wc;
@@ -178,13 +254,10 @@ __global__ void __launch_bounds__(256) k_lv_post(LvArgs a, int l) {
     });
 }
 
-// Level K statistics: c = 1^T b / 1^T A 1 and s = RMS(b - c A 1) over the
-// fluid cells; reserves a harvest slot when harvest_cap > 0.
-__global__ void __launch_bounds__(256) k_lv_cmode(LvArgs a, int l, double one_a_one,
-                                                  long long harvest_cap) {
-    FcgState* fs = a.fs;
-    if (fs->done) return;
-    const MGCtx m = lv_ctx(a);
+// Level K sums for c = 1^T b / 1^T A 1 and s = RMS(b - c A 1) (partials)
+__global__ void __launch_bounds__(256) k_lv_cmode(LvArgs a, int l) {
+    if (a.fs->done) return;
+    LV_CTX_LDS(a);
     const MGLevelDev L = a.mg->lev[l];
     double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     lv_cells(L.nx, L.ny, [&](int i, int j) {
@@ -199,7 +272,18 @@ __global__ void __launch_bounds__(256) k_lv_cmode(LvArgs a, int l, double one_a_
         s[3] += a1 * a1;
         s[4] += 1.0;
     });
-    if (lv_reduce<5>(s, a.part, &fs->ticket[0]) && threadIdx.x == 0) {
+    lv_partial<5>(s, a.part + LV_P_CM * LV_SLOT);
+}
+
+// one work-group: c and s of the current coarse solve; reserves a harvest
+// slot when harvest_cap > 0
+__global__ void __launch_bounds__(256) k_lv_cmode_fin(LvArgs a, int nb, double one_a_one,
+                                                      long long harvest_cap) {
+    FcgState* fs = a.fs;
+    if (fs->done) return;
+    double s[5];
+    lv_total<5>(a.part + LV_P_CM * LV_SLOT, nb, s);
+    if (threadIdx.x == 0) {
         const double cm = one_a_one > 0.0 ? s[0] / one_a_one : 0.0;
         const double bb = s[1] - 2.0 * cm * s[2] + cm * cm * s[3];
         fs->cm = cm;
@@ -218,20 +302,20 @@ __global__ void __launch_bounds__(256) k_lv_cmode(LvArgs a, int l, double one_a_
 __global__ void __launch_bounds__(256) k_lv_net_in(LvArgs a, int l, float* img, int H, int W) {
     const FcgState* fs = a.fs;
     if (fs->done) return;
-    const MGCtx m = lv_ctx(a);
+    LV_CTX_LDS(a);
     const MGLevelDev L = a.mg->lev[l];
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (long)H * W) return;
     const int i = (int)(k / W) + 1, j = (int)(k % W) + 1;
-    float v = 0.0f;
+    float val = 0.0f;
     if (i <= L.nx && j <= L.ny) {
         const long c = ix(i, j, L.P);
         const float d = mg_diag(m, l, L, c);
         const double s = fs->s_in;
         if (d != 0.0f && s > 0.0)
-            v = (float)(((double)L.b[c] - fs->cm * (double)lv_a1(m, l, L, i, j, d)) / s);
+            val = (float)(((double)L.b[c] - fs->cm * (double)lv_a1(m, l, L, i, j, d)) / s);
     }
-    img[k] = v;
+    img[k] = val;
 }
 
 // level-K correction t = c + gain s y (gain = sigma / D; use_net = 0: t = c)
@@ -239,7 +323,7 @@ __global__ void __launch_bounds__(256) k_lv_net_out(LvArgs a, int l, const float
                                                     float gain, int use_net) {
     const FcgState* fs = a.fs;
     if (fs->done) return;
-    const MGCtx m = lv_ctx(a);
+    LV_CTX_LDS(a);
     const MGLevelDev L = a.mg->lev[l];
     const double cm = fs->cm, s = fs->s_in;
     lv_cells(L.nx, L.ny, [&](int i, int j) {
@@ -382,35 +466,26 @@ __global__ void __launch_bounds__(256) k_lv_harvest(LvArgs a, int l, float* hin,
 }
 
 // ---- flexible CG steps on the fine level (z = lev[0].t) ----
-// beta = (r.z - r_old.z) / rho_old (0 at a (re)start), rho = r.z
-__global__ void __launch_bounds__(256) k_fcg_dots(LvArgs a, FcgVec v) {
+// beta = (r.z - r_old.z) / rho_old (0 at a (re)start) from the dot partials of
+// nb work-groups; d = z + beta d_old, q = A d (neighbours' d rebuilt from z and
+// d_old), partial d.q
+__global__ void __launch_bounds__(256) k_fcg_dir(LvArgs a, FcgVec v, int nb) {
     FcgState* fs = a.fs;
     if (fs->done) return;
-    const float* z = a.mg->lev[0].t;
+    double dots[2];
+    lv_total<2>(a.part + LV_P_DOTS * LV_SLOT, nb, dots);
     const bool first = fs->first;
-    double s[2] = {0.0, 0.0};
-    lv_cells(v.nx, a.mg->lev[0].ny, [&](int i, int j) {
-        const long c = ix(i, j, v.pc);
-        if (!(a.code[c] & 63)) return;
-        const double zc = z[c];
-        s[0] += v.r[c] * zc;
-        if (!first) s[1] += v.rold[c] * zc;
-    });
-    if (lv_reduce<2>(s, a.part, &fs->ticket[1]) && threadIdx.x == 0) {
-        fs->beta = first ? 0.0 : (s[0] - s[1]) / fs->rho_old;
-        fs->rho = s[0];
+    const double beta = first ? 0.0 : (dots[0] - dots[1]) / fs->rho_old;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        fs->rho = dots[0];
+        fs->beta = beta;
     }
-}
-
-// d = z + beta d_old, q = A d (neighbours' d rebuilt from z and d_old), alpha = rho / d.q
-__global__ void __launch_bounds__(256) k_fcg_dir(LvArgs a, FcgVec v) {
-    FcgState* fs = a.fs;
-    if (fs->done) return;
+    __shared__ double s_lut[64];
+    if (threadIdx.x < 64) s_lut[threadIdx.x] = v.lut[threadIdx.x];
+    __syncthreads();
     const float* z = a.mg->lev[0].t;
     const int ny = a.mg->lev[0].ny;
     const int k = fs->k;
-    const bool first = fs->first;
-    const double beta = fs->beta;
     double* dnew = (k & 1) ? v.d1 : v.d0;
     const double* dold = (k & 1) ? v.d0 : v.d1;
     double pq[1] = {0.0};
@@ -425,30 +500,35 @@ __global__ void __launch_bounds__(256) k_fcg_dir(LvArgs a, FcgVec v) {
             return first ? zc : zc + beta * dold[cc];
         };
         const double dc = dirv(c);
-        const double q = v.lut[li] * dc - v.ax * (dirv(ix(iE, j, v.pc)) + dirv(ix(iW, j, v.pc))) -
+        const double q = s_lut[li] * dc - v.ax * (dirv(ix(iE, j, v.pc)) + dirv(ix(iW, j, v.pc))) -
                          v.ay * (dirv(ix(i, j + 1, v.pc)) + dirv(ix(i, j - 1, v.pc)));
         dnew[c] = dc;
         v.q[c] = q;
         pq[0] += dc * q;
     });
-    if (lv_reduce<1>(pq, a.part, &fs->ticket[2]) && threadIdx.x == 0) {
-        const double alpha = fs->rho / pq[0];
-        fs->alpha = alpha;
-        if (!(pq[0] > 0.0) || !(fabs(alpha) < 1e300)) {   // breakdown or non-finite
+    lv_partial<1>(pq, a.part + LV_P_PQ * LV_SLOT);
+}
+
+// alpha = rho / d.q; x += alpha d, r_old = r, r -= alpha q; next V-cycle
+// input b0 = r, x0 = w r / d; partial r.r
+__global__ void __launch_bounds__(256) k_fcg_update(LvArgs a, FcgVec v, int nb) {
+    FcgState* fs = a.fs;
+    if (fs->done) return;
+    double pq[1];
+    lv_total<1>(a.part + LV_P_PQ * LV_SLOT, nb, pq);
+    const double alpha = fs->rho / pq[0];
+    if (!(pq[0] > 0.0) || !(fabs(alpha) < 1e300)) {   // breakdown or non-finite
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
             fs->done = 1;
             fs->info = -1;
         }
+        return;
     }
-}
-
-// x += alpha d, r_old = r, r -= alpha q; next V-cycle input b0 = r, x0 = w r / d
-__global__ void __launch_bounds__(256) k_fcg_update(LvArgs a, FcgVec v) {
-    FcgState* fs = a.fs;
-    if (fs->done) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) fs->alpha = alpha;
+    LV_CTX_LDS(a);
     const MGLevelDev L0 = a.mg->lev[0];
     const float w = a.mg->omega;
     const int k = fs->k;
-    const double alpha = fs->alpha;
     const double* dnew = (k & 1) ? v.d1 : v.d0;
     double s[1] = {0.0};
     lv_cells(v.nx, L0.ny, [&](int i, int j) {
@@ -463,28 +543,39 @@ __global__ void __launch_bounds__(256) k_fcg_update(LvArgs a, FcgVec v) {
         s[0] += rn * rn;
         const float rf = (float)rn;
         L0.b[c] = rf;
-        L0.x[c] = w * rf / a.diag0[li];
+        L0.x[c] = w * rf / m.diag0[li];
     });
-    if (lv_reduce<1>(s, a.part, &fs->ticket[3]) && threadIdx.x == 0) {
-        fs->rr = s[0];
+    lv_partial<1>(s, a.part + LV_P_RR * LV_SLOT);
+}
+
+// one work-group: end of an iteration (recursive residual test)
+__global__ void __launch_bounds__(256) k_fcg_fin(LvArgs a, int nb) {
+    FcgState* fs = a.fs;
+    if (fs->done) return;
+    double rr[1];
+    lv_total<1>(a.part + LV_P_RR * LV_SLOT, nb, rr);
+    if (threadIdx.x == 0) {
+        const int k = fs->k;
+        fs->rr = rr[0];
         fs->rho_old = fs->rho;
         fs->k = k + 1;
         fs->iters = k + 1;
         fs->first = 0;
-        if (!(s[0] == s[0])) {
+        if (!(rr[0] == rr[0])) {
             fs->done = 1;
             fs->info = -1;
-        } else if (sqrt(s[0]) < fs->atol || k + 1 >= fs->maxiter) {
+        } else if (sqrt(rr[0]) < fs->atol || k + 1 >= fs->maxiter) {
             fs->pending = 1;
         }
     }
 }
 
 // true residual r = b - A x once the recursive one has converged (or at
-// maxiter): finish, or restart the flexible CG from it
+// maxiter), with the next V-cycle input; partial r.r
 __global__ void __launch_bounds__(256) k_fcg_true(LvArgs a, FcgVec v) {
     FcgState* fs = a.fs;
     if (fs->done || !fs->pending) return;
+    LV_CTX_LDS(a);
     const MGLevelDev L0 = a.mg->lev[0];
     const float w = a.mg->omega;
     double s[1] = {0.0};
@@ -497,16 +588,25 @@ __global__ void __launch_bounds__(256) k_fcg_true(LvArgs a, FcgVec v) {
         s[0] += rt * rt;
         const float rf = (float)rt;
         L0.b[c] = rf;
-        L0.x[c] = w * rf / a.diag0[li];
+        L0.x[c] = w * rf / m.diag0[li];
     });
-    if (lv_reduce<1>(s, a.part, &fs->ticket[0]) && threadIdx.x == 0) {
-        fs->rr = s[0];
+    lv_partial<1>(s, a.part + LV_P_RR * LV_SLOT);
+}
+
+// one work-group: finish, or restart the flexible CG from the true residual
+__global__ void __launch_bounds__(256) k_fcg_true_fin(LvArgs a, int nb) {
+    FcgState* fs = a.fs;
+    if (fs->done || !fs->pending) return;
+    double rr[1];
+    lv_total<1>(a.part + LV_P_RR * LV_SLOT, nb, rr);
+    if (threadIdx.x == 0) {
+        fs->rr = rr[0];
         fs->reps += 1;
         fs->pending = 0;
-        if (sqrt(s[0]) < fs->atol) {
+        if (sqrt(rr[0]) < fs->atol) {
             fs->done = 1;
             fs->info = 0;
-        } else if (!(s[0] == s[0])) {
+        } else if (!(rr[0] == rr[0])) {
             fs->done = 1;
             fs->info = -1;
         } else if (fs->k >= fs->maxiter) {

@@ -51,59 +51,84 @@ __global__ void k_mfma_layout_check(const float* A, const float* B, float* D) {
 // converged leaves the rest of its captured iteration idle); skip may be NULL.
 // Output either bf16 (Yb) or fp32 (Yf); with fp32 output, channels >= csplit
 // go to Yf2 (channel stride Cout - csplit), the others to Yf (stride csplit):
-// the gradient of a concatenated input lands in its two parts.  One wave per
-// 16-pixel tile; the block's 4 waves take 4 consecutive tiles.
+// the gradient of a concatenated input lands in its two parts.
+// One wave computes a 16-pixel tile (of one image row) times G 16-channel
+// output chunks; consecutive waves take the other chunks of the same tile.
+// The K loop runs over (tap, 16-channel input chunk) steps, loading the
+// operands of 8 steps before their matrix-core instructions (small images
+// leave few waves per CU, so load latency rather than issue rate limits).
 // ---------------------------------------------------------------------------
-template <int NCO>   // Cout / 16 accumulators per wave
+template <int G>
 __global__ void __launch_bounds__(256) k_conv3x3_mfma(
     const unsigned short* __restrict__ X1, int C1, const unsigned short* __restrict__ X2, int C2,
-    const unsigned short* __restrict__ Wt, const float* __restrict__ bias, int H, int Wd,
+    const unsigned short* __restrict__ Wt, const float* __restrict__ bias, int H, int Wd, int Cout,
     unsigned short* __restrict__ Yb, float* __restrict__ Yf, float* __restrict__ Yf2, int csplit,
     int mode, const int* skip) {
     if (skip && *skip) return;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int tiles_w = (Wd + 15) / 16;
-    const long tile = (long)blockIdx.x * 4 + wave;
-    if (tile >= (long)tiles_w * H) return;
+    const int ngrp = Cout / (16 * G);
+    const long task = (long)blockIdx.x * 4 + wave;
+    if (task >= (long)tiles_w * H * ngrp) return;
+    const int grp = (int)(task % ngrp);
+    const long tile = task / ngrp;
     const int h = (int)(tile / tiles_w);
     const int w0 = (int)(tile - (long)h * tiles_w) * 16;
     const int Cin = C1 + C2;
-    const int Cout = NCO * 16;
+    const int nk = Cin / 16;
+    const int nsteps = 9 * nk;
     const int m = lane % 16, kq = 4 * (lane / 16);
-    f32x4 acc[NCO];
+    f32x4 acc[G];
 #pragma unroll
-    for (int n = 0; n < NCO; ++n) acc[n] = (f32x4){0.f, 0.f, 0.f, 0.f};
-    for (int t = 0; t < 9; ++t) {
+    for (int g = 0; g < G; ++g) acc[g] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    auto load = [&](int st, bf16x4& a, bf16x4 (&b)[G]) {
+        const int t = st / nk;
+        const int k0 = (st - t * nk) * 16;
         const int hh = h + t / 3 - 1;
         const int ww = w0 + m + t % 3 - 1;
-        const bool in = hh >= 0 && hh < H && ww >= 0 && ww < Wd;
-        const long pix = (long)hh * Wd + ww;
-        for (int k0 = 0; k0 < Cin; k0 += 16) {
-            bf16x4 a = {0, 0, 0, 0};
-            if (in) {
-                const unsigned short* src = k0 < C1 ? X1 + pix * C1 + k0 + kq
-                                                    : X2 + pix * C2 + (k0 - C1) + kq;
-                a = *reinterpret_cast<const bf16x4*>(src);
-            }
+        a = (bf16x4){0, 0, 0, 0};
+        if (hh >= 0 && hh < H && ww >= 0 && ww < Wd) {
+            const long pix = (long)hh * Wd + ww;
+            const unsigned short* src = k0 < C1 ? X1 + pix * C1 + k0 + kq
+                                                : X2 + pix * C2 + (k0 - C1) + kq;
+            a = *reinterpret_cast<const bf16x4*>(src);
+        }
 #pragma unroll
-            for (int n = 0; n < NCO; ++n) {
-                const bf16x4 b = *reinterpret_cast<const bf16x4*>(
-                    Wt + ((long)t * Cout + n * 16 + m) * Cin + k0 + kq);
-                acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[n], 0, 0, 0);
+        for (int g = 0; g < G; ++g)
+            b[g] = *reinterpret_cast<const bf16x4*>(
+                Wt + ((long)t * Cout + (grp * G + g) * 16 + m) * Cin + k0 + kq);
+    };
+    // U steps of operands in flight at a time (steps past the end load zeros)
+    constexpr int U = 8;
+    for (int s0 = 0; s0 < nsteps; s0 += U) {
+        bf16x4 a[U], b[U][G];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (s0 + u < nsteps) {
+                load(s0 + u, a[u], b[u]);
+            } else {
+                a[u] = (bf16x4){0, 0, 0, 0};
+#pragma unroll
+                for (int g = 0; g < G; ++g) b[u][g] = (bf16x4){0, 0, 0, 0};
             }
         }
-    }
-    // D: lane holds pixels w0 + kq + j (j = 0..3) of channel n*16 + m
 #pragma unroll
-    for (int n = 0; n < NCO; ++n) {
-        const int co = n * 16 + m;
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+                acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], b[u][g], acc[g], 0, 0, 0);
+    }
+    // D: lane holds pixels w0 + kq + j (j = 0..3) of channel (grp*G + g)*16 + m
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int co = (grp * G + g) * 16 + m;
         const float bv = (mode & 1) ? bias[co] : 0.f;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int w = w0 + kq + j;
             if (w >= Wd) continue;
-            float v = acc[n][j] + bv;
+            float v = acc[g][j] + bv;
             const long px = (long)h * Wd + w;
             if (Yf) {
                 float* dst = co < csplit ? Yf + px * csplit + co
@@ -288,7 +313,8 @@ __global__ void k_conv_in(const float* __restrict__ x, int H, int Wd, const floa
     Y[k] = f2bf(fmaxf(s, 0.f));
 }
 
-// last layer (C -> 1), scalar, linear: y = b + sum_t sum_c W[t][c] X[p+off_t][c]
+// last layer (C -> 1), linear: y = b + sum_t sum_c W[t][c] X[p+off_t][c]
+// (C a multiple of 8: 16-byte loads of 8 channels)
 __global__ void k_conv_out(const unsigned short* __restrict__ X, int H, int Wd, int C,
                            const float* __restrict__ W, const float* __restrict__ b,
                            float* __restrict__ y, const int* skip) {
@@ -297,11 +323,21 @@ __global__ void k_conv_out(const unsigned short* __restrict__ X, int H, int Wd, 
     if (p >= (long)H * Wd) return;
     const int h = (int)(p / Wd), w = (int)(p % Wd);
     float s = b[0];
+#pragma unroll
     for (int t = 0; t < 9; ++t) {
         const int hh = h + t / 3 - 1, ww = w + t % 3 - 1;
         if (hh < 0 || hh >= H || ww < 0 || ww >= Wd) continue;
-        const unsigned short* xp = X + ((long)hh * Wd + ww) * C;
-        for (int c = 0; c < C; ++c) s += W[t * C + c] * bf2f(xp[c]);
+        const uint4* xp = reinterpret_cast<const uint4*>(X + ((long)hh * Wd + ww) * C);
+        for (int c8 = 0; c8 < C / 8; ++c8) {
+            const uint4 q = xp[c8];
+            const unsigned int u[4] = {q.x, q.y, q.z, q.w};
+            const float* wt = W + t * C + c8 * 8;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                s += wt[2 * e] * __uint_as_float(u[e] << 16);
+                s += wt[2 * e + 1] * __uint_as_float(u[e] & 0xffff0000u);
+            }
+        }
     }
     y[p] = s;
 }

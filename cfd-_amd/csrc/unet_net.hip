@@ -100,36 +100,19 @@ int conv_mfma(cfd_unet* u, const unsigned short* x1, int c1, const unsigned shor
               const unsigned short* wt, const float* bias, int H, int W, int cout,
               unsigned short* yb, float* yf, float* yf2, int csplit, int mode,
               const int* skip) {
+    if (cout % 16 || c1 % 16 || c2 % 16 || cout > 128)
+        return set_err(CFD_ERR_ARG, "unsupported channel counts %d+%d -> %d", c1, c2, cout);
+    // two output chunks per wave, unless that leaves fewer than 512 waves
     const long tiles = (long)H * ((W + 15) / 16);
-    const dim3 grid((unsigned)((tiles + 3) / 4));
-    switch (cout) {
-    case 16:
-        hipLaunchKernelGGL(k_conv3x3_mfma<1>, grid, dim3(256), 0, u->stream, x1, c1, x2, c2, wt,
-                           bias, H, W, yb, yf, yf2, csplit, mode, skip);
-        break;
-    case 32:
+    const int G = (cout % 32 == 0 && tiles * (cout / 32) >= 512) ? 2 : 1;
+    const long tasks = tiles * (cout / (16 * G));
+    const dim3 grid((unsigned)((tasks + 3) / 4));
+    if (G == 2)
         hipLaunchKernelGGL(k_conv3x3_mfma<2>, grid, dim3(256), 0, u->stream, x1, c1, x2, c2, wt,
-                           bias, H, W, yb, yf, yf2, csplit, mode, skip);
-        break;
-    case 48:
-        hipLaunchKernelGGL(k_conv3x3_mfma<3>, grid, dim3(256), 0, u->stream, x1, c1, x2, c2, wt,
-                           bias, H, W, yb, yf, yf2, csplit, mode, skip);
-        break;
-    case 64:
-        hipLaunchKernelGGL(k_conv3x3_mfma<4>, grid, dim3(256), 0, u->stream, x1, c1, x2, c2, wt,
-                           bias, H, W, yb, yf, yf2, csplit, mode, skip);
-        break;
-    case 96:
-        hipLaunchKernelGGL(k_conv3x3_mfma<6>, grid, dim3(256), 0, u->stream, x1, c1, x2, c2, wt,
-                           bias, H, W, yb, yf, yf2, csplit, mode, skip);
-        break;
-    case 128:
-        hipLaunchKernelGGL(k_conv3x3_mfma<8>, grid, dim3(256), 0, u->stream, x1, c1, x2, c2, wt,
-                           bias, H, W, yb, yf, yf2, csplit, mode, skip);
-        break;
-    default:
-        return set_err(CFD_ERR_ARG, "unsupported channel count %d", cout);
-    }
+                           bias, H, W, cout, yb, yf, yf2, csplit, mode, skip);
+    else
+        hipLaunchKernelGGL(k_conv3x3_mfma<1>, grid, dim3(256), 0, u->stream, x1, c1, x2, c2, wt,
+                           bias, H, W, cout, yb, yf, yf2, csplit, mode, skip);
     HIPC(hipGetLastError());
     return 0;
 }

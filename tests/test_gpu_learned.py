@@ -1,0 +1,203 @@
+"""The learned flexible CG (pressure_solver="cg_learned") on the GPU.
+
+The preconditioner is a V-cycle whose coarse correction comes from the U-Net
+(shipped weights, trained on the GPU by the library's own kernels).  Checked:
+the launch-based V-cycle against the persistent multigrid solver, every mode
+against the FP64 solvers and the FP64 oracle, the gain of the trained network
+over the same solver without it, the harvested training pairs, the Jacobi-PCG
+fallback, and a short harvest -> train -> solve run from scratch.
+"""
+
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _cmp(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max())
+
+
+def _cavity(n, solver, learned=None, re=1000.0, tol=1e-6):
+    from cfd2d_amd import cases
+
+    return cases.lid_driven_cavity(re=re, n=n, pressure_solver=solver, pressure_tol=tol,
+                                   learned=learned).sim
+
+
+def _run(sim, steps):
+    for _ in range(steps):
+        sim.step()
+    sim._flush()
+    return sim
+
+
+def test_launch_vcycle_equals_persistent_multigrid(gpu):
+    """Teacher mode on level 0 with one inner iteration is the multigrid
+    V-cycle of cg_mixed (scaled by a line search, which flexible CG undoes):
+    same iteration counts, same fields to round-off."""
+    from cfd2d_amd.learned import LearnedConfig
+
+    mg = _run(_cavity(64, "cg_mixed"), 20)
+    lv = _run(_cavity(64, "cg_learned", LearnedConfig(mode="teacher", level=0,
+                                                      teacher_cycles=1)), 20)
+    assert lv.cg_iterations == mg.cg_iterations
+    assert _cmp(lv.u, mg.u) < 1e-9 and _cmp(lv.p, mg.p) < 1e-8
+
+
+@pytest.mark.parametrize("mode", ["net", "none", "teacher"])
+def test_learned_modes_match_fp64_cg(gpu, mode):
+    from cfd2d_amd.learned import LearnedConfig
+
+    ref = _run(_cavity(64, "cg", tol=1e-10), 30)
+    sim = _run(_cavity(64, "cg_learned", LearnedConfig(mode=mode)), 30)
+    assert max(sim.cg_relres) <= 1e-6
+    assert all(i == 0 for i in sim.cg_info)
+    assert sim.pressure.learned.stats()["fallbacks"] == 0
+    assert _cmp(sim.u, ref.u) < 1e-6 and _cmp(sim.v, ref.v) < 1e-6
+
+
+def test_graph_replay_equals_direct_launches(gpu):
+    from cfd2d_amd.learned import LearnedConfig
+
+    a = _run(_cavity(128, "cg_learned", LearnedConfig(use_graph=True)), 15)
+    b = _run(_cavity(128, "cg_learned", LearnedConfig(use_graph=False)), 15)
+    assert a.cg_iterations == b.cg_iterations
+    assert np.array_equal(a.u, b.u) and np.array_equal(a.p, b.p)
+
+
+def test_learned_matches_fp64_oracle(gpu):
+    """Velocity within 1e-5 of the FP64 oracle (LU) after 60 steps at 128^2,
+    Re 1000, every solve to a true relative residual of 1e-6."""
+    from oracle import cfd_oracle as O
+
+    sim = _cavity(128, "cg_learned")
+    ref = O.cavity(re=1000, n=128, pressure_solver="direct")
+    for _ in range(60):
+        sim.step()
+        ref.step()
+    sim._flush()
+    eu, ev = _cmp(sim.u, ref.u), _cmp(sim.v, ref.v)
+    print(f"128^2 after 60 steps: max|du| {eu:.2e} max|dv| {ev:.2e}")
+    assert max(sim.cg_relres) <= 1e-6
+    assert eu < 1e-5 and ev < 1e-5
+
+
+@pytest.mark.parametrize("n", [256, 1024])
+def test_network_gain(gpu, n):
+    """The trained network against the same solver without it (constant-vector
+    coarse correction only): far fewer iterations and faster steps."""
+    from cfd2d_amd.learned import LearnedConfig
+
+    out = {}
+    for mode in ("net", "none"):
+        sim = _run(_cavity(n, "cg_learned", LearnedConfig(mode=mode)), 3)
+        t0 = time.perf_counter()
+        _run(sim, 10)
+        out[mode] = ((time.perf_counter() - t0) / 10, float(np.mean(sim.cg_iterations[-10:])))
+        assert max(sim.cg_relres[-10:]) <= 1e-6
+        sim.close()
+    print(f"{n}^2: with the network {out['net'][1]:.1f} iterations {1e3 * out['net'][0]:.2f} ms/step, "
+          f"without {out['none'][1]:.1f} iterations {1e3 * out['none'][0]:.2f} ms/step")
+    assert out["net"][1] < 0.4 * out["none"][1]
+    assert out["net"][0] < out["none"][0]
+
+
+def test_config2_velocity_error_against_fp64_cg(gpu):
+    """The benchmark workload (1024^2, Re 1000, van Leer scalar, 40 steps):
+    learned-PCG at rtol 1e-6 against the FP64 Jacobi-PCG path at rtol 1e-10
+    (the GPU FP64 path reproduces the oracle's trajectories): velocity within
+    1e-5, scalar within 1e-6."""
+    from bench import build_case
+
+    a = build_case("cg_learned").sim
+    ref_case = build_case("cg")
+    ref = ref_case.sim
+    ref.pressure.tol = 1e-10
+    _run(a, 40)
+    _run(ref, 40)
+    eu, ev = _cmp(a.u, ref.u), _cmp(a.v, ref.v)
+    ec = _cmp(a.scalars["c"].phi, ref.scalars["c"].phi)
+    print(f"1024^2 after 40 steps: max|du| {eu:.2e} max|dv| {ev:.2e} max|dc| {ec:.2e}")
+    assert max(a.cg_relres) <= 1e-6
+    assert eu < 1e-5 and ev < 1e-5 and ec < 1e-6
+
+
+def test_harvested_pairs_solve_the_coarse_system(gpu):
+    """Teacher pairs (input x, target t): A_K t / D = x to the teacher's
+    accuracy on the coarse level K (its Galerkin matrix from the hierarchy)."""
+    from cfd2d_amd.learned import LearnedConfig
+
+    sim = _run(_cavity(256, "cg_learned", LearnedConfig(mode="teacher", harvest_cap=64)), 10)
+    att = sim.pressure.learned
+    x, t = att.harvest()
+    assert x.shape[0] == min(64, att.stats()["harvested"]) and x.shape[0] > 10
+    L = sim.pressure.hierarchy.levels[att.level]
+    fl = L["fluid"]
+    D = 2.0 / sim.grid.dx ** 2 + 2.0 / sim.grid.dy ** 2
+    worst = 0.0
+    for k in range(x.shape[0]):
+        tv = t[k, :L["nx"], :L["ny"]][fl].astype(np.float64)
+        xv = x[k, :L["nx"], :L["ny"]][fl].astype(np.float64)
+        worst = max(worst, np.linalg.norm(L["A"] @ tv / D - xv) / np.linalg.norm(xv))
+    assert worst < 1e-3
+    # the inputs are deflated (orthogonal to the constant vector) and normalised
+    assert np.allclose(np.sqrt(np.mean(x[:, :L["nx"], :L["ny"]] ** 2, axis=(1, 2))), 1.0,
+                       rtol=1e-3)
+
+
+def test_fallback_to_jacobi_pcg(gpu):
+    """A learned solve that cannot finish in max_iter iterations is continued
+    by the FP64 Jacobi-PCG; the result still meets the tolerance."""
+    from cfd2d_amd.learned import LearnedConfig
+
+    ref = _run(_cavity(64, "cg", tol=1e-10), 10)
+    sim = _run(_cavity(64, "cg_learned", LearnedConfig(mode="none", max_iter=3)), 10)
+    assert sim.pressure.learned.stats()["fallbacks"] >= 10
+    assert max(sim.cg_relres) <= 1e-6 and all(i == 0 for i in sim.cg_info)
+    assert _cmp(sim.u, ref.u) < 1e-6
+
+
+def test_pressure_solve_learned_equals_dct(gpu):
+    import scipy.fft
+
+    n = 128
+    sim = _cavity(n, "cg_learned", tol=1e-10)
+    rng = np.random.default_rng(5)
+    rhs = rng.standard_normal((n, n))
+    p = sim.pressure.solve(rhs)
+    h = 1.0 / n
+    lam = (2 - 2 * np.cos(np.pi * np.arange(n) / n)) / h ** 2
+    L = lam[:, None] + lam[None, :]
+    L[0, 0] = 1.0
+    r = scipy.fft.dctn(-(rhs - rhs.mean()), type=2, norm="ortho") / L
+    r[0, 0] = 0.0
+    exact = scipy.fft.idctn(r, type=2, norm="ortho")
+    assert sim.pressure.last_relres <= 1e-10
+    assert np.abs(p - exact).max() < 1e-7 * np.abs(exact).max()
+
+
+def test_harvest_train_solve_from_scratch(gpu):
+    """The whole learned pipeline in miniature: harvest pairs from the
+    solver's own solves, train a fresh network on the GPU for a few seconds,
+    and use it: fewer iterations than without a network."""
+    from cfd2d_amd.learned import LearnedConfig, train
+
+    sim = _run(_cavity(256, "cg_learned", LearnedConfig(mode="teacher", harvest_cap=400)), 60)
+    x, t = sim.pressure.learned.harvest()
+    sim.close()
+    params, sigma, hist = train(x, t, epochs=200, batch=8, lr=1e-3, per_epoch=400, seed=3,
+                                time_limit=15.0)
+    assert hist[-1] < 0.5 * hist[0]
+    its = {}
+    for mode in ("net", "none"):
+        s = _run(_cavity(256, "cg_learned", LearnedConfig(mode=mode, params=params,
+                                                          sigma=sigma)), 8)
+        its[mode] = float(np.mean(s.cg_iterations[-5:]))
+        assert max(s.cg_relres) <= 1e-6
+        s.close()
+    print(f"fresh network after {len(hist)} epochs (loss {hist[0]:.3f} -> {hist[-1]:.4f}): "
+          f"{its['net']:.1f} iterations vs {its['none']:.1f} without")
+    assert its["net"] < 0.7 * its["none"]
