@@ -44,6 +44,7 @@ RE = 1000.0
 RTOL = 1e-6
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_RATIO_MGCG = 1.10
+TRAFFIC_RATIO_LEARNED = 1.41
 WORKLOAD = ("lid-driven cavity Re=1000, 1024x1024 MAC grid, AB2, gamma auto, tau 0.5, "
             "passive van Leer scalar (D=1e-3, 8 Gaussian blobs, seed 0), "
             "pressure solve to relative residual 1e-6")
@@ -69,10 +70,15 @@ LEGS = {"cg": ("cg", None), "cg_mixed": ("cg_mixed", None),
         "cg_learned_none": ("cg_learned", {"mode": "none"})}
 
 
+USE_GRAPH = True   # learned legs: replay each iteration as a captured HIP graph
+
+
 def build_case(leg: str, n: int = N):
     from cfd2d_amd import cases
 
     solver, learned = LEGS[leg]
+    if learned is not None:
+        learned = dict(learned, use_graph=USE_GRAPH)
     case = cases.lid_driven_cavity(re=RE, n=n, pressure_solver=solver, pressure_tol=RTOL,
                                    pressure_maxiter=50000, learned=learned)
     cases.add_scalar(case, "c", diffusivity=1e-3, limiter="vanleer", initial=gaussian_blobs(n))
@@ -142,15 +148,23 @@ def roofline_for(leg: dict) -> dict:
         bytes_per_launch = n_cells * s * (3 + 7 + 10 * n_it)
     t = leg["proj_ms_mean"] * 1e-3
     achieved = bytes_per_launch / t / 1e9 if t > 0 else 0.0
-    # measured HBM traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated against a
-    # 256 MiB copy) was 1.10x the algorithmic bytes for the multigrid launches
-    traffic = round(TRAFFIC_RATIO_MGCG * bytes_per_launch) if leg["solver"] == "cg_mixed" else None
     learned = leg["solver"].startswith("cg_learned")
+    # measured HBM traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated against a
+    # 256 MiB copy) over the algorithmic bytes of the same launches
+    if learned:
+        traffic = round(TRAFFIC_RATIO_LEARNED * bytes_per_launch)
+        source = ("rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE summed over the projection kernels "
+                  "of 10 steps (ratio 1.41 to algorithmic), "
+                  "profiles/learned_projection_cavity1024.md")
+    elif leg["solver"] == "cg_mixed":
+        traffic = round(TRAFFIC_RATIO_MGCG * bytes_per_launch)
+        source = ("rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE of 6-iteration launches "
+                  "(ratio 1.10 to algorithmic), profiles/k_project_traffic_cavity1024.md")
+    else:
+        traffic, source = None, None
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE of 6-iteration launches "
-                              "(ratio 1.10 to algorithmic), profiles/k_project_traffic_cavity1024.md"
-            if traffic else None,
+            "traffic_source": source,
             "kernel": ("learned projection: k_project (right-hand side; write-back and "
                        "correction) + the flexible-CG iteration kernels" if learned
                        else "k_project"),
@@ -227,7 +241,12 @@ def main() -> None:
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device-to-device (a known-size kernel for "
                          "scaling rocprof FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="learned legs launch their iteration kernels directly instead of "
+                         "replaying a captured graph (for profilers that do not follow graphs)")
     args = ap.parse_args()
+    global USE_GRAPH
+    USE_GRAPH = not args.no_graph
     if args.cpu_child:
         print(json.dumps(cpu_baseline_child(float(args.cpu_child[0]), int(args.cpu_child[1]))))
         return
