@@ -383,27 +383,35 @@ FcgVec fcg_vec(cfd_ctx* c) {
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, c->stream, __VA_ARGS__);       \
         HIPC(hipGetLastError());                                                           \
     } while (0)
+#define LV_LAUNCH512(kern, blocks, ...)                                                    \
+    do {                                                                                   \
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(512), 0, c->stream, __VA_ARGS__);       \
+        HIPC(hipGetLastError());                                                           \
+    } while (0)
 
-// the kernels of one flexible-CG iteration, in stream order
+// the kernels of one flexible-CG iteration, in stream order.  Reducing
+// kernels run in bands of rows (at most 512 work-groups, whose partial sums the
+// next kernel adds); the other sweeps use one cell per thread.
 int learned_iteration(cfd_ctx* c) {
     const LvArgs a = lv_args(c);
     const FcgVec v = fcg_vec(c);
     const MGDev& mg = c->mg_host;
     const int K = c->l_level, NL = mg.nlev;
-    auto blocks = [&](int l) { return std::min(mg.lev[l].nx, 512); };
-    const int nb0 = blocks(0), nbK = blocks(K);
+    auto bands = [&](int l) { return std::min(mg.lev[l].nx, 512); };
+    auto cells = [&](int l) {
+        return (unsigned)(mg.lev[l].nx * ((mg.lev[l].ny + 255) / 256));
+    };
+    const int nb0 = bands(0), nbK = bands(K);
     const unsigned nimg = (unsigned)(((long)c->l_H * c->l_W + 255) / 256);
     for (int l = 0; l < K; ++l) {
-        LV_LAUNCH(k_lv_smooth, blocks(l), a, l, 0);
-        LV_LAUNCH(k_lv_restrict, blocks(l + 1), a, l);
+        LV_LAUNCH(k_lv_smooth, cells(l), a, l, 0);
+        LV_LAUNCH(k_lv_restrict, cells(l + 1), a, l);
     }
     LV_LAUNCH(k_lv_cmode, nbK, a, K);
-    LV_LAUNCH(k_lv_cmode_fin, 1, a, nbK, c->l_1a1,
-              c->l_mode == LEARNED_TEACHER ? c->l_hcap : 0LL);
     if (c->l_mode != LEARNED_TEACHER) {
         const bool net = c->l_mode == LEARNED_NET;
         if (net) {
-            LV_LAUNCH(k_lv_net_in, nimg, a, K, c->l_net->xin, c->l_H, c->l_W);
+            LV_LAUNCH(k_lv_net_in, nimg, a, K, c->l_net->xin, c->l_H, c->l_W, nbK, c->l_1a1);
             // the network runs on this context's stream and idles once the solve is done
             cfd_unet* u = c->l_net;
             const hipStream_t s0 = u->stream;
@@ -415,22 +423,23 @@ int learned_iteration(cfd_ctx* c) {
             u->skip = k0;
             TRY(rc);
         }
-        LV_LAUNCH(k_lv_net_out, nbK, a, K, net ? (const float*)c->l_net->y : nullptr, c->l_W,
-                  c->l_sigma / c->l_D, net ? 1 : 0);
-        LV_LAUNCH(k_lv_smooth, nbK, a, K, 1);
-        LV_LAUNCH(k_lv_smooth, nbK, a, K, 0);
+        LV_LAUNCH(k_lv_net_out, cells(K), a, K, net ? (const float*)c->l_net->y : nullptr, c->l_W,
+                  c->l_sigma / c->l_D, net ? 1 : 0, nbK, c->l_1a1);
+        LV_LAUNCH(k_lv_smooth, cells(K), a, K, 1);
+        LV_LAUNCH(k_lv_smooth, cells(K), a, K, 0);
     } else {
         const TeachVec tv = c->l_tv;
+        LV_LAUNCH(k_lv_cmode_fin, 1, a, nbK, c->l_1a1, c->l_hcap);
         LV_LAUNCH(k_tch_edge, nbK, a, K, tv, 0);
         for (int it = 0; it < c->l_cycles; ++it) {
             for (int l = K; l < NL - 1; ++l) {
-                LV_LAUNCH(k_lv_smooth, blocks(l), a, l, 0);
-                LV_LAUNCH(k_lv_restrict, blocks(l + 1), a, l);
+                LV_LAUNCH(k_lv_smooth, cells(l), a, l, 0);
+                LV_LAUNCH(k_lv_restrict, cells(l + 1), a, l);
             }
             LV_LAUNCH(k_lv_dense, 1, a);
             for (int l = NL - 2; l >= K; --l) {
-                LV_LAUNCH(k_lv_post, blocks(l), a, l);
-                LV_LAUNCH(k_lv_smooth, blocks(l), a, l, 0);
+                LV_LAUNCH(k_lv_post, cells(l), a, l);
+                LV_LAUNCH(k_lv_smooth, cells(l), a, l, 0);
             }
             LV_LAUNCH(k_tch_dots, nbK, a, K, tv, it);
             LV_LAUNCH(k_tch_dir, nbK, a, K, tv, it);
@@ -441,19 +450,19 @@ int learned_iteration(cfd_ctx* c) {
             LV_LAUNCH(k_lv_harvest, nimg, a, K, c->l_hin, c->l_hout, c->l_H, c->l_W, c->l_D);
     }
     for (int l = K - 1; l >= 1; --l) {
-        LV_LAUNCH(k_lv_post, blocks(l), a, l);
-        LV_LAUNCH(k_lv_smooth, blocks(l), a, l, 0);
+        LV_LAUNCH(k_lv_post, cells(l), a, l);
+        LV_LAUNCH(k_lv_smooth, cells(l), a, l, 0);
     }
     if (K >= 1) {
-        LV_LAUNCH(k_lv_post, nb0, a, 0);
-        LV_LAUNCH(k_lv_smooth_dots, nb0, a, v);
+        LV_LAUNCH(k_lv_post, cells(0), a, 0);
+        LV_LAUNCH512(k_lv_smooth_dots, nb0, a, v);
     } else {
-        LV_LAUNCH(k_fcg_dots, nb0, a, v);
+        LV_LAUNCH512(k_fcg_dots, nb0, a, v);
     }
-    LV_LAUNCH(k_fcg_dir, nb0, a, v, nb0);
-    LV_LAUNCH(k_fcg_update, nb0, a, v, nb0);
+    LV_LAUNCH512(k_fcg_dir, nb0, a, v, nb0);
+    LV_LAUNCH512(k_fcg_update, nb0, a, v, nb0);
     LV_LAUNCH(k_fcg_fin, 1, a, nb0);
-    LV_LAUNCH(k_fcg_true, nb0, a, v);
+    LV_LAUNCH512(k_fcg_true, nb0, a, v);
     LV_LAUNCH(k_fcg_true_fin, 1, a, nb0);
     return 0;
 }

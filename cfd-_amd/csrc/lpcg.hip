@@ -62,14 +62,37 @@ __device__ __forceinline__ MGCtx lv_ctx(const LvArgs& a) {
     __syncthreads();                                                           \
     const MGCtx m{(a).mg, (a).code, s_d0_, (a).periodic, (a).ax, (a).ay}
 
+// Work-groups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  xcd_block maps the hardware index to a logical one such that the
+// work-groups of one XCD hold consecutive logical indices: neighbouring row
+// bands then share an L2 instead of being fetched again by another XCD.
+__device__ __forceinline__ int xcd_block(int hw, int total) {
+    const int x = hw & 7;
+    int off = 0;
+    for (int k = 0; k < x; ++k) off += (total - k + 7) >> 3;
+    return off + (hw >> 3);
+}
+
 // f(i, j) over the level's interior cells: a band of rows per work-group,
 // threads along the contiguous j direction
 template <typename F>
 __device__ __forceinline__ void lv_cells(int nx, int ny, F f) {
-    const int i0 = 1 + (int)(((long)nx * blockIdx.x) / gridDim.x);
-    const int i1 = 1 + (int)(((long)nx * (blockIdx.x + 1)) / gridDim.x);
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int i0 = 1 + (int)(((long)nx * b) / gridDim.x);
+    const int i1 = 1 + (int)(((long)nx * (b + 1)) / gridDim.x);
     for (int i = i0; i < i1; ++i)
         for (int j = 1 + (int)threadIdx.x; j <= ny; j += blockDim.x) f(i, j);
+}
+
+// f(i, j) for one cell per thread: work-group = (row, 256-cell chunk of the
+// row), launched with nx * ceil(ny / 256) work-groups of 256 threads
+template <typename F>
+__device__ __forceinline__ void lv_cell1(int nx, int ny, F f) {
+    const int cj = (ny + 255) >> 8;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int i = b / cj + 1;
+    const int j = (b - (i - 1) * cj) * 256 + (int)threadIdx.x + 1;
+    if (i <= nx && j <= ny) f(i, j);
 }
 
 // this work-group's sums of v into slot[blockIdx.x * NV + q]
@@ -144,7 +167,7 @@ __global__ void __launch_bounds__(256) k_lv_smooth(LvArgs a, int l, int from_t) 
     const float w = a.mg->omega;
     const float* src = from_t ? L.t : L.x;
     float* dst = from_t ? L.x : L.t;
-    lv_cells(L.nx, L.ny, [&](int i, int j) {
+    lv_cell1(L.nx, L.ny, [&](int i, int j) {
         const long c = ix(i, j, L.P);
         const float d = mg_diag(m, l, L, c);
         if (d == 0.0f) return;
@@ -155,7 +178,7 @@ __global__ void __launch_bounds__(256) k_lv_smooth(LvArgs a, int l, int from_t) 
 
 // last sweep of the V-cycle on the fine level, z = t = x + w (b - A x) / d,
 // with the partial dot products r.z and r_old.z of the flexible CG
-__global__ void __launch_bounds__(256) k_lv_smooth_dots(LvArgs a, FcgVec v) {
+__global__ void __launch_bounds__(512) k_lv_smooth_dots(LvArgs a, FcgVec v) {
     const FcgState* fs = a.fs;
     if (fs->done) return;
     LV_CTX_LDS(a);
@@ -177,7 +200,7 @@ __global__ void __launch_bounds__(256) k_lv_smooth_dots(LvArgs a, FcgVec v) {
 }
 
 // partial dot products r.z and r_old.z with z = lev[0].t already computed
-__global__ void __launch_bounds__(256) k_fcg_dots(LvArgs a, FcgVec v) {
+__global__ void __launch_bounds__(512) k_fcg_dots(LvArgs a, FcgVec v) {
     const FcgState* fs = a.fs;
     if (fs->done) return;
     const float* z = a.mg->lev[0].t;
@@ -200,7 +223,7 @@ __global__ void __launch_bounds__(256) k_lv_restrict(LvArgs a, int l) {
     const MGLevelDev L = a.mg->lev[l];
     const MGLevelDev C = a.mg->lev[l + 1];
     const float w = a.mg->omega;
-    lv_cells(C.nx, C.ny, [&](int ci, int cj) {
+    lv_cell1(C.nx, C.ny, [&](int ci, int cj) {
         const long cc = ix(ci, cj, C.P);
         float bc = 0.0f;
         for (int di = 0; di < 2; ++di)
@@ -245,7 +268,7 @@ __global__ void __launch_bounds__(256) k_lv_post(LvArgs a, int l) {
         if (mg_diag(m, l, L, c) == 0.0f) return 0.0f;
         return L.t[c] + wc * C.t[ix((p + 1) >> 1, (q + 1) >> 1, C.P)];
     };
-    lv_cells(L.nx, L.ny, [&](int i, int j) {
+    lv_cell1(L.nx, L.ny, [&](int i, int j) {
         const long c = ix(i, j, L.P);
         const float d = mg_diag(m, l, L, c);
         if (d == 0.0f) return;
@@ -297,11 +320,29 @@ __global__ void __launch_bounds__(256) k_lv_cmode_fin(LvArgs a, int nb, double o
     }
 }
 
+// c and s of the current coarse solve from the partials of k_lv_cmode
+__device__ __forceinline__ void lv_cstats(const LvArgs& a, int nb, double one_a_one, double& cm,
+                                          double& s_in) {
+    double s[5];
+    lv_total<5>(a.part + LV_P_CM * LV_SLOT, nb, s);
+    cm = one_a_one > 0.0 ? s[0] / one_a_one : 0.0;
+    const double bb = s[1] - 2.0 * cm * s[2] + cm * cm * s[3];
+    s_in = s[4] > 0.0 ? sqrt(fmax(bb, 0.0) / s[4]) : 0.0;
+}
+
 // network input: the deflated, normalised level-K right-hand side on the
-// padded H x W image (zero outside the fluid cells)
-__global__ void __launch_bounds__(256) k_lv_net_in(LvArgs a, int l, float* img, int H, int W) {
-    const FcgState* fs = a.fs;
+// padded H x W image (zero outside the fluid cells); nb = work-groups of the
+// k_lv_cmode launch
+__global__ void __launch_bounds__(256) k_lv_net_in(LvArgs a, int l, float* img, int H, int W,
+                                                   int nb, double one_a_one) {
+    FcgState* fs = a.fs;
     if (fs->done) return;
+    double cm, s;
+    lv_cstats(a, nb, one_a_one, cm, s);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        fs->cm = cm;
+        fs->s_in = s;
+    }
     LV_CTX_LDS(a);
     const MGLevelDev L = a.mg->lev[l];
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -311,22 +352,23 @@ __global__ void __launch_bounds__(256) k_lv_net_in(LvArgs a, int l, float* img, 
     if (i <= L.nx && j <= L.ny) {
         const long c = ix(i, j, L.P);
         const float d = mg_diag(m, l, L, c);
-        const double s = fs->s_in;
         if (d != 0.0f && s > 0.0)
-            val = (float)(((double)L.b[c] - fs->cm * (double)lv_a1(m, l, L, i, j, d)) / s);
+            val = (float)(((double)L.b[c] - cm * (double)lv_a1(m, l, L, i, j, d)) / s);
     }
     img[k] = val;
 }
 
 // level-K correction t = c + gain s y (gain = sigma / D; use_net = 0: t = c)
 __global__ void __launch_bounds__(256) k_lv_net_out(LvArgs a, int l, const float* y, int W,
-                                                    float gain, int use_net) {
+                                                    float gain, int use_net, int nb,
+                                                    double one_a_one) {
     const FcgState* fs = a.fs;
     if (fs->done) return;
+    double cm, s;
+    lv_cstats(a, nb, one_a_one, cm, s);
     LV_CTX_LDS(a);
     const MGLevelDev L = a.mg->lev[l];
-    const double cm = fs->cm, s = fs->s_in;
-    lv_cells(L.nx, L.ny, [&](int i, int j) {
+    lv_cell1(L.nx, L.ny, [&](int i, int j) {
         const long c = ix(i, j, L.P);
         if (mg_diag(m, l, L, c) == 0.0f) return;
         double t = cm;
@@ -469,7 +511,7 @@ __global__ void __launch_bounds__(256) k_lv_harvest(LvArgs a, int l, float* hin,
 // beta = (r.z - r_old.z) / rho_old (0 at a (re)start) from the dot partials of
 // nb work-groups; d = z + beta d_old, q = A d (neighbours' d rebuilt from z and
 // d_old), partial d.q
-__global__ void __launch_bounds__(256) k_fcg_dir(LvArgs a, FcgVec v, int nb) {
+__global__ void __launch_bounds__(512) k_fcg_dir(LvArgs a, FcgVec v, int nb) {
     FcgState* fs = a.fs;
     if (fs->done) return;
     double dots[2];
@@ -511,7 +553,7 @@ __global__ void __launch_bounds__(256) k_fcg_dir(LvArgs a, FcgVec v, int nb) {
 
 // alpha = rho / d.q; x += alpha d, r_old = r, r -= alpha q; next V-cycle
 // input b0 = r, x0 = w r / d; partial r.r
-__global__ void __launch_bounds__(256) k_fcg_update(LvArgs a, FcgVec v, int nb) {
+__global__ void __launch_bounds__(512) k_fcg_update(LvArgs a, FcgVec v, int nb) {
     FcgState* fs = a.fs;
     if (fs->done) return;
     double pq[1];
@@ -572,7 +614,7 @@ __global__ void __launch_bounds__(256) k_fcg_fin(LvArgs a, int nb) {
 
 // true residual r = b - A x once the recursive one has converged (or at
 // maxiter), with the next V-cycle input; partial r.r
-__global__ void __launch_bounds__(256) k_fcg_true(LvArgs a, FcgVec v) {
+__global__ void __launch_bounds__(512) k_fcg_true(LvArgs a, FcgVec v) {
     FcgState* fs = a.fs;
     if (fs->done || !fs->pending) return;
     LV_CTX_LDS(a);

@@ -52,11 +52,12 @@ __global__ void k_mfma_layout_check(const float* A, const float* B, float* D) {
 // Output either bf16 (Yb) or fp32 (Yf); with fp32 output, channels >= csplit
 // go to Yf2 (channel stride Cout - csplit), the others to Yf (stride csplit):
 // the gradient of a concatenated input lands in its two parts.
-// One wave computes a 16-pixel tile (of one image row) times G 16-channel
-// output chunks; consecutive waves take the other chunks of the same tile.
-// The K loop runs over (tap, 16-channel input chunk) steps, loading the
-// operands of 8 steps before their matrix-core instructions (small images
-// leave few waves per CU, so load latency rather than issue rate limits).
+// One work-group computes a 16-pixel tile (of one image row) times G
+// 16-channel output chunks; its 4 waves split the K loop over (tap, 16-channel
+// input chunk) steps in batches of 8 (operands of a batch loaded before its
+// matrix-core instructions) and add their accumulators through LDS: on the
+// small images of the coarse levels the chains of dependent loads, not the
+// matrix cores, set the time.
 // ---------------------------------------------------------------------------
 template <int G>
 __global__ void __launch_bounds__(256) k_conv3x3_mfma(
@@ -65,12 +66,13 @@ __global__ void __launch_bounds__(256) k_conv3x3_mfma(
     unsigned short* __restrict__ Yb, float* __restrict__ Yf, float* __restrict__ Yf2, int csplit,
     int mode, const int* skip) {
     if (skip && *skip) return;
+    __shared__ f32x4 red[3][G][64];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int tiles_w = (Wd + 15) / 16;
     const int ngrp = Cout / (16 * G);
-    const long task = (long)blockIdx.x * 4 + wave;
-    if (task >= (long)tiles_w * H * ngrp) return;
+    const long task = blockIdx.x;
+    if (task >= (long)tiles_w * H * ngrp) return;   // uniform over the work-group
     const int grp = (int)(task % ngrp);
     const long tile = task / ngrp;
     const int h = (int)(tile / tiles_w);
@@ -99,9 +101,8 @@ __global__ void __launch_bounds__(256) k_conv3x3_mfma(
             b[g] = *reinterpret_cast<const bf16x4*>(
                 Wt + ((long)t * Cout + (grp * G + g) * 16 + m) * Cin + k0 + kq);
     };
-    // U steps of operands in flight at a time (steps past the end load zeros)
     constexpr int U = 8;
-    for (int s0 = 0; s0 < nsteps; s0 += U) {
+    for (int s0 = wave * U; s0 < nsteps; s0 += 4 * U) {
         bf16x4 a[U], b[U][G];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -119,6 +120,21 @@ __global__ void __launch_bounds__(256) k_conv3x3_mfma(
             for (int g = 0; g < G; ++g)
                 acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], b[u][g], acc[g], 0, 0, 0);
     }
+    if (wave > 0)
+#pragma unroll
+        for (int g = 0; g < G; ++g) red[wave - 1][g][lane] = acc[g];
+    __syncthreads();
+    if (wave > 0) return;
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const f32x4 o = red[q][g][lane];
+            acc[g][0] += o[0];
+            acc[g][1] += o[1];
+            acc[g][2] += o[2];
+            acc[g][3] += o[3];
+        }
     // D: lane holds pixels w0 + kq + j (j = 0..3) of channel (grp*G + g)*16 + m
 #pragma unroll
     for (int g = 0; g < G; ++g) {

@@ -102,11 +102,12 @@ int conv_mfma(cfd_unet* u, const unsigned short* x1, int c1, const unsigned shor
               const int* skip) {
     if (cout % 16 || c1 % 16 || c2 % 16 || cout > 128)
         return set_err(CFD_ERR_ARG, "unsupported channel counts %d+%d -> %d", c1, c2, cout);
-    // two output chunks per wave, unless that leaves fewer than 512 waves
+    // one work-group per (tile, output group); two 16-channel output chunks per
+    // group unless that leaves fewer than 1024 work-groups
     const long tiles = (long)H * ((W + 15) / 16);
-    const int G = (cout % 32 == 0 && tiles * (cout / 32) >= 512) ? 2 : 1;
+    const int G = (cout % 32 == 0 && tiles * (cout / 32) >= 1024) ? 2 : 1;
     const long tasks = tiles * (cout / (16 * G));
-    const dim3 grid((unsigned)((tasks + 3) / 4));
+    const dim3 grid((unsigned)tasks);
     if (G == 2)
         hipLaunchKernelGGL(k_conv3x3_mfma<2>, grid, dim3(256), 0, u->stream, x1, c1, x2, c2, wt,
                            bias, H, W, cout, yb, yf, yf2, csplit, mode, skip);
