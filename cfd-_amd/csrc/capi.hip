@@ -418,7 +418,7 @@ int learned_iteration(cfd_ctx* c) {
             const int* k0 = u->skip;
             u->stream = c->stream;
             u->skip = &c->fcg->done;
-            const int rc = unet_forward(u);
+            const int rc = unet_forward(u, true);
             u->stream = s0;
             u->skip = k0;
             TRY(rc);
@@ -461,8 +461,7 @@ int learned_iteration(cfd_ctx* c) {
     }
     LV_LAUNCH512(k_fcg_dir, nb0, a, v, nb0);
     LV_LAUNCH512(k_fcg_update, nb0, a, v, nb0);
-    LV_LAUNCH(k_fcg_fin, 1, a, nb0);
-    LV_LAUNCH512(k_fcg_true, nb0, a, v);
+    LV_LAUNCH512(k_fcg_true, nb0, a, v, nb0);
     LV_LAUNCH(k_fcg_true_fin, 1, a, nb0);
     return 0;
 }

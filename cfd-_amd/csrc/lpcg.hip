@@ -590,33 +590,30 @@ __global__ void __launch_bounds__(512) k_fcg_update(LvArgs a, FcgVec v, int nb) 
     lv_partial<1>(s, a.part + LV_P_RR * LV_SLOT);
 }
 
-// one work-group: end of an iteration (recursive residual test)
-__global__ void __launch_bounds__(256) k_fcg_fin(LvArgs a, int nb) {
+// End of an iteration: every work-group adds the r.r partials of
+// k_fcg_update and applies the recursive-residual test (work-group 0 records
+// it); once it passes (or at maxiter) the true residual r = b - A x is formed
+// with the next V-cycle input, and its partial r.r stored for k_fcg_true_fin.
+__global__ void __launch_bounds__(512) k_fcg_true(LvArgs a, FcgVec v, int nb) {
     FcgState* fs = a.fs;
     if (fs->done) return;
     double rr[1];
     lv_total<1>(a.part + LV_P_RR * LV_SLOT, nb, rr);
-    if (threadIdx.x == 0) {
-        const int k = fs->k;
+    const int k = fs->k + 1;   // fs->k itself advances in k_fcg_true_fin
+    const bool nan = !(rr[0] == rr[0]);
+    const bool pending = !nan && (sqrt(rr[0]) < fs->atol || k >= fs->maxiter);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {   // fields no work-group reads here
         fs->rr = rr[0];
         fs->rho_old = fs->rho;
-        fs->k = k + 1;
-        fs->iters = k + 1;
+        fs->iters = k;
         fs->first = 0;
-        if (!(rr[0] == rr[0])) {
+        fs->pending = pending ? 1 : 0;
+        if (nan) {
             fs->done = 1;
             fs->info = -1;
-        } else if (sqrt(rr[0]) < fs->atol || k + 1 >= fs->maxiter) {
-            fs->pending = 1;
         }
     }
-}
-
-// true residual r = b - A x once the recursive one has converged (or at
-// maxiter), with the next V-cycle input; partial r.r
-__global__ void __launch_bounds__(512) k_fcg_true(LvArgs a, FcgVec v) {
-    FcgState* fs = a.fs;
-    if (fs->done || !fs->pending) return;
+    if (!pending) return;
     LV_CTX_LDS(a);
     const MGLevelDev L0 = a.mg->lev[0];
     const float w = a.mg->omega;
@@ -635,10 +632,13 @@ __global__ void __launch_bounds__(512) k_fcg_true(LvArgs a, FcgVec v) {
     lv_partial<1>(s, a.part + LV_P_RR * LV_SLOT);
 }
 
-// one work-group: finish, or restart the flexible CG from the true residual
+// one work-group: advance the iteration count; after a true residual, finish
+// or restart the flexible CG from it
 __global__ void __launch_bounds__(256) k_fcg_true_fin(LvArgs a, int nb) {
     FcgState* fs = a.fs;
-    if (fs->done || !fs->pending) return;
+    if (fs->done) return;
+    if (threadIdx.x == 0) fs->k += 1;
+    if (!fs->pending) return;
     double rr[1];
     lv_total<1>(a.part + LV_P_RR * LV_SLOT, nb, rr);
     if (threadIdx.x == 0) {

@@ -59,12 +59,15 @@ __global__ void k_mfma_layout_check(const float* A, const float* B, float* D) {
 // small images of the coarse levels the chains of dependent loads, not the
 // matrix cores, set the time.
 // ---------------------------------------------------------------------------
+// rs1 (resampling of X1): 0 X1 is H x W; 1 X1 is 2H x 2W and is read through
+// 2x2 average pooling (k_avgpool2's arithmetic); 2 X1 is H/2 x W/2 and is read
+// through nearest upsampling.
 template <int G>
 __global__ void __launch_bounds__(256) k_conv3x3_mfma(
     const unsigned short* __restrict__ X1, int C1, const unsigned short* __restrict__ X2, int C2,
     const unsigned short* __restrict__ Wt, const float* __restrict__ bias, int H, int Wd, int Cout,
     unsigned short* __restrict__ Yb, float* __restrict__ Yf, float* __restrict__ Yf2, int csplit,
-    int mode, const int* skip) {
+    int mode, const int* skip, int rs1) {
     if (skip && *skip) return;
     __shared__ f32x4 red[3][G][64];
     const int lane = threadIdx.x & 63;
@@ -91,10 +94,27 @@ __global__ void __launch_bounds__(256) k_conv3x3_mfma(
         const int ww = w0 + m + t % 3 - 1;
         a = (bf16x4){0, 0, 0, 0};
         if (hh >= 0 && hh < H && ww >= 0 && ww < Wd) {
-            const long pix = (long)hh * Wd + ww;
-            const unsigned short* src = k0 < C1 ? X1 + pix * C1 + k0 + kq
-                                                : X2 + pix * C2 + (k0 - C1) + kq;
-            a = *reinterpret_cast<const bf16x4*>(src);
+            if (k0 >= C1) {
+                a = *reinterpret_cast<const bf16x4*>(X2 + ((long)hh * Wd + ww) * C2 + (k0 - C1) + kq);
+            } else if (rs1 == 0) {
+                a = *reinterpret_cast<const bf16x4*>(X1 + ((long)hh * Wd + ww) * C1 + k0 + kq);
+            } else if (rs1 == 2) {
+                a = *reinterpret_cast<const bf16x4*>(
+                    X1 + ((long)(hh >> 1) * (Wd >> 1) + (ww >> 1)) * C1 + k0 + kq);
+            } else {
+                const long W2 = 2L * Wd;
+                const unsigned short* p0 = X1 + ((long)(2 * hh) * W2 + 2 * ww) * C1 + k0 + kq;
+                const bf16x4 q0 = *reinterpret_cast<const bf16x4*>(p0);
+                const bf16x4 q1 = *reinterpret_cast<const bf16x4*>(p0 + C1);
+                const bf16x4 q2 = *reinterpret_cast<const bf16x4*>(p0 + W2 * C1);
+                const bf16x4 q3 = *reinterpret_cast<const bf16x4*>(p0 + W2 * C1 + C1);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float sm = bf2f((unsigned short)q0[e]) + bf2f((unsigned short)q1[e]) +
+                                     bf2f((unsigned short)q2[e]) + bf2f((unsigned short)q3[e]);
+                    a[e] = (short)f2bf(0.25f * sm);
+                }
+            }
         }
 #pragma unroll
         for (int g = 0; g < G; ++g)

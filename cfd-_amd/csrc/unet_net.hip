@@ -99,7 +99,7 @@ inline unsigned nblk(long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs
 int conv_mfma(cfd_unet* u, const unsigned short* x1, int c1, const unsigned short* x2, int c2,
               const unsigned short* wt, const float* bias, int H, int W, int cout,
               unsigned short* yb, float* yf, float* yf2, int csplit, int mode,
-              const int* skip) {
+              const int* skip, int rs1 = 0) {
     if (cout % 16 || c1 % 16 || c2 % 16 || cout > 128)
         return set_err(CFD_ERR_ARG, "unsupported channel counts %d+%d -> %d", c1, c2, cout);
     // one work-group per (tile, output group); two 16-channel output chunks per
@@ -110,21 +110,34 @@ int conv_mfma(cfd_unet* u, const unsigned short* x1, int c1, const unsigned shor
     const dim3 grid((unsigned)tasks);
     if (G == 2)
         hipLaunchKernelGGL(k_conv3x3_mfma<2>, grid, dim3(256), 0, u->stream, x1, c1, x2, c2, wt,
-                           bias, H, W, cout, yb, yf, yf2, csplit, mode, skip);
+                           bias, H, W, cout, yb, yf, yf2, csplit, mode, skip, rs1);
     else
         hipLaunchKernelGGL(k_conv3x3_mfma<1>, grid, dim3(256), 0, u->stream, x1, c1, x2, c2, wt,
-                           bias, H, W, cout, yb, yf, yf2, csplit, mode, skip);
+                           bias, H, W, cout, yb, yf, yf2, csplit, mode, skip, rs1);
     HIPC(hipGetLastError());
     return 0;
 }
 
 const unsigned short* act_in(cfd_unet* u, int id) { return id < 0 ? nullptr : u->act[id]; }
 
-int unet_forward(cfd_unet* u) {
+// infer: the pooled and upsampled tensors are not stored; the convolutions
+// that consume them read their sources through the same resampling (the
+// results are identical; the backward pass needs the stored tensors)
+int unet_forward(cfd_unet* u, bool infer = false) {
     using namespace unet;
     for (int l = 0; l < NL; ++l) {
         const Layer& Ly = u->L[l];
         const int H = u->hs[Ly.level], W = u->ws[Ly.level];
+        const bool pool_in = l == 2 || l == 4 || l == 6;
+        const bool up_in = l == 8 || l == 10 || l == 12;
+        if (infer && Ly.type == L_MFMA && (pool_in || up_in)) {
+            const int src = pool_in ? (l == 2 ? A0B : (l == 4 ? A1B : A2B))
+                                    : (l == 8 ? A3B : (l == 10 ? B2B : B1B));
+            TRY(conv_mfma(u, u->act[src], Ly.cin1, act_in(u, LIO[l].x2), Ly.cin2,
+                          u->wb + Ly.woff, u->w + Ly.boff, H, W, Ly.cout, u->act[LIO[l].y],
+                          nullptr, nullptr, Ly.cout, 3, u->skip, pool_in ? 1 : 2));
+            continue;
+        }
         // resampling feeding this layer
         if (l == 2 || l == 4 || l == 6) {
             const int src = l == 2 ? A0B : (l == 4 ? A1B : A2B);
@@ -368,7 +381,7 @@ int cfd_unet_forward(cfd_unet* u, const float* x_dev, float* y_dev) {
     if (!u || !x_dev || !y_dev) return set_err(CFD_ERR_ARG, "null argument");
     const long n = (long)u->H * u->W;
     if (x_dev != u->xin) HIPC(hipMemcpyAsync(u->xin, x_dev, n * sizeof(float), hipMemcpyDeviceToDevice, u->stream));
-    TRY(unet_forward(u));
+    TRY(unet_forward(u, true));
     if (y_dev != u->y) HIPC(hipMemcpyAsync(y_dev, u->y, n * sizeof(float), hipMemcpyDeviceToDevice, u->stream));
     return 0;
 }
