@@ -13,6 +13,10 @@ transport, monitors), all inputs resident in HBM.  Solver legs:
 * ``cg_learned_none``  the same solver without the network (the gain it brings);
 * ``cg_mixed``         FP64 flexible CG with the FP32 multigrid V-cycle.
 
+``ensemble`` adds the batched-ensemble workload (256 lid / heated cavities of
+sides 64-512, FP32 bulk, shared learned PCG) as aggregate cell updates per
+second, with and without the network.
+
 Prints one JSON line.  ``roofline`` describes the pressure projection of the
 headline leg (it takes nearly all of the step): algorithmic bytes per
 projection (per cell: 80 B for the right-hand side, pressure write-back and
@@ -172,6 +176,35 @@ def roofline_for(leg: dict) -> dict:
             "launch_ms": round(leg["proj_ms_mean"], 4), "cg_iters_per_launch": round(n_it, 2)}
 
 
+def run_ensemble(steps: int = 10, warmup: int = 3, net: bool = True) -> dict:
+    """The batched-ensemble workload: 256 lid / heated cavities of sides
+    {64, 128, 256, 512} (seed 0), FP32 bulk, shared learned PCG; aggregate
+    cell updates per second over `steps` steps after `warmup`."""
+    from cfd2d_amd.ensemble import Ensemble, sample_instances
+
+    ins = sample_instances(256, 0)
+    t0 = time.perf_counter()
+    ens = Ensemble(ins, net=net)
+    setup = time.perf_counter() - t0
+    ens.step(warmup)
+    s0 = ens.stats(0)["sweeps_total"]
+    t0 = time.perf_counter()
+    ens.step(steps)
+    wall = time.perf_counter() - t0
+    sweeps = ens.stats(0)["sweeps_total"] - s0
+    rel = max(ens.stats(k)["relres"] for k in range(len(ins)))
+    out = {"instances": len(ins), "cells": ens.cells, "steps": steps, "warmup": warmup,
+           "ms_per_step": round(1e3 * wall / steps, 3),
+           "cell_updates_per_s": round(ens.cells * steps / wall, 1),
+           "solver_iterations_per_step": round(sweeps / steps, 2),
+           "relres_max": rel, "setup_s": round(setup, 2),
+           "predictor_shapes": {str(ins[c[0]].n): [ens.stats(c[0])["pred_tx"],
+                                                    ens.stats(c[0])["pred_ty"]]
+                                for c in ens.classes}}
+    ens.close()
+    return out
+
+
 # --------------------------------------------------------------------------- #
 # CPU baseline: the FP64 oracle (NumPy/SciPy + serial C kernels) on one core
 # --------------------------------------------------------------------------- #
@@ -236,6 +269,7 @@ def main() -> None:
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--legs", default="cg,cg_learned,cg_learned_none,cg_mixed")
     ap.add_argument("--cpu-child", nargs=2, default=None)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -274,6 +308,14 @@ def main() -> None:
         sys.exit(1)
     base = good("cg")
     none = good("cg_learned_none")
+    ensemble = None
+    if not args.no_ensemble:
+        try:
+            ensemble = run_ensemble(net=True)
+            ensemble["without_network"] = run_ensemble(net=False)
+        except (ValueError, RuntimeError) as e:
+            ensemble = {"error": str(e)}
+        print(f"ensemble: {json.dumps(ensemble)}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "MAC steps/s @1024² cavity: learned-PCG fp32 vs plain fp64 CG "
@@ -301,6 +343,7 @@ def main() -> None:
                           "iters_with": round(head["cg_iters_mean"], 2),
                           "steps_per_s_without": round(none["steps_per_s"], 3)}
                          if none and head["solver"] == "cg_learned" else None),
+        "ensemble": ensemble,
         "legs": legs,
     }
     print(json.dumps(line))

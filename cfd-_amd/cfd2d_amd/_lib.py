@@ -76,6 +76,30 @@ class LearnedStats(ctypes.Structure):
                 ("last_iters", _i32), ("last_info", _i32), ("last_relres", _d)]
 
 
+ENS_MAX_CLASSES = 8
+ENS_MAX_LEVELS = 12
+
+
+class EnsClassDesc(ctypes.Structure):
+    _fields_ = [("n", _i32), ("B", _i32), ("nlev", _i32), ("nu", _vp), ("lid", _vp),
+                ("kappa", _vp), ("lev_n", _i32 * ENS_MAX_LEVELS), ("diag", _vp * ENS_MAX_LEVELS),
+                ("ce", _vp * ENS_MAX_LEVELS), ("cn", _vp * ENS_MAX_LEVELS), ("cinv", _vp),
+                ("one_a_one", _d)]
+
+
+class EnsDesc(ctypes.Structure):
+    _fields_ = [("nclass", _i32), ("cls", EnsClassDesc * ENS_MAX_CLASSES), ("tau", _d),
+                ("rtol", _d), ("maxiter", _i32), ("omega", ctypes.c_float),
+                ("wc", ctypes.c_float), ("autotune", _i32), ("use_graph", _i32)]
+
+
+class EnsStats(ctypes.Structure):
+    _fields_ = [("t", _d), ("dt", _d), ("gamma", _d), ("kinetic_energy", _d),
+                ("divergence", _d), ("residual", _d), ("relres", _d), ("steps", _i64),
+                ("sweeps_total", _i64), ("iters", _i32), ("info", _i32), ("pred_tx", _i32),
+                ("pred_ty", _i32), ("pred_us", ctypes.c_float)]
+
+
 class State(ctypes.Structure):
     _fields_ = [("t", _d), ("dt", _d), ("dt_old", _d), ("gamma", _d), ("flow_force", _d),
                 ("n", _i64), ("has_old", _i32), ("done", _i32), ("error", _i32), ("pad", _i32)]
@@ -134,6 +158,12 @@ def lib() -> ctypes.CDLL:
         "cfd_set_learned": ([_vp, P(LearnedDesc)], _i32),
         "cfd_learned_stats": ([_vp, P(LearnedStats)], _i32),
         "cfd_learned_reset": ([_vp], _i32),
+        "cfd_ens_create": ([P(EnsDesc), P(_vp)], _i32),
+        "cfd_ens_destroy": ([_vp], _i32),
+        "cfd_ens_set_net": ([_vp, _vp, ctypes.c_float], _i32),
+        "cfd_ens_step": ([_vp, _i32], _i32),
+        "cfd_ens_download": ([_vp, _i32, _i32, _i32, _vp], _i32),
+        "cfd_ens_stats": ([_vp, _i32, _i32, P(EnsStats)], _i32),
         "cfd_unet_param_count": ([], _i64),
         "cfd_unet_create": ([_i32, _i32, P(_vp)], _i32),
         "cfd_unet_destroy": ([_vp], _i32),
@@ -145,6 +175,7 @@ def lib() -> ctypes.CDLL:
         "cfd_unet_loss_grad": ([_vp, _vp, _vp, _vp, _d, P(_d)], _i32),
         "cfd_unet_adam": ([_vp, ctypes.c_float, ctypes.c_float], _i32),
         "cfd_unet_set_loss": ([_vp, _i32, _i32, _i32, ctypes.c_float], _i32),
+        "cfd_unet_set_batch": ([_vp, _i32], _i32),
         "cfd_mfma_layout_check": ([_vp, _vp, _vp], _i32),
     }
     for name, (args, res) in sig.items():
@@ -165,11 +196,13 @@ def exported_symbols() -> list[str]:
                         "cfd_pressure_solve", "cfd_dev_alloc",
                         "cfd_dev_free", "cfd_memcpy", "cfd_momentum_tendencies",
                         "cfd_obstacle_bc", "cfd_scalar_transport", "cfd_set_learned",
-                        "cfd_learned_stats", "cfd_learned_reset", "cfd_unet_param_count",
+                        "cfd_learned_stats", "cfd_learned_reset", "cfd_ens_create",
+                        "cfd_ens_destroy", "cfd_ens_set_net", "cfd_ens_step", "cfd_ens_download",
+                        "cfd_ens_stats", "cfd_unet_param_count",
                         "cfd_unet_create", "cfd_unet_destroy", "cfd_unet_set_params",
                         "cfd_unet_get_params", "cfd_unet_get_grad", "cfd_unet_zero_grad",
                         "cfd_unet_forward", "cfd_unet_loss_grad", "cfd_unet_adam",
-                        "cfd_unet_set_loss",
+                        "cfd_unet_set_loss", "cfd_unet_set_batch",
                         "cfd_mfma_layout_check")]
 
 

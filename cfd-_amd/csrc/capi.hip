@@ -16,6 +16,7 @@
 #include "pcg.hip"
 #include "unet.hip"
 #include "lpcg.hip"
+#include "ens.hip"
 
 namespace {
 
@@ -550,6 +551,8 @@ int run_learned(cfd_ctx* c, ProjArgs a) {
 }
 
 }  // namespace
+
+#include "ens_host.hip"
 
 // ===========================================================================
 // C ABI

@@ -1,0 +1,426 @@
+// Host side of the batched cavity ensemble (ens.hip): classes of equal-size
+// instances, one launch sequence per step for all of them, the learned
+// flexible CG with one batched network pass per iteration, and the predictor
+// tile shape chosen per class by timing the candidates at setup.
+
+struct cfd_ens {
+    int ncls = 0;
+    EnsClass h[CFD_ENS_MAX_CLASSES];
+    EnsClass* d = nullptr;              // device copies
+    int pred_tx[CFD_ENS_MAX_CLASSES], pred_ty[CFD_ENS_MAX_CLASSES];
+    float pred_us[CFD_ENS_MAX_CLASSES];
+    int nimg = 0;
+    float* img = nullptr;               // network input batch (nimg x 32 x 32)
+    cfd_unet* net = nullptr;
+    int use_net = 0;
+    int* notdone = nullptr;
+    int* notdone_h = nullptr;           // pinned
+    int par = 0;
+    long long steps = 0;
+    int prev_iters = 0;
+    int use_graph = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_x = nullptr;
+    long long iters_total = 0, solves = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+template <typename T>
+int ealloc(cfd_ens* e, T** p, size_t n) {
+    void* q = nullptr;
+    HIPC(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
+    HIPC(hipMemset(q, 0, std::max<size_t>(n, 1) * sizeof(T)));
+    e->allocs.push_back(q);
+    *p = (T*)q;
+    return 0;
+}
+
+#define E_LAUNCH(kern, grid, block, ...)                                              \
+    do {                                                                              \
+        hipLaunchKernelGGL(kern, grid, block, 0, e->stream, __VA_ARGS__);             \
+        HIPC(hipGetLastError());                                                      \
+    } while (0)
+
+unsigned cells1(const EnsLevel& L) { return (unsigned)(L.n * ((L.n + 255) / 256)); }
+
+int ens_pred(cfd_ens* e, int k, int tx, int ty) {
+    const EnsClass& c = e->h[k];
+    const dim3 grid((c.n + 2 + tx - 1) / tx, (c.n + 2 + ty - 1) / ty, c.B);
+    E_LAUNCH(k_e_pred, grid, dim3(tx, ty), e->d + k, e->par);
+    return 0;
+}
+
+int ens_iteration(cfd_ens* e) {
+    for (int k = 0; k < e->ncls; ++k) {
+        const EnsClass& c = e->h[k];
+        EnsClass* dc = e->d + k;
+        for (int l = 0; l < c.K; ++l) {
+            E_LAUNCH(k_e_smooth, dim3(cells1(c.lev[l]), c.B), dim3(256), dc, l, 0);
+            E_LAUNCH(k_e_restrict, dim3(cells1(c.lev[l + 1]), c.B), dim3(256), dc, l);
+        }
+        E_LAUNCH(k_e_cmode, dim3(ENS_BANDS, c.B), dim3(256), dc);
+        E_LAUNCH(k_e_net_in, dim3(4, c.B), dim3(256), dc);
+    }
+    if (e->use_net) {
+        cfd_unet* u = e->net;
+        const hipStream_t s0 = u->stream;
+        u->stream = e->stream;
+        const int rc = unet_forward(u, true);
+        u->stream = s0;
+        TRY(rc);
+    }
+    for (int k = 0; k < e->ncls; ++k) {
+        const EnsClass& c = e->h[k];
+        EnsClass* dc = e->d + k;
+        const EnsLevel& LK = c.lev[c.K];
+        E_LAUNCH(k_e_net_out, dim3(cells1(LK), c.B), dim3(256), dc);
+        E_LAUNCH(k_e_smooth, dim3(cells1(LK), c.B), dim3(256), dc, c.K, 1);
+        E_LAUNCH(k_e_smooth, dim3(cells1(LK), c.B), dim3(256), dc, c.K, 0);
+        for (int l = c.K - 1; l >= 1; --l) {
+            E_LAUNCH(k_e_post, dim3(cells1(c.lev[l]), c.B), dim3(256), dc, l);
+            E_LAUNCH(k_e_smooth, dim3(cells1(c.lev[l]), c.B), dim3(256), dc, l, 0);
+        }
+        E_LAUNCH(k_e_post, dim3(cells1(c.lev[0]), c.B), dim3(256), dc, 0);
+        E_LAUNCH(k_e_smooth_dots, dim3(ENS_BANDS, c.B), dim3(256), dc);
+        E_LAUNCH(k_e_dir, dim3(ENS_BANDS, c.B), dim3(256), dc);
+        E_LAUNCH(k_e_update, dim3(ENS_BANDS, c.B), dim3(256), dc);
+        E_LAUNCH(k_e_true, dim3(ENS_BANDS, c.B), dim3(256), dc);
+        E_LAUNCH(k_e_true_fin, dim3(1), dim3(256), dc);
+    }
+    return 0;
+}
+
+void ens_drop_graph(cfd_ens* e) {
+    if (e->graph_x) hipGraphExecDestroy(e->graph_x);
+    if (e->graph) hipGraphDestroy(e->graph);
+    e->graph_x = nullptr;
+    e->graph = nullptr;
+}
+
+// one solver iteration of every class, replayed from a captured graph
+int ens_iteration_launch(cfd_ens* e) {
+    if (!e->use_graph) return ens_iteration(e);
+    if (!e->graph_x) {
+        HIPC(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
+        const int rc = ens_iteration(e);
+        hipGraph_t g = nullptr;
+        const hipError_t err = hipStreamEndCapture(e->stream, &g);
+        if (rc) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        HIPC(err);
+        e->graph = g;
+        HIPC(hipGraphInstantiate(&e->graph_x, g, nullptr, nullptr, 0));
+    }
+    HIPC(hipGraphLaunch(e->graph_x, e->stream));
+    return 0;
+}
+
+int ens_notdone(cfd_ens* e, int* total) {
+    HIPC(hipMemcpyAsync(e->notdone_h, e->notdone, e->ncls * sizeof(int), hipMemcpyDeviceToHost,
+                        e->stream));
+    HIPC(hipStreamSynchronize(e->stream));
+    int t = 0;
+    for (int k = 0; k < e->ncls; ++k) t += e->notdone_h[k];
+    *total = t;
+    return 0;
+}
+
+int ens_step(cfd_ens* e) {
+    const int par = e->par;
+    for (int k = 0; k < e->ncls; ++k) {
+        const EnsClass& c = e->h[k];
+        EnsClass* dc = e->d + k;
+        E_LAUNCH(k_e_bc, dim3(c.B), dim3(256), dc);
+        E_LAUNCH(k_e_maxuv, dim3(ENS_BANDS, c.B), dim3(256), dc);
+        E_LAUNCH(k_e_dt, dim3(1), dim3(256), dc);
+        TRY(ens_pred(e, k, e->pred_tx[k], e->pred_ty[k]));
+        E_LAUNCH(k_e_rhs, dim3(ENS_BANDS, c.B), dim3(256), dc);
+        E_LAUNCH(k_e_res, dim3(ENS_BANDS, c.B), dim3(256), dc);
+        E_LAUNCH(k_e_init, dim3(1), dim3(256), dc);
+    }
+    // iterations until every instance has finished (polled after the
+    // predicted count, then after each iteration)
+    int batch = std::max(1, e->prev_iters - 1), launched = 0, left = 1, maxit = 0;
+    for (int k = 0; k < e->ncls; ++k) maxit = std::max(maxit, e->h[k].maxiter);
+    for (;;) {
+        for (int q = 0; q < batch; ++q) TRY(ens_iteration_launch(e));
+        launched += batch;
+        TRY(ens_notdone(e, &left));
+        if (left == 0 || launched > maxit + 2) break;
+        batch = 1;
+    }
+    e->prev_iters = launched;
+    e->iters_total += launched;
+    e->solves += 1;
+    for (int k = 0; k < e->ncls; ++k) {
+        const EnsClass& c = e->h[k];
+        EnsClass* dc = e->d + k;
+        E_LAUNCH(k_e_xsum, dim3(ENS_BANDS, c.B), dim3(256), dc);
+        E_LAUNCH(k_e_pwrite, dim3(ENS_BANDS, c.B), dim3(256), dc);
+        E_LAUNCH(k_e_correct, dim3(ENS_BANDS, c.B), dim3(256), dc);
+        E_LAUNCH(k_e_bc, dim3(c.B), dim3(256), dc);
+        const dim3 gt((c.n + 2 + e->pred_tx[k] - 1) / e->pred_tx[k],
+                      (c.n + 2 + e->pred_ty[k] - 1) / e->pred_ty[k], c.B);
+        E_LAUNCH(k_e_temp, gt, dim3(e->pred_tx[k], e->pred_ty[k]), dc, par);
+        E_LAUNCH(k_e_tghost, dim3(c.B), dim3(256), dc, par);
+        E_LAUNCH(k_e_record, dim3(ENS_BANDS, c.B), dim3(256), dc);
+        E_LAUNCH(k_e_advance, dim3(1), dim3(256), dc);
+    }
+    e->par ^= 1;
+    e->steps += 1;
+    return 0;
+}
+
+// Autotuner: the predictor's work-group shape per class, the fastest of the
+// candidates over 5 timed launches each (the fields are untouched: the
+// predictor only writes F, G and the tendency buffer of this parity).
+int ens_autotune(cfd_ens* e) {
+    static const int cand[][2] = {{64, 4}, {128, 2}, {256, 1}, {64, 8}, {128, 4}, {32, 8}};
+    hipEvent_t a, b;
+    HIPC(hipEventCreate(&a));
+    HIPC(hipEventCreate(&b));
+    for (int k = 0; k < e->ncls; ++k) {
+        float best = 1e30f;
+        for (auto& cd : cand) {
+            TRY(ens_pred(e, k, cd[0], cd[1]));   // warm
+            HIPC(hipEventRecord(a, e->stream));
+            for (int r = 0; r < 5; ++r) TRY(ens_pred(e, k, cd[0], cd[1]));
+            HIPC(hipEventRecord(b, e->stream));
+            HIPC(hipEventSynchronize(b));
+            float ms = 0.f;
+            HIPC(hipEventElapsedTime(&ms, a, b));
+            if (ms < best) {
+                best = ms;
+                e->pred_tx[k] = cd[0];
+                e->pred_ty[k] = cd[1];
+            }
+        }
+        e->pred_us[k] = 1e3f * best / 5;
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cfd_ens_create(const cfd_ens_desc* d, cfd_ens** out) {
+    if (!d || !out || d->nclass < 1 || d->nclass > CFD_ENS_MAX_CLASSES)
+        return set_err(CFD_ERR_ARG, "ensemble needs 1..%d classes", CFD_ENS_MAX_CLASSES);
+    cfd_ens* e = new cfd_ens();
+    auto fail = [&](int rc) {
+        for (void* p : e->allocs) hipFree(p);
+        if (e->notdone_h) hipHostFree(e->notdone_h);
+        if (e->stream) hipStreamDestroy(e->stream);
+        delete e;
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_err(CFD_ERR_HIP, "stream creation failed"));
+    e->ncls = d->nclass;
+    int rc = 0;
+    int img = 0;
+    for (int k = 0; k < e->ncls; ++k) {
+        const cfd_ens_class_desc& cd = d->cls[k];
+        EnsClass& c = e->h[k];
+        std::memset(&c, 0, sizeof(c));
+        if (cd.n < 32 || cd.n % 32 || (cd.n & (cd.n - 1)) || cd.B < 1 || cd.B > 4096)
+            return fail(set_err(CFD_ERR_ARG, "class %d: n %d (a power of two >= 32), B %d", k,
+                                cd.n, cd.B));
+        if (cd.nlev < 2 || cd.nlev > ENS_MAXLEV || !cd.cinv)
+            return fail(set_err(CFD_ERR_ARG, "class %d: hierarchy of %d levels", k, cd.nlev));
+        c.n = cd.n;
+        c.P = ((cd.n + 2 + 31) / 32) * 32;
+        c.B = cd.B;
+        c.S = (long)(cd.n + 2) * c.P;
+        c.h = 1.0f / cd.n;
+        c.hd = 1.0 / cd.n;
+        c.ax = (float)cd.n * (float)cd.n;
+        c.axd = (double)cd.n * cd.n;
+        c.tau = d->tau;
+        c.nlev = cd.nlev;
+        int K = 0;
+        while ((cd.n >> K) > 32) ++K;
+        c.K = K;
+        if (K >= cd.nlev) return fail(set_err(CFD_ERR_ARG, "class %d: no 32x32 level", k));
+        c.omega = d->omega;
+        c.wc = d->wc;
+        c.rtol = d->rtol;
+        c.maxiter = d->maxiter;
+        c.one_a_one = cd.one_a_one;
+        c.img0 = img;
+        img += cd.B;
+        float *nu, *lid, *kap;
+        if ((rc = ealloc(e, &nu, cd.B)) || (rc = ealloc(e, &lid, cd.B)) || (rc = ealloc(e, &kap, cd.B)))
+            return fail(rc);
+        if (hipMemcpy(nu, cd.nu, cd.B * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(lid, cd.lid, cd.B * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(kap, cd.kappa, cd.B * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_err(CFD_ERR_HIP, "parameter upload failed"));
+        c.nu = nu;
+        c.lid = lid;
+        c.kappa = kap;
+        const size_t N = (size_t)c.S * cd.B;
+        float** f32[] = {&c.u, &c.v, &c.F, &c.G, &c.hu[0], &c.hu[1], &c.hv[0], &c.hv[1],
+                         &c.T[0], &c.T[1], &c.p};
+        for (float** p : f32)
+            if ((rc = ealloc(e, p, N))) return fail(rc);
+        double** f64[] = {&c.x, &c.b, &c.r, &c.ro, &c.q, &c.d0, &c.d1};
+        for (double** p : f64)
+            if ((rc = ealloc(e, p, N))) return fail(rc);
+        for (int l = 0; l < cd.nlev; ++l) {
+            EnsLevel& L = c.lev[l];
+            L.n = cd.n >> l;
+            if (cd.lev_n[l] != L.n)
+                return fail(set_err(CFD_ERR_ARG, "class %d level %d: %d cells per side, expected %d",
+                                    k, l, cd.lev_n[l], L.n));
+            L.P = l == 0 ? c.P : ((L.n + 2 + 31) / 32) * 32;
+            L.S = (long)(L.n + 2) * L.P;
+            float *b, *x, *t;
+            if ((rc = ealloc(e, &b, (size_t)L.S * cd.B)) || (rc = ealloc(e, &x, (size_t)L.S * cd.B)) ||
+                (rc = ealloc(e, &t, (size_t)L.S * cd.B)))
+                return fail(rc);
+            L.b = b;
+            L.x = x;
+            L.t = t;
+            if (l > 0) {
+                float* co[3];
+                const float* src[3] = {cd.diag[l], cd.ce[l], cd.cn[l]};
+                for (int q = 0; q < 3; ++q) {
+                    if (!src[q]) return fail(set_err(CFD_ERR_ARG, "class %d level %d operator", k, l));
+                    if ((rc = ealloc(e, &co[q], (size_t)L.S))) return fail(rc);
+                    if (hipMemcpy2D(co[q] + L.P + 1, L.P * sizeof(float), src[q], L.n * sizeof(float),
+                                    L.n * sizeof(float), L.n, hipMemcpyHostToDevice) != hipSuccess)
+                        return fail(set_err(CFD_ERR_HIP, "operator upload failed"));
+                }
+                L.diag = co[0];
+                L.cE = co[1];
+                L.cN = co[2];
+            }
+        }
+        const int nc = cd.lev_n[cd.nlev - 1] * cd.lev_n[cd.nlev - 1];
+        c.ncoarse = nc;
+        float* cinv;
+        if ((rc = ealloc(e, &cinv, (size_t)nc * nc))) return fail(rc);
+        if (hipMemcpy(cinv, cd.cinv, (size_t)nc * nc * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_err(CFD_ERR_HIP, "coarse inverse upload failed"));
+        c.cinv = cinv;
+        EnsState* st;
+        double* part;
+        if ((rc = ealloc(e, &st, cd.B)) ||
+            (rc = ealloc(e, &part, (size_t)ENS_NSLOT * cd.B * ENS_BANDS * ENS_NV)))
+            return fail(rc);
+        c.st = st;
+        c.part = part;
+    }
+    e->nimg = img;
+    if ((rc = ealloc(e, &e->img, (size_t)img * 1024)) || (rc = ealloc(e, &e->d, e->ncls)) ||
+        (rc = ealloc(e, &e->notdone, e->ncls)))
+        return fail(rc);
+    if (hipHostMalloc((void**)&e->notdone_h, CFD_ENS_MAX_CLASSES * sizeof(int), 0) != hipSuccess)
+        return fail(set_err(CFD_ERR_HIP, "pinned allocation failed"));
+    for (int k = 0; k < e->ncls; ++k) {
+        e->h[k].img = e->img;
+        e->h[k].notdone = e->notdone + k;
+        e->pred_tx[k] = 256;
+        e->pred_ty[k] = 1;
+    }
+    if (hipMemcpy(e->d, e->h, e->ncls * sizeof(EnsClass), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(set_err(CFD_ERR_HIP, "class upload failed"));
+    if (d->autotune && (rc = ens_autotune(e))) return fail(rc);
+    e->use_graph = d->use_graph ? 1 : 0;
+    *out = e;
+    return 0;
+}
+
+int cfd_ens_destroy(cfd_ens* e) {
+    if (!e) return 0;
+    hipStreamSynchronize(e->stream);
+    ens_drop_graph(e);
+    for (void* p : e->allocs) hipFree(p);
+    if (e->notdone_h) hipHostFree(e->notdone_h);
+    hipStreamDestroy(e->stream);
+    delete e;
+    return 0;
+}
+
+// the shared network: image batch nimg x 32 x 32 (stacked), or NULL for the
+// constant-vector coarse correction alone
+int cfd_ens_set_net(cfd_ens* e, cfd_unet* net, float sigma) {
+    if (!e) return set_err(CFD_ERR_ARG, "null ensemble");
+    if (net && (net->H != 32 * e->nimg || net->W != 32 || net->img_h != 32))
+        return set_err(CFD_ERR_ARG, "network must be a batch of %d images of 32 x 32", e->nimg);
+    HIPC(hipStreamSynchronize(e->stream));
+    ens_drop_graph(e);
+    e->net = net;
+    e->use_net = net ? 1 : 0;
+    for (int k = 0; k < e->ncls; ++k) {
+        EnsClass& c = e->h[k];
+        c.use_net = e->use_net;
+        c.img = net ? net->xin : e->img;
+        c.y = net ? net->y : nullptr;
+        c.gain = sigma / (4.0f * c.ax);
+    }
+    HIPC(hipMemcpy(e->d, e->h, e->ncls * sizeof(EnsClass), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int cfd_ens_step(cfd_ens* e, int32_t count) {
+    if (!e || count < 0) return set_err(CFD_ERR_ARG, "bad arguments");
+    for (int s = 0; s < count; ++s) TRY(ens_step(e));
+    HIPC(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+// fields of one instance in the reference layout: 0 u ((n+1) x (n+2)),
+// 1 v ((n+2) x (n+1)), 2 p ((n+2) x (n+2)), 3 T ((n+2) x (n+2)); float
+int cfd_ens_download(cfd_ens* e, int32_t cls, int32_t inst, int32_t field, float* host) {
+    if (!e || !host || cls < 0 || cls >= e->ncls || inst < 0 || inst >= e->h[cls].B)
+        return set_err(CFD_ERR_ARG, "bad arguments");
+    const EnsClass& c = e->h[cls];
+    const float* src;
+    int rows, cols;
+    switch (field) {
+    case 0: src = c.u; rows = c.n + 1; cols = c.n + 2; break;
+    case 1: src = c.v; rows = c.n + 2; cols = c.n + 1; break;
+    case 2: src = c.p; rows = c.n + 2; cols = c.n + 2; break;
+    case 3: src = c.T[e->par]; rows = c.n + 2; cols = c.n + 2; break;
+    default: return set_err(CFD_ERR_ARG, "unknown field %d", field);
+    }
+    HIPC(hipStreamSynchronize(e->stream));
+    HIPC(hipMemcpy2D(host, cols * sizeof(float), src + inst * c.S, c.P * sizeof(float),
+                     cols * sizeof(float), rows, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int cfd_ens_stats(cfd_ens* e, int32_t cls, int32_t inst, cfd_ens_stats_t* out) {
+    if (!e || !out || cls < 0 || cls >= e->ncls || inst < 0 || inst >= e->h[cls].B)
+        return set_err(CFD_ERR_ARG, "bad arguments");
+    HIPC(hipStreamSynchronize(e->stream));
+    EnsState s;
+    HIPC(hipMemcpy(&s, e->h[cls].st + inst, sizeof(s), hipMemcpyDeviceToHost));
+    out->t = s.t;
+    out->dt = s.dt;
+    out->gamma = s.gamma;
+    out->kinetic_energy = s.ke;
+    out->divergence = s.divmax;
+    out->residual = s.residual;
+    out->relres = s.relres;
+    out->steps = s.nstep;
+    out->iters = s.iters;
+    out->info = s.info;
+    out->pred_tx = e->pred_tx[cls];
+    out->pred_ty = e->pred_ty[cls];
+    out->pred_us = e->pred_us[cls];
+    out->sweeps_total = e->iters_total;
+    return 0;
+}
+
+}  // extern "C"

@@ -61,13 +61,14 @@ __global__ void k_mfma_layout_check(const float* A, const float* B, float* D) {
 // ---------------------------------------------------------------------------
 // rs1 (resampling of X1): 0 X1 is H x W; 1 X1 is 2H x 2W and is read through
 // 2x2 average pooling (k_avgpool2's arithmetic); 2 X1 is H/2 x W/2 and is read
-// through nearest upsampling.
+// through nearest upsampling.  Hi: rows per image (H for one image; a batch
+// of images stacked along H has zero padding at every image boundary).
 template <int G>
 __global__ void __launch_bounds__(256) k_conv3x3_mfma(
     const unsigned short* __restrict__ X1, int C1, const unsigned short* __restrict__ X2, int C2,
     const unsigned short* __restrict__ Wt, const float* __restrict__ bias, int H, int Wd, int Cout,
     unsigned short* __restrict__ Yb, float* __restrict__ Yf, float* __restrict__ Yf2, int csplit,
-    int mode, const int* skip, int rs1) {
+    int mode, const int* skip, int rs1, int Hi) {
     if (skip && *skip) return;
     __shared__ f32x4 red[3][G][64];
     const int lane = threadIdx.x & 63;
@@ -79,6 +80,7 @@ __global__ void __launch_bounds__(256) k_conv3x3_mfma(
     const int grp = (int)(task % ngrp);
     const long tile = task / ngrp;
     const int h = (int)(tile / tiles_w);
+    const int hl = h % Hi;   // row within its image (a batch stacks images of Hi rows)
     const int w0 = (int)(tile - (long)h * tiles_w) * 16;
     const int Cin = C1 + C2;
     const int nk = Cin / 16;
@@ -93,7 +95,7 @@ __global__ void __launch_bounds__(256) k_conv3x3_mfma(
         const int hh = h + t / 3 - 1;
         const int ww = w0 + m + t % 3 - 1;
         a = (bf16x4){0, 0, 0, 0};
-        if (hh >= 0 && hh < H && ww >= 0 && ww < Wd) {
+        if (hl + t / 3 - 1 >= 0 && hl + t / 3 - 1 < Hi && ww >= 0 && ww < Wd) {
             if (k0 >= C1) {
                 a = *reinterpret_cast<const bf16x4*>(X2 + ((long)hh * Wd + ww) * C2 + (k0 - C1) + kq);
             } else if (rs1 == 0) {
@@ -334,17 +336,19 @@ __global__ void k_upsample2_bwd(const float* __restrict__ dY, int H, int Wd, int
 // first layer (1 -> C channels), scalar: Y = relu(b + sum_t W[t][c] x[p+off_t])
 __global__ void k_conv_in(const float* __restrict__ x, int H, int Wd, const float* __restrict__ W,
                           const float* __restrict__ b, int C, unsigned short* __restrict__ Y,
-                          const int* skip) {
+                          const int* skip, int Hi) {
     if (skip && *skip) return;
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (long)H * Wd * C) return;
     const int c = (int)(k % C);
     const long p = k / C;
     const int h = (int)(p / Wd), w = (int)(p % Wd);
+    const int hl = h % Hi;
     float s = b[c];
     for (int t = 0; t < 9; ++t) {
         const int hh = h + t / 3 - 1, ww = w + t % 3 - 1;
-        if (hh >= 0 && hh < H && ww >= 0 && ww < Wd) s += W[t * C + c] * x[(long)hh * Wd + ww];
+        if (hl + t / 3 - 1 >= 0 && hl + t / 3 - 1 < Hi && ww >= 0 && ww < Wd)
+            s += W[t * C + c] * x[(long)hh * Wd + ww];
     }
     Y[k] = f2bf(fmaxf(s, 0.f));
 }
@@ -353,16 +357,17 @@ __global__ void k_conv_in(const float* __restrict__ x, int H, int Wd, const floa
 // (C a multiple of 8: 16-byte loads of 8 channels)
 __global__ void k_conv_out(const unsigned short* __restrict__ X, int H, int Wd, int C,
                            const float* __restrict__ W, const float* __restrict__ b,
-                           float* __restrict__ y, const int* skip) {
+                           float* __restrict__ y, const int* skip, int Hi) {
     if (skip && *skip) return;
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (long)H * Wd) return;
     const int h = (int)(p / Wd), w = (int)(p % Wd);
+    const int hl = h % Hi;
     float s = b[0];
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
         const int hh = h + t / 3 - 1, ww = w + t % 3 - 1;
-        if (hh < 0 || hh >= H || ww < 0 || ww >= Wd) continue;
+        if (hl + t / 3 - 1 < 0 || hl + t / 3 - 1 >= Hi || ww < 0 || ww >= Wd) continue;
         const uint4* xp = reinterpret_cast<const uint4*>(X + ((long)hh * Wd + ww) * C);
         for (int c8 = 0; c8 < C / 8; ++c8) {
             const uint4 q = xp[c8];

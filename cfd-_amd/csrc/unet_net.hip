@@ -74,6 +74,7 @@ struct cfd_unet {
     float* part = nullptr;
     long part_n = 0;
     int steps = 0;
+    int img_h = 0;                   // rows per image of a stacked batch (level 0)
     int loss_kind = 0;               // 0 masked squared error, 1 energy norm
     int loss_nx = 0, loss_ny = 0;
     float loss_eps = 0.f;
@@ -99,7 +100,7 @@ inline unsigned nblk(long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs
 int conv_mfma(cfd_unet* u, const unsigned short* x1, int c1, const unsigned short* x2, int c2,
               const unsigned short* wt, const float* bias, int H, int W, int cout,
               unsigned short* yb, float* yf, float* yf2, int csplit, int mode,
-              const int* skip, int rs1 = 0) {
+              const int* skip, int rs1 = 0, int Hi = 0) {
     if (cout % 16 || c1 % 16 || c2 % 16 || cout > 128)
         return set_err(CFD_ERR_ARG, "unsupported channel counts %d+%d -> %d", c1, c2, cout);
     // one work-group per (tile, output group); two 16-channel output chunks per
@@ -110,10 +111,10 @@ int conv_mfma(cfd_unet* u, const unsigned short* x1, int c1, const unsigned shor
     const dim3 grid((unsigned)tasks);
     if (G == 2)
         hipLaunchKernelGGL(k_conv3x3_mfma<2>, grid, dim3(256), 0, u->stream, x1, c1, x2, c2, wt,
-                           bias, H, W, cout, yb, yf, yf2, csplit, mode, skip, rs1);
+                           bias, H, W, cout, yb, yf, yf2, csplit, mode, skip, rs1, Hi > 0 ? Hi : H);
     else
         hipLaunchKernelGGL(k_conv3x3_mfma<1>, grid, dim3(256), 0, u->stream, x1, c1, x2, c2, wt,
-                           bias, H, W, cout, yb, yf, yf2, csplit, mode, skip, rs1);
+                           bias, H, W, cout, yb, yf, yf2, csplit, mode, skip, rs1, Hi > 0 ? Hi : H);
     HIPC(hipGetLastError());
     return 0;
 }
@@ -130,12 +131,13 @@ int unet_forward(cfd_unet* u, bool infer = false) {
         const int H = u->hs[Ly.level], W = u->ws[Ly.level];
         const bool pool_in = l == 2 || l == 4 || l == 6;
         const bool up_in = l == 8 || l == 10 || l == 12;
+        const int Hi = (u->img_h > 0 ? u->img_h : u->H) >> Ly.level;   // rows per image
         if (infer && Ly.type == L_MFMA && (pool_in || up_in)) {
             const int src = pool_in ? (l == 2 ? A0B : (l == 4 ? A1B : A2B))
                                     : (l == 8 ? A3B : (l == 10 ? B2B : B1B));
             TRY(conv_mfma(u, u->act[src], Ly.cin1, act_in(u, LIO[l].x2), Ly.cin2,
                           u->wb + Ly.woff, u->w + Ly.boff, H, W, Ly.cout, u->act[LIO[l].y],
-                          nullptr, nullptr, Ly.cout, 3, u->skip, pool_in ? 1 : 2));
+                          nullptr, nullptr, Ly.cout, 3, u->skip, pool_in ? 1 : 2, Hi));
             continue;
         }
         // resampling feeding this layer
@@ -158,15 +160,15 @@ int unet_forward(cfd_unet* u, bool infer = false) {
             const long n = (long)H * W * Ly.cout;
             hipLaunchKernelGGL(k_conv_in, dim3(nblk(n)), dim3(256), 0, u->stream, u->xin, H, W,
                                u->w + Ly.woff, u->w + Ly.boff, Ly.cout, u->act[LIO[l].y],
-                               u->skip);
+                               u->skip, Hi);
         } else if (Ly.type == L_OUT) {
             hipLaunchKernelGGL(k_conv_out, dim3(nblk((long)H * W)), dim3(256), 0, u->stream,
                                u->act[LIO[l].x1], H, W, Ly.cin1, u->w + Ly.woff, u->w + Ly.boff,
-                               u->y, u->skip);
+                               u->y, u->skip, Hi);
         } else {
             TRY(conv_mfma(u, u->act[LIO[l].x1], Ly.cin1, act_in(u, LIO[l].x2), Ly.cin2,
                           u->wb + Ly.woff, u->w + Ly.boff, H, W, Ly.cout, u->act[LIO[l].y],
-                          nullptr, nullptr, Ly.cout, 3, u->skip));
+                          nullptr, nullptr, Ly.cout, 3, u->skip, 0, Hi));
         }
         HIPC(hipGetLastError());
     }
@@ -410,6 +412,14 @@ int cfd_unet_loss_grad(cfd_unet* u, const float* x_dev, const float* t_dev, cons
     for (float v : lp) s += v;
     *loss = s;
     TRY(unet_backward(u));
+    return 0;
+}
+
+int cfd_unet_set_batch(cfd_unet* u, int32_t image_rows) {
+    if (!u || image_rows < 8 || image_rows % 8 || u->H % image_rows)
+        return set_err(CFD_ERR_ARG, "batch image rows %d do not divide %d in multiples of 8",
+                       image_rows, u ? u->H : 0);
+    u->img_h = image_rows;
     return 0;
 }
 

@@ -281,6 +281,10 @@ int cfd_unet_set_params(cfd_unet* net, const float* host);
 int cfd_unet_get_params(cfd_unet* net, float* host);
 int cfd_unet_get_grad(cfd_unet* net, float* host);
 int cfd_unet_zero_grad(cfd_unet* net);
+/* forward passes treat the H x W image as a batch of H / image_rows images
+   stacked along H (each zero-padded at its own borders); training is for
+   single images */
+int cfd_unet_set_batch(cfd_unet* net, int32_t image_rows);
 /* y = net(x) */
 int cfd_unet_forward(cfd_unet* net, const float* x_dev, float* y_dev);
 /* loss = sum mask (net(x) - t)^2 / tnorm2 (mask may be NULL); the gradient of
@@ -297,6 +301,60 @@ int cfd_unet_adam(cfd_unet* net, float lr, float scale);
 /* D = A B for 16 x 16 row-major matrices, through one bf16 matrix-core
    instruction (checks the operand lane layout the convolutions rely on) */
 int cfd_mfma_layout_check(const float* A, const float* B, float* D);
+
+/* ---- batched ensembles (no counterpart in the reference, which runs one case
+   per process): many lid-driven / heated cavities on the unit square
+   (cases.py:60-70, 207-223; AB2, gamma auto, no-slip walls) advanced in
+   lockstep.  Instances of one side length n (a power of two >= 64) form a
+   class; every kernel of a step serves all instances of a class at once.
+   Bulk in FP32 (velocities, temperature, pressure, V-cycle), the pressure
+   solve an FP64 flexible CG to a true relative residual of rtol,
+   preconditioned by V-cycles down to each instance's 32 x 32 level where one
+   batched pass of the shared U-Net (cfd_ens_set_net) gives the coarse
+   corrections.  Per class: nu (1/Re or sqrt(Pr/Ra)), lid velocity (1 or 0),
+   thermal diffusivity kappa (0 for a lid cavity, 1/sqrt(Ra Pr) for a heated
+   one: T = 0.5 / -0.5 on the left / right walls, buoyancy (0, -T)); the
+   pressure multigrid hierarchy of the n x n pinned Neumann cavity (levels >= 1
+   as n_l x n_l float arrays, the coarsest one's dense inverse). ---- */
+#define CFD_ENS_MAX_CLASSES 8
+#define CFD_ENS_MAX_LEVELS 12
+typedef struct cfd_ens cfd_ens;
+typedef struct cfd_ens_class_desc {
+    int32_t n, B, nlev;
+    const float* nu;
+    const float* lid;
+    const float* kappa;
+    int32_t lev_n[CFD_ENS_MAX_LEVELS];
+    const float* diag[CFD_ENS_MAX_LEVELS];
+    const float* ce[CFD_ENS_MAX_LEVELS];
+    const float* cn[CFD_ENS_MAX_LEVELS];
+    const float* cinv;
+    double one_a_one;          /* 1^T A_K 1 of the 32 x 32 level */
+} cfd_ens_class_desc;
+typedef struct cfd_ens_desc {
+    int32_t nclass;
+    cfd_ens_class_desc cls[CFD_ENS_MAX_CLASSES];
+    double tau, rtol;
+    int32_t maxiter;
+    float omega, wc;           /* damped-Jacobi weight, coarse-correction weight */
+    int32_t autotune;          /* 1: time the predictor's work-group shapes per class */
+    int32_t use_graph;         /* 1: replay each solver iteration as a captured HIP graph */
+} cfd_ens_desc;
+typedef struct cfd_ens_stats_t {
+    double t, dt, gamma, kinetic_energy, divergence, residual, relres;
+    int64_t steps, sweeps_total;
+    int32_t iters, info, pred_tx, pred_ty;
+    float pred_us;
+} cfd_ens_stats_t;
+int cfd_ens_create(const cfd_ens_desc* desc, cfd_ens** out);
+int cfd_ens_destroy(cfd_ens* ens);
+/* net: a batch of (total instances) images of 32 x 32 (cfd_unet_set_batch(net,
+   32)), or NULL for the constant-vector coarse correction alone */
+int cfd_ens_set_net(cfd_ens* ens, cfd_unet* net, float sigma);
+int cfd_ens_step(cfd_ens* ens, int32_t count);
+/* field 0 u, 1 v, 2 p, 3 T of one instance, reference layout, float */
+int cfd_ens_download(cfd_ens* ens, int32_t cls, int32_t inst, int32_t field, float* host);
+int cfd_ens_stats(cfd_ens* ens, int32_t cls, int32_t inst, cfd_ens_stats_t* out);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,97 @@
+"""The batched cavity ensemble on the GPU: every instance against its
+single-case FP64 run, instances independent of their batch, graph replay,
+the shared network's gain, and the full 256-instance workload."""
+
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed():
+    from cfd2d_amd.ensemble import Instance
+
+    return [Instance("lid", 64, re=100.0), Instance("heated", 64, ra=1e5),
+            Instance("lid", 128, re=1000.0), Instance("heated", 128, ra=1e4),
+            Instance("lid", 256, re=3200.0)]
+
+
+def test_instances_match_single_case_fp64(gpu):
+    """FP32 bulk against the FP64 single-case path (parity-checked against the
+    oracle) after 30 steps: same t and dt, velocity and temperature within
+    1e-5 (FP32 rounding and the 1e-6 solver tolerance)."""
+    from cfd2d_amd.ensemble import Ensemble
+
+    ins = _mixed()
+    ens = Ensemble(ins)
+    ens.step(30)
+    for k, inst in enumerate(ins):
+        st = ens.stats(k)
+        sim = inst.single_case(pressure_solver="cg", pressure_tol=1e-10).sim
+        for _ in range(30):
+            sim.step()
+        sim._flush()
+        assert st["steps"] == 30 and st["t"] == pytest.approx(sim.t, rel=1e-12)
+        assert st["relres"] <= 1e-6 and st["info"] == 0
+        du = np.abs(ens.field(k, "u") - sim.u).max()
+        dv = np.abs(ens.field(k, "v") - sim.v).max()
+        msg = f"{inst.kind} n={inst.n}: max|du| {du:.2e} max|dv| {dv:.2e}"
+        assert du < 1e-5 and dv < 1e-5, msg
+        if inst.kind == "heated":
+            dT = np.abs(ens.field(k, "T")[1:-1, 1:-1] - sim.scalars["T"].phi[1:-1, 1:-1]).max()
+            msg += f" max|dT| {dT:.2e}"
+            assert dT < 1e-5, msg
+        assert st["kinetic_energy"] == pytest.approx(sim.history.kinetic_energy[-1], rel=1e-4)
+        print(msg)
+        sim.close()
+    ens.close()
+
+
+def test_instance_independent_of_batch_and_graph(gpu):
+    from cfd2d_amd.ensemble import Ensemble
+
+    ins = _mixed()
+    full = Ensemble(ins)
+    full.step(12)
+    alone = Ensemble([ins[3]], use_graph=False)
+    alone.step(12)
+    assert np.array_equal(alone.field(0, "u"), full.field(3, "u"))
+    assert np.array_equal(alone.field(0, "T"), full.field(3, "T"))
+    full.close()
+    alone.close()
+
+
+def test_network_gain_in_ensemble(gpu):
+    from cfd2d_amd.ensemble import Ensemble
+
+    its = {}
+    for net in (True, False):
+        ens = Ensemble(_mixed(), net=net)
+        ens.step(15)
+        its[net] = ens.stats(0)["sweeps_total"]
+        ens.close()
+    print(f"solver iterations over 15 steps: {its[True]} with the network, {its[False]} without")
+    assert its[True] < 0.6 * its[False]
+
+
+def test_full_workload_256_instances(gpu):
+    """The 256-instance workload (24.7 M cells): every solve converges, the
+    autotuner picked a shape per class, throughput is reported."""
+    from cfd2d_amd.ensemble import Ensemble, sample_instances
+
+    ins = sample_instances(256, 0)
+    ens = Ensemble(ins)
+    ens.step(2)
+    t0 = time.perf_counter()
+    ens.step(5)
+    wall = time.perf_counter() - t0
+    rel = [ens.stats(k)["relres"] for k in range(0, 256, 17)]
+    assert max(rel) <= 1e-6
+    assert all(ens.stats(k)["info"] == 0 for k in range(0, 256, 17))
+    tx = {ens.stats(c[0])["pred_tx"] * ens.stats(c[0])["pred_ty"] for c in ens.classes}
+    assert tx and all(t in (256, 512) for t in tx)
+    print(f"256 instances, {ens.cells / 1e6:.2f} M cells: {5 * ens.cells / wall / 1e9:.3f} G "
+          f"cell-updates/s")
+    ens.close()

@@ -156,3 +156,38 @@ def test_case_json_roundtrip(tmp_path):
     assert cases.load_case(tmp_path / "case.json") == spec
     with pytest.raises(ValueError):
         cases.build_case({"fall": "warp_drive"})
+
+
+def test_ctypes_structs_match_the_c_header(tmp_path):
+    """Every struct of include/cfd2d_hip.h has the size its ctypes mirror has."""
+    import ctypes
+
+    from cfd2d_amd import _lib
+
+    pairs = {"cfd_grid_desc": _lib.GridDesc, "cfd_scalar_desc": _lib.ScalarDesc,
+             "cfd_step_params": _lib.StepParams, "cfd_state": _lib.State,
+             "cfd_record": _lib.Record, "cfd_mg_desc": _lib.MGDesc,
+             "cfd_learned_desc": _lib.LearnedDesc, "cfd_learned_stats_t": _lib.LearnedStats,
+             "cfd_ens_desc": _lib.EnsDesc, "cfd_ens_stats_t": _lib.EnsStats}
+    src = tmp_path / "sizes.c"
+    src.write_text('#include <stdio.h>\n#include "cfd2d_hip.h"\nint main(void){\n' +
+                   "".join(f'printf("%zu\\n", sizeof({k}));\n' for k in pairs) + "return 0;}\n")
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), "-o", str(exe), str(src)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    assert got == [ctypes.sizeof(v) for v in pairs.values()], dict(zip(pairs, got))
+
+
+def test_ensemble_workload_sampler():
+    from cfd2d_amd.ensemble import sample_instances
+
+    a = sample_instances(256, 0)
+    b = sample_instances(256, 0)
+    assert a == b and len(a) == 256
+    assert {i.n for i in a} <= {64, 128, 256, 512}
+    lid = [i for i in a if i.kind == "lid"]
+    hot = [i for i in a if i.kind == "heated"]
+    assert len(lid) == len(hot) == 128
+    assert all(100 <= i.re <= 5000 for i in lid) and all(1e4 <= i.ra <= 1e6 for i in hot)
+    assert hot[0].nu == pytest.approx(np.sqrt(0.71 / hot[0].ra))
