@@ -115,7 +115,7 @@ class Ensemble:
             h = _cavity_hierarchy(n)
             cd = d.cls[c]
             cd.n, cd.B, cd.nlev = n, len(members), len(h.levels)
-            arrs = [np.array([getattr(m, a) for m in members], dtype=np.float32)
+            arrs = [np.array([getattr(m, a) for m in members], dtype=np.float64)
                     for a in ("nu", "lid", "kappa")]
             keep += arrs
             cd.nu, cd.lid, cd.kappa = (_lib.ptr(a) for a in arrs)

@@ -42,9 +42,9 @@ struct EnsClass {
     float h, ax;              // h = 1/n, ax = n^2
     double hd, axd;
     double tau;
-    const float* nu;          // per instance
-    const float* lid;
-    const float* kappa;       // thermal diffusivity (0: no temperature)
+    const double* nu;         // per instance
+    const double* lid;
+    const double* kappa;      // thermal diffusivity (0: no temperature)
     float *u, *v, *F, *G, *hu[2], *hv[2], *T[2], *p;
     double *x, *b, *r, *ro, *q, *d0, *d1;
     EnsLevel lev[ENS_MAXLEV];
@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(256) k_e_bc(const EnsClass* __restrict__ c) {
     const int b = blockIdx.x, n = c->n, P = c->P;
     float* u = c->u + b * c->S;
     float* v = c->v + b * c->S;
-    const float lid = c->lid[b];
+    const float lid = (float)c->lid[b];
     for (int j = threadIdx.x; j <= n; j += blockDim.x) {
         if (j >= 1) {
             u[ix(0, j, P)] = 0.0f;
@@ -261,9 +261,9 @@ __global__ void k_e_pred(const EnsClass* __restrict__ c, int par) {
     const float dt = (float)s.dt, gamma = (float)s.gamma;
     const float A = (float)s.ab_a, Bw = (float)s.ab_b;
     const bool ab2 = s.has_old;
-    const float nu = c->nu[b];
+    const float nu = (float)c->nu[b];
     const float idx = (float)c->n, idx2 = c->ax;
-    const float* T = c->kappa[b] > 0.0f ? c->T[par] + o : nullptr;
+    const float* T = c->kappa[b] > 0.0 ? c->T[par] + o : nullptr;
     if (i <= n) {
         const bool uf = i >= 1 && i <= n - 1 && j >= 1 && j <= n;
         float h = 0.0f;
@@ -787,7 +787,7 @@ __global__ void k_e_temp(const EnsClass* __restrict__ c, int par) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y * blockDim.y + threadIdx.y;
     const int b = blockIdx.z, n = c->n, P = c->P;
-    const float D = c->kappa[b];
+    const float D = (float)c->kappa[b];
     if (D <= 0.0f || i > n + 1 || j > n + 1) return;
     const long o = b * c->S;
     const float* phi = c->T[par] + o;
@@ -852,7 +852,7 @@ __global__ void k_e_temp(const EnsClass* __restrict__ c, int par) {
 // temperature ghosts (models.py:139-183 for these codes), one work-group per instance
 __global__ void __launch_bounds__(256) k_e_tghost(const EnsClass* __restrict__ c, int par) {
     const int b = blockIdx.x, n = c->n, P = c->P;
-    if (c->kappa[b] <= 0.0f) return;
+    if (c->kappa[b] <= 0.0) return;
     float* T = c->T[par ^ 1] + b * c->S;
     for (int k = 1 + threadIdx.x; k <= n; k += blockDim.x) {
         T[ix(0, k, P)] = 2.0f * 0.5f - T[ix(1, k, P)];

@@ -257,12 +257,12 @@ int cfd_ens_create(const cfd_ens_desc* d, cfd_ens** out) {
         c.one_a_one = cd.one_a_one;
         c.img0 = img;
         img += cd.B;
-        float *nu, *lid, *kap;
+        double *nu, *lid, *kap;
         if ((rc = ealloc(e, &nu, cd.B)) || (rc = ealloc(e, &lid, cd.B)) || (rc = ealloc(e, &kap, cd.B)))
             return fail(rc);
-        if (hipMemcpy(nu, cd.nu, cd.B * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(lid, cd.lid, cd.B * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(kap, cd.kappa, cd.B * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        if (hipMemcpy(nu, cd.nu, cd.B * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(lid, cd.lid, cd.B * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(kap, cd.kappa, cd.B * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
             return fail(set_err(CFD_ERR_HIP, "parameter upload failed"));
         c.nu = nu;
         c.lid = lid;

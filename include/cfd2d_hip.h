@@ -321,9 +321,9 @@ int cfd_mfma_layout_check(const float* A, const float* B, float* D);
 typedef struct cfd_ens cfd_ens;
 typedef struct cfd_ens_class_desc {
     int32_t n, B, nlev;
-    const float* nu;
-    const float* lid;
-    const float* kappa;
+    const double* nu;          /* B values each */
+    const double* lid;
+    const double* kappa;
     int32_t lev_n[CFD_ENS_MAX_LEVELS];
     const float* diag[CFD_ENS_MAX_LEVELS];
     const float* ce[CFD_ENS_MAX_LEVELS];
