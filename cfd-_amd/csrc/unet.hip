@@ -543,3 +543,162 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float
     w[k] = v;
     if (wb) wb[k] = f2bf(v);
 }
+
+// ---------------------------------------------------------------------------
+// The whole forward pass of the network for 32 x 32 images in one work-group
+// per image (1024 threads), activations in LDS: one launch instead of fourteen
+// for large batches of coarse-level images (the ensemble's).  Same layers, weights,
+// BF16 rounding and fused pooling/upsampling reads as the per-layer kernels;
+// the convolutions' K loop runs in one wave per (tile, output chunk).
+// LDS: level-0 skip S0 and work W0a (32 KB each); levels 1-3 (S1, W1a, W1b,
+// S2, W2a, W2b, W3a, W3b) in a 76 KB region that the last level-0 tensor W0b
+// reuses once they are dead.
+// ---------------------------------------------------------------------------
+struct Unet32Off {
+    long woff[15], boff[15];
+};
+
+namespace {
+
+// 3x3 convolution of S x S images held in LDS (X1 read directly, through 2x2
+// pooling of a 2S x 2S tensor (rs1 = 1) or nearest upsampling of an S/2 x S/2
+// one (rs1 = 2); X2 direct), bias + ReLU, bf16 out to LDS
+__device__ __forceinline__ void conv32_lds(const unsigned short* X1, int C1, int rs1,
+                                           const unsigned short* X2, int C2, int S,
+                                           const unsigned short* __restrict__ Wt,
+                                           const float* __restrict__ bias, int Cout,
+                                           unsigned short* Y) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    const int m = lane % 16, kq = 4 * (lane / 16);
+    const int tiles_w = (S + 15) / 16, ntile = S * tiles_w, nchunk = Cout / 16;
+    const int Cin = C1 + C2, nk = Cin / 16, nsteps = 9 * nk;
+    for (int task = wave; task < ntile * nchunk; task += nwave) {
+        const int ch = task % nchunk, tile = task / nchunk;
+        const int h = tile / tiles_w, w0 = (tile % tiles_w) * 16;
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        auto load = [&](int st, bf16x4& a, bf16x4& b) {
+            const int t = st / nk, k0 = (st - t * nk) * 16;
+            const int hh = h + t / 3 - 1, ww = w0 + m + t % 3 - 1;
+            a = (bf16x4){0, 0, 0, 0};
+            if (hh >= 0 && hh < S && ww >= 0 && ww < S) {
+                if (k0 >= C1) {
+                    a = *reinterpret_cast<const bf16x4*>(X2 + (hh * S + ww) * C2 + (k0 - C1) + kq);
+                } else if (rs1 == 0) {
+                    a = *reinterpret_cast<const bf16x4*>(X1 + (hh * S + ww) * C1 + k0 + kq);
+                } else if (rs1 == 2) {
+                    a = *reinterpret_cast<const bf16x4*>(
+                        X1 + ((hh >> 1) * (S >> 1) + (ww >> 1)) * C1 + k0 + kq);
+                } else {
+                    const int S2 = 2 * S;
+                    const unsigned short* p0 = X1 + ((2 * hh) * S2 + 2 * ww) * C1 + k0 + kq;
+                    const bf16x4 q0 = *reinterpret_cast<const bf16x4*>(p0);
+                    const bf16x4 q1 = *reinterpret_cast<const bf16x4*>(p0 + C1);
+                    const bf16x4 q2 = *reinterpret_cast<const bf16x4*>(p0 + S2 * C1);
+                    const bf16x4 q3 = *reinterpret_cast<const bf16x4*>(p0 + S2 * C1 + C1);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const float sm = bf2f((unsigned short)q0[e]) + bf2f((unsigned short)q1[e]) +
+                                         bf2f((unsigned short)q2[e]) + bf2f((unsigned short)q3[e]);
+                        a[e] = (short)f2bf(0.25f * sm);
+                    }
+                }
+            }
+            b = *reinterpret_cast<const bf16x4*>(Wt + ((long)t * Cout + ch * 16 + m) * Cin + k0 + kq);
+        };
+        // operands of 8 steps in flight before their matrix-core instructions
+        constexpr int U = 8;
+        for (int s0 = 0; s0 < nsteps; s0 += U) {
+            bf16x4 a[U], b[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (s0 + u < nsteps) {
+                    load(s0 + u, a[u], b[u]);
+                } else {
+                    a[u] = (bf16x4){0, 0, 0, 0};
+                    b[u] = (bf16x4){0, 0, 0, 0};
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], b[u], acc, 0, 0, 0);
+        }
+        const int co = ch * 16 + m;
+        const float bv = bias[co];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int w = w0 + kq + j;
+            if (w < S) Y[(h * S + w) * Cout + co] = f2bf(fmaxf(acc[j] + bv, 0.f));
+        }
+    }
+    __syncthreads();
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(1024) k_unet32(const float* __restrict__ x, float* __restrict__ y,
+                                                 const unsigned short* __restrict__ wb,
+                                                 const float* __restrict__ w, Unet32Off off,
+                                                 const int* skip) {
+    if (skip && *skip) return;
+    __shared__ __align__(16) unsigned short S0[1024 * 16], W0a[1024 * 16];
+    __shared__ __align__(16) unsigned short R[38912];   // levels 1-3, later W0b
+    unsigned short* S1 = R;                 // 256 x 32
+    unsigned short* W1a = R + 8192;         // 256 x 32
+    unsigned short* W1b = R + 16384;        // 256 x 32
+    unsigned short* S2 = R + 24576;         // 64 x 64
+    unsigned short* W2a = R + 28672;        // 64 x 64
+    unsigned short* W2b = R + 32768;        // 64 x 64
+    unsigned short* W3a = R + 36864;        // 16 x 64
+    unsigned short* W3b = R + 37888;        // 16 x 64
+    unsigned short* W0b = R;                // 1024 x 16 (after levels 1-3 are dead)
+    const float* xi = x + (long)blockIdx.x * 1024;
+    // input layer 1 -> 16 (scalar), ReLU
+    {
+        const float* W = w + off.woff[0];
+        const float* b = w + off.boff[0];
+        for (int k = threadIdx.x; k < 1024 * 16; k += blockDim.x) {
+            const int c = k % 16, p = k / 16, h = p / 32, ww = p % 32;
+            float s = b[c];
+            for (int t = 0; t < 9; ++t) {
+                const int hh = h + t / 3 - 1, w2 = ww + t % 3 - 1;
+                if (hh >= 0 && hh < 32 && w2 >= 0 && w2 < 32) s += W[t * 16 + c] * xi[hh * 32 + w2];
+            }
+            W0a[k] = f2bf(fmaxf(s, 0.f));
+        }
+        __syncthreads();
+    }
+    auto L = [&](int l, const unsigned short* X1, int C1, int rs1, const unsigned short* X2,
+                 int C2, int S, int Cout, unsigned short* Y) {
+        conv32_lds(X1, C1, rs1, X2, C2, S, wb + off.woff[l], w + off.boff[l], Cout, Y);
+    };
+    L(1, W0a, 16, 0, nullptr, 0, 32, 16, S0);     // A0B
+    L(2, S0, 16, 1, nullptr, 0, 16, 32, W1a);     // A1A (pool)
+    L(3, W1a, 32, 0, nullptr, 0, 16, 32, S1);     // A1B
+    L(4, S1, 32, 1, nullptr, 0, 8, 64, W2a);      // A2A (pool)
+    L(5, W2a, 64, 0, nullptr, 0, 8, 64, S2);      // A2B
+    L(6, S2, 64, 1, nullptr, 0, 4, 64, W3a);      // A3A (pool)
+    L(7, W3a, 64, 0, nullptr, 0, 4, 64, W3b);     // A3B
+    L(8, W3b, 64, 2, S2, 64, 8, 64, W2a);         // B2A (up + skip)
+    L(9, W2a, 64, 0, nullptr, 0, 8, 64, W2b);     // B2B
+    L(10, W2b, 64, 2, S1, 32, 16, 32, W1a);       // B1A
+    L(11, W1a, 32, 0, nullptr, 0, 16, 32, W1b);   // B1B
+    L(12, W1b, 32, 2, S0, 16, 32, 16, W0a);       // B0A
+    L(13, W0a, 16, 0, nullptr, 0, 32, 16, W0b);   // B0B (levels 1-3 dead)
+    // output layer 16 -> 1 (linear)
+    {
+        const float* W = w + off.woff[14];
+        const float b = w[off.boff[14]];
+        float* yo = y + (long)blockIdx.x * 1024;
+        for (int p = threadIdx.x; p < 1024; p += blockDim.x) {
+            const int h = p / 32, ww = p % 32;
+            float s = b;
+            for (int t = 0; t < 9; ++t) {
+                const int hh = h + t / 3 - 1, w2 = ww + t % 3 - 1;
+                if (hh < 0 || hh >= 32 || w2 < 0 || w2 >= 32) continue;
+                const unsigned short* xp = W0b + (hh * 32 + w2) * 16;
+                for (int c = 0; c < 16; ++c) s += W[t * 16 + c] * bf2f(xp[c]);
+            }
+            yo[p] = s;
+        }
+    }
+}

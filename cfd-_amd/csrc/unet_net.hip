@@ -126,6 +126,21 @@ const unsigned short* act_in(cfd_unet* u, int id) { return id < 0 ? nullptr : u-
 // results are identical; the backward pass needs the stored tensors)
 int unet_forward(cfd_unet* u, bool infer = false) {
     using namespace unet;
+    const int Hi0 = u->img_h > 0 ? u->img_h : u->H;
+    if (infer && Hi0 == 32 && u->W == 32 && u->H / 32 >= 64) {
+        // batches of >= 64 images of 32 x 32: the whole network in one
+        // work-group per image (one CU each; for fewer images the per-layer
+        // kernels, which spread every layer over many CUs, are faster)
+        Unet32Off off;
+        for (int l = 0; l < NL; ++l) {
+            off.woff[l] = u->L[l].woff;
+            off.boff[l] = u->L[l].boff;
+        }
+        hipLaunchKernelGGL(k_unet32, dim3(u->H / 32), dim3(1024), 0, u->stream, u->xin, u->y,
+                           u->wb, u->w, off, u->skip);
+        HIPC(hipGetLastError());
+        return 0;
+    }
     for (int l = 0; l < NL; ++l) {
         const Layer& Ly = u->L[l];
         const int H = u->hs[Ly.level], W = u->ws[Ly.level];

@@ -140,3 +140,38 @@ def test_training_reduces_the_loss(gpu):
     last = tot / len(pairs)
     print(f"loss {first:.4f} -> {last:.4f}")
     assert last < first / 5
+
+
+@pytest.mark.parametrize("nimg", [5, 70])
+def test_batched_32x32_forward(gpu, nimg):
+    """Stacked batches of 32 x 32 images (the learned solver's coarse level;
+    70 images take the one-work-group-per-image kernel with activations in
+    LDS): every image against the bf16-emulating oracle and against its own
+    single-image pass (per-layer kernels)."""
+    import ctypes
+
+    from cfd2d_amd import _lib
+    from cfd2d_amd.unet import UNet
+
+    p = _params(4)
+    rng = np.random.default_rng(11)
+    xs = rng.standard_normal((nimg, 32, 32)).astype(np.float32)
+    net = UNet(32 * nimg, 32, p)
+    _lib.check(_lib.lib().cfd_unet_set_batch(net.handle, 32), "batch")
+    dx, dy = _dev(xs.reshape(32 * nimg, 32)), _dev(np.zeros((32 * nimg, 32)))
+    net.forward(dx, dy)
+    y = dy.get().reshape(nimg, 32, 32)
+    one = UNet(32, 32, p)
+    for k in range(0, nimg, max(1, nimg // 6)):
+        # bf16 activations: a different fp32 summation order of a convolution
+        # flips some roundings, which 15 layers compound to ~1% on noise inputs
+        y_bf, _ = U.forward(p, xs[k], bf16_emulate=True)
+        assert _rel(y[k], y_bf) < 2e-2, k
+        d1, d2 = _dev(xs[k]), _dev(np.zeros((32, 32)))
+        one.forward(d1, d2)
+        if nimg < 64:
+            assert np.array_equal(d2.get(), y[k])
+        else:   # other kernel, other summation order
+            assert _rel(y[k], d2.get()) < 2e-2
+    net.close()
+    one.close()
