@@ -184,7 +184,7 @@ def run_ensemble(steps: int = 10, warmup: int = 3, net: bool = True) -> dict:
 
     ins = sample_instances(256, 0)
     t0 = time.perf_counter()
-    ens = Ensemble(ins, net=net)
+    ens = Ensemble(ins, net=net, use_graph=USE_GRAPH)
     setup = time.perf_counter() - t0
     ens.step(warmup)
     s0 = ens.stats(0)["sweeps_total"]

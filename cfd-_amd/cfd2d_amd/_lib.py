@@ -73,7 +73,8 @@ class LearnedDesc(ctypes.Structure):
 
 class LearnedStats(ctypes.Structure):
     _fields_ = [("harvested", _i64), ("solves", _i64), ("iterations", _i64), ("fallbacks", _i64),
-                ("last_iters", _i32), ("last_info", _i32), ("last_relres", _d)]
+                ("last_iters", _i32), ("last_info", _i32), ("last_relres", _d),
+                ("net_off", _i32), ("pad", _i32)]
 
 
 ENS_MAX_CLASSES = 8

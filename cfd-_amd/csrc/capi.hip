@@ -418,7 +418,7 @@ int learned_iteration(cfd_ctx* c) {
             const hipStream_t s0 = u->stream;
             const int* k0 = u->skip;
             u->stream = c->stream;
-            u->skip = &c->fcg->done;
+            u->skip = &c->fcg->skip_net;
             const int rc = unet_forward(u, true);
             u->stream = s0;
             u->skip = k0;
@@ -463,7 +463,7 @@ int learned_iteration(cfd_ctx* c) {
     LV_LAUNCH512(k_fcg_dir, nb0, a, v, nb0);
     LV_LAUNCH512(k_fcg_update, nb0, a, v, nb0);
     LV_LAUNCH512(k_fcg_true, nb0, a, v, nb0);
-    LV_LAUNCH(k_fcg_true_fin, 1, a, nb0);
+    LV_LAUNCH(k_fcg_true_fin, 1, a, nb0, c->l_mode == LEARNED_NET ? 1 : 0);
     return 0;
 }
 
@@ -887,6 +887,7 @@ int cfd_learned_stats(cfd_ctx* c, cfd_learned_stats_t* out) {
     out->last_iters = h.iters;
     out->last_info = h.info;
     out->last_relres = h.bnorm > 0.0 ? sqrt(h.rr) / h.bnorm : 0.0;
+    out->net_off = h.net_off;
     return 0;
 }
 

@@ -167,6 +167,9 @@ struct FcgState {
     double s_in;             // RMS of the current coarse right-hand side (network input scale)
     double irho, irho_old, ibeta, ialpha;   // teacher: flexible CG on the coarse level
     int k, first, done, pending, iters, info, reps, maxiter;
+    int net_off;             // the network is switched off for this context
+    int skip_net;            // done || net_off: the network's kernels idle
+    double rr0;              // r.r at the start of the solve
     long long harvested, hidx;
     unsigned ticket[4];
 };

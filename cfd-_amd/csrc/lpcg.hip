@@ -27,6 +27,10 @@
 
 enum { LEARNED_NET = 0, LEARNED_NONE = 1, LEARNED_TEACHER = 2 };
 
+// iteration after which a network that has not reduced ||r|| tenfold is
+// switched off (the solver then runs as without it)
+constexpr int LV_NET_CHECK = 8;
+
 // partial-sum slots (LV_SLOT doubles each) of LvArgs::part
 enum { LV_P_DOTS = 0, LV_P_PQ = 1, LV_P_RR = 2, LV_P_CM = 3, LV_NSLOT = 4 };
 constexpr int LV_MAXB = 1024;            // work-groups of any reducing kernel
@@ -336,7 +340,7 @@ __device__ __forceinline__ void lv_cstats(const LvArgs& a, int nb, double one_a_
 __global__ void __launch_bounds__(256) k_lv_net_in(LvArgs a, int l, float* img, int H, int W,
                                                    int nb, double one_a_one) {
     FcgState* fs = a.fs;
-    if (fs->done) return;
+    if (fs->skip_net) return;
     double cm, s;
     lv_cstats(a, nb, one_a_one, cm, s);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -372,7 +376,7 @@ __global__ void __launch_bounds__(256) k_lv_net_out(LvArgs a, int l, const float
         const long c = ix(i, j, L.P);
         if (mg_diag(m, l, L, c) == 0.0f) return;
         double t = cm;
-        if (use_net) t += (double)gain * s * (double)y[(long)(i - 1) * W + (j - 1)];
+        if (use_net && !fs->net_off) t += (double)gain * s * (double)y[(long)(i - 1) * W + (j - 1)];
         L.t[c] = (float)t;
     });
 }
@@ -634,10 +638,21 @@ __global__ void __launch_bounds__(512) k_fcg_true(LvArgs a, FcgVec v, int nb) {
 
 // one work-group: advance the iteration count; after a true residual, finish
 // or restart the flexible CG from it
-__global__ void __launch_bounds__(256) k_fcg_true_fin(LvArgs a, int nb) {
+__global__ void __launch_bounds__(256) k_fcg_true_fin(LvArgs a, int nb, int net_used) {
     FcgState* fs = a.fs;
     if (fs->done) return;
-    if (threadIdx.x == 0) fs->k += 1;
+    if (threadIdx.x == 0) {
+        fs->k += 1;
+        // a network that has not reduced ||r|| tenfold in LV_NET_CHECK
+        // iterations is switched off for the context; the CG restarts
+        if (net_used && !fs->net_off && !fs->pending && fs->k == LV_NET_CHECK &&
+            !(fs->rr <= 1e-2 * fs->rr0)) {
+            fs->net_off = 1;
+            fs->first = 1;
+        }
+        fs->skip_net = fs->done | fs->net_off;
+    }
+    __syncthreads();
     if (!fs->pending) return;
     double rr[1];
     lv_total<1>(a.part + LV_P_RR * LV_SLOT, nb, rr);
@@ -657,5 +672,6 @@ __global__ void __launch_bounds__(256) k_fcg_true_fin(LvArgs a, int nb) {
         } else {
             fs->first = 1;
         }
+        fs->skip_net = fs->done | fs->net_off;
     }
 }

@@ -326,6 +326,8 @@ __global__ void __launch_bounds__(PROJ_BS) k_project(ProjArgs a, Geo g) {
             fs->reps = 0;
             fs->maxiter = a.maxiter;
             fs->done = (bnorm == 0.0 || sqrt(rr) < atol) ? 1 : 0;
+            fs->rr0 = fs->rr;
+            fs->skip_net = fs->done | fs->net_off;
         }
         const MGLevelDev& L0 = a.mg->lev[0];
         const float w = a.mg->omega;

@@ -237,7 +237,9 @@ int cfd_scalar_transport(const double* phi, const double* u, const double* v,
    buffers, each iteration stores the pair (network input, normalised
    correction) as two H x W float images, H and W the level's sizes rounded up
    to multiples of 8).  A solve that does not converge within max_iter
-   iterations is continued by the FP64 Jacobi-PCG when `fallback` is set.
+   iterations is continued by the FP64 Jacobi-PCG when `fallback` is set.  A
+   network that has not reduced ||r|| tenfold after 8 iterations of a solve is
+   switched off for the context (the solver then runs as in mode 1).
    The context keeps `net` (and the harvest buffers) until cfd_set_learned is
    called again or the context is destroyed: the caller keeps them alive. ---- */
 typedef struct cfd_unet cfd_unet;
@@ -259,6 +261,8 @@ typedef struct cfd_learned_stats_t {
     int64_t harvested, solves, iterations, fallbacks;
     int32_t last_iters, last_info;
     double last_relres;
+    int32_t net_off;           /* 1: the network failed its check and is switched off */
+    int32_t pad;
 } cfd_learned_stats_t;
 int cfd_set_learned(cfd_ctx* ctx, const cfd_learned_desc* desc);
 int cfd_learned_stats(cfd_ctx* ctx, cfd_learned_stats_t* out);

@@ -201,3 +201,30 @@ def test_harvest_train_solve_from_scratch(gpu):
     print(f"fresh network after {len(hist)} epochs (loss {hist[0]:.3f} -> {hist[-1]:.4f}): "
           f"{its['net']:.1f} iterations vs {its['none']:.1f} without")
     assert its["net"] < 0.7 * its["none"]
+
+
+def test_network_switched_off_outside_its_training(gpu):
+    """The shipped network was trained on cavities; on the obstacle channel
+    (outflow, masks, 32 x 8 coarse image) it does not help, so the solver
+    switches it off after its check and runs as without it: no Jacobi-PCG
+    fallbacks, the same order of iterations as the network-free mode."""
+    from cfd2d_amd import cases
+    from cfd2d_amd.learned import LearnedConfig
+
+    def channel(mode):
+        rng = np.random.default_rng(0)
+        xs, ys, rs = rng.uniform(0.3, 3.0, 12), rng.uniform(0.1, 0.9, 12), rng.uniform(0.03, 0.08, 12)
+        obst = [{"typ": "kreis", "x": float(x), "y": float(y), "r": float(r)}
+                for x, y, r in zip(xs, ys, rs)]
+        case = cases.custom_obstacles(obst, lx=4.0, ly=1.0, nx=1024, ny=256, re=200, L_ref=1.0,
+                                      inflow_profile="parabolic", pressure_solver="cg_learned",
+                                      pressure_tol=1e-6, learned=LearnedConfig(mode=mode))
+        return _run(case.sim, 15)
+
+    net, none = channel("net"), channel("none")
+    st = net.pressure.learned.stats()
+    print(f"channel: network off {st['net_off']}, iterations {np.mean(net.cg_iterations):.1f} "
+          f"vs {np.mean(none.cg_iterations):.1f} without, fallbacks {st['fallbacks']}")
+    assert max(net.cg_relres) <= 1e-6 and all(i == 0 for i in net.cg_info)
+    assert st["fallbacks"] == 0
+    assert np.mean(net.cg_iterations) < 1.5 * np.mean(none.cg_iterations) + 8
