@@ -77,6 +77,33 @@ __device__ __forceinline__ void ens_band(int lo, int hi, int band, int nb, int& 
     i1 = lo + (int)((len * (band + 1)) / nb);
 }
 
+// the interior cells (i, j) of band blockIdx.x of an n x n instance, U cells
+// per thread at a time: gather (loads and arithmetic, no stores) for all U
+// cells before scatter stores any, so their loads are in flight together.
+// Per thread the cells come in the order of the row loop (i, then j by
+// blockDim.x) when n is a multiple of blockDim.x.
+template <int U, typename G, typename S>
+__device__ __forceinline__ void ens_cells_batch(int n, G gather, S scatter) {
+    int i0, i1;
+    ens_band(1, n + 1, blockIdx.x, ENS_BANDS, i0, i1);
+    const int ncell = (i1 - i0) * n, bs = blockDim.x;
+    using T = decltype(gather(1, 1));
+    for (int q0 = threadIdx.x; q0 < ncell; q0 += U * bs) {
+        T val[U];
+        int ii[U], jj[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = q0 + u * bs < ncell ? q0 + u * bs : q0;
+            ii[u] = i0 + q / n;
+            jj[u] = 1 + q % n;
+            val[u] = gather(ii[u], jj[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (q0 + u * bs < ncell) scatter(ii[u], jj[u], val[u]);
+    }
+}
+
 template <int NV>
 __device__ __forceinline__ void ens_partial(double (&v)[NV], double* slot, int band) {
     __shared__ double lds[NV * 16];
@@ -95,6 +122,20 @@ __device__ __forceinline__ void ens_partial_max(double (&v)[NV], double* slot, i
         for (int q = 0; q < NV; ++q) slot[band * ENS_NV + q] = v[q];
 }
 
+// totals of an instance's band partials, by the whole work-group (one band per
+// thread, then a block sum; every thread of the work-group must call it)
+template <int NV>
+__device__ __forceinline__ void ens_total_wg(const double* slot, double (&v)[NV]) {
+    __shared__ double lds[NV * 16];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = 0.0;
+    for (int k = threadIdx.x; k < ENS_BANDS; k += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) v[q] += slot[k * ENS_NV + q];
+    block_sum<NV>(v, lds);
+}
+
+// the same totals by one thread (kernels with a thread per instance)
 template <int NV>
 __device__ __forceinline__ void ens_total(const double* slot, double (&v)[NV]) {
 #pragma unroll
@@ -321,7 +362,7 @@ __global__ void __launch_bounds__(256) k_e_res(const EnsClass* __restrict__ c) {
     const int b = blockIdx.y, n = c->n, P = c->P;
     const long o = b * c->S;
     double tot[1];
-    ens_total<1>(ens_part(c, ENS_P_A, b), tot);
+    ens_total_wg<1>(ens_part(c, ENS_P_A, b), tot);
     const double mean = tot[0] / ((double)n * n);
     const EnsLevel& L0 = c->lev[0];
     int i0, i1;
@@ -483,7 +524,7 @@ __global__ void __launch_bounds__(256) k_e_cmode(const EnsClass* __restrict__ c)
 
 __device__ __forceinline__ void ens_cstats(const EnsClass* c, int b, double& cm, double& s_in) {
     double s[4];
-    ens_total<4>(ens_part(c, ENS_P_C, b), s);
+    ens_total_wg<4>(ens_part(c, ENS_P_C, b), s);
     const double nf = (double)c->lev[c->K].n * c->lev[c->K].n;
     cm = c->one_a_one > 0.0 ? s[0] / c->one_a_one : 0.0;
     const double bb = s[1] - 2.0 * cm * s[2] + cm * cm * s[3];
@@ -496,12 +537,13 @@ __global__ void __launch_bounds__(256) k_e_net_in(const EnsClass* __restrict__ c
     const int l = c->K;
     const EnsLevel& L = c->lev[l];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;   // 0 .. 1023
+    const bool done = c->st[b].done;
+    double cm = 0.0, s = 0.0;
+    if (!done) ens_cstats(c, b, cm, s);   // the whole work-group, before any thread leaves
     if (k >= L.n * L.n) return;
     const int i = k / L.n + 1, j = k % L.n + 1;
     float val = 0.0f;
-    if (!c->st[b].done) {
-        double cm, s;
-        ens_cstats(c, b, cm, s);
+    if (!done) {
         const float d = lev_diag(c, l, L, i, j);
         auto one = [&](int p, int q) -> float { return (p >= 1 && p <= L.n && q >= 1 && q <= L.n) ? 1.0f : 0.0f; };
         if (d != 0.0f && s > 0.0)
@@ -516,9 +558,9 @@ __global__ void __launch_bounds__(256) k_e_net_out(const EnsClass* __restrict__ 
     if (c->st[b].done) return;
     const int l = c->K;
     const EnsLevel& L = c->lev[l];
-    ENS_CELL(L);
     double cm, s;
-    ens_cstats(c, b, cm, s);
+    ens_cstats(c, b, cm, s);   // the whole work-group, before any thread leaves
+    ENS_CELL(L);
     double t = cm;
     if (c->use_net)
         t += (double)c->gain * s * (double)c->y[(long)(c->img0 + b) * L.n * L.n + (i - 1) * L.n + (j - 1)];
@@ -534,19 +576,29 @@ __global__ void __launch_bounds__(256) k_e_smooth_dots(const EnsClass* __restric
     const int n = c->n, P = c->P;
     const long o = b * L.S;
     const bool first = st.first;
-    int i0, i1;
-    ens_band(1, n + 1, blockIdx.x, ENS_BANDS, i0, i1);
     double s[2] = {0.0, 0.0};
-    for (int i = i0; i < i1; ++i)
-        for (int j = 1 + threadIdx.x; j <= n; j += blockDim.x) {
+    struct Z {
+        float z;
+        double r, ro;
+    };
+    ens_cells_batch<8>(
+        n,
+        [&](int i, int j) {
             const float d = lev_diag(c, 0, L, i, j);
             auto xv = [&](int p, int q) -> float { return L.x[o + ix(p, q, P)]; };
-            const float z = L.x[o + ix(i, j, P)] + c->omega * (L.b[o + ix(i, j, P)] - lev_apply(c, 0, L, i, j, d, xv)) / d;
-            L.t[o + ix(i, j, P)] = z;
+            Z w;
+            w.z = L.x[o + ix(i, j, P)] +
+                  c->omega * (L.b[o + ix(i, j, P)] - lev_apply(c, 0, L, i, j, d, xv)) / d;
             const long k = b * c->S + ix(i, j, P);
-            s[0] += c->r[k] * (double)z;
-            if (!first) s[1] += c->ro[k] * (double)z;
-        }
+            w.r = c->r[k];
+            w.ro = c->ro[k];
+            return w;
+        },
+        [&](int i, int j, const Z& w) {
+            L.t[o + ix(i, j, P)] = w.z;
+            s[0] += w.r * (double)w.z;
+            if (!first) s[1] += w.ro * (double)w.z;
+        });
     ens_partial<2>(s, ens_part(c, ENS_P_A, b), blockIdx.x);
 }
 
@@ -556,7 +608,7 @@ __global__ void __launch_bounds__(256) k_e_dir(const EnsClass* __restrict__ c) {
     EnsState& st = c->st[b];
     if (st.done) return;
     double dots[2];
-    ens_total<2>(ens_part(c, ENS_P_A, b), dots);
+    ens_total_wg<2>(ens_part(c, ENS_P_A, b), dots);
     const bool first = st.first;
     const double beta = first ? 0.0 : (dots[0] - dots[1]) / st.rho_old;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -568,23 +620,36 @@ __global__ void __launch_bounds__(256) k_e_dir(const EnsClass* __restrict__ c) {
     const float* z = c->lev[0].t + b * c->lev[0].S;
     double* dnew = ((st.k & 1) ? c->d1 : c->d0) + o;
     const double* dold = ((st.k & 1) ? c->d0 : c->d1) + o;
-    int i0, i1;
-    ens_band(1, n + 1, blockIdx.x, ENS_BANDS, i0, i1);
     double pq[1] = {0.0};
-    for (int i = i0; i < i1; ++i)
-        for (int j = 1 + threadIdx.x; j <= n; j += blockDim.x) {
-            auto dirv = [&](int p, int q) -> double {
-                if (p < 1 || p > n || q < 1 || q > n) return 0.0;
-                const long k = ix(p, q, P);
-                return first ? (double)z[k] : (double)z[k] + beta * dold[k];
-            };
-            const double dc = dirv(i, j);
-            const double qv = c->axd * diag_count(i, j, n) * dc - c->axd * (dirv(i + 1, j) + dirv(i - 1, j)) -
-                              c->axd * (dirv(i, j + 1) + dirv(i, j - 1));
-            dnew[ix(i, j, P)] = dc;
-            c->q[o + ix(i, j, P)] = qv;
-            pq[0] += dc * qv;
-        }
+    struct D {
+        double dc, qv;
+    };
+    // z and both direction buffers are 0 on the ghost ring (zeroed at
+    // creation, only interior cells written), so d there is 0 as well; the
+    // two variants keep the loads free of branches
+    auto sweep = [&](auto first_c) {
+        constexpr bool FIRST = decltype(first_c)::value;
+        ens_cells_batch<8>(
+            n,
+            [&](int i, int j) {
+                auto dirv = [&](int p, int q) -> double {
+                    const long k = ix(p, q, P);
+                    return FIRST ? (double)z[k] : (double)z[k] + beta * dold[k];
+                };
+                D w;
+                w.dc = dirv(i, j);
+                w.qv = c->axd * diag_count(i, j, n) * w.dc - c->axd * (dirv(i + 1, j) + dirv(i - 1, j)) -
+                       c->axd * (dirv(i, j + 1) + dirv(i, j - 1));
+                return w;
+            },
+            [&](int i, int j, const D& w) {
+                dnew[ix(i, j, P)] = w.dc;
+                c->q[o + ix(i, j, P)] = w.qv;
+                pq[0] += w.dc * w.qv;
+            });
+    };
+    if (first) sweep(std::true_type{});
+    else sweep(std::false_type{});
     ens_partial<1>(pq, ens_part(c, ENS_P_B, b), blockIdx.x);
 }
 
@@ -595,7 +660,7 @@ __global__ void __launch_bounds__(256) k_e_update(const EnsClass* __restrict__ c
     EnsState& st = c->st[b];
     if (st.done) return;
     double pq[1];
-    ens_total<1>(ens_part(c, ENS_P_B, b), pq);
+    ens_total_wg<1>(ens_part(c, ENS_P_B, b), pq);
     const double alpha = st.rho / pq[0];
     if (!(pq[0] > 0.0) || !(fabs(alpha) < 1e300)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -608,22 +673,30 @@ __global__ void __launch_bounds__(256) k_e_update(const EnsClass* __restrict__ c
     const long o = b * c->S;
     const EnsLevel& L0 = c->lev[0];
     const double* dnew = ((st.k & 1) ? c->d1 : c->d0) + o;
-    int i0, i1;
-    ens_band(1, n + 1, blockIdx.x, ENS_BANDS, i0, i1);
     double s[1] = {0.0};
-    for (int i = i0; i < i1; ++i)
-        for (int j = 1 + threadIdx.x; j <= n; j += blockDim.x) {
+    struct Up {
+        double x, r, rn;
+    };
+    ens_cells_batch<8>(
+        n,
+        [&](int i, int j) {
             const long k = o + ix(i, j, P);
-            c->x[k] = c->x[k] + alpha * dnew[ix(i, j, P)];
-            const double rv = c->r[k];
-            c->ro[k] = rv;
-            const double rn = rv - alpha * c->q[k];
-            c->r[k] = rn;
-            s[0] += rn * rn;
-            const float rf = (float)rn;
+            Up w;
+            w.x = c->x[k] + alpha * dnew[ix(i, j, P)];
+            w.r = c->r[k];
+            w.rn = w.r - alpha * c->q[k];
+            return w;
+        },
+        [&](int i, int j, const Up& w) {
+            const long k = o + ix(i, j, P);
+            c->x[k] = w.x;
+            c->ro[k] = w.r;
+            c->r[k] = w.rn;
+            s[0] += w.rn * w.rn;
+            const float rf = (float)w.rn;
             L0.b[b * L0.S + ix(i, j, P)] = rf;
             L0.x[b * L0.S + ix(i, j, P)] = c->omega * rf / (c->ax * (float)diag_count(i, j, n));
-        }
+        });
     ens_partial<1>(s, ens_part(c, ENS_P_C, b), blockIdx.x);
 }
 
@@ -634,7 +707,7 @@ __global__ void __launch_bounds__(256) k_e_true(const EnsClass* __restrict__ c) 
     EnsState& st = c->st[b];
     if (st.done) return;
     double rr[1];
-    ens_total<1>(ens_part(c, ENS_P_C, b), rr);
+    ens_total_wg<1>(ens_part(c, ENS_P_C, b), rr);
     const int k = st.k + 1;   // st.k advances in k_e_true_fin
     const bool nan = !(rr[0] == rr[0]);
     const bool pending = !nan && (sqrt(rr[0]) < st.atol || k >= c->maxiter);
@@ -654,21 +727,22 @@ __global__ void __launch_bounds__(256) k_e_true(const EnsClass* __restrict__ c) 
     const long o = b * c->S;
     const EnsLevel& L0 = c->lev[0];
     const double* x = c->x + o;
-    int i0, i1;
-    ens_band(1, n + 1, blockIdx.x, ENS_BANDS, i0, i1);
     double s[1] = {0.0};
-    for (int i = i0; i < i1; ++i)
-        for (int j = 1 + threadIdx.x; j <= n; j += blockDim.x) {
+    ens_cells_batch<8>(
+        n,
+        [&](int i, int j) {
             const double d = c->axd * diag_count(i, j, n);
             const double Ax = d * x[ix(i, j, P)] - c->axd * (x[ix(i + 1, j, P)] + x[ix(i - 1, j, P)]) -
                               c->axd * (x[ix(i, j + 1, P)] + x[ix(i, j - 1, P)]);
-            const double rt = c->b[o + ix(i, j, P)] - Ax;
+            return c->b[o + ix(i, j, P)] - Ax;
+        },
+        [&](int i, int j, double rt) {
             c->r[o + ix(i, j, P)] = rt;
             s[0] += rt * rt;
             const float rf = (float)rt;
             L0.b[b * L0.S + ix(i, j, P)] = rf;
             L0.x[b * L0.S + ix(i, j, P)] = c->omega * rf / (c->ax * (float)diag_count(i, j, n));
-        }
+        });
     ens_partial<1>(s, ens_part(c, ENS_P_D, b), blockIdx.x);
 }
 
@@ -726,7 +800,7 @@ __global__ void __launch_bounds__(256) k_e_pwrite(const EnsClass* __restrict__ c
     const int b = blockIdx.y, n = c->n, P = c->P;
     const long o = b * c->S;
     double tot[1];
-    ens_total<1>(ens_part(c, ENS_P_A, b), tot);
+    ens_total_wg<1>(ens_part(c, ENS_P_A, b), tot);
     const double mean = tot[0] / ((double)n * n);
     int i0, i1;
     ens_band(1, n + 1, blockIdx.x, ENS_BANDS, i0, i1);

@@ -88,6 +88,35 @@ __device__ __forceinline__ void lv_cells(int nx, int ny, F f) {
         for (int j = 1 + (int)threadIdx.x; j <= ny; j += blockDim.x) f(i, j);
 }
 
+// The same band of cells, U cells per thread at a time: gather(i, j) (loads
+// and arithmetic only, no stores, no branches) runs for all U cells before
+// scatter(i, j, value) stores any of them, so the loads of U cells are in
+// flight together instead of one cell's chain of dependent loads after the
+// other.  A thread visits its cells in the order of lv_cells when ny is a
+// multiple of the work-group size.
+template <int U, typename G, typename S>
+__device__ __forceinline__ void lv_cells_batch(int nx, int ny, G gather, S scatter) {
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int i0 = 1 + (int)(((long)nx * b) / gridDim.x);
+    const int i1 = 1 + (int)(((long)nx * (b + 1)) / gridDim.x);
+    const int ncell = (i1 - i0) * ny, bs = blockDim.x;
+    using T = decltype(gather(1, 1));
+    for (int q0 = threadIdx.x; q0 < ncell; q0 += U * bs) {
+        T val[U];
+        int ii[U], jj[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = q0 + u * bs < ncell ? q0 + u * bs : q0;   // a valid cell
+            ii[u] = i0 + q / ny;
+            jj[u] = 1 + q % ny;
+            val[u] = gather(ii[u], jj[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (q0 + u * bs < ncell) scatter(ii[u], jj[u], val[u]);
+    }
+}
+
 // f(i, j) for one cell per thread: work-group = (row, 256-cell chunk of the
 // row), launched with nx * ceil(ny / 256) work-groups of 256 threads
 template <typename F>
@@ -190,16 +219,28 @@ __global__ void __launch_bounds__(512) k_lv_smooth_dots(LvArgs a, FcgVec v) {
     const float w = a.mg->omega;
     const bool first = fs->first;
     double s[2] = {0.0, 0.0};
-    lv_cells(L.nx, L.ny, [&](int i, int j) {
-        const long c = ix(i, j, L.P);
-        const float d = mg_diag(m, 0, L, c);
-        if (d == 0.0f) return;
-        auto xv = [&](int p, int q) -> float { return L.x[ix(p, q, L.P)]; };
-        const float z = L.x[c] + w * (L.b[c] - mg_apply(m, 0, L, i, j, d, xv)) / d;
-        L.t[c] = z;
-        s[0] += v.r[c] * (double)z;
-        if (!first) s[1] += v.rold[c] * (double)z;
-    });
+    struct Z {
+        float d, z;
+        double r, ro;
+    };
+    lv_cells_batch<4>(
+        L.nx, L.ny,
+        [&](int i, int j) {
+            const long c = ix(i, j, L.P);
+            Z o;
+            o.d = mg_diag(m, 0, L, c);
+            auto xv = [&](int p, int q) -> float { return L.x[ix(p, q, L.P)]; };
+            o.z = L.x[c] + w * (L.b[c] - mg_apply(m, 0, L, i, j, o.d, xv)) / o.d;
+            o.r = v.r[c];
+            o.ro = v.rold[c];
+            return o;
+        },
+        [&](int i, int j, const Z& o) {
+            if (o.d == 0.0f) return;
+            L.t[ix(i, j, L.P)] = o.z;
+            s[0] += o.r * (double)o.z;
+            if (!first) s[1] += o.ro * (double)o.z;
+        });
     lv_partial<2>(s, a.part + LV_P_DOTS * LV_SLOT);
 }
 
@@ -535,23 +576,40 @@ __global__ void __launch_bounds__(512) k_fcg_dir(LvArgs a, FcgVec v, int nb) {
     double* dnew = (k & 1) ? v.d1 : v.d0;
     const double* dold = (k & 1) ? v.d0 : v.d1;
     double pq[1] = {0.0};
-    lv_cells(v.nx, ny, [&](int i, int j) {
-        const long c = ix(i, j, v.pc);
-        const int li = a.code[c] & 63;
-        if (!li) return;
-        const int iE = (a.periodic && i == v.nx) ? 1 : i + 1;
-        const int iW = (a.periodic && i == 1) ? v.nx : i - 1;
-        auto dirv = [&](long cc) -> double {
-            const double zc = z[cc];
-            return first ? zc : zc + beta * dold[cc];
-        };
-        const double dc = dirv(c);
-        const double q = s_lut[li] * dc - v.ax * (dirv(ix(iE, j, v.pc)) + dirv(ix(iW, j, v.pc))) -
-                         v.ay * (dirv(ix(i, j + 1, v.pc)) + dirv(ix(i, j - 1, v.pc)));
-        dnew[c] = dc;
-        v.q[c] = q;
-        pq[0] += dc * q;
-    });
+    struct D {
+        int li;
+        double dc, q;
+    };
+    // two variants (restart or not) keep the loads free of branches
+    auto sweep = [&](auto first_c) {
+        constexpr bool FIRST = decltype(first_c)::value;
+        lv_cells_batch<4>(
+            v.nx, ny,
+            [&](int i, int j) {
+                const long c = ix(i, j, v.pc);
+                D o;
+                o.li = a.code[c] & 63;
+                const int iE = (a.periodic && i == v.nx) ? 1 : i + 1;
+                const int iW = (a.periodic && i == 1) ? v.nx : i - 1;
+                auto dirv = [&](long cc) -> double {
+                    const double zc = z[cc];
+                    return FIRST ? zc : zc + beta * dold[cc];
+                };
+                o.dc = dirv(c);
+                o.q = s_lut[o.li] * o.dc - v.ax * (dirv(ix(iE, j, v.pc)) + dirv(ix(iW, j, v.pc))) -
+                      v.ay * (dirv(ix(i, j + 1, v.pc)) + dirv(ix(i, j - 1, v.pc)));
+                return o;
+            },
+            [&](int i, int j, const D& o) {
+                if (!o.li) return;
+                const long c = ix(i, j, v.pc);
+                dnew[c] = o.dc;
+                v.q[c] = o.q;
+                pq[0] += o.dc * o.q;
+            });
+    };
+    if (first) sweep(std::true_type{});
+    else sweep(std::false_type{});
     lv_partial<1>(pq, a.part + LV_P_PQ * LV_SLOT);
 }
 
@@ -577,20 +635,32 @@ __global__ void __launch_bounds__(512) k_fcg_update(LvArgs a, FcgVec v, int nb) 
     const int k = fs->k;
     const double* dnew = (k & 1) ? v.d1 : v.d0;
     double s[1] = {0.0};
-    lv_cells(v.nx, L0.ny, [&](int i, int j) {
-        const long c = ix(i, j, v.pc);
-        const int li = a.code[c] & 63;
-        if (!li) return;
-        v.x[c] = v.x[c] + alpha * dnew[c];
-        const double rv = v.r[c];
-        v.rold[c] = rv;
-        const double rn = rv - alpha * v.q[c];
-        v.r[c] = rn;
-        s[0] += rn * rn;
-        const float rf = (float)rn;
-        L0.b[c] = rf;
-        L0.x[c] = w * rf / m.diag0[li];
-    });
+    struct Up {
+        int li;
+        double x, r, rn;
+    };
+    lv_cells_batch<4>(
+        v.nx, L0.ny,
+        [&](int i, int j) {
+            const long c = ix(i, j, v.pc);
+            Up o;
+            o.li = a.code[c] & 63;
+            o.x = v.x[c] + alpha * dnew[c];
+            o.r = v.r[c];
+            o.rn = o.r - alpha * v.q[c];
+            return o;
+        },
+        [&](int i, int j, const Up& o) {
+            if (!o.li) return;
+            const long c = ix(i, j, v.pc);
+            v.x[c] = o.x;
+            v.rold[c] = o.r;
+            v.r[c] = o.rn;
+            s[0] += o.rn * o.rn;
+            const float rf = (float)o.rn;
+            L0.b[c] = rf;
+            L0.x[c] = w * rf / m.diag0[o.li];
+        });
     lv_partial<1>(s, a.part + LV_P_RR * LV_SLOT);
 }
 
@@ -622,17 +692,29 @@ __global__ void __launch_bounds__(512) k_fcg_true(LvArgs a, FcgVec v, int nb) {
     const MGLevelDev L0 = a.mg->lev[0];
     const float w = a.mg->omega;
     double s[1] = {0.0};
-    lv_cells(v.nx, L0.ny, [&](int i, int j) {
-        const long c = ix(i, j, v.pc);
-        const int li = a.code[c] & 63;
-        if (!li) return;
-        const double rt = v.b[c] - apply_A(v.x, i, j, v.pc, v.nx, a.periodic, v.lut[li], v.ax, v.ay);
-        v.r[c] = rt;
-        s[0] += rt * rt;
-        const float rf = (float)rt;
-        L0.b[c] = rf;
-        L0.x[c] = w * rf / m.diag0[li];
-    });
+    struct Tr {
+        int li;
+        double rt;
+    };
+    lv_cells_batch<4>(
+        v.nx, L0.ny,
+        [&](int i, int j) {
+            const long c = ix(i, j, v.pc);
+            Tr o;
+            o.li = a.code[c] & 63;
+            o.rt = v.b[c] -
+                   apply_A(v.x, i, j, v.pc, v.nx, a.periodic, v.lut[o.li], v.ax, v.ay);
+            return o;
+        },
+        [&](int i, int j, const Tr& o) {
+            if (!o.li) return;
+            const long c = ix(i, j, v.pc);
+            v.r[c] = o.rt;
+            s[0] += o.rt * o.rt;
+            const float rf = (float)o.rt;
+            L0.b[c] = rf;
+            L0.x[c] = w * rf / m.diag0[o.li];
+        });
     lv_partial<1>(s, a.part + LV_P_RR * LV_SLOT);
 }
 
