@@ -68,7 +68,8 @@ class LearnedDesc(ctypes.Structure):
     _fields_ = [("net", _vp), ("level", _i32), ("mode", _i32), ("teacher_cycles", _i32),
                 ("max_iter", _i32), ("fallback", _i32), ("use_graph", _i32),
                 ("sigma", ctypes.c_float), ("one_a_one", _d), ("harvest_in", _vp),
-                ("harvest_out", _vp), ("harvest_cap", _i64)]
+                ("harvest_out", _vp), ("harvest_cap", _i64), ("coarse_fused", _i32),
+                ("pad", _i32)]
 
 
 class LearnedStats(ctypes.Structure):

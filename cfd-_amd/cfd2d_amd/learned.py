@@ -41,7 +41,10 @@ class LearnedConfig:
     method without the network) or "teacher" (level K solved by
     ``teacher_cycles`` V-cycles; with ``harvest_cap`` > 0 the training pairs
     are stored).  ``params``/``sigma`` default to the weights shipped in
-    ``cfd2d_amd/weights``; ``level`` None picks the level automatically."""
+    ``cfd2d_amd/weights``; ``level`` None picks the level automatically.
+    ``coarse_fused``: on a 32 x 32 level K, the levels of at most 64 x 64
+    cells, the network and the level-K correction run in one work-group
+    launch (k_lv_coarse) instead of ~25 launches."""
 
     mode: str = "net"
     params: np.ndarray | None = None
@@ -51,6 +54,7 @@ class LearnedConfig:
     max_iter: int = 200
     fallback: bool = True
     use_graph: bool = True
+    coarse_fused: bool = True
     harvest_cap: int = 0
     weights: str | None = None
     side: int = TARGET_SIDE
@@ -140,6 +144,7 @@ class Attached:
         d.max_iter = int(cfg.max_iter)
         d.fallback = 1 if cfg.fallback else 0
         d.use_graph = 1 if cfg.use_graph else 0
+        d.coarse_fused = 1 if cfg.coarse_fused else 0
         d.sigma = self.sigma
         d.one_a_one = one_a_one(hierarchy, self.level)
         d.harvest_in = self.hin.ptr if self.hin else None

@@ -88,6 +88,7 @@ struct cfd_ctx {
     // learned flexible CG (lpcg.hip)
     int l_ready = 0;
     int l_level = 0, l_mode = 0, l_cycles = 0, l_fallback = 1, l_graph = 1, l_maxiter = 200;
+    int l_fused = 0, l_lc = 0;   // coarse levels l_lc .. l_level in one k_lv_coarse launch
     int l_H = 0, l_W = 0;
     float l_sigma = 1.0f, l_D = 1.0f;
     double l_1a1 = 0.0;
@@ -404,6 +405,36 @@ int learned_iteration(cfd_ctx* c) {
     };
     const int nb0 = bands(0), nbK = bands(K);
     const unsigned nimg = (unsigned)(((long)c->l_H * c->l_W + 255) / 256);
+    if (c->l_fused) {
+        const int Lc = c->l_lc;
+        for (int l = 0; l < Lc; ++l) {
+            LV_LAUNCH(k_lv_smooth, cells(l), a, l, 0);
+            LV_LAUNCH(k_lv_restrict, cells(l + 1), a, l);
+        }
+        CoarseNet cn;
+        std::memset(&cn, 0, sizeof(cn));
+        if (c->l_mode == LEARNED_NET) {
+            const cfd_unet* u = c->l_net;
+            cn.wsw = u->wsw;
+            cn.w = u->w;
+            cn.use_net = 1;
+        }
+        cn.gain = c->l_sigma / c->l_D;
+        cn.one_a_one = c->l_1a1;
+        hipLaunchKernelGGL(k_lv_coarse, dim3(1), dim3(1024), 0, c->stream, a, Lc, K, cn);
+        HIPC(hipGetLastError());
+        for (int l = Lc - 1; l >= 1; --l) {
+            LV_LAUNCH(k_lv_post, cells(l), a, l);
+            LV_LAUNCH(k_lv_smooth, cells(l), a, l, 0);
+        }
+        LV_LAUNCH(k_lv_post, cells(0), a, 0);
+        LV_LAUNCH512(k_lv_smooth_dots, nb0, a, v);
+        LV_LAUNCH512(k_fcg_dir, nb0, a, v, nb0);
+        LV_LAUNCH512(k_fcg_update, nb0, a, v, nb0);
+        LV_LAUNCH512(k_fcg_true, nb0, a, v, nb0);
+        LV_LAUNCH(k_fcg_true_fin, 1, a, nb0, c->l_mode == LEARNED_NET ? 1 : 0);
+        return 0;
+    }
     for (int l = 0; l < K; ++l) {
         LV_LAUNCH(k_lv_smooth, cells(l), a, l, 0);
         LV_LAUNCH(k_lv_restrict, cells(l + 1), a, l);
@@ -863,6 +894,18 @@ int cfd_set_learned(cfd_ctx* c, const cfd_learned_desc* d) {
     c->l_fallback = d->fallback ? 1 : 0;
     c->l_graph = d->use_graph ? 1 : 0;
     c->l_maxiter = d->max_iter;
+    // the fused coarse end: a 32 x 32 level K below the fine level, entered at
+    // the first level of at most 64 x 64 cells (never the fine level); one
+    // work-group sweeps 64 x 64 cells faster than a launch costs, 128 x 128
+    // cells slower
+    c->l_fused = 0;
+    if (d->coarse_fused && d->mode != LEARNED_TEACHER && d->level >= 1 && L.nx == 32 &&
+        L.ny == 32 && (d->mode != LEARNED_NET || (d->net->H == 32 && d->net->W == 32))) {
+        int lc = d->level;
+        while (lc - 1 >= 1 && (long)mg.lev[lc - 1].nx * mg.lev[lc - 1].ny <= 64L * 64) --lc;
+        c->l_lc = lc;
+        c->l_fused = 1;
+    }
     c->l_H = H;
     c->l_W = W;
     c->l_net = d->mode == LEARNED_NET ? d->net : nullptr;

@@ -757,3 +757,200 @@ __global__ void __launch_bounds__(256) k_fcg_true_fin(LvArgs a, int nb, int net_
         fs->skip_net = fs->done | fs->net_off;
     }
 }
+
+// ---- the coarse end of the V-cycle in one work-group ----------------------
+// Levels Lc .. K of one preconditioner application for a 32 x 32 level K:
+// the down-leg sweeps and restrictions from level Lc, the constant-vector
+// statistics c and s, the network on the deflated, normalised level-K
+// right-hand side (activations in LDS), the correction t_K = c + gain s y, the
+// two level-K sweeps and the up-leg back to level Lc -- the work of
+// k_lv_smooth / k_lv_restrict / k_lv_cmode / k_lv_net_in / the network /
+// k_lv_net_out / k_lv_post on these levels, cell for cell the same FP32
+// arithmetic, with __syncthreads between the sweeps instead of launches.
+// One work-group of 1024 threads; returns at once after the solve finished.
+struct CoarseNet {
+    const unsigned short* wsw;   // lane-ordered matrix-core weights
+    const float* w;              // FP32 parameters
+    float gain;                  // sigma / D
+    int use_net;
+    double one_a_one;            // 1^T A_K 1
+};
+
+namespace {
+
+// f over the nx x ny cells of a level by the whole work-group, U cells per
+// thread at a time: gather (loads and arithmetic, no stores) for all U before
+// scatter stores any, so that the loads of U cells are in flight together
+template <int U, typename G, typename S>
+__device__ __forceinline__ void wg_cells(int nx, int ny, G gather, S scatter) {
+    const int n = nx * ny, bs = blockDim.x;
+    using T = decltype(gather(1, 1));
+    for (int q0 = threadIdx.x; q0 < n; q0 += U * bs) {
+        T val[U];
+        int ii[U], jj[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = q0 + u * bs < n ? q0 + u * bs : q0;
+            ii[u] = q / ny + 1;
+            jj[u] = q % ny + 1;
+            val[u] = gather(ii[u], jj[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (q0 + u * bs < n) scatter(ii[u], jj[u], val[u]);
+    }
+    __syncthreads();
+}
+
+struct CellVal {
+    float d, v;
+};
+
+// wg_cells with 4 cells per thread at a time on levels of >= 4 cells per
+// thread, else 1 (the clamped extra gathers of a partial batch cost loads)
+template <typename G, typename S>
+__device__ __forceinline__ void wg_cells_auto(int nx, int ny, G gather, S scatter) {
+    if (nx * ny >= 4 * (int)blockDim.x) wg_cells<4>(nx, ny, gather, scatter);
+    else wg_cells<1>(nx, ny, gather, scatter);
+}
+
+// dst = src + w (b - A src) / d over level l, by the whole work-group
+__device__ __forceinline__ void wg_smooth(const MGCtx& m, int l, const MGLevelDev& L, float w,
+                                          const float* src, float* dst) {
+    wg_cells_auto(
+        L.nx, L.ny,
+        [&](int i, int j) {
+            const long c = ix(i, j, L.P);
+            CellVal o;
+            o.d = mg_diag(m, l, L, c);
+            auto sv = [&](int p, int q) -> float { return src[ix(p, q, L.P)]; };
+            o.v = src[c] + w * (L.b[c] - mg_apply(m, l, L, i, j, o.d, sv)) / o.d;
+            return o;
+        },
+        [&](int i, int j, const CellVal& o) {
+            if (o.d != 0.0f) dst[ix(i, j, L.P)] = o.v;
+        });
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(1024) k_lv_coarse(LvArgs a, int Lc, int K, CoarseNet net) {
+    FcgState* fs = a.fs;
+    if (fs->done) return;
+    __shared__ __align__(16) unsigned short S0[IMG_LDS_S0], W0a[IMG_LDS_S0];
+    __shared__ __align__(16) unsigned short R[IMG_LDS_R];
+    __shared__ double red[5 * 16];
+    __shared__ float s_d0_[64];
+    if (threadIdx.x < 64) s_d0_[threadIdx.x] = a.diag0[threadIdx.x];
+    __syncthreads();
+    const MGCtx m{a.mg, a.code, s_d0_, a.periodic, a.ax, a.ay};
+    const MGDev& mg = *a.mg;
+    const float w = mg.omega, wc = mg.wc;
+    // down-leg: pre-smoothing sweep, restriction of the residual
+    for (int l = Lc; l < K; ++l) {
+        const MGLevelDev L = mg.lev[l];
+        const MGLevelDev C = mg.lev[l + 1];
+        wg_smooth(m, l, L, w, L.x, L.t);
+        wg_cells_auto(
+            C.nx, C.ny,
+            [&](int ci, int cj) {
+                float bc = 0.0f;
+                for (int di = 0; di < 2; ++di)
+                    for (int dj = 0; dj < 2; ++dj) {
+                        const int i = 2 * ci - 1 + di, j = 2 * cj - 1 + dj;
+                        const bool in = i <= L.nx && j <= L.ny;
+                        const long c = ix(in ? i : 1, in ? j : 1, L.P);
+                        const float d = mg_diag(m, l, L, c);
+                        auto tv = [&](int p, int q) -> float { return L.t[ix(p, q, L.P)]; };
+                        const float r = L.b[c] - mg_apply(m, l, L, in ? i : 1, in ? j : 1, d, tv);
+                        if (in && d != 0.0f) bc += r;
+                    }
+                return bc;
+            },
+            [&](int ci, int cj, float bc) {
+                const long cc = ix(ci, cj, C.P);
+                C.b[cc] = bc;
+                const float dc = C.diag[cc];
+                C.x[cc] = dc != 0.0f ? w * bc / dc : 0.0f;
+            });
+    }
+    // level K: c = 1^T b / 1^T A 1 and s = RMS(b - c A 1) over the fluid cells
+    const MGLevelDev LK = mg.lev[K];
+    const int nK = LK.nx * LK.ny;
+    double st[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = threadIdx.x; k < nK; k += blockDim.x) {
+        const int i = k / LK.ny + 1, j = k % LK.ny + 1;
+        const long c = ix(i, j, LK.P);
+        const float d = mg_diag(m, K, LK, c);
+        if (d == 0.0f) continue;
+        const double a1 = lv_a1(m, K, LK, i, j, d);
+        const double bv = LK.b[c];
+        st[0] += bv;
+        st[1] += bv * bv;
+        st[2] += bv * a1;
+        st[3] += a1 * a1;
+        st[4] += 1.0;
+    }
+    block_sum<5>(st, red);
+    const double cm = net.one_a_one > 0.0 ? st[0] / net.one_a_one : 0.0;
+    const double bb = st[1] - 2.0 * cm * st[2] + cm * cm * st[3];
+    const double s = st[4] > 0.0 ? sqrt(fmax(bb, 0.0) / st[4]) : 0.0;
+    if (threadIdx.x == 0) {
+        fs->cm = cm;
+        fs->s_in = s;
+    }
+    const bool use_net = net.use_net && !fs->net_off;
+    if (use_net) {
+        // network input (FP32 in R), the network, t_K = c + gain s y
+        float* xin = reinterpret_cast<float*>(R);
+        for (int k = threadIdx.x; k < 1024; k += blockDim.x) {
+            const int i = k / 32 + 1, j = k % 32 + 1;
+            const long c = ix(i, j, LK.P);
+            const float d = mg_diag(m, K, LK, c);
+            float val = 0.0f;
+            if (d != 0.0f && s > 0.0)
+                val = (float)(((double)LK.b[c] - cm * (double)lv_a1(m, K, LK, i, j, d)) / s);
+            xin[k] = val;
+        }
+        __syncthreads();
+        const double gs = (double)net.gain * s;
+        unet_img32(xin, S0, W0a, R, net.wsw, net.w, [&](int p, float y) {
+            const int i = p / 32 + 1, j = p % 32 + 1;
+            const long c = ix(i, j, LK.P);
+            if (mg_diag(m, K, LK, c) != 0.0f) LK.t[c] = (float)(cm + gs * (double)y);
+        });
+    } else {
+        for (int k = threadIdx.x; k < nK; k += blockDim.x) {
+            const int i = k / LK.ny + 1, j = k % LK.ny + 1;
+            const long c = ix(i, j, LK.P);
+            if (mg_diag(m, K, LK, c) != 0.0f) LK.t[c] = (float)cm;
+        }
+    }
+    __syncthreads();
+    wg_smooth(m, K, LK, w, LK.t, LK.x);
+    wg_smooth(m, K, LK, w, LK.x, LK.t);
+    // up-leg: prolongation + post-smoothing sweep, second sweep
+    for (int l = K - 1; l >= Lc; --l) {
+        const MGLevelDev L = mg.lev[l];
+        const MGLevelDev C = mg.lev[l + 1];
+        auto corr = [&](int p, int q) -> float {   // loads in bounds on the ghost ring too
+            const long c = ix(p, q, L.P);
+            const float v = L.t[c] + wc * C.t[ix((p + 1) >> 1, (q + 1) >> 1, C.P)];
+            return mg_diag(m, l, L, c) == 0.0f ? 0.0f : v;
+        };
+        wg_cells_auto(
+            L.nx, L.ny,
+            [&](int i, int j) {
+                const long c = ix(i, j, L.P);
+                CellVal o;
+                o.d = mg_diag(m, l, L, c);
+                const float y = corr(i, j);
+                o.v = y + w * (L.b[c] - mg_apply(m, l, L, i, j, o.d, corr)) / o.d;
+                return o;
+            },
+            [&](int i, int j, const CellVal& o) {
+                if (o.d != 0.0f) L.x[ix(i, j, L.P)] = o.v;
+            });
+        wg_smooth(m, l, L, w, L.x, L.t);
+    }
+}

@@ -545,160 +545,285 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float
 }
 
 // ---------------------------------------------------------------------------
-// The whole forward pass of the network for 32 x 32 images in one work-group
-// per image (1024 threads), activations in LDS: one launch instead of fourteen
-// for large batches of coarse-level images (the ensemble's).  Same layers, weights,
-// BF16 rounding and fused pooling/upsampling reads as the per-layer kernels;
-// the convolutions' K loop runs in one wave per (tile, output chunk).
-// LDS: level-0 skip S0 and work W0a (32 KB each); levels 1-3 (S1, W1a, W1b,
-// S2, W2a, W2b, W3a, W3b) in a 76 KB region that the last level-0 tensor W0b
-// reuses once they are dead.
+// The whole forward pass of the network for one 32 x 32 image in one
+// work-group of 16 wavefronts, activations in LDS.  Same layers, weights and
+// BF16 rounding as the per-layer kernels (pooled tensors are formed once per
+// layer, with the same arithmetic as the per-layer kernels' pooling read).
+//
+// Matrix-core operands: the B fragments (weights) come from a copy stored in
+// lane order (k_swizzle_b: the 64 lanes' 4 BF16 values of one K step are 512
+// contiguous bytes), loaded one batch of 9 K steps ahead of their use.  A
+// tile is 16 consecutive pixels in row-major order (several rows of the small
+// images), the A fragments are LDS reads at constant offsets, and the pixel
+// stride of every LDS tensor is its channel count + 4 so that the 16 pixels
+// of a fragment fall in different LDS banks.  Each wavefront owns one
+// 16-channel output chunk and 1-4 tiles; on the 4 x 4 level, where there are
+// fewer (tile, chunk) pairs than wavefronts, 4 wavefronts split the K loop of
+// a pair and add their accumulators through LDS in a fixed order.
+//
+// LDS (BF16 elements): level 0 S0 and W0a (20480 each); during levels 1-3 the
+// work tensors W1a, W2a, W2b, W3a, W3b live inside W0a, the skips S1, S2 and
+// W1b in R (22784); the last level-0 tensor W0b reuses S0.  The network input
+// (FP32) sits in R before level 1 starts.
 // ---------------------------------------------------------------------------
-struct Unet32Off {
-    long woff[15], boff[15];
-};
+// offsets of layer l's weights and bias in the flat parameter vector (per
+// layer: 9 Cin Cout weights, then Cout biases), compile-time constants so
+// that the one-image network keeps no per-layer offsets in registers
+// (cfd_unet_create checks them against the layer table)
+namespace img_layout {
+constexpr int CIN[15] = {1, 16, 16, 32, 32, 64, 64, 64, 128, 64, 96, 32, 48, 16, 16};
+constexpr int COUT[15] = {16, 16, 32, 32, 64, 64, 64, 64, 64, 64, 32, 32, 16, 16, 1};
+constexpr long woff(int l) {
+    long o = 0;
+    for (int k = 0; k < l; ++k) o += 9L * CIN[k] * COUT[k] + COUT[k];
+    return o;
+}
+constexpr long boff(int l) { return woff(l) + 9L * CIN[l] * COUT[l]; }
+}  // namespace img_layout
+
+// B operands of a matrix-core layer in lane order: fragment (ch, st) for
+// output chunk ch and K step st (tap t = st / nk, input channels
+// 16 (st % nk) .. +15, nk = Cin / 16) holds, for lane = m + 16 q, the weights
+// W[t][16 ch + m][16 (st % nk) + 4 q .. + 3] of the [tap][Cout][Cin] layout
+__global__ void k_swizzle_b(const unsigned short* __restrict__ wb, int Cin, int Cout,
+                            unsigned short* __restrict__ out) {
+    const long n = 9L * Cin * Cout;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int nk = Cin / 16, nsteps = 9 * nk;
+    const int j = (int)(e & 3), lane = (int)((e >> 2) & 63);
+    const long frag = e >> 8;
+    const int st = (int)(frag % nsteps), ch = (int)(frag / nsteps);
+    const int t = st / nk, k0 = (st % nk) * 16;
+    const int m = lane & 15, kq = 4 * (lane >> 4);
+    out[e] = wb[((long)t * Cout + ch * 16 + m) * Cin + k0 + kq + j];
+}
 
 namespace {
 
-// 3x3 convolution of S x S images held in LDS (X1 read directly, through 2x2
-// pooling of a 2S x 2S tensor (rs1 = 1) or nearest upsampling of an S/2 x S/2
-// one (rs1 = 2); X2 direct), bias + ReLU, bf16 out to LDS
-__device__ __forceinline__ void conv32_lds(const unsigned short* X1, int C1, int rs1,
-                                           const unsigned short* X2, int C2, int S,
-                                           const unsigned short* __restrict__ Wt,
-                                           const float* __restrict__ bias, int Cout,
-                                           unsigned short* Y) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    const int m = lane % 16, kq = 4 * (lane / 16);
-    const int tiles_w = (S + 15) / 16, ntile = S * tiles_w, nchunk = Cout / 16;
-    const int Cin = C1 + C2, nk = Cin / 16, nsteps = 9 * nk;
-    for (int task = wave; task < ntile * nchunk; task += nwave) {
-        const int ch = task % nchunk, tile = task / nchunk;
-        const int h = tile / tiles_w, w0 = (tile % tiles_w) * 16;
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        auto load = [&](int st, bf16x4& a, bf16x4& b) {
-            const int t = st / nk, k0 = (st - t * nk) * 16;
-            const int hh = h + t / 3 - 1, ww = w0 + m + t % 3 - 1;
-            a = (bf16x4){0, 0, 0, 0};
-            if (hh >= 0 && hh < S && ww >= 0 && ww < S) {
-                if (k0 >= C1) {
-                    a = *reinterpret_cast<const bf16x4*>(X2 + (hh * S + ww) * C2 + (k0 - C1) + kq);
-                } else if (rs1 == 0) {
-                    a = *reinterpret_cast<const bf16x4*>(X1 + (hh * S + ww) * C1 + k0 + kq);
-                } else if (rs1 == 2) {
-                    a = *reinterpret_cast<const bf16x4*>(
-                        X1 + ((hh >> 1) * (S >> 1) + (ww >> 1)) * C1 + k0 + kq);
-                } else {
-                    const int S2 = 2 * S;
-                    const unsigned short* p0 = X1 + ((2 * hh) * S2 + 2 * ww) * C1 + k0 + kq;
-                    const bf16x4 q0 = *reinterpret_cast<const bf16x4*>(p0);
-                    const bf16x4 q1 = *reinterpret_cast<const bf16x4*>(p0 + C1);
-                    const bf16x4 q2 = *reinterpret_cast<const bf16x4*>(p0 + S2 * C1);
-                    const bf16x4 q3 = *reinterpret_cast<const bf16x4*>(p0 + S2 * C1 + C1);
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const float sm = bf2f((unsigned short)q0[e]) + bf2f((unsigned short)q1[e]) +
-                                         bf2f((unsigned short)q2[e]) + bf2f((unsigned short)q3[e]);
-                        a[e] = (short)f2bf(0.25f * sm);
-                    }
-                }
-            }
-            b = *reinterpret_cast<const bf16x4*>(Wt + ((long)t * Cout + ch * 16 + m) * Cin + k0 + kq);
-        };
-        // operands of 8 steps in flight before their matrix-core instructions
-        constexpr int U = 8;
-        for (int s0 = 0; s0 < nsteps; s0 += U) {
-            bf16x4 a[U], b[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (s0 + u < nsteps) {
-                    load(s0 + u, a[u], b[u]);
-                } else {
-                    a[u] = (bf16x4){0, 0, 0, 0};
-                    b[u] = (bf16x4){0, 0, 0, 0};
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], b[u], acc, 0, 0, 0);
-        }
-        const int co = ch * 16 + m;
-        const float bv = bias[co];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int w = w0 + kq + j;
-            if (w < S) Y[(h * S + w) * Cout + co] = f2bf(fmaxf(acc[j] + bv, 0.f));
-        }
+constexpr int IMG_WAVES = 16;   // wavefronts of the one-image work-group
+constexpr int IMG_PAD = 4;      // extra BF16 elements per pixel in LDS
+
+// 2x2 average pooling of a 2S x 2S x C tensor into S x S x C (BF16 in LDS,
+// pixel strides C + IMG_PAD; the sum of the four values in FP32, times 0.25,
+// rounded to BF16: the per-layer kernels' pooling read)
+template <int S, int C>
+__device__ __forceinline__ void pool_img(const unsigned short* X, unsigned short* P) {
+    constexpr int S2 = 2 * S, CS = C + IMG_PAD;
+    for (int e = threadIdx.x; e < S * S * C; e += blockDim.x) {
+        const int c = e % C, p = e / C, h = p / S, w = p % S;
+        const unsigned short* p0 = X + ((2 * h) * S2 + 2 * w) * CS + c;
+        const float sm = bf2f(p0[0]) + bf2f(p0[CS]) + bf2f(p0[S2 * CS]) + bf2f(p0[S2 * CS + CS]);
+        P[p * CS + c] = f2bf(0.25f * sm);
     }
     __syncthreads();
 }
 
-}  // namespace
+// 3x3 convolution of an S x S image held in LDS: input channels X1 (C1 of
+// them, read directly (UP = 0) or through nearest upsampling of an
+// S/2 x S/2 tensor (UP = 1)) then X2 (C2, direct; with C2 = 0 X2 is never
+// read, but pass a valid LDS pointer: a null one has crashed the compiler),
+// bias + ReLU, BF16 out to LDS.  Bsw: the layer's lane-ordered B fragments;
+// red: LDS scratch of 4096 floats for the split K loop (S = 4 only).  Shapes
+// are template constants and the K loop is unrolled; not inlined, so that the
+// layers' registers are not live at once.
+template <int UP, int C1, int C2, int S, int COUT>
+__device__ __attribute__((noinline)) void conv_img(const unsigned short* X1,
+                                                   const unsigned short* X2,
+                                                   const unsigned short* __restrict__ Bsw,
+                                                   const float* __restrict__ bias,
+                                                   unsigned short* Y, float* red) {
+    static_assert(COUT == 16 || COUT == 32 || COUT == 64, "output channels");
+    static_assert(C1 % 16 == 0 && C2 % 16 == 0 && S >= 4, "shapes");
+    constexpr int CS1 = C1 + IMG_PAD, CS2 = C2 + IMG_PAD, CSO = COUT + IMG_PAD;
+    constexpr int ntile = S * S / 16, nchunk = COUT / 16, pairs = ntile * nchunk;
+    constexpr int nk1 = C1 / 16, nk = (C1 + C2) / 16, nsteps = 9 * nk;
+    constexpr int ksplit = pairs >= IMG_WAVES ? 1 : IMG_WAVES / pairs;
+    constexpr int T = pairs >= IMG_WAVES ? pairs / IMG_WAVES : 1;   // tiles per wavefront
+    constexpr int kst = nsteps / ksplit;                             // K steps per wavefront
+    static_assert(T <= 4 && pairs % (IMG_WAVES / ksplit) == 0, "work split");
+    static_assert(kst % 9 == 0, "K steps per wavefront");
+    static_assert(ksplit == 1 || pairs * ksplit * 256 <= 4096, "reduction scratch");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, kq = 4 * (lane >> 4);
+    const int pw = wave % (IMG_WAVES / ksplit);       // (tile, chunk) work item
+    const int kpart = wave / (IMG_WAVES / ksplit);
+    const int ch = pw % nchunk, t0 = pw / nchunk;
+    constexpr int tstride = IMG_WAVES / ksplit / nchunk;   // tile stride between a wave's tiles
+    int ph[T], pwc[T];   // this lane's A-row pixel (h, w) per tile
+#pragma unroll
+    for (int q = 0; q < T; ++q) {
+        const int p = (t0 + q * tstride) * 16 + m;
+        ph[q] = p / S;
+        pwc[q] = p % S;
+    }
+    f32x4 acc[T];
+#pragma unroll
+    for (int q = 0; q < T; ++q) acc[q] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    const int sbeg = kpart * kst;
+    const bf16x4* B = reinterpret_cast<const bf16x4*>(Bsw) + ((long)ch * nsteps + sbeg) * 64 + lane;
+    constexpr int U = 9;   // K steps per batch of B fragments
+    bf16x4 bc[U], bn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) bc[u] = B[u * 64];
+#pragma unroll
+    for (int s0 = 0; s0 < kst; s0 += U) {
+        if (s0 + U < kst) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) bn[u] = B[(s0 + U + u) * 64];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            // the wave's K range starts at a multiple of 9 steps: tap and
+            // channel chunk follow from the step within the range when
+            // ksplit divides nk, else from the absolute step
+            const int st = sbeg + s0 + u, t = st / nk, kc = st - t * nk;
+            const int dh = t / 3 - 1, dw = t % 3 - 1;
+#pragma unroll
+            for (int q = 0; q < T; ++q) {
+                const int hh = ph[q] + dh, ww = pwc[q] + dw;
+                const bool in = hh >= 0 && hh < S && ww >= 0 && ww < S;
+                const int h2 = in ? hh : 0, w2 = in ? ww : 0;
+                bf16x4 a;
+                if (kc >= nk1)
+                    a = *reinterpret_cast<const bf16x4*>(X2 + (h2 * S + w2) * CS2 + (kc - nk1) * 16 + kq);
+                else if (UP)
+                    a = *reinterpret_cast<const bf16x4*>(
+                        X1 + ((h2 >> 1) * (S / 2) + (w2 >> 1)) * CS1 + kc * 16 + kq);
+                else
+                    a = *reinterpret_cast<const bf16x4*>(X1 + (h2 * S + w2) * CS1 + kc * 16 + kq);
+                if (!in) a = (bf16x4){0, 0, 0, 0};
+                acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bc[u], acc[q], 0, 0, 0);
+            }
+        }
+        if (s0 + U < kst) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) bc[u] = bn[u];
+        }
+    }
+    const int co = ch * 16 + m;
+    const float bv = bias[co];
+    if constexpr (ksplit > 1) {
+        // partial accumulators of the K parts, added in part order
+        float* rp = red + ((pw * ksplit + kpart) * 64 + lane) * 4;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) rp[j] = acc[0][j];
+        __syncthreads();
+        if (kpart == 0) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float v = 0.f;
+#pragma unroll
+                for (int k = 0; k < ksplit; ++k) v += red[((pw * ksplit + k) * 64 + lane) * 4 + j];
+                const int p = t0 * 16 + kq + j;
+                Y[p * CSO + co] = f2bf(fmaxf(v + bv, 0.f));
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < T; ++q)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int p = (t0 + q * tstride) * 16 + kq + j;
+                Y[p * CSO + co] = f2bf(fmaxf(acc[q][j] + bv, 0.f));
+            }
+    }
+    __syncthreads();
+}
 
-__global__ void __launch_bounds__(1024) k_unet32(const float* __restrict__ x, float* __restrict__ y,
-                                                 const unsigned short* __restrict__ wb,
-                                                 const float* __restrict__ w, Unet32Off off,
-                                                 const int* skip) {
-    if (skip && *skip) return;
-    __shared__ __align__(16) unsigned short S0[1024 * 16], W0a[1024 * 16];
-    __shared__ __align__(16) unsigned short R[38912];   // levels 1-3, later W0b
-    unsigned short* S1 = R;                 // 256 x 32
-    unsigned short* W1a = R + 8192;         // 256 x 32
-    unsigned short* W1b = R + 16384;        // 256 x 32
-    unsigned short* S2 = R + 24576;         // 64 x 64
-    unsigned short* W2a = R + 28672;        // 64 x 64
-    unsigned short* W2b = R + 32768;        // 64 x 64
-    unsigned short* W3a = R + 36864;        // 16 x 64
-    unsigned short* W3b = R + 37888;        // 16 x 64
-    unsigned short* W0b = R;                // 1024 x 16 (after levels 1-3 are dead)
-    const float* xi = x + (long)blockIdx.x * 1024;
+// LDS of the one-image network (BF16 elements): S0 and W0a, and R
+constexpr int IMG_LDS_S0 = 1024 * (16 + IMG_PAD);
+constexpr int IMG_LDS_R = 256 * (32 + IMG_PAD) * 2 + 64 * (64 + IMG_PAD);
+
+// The network on one 32 x 32 image: xin (FP32, 1024 values in LDS, may lie in
+// R), out(p, y) called once per pixel p = 32 h + w with the network output.
+// wsw: lane-ordered B fragments (at the weight offsets woff of the MFMA
+// layers), w: FP32 parameters (input/output layer weights, all biases).  All
+// threads of the 1024-thread work-group must call it.
+template <typename Out>
+__device__ __forceinline__ void unet_img32(const float* xin, unsigned short* S0,
+                                           unsigned short* W0a, unsigned short* R,
+                                           const unsigned short* __restrict__ wsw,
+                                           const float* __restrict__ w, Out out) {
+    using img_layout::woff;
+    using img_layout::boff;
+    constexpr int P16 = 16 + IMG_PAD, P32 = 32 + IMG_PAD, P64 = 64 + IMG_PAD;
+    unsigned short* W1a = W0a;                       // 256 x 32  (inside W0a, levels 1-3)
+    unsigned short* W2a = W0a + 256 * P32;           // 64 x 64
+    unsigned short* W2b = W2a + 64 * P64;            // 64 x 64
+    unsigned short* W3a = W2b + 64 * P64;            // 16 x 64
+    unsigned short* W3b = W3a + 16 * P64;            // 16 x 64
+    unsigned short* S1 = R;                          // 256 x 32
+    unsigned short* W1b = R + 256 * P32;             // 256 x 32
+    unsigned short* S2 = W1b + 256 * P32;            // 64 x 64
+    unsigned short* W0b = S0;                        // 1024 x 16 (after S0 is dead)
+    float* red = reinterpret_cast<float*>(W1a);      // split-K partials (levels 3)
+    static_assert(256 * P32 + 2 * 64 * P64 + 2 * 16 * P64 <= IMG_LDS_S0, "W0a holds levels 1-3");
     // input layer 1 -> 16 (scalar), ReLU
     {
-        const float* W = w + off.woff[0];
-        const float* b = w + off.boff[0];
+        const float* W = w + woff(0);
+        const float* b = w + boff(0);
         for (int k = threadIdx.x; k < 1024 * 16; k += blockDim.x) {
             const int c = k % 16, p = k / 16, h = p / 32, ww = p % 32;
             float s = b[c];
             for (int t = 0; t < 9; ++t) {
                 const int hh = h + t / 3 - 1, w2 = ww + t % 3 - 1;
-                if (hh >= 0 && hh < 32 && w2 >= 0 && w2 < 32) s += W[t * 16 + c] * xi[hh * 32 + w2];
+                if (hh >= 0 && hh < 32 && w2 >= 0 && w2 < 32) s += W[t * 16 + c] * xin[hh * 32 + w2];
             }
-            W0a[k] = f2bf(fmaxf(s, 0.f));
+            W0a[p * P16 + c] = f2bf(fmaxf(s, 0.f));
         }
         __syncthreads();
     }
-    auto L = [&](int l, const unsigned short* X1, int C1, int rs1, const unsigned short* X2,
-                 int C2, int S, int Cout, unsigned short* Y) {
-        conv32_lds(X1, C1, rs1, X2, C2, S, wb + off.woff[l], w + off.boff[l], Cout, Y);
-    };
-    L(1, W0a, 16, 0, nullptr, 0, 32, 16, S0);     // A0B
-    L(2, S0, 16, 1, nullptr, 0, 16, 32, W1a);     // A1A (pool)
-    L(3, W1a, 32, 0, nullptr, 0, 16, 32, S1);     // A1B
-    L(4, S1, 32, 1, nullptr, 0, 8, 64, W2a);      // A2A (pool)
-    L(5, W2a, 64, 0, nullptr, 0, 8, 64, S2);      // A2B
-    L(6, S2, 64, 1, nullptr, 0, 4, 64, W3a);      // A3A (pool)
-    L(7, W3a, 64, 0, nullptr, 0, 4, 64, W3b);     // A3B
-    L(8, W3b, 64, 2, S2, 64, 8, 64, W2a);         // B2A (up + skip)
-    L(9, W2a, 64, 0, nullptr, 0, 8, 64, W2b);     // B2B
-    L(10, W2b, 64, 2, S1, 32, 16, 32, W1a);       // B1A
-    L(11, W1a, 32, 0, nullptr, 0, 16, 32, W1b);   // B1B
-    L(12, W1b, 32, 2, S0, 16, 32, 16, W0a);       // B0A
-    L(13, W0a, 16, 0, nullptr, 0, 32, 16, W0b);   // B0B (levels 1-3 dead)
+    const float* bias = w;
+    // pooled inputs go to regions that are free at the time (W1b, W2b, W3b)
+    conv_img<0, 16, 0, 32, 16>(W0a, W0a, wsw + woff(1), bias + boff(1), S0, red);      // A0B
+    pool_img<16, 16>(S0, W1b);                                                        // P1
+    conv_img<0, 16, 0, 16, 32>(W1b, W1b, wsw + woff(2), bias + boff(2), W1a, red);     // A1A
+    conv_img<0, 32, 0, 16, 32>(W1a, W1a, wsw + woff(3), bias + boff(3), S1, red);      // A1B
+    pool_img<8, 32>(S1, W2b);                                                         // P2
+    conv_img<0, 32, 0, 8, 64>(W2b, W2b, wsw + woff(4), bias + boff(4), W2a, red);      // A2A
+    conv_img<0, 64, 0, 8, 64>(W2a, W2a, wsw + woff(5), bias + boff(5), S2, red);       // A2B
+    pool_img<4, 64>(S2, W3b);                                                         // P3
+    conv_img<0, 64, 0, 4, 64>(W3b, W3b, wsw + woff(6), bias + boff(6), W3a, red);      // A3A
+    conv_img<0, 64, 0, 4, 64>(W3a, W3a, wsw + woff(7), bias + boff(7), W3b, red);      // A3B
+    conv_img<1, 64, 64, 8, 64>(W3b, S2, wsw + woff(8), bias + boff(8), W2a, red);      // B2A
+    conv_img<0, 64, 0, 8, 64>(W2a, W2a, wsw + woff(9), bias + boff(9), W2b, red);      // B2B
+    conv_img<1, 64, 32, 16, 32>(W2b, S1, wsw + woff(10), bias + boff(10), W1a, red);   // B1A
+    conv_img<0, 32, 0, 16, 32>(W1a, W1a, wsw + woff(11), bias + boff(11), W1b, red);   // B1B
+    conv_img<1, 32, 16, 32, 16>(W1b, S0, wsw + woff(12), bias + boff(12), W0a, red);   // B0A
+    conv_img<0, 16, 0, 32, 16>(W0a, W0a, wsw + woff(13), bias + boff(13), W0b, red);   // B0B
     // output layer 16 -> 1 (linear)
     {
-        const float* W = w + off.woff[14];
-        const float b = w[off.boff[14]];
-        float* yo = y + (long)blockIdx.x * 1024;
+        const float* W = w + woff(14);
+        const float b = w[boff(14)];
         for (int p = threadIdx.x; p < 1024; p += blockDim.x) {
             const int h = p / 32, ww = p % 32;
             float s = b;
             for (int t = 0; t < 9; ++t) {
                 const int hh = h + t / 3 - 1, w2 = ww + t % 3 - 1;
                 if (hh < 0 || hh >= 32 || w2 < 0 || w2 >= 32) continue;
-                const unsigned short* xp = W0b + (hh * 32 + w2) * 16;
+                const unsigned short* xp = W0b + (hh * 32 + w2) * P16;
                 for (int c = 0; c < 16; ++c) s += W[t * 16 + c] * bf2f(xp[c]);
             }
-            yo[p] = s;
+            out(p, s);
         }
     }
+}
+
+}  // namespace
+
+// batches of 32 x 32 images (image k = rows 32 k .. 32 k + 31 of x and y):
+// the network on one image per work-group
+__global__ void __launch_bounds__(1024) k_unet32(const float* __restrict__ x, float* __restrict__ y,
+                                                 const unsigned short* __restrict__ wsw,
+                                                 const float* __restrict__ w, const int* skip) {
+    if (skip && *skip) return;
+    __shared__ __align__(16) unsigned short S0[IMG_LDS_S0], W0a[IMG_LDS_S0];
+    __shared__ __align__(16) unsigned short R[IMG_LDS_R];
+    float* xin = reinterpret_cast<float*>(R);
+    const float* xi = x + (long)blockIdx.x * 1024;
+    for (int p = threadIdx.x; p < 1024; p += blockDim.x) xin[p] = xi[p];
+    __syncthreads();
+    float* yo = y + (long)blockIdx.x * 1024;
+    unet_img32(xin, S0, W0a, R, wsw, w, [&](int p, float v) { yo[p] = v; });
 }

@@ -63,6 +63,7 @@ struct cfd_unet {
     unet::Layer L[unet::NL];
     float* w = nullptr;             // fp32 masters
     unsigned short* wb = nullptr;   // bf16 copy
+    unsigned short* wsw = nullptr;  // bf16 matrix-core weights in lane order (k_swizzle_b)
     unsigned short* wflip = nullptr;  // flipped/transposed bf16 weights (one layer at a time)
     float *g = nullptr, *m1 = nullptr, *m2 = nullptr;
     unsigned short* act[unet::NACT] = {};
@@ -96,6 +97,20 @@ int ualloc(cfd_unet* u, T** p, size_t n) {
 }
 
 inline unsigned nblk(long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+// the lane-ordered copy of the matrix-core layers' bf16 weights (after every
+// change of wb)
+int unet_swizzle(cfd_unet* u) {
+    for (int l = 0; l < unet::NL; ++l) {
+        const unet::Layer& Ly = u->L[l];
+        if (Ly.type != unet::L_MFMA) continue;
+        const int cin = Ly.cin1 + Ly.cin2;
+        hipLaunchKernelGGL(k_swizzle_b, dim3(nblk(9L * cin * Ly.cout)), dim3(256), 0, u->stream,
+                           u->wb + Ly.woff, cin, Ly.cout, u->wsw + Ly.woff);
+        HIPC(hipGetLastError());
+    }
+    return 0;
+}
 
 int conv_mfma(cfd_unet* u, const unsigned short* x1, int c1, const unsigned short* x2, int c2,
               const unsigned short* wt, const float* bias, int H, int W, int cout,
@@ -131,13 +146,8 @@ int unet_forward(cfd_unet* u, bool infer = false) {
         // batches of >= 64 images of 32 x 32: the whole network in one
         // work-group per image (one CU each; for fewer images the per-layer
         // kernels, which spread every layer over many CUs, are faster)
-        Unet32Off off;
-        for (int l = 0; l < NL; ++l) {
-            off.woff[l] = u->L[l].woff;
-            off.boff[l] = u->L[l].boff;
-        }
         hipLaunchKernelGGL(k_unet32, dim3(u->H / 32), dim3(1024), 0, u->stream, u->xin, u->y,
-                           u->wb, u->w, off, u->skip);
+                           u->wsw, u->w, u->skip);
         HIPC(hipGetLastError());
         return 0;
     }
@@ -313,6 +323,10 @@ int cfd_unet_create(int32_t H, int32_t W, cfd_unet** out) {
         off += 9L * (Ly.cin1 + Ly.cin2) * Ly.cout;
         Ly.boff = off;
         off += Ly.cout;
+        if (Ly.woff != img_layout::woff(l) || Ly.boff != img_layout::boff(l)) {
+            delete u;
+            return set_err(CFD_ERR_STATE, "layer %d offsets disagree with the one-image network", l);
+        }
     }
     u->npar = off;
     auto fail = [&](int rc) {
@@ -340,6 +354,7 @@ int cfd_unet_create(int32_t H, int32_t W, cfd_unet** out) {
     }
     u->part_n = pn;
     if ((rc = ualloc(u, &u->w, u->npar)) || (rc = ualloc(u, &u->wb, u->npar)) ||
+        (rc = ualloc(u, &u->wsw, u->npar)) ||
         (rc = ualloc(u, &u->wflip, 9L * 128 * 64)) || (rc = ualloc(u, &u->g, u->npar)) ||
         (rc = ualloc(u, &u->m1, u->npar)) || (rc = ualloc(u, &u->m2, u->npar)) ||
         (rc = ualloc(u, &u->gbuf, maxact)) || (rc = ualloc(u, &u->xin, n0)) ||
@@ -365,6 +380,7 @@ int cfd_unet_set_params(cfd_unet* u, const float* host) {
     hipLaunchKernelGGL(k_to_bf16, dim3(nblk(u->npar)), dim3(256), 0, u->stream, u->w, u->npar,
                        u->wb);
     HIPC(hipGetLastError());
+    TRY(unet_swizzle(u));
     HIPC(hipMemset(u->m1, 0, u->npar * sizeof(float)));
     HIPC(hipMemset(u->m2, 0, u->npar * sizeof(float)));
     u->steps = 0;
@@ -463,6 +479,7 @@ int cfd_unet_adam(cfd_unet* u, float lr, float scale) {
     hipLaunchKernelGGL(k_adam, dim3(nblk(u->npar)), dim3(256), 0, u->stream, u->w, u->g, u->m1,
                        u->m2, u->npar, lr, b1, b2, 1e-8f, c1, c2, u->wb);
     HIPC(hipGetLastError());
+    TRY(unet_swizzle(u));
     TRY(cfd_unet_zero_grad(u));
     return 0;
 }

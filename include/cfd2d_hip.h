@@ -256,6 +256,10 @@ typedef struct cfd_learned_desc {
     float* harvest_in;        /* device, harvest_cap x H x W floats (mode 2) */
     float* harvest_out;
     int64_t harvest_cap;
+    int32_t coarse_fused;     /* 1: levels with <= 64 x 64 cells, the network and the
+                                 level-K correction in one work-group launch (modes 0 and
+                                 1 on a 32 x 32 level K; ignored otherwise) */
+    int32_t pad;
 } cfd_learned_desc;
 typedef struct cfd_learned_stats_t {
     int64_t harvested, solves, iterations, fallbacks;

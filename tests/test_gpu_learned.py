@@ -228,3 +228,23 @@ def test_network_switched_off_outside_its_training(gpu):
     assert max(net.cg_relres) <= 1e-6 and all(i == 0 for i in net.cg_info)
     assert st["fallbacks"] == 0
     assert np.mean(net.cg_iterations) < 1.5 * np.mean(none.cg_iterations) + 8
+
+
+@pytest.mark.parametrize("mode", ["net", "none"])
+def test_fused_coarse_end_matches_launch_sequence(gpu, mode):
+    """The coarse end of the V-cycle in one work-group (k_lv_coarse: the
+    levels of at most 64^2 cells, the network, the level-K correction)
+    against the same preconditioner as separate launches: the same iteration
+    counts to within one per solve on average, velocities within the solve
+    tolerance.  (The sweeps are the same FP32 arithmetic; the statistics c, s
+    and the network's accumulation order differ in rounding.)"""
+    from cfd2d_amd.learned import LearnedConfig
+
+    a = _run(_cavity(256, "cg_learned", LearnedConfig(mode=mode, coarse_fused=True)), 20)
+    b = _run(_cavity(256, "cg_learned", LearnedConfig(mode=mode, coarse_fused=False)), 20)
+    ia, ib = np.mean(a.cg_iterations), np.mean(b.cg_iterations)
+    print(f"{mode}: fused {ia:.2f} iterations, separate launches {ib:.2f}; "
+          f"max|du| {_cmp(a.u, b.u):.1e}")
+    assert max(a.cg_relres) <= 1e-6 and all(i == 0 for i in a.cg_info)
+    assert abs(ia - ib) <= 1.0
+    assert _cmp(a.u, b.u) < 1e-6 and _cmp(a.v, b.v) < 1e-6
