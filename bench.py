@@ -69,9 +69,13 @@ def gaussian_blobs(n: int, seed: int = 0, count: int = 8) -> np.ndarray:
     return phi
 
 
+# the learned legs start each solve from the linear extrapolation in time of
+# the last two solutions; cg_learned_x0prev is the headline solver with the
+# reference's warm start (the last solution), cg and cg_mixed keep that start
 LEGS = {"cg": ("cg", None), "cg_mixed": ("cg_mixed", None),
-        "cg_learned": ("cg_learned", {"mode": "net"}),
-        "cg_learned_none": ("cg_learned", {"mode": "none"})}
+        "cg_learned": ("cg_learned", {"mode": "net", "x0": "extrapolate"}),
+        "cg_learned_none": ("cg_learned", {"mode": "none", "x0": "extrapolate"}),
+        "cg_learned_x0prev": ("cg_learned", {"mode": "net", "x0": "previous"})}
 
 
 USE_GRAPH = True   # learned legs: replay each iteration as a captured HIP graph
@@ -114,6 +118,7 @@ def run_leg(solver: str, steps: int, warmup: int) -> dict:
     divmax = max(r.divergence for r in recs)
     out = {
         "solver": solver,
+        "x0": (LEGS[solver][1] or {}).get("x0", "previous"),
         "steps_per_s": steps / wall,
         "ms_per_step": 1e3 * wall / steps,
         "cg_iters_mean": float(iters.mean()),
@@ -141,8 +146,10 @@ def roofline_for(leg: dict) -> dict:
         # with the r.z and r_old.z partials 29 B), FP64 direction + A d (29 B),
         # FP64 update of x, r, r_old plus the next V-cycle input (73 B); coarse
         # levels add a third of the level-0 float traffic with their
-        # coefficients (21 B); the U-Net on the 32x32 level is negligible
-        bytes_per_launch = n_cells * (8.0 * (3 + 7) + n_it * (169.0 + 21.0))
+        # coefficients (21 B); the U-Net on the 32x32 level is negligible.
+        # The extrapolated initial iterate reads and writes x and x_prev (32 B)
+        x0_bytes = 32.0 if leg.get("x0") == "extrapolate" else 0.0
+        bytes_per_launch = n_cells * (8.0 * (3 + 7) + x0_bytes + n_it * (169.0 + 21.0))
     elif leg["solver"] == "cg_mixed":
         # per flexible-CG iteration: FP64 outer vectors (r, r_old, x, p, q: 104 B/cell)
         # plus the FP32 V-cycle (fine level 50 B/cell, coarse levels 1/3 more
@@ -270,7 +277,7 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
-    ap.add_argument("--legs", default="cg,cg_learned,cg_learned_none,cg_mixed")
+    ap.add_argument("--legs", default="cg,cg_learned,cg_learned_none,cg_learned_x0prev,cg_mixed")
     ap.add_argument("--cpu-child", nargs=2, default=None)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device-to-device (a known-size kernel for "

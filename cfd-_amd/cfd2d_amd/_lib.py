@@ -69,7 +69,7 @@ class LearnedDesc(ctypes.Structure):
                 ("max_iter", _i32), ("fallback", _i32), ("use_graph", _i32),
                 ("sigma", ctypes.c_float), ("one_a_one", _d), ("harvest_in", _vp),
                 ("harvest_out", _vp), ("harvest_cap", _i64), ("coarse_fused", _i32),
-                ("pad", _i32)]
+                ("x0_extrapolate", _i32)]
 
 
 class LearnedStats(ctypes.Structure):
@@ -92,7 +92,8 @@ class EnsClassDesc(ctypes.Structure):
 class EnsDesc(ctypes.Structure):
     _fields_ = [("nclass", _i32), ("cls", EnsClassDesc * ENS_MAX_CLASSES), ("tau", _d),
                 ("rtol", _d), ("maxiter", _i32), ("omega", ctypes.c_float),
-                ("wc", ctypes.c_float), ("autotune", _i32), ("use_graph", _i32)]
+                ("wc", ctypes.c_float), ("autotune", _i32), ("use_graph", _i32),
+                ("x0_extrapolate", _i32), ("pad", _i32)]
 
 
 class EnsStats(ctypes.Structure):

@@ -8,8 +8,11 @@ advanced with AB2, gamma "auto" and tau 0.5 from rest.  Instances of one side
 length form a class; each kernel of a step serves a whole class.  The bulk is
 FP32; the pressure solve is the learned flexible CG (FP64 vectors and dot
 products, true residual <= rtol), whose coarse corrections for all instances
-come from one batched pass of the shared U-Net per iteration.  The predictor's
-work-group shape is chosen per class by timing at setup (``autotune``).
+come from one batched pass of the shared U-Net per iteration; each solve
+starts from the linear extrapolation in time of the instance's last two
+solutions (``x0="extrapolate"``; "previous": from the last one).  The
+predictor's work-group shape is chosen per class by timing at setup
+(``autotune``).
 """
 
 from __future__ import annotations
@@ -91,7 +94,7 @@ class Ensemble:
 
     def __init__(self, instances: list[Instance], net: bool | str = True, params=None,
                  sigma=None, rtol: float = 1e-6, maxiter: int = 200, autotune: bool = True,
-                 use_graph: bool = True):
+                 use_graph: bool = True, x0: str = "extrapolate"):
         self.instances = list(instances)
         sides = sorted({i.n for i in self.instances})
         if any(n not in SIDES for n in sides):
@@ -109,6 +112,9 @@ class Ensemble:
         d.omega, d.wc = 0.8, 1.0
         d.autotune = 1 if autotune else 0
         d.use_graph = 1 if use_graph else 0
+        if x0 not in ("extrapolate", "previous"):
+            raise ValueError(f"x0 must be 'extrapolate' or 'previous', got {x0!r}")
+        d.x0_extrapolate = 1 if x0 == "extrapolate" else 0
         keep = []
         for c, n in enumerate(sides):
             members = [self.instances[k] for k in self.classes[c]]

@@ -44,7 +44,10 @@ class LearnedConfig:
     ``cfd2d_amd/weights``; ``level`` None picks the level automatically.
     ``coarse_fused``: on a 32 x 32 level K, the levels of at most 64 x 64
     cells, the network and the level-K correction run in one work-group
-    launch (k_lv_coarse) instead of ~25 launches."""
+    launch (k_lv_coarse) instead of ~25 launches.
+    ``x0``: "previous" starts each solve from the last solution (SciPy's
+    warm start in the reference), "extrapolate" from the linear extrapolation
+    of the last two solutions in time."""
 
     mode: str = "net"
     params: np.ndarray | None = None
@@ -55,6 +58,7 @@ class LearnedConfig:
     fallback: bool = True
     use_graph: bool = True
     coarse_fused: bool = True
+    x0: str = "previous"
     harvest_cap: int = 0
     weights: str | None = None
     side: int = TARGET_SIDE
@@ -145,6 +149,9 @@ class Attached:
         d.fallback = 1 if cfg.fallback else 0
         d.use_graph = 1 if cfg.use_graph else 0
         d.coarse_fused = 1 if cfg.coarse_fused else 0
+        if cfg.x0 not in ("extrapolate", "previous"):
+            raise ValueError(f"x0 must be 'extrapolate' or 'previous', got {cfg.x0!r}")
+        d.x0_extrapolate = 1 if cfg.x0 == "extrapolate" else 0
         d.sigma = self.sigma
         d.one_a_one = one_a_one(hierarchy, self.level)
         d.harvest_in = self.hin.ptr if self.hin else None

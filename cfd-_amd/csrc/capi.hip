@@ -89,6 +89,8 @@ struct cfd_ctx {
     int l_ready = 0;
     int l_level = 0, l_mode = 0, l_cycles = 0, l_fallback = 1, l_graph = 1, l_maxiter = 200;
     int l_fused = 0, l_lc = 0;   // coarse levels l_lc .. l_level in one k_lv_coarse launch
+    int l_x0 = 0, l_hist = 0;    // extrapolated initial iterate; x_prev holds the last solution
+    double* xprev = nullptr;
     int l_H = 0, l_W = 0;
     float l_sigma = 1.0f, l_D = 1.0f;
     double l_1a1 = 0.0;
@@ -405,12 +407,26 @@ int learned_iteration(cfd_ctx* c) {
     };
     const int nb0 = bands(0), nbK = bands(K);
     const unsigned nimg = (unsigned)(((long)c->l_H * c->l_W + 255) / 256);
+    // the legs of the V-cycle on level l, and the level-0 up-leg ending in
+    // z with the partials of r.z and r_old.z
+    auto down = [&](int l) -> int {
+        LV_LAUNCH(k_lv_smooth, cells(l), a, l, 0);
+        LV_LAUNCH(k_lv_restrict, cells(l + 1), a, l);
+        return 0;
+    };
+    auto up = [&](int l) -> int {
+        LV_LAUNCH(k_lv_post, cells(l), a, l);
+        LV_LAUNCH(k_lv_smooth, cells(l), a, l, 0);
+        return 0;
+    };
+    auto up0 = [&]() -> int {
+        LV_LAUNCH(k_lv_post, cells(0), a, 0);
+        LV_LAUNCH512(k_lv_smooth_dots, nb0, a, v);
+        return 0;
+    };
     if (c->l_fused) {
         const int Lc = c->l_lc;
-        for (int l = 0; l < Lc; ++l) {
-            LV_LAUNCH(k_lv_smooth, cells(l), a, l, 0);
-            LV_LAUNCH(k_lv_restrict, cells(l + 1), a, l);
-        }
+        for (int l = 0; l < Lc; ++l) TRY(down(l));
         CoarseNet cn;
         std::memset(&cn, 0, sizeof(cn));
         if (c->l_mode == LEARNED_NET) {
@@ -423,22 +439,15 @@ int learned_iteration(cfd_ctx* c) {
         cn.one_a_one = c->l_1a1;
         hipLaunchKernelGGL(k_lv_coarse, dim3(1), dim3(1024), 0, c->stream, a, Lc, K, cn);
         HIPC(hipGetLastError());
-        for (int l = Lc - 1; l >= 1; --l) {
-            LV_LAUNCH(k_lv_post, cells(l), a, l);
-            LV_LAUNCH(k_lv_smooth, cells(l), a, l, 0);
-        }
-        LV_LAUNCH(k_lv_post, cells(0), a, 0);
-        LV_LAUNCH512(k_lv_smooth_dots, nb0, a, v);
+        for (int l = Lc - 1; l >= 1; --l) TRY(up(l));
+        TRY(up0());
         LV_LAUNCH512(k_fcg_dir, nb0, a, v, nb0);
         LV_LAUNCH512(k_fcg_update, nb0, a, v, nb0);
         LV_LAUNCH512(k_fcg_true, nb0, a, v, nb0);
         LV_LAUNCH(k_fcg_true_fin, 1, a, nb0, c->l_mode == LEARNED_NET ? 1 : 0);
         return 0;
     }
-    for (int l = 0; l < K; ++l) {
-        LV_LAUNCH(k_lv_smooth, cells(l), a, l, 0);
-        LV_LAUNCH(k_lv_restrict, cells(l + 1), a, l);
-    }
+    for (int l = 0; l < K; ++l) TRY(down(l));
     LV_LAUNCH(k_lv_cmode, nbK, a, K);
     if (c->l_mode != LEARNED_TEACHER) {
         const bool net = c->l_mode == LEARNED_NET;
@@ -481,16 +490,9 @@ int learned_iteration(cfd_ctx* c) {
         if (c->l_hcap > 0)
             LV_LAUNCH(k_lv_harvest, nimg, a, K, c->l_hin, c->l_hout, c->l_H, c->l_W, c->l_D);
     }
-    for (int l = K - 1; l >= 1; --l) {
-        LV_LAUNCH(k_lv_post, cells(l), a, l);
-        LV_LAUNCH(k_lv_smooth, cells(l), a, l, 0);
-    }
-    if (K >= 1) {
-        LV_LAUNCH(k_lv_post, cells(0), a, 0);
-        LV_LAUNCH512(k_lv_smooth_dots, nb0, a, v);
-    } else {
-        LV_LAUNCH512(k_fcg_dots, nb0, a, v);
-    }
+    for (int l = K - 1; l >= 1; --l) TRY(up(l));
+    if (K >= 1) TRY(up0());
+    else LV_LAUNCH512(k_fcg_dots, nb0, a, v);
     LV_LAUNCH512(k_fcg_dir, nb0, a, v, nb0);
     LV_LAUNCH512(k_fcg_update, nb0, a, v, nb0);
     LV_LAUNCH512(k_fcg_true, nb0, a, v, nb0);
@@ -541,6 +543,13 @@ int run_learned(cfd_ctx* c, ProjArgs a) {
             }
         }
         HIPC(hipEventRecord(c->prof_ev[c->prof_used], c->stream));
+    }
+    if (c->l_x0 && a.dt_override == 0.0) {   // time steps only, not the initial projection
+        const long n = (long)(c->nx + 2) * c->P;
+        hipLaunchKernelGGL(k_x0_extrap, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                           c->x, c->xprev, n, (const DevState*)c->st, c->l_hist);
+        HIPC(hipGetLastError());
+        c->l_hist = 1;
     }
     TRY(launch_coop(c, a, false));
     int batch = std::max(1, c->l_prev_iters - 1);
@@ -894,6 +903,9 @@ int cfd_set_learned(cfd_ctx* c, const cfd_learned_desc* d) {
     c->l_fallback = d->fallback ? 1 : 0;
     c->l_graph = d->use_graph ? 1 : 0;
     c->l_maxiter = d->max_iter;
+    c->l_x0 = d->x0_extrapolate ? 1 : 0;
+    c->l_hist = 0;
+    if (c->l_x0 && !c->xprev) TRY(dalloc(c, &c->xprev, (size_t)(c->nx + 2) * c->P));
     // the fused coarse end: a 32 x 32 level K below the fine level, entered at
     // the first level of at most 64 x 64 cells (never the fine level); one
     // work-group sweeps 64 x 64 cells faster than a launch costs, 128 x 128
@@ -953,6 +965,7 @@ int cfd_upload(cfd_ctx* c, int32_t field, const double* host) {
     int par = 0;
     TRY(host_parity(c, &par));
     double* dst = field_ptr(c, field, par);
+    if (field == CFD_FIELD_X) c->l_hist = 0;   // a new iterate starts a new history
     HIPC(hipMemcpy2DAsync(dst + (size_t)oi * c->P + oj, c->P * sizeof(double), host,
                           cols * sizeof(double), cols * sizeof(double), rows,
                           hipMemcpyHostToDevice, c->stream));
@@ -1068,6 +1081,7 @@ int cfd_project_initial(cfd_ctx* c, const cfd_step_params* prm) {
     a.dt_override = 1.0;
     a.zero_p_after = 1;
     TRY(launch_project(c, a));
+    c->l_hist = 0;   // no time history for an extrapolated initial iterate yet
     TRY(launch_bc(c, false));
     HIPC(hipStreamSynchronize(c->stream));
     return 0;

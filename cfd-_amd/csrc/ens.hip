@@ -47,6 +47,7 @@ struct EnsClass {
     const double* kappa;      // thermal diffusivity (0: no temperature)
     float *u, *v, *F, *G, *hu[2], *hv[2], *T[2], *p;
     double *x, *b, *r, *ro, *q, *d0, *d1;
+    double* xp;               // previous solution (extrapolated initial iterate), or null
     EnsLevel lev[ENS_MAXLEV];
     int ncoarse;
     const float* cinv;
@@ -354,6 +355,24 @@ __global__ void __launch_bounds__(256) k_e_rhs(const EnsClass* __restrict__ c) {
             s[0] += rhs;
         }
     ens_partial<1>(s, ens_part(c, ENS_P_A, b), blockIdx.x);
+}
+
+// initial iterate of the instance's solve: x <- x + theta (x - x_prev) with
+// theta = dt / dt_old, x_prev <- the old x (hist = 0: the copy only)
+__global__ void __launch_bounds__(256) k_e_x0(const EnsClass* __restrict__ c, int hist) {
+    const int b = blockIdx.y, n = c->n, P = c->P;
+    const EnsState& st = c->st[b];
+    const double th = hist && st.dt_old > 0.0 ? st.dt / st.dt_old : 0.0;
+    const long o = b * c->S;
+    int i0, i1;
+    ens_band(1, n + 1, blockIdx.x, ENS_BANDS, i0, i1);
+    for (int i = i0; i < i1; ++i)
+        for (int j = 1 + threadIdx.x; j <= n; j += blockDim.x) {
+            const long k = o + ix(i, j, P);
+            const double xn = c->x[k];
+            if (hist) c->x[k] = xn + th * (xn - c->xp[k]);
+            c->xp[k] = xn;
+        }
 }
 
 // b = -(rhs - mean) (compatibility), r = b - A x, first V-cycle input;

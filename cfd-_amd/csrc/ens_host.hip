@@ -19,6 +19,7 @@ struct cfd_ens {
     long long steps = 0;
     int prev_iters = 0;
     int use_graph = 0;
+    int x0_extrap = 0, x0_hist = 0;     // extrapolated initial iterates; history present
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_x = nullptr;
     long long iters_total = 0, solves = 0;
@@ -140,6 +141,7 @@ int ens_step(cfd_ens* e) {
         E_LAUNCH(k_e_dt, dim3(1), dim3(256), dc);
         TRY(ens_pred(e, k, e->pred_tx[k], e->pred_ty[k]));
         E_LAUNCH(k_e_rhs, dim3(ENS_BANDS, c.B), dim3(256), dc);
+        if (e->x0_extrap) E_LAUNCH(k_e_x0, dim3(ENS_BANDS, c.B), dim3(256), dc, e->x0_hist);
         E_LAUNCH(k_e_res, dim3(ENS_BANDS, c.B), dim3(256), dc);
         E_LAUNCH(k_e_init, dim3(1), dim3(256), dc);
     }
@@ -173,6 +175,7 @@ int ens_step(cfd_ens* e) {
     }
     e->par ^= 1;
     e->steps += 1;
+    e->x0_hist = e->x0_extrap;
     return 0;
 }
 
@@ -275,6 +278,8 @@ int cfd_ens_create(const cfd_ens_desc* d, cfd_ens** out) {
         double** f64[] = {&c.x, &c.b, &c.r, &c.ro, &c.q, &c.d0, &c.d1};
         for (double** p : f64)
             if ((rc = ealloc(e, p, N))) return fail(rc);
+        c.xp = nullptr;
+        if (d->x0_extrapolate && (rc = ealloc(e, &c.xp, N))) return fail(rc);
         for (int l = 0; l < cd.nlev; ++l) {
             EnsLevel& L = c.lev[l];
             L.n = cd.n >> l;
@@ -336,6 +341,7 @@ int cfd_ens_create(const cfd_ens_desc* d, cfd_ens** out) {
         return fail(set_err(CFD_ERR_HIP, "class upload failed"));
     if (d->autotune && (rc = ens_autotune(e))) return fail(rc);
     e->use_graph = d->use_graph ? 1 : 0;
+    e->x0_extrap = d->x0_extrapolate ? 1 : 0;
     *out = e;
     return 0;
 }

@@ -954,3 +954,17 @@ __global__ void __launch_bounds__(1024) k_lv_coarse(LvArgs a, int Lc, int K, Coa
         wg_smooth(m, l, L, w, L.x, L.t);
     }
 }
+
+// initial iterate of a time step's learned solve: x <- x + theta (x - x_prev)
+// with theta = dt / dt_old, and x_prev <- the old x (hist = 0: the copy only)
+__global__ void __launch_bounds__(256) k_x0_extrap(double* __restrict__ x, double* __restrict__ xprev,
+                                                   long n, const DevState* st, int hist) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double xn = x[k];
+    if (hist) {
+        const double th = st->dt_old > 0.0 ? st->dt / st->dt_old : 1.0;
+        x[k] = xn + th * (xn - xprev[k]);
+    }
+    xprev[k] = xn;
+}

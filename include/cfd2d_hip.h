@@ -259,7 +259,9 @@ typedef struct cfd_learned_desc {
     int32_t coarse_fused;     /* 1: levels with <= 64 x 64 cells, the network and the
                                  level-K correction in one work-group launch (modes 0 and
                                  1 on a 32 x 32 level K; ignored otherwise) */
-    int32_t pad;
+    int32_t x0_extrapolate;   /* 1: start each solve of a time step from the linear
+                                 extrapolation x + (dt/dt_old)(x - x_previous) of the last
+                                 two solutions; 0: from the last solution (SciPy's x0) */
 } cfd_learned_desc;
 typedef struct cfd_learned_stats_t {
     int64_t harvested, solves, iterations, fallbacks;
@@ -347,6 +349,9 @@ typedef struct cfd_ens_desc {
     float omega, wc;           /* damped-Jacobi weight, coarse-correction weight */
     int32_t autotune;          /* 1: time the predictor's work-group shapes per class */
     int32_t use_graph;         /* 1: replay each solver iteration as a captured HIP graph */
+    int32_t x0_extrapolate;    /* 1: each solve starts from the linear extrapolation in time
+                                  of the instance's last two solutions */
+    int32_t pad;
 } cfd_ens_desc;
 typedef struct cfd_ens_stats_t {
     double t, dt, gamma, kinetic_energy, divergence, residual, relres;

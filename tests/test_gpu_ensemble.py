@@ -95,3 +95,27 @@ def test_full_workload_256_instances(gpu):
     print(f"256 instances, {ens.cells / 1e6:.2f} M cells: {5 * ens.cells / wall / 1e9:.3f} G "
           f"cell-updates/s")
     ens.close()
+
+
+def test_extrapolated_start_saves_iterations(gpu):
+    """Each instance's solve from the extrapolation of its last two solutions
+    against the last solution alone: fewer lockstep iterations, fields within
+    the solve tolerance."""
+    from cfd2d_amd.ensemble import Ensemble
+
+    ins = _mixed()
+    out = {}
+    for x0 in ("extrapolate", "previous"):
+        ens = Ensemble(ins, x0=x0)
+        ens.step(6)
+        s0 = ens.stats(0)["sweeps_total"]
+        ens.step(10)
+        out[x0] = (ens.stats(0)["sweeps_total"] - s0, [ens.field(k, "u") for k in range(len(ins))],
+                   max(ens.stats(k)["relres"] for k in range(len(ins))))
+        ens.close()
+    print(f"iterations over steps 7-16: {out['extrapolate'][0]} extrapolated, "
+          f"{out['previous'][0]} from the last solution")
+    assert out["extrapolate"][2] <= 1e-6
+    assert out["extrapolate"][0] < 0.8 * out["previous"][0]
+    for ue, up in zip(out["extrapolate"][1], out["previous"][1]):
+        assert np.abs(ue - up).max() < 1e-5

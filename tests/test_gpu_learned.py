@@ -248,3 +248,21 @@ def test_fused_coarse_end_matches_launch_sequence(gpu, mode):
     assert max(a.cg_relres) <= 1e-6 and all(i == 0 for i in a.cg_info)
     assert abs(ia - ib) <= 1.0
     assert _cmp(a.u, b.u) < 1e-6 and _cmp(a.v, b.v) < 1e-6
+
+
+def test_extrapolated_start(gpu):
+    """x0 = "extrapolate": each solve starts from x + (dt/dt_old)(x - x_prev);
+    the solves still end on a true relative residual <= 1e-6, the velocities
+    agree with the reference's warm start (the last solution) to the solve
+    tolerance, and they take fewer iterations."""
+    from cfd2d_amd.learned import LearnedConfig
+
+    prev = _run(_cavity(256, "cg_learned", LearnedConfig(x0="previous")), 25)
+    ext = _run(_cavity(256, "cg_learned", LearnedConfig(x0="extrapolate")), 25)
+    ip, ie = np.mean(prev.cg_iterations[5:]), np.mean(ext.cg_iterations[5:])
+    print(f"256^2 steps 6-25: {ie:.2f} iterations extrapolated, {ip:.2f} from the last solution; "
+          f"max|du| {_cmp(ext.u, prev.u):.1e}")
+    assert max(ext.cg_relres) <= 1e-6 and all(i == 0 for i in ext.cg_info)
+    assert ie < 0.8 * ip
+    assert _cmp(ext.u, prev.u) < 1e-6 and _cmp(ext.v, prev.v) < 1e-6
+
