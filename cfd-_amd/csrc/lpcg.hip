@@ -846,6 +846,7 @@ __global__ void __launch_bounds__(1024) k_lv_coarse(LvArgs a, int Lc, int K, Coa
     const MGCtx m{a.mg, a.code, s_d0_, a.periodic, a.ax, a.ay};
     const MGDev& mg = *a.mg;
     const float w = mg.omega, wc = mg.wc;
+    const bool use_net = net.use_net && !fs->net_off;
     // down-leg: pre-smoothing sweep, restriction of the residual
     for (int l = Lc; l < K; ++l) {
         const MGLevelDev L = mg.lev[l];
@@ -899,7 +900,6 @@ __global__ void __launch_bounds__(1024) k_lv_coarse(LvArgs a, int Lc, int K, Coa
         fs->cm = cm;
         fs->s_in = s;
     }
-    const bool use_net = net.use_net && !fs->net_off;
     if (use_net) {
         // network input (FP32 in R), the network, t_K = c + gain s y
         float* xin = reinterpret_cast<float*>(R);

@@ -604,15 +604,24 @@ namespace {
 constexpr int IMG_WAVES = 16;   // wavefronts of the one-image work-group
 constexpr int IMG_PAD = 4;      // extra BF16 elements per pixel in LDS
 
+// pointers with their address space spelled out: the layer functions are not
+// inlined, and through generic pointers every LDS access would carry a 64-bit
+// address conversion and null check, every weight load a flat access
+typedef __attribute__((address_space(3))) unsigned short lds_u16;
+typedef __attribute__((address_space(3))) bf16x4 lds_bf16x4;
+typedef __attribute__((address_space(3))) float lds_f32;
+typedef __attribute__((address_space(1))) bf16x4 glb_bf16x4;
+typedef __attribute__((address_space(1))) float glb_f32;
+
 // 2x2 average pooling of a 2S x 2S x C tensor into S x S x C (BF16 in LDS,
 // pixel strides C + IMG_PAD; the sum of the four values in FP32, times 0.25,
 // rounded to BF16: the per-layer kernels' pooling read)
 template <int S, int C>
-__device__ __forceinline__ void pool_img(const unsigned short* X, unsigned short* P) {
+__device__ __forceinline__ void pool_img(const lds_u16* X, lds_u16* P) {
     constexpr int S2 = 2 * S, CS = C + IMG_PAD;
     for (int e = threadIdx.x; e < S * S * C; e += blockDim.x) {
         const int c = e % C, p = e / C, h = p / S, w = p % S;
-        const unsigned short* p0 = X + ((2 * h) * S2 + 2 * w) * CS + c;
+        const lds_u16* p0 = X + ((2 * h) * S2 + 2 * w) * CS + c;
         const float sm = bf2f(p0[0]) + bf2f(p0[CS]) + bf2f(p0[S2 * CS]) + bf2f(p0[S2 * CS + CS]);
         P[p * CS + c] = f2bf(0.25f * sm);
     }
@@ -628,11 +637,10 @@ __device__ __forceinline__ void pool_img(const unsigned short* X, unsigned short
 // are template constants and the K loop is unrolled; not inlined, so that the
 // layers' registers are not live at once.
 template <int UP, int C1, int C2, int S, int COUT>
-__device__ __attribute__((noinline)) void conv_img(const unsigned short* X1,
-                                                   const unsigned short* X2,
-                                                   const unsigned short* __restrict__ Bsw,
-                                                   const float* __restrict__ bias,
-                                                   unsigned short* Y, float* red) {
+__device__ __attribute__((noinline)) void conv_img(const lds_u16* X1, const lds_u16* X2,
+                                                   const glb_bf16x4* __restrict__ Bsw,
+                                                   const glb_f32* __restrict__ bias, lds_u16* Y,
+                                                   lds_f32* red) {
     static_assert(COUT == 16 || COUT == 32 || COUT == 64, "output channels");
     static_assert(C1 % 16 == 0 && C2 % 16 == 0 && S >= 4, "shapes");
     constexpr int CS1 = C1 + IMG_PAD, CS2 = C2 + IMG_PAD, CSO = COUT + IMG_PAD;
@@ -661,7 +669,7 @@ __device__ __attribute__((noinline)) void conv_img(const unsigned short* X1,
 #pragma unroll
     for (int q = 0; q < T; ++q) acc[q] = (f32x4){0.f, 0.f, 0.f, 0.f};
     const int sbeg = kpart * kst;
-    const bf16x4* B = reinterpret_cast<const bf16x4*>(Bsw) + ((long)ch * nsteps + sbeg) * 64 + lane;
+    const glb_bf16x4* B = Bsw + ((long)ch * nsteps + sbeg) * 64 + lane;
     constexpr int U = 9;   // K steps per batch of B fragments
     bf16x4 bc[U], bn[U];
 #pragma unroll
@@ -674,9 +682,8 @@ __device__ __attribute__((noinline)) void conv_img(const unsigned short* X1,
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            // the wave's K range starts at a multiple of 9 steps: tap and
-            // channel chunk follow from the step within the range when
-            // ksplit divides nk, else from the absolute step
+            // the wave's K range starts at a multiple of 9 steps; tap and
+            // channel chunk follow from the absolute step
             const int st = sbeg + s0 + u, t = st / nk, kc = st - t * nk;
             const int dh = t / 3 - 1, dw = t % 3 - 1;
 #pragma unroll
@@ -686,12 +693,12 @@ __device__ __attribute__((noinline)) void conv_img(const unsigned short* X1,
                 const int h2 = in ? hh : 0, w2 = in ? ww : 0;
                 bf16x4 a;
                 if (kc >= nk1)
-                    a = *reinterpret_cast<const bf16x4*>(X2 + (h2 * S + w2) * CS2 + (kc - nk1) * 16 + kq);
+                    a = *reinterpret_cast<const lds_bf16x4*>(X2 + (h2 * S + w2) * CS2 + (kc - nk1) * 16 + kq);
                 else if (UP)
-                    a = *reinterpret_cast<const bf16x4*>(
+                    a = *reinterpret_cast<const lds_bf16x4*>(
                         X1 + ((h2 >> 1) * (S / 2) + (w2 >> 1)) * CS1 + kc * 16 + kq);
                 else
-                    a = *reinterpret_cast<const bf16x4*>(X1 + (h2 * S + w2) * CS1 + kc * 16 + kq);
+                    a = *reinterpret_cast<const lds_bf16x4*>(X1 + (h2 * S + w2) * CS1 + kc * 16 + kq);
                 if (!in) a = (bf16x4){0, 0, 0, 0};
                 acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bc[u], acc[q], 0, 0, 0);
             }
@@ -705,7 +712,7 @@ __device__ __attribute__((noinline)) void conv_img(const unsigned short* X1,
     const float bv = bias[co];
     if constexpr (ksplit > 1) {
         // partial accumulators of the K parts, added in part order
-        float* rp = red + ((pw * ksplit + kpart) * 64 + lane) * 4;
+        lds_f32* rp = red + ((pw * ksplit + kpart) * 64 + lane) * 4;
 #pragma unroll
         for (int j = 0; j < 4; ++j) rp[j] = acc[0][j];
         __syncthreads();
@@ -774,24 +781,28 @@ __device__ __forceinline__ void unet_img32(const float* xin, unsigned short* S0,
         }
         __syncthreads();
     }
-    const float* bias = w;
+    // LDS and global views of the tensors and weights for the layer functions
+    auto L = [](unsigned short* p) { return (lds_u16*)p; };
+    auto Wt = [&](int l) { return (const glb_bf16x4*)(wsw + woff(l)); };
+    auto Bs = [&](int l) { return (const glb_f32*)(w + boff(l)); };
+    lds_f32* rd = (lds_f32*)red;
     // pooled inputs go to regions that are free at the time (W1b, W2b, W3b)
-    conv_img<0, 16, 0, 32, 16>(W0a, W0a, wsw + woff(1), bias + boff(1), S0, red);      // A0B
-    pool_img<16, 16>(S0, W1b);                                                        // P1
-    conv_img<0, 16, 0, 16, 32>(W1b, W1b, wsw + woff(2), bias + boff(2), W1a, red);     // A1A
-    conv_img<0, 32, 0, 16, 32>(W1a, W1a, wsw + woff(3), bias + boff(3), S1, red);      // A1B
-    pool_img<8, 32>(S1, W2b);                                                         // P2
-    conv_img<0, 32, 0, 8, 64>(W2b, W2b, wsw + woff(4), bias + boff(4), W2a, red);      // A2A
-    conv_img<0, 64, 0, 8, 64>(W2a, W2a, wsw + woff(5), bias + boff(5), S2, red);       // A2B
-    pool_img<4, 64>(S2, W3b);                                                         // P3
-    conv_img<0, 64, 0, 4, 64>(W3b, W3b, wsw + woff(6), bias + boff(6), W3a, red);      // A3A
-    conv_img<0, 64, 0, 4, 64>(W3a, W3a, wsw + woff(7), bias + boff(7), W3b, red);      // A3B
-    conv_img<1, 64, 64, 8, 64>(W3b, S2, wsw + woff(8), bias + boff(8), W2a, red);      // B2A
-    conv_img<0, 64, 0, 8, 64>(W2a, W2a, wsw + woff(9), bias + boff(9), W2b, red);      // B2B
-    conv_img<1, 64, 32, 16, 32>(W2b, S1, wsw + woff(10), bias + boff(10), W1a, red);   // B1A
-    conv_img<0, 32, 0, 16, 32>(W1a, W1a, wsw + woff(11), bias + boff(11), W1b, red);   // B1B
-    conv_img<1, 32, 16, 32, 16>(W1b, S0, wsw + woff(12), bias + boff(12), W0a, red);   // B0A
-    conv_img<0, 16, 0, 32, 16>(W0a, W0a, wsw + woff(13), bias + boff(13), W0b, red);   // B0B
+    conv_img<0, 16, 0, 32, 16>(L(W0a), L(W0a), Wt(1), Bs(1), L(S0), rd);       // A0B
+    pool_img<16, 16>(L(S0), L(W1b));                                           // P1
+    conv_img<0, 16, 0, 16, 32>(L(W1b), L(W1b), Wt(2), Bs(2), L(W1a), rd);      // A1A
+    conv_img<0, 32, 0, 16, 32>(L(W1a), L(W1a), Wt(3), Bs(3), L(S1), rd);       // A1B
+    pool_img<8, 32>(L(S1), L(W2b));                                            // P2
+    conv_img<0, 32, 0, 8, 64>(L(W2b), L(W2b), Wt(4), Bs(4), L(W2a), rd);       // A2A
+    conv_img<0, 64, 0, 8, 64>(L(W2a), L(W2a), Wt(5), Bs(5), L(S2), rd);        // A2B
+    pool_img<4, 64>(L(S2), L(W3b));                                            // P3
+    conv_img<0, 64, 0, 4, 64>(L(W3b), L(W3b), Wt(6), Bs(6), L(W3a), rd);       // A3A
+    conv_img<0, 64, 0, 4, 64>(L(W3a), L(W3a), Wt(7), Bs(7), L(W3b), rd);       // A3B
+    conv_img<1, 64, 64, 8, 64>(L(W3b), L(S2), Wt(8), Bs(8), L(W2a), rd);       // B2A
+    conv_img<0, 64, 0, 8, 64>(L(W2a), L(W2a), Wt(9), Bs(9), L(W2b), rd);       // B2B
+    conv_img<1, 64, 32, 16, 32>(L(W2b), L(S1), Wt(10), Bs(10), L(W1a), rd);    // B1A
+    conv_img<0, 32, 0, 16, 32>(L(W1a), L(W1a), Wt(11), Bs(11), L(W1b), rd);    // B1B
+    conv_img<1, 32, 16, 32, 16>(L(W1b), L(S0), Wt(12), Bs(12), L(W0a), rd);    // B0A
+    conv_img<0, 16, 0, 32, 16>(L(W0a), L(W0a), Wt(13), Bs(13), L(W0b), rd);    // B0B
     // output layer 16 -> 1 (linear)
     {
         const float* W = w + woff(14);
