@@ -522,16 +522,41 @@ int learned_capture(cfd_ctx* c) {
     return 0;
 }
 
-// A projection with the learned flexible CG: k_project up to the initial
-// residual, iterations until the device reports the solve finished (the host
-// polls the state after the predicted number of iterations, then after each
-// one), then k_project from the pressure write-back on; a solve that does not
-// converge is continued by the FP64 Jacobi-PCG of k_project when enabled.
+// start and end of a learned projection as band launches (the steps of
+// k_project before and after its solve, without grid barriers)
+int learned_start(cfd_ctx* c, const ProjArgs& a) {
+    const LvArgs la = lv_args(c);
+    const int nb = std::min(c->nx, 512);
+    LV_LAUNCH512(k_lp_rhs, nb, a, c->geo, c->l_part);
+    LV_LAUNCH512(k_lp_res, nb, a, c->geo, la, nb);
+    LV_LAUNCH(k_lp_init, 1, a, la, nb);
+    return 0;
+}
+
+int learned_end(cfd_ctx* c, const ProjArgs& a) {
+    const LvArgs la = lv_args(c);
+    const int nb = std::min(c->nx, 512);
+    LV_LAUNCH512(k_lp_xsum, nb, a, c->geo, c->l_part);
+    LV_LAUNCH512(k_lp_pwrite, nb, a, c->geo, la, nb);
+    LV_LAUNCH512(k_lp_correct, nb, a, c->geo, c->l_part);
+    LV_LAUNCH(k_lp_fin, 1, a, c->geo, c->l_part, nb);
+    if (a.zero_p_after) {
+        const long n = (long)(c->nx + 2) * c->P;
+        LV_LAUNCH(k_lp_pzero, (unsigned)((n + 255) / 256), c->p, n, (const DevState*)c->st);
+    }
+    return 0;
+}
+
+// A projection with the learned flexible CG: the start launches (right-hand
+// side, initial residual), iterations until the device reports the solve
+// finished (the host polls the state after the predicted number of
+// iterations, then after each one), then the end launches (pressure
+// write-back, correction); a solve that does not converge is continued by the
+// FP64 Jacobi-PCG of k_project when enabled.
 int run_learned(cfd_ctx* c, ProjArgs a) {
     const int maxiter_all = a.maxiter;
     a.fcg = c->fcg;
     a.maxiter = std::min(a.maxiter, c->l_maxiter);
-    a.phase = 1;
     if (c->l_graph && !c->l_graph_x) TRY(learned_capture(c));
     const bool prof = c->prof_on;
     if (prof) {   // the whole projection is timed as one unit
@@ -551,7 +576,7 @@ int run_learned(cfd_ctx* c, ProjArgs a) {
         HIPC(hipGetLastError());
         c->l_hist = 1;
     }
-    TRY(launch_coop(c, a, false));
+    TRY(learned_start(c, a));
     int batch = std::max(1, c->l_prev_iters - 1);
     int launched = 0;
     for (;;) {
@@ -576,12 +601,10 @@ int run_learned(cfd_ctx* c, ProjArgs a) {
         c->l_fallbacks += 1;
         ProjArgs f = a;
         f.method = PROJ_PCG64;
-        f.phase = 0;
         f.maxiter = maxiter_all;
         TRY(launch_coop(c, f, false));
     } else {
-        a.phase = 2;
-        TRY(launch_coop(c, a, false));
+        TRY(learned_end(c, a));
     }
     if (prof) {
         HIPC(hipEventRecord(c->prof_ev[c->prof_used + 1], c->stream));

@@ -152,8 +152,6 @@ struct ProjArgs {
     int zero_p_after;        // initial projection: p := 0 afterwards
     int method;              // PROJ_PCG64, PROJ_MG, PROJ_LEARNED
     double reliable_delta;   // mixed: true-residual replacement when ||r|| drops by this factor
-    int phase;               // 0 whole projection; PROJ_LEARNED: 1 = up to the initial
-                             // residual, 2 = from the pressure write-back on
     struct FcgState* fcg;    // PROJ_LEARNED: state of the host-driven flexible CG
 };
 

@@ -968,3 +968,290 @@ __global__ void __launch_bounds__(256) k_x0_extrap(double* __restrict__ x, doubl
     }
     xprev[k] = xn;
 }
+
+// ---- start and end of a learned projection as band launches ---------------
+// k_project's steps before and after the solve, for the learned solver: the
+// same cell arithmetic (right-hand side, compatibility shift, initial residual and
+// first V-cycle input; pressure mean, write-back, velocity correction and
+// residual monitor), with the sums taken per work-group and added by the
+// consuming kernel in work-group order instead of grid barriers.  Band
+// kernels: min(nx, LV_MAXB) work-groups of 512 threads.
+
+namespace {
+
+// rows of logical band b out of nb over rows 1..nx
+__device__ __forceinline__ void lp_band(int nx, int b, int nb, int& i0, int& i1) {
+    i0 = 1 + (int)(((long)nx * b) / nb);
+    i1 = 1 + (int)(((long)nx * (b + 1)) / nb);
+}
+
+__device__ __forceinline__ void lp_means(const ProjArgs& a, const double* slot, int nb,
+                                         double (&mean)[CFD_MAX_COMP]) {
+    double cs[CFD_MAX_COMP];
+    lv_total<CFD_MAX_COMP>(slot, nb, cs);
+#pragma unroll
+    for (int k = 0; k < CFD_MAX_COMP; ++k)
+        mean[k] = (a.ncomp > 0 && k < a.ncomp) ? cs[k] / a.comp_count[k] : 0.0;
+}
+
+}  // namespace
+
+// b = div(F, G) / dt (or the given right-hand side) on the fluid cells, and
+// the partial sums per Neumann component
+__global__ void __launch_bounds__(512) k_lp_rhs(ProjArgs a, Geo g, double* part) {
+    const DevState* st = a.st;
+    if (st->done) return;
+    const double dt = a.dt_override > 0.0 ? a.dt_override : st->dt;
+    double cs[CFD_MAX_COMP];
+#pragma unroll
+    for (int k = 0; k < CFD_MAX_COMP; ++k) cs[k] = 0.0;
+    struct Rhs {
+        int fluid, cm;
+        double rhs;
+    };
+    lv_cells_batch<4>(
+        g.nx, g.ny,
+        [&](int i, int j) {
+            const long c = ix(i, j, g.pc);
+            Rhs o;
+            o.fluid = a.code[c] & 63;
+            o.cm = a.comp[c];
+            if (a.rhs_in) {
+                o.rhs = a.rhs_in[c];
+            } else {
+                const double div = (a.F[ix(i, j, g.pu)] - a.F[ix(i - 1, j, g.pu)]) / g.dx +
+                                   (a.G[ix(i, j, g.pv)] - a.G[ix(i, j - 1, g.pv)]) / g.dy;
+                o.rhs = div / dt;
+            }
+            return o;
+        },
+        [&](int i, int j, const Rhs& o) {
+            if (!o.fluid) return;
+            a.b[ix(i, j, g.pc)] = o.rhs;
+#pragma unroll
+            for (int k = 0; k < CFD_MAX_COMP; ++k) cs[k] += (o.cm == k + 1) ? o.rhs : 0.0;
+        });
+    lv_partial<CFD_MAX_COMP>(cs, part + LV_P_CM * LV_SLOT);
+}
+
+// b = -(rhs - mean), r = b - A x, the first V-cycle input (lev[0].b = r,
+// lev[0].x = w r / d), partial b.b and r.r; nb = work-groups of k_lp_rhs
+__global__ void __launch_bounds__(512) k_lp_res(ProjArgs a, Geo g, LvArgs la, int nb) {
+    if (a.st->done) return;
+    double mean[CFD_MAX_COMP];
+    lp_means(a, la.part + LV_P_CM * LV_SLOT, nb, mean);
+    __shared__ double s_diag[64];
+    __shared__ float s_diagf[64];
+    if (threadIdx.x < 64) {
+        s_diag[threadIdx.x] = a.lut[threadIdx.x];
+        s_diagf[threadIdx.x] = (float)a.lut[threadIdx.x];
+    }
+    __syncthreads();
+    const double ax = 1.0 / (g.dx * g.dx), ay = 1.0 / (g.dy * g.dy);
+    const MGLevelDev L0 = la.mg->lev[0];
+    const float w = la.mg->omega;
+    double s2[2] = {0.0, 0.0};
+    struct Res {
+        int li;
+        double bv, rv;
+    };
+    lv_cells_batch<4>(
+        g.nx, g.ny,
+        [&](int i, int j) {
+            const long c = ix(i, j, g.pc);
+            Res o;
+            o.li = a.code[c] & 63;
+            double bv = a.b[c];
+            const int cm = a.comp[c];
+            if (cm) bv = bv - sel_comp(mean, cm);
+            o.bv = -bv;
+            o.rv = o.bv - apply_A(a.x, i, j, g.pc, g.nx, g.periodic, s_diag[o.li], ax, ay);
+            return o;
+        },
+        [&](int i, int j, const Res& o) {
+            if (!o.li) return;
+            const long c = ix(i, j, g.pc);
+            a.b[c] = o.bv;
+            a.r[c] = o.rv;
+            s2[0] += o.bv * o.bv;
+            s2[1] += o.rv * o.rv;
+            const float rf = (float)o.rv;
+            L0.b[c] = rf;
+            L0.x[c] = w * rf / s_diagf[o.li];
+        });
+    lv_partial<2>(s2, la.part + LV_P_RR * LV_SLOT);
+}
+
+// one work-group: ||b||, the flexible CG's initial state (a zero right-hand
+// side finishes the solve: x is zeroed by k_lp_xsum)
+__global__ void __launch_bounds__(256) k_lp_init(ProjArgs a, LvArgs la, int nb) {
+    if (a.st->done) return;
+    double s2[2];
+    lv_total<2>(la.part + LV_P_RR * LV_SLOT, nb, s2);
+    if (threadIdx.x != 0) return;
+    FcgState* fs = a.fcg;
+    const double bnorm = sqrt(s2[0]);
+    const double atol = a.rtol * bnorm;
+    const double rr = s2[1];
+    fs->bnorm = bnorm;
+    fs->atol = atol;
+    fs->rr = bnorm == 0.0 ? 0.0 : rr;
+    fs->rho_old = 1.0;
+    fs->k = 0;
+    fs->first = 1;
+    fs->pending = 0;
+    fs->iters = 0;
+    fs->info = 0;
+    fs->reps = 0;
+    fs->maxiter = a.maxiter;
+    fs->done = (bnorm == 0.0 || sqrt(rr) < atol) ? 1 : 0;
+    fs->rr0 = fs->rr;
+    fs->skip_net = fs->done | fs->net_off;
+}
+
+// partial sums of x per Neumann component (x := 0 first on the fluid cells
+// when the right-hand side was zero: SciPy returns b itself)
+__global__ void __launch_bounds__(512) k_lp_xsum(ProjArgs a, Geo g, double* part) {
+    if (a.st->done) return;
+    const bool zero = a.fcg->bnorm == 0.0;
+    double cs[CFD_MAX_COMP];
+#pragma unroll
+    for (int k = 0; k < CFD_MAX_COMP; ++k) cs[k] = 0.0;
+    lv_cells(g.nx, g.ny, [&](int i, int j) {
+        const long c = ix(i, j, g.pc);
+        double xv = a.x[c];
+        if (zero && (a.code[c] & 63)) {
+            a.x[c] = 0.0;
+            xv = 0.0;
+        }
+        const int cm = a.comp[c];
+#pragma unroll
+        for (int k = 0; k < CFD_MAX_COMP; ++k) cs[k] += (cm == k + 1) ? xv : 0.0;
+    });
+    lv_partial<CFD_MAX_COMP>(cs, part + LV_P_CM * LV_SLOT);
+}
+
+// p = x - mean on the fluid cells, 0 elsewhere and on the ghost columns; the
+// ghost rows 0 and nx + 1 (periodic copies of cells) by the first and last band
+__global__ void __launch_bounds__(512) k_lp_pwrite(ProjArgs a, Geo g, LvArgs la, int nb) {
+    if (a.st->done) return;
+    double mean[CFD_MAX_COMP];
+    lp_means(a, la.part + LV_P_CM * LV_SLOT, nb, mean);
+    const int nx = g.nx, ny = g.ny, pc = g.pc;
+    auto pval = [&](int i, int j) -> double {
+        const long c = ix(i, j, pc);
+        if (!(a.code[c] & 63)) return 0.0;
+        const int cm = a.comp[c];
+        const double xv = a.x[c];
+        return cm ? xv - sel_comp(mean, cm) : xv;
+    };
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    int i0, i1;
+    lp_band(nx, b, gridDim.x, i0, i1);
+    for (int i = i0; i < i1; ++i)
+        for (int j = threadIdx.x; j <= ny + 1; j += blockDim.x)
+            a.p[ix(i, j, pc)] = (j >= 1 && j <= ny) ? pval(i, j) : 0.0;
+    if (b == 0)
+        for (int j = threadIdx.x; j <= ny + 1; j += blockDim.x)
+            a.p[ix(0, j, pc)] = (g.periodic && j >= 1 && j <= ny) ? pval(nx, j) : 0.0;
+    if (b == (int)gridDim.x - 1)
+        for (int j = threadIdx.x; j <= ny + 1; j += blockDim.x)
+            a.p[ix(nx + 1, j, pc)] = (g.periodic && j >= 1 && j <= ny) ? pval(1, j) : 0.0;
+}
+
+// u = F - dt grad p, v = G - dt grad p (boundary faces as in k_project), the
+// partial max |u_new - u_old| over the faces between fluid cells
+__global__ void __launch_bounds__(512) k_lp_correct(ProjArgs a, Geo g, double* part) {
+    if (!a.correct || a.st->done) return;
+    const double dt = a.dt_override > 0.0 ? a.dt_override : a.st->dt;
+    const int nx = g.nx, ny = g.ny, pc = g.pc;
+    const int periodic = g.periodic;
+    const double* p = a.p;
+    auto fluid = [&](int i, int j) -> bool { return a.code[ix(i, j, pc)] & 63; };
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    int i0, i1;
+    lp_band(nx, b, gridDim.x, i0, i1);
+    double res[1] = {0.0};
+    for (int i = i0; i < i1; ++i) {
+        for (int j = 1 + threadIdx.x; j <= ny; j += blockDim.x) {
+            const long cu = ix(i, j, g.pu);
+            const double F = a.F[cu];
+            double un;
+            bool uf = false;
+            if (i < nx) {
+                uf = fluid(i, j) && fluid(i + 1, j);
+                un = uf ? F - dt * (p[ix(i + 1, j, pc)] - p[ix(i, j, pc)]) / g.dx : F;
+            } else if (periodic) {
+                uf = fluid(nx, j) && fluid(1, j);
+                un = uf ? F - dt * (p[ix(1, j, pc)] - p[ix(nx, j, pc)]) / g.dx : F;
+            } else {
+                un = (a.code[ix(nx, j, pc)] & 64) ? F + 2 * dt * p[ix(nx, j, pc)] / g.dx : F;
+            }
+            if (uf) res[0] = nanmax(res[0], fabs(un - a.u[cu]));
+            a.u[cu] = un;
+            if (periodic && i == nx) a.u[ix(0, j, g.pu)] = un;
+        }
+        for (int j = threadIdx.x; j <= ny; j += blockDim.x) {
+            const long cv = ix(i, j, g.pv);
+            const double G = a.G[cv];
+            double vn;
+            bool vf = false;
+            if (j >= 1 && j < ny) {
+                vf = fluid(i, j) && fluid(i, j + 1);
+                vn = vf ? G - dt * (p[ix(i, j + 1, pc)] - p[ix(i, j, pc)]) / g.dy : G;
+            } else if (periodic) {
+                continue;   // solver.py:300-306 returns before the boundary faces
+            } else if (j == 0) {
+                vn = (a.code[ix(i, 1, pc)] & 128) ? G - 2 * dt * p[ix(i, 1, pc)] / g.dy : G;
+            } else {
+                vn = (a.code[ix(i, ny, pc)] & 128) ? G + 2 * dt * p[ix(i, ny, pc)] / g.dy : G;
+            }
+            if (vf) res[0] = nanmax(res[0], fabs(vn - a.v[cv]));
+            a.v[cv] = vn;
+        }
+    }
+    if (b == 0 && !periodic) {
+        for (int j = 1 + threadIdx.x; j <= ny; j += blockDim.x) {
+            const long cu = ix(0, j, g.pu);
+            const double F = a.F[cu];
+            a.u[cu] = (a.code[ix(1, j, pc)] & 64) ? F - 2 * dt * p[ix(1, j, pc)] / g.dx : F;
+        }
+    }
+    __shared__ double lds[16];
+    block_max<1>(res, lds);
+    if (threadIdx.x == 0) part[LV_P_PQ * LV_SLOT + blockIdx.x] = res[0];
+}
+
+// one work-group: the solve's statistics into the step state, the residual
+// monitor, the periodic corner faces
+__global__ void __launch_bounds__(256) k_lp_fin(ProjArgs a, Geo g, double* part, int nb) {
+    DevState* st = a.st;
+    if (st->done) return;
+    double res = 0.0;
+    if (a.correct)
+        for (int k = threadIdx.x; k < nb; k += blockDim.x)
+            res = nanmax(res, part[LV_P_PQ * LV_SLOT + k]);
+    __shared__ double lds[16];
+    double m[1] = {res};
+    block_max<1>(m, lds);
+    if (threadIdx.x != 0) return;
+    const FcgState* fs = a.fcg;
+    st->cg_iters = fs->iters;
+    st->cg_info = fs->info;
+    st->cg_outer = fs->reps;
+    st->cg_bnorm = fs->bnorm;
+    st->cg_relres = fs->bnorm > 0.0 ? sqrt(fs->rr) / fs->bnorm : 0.0;
+    if (!a.correct) return;
+    if (a.dt_override <= 0.0) st->residual = m[0];
+    if (g.periodic) {
+        a.u[ix(0, 0, g.pu)] = a.u[ix(g.nx, 0, g.pu)];
+        a.u[ix(0, g.ny + 1, g.pu)] = a.u[ix(g.nx, g.ny + 1, g.pu)];
+    }
+}
+
+// p := 0 on the whole padded array (after the initial projection)
+__global__ void __launch_bounds__(256) k_lp_pzero(double* p, long n, const DevState* st) {
+    if (st->done) return;
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0.0;
+}

@@ -242,18 +242,6 @@ __global__ void __launch_bounds__(PROJ_BS) k_project(ProjArgs a, Geo g) {
 
     double cs[CFD_MAX_COMP];
     double mean[CFD_MAX_COMP];
-    if (a.phase == 2) {
-        // learned flexible CG: the iterations ran as separate launches
-        if (lead) {
-            const FcgState* fs = a.fcg;
-            st->cg_iters = fs->iters;
-            st->cg_info = fs->info;
-            st->cg_outer = fs->reps;
-            st->cg_bnorm = fs->bnorm;
-            st->cg_relres = fs->bnorm > 0.0 ? sqrt(fs->rr) / fs->bnorm : 0.0;
-        }
-        goto pressure_out;
-    }
     {
     // ---- right-hand side div(F,G)/dt and per-component sums ----
 #pragma unroll
@@ -310,38 +298,6 @@ __global__ void __launch_bounds__(PROJ_BS) k_project(ProjArgs a, Geo g) {
     double rr = s3[1], rz = s3[2];
     int iters = 0, info = 0;
 
-    if (a.phase == 1) {
-        // learned flexible CG: initial state and the first V-cycle input
-        if (lead) {
-            FcgState* fs = a.fcg;
-            fs->bnorm = bnorm;
-            fs->atol = atol;
-            fs->rr = bnorm == 0.0 ? 0.0 : rr;
-            fs->rho_old = 1.0;
-            fs->k = 0;
-            fs->first = 1;
-            fs->pending = 0;
-            fs->iters = 0;
-            fs->info = 0;
-            fs->reps = 0;
-            fs->maxiter = a.maxiter;
-            fs->done = (bnorm == 0.0 || sqrt(rr) < atol) ? 1 : 0;
-            fs->rr0 = fs->rr;
-            fs->skip_net = fs->done | fs->net_off;
-        }
-        const MGLevelDev& L0 = a.mg->lev[0];
-        const float w = a.mg->omega;
-        FOR_CELLS {
-            const long c = ix(i, j, pc);
-            const int li = a.code[c] & 63;
-            if (!li) continue;
-            if (bnorm == 0.0) a.x[c] = 0.0;
-            const float rf = bnorm == 0.0 ? 0.0f : (float)a.r[c];
-            L0.b[c] = rf;
-            L0.x[c] = w * rf / s_diagf[li];
-        }
-        return;
-    }
     if (bnorm == 0.0) {
         // SciPy returns b itself (all zeros) as the solution
         FOR_CELLS {
@@ -509,7 +465,6 @@ __global__ void __launch_bounds__(PROJ_BS) k_project(ProjArgs a, Geo g) {
     }
 
     // ---- pressure = x minus its mean per Neumann component ----
-pressure_out:
     if (a.ncomp > 0) {
 #pragma unroll
         for (int k = 0; k < CFD_MAX_COMP; ++k) cs[k] = 0.0;

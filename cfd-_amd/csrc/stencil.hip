@@ -343,10 +343,15 @@ __global__ void k_dt_gamma(const double* __restrict__ u, const double* __restric
         // row band of this block over rows 0..nx+1; threads along j
         const int r0 = (int)(((long)(nx + 2) * blockIdx.x) / gridDim.x);
         const int r1 = (int)(((long)(nx + 2) * (blockIdx.x + 1)) / gridDim.x);
+        // branch-free body (clamped indices, selects) so that the unrolled
+        // iterations' loads are in flight together
         for (int i = r0; i < r1; ++i) {
+#pragma unroll 4
             for (int j = threadIdx.x; j <= ny + 1; j += blockDim.x) {
-                if (i <= nx) m[0] = nanmax(m[0], fabs(u[ix(i, j, g.pu)]));
-                if (j <= ny) m[1] = nanmax(m[1], fabs(v[ix(i, j, g.pv)]));
+                const double au = fabs(u[ix(i <= nx ? i : nx, j, g.pu)]);
+                const double av = fabs(v[ix(i, j <= ny ? j : ny, g.pv)]);
+                m[0] = i <= nx ? nanmax(m[0], au) : m[0];
+                m[1] = j <= ny ? nanmax(m[1], av) : m[1];
             }
         }
     }
@@ -660,17 +665,26 @@ __global__ void k_record(RecordArgs a, Geo g) {
         const int nth = gridDim.x * blockDim.x;
         const int r0 = 1 + (int)(((long)nx * blockIdx.x) / gridDim.x);
         const int r1 = 1 + (int)(((long)nx * (blockIdx.x + 1)) / gridDim.x);
+        // the body loads unconditionally and accumulates through selects, so
+        // that the unrolled iterations' loads are in flight together
         for (int i = r0; i < r1; ++i)
+#pragma unroll 4
             for (int j = 1 + threadIdx.x; j <= ny; j += blockDim.x) {
-            if (!(a.fl[ix(i, j, pc)] & 1)) continue;
+            const bool f = a.fl[ix(i, j, pc)] & 1;
             const double div = (a.u[ix(i, j, g.pu)] - a.u[ix(i - 1, j, g.pu)]) / g.dx +
                                (a.v[ix(i, j, g.pv)] - a.v[ix(i, j - 1, g.pv)]) / g.dy;
-            val[0] = nanmax(val[0], fabs(div));
+            val[0] = f ? nanmax(val[0], fabs(div)) : val[0];
             const double uc = 0.5 * (a.u[ix(i - 1, j, g.pu)] + a.u[ix(i, j, g.pu)]);
             const double vc = 0.5 * (a.v[ix(i, j - 1, g.pv)] + a.v[ix(i, j, g.pv)]);
-            val[1] += uc * uc + vc * vc;
-            for (int s = 0; s < a.nscalar; ++s) val[10 + 5 * s] += a.phi[s][ix(i, j, pc)];
-            if (a.has_solid) {
+            val[1] = f ? val[1] + (uc * uc + vc * vc) : val[1];
+            // compile-time scalar indices keep val[] in registers
+#pragma unroll
+            for (int s = 0; s < CFD_MAX_SCALARS; ++s) {
+                if (s >= a.nscalar) break;
+                const double ph = a.phi[s][ix(i, j, pc)];
+                val[10 + 5 * s] = f ? val[10 + 5 * s] + ph : val[10 + 5 * s];
+            }
+            if (a.has_solid && f) {
                 // fluid cells next to interior solid cells (solver.py:542-567)
                 auto solid = [&](int si, int sj) -> bool {
                     return si >= 1 && si <= nx && sj >= 1 && sj <= ny &&
@@ -684,9 +698,11 @@ __global__ void k_record(RecordArgs a, Geo g) {
             }
         }
         // wall fluxes of scalars with fixed-value or fixed-flux sides (models.py:228-250)
-        for (int s = 0; s < a.nscalar; ++s) {
-            if (!a.wallflux[s]) continue;
+#pragma unroll
+        for (int s = 0; s < CFD_MAX_SCALARS; ++s) {
+            if (s >= a.nscalar || !a.wallflux[s]) continue;
             const double* phi = a.phi[s];
+#pragma unroll
             for (int side = 0; side < 4; ++side) {
                 const int len = side < 2 ? ny : nx;
                 const double h = side < 2 ? g.dx : g.dy;
