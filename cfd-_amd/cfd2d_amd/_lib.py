@@ -72,6 +72,11 @@ class LearnedDesc(ctypes.Structure):
                 ("x0_extrapolate", _i32)]
 
 
+class ProbeDesc(ctypes.Structure):
+    _fields_ = [("nsample", _i32), ("every", _i32), ("capacity", _i32), ("pad", _i32),
+                ("src", _vp), ("corner", _vp), ("frac", _vp)]
+
+
 class LearnedStats(ctypes.Structure):
     _fields_ = [("harvested", _i64), ("solves", _i64), ("iterations", _i64), ("fallbacks", _i64),
                 ("last_iters", _i32), ("last_info", _i32), ("last_relres", _d),
@@ -158,6 +163,10 @@ def lib() -> ctypes.CDLL:
         "cfd_obstacle_bc": ([_vp, _vp, _vp, _i32, _i32, _i32, _vp], _i32),
         "cfd_scalar_transport": ([_vp, _vp, _vp, _vp, _i32, _i32, _d, _d, _vp, _d, _i32, _i32]
                                  + [_vp] * 8 + [_i32, _d, _vp, _vp], _i32),
+        "cfd_probes_set": ([_vp, P(ProbeDesc)], _i32),
+        "cfd_probes_read": ([_vp, _i32, _vp, _vp, P(_i32), P(_i64)], _i32),
+        "cfd_favg_set": ([_vp, _i32, _d], _i32),
+        "cfd_favg_read": ([_vp, P(_d), _vp, _vp], _i32),
         "cfd_set_learned": ([_vp, P(LearnedDesc)], _i32),
         "cfd_learned_stats": ([_vp, P(LearnedStats)], _i32),
         "cfd_learned_reset": ([_vp], _i32),
@@ -198,7 +207,8 @@ def exported_symbols() -> list[str]:
                         "cfd_last_elapsed_ms", "cfd_set_profiling", "cfd_projection_times",
                         "cfd_pressure_solve", "cfd_dev_alloc",
                         "cfd_dev_free", "cfd_memcpy", "cfd_momentum_tendencies",
-                        "cfd_obstacle_bc", "cfd_scalar_transport", "cfd_set_learned",
+                        "cfd_obstacle_bc", "cfd_scalar_transport", "cfd_probes_set",
+                        "cfd_probes_read", "cfd_favg_set", "cfd_favg_read", "cfd_set_learned",
                         "cfd_learned_stats", "cfd_learned_reset", "cfd_ens_create",
                         "cfd_ens_destroy", "cfd_ens_set_net", "cfd_ens_step", "cfd_ens_download",
                         "cfd_ens_stats", "cfd_unet_param_count",

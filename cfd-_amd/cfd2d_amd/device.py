@@ -124,6 +124,10 @@ class DeviceContext:
     def step(self, prm: _lib.StepParams, count: int = 1) -> None:
         _lib.check(_lib.lib().cfd_step(self.handle, ctypes.byref(prm), int(count)), "step")
 
+    def call(self, fn: str, *args) -> int:
+        """``fn(handle, *args)`` of the C interface, status checked."""
+        return _lib.check(getattr(_lib.lib(), fn)(self.handle, *args), fn)
+
     def sync(self) -> None:
         _lib.check(_lib.lib().cfd_sync(self.handle), "sync")
 

@@ -199,6 +199,10 @@ class Model:
     def fields(self, sim) -> dict:
         return {}
 
+    def field_keys(self) -> list[str]:
+        """The names fields() returns, without reading the data."""
+        return []
+
     def state(self) -> dict:
         return {}
 
@@ -242,6 +246,9 @@ class PassiveScalar(Model):
 
     def fields(self, sim) -> dict:
         return {self.name: (self.field.label, self.field.cell())}
+
+    def field_keys(self) -> list[str]:
+        return [self.name]
 
 
 class BoussinesqTemperature(Model):
@@ -307,3 +314,6 @@ class BoussinesqTemperature(Model):
 
     def fields(self, sim) -> dict:
         return {"T": ("Temperatur T", self.field.cell())}
+
+    def field_keys(self) -> list[str]:
+        return ["T"]

@@ -115,6 +115,9 @@ class Simulation:
         self.flow_force = 0.0
         self.scalars: dict = {}
         self.function_objects: list = []
+        self._fo_key = None
+        self._fo_host: list = []
+        self._probe_set = None
         self.models = []
         for m in models or []:
             self.add_model(m)
@@ -301,6 +304,8 @@ class Simulation:
             if r.nmon < 0:   # the unstable step: counted, not recorded
                 continue
             self._append(r)
+        if self._probe_set is not None:
+            self._probe_set.collect(self)
         self.t, self.n, self.dt = float(st.t), int(st.n), float(st.dt)
         self.dt_old = float(st.dt_old) if st.n > 0 else None
         self.gamma = float(st.gamma)
@@ -349,15 +354,34 @@ class Simulation:
         fy += self.nu * (m[8] + m[9]) / (0.5 * g.dx) * g.dy
         return float(fx), float(fy)
 
+    def _host_function_objects(self) -> list:
+        """Set the function objects up (once per list): probes and the field
+        average run on the device in the step's launch sequence; the returned
+        rest runs on the host after every step."""
+        key = tuple(id(f) for f in self.function_objects)
+        if key != self._fo_key:
+            from .functions import attach_function_objects
+
+            self.collect_function_objects()
+            self._fo_host = attach_function_objects(self, self.function_objects)
+            self._fo_key = key
+        return self._fo_host
+
+    def collect_function_objects(self) -> None:
+        """Bring the device records of the function objects to the host."""
+        self._pending = True
+        self._flush()
+
     def step(self) -> None:
         """One projection step (solver.py:326-403), enqueued on the device."""
         t0 = time.perf_counter()
+        host_fo = self._host_function_objects()
         self._ctx.step(self._params(), 1)
         self._pending = True
         self._since_sync += 1
-        if self._since_sync >= self.sync_every:
+        if host_fo or self._since_sync >= self.sync_every:
             self._flush()
-            for fo in self.function_objects:
+            for fo in host_fo:
                 fo.execute(self)
         self.wall_time += time.perf_counter() - t0
 
@@ -379,7 +403,8 @@ class Simulation:
                     chunk = min(chunk, callback_every - self.n % callback_every)
                 if verbose:
                     chunk = min(chunk, print_every - self.n % print_every)
-                if self.function_objects:
+                host_fo = self._host_function_objects()
+                if host_fo:
                     chunk = 1
                 if max_steps is not None:
                     chunk = min(chunk, max_steps - self.n)
@@ -394,7 +419,7 @@ class Simulation:
                     if st.done:
                         st.done = 0
                         self._ctx.set_state(st)
-                for fo in self.function_objects:
+                for fo in host_fo:
                     fo.execute(self)
                 if verbose and self.n % print_every == 0:
                     print(self.status_line())

@@ -17,6 +17,7 @@
 #include "unet.hip"
 #include "lpcg.hip"
 #include "ens.hip"
+#include "monitors.hip"
 
 namespace {
 
@@ -81,6 +82,21 @@ struct cfd_ctx {
     DevState* st = nullptr;
     StepRecord* ring = nullptr;
     std::vector<ScalarDev> scalars;
+    // function objects on the device (monitors.hip)
+    struct {
+        int nsample = 0, every = 1, cap = 0;
+        int* src = nullptr;
+        long long* base = nullptr;
+        double *frac = nullptr, *times = nullptr, *vals = nullptr;
+        int* count = nullptr;
+        std::vector<void*> mem;
+    } probes;
+    struct {
+        int on = 0;
+        double t_start = 0.0;
+        double *sum = nullptr, *sum2 = nullptr, *T = nullptr;
+    } favg;
+    long long* mon_last_n = nullptr;
     double* rold = nullptr;
     MGDev* mg_dev = nullptr;     // device copy of the hierarchy description
     MGDev mg_host;               // host copy (level sizes and array pointers)
@@ -763,6 +779,7 @@ int cfd_ctx_destroy(cfd_ctx* c) {
     if (c->l_graph_g) hipGraphDestroy(c->l_graph_g);
     if (c->fcg_host) hipHostFree(c->fcg_host);
     for (void* p : c->allocs) hipFree(p);
+    for (void* p : c->probes.mem) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->own_stream) hipStreamDestroy(c->own_stream);
@@ -1110,6 +1127,66 @@ int cfd_project_initial(cfd_ctx* c, const cfd_step_params* prm) {
     return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+// the function objects of the step just issued (after k_record: st->n, t, dt
+// are the new step's); ra.phi holds the scalars' current buffers
+int launch_monitors(cfd_ctx* c, const RecordArgs& ra) {
+    if (!c->probes.nsample && !c->favg.on) return 0;
+    if (c->probes.nsample) {
+        ProbeArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.u = c->u;
+        a.v = c->v;
+        a.p = c->p;
+        a.fl = c->fl;
+        for (int s = 0; s < ra.nscalar && s < MON_MAX_SCALARS; ++s) a.phi[s] = ra.phi[s];
+        a.st = c->st;
+        a.last_n = c->mon_last_n;
+        a.nsample = c->probes.nsample;
+        a.every = c->probes.every;
+        a.cap = c->probes.cap;
+        a.P = c->P;
+        a.dx = c->geo.dx;
+        a.dy = c->geo.dy;
+        a.src = c->probes.src;
+        a.base = c->probes.base;
+        a.frac = c->probes.frac;
+        a.times = c->probes.times;
+        a.vals = c->probes.vals;
+        a.count = c->probes.count;
+        hipLaunchKernelGGL(k_probes, dim3(1), dim3(256), 0, c->stream, a);
+        HIPC(hipGetLastError());
+    }
+    if (c->favg.on) {
+        FavgArgs f;
+        f.u = c->u;
+        f.v = c->v;
+        f.p = c->p;
+        f.fl = c->fl;
+        f.st = c->st;
+        f.last_n = c->mon_last_n;
+        f.t_start = c->favg.t_start;
+        f.sum = c->favg.sum;
+        f.sum2 = c->favg.sum2;
+        f.T = c->favg.T;
+        const long n = (long)c->nx * c->ny;
+        hipLaunchKernelGGL(k_favg, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, f,
+                           c->geo);
+        HIPC(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_mon_fin, dim3(1), dim3(1), 0, c->stream, (const DevState*)c->st,
+                       c->mon_last_n);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
 int cfd_step(cfd_ctx* c, const cfd_step_params* prm, int32_t count) {
     if (!c || !prm) return set_err(CFD_ERR_ARG, "null argument");
     if (prm->nu <= 0.0) return set_err(CFD_ERR_ARG, "viscosity must be positive");
@@ -1182,6 +1259,7 @@ int cfd_step(cfd_ctx* c, const cfd_step_params* prm, int32_t count) {
         ra.max_steps = prm->max_steps;
         hipLaunchKernelGGL(k_record, dim3(c->red_blocks), dim3(256), 0, c->stream, ra, c->geo);
         HIPC(hipGetLastError());
+        TRY(launch_monitors(c, ra));
     }
     HIPC(hipEventRecord(c->ev1, c->stream));
     return 0;
@@ -1361,6 +1439,124 @@ int cfd_scalar_transport(const double* phi, const double* u, const double* v,
         hipLaunchKernelGGL((k_scalar_transport<true, 0>), grid, dim3(BS2D), 0,
                            (hipStream_t)stream, a, g);
     HIPC(hipGetLastError());
+    return 0;
+}
+
+// ---- function objects on the device --------------------------------------
+int cfd_probes_set(cfd_ctx* c, const cfd_probe_desc* d) {
+    if (!c || !d) return set_err(CFD_ERR_ARG, "null argument");
+    HIPC(hipStreamSynchronize(c->stream));
+    auto release = [&]() {
+        for (void* q : c->probes.mem) hipFree(q);
+        c->probes.mem.clear();
+        c->probes.nsample = 0;
+    };
+    if (d->nsample == 0) {
+        release();
+        return 0;
+    }
+    if (d->nsample < 0 || d->every < 1 || d->capacity < 1 || !d->src || !d->corner || !d->frac)
+        return set_err(CFD_ERR_ARG, "probe descriptor");
+    const int n = d->nsample;
+    const int nsc = std::min((int)c->scalars.size(), MON_MAX_SCALARS);
+    std::vector<long long> base(n);
+    for (int s = 0; s < n; ++s) {
+        const int sr = d->src[s];
+        if (sr < CFD_PROBE_U || sr >= CFD_PROBE_SCALAR + nsc)
+            return set_err(CFD_ERR_ARG, "probe %d: field %d", s, sr);
+        // host-layout extent of the field the corner indexes; the
+        // interpolation cell's upper corner (i+1, j+1) must lie inside it
+        const int rows = sr == CFD_PROBE_U ? c->nx + 1 : (sr == CFD_PROBE_V ? c->nx + 2 : c->nx);
+        const int cols = sr == CFD_PROBE_U ? c->ny + 2 : (sr == CFD_PROBE_V ? c->ny + 1 : c->ny);
+        const int off = sr >= CFD_PROBE_PRESSURE ? 1 : 0;   // cells sit inside the ghost ring
+        const int i = d->corner[2 * s], j = d->corner[2 * s + 1];
+        if (i < 0 || i + 1 >= rows || j < 0 || j + 1 >= cols)
+            return set_err(CFD_ERR_ARG, "probe %d: corner (%d, %d) needs %d x %d with room for "
+                           "(i+1, j+1)", s, i, j, rows, cols);
+        base[s] = (long long)(i + off) * c->P + (j + off);
+    }
+    release();   // a rejected descriptor leaves the previous probes in place
+    auto alloc = [&](void** q, size_t bytes) -> int {
+        HIPC(hipMalloc(q, std::max<size_t>(bytes, 8)));
+        HIPC(hipMemset(*q, 0, std::max<size_t>(bytes, 8)));
+        c->probes.mem.push_back(*q);
+        return 0;
+    };
+    TRY(alloc((void**)&c->probes.src, sizeof(int) * n));
+    TRY(alloc((void**)&c->probes.base, sizeof(long long) * n));
+    TRY(alloc((void**)&c->probes.frac, sizeof(double) * 2L * n));
+    TRY(alloc((void**)&c->probes.times, sizeof(double) * d->capacity));
+    TRY(alloc((void**)&c->probes.vals, sizeof(double) * (size_t)d->capacity * n));
+    TRY(alloc((void**)&c->probes.count, sizeof(int) * 2));
+    if (!c->mon_last_n) TRY(dalloc(c, &c->mon_last_n, 1));
+    HIPC(hipMemcpy(c->probes.src, d->src, sizeof(int) * n, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->probes.base, base.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->probes.frac, d->frac, sizeof(double) * 2L * n, hipMemcpyHostToDevice));
+    DevState h;
+    TRY(read_state(c, &h));
+    HIPC(hipMemcpy(c->mon_last_n, &h.n, sizeof(long long), hipMemcpyHostToDevice));
+    c->probes.every = d->every;
+    c->probes.cap = d->capacity;
+    c->probes.nsample = n;
+    return 0;
+}
+
+int cfd_probes_read(cfd_ctx* c, int32_t max_records, double* times, double* values,
+                    int32_t* nrecords, int64_t* lost) {
+    if (!c || !nrecords) return set_err(CFD_ERR_ARG, "null argument");
+    *nrecords = 0;
+    if (lost) *lost = 0;
+    if (!c->probes.nsample) return 0;
+    int cnt[2];
+    HIPC(hipMemcpyAsync(cnt, c->probes.count, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    const int n = std::min(cnt[0], std::max(0, max_records));
+    if (n < cnt[0]) return set_err(CFD_ERR_LIMIT, "%d probe records held, room for %d", cnt[0],
+                                   max_records);
+    if (n > 0) {
+        if (!times || !values) return set_err(CFD_ERR_ARG, "null output");
+        HIPC(hipMemcpy(times, c->probes.times, sizeof(double) * n, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(values, c->probes.vals, sizeof(double) * (size_t)n * c->probes.nsample,
+                       hipMemcpyDeviceToHost));
+    }
+    HIPC(hipMemsetAsync(c->probes.count, 0, sizeof(int) * 2, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    *nrecords = n;
+    if (lost) *lost = cnt[1];
+    return 0;
+}
+
+int cfd_favg_set(cfd_ctx* c, int32_t enable, double t_start) {
+    if (!c) return set_err(CFD_ERR_ARG, "null context");
+    HIPC(hipStreamSynchronize(c->stream));
+    c->favg.on = 0;
+    if (!enable) return 0;
+    const size_t n = 3 * (size_t)c->nx * c->ny;
+    if (!c->favg.sum) {
+        TRY(dalloc(c, &c->favg.sum, n));
+        TRY(dalloc(c, &c->favg.sum2, n));
+        TRY(dalloc(c, &c->favg.T, 1));
+    }
+    HIPC(hipMemset(c->favg.sum, 0, n * sizeof(double)));
+    HIPC(hipMemset(c->favg.sum2, 0, n * sizeof(double)));
+    HIPC(hipMemset(c->favg.T, 0, sizeof(double)));
+    if (!c->mon_last_n) TRY(dalloc(c, &c->mon_last_n, 1));
+    DevState h;
+    TRY(read_state(c, &h));
+    HIPC(hipMemcpy(c->mon_last_n, &h.n, sizeof(long long), hipMemcpyHostToDevice));
+    c->favg.t_start = t_start;
+    c->favg.on = 1;
+    return 0;
+}
+
+int cfd_favg_read(cfd_ctx* c, double* T, double* sum, double* sum2) {
+    if (!c || !T) return set_err(CFD_ERR_ARG, "null argument");
+    if (!c->favg.sum) return set_err(CFD_ERR_STATE, "field averaging was never enabled");
+    HIPC(hipStreamSynchronize(c->stream));
+    const size_t n = 3 * (size_t)c->nx * c->ny;
+    HIPC(hipMemcpy(T, c->favg.T, sizeof(double), hipMemcpyDeviceToHost));
+    if (sum) HIPC(hipMemcpy(sum, c->favg.sum, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (sum2) HIPC(hipMemcpy(sum2, c->favg.sum2, n * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }
 

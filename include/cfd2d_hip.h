@@ -275,6 +275,42 @@ int cfd_learned_stats(cfd_ctx* ctx, cfd_learned_stats_t* out);
 /* zero the harvest counter and the solve statistics */
 int cfd_learned_reset(cfd_ctx* ctx);
 
+/* ---- function objects on the device (cfd2d/functions.py:59-240 evaluates
+   Probes and FieldAverage on the host after every step,
+   solver.py:402-403; here they run in the step's launch sequence, so a
+   multi-step cfd_step needs no host synchronisation) ---- */
+enum {
+    CFD_PROBE_U = 0,          /* face u, grid (i dx, (j - 1/2) dy), host layout (nx+1, ny+2) */
+    CFD_PROBE_V = 1,          /* face v, grid ((i - 1/2) dx, j dy), host layout (nx+2, ny+1) */
+    CFD_PROBE_PRESSURE = 2,   /* cell fields from here on: cell centres, host layout (nx, ny), */
+    CFD_PROBE_SPEED = 3,      /*   NaN in solid cells (solver.py:517-540, 563-567) */
+    CFD_PROBE_VORTICITY = 4,
+    CFD_PROBE_SCALAR = 5      /* + scalar id */
+};
+typedef struct cfd_probe_desc {
+    int32_t nsample;          /* (field, point) pairs; 0 removes the probes */
+    int32_t every;            /* record after the steps n with n % every == 0 */
+    int32_t capacity;         /* records held until cfd_probes_read drains them */
+    int32_t pad;
+    const int32_t* src;       /* per sample: CFD_PROBE_* */
+    const int32_t* corner;    /* per sample (i, j): lower corner of the interpolation cell in
+                                 the field's host layout, 0 <= i <= rows-2, 0 <= j <= cols-2 */
+    const double* frac;       /* per sample (a, b): normalised distances along x and y; value =
+                                 f00 (1-a)(1-b) + f01 (1-a) b + f10 a (1-b) + f11 a b, evaluated
+                                 left to right (scipy's linear RegularGridInterpolator) */
+} cfd_probe_desc;
+int cfd_probes_set(cfd_ctx* ctx, const cfd_probe_desc* desc);
+/* the held records (times [n], values [n][nsample]) and the count dropped for
+   lack of room; the device buffer is emptied */
+int cfd_probes_read(cfd_ctx* ctx, int32_t max_records, double* times, double* values,
+                    int32_t* nrecords, int64_t* lost);
+/* FieldAverage (functions.py:180-210): from the first step with t >= t_start,
+   sum += a dt and sum2 += a^2 dt of the cell-centred u, v and pressure (NaN in
+   solid cells); enable = 0 stops it; enabling clears the sums */
+int cfd_favg_set(cfd_ctx* ctx, int32_t enable, double t_start);
+/* T and the sums [3][nx][ny] (u, v, p; either pointer may be NULL) */
+int cfd_favg_read(cfd_ctx* ctx, double* T, double* sum, double* sum2);
+
 /* ---- learned preconditioner network (no counterpart in the reference: the
    U-Net of the learned flexible PCG, trained in-process on residual ->
    correction pairs harvested from the pressure solves) ----
