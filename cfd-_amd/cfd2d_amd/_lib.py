@@ -167,6 +167,7 @@ def lib() -> ctypes.CDLL:
         "cfd_probes_read": ([_vp, _i32, _vp, _vp, P(_i32), P(_i64)], _i32),
         "cfd_favg_set": ([_vp, _i32, _d], _i32),
         "cfd_favg_read": ([_vp, P(_d), _vp, _vp], _i32),
+        "cfd_download_derived": ([_vp, _i32, _vp], _i32),
         "cfd_set_learned": ([_vp, P(LearnedDesc)], _i32),
         "cfd_learned_stats": ([_vp, P(LearnedStats)], _i32),
         "cfd_learned_reset": ([_vp], _i32),
@@ -208,7 +209,8 @@ def exported_symbols() -> list[str]:
                         "cfd_pressure_solve", "cfd_dev_alloc",
                         "cfd_dev_free", "cfd_memcpy", "cfd_momentum_tendencies",
                         "cfd_obstacle_bc", "cfd_scalar_transport", "cfd_probes_set",
-                        "cfd_probes_read", "cfd_favg_set", "cfd_favg_read", "cfd_set_learned",
+                        "cfd_probes_read", "cfd_favg_set", "cfd_favg_read", "cfd_download_derived",
+                        "cfd_set_learned",
                         "cfd_learned_stats", "cfd_learned_reset", "cfd_ens_create",
                         "cfd_ens_destroy", "cfd_ens_set_net", "cfd_ens_step", "cfd_ens_download",
                         "cfd_ens_stats", "cfd_unet_param_count",

@@ -495,10 +495,17 @@ class Simulation:
         fy += self.nu * (vc[east].sum() + vc[west].sum()) / (0.5 * g.dx) * g.dy
         return float(fx), float(fy)
 
-    def vorticity(self) -> np.ndarray:
+    def _derived(self, kind: int) -> np.ndarray:
+        self._flush()
         g = self.grid
-        u, v = self.u, self.v
-        return (v[1:, :] - v[:-1, :]) / g.dx - (u[:, 1:] - u[:, :-1]) / g.dy
+        out = np.empty((g.nx + 1, g.ny + 1))
+        self._ctx.call("cfd_download_derived", kind, _lib.ptr(out))
+        return out
+
+    def vorticity(self) -> np.ndarray:
+        """dv/dx - du/dy at the grid nodes, (nx+1, ny+1) (solver.py:569-573),
+        computed on the device."""
+        return self._derived(0)
 
     def cell_vorticity(self) -> np.ndarray:
         w = self.vorticity()
@@ -506,17 +513,9 @@ class Simulation:
         return np.where(self.grid.solid, np.nan, wc)
 
     def streamfunction(self) -> np.ndarray:
-        g = self.grid
-        fl = g.fluid
-        u_open = fl[:-1, 1:-1] | fl[1:, 1:-1]
-        v_open = fl[1:-1, 0] | fl[1:-1, 1]
-        uu, vv = self.u, self.v
-        u = np.where(u_open, uu[:, 1:-1], 0.0)
-        v0 = np.where(v_open, vv[1:-1, 0], 0.0)
-        psi = np.zeros((g.nx + 1, g.ny + 1))
-        psi[1:, 0] = -np.cumsum(v0) * g.dx
-        psi[:, 1:] = psi[:, :1] + np.cumsum(u, axis=1) * g.dy
-        return psi
+        """psi at the grid nodes with u = dpsi/dy, v = -dpsi/dx, psi(0, 0) = 0
+        (solver.py:580-596), computed on the device."""
+        return self._derived(1)
 
     def sample_u(self, x, y) -> np.ndarray:
         from scipy.interpolate import RegularGridInterpolator

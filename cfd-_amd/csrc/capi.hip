@@ -97,6 +97,7 @@ struct cfd_ctx {
         double *sum = nullptr, *sum2 = nullptr, *T = nullptr;
     } favg;
     long long* mon_last_n = nullptr;
+    double* derived = nullptr;   // (nx+1) x (ny+1) node field for reports
     double* rold = nullptr;
     MGDev* mg_dev = nullptr;     // device copy of the hierarchy description
     MGDev mg_host;               // host copy (level sizes and array pointers)
@@ -1557,6 +1558,29 @@ int cfd_favg_read(cfd_ctx* c, double* T, double* sum, double* sum2) {
     HIPC(hipMemcpy(T, c->favg.T, sizeof(double), hipMemcpyDeviceToHost));
     if (sum) HIPC(hipMemcpy(sum, c->favg.sum, n * sizeof(double), hipMemcpyDeviceToHost));
     if (sum2) HIPC(hipMemcpy(sum2, c->favg.sum2, n * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int cfd_download_derived(cfd_ctx* c, int32_t kind, double* host) {
+    if (!c || !host) return set_err(CFD_ERR_ARG, "null argument");
+    if (kind != CFD_DERIVED_VORTICITY && kind != CFD_DERIVED_STREAMFUNCTION)
+        return set_err(CFD_ERR_ARG, "derived field %d", kind);
+    const size_t n = (size_t)(c->nx + 1) * (c->ny + 1);
+    if (!c->derived) TRY(dalloc(c, &c->derived, n));
+    if (kind == CFD_DERIVED_VORTICITY) {
+        hipLaunchKernelGGL(k_vort_nodes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                           (const double*)c->u, (const double*)c->v, c->geo, c->P, c->derived);
+    } else {
+        hipLaunchKernelGGL(k_psi_bottom, dim3(1), dim3(1024), 0, c->stream, (const double*)c->v,
+                           (const uint8_t*)c->fl, c->geo, c->P, c->derived);
+        HIPC(hipGetLastError());
+        hipLaunchKernelGGL(k_psi_cols, dim3((unsigned)((c->nx + 1 + 255) / 256)), dim3(256), 0,
+                           c->stream, (const double*)c->u, (const uint8_t*)c->fl, c->geo, c->P,
+                           c->derived);
+    }
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(host, c->derived, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
     return 0;
 }
 

@@ -34,13 +34,13 @@ struct ProbeArgs {
 __device__ __forceinline__ double qnan() { return __longlong_as_double(0x7ff8000000000000ll); }
 
 // vorticity dv/dx - du/dy at the grid node whose u face below it and v face
-// left of it sit at device offset o (solver.py:563-567)
+// left of it sit at device offset o (solver.py:569-573)
 __device__ __forceinline__ double node_vort(const ProbeArgs& a, long long o) {
     return (a.v[o + a.P] - a.v[o]) / a.dx - (a.u[o + 1] - a.u[o]) / a.dy;
 }
 
 // a cell-centred field at the cell with device offset c, NaN in solid cells
-// (solver.py:517-540, 569-572)
+// (solver.py:526-537, 575-578)
 __device__ double cell_value(const ProbeArgs& a, int sr, long long c) {
     if (!(a.fl[c] & 1)) return qnan();
     if (sr == PR_P) return a.p[c];
@@ -139,3 +139,67 @@ __global__ void __launch_bounds__(256) k_favg(FavgArgs a, Geo g) {
 
 // the step count the monitors compare against, after all monitor kernels
 __global__ void k_mon_fin(const DevState* st, long long* last_n) { *last_n = st->n; }
+
+// ---- fields for reports, computed where the velocity lives ----------------
+// Node vorticity dv/dx - du/dy (solver.py:569-573) into a dense
+// (nx+1) x (ny+1) array; node (i, j) sits between u faces (i, j), (i, j+1)
+// and v faces (i, j), (i+1, j) of the device layout.
+__global__ void __launch_bounds__(256) k_vort_nodes(const double* __restrict__ u,
+                                                    const double* __restrict__ v, Geo g, int P,
+                                                    double* __restrict__ out) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int cols = g.ny + 1;
+    if (k >= (long)(g.nx + 1) * cols) return;
+    const long o = (k / cols) * P + k % cols;
+    out[k] = (v[o + P] - v[o]) / g.dx - (u[o + 1] - u[o]) / g.dy;
+}
+
+// Streamfunction with u = dpsi/dy, v = -dpsi/dx and psi(0, 0) = 0
+// (solver.py:580-596): faces with no fluid cell beside them carry no flux.
+// Bottom row first: psi(i+1, 0) = -(sum_{k<=i} v0_k) dx, the sum running
+// left to right as numpy's cumsum does; the work-group stages the open
+// bottom faces 1024 at a time in LDS and one lane carries the sum.
+__global__ void __launch_bounds__(1024) k_psi_bottom(const double* __restrict__ v,
+                                                     const uint8_t* __restrict__ fl, Geo g, int P,
+                                                     double* __restrict__ psi) {
+    __shared__ double s_v[1024];
+    const int cols = g.ny + 1;
+    double c = 0.0;
+    if (threadIdx.x == 0) psi[0] = 0.0;
+    for (int base = 0; base < g.nx; base += 1024) {
+        const int i = base + (int)threadIdx.x;
+        if (i < g.nx) {
+            const long o = (long)(i + 1) * P;
+            s_v[threadIdx.x] = (fl[o] | fl[o + 1]) ? v[o] : 0.0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int m = min(1024, g.nx - base);
+            for (int k = 0; k < m; ++k) {
+                c = (base + k == 0) ? s_v[0] : c + s_v[k];
+                psi[(long)(base + k + 1) * cols] = -c * g.dx;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// then every column upwards from its bottom value: psi(i, j+1) =
+// psi(i, 0) + (sum_{k<=j} u_ik) dy, one lane per column
+__global__ void __launch_bounds__(256) k_psi_cols(const double* __restrict__ u,
+                                                  const uint8_t* __restrict__ fl, Geo g, int P,
+                                                  double* __restrict__ psi) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > g.nx) return;
+    const int cols = g.ny + 1;
+    const long o = (long)i * P;
+    double* col = psi + (long)i * cols;
+    const double p0 = col[0];
+    double s = 0.0;
+    for (int j = 0; j < g.ny; ++j) {
+        const bool open = fl[o + j + 1] | fl[o + P + j + 1];
+        const double x = open ? u[o + j + 1] : 0.0;
+        s = j == 0 ? x : s + x;
+        col[j + 1] = p0 + s * g.dy;
+    }
+}

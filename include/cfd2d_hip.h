@@ -283,7 +283,7 @@ enum {
     CFD_PROBE_U = 0,          /* face u, grid (i dx, (j - 1/2) dy), host layout (nx+1, ny+2) */
     CFD_PROBE_V = 1,          /* face v, grid ((i - 1/2) dx, j dy), host layout (nx+2, ny+1) */
     CFD_PROBE_PRESSURE = 2,   /* cell fields from here on: cell centres, host layout (nx, ny), */
-    CFD_PROBE_SPEED = 3,      /*   NaN in solid cells (solver.py:517-540, 563-567) */
+    CFD_PROBE_SPEED = 3,      /*   NaN in solid cells (solver.py:526-537, 569-578) */
     CFD_PROBE_VORTICITY = 4,
     CFD_PROBE_SCALAR = 5      /* + scalar id */
 };
@@ -310,6 +310,13 @@ int cfd_probes_read(cfd_ctx* ctx, int32_t max_records, double* times, double* va
 int cfd_favg_set(cfd_ctx* ctx, int32_t enable, double t_start);
 /* T and the sums [3][nx][ny] (u, v, p; either pointer may be NULL) */
 int cfd_favg_read(cfd_ctx* ctx, double* T, double* sum, double* sum2);
+/* Node fields for reports, computed on the device from the current velocity
+   and copied to host (nx+1) x (ny+1): the vorticity dv/dx - du/dy
+   (solver.py:569-573) or the streamfunction with psi(0, 0) = 0 and no flux
+   through faces without a fluid cell beside them, the sums running as numpy's
+   cumsum does (solver.py:580-596) */
+enum { CFD_DERIVED_VORTICITY = 0, CFD_DERIVED_STREAMFUNCTION = 1 };
+int cfd_download_derived(cfd_ctx* ctx, int32_t kind, double* host);
 
 /* ---- learned preconditioner network (no counterpart in the reference: the
    U-Net of the learned flexible PCG, trained in-process on residual ->

@@ -13,6 +13,7 @@ from scipy.interpolate import RegularGridInterpolator
 from cfd2d_amd import functions as F
 from cfd2d_amd.grid import Grid
 from cfd2d_amd.solver import Simulation
+from hostfields import node_vorticity, streamfunction
 
 
 class HostFields:
@@ -20,9 +21,7 @@ class HostFields:
 
     cell_velocity = Simulation.cell_velocity
     cell_pressure = Simulation.cell_pressure
-    vorticity = Simulation.vorticity
     cell_vorticity = Simulation.cell_vorticity
-    streamfunction = Simulation.streamfunction
     sample_u = Simulation.sample_u
     sample_v = Simulation.sample_v
 
@@ -35,6 +34,12 @@ class HostFields:
         self.p = rng.standard_normal((g.nx + 2, g.ny + 2))
         self.t, self.n, self.dt = 0.0, 0, 0.01
         self.models = []
+
+    def vorticity(self):
+        return node_vorticity(self.grid, self.u, self.v)
+
+    def streamfunction(self):
+        return streamfunction(self.grid, self.u, self.v)
 
 
 def _grid(nx=12, ny=9, obstacle=True):

@@ -170,3 +170,41 @@ def test_case_function_objects_run(gpu, tmp_path):
         assert (out / f).exists(), f
     pr = case.sim.function_objects[0]
     assert pr.on_device and len(pr.t) == case.sim.n
+
+
+@pytest.mark.parametrize("shape", ["channel", "wide"])
+def test_vorticity_and_streamfunction_on_device(gpu, shape):
+    """Node vorticity and streamfunction computed by the device kernels equal
+    the reference's numpy evaluation on the downloaded velocity; "wide" runs
+    the bottom-row sum over more than one 1024-lane stage."""
+    from hostfields import node_vorticity, streamfunction
+
+    from cfd2d_amd.boundary import BoundaryCondition as BC
+    from cfd2d_amd.grid import Grid
+    from cfd2d_amd.solver import Simulation
+
+    if shape == "channel":
+        sim = _channel("cg").sim
+        steps = 10
+    else:
+        grid = Grid(2100, 6, 35.0, 0.1)
+        m = np.zeros((2100, 6), bool)
+        m[1500:1510, 2:4] = True
+        grid.add_obstacle(m)
+        grid.finalize()
+        bcs = dict(left=BC("inflow", 1.0), right=BC("outflow"), bottom=BC("noslip"),
+                   top=BC("noslip"))
+        sim = Simulation(grid, bcs, nu=1e-2, pressure_solver="cg")
+        steps = 3
+    for _ in range(steps):
+        sim.step()
+    u, v = sim.u, sim.v
+    w = sim.vorticity()
+    print(shape, "vorticity", float(np.abs(w).max()), flush=True)
+    assert np.array_equal(w, node_vorticity(sim.grid, u, v))
+    psi = sim.streamfunction()
+    print(shape, "streamfunction", float(np.abs(psi).max()), flush=True)
+    assert np.array_equal(psi, streamfunction(sim.grid, u, v))
+    assert np.abs(psi[:, -1]).max() > 0
+    wc = sim.cell_vorticity()
+    assert np.isnan(wc[sim.grid.solid]).all() and np.isfinite(wc[~sim.grid.solid]).all()
