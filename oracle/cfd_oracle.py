@@ -19,9 +19,10 @@ Which reference lines each piece follows:
 ``inflow_profile``         cfd2d/boundary.py:63-89
 ``OBoundary``              cfd2d/boundary.py:92-242
 ``OPressure``              cfd2d/pressure.py:32-185
-``OScalar``                cfd2d/models.py:73-250 (no source term, no clipping)
+``OScalar``                cfd2d/models.py:73-250 (sources, clipping)
 ``OBoussinesq``            cfd2d/models.py:339-426 (momentum_source 386-396)
-``OSim``                   cfd2d/solver.py:95-540 (laminar, constant density)
+``OSim``                   cfd2d/solver.py:95-540 (constant density; the
+                           variable viscosity of turb_oracle.py's models)
 ``cavity`` etc.            cfd2d/cases.py:51-70, 161-223, 340-379
 ``save/load_checkpoint``   cfd2d/functions.py:326-379
 =========================  =====================================================
@@ -72,6 +73,7 @@ def lib():
         i = ctypes.c_int
         vp = ctypes.c_void_p
         L.oracle_momentum_tendencies.argtypes = [vp, vp, vp, i, i, d, d, d, d, i, vp, vp]
+        L.oracle_momentum_tendencies_var.argtypes = [vp, vp, vp, i, i, d, d, vp, d, i, vp, vp]
         L.oracle_obstacle_bc.argtypes = [vp, vp, vp, i, i, i]
         L.oracle_scalar_transport.argtypes = ([vp, vp, vp, vp, i, i, d, d, vp, i, i]
                                               + [vp] * 8 + [i, d, vp])
@@ -100,6 +102,19 @@ def momentum_tendencies(u, v, fluid, nx, ny, dx, dy, nu0, gamma, periodic, hu, h
     _chk(hv, (nx + 2, ny + 1), np.float64)
     lib().oracle_momentum_tendencies(_p(u), _p(v), _p(fl), nx, ny, dx, dy, nu0, gamma,
                                      int(bool(periodic)), _p(hu), _p(hv))
+
+
+def momentum_tendencies_var(u, v, fluid, nx, ny, dx, dy, nu, gamma, periodic, hu, hv) -> None:
+    """kernels.py:44-155 with ``variable = True``: stress-form diffusion of the
+    cell viscosity field ``nu`` (nu_eff, (nx+2, ny+2))."""
+    fl = np.ascontiguousarray(fluid, dtype=np.uint8)
+    _chk(u, (nx + 1, ny + 2), np.float64)
+    _chk(v, (nx + 2, ny + 1), np.float64)
+    _chk(nu, (nx + 2, ny + 2), np.float64)
+    _chk(hu, (nx + 1, ny + 2), np.float64)
+    _chk(hv, (nx + 2, ny + 1), np.float64)
+    lib().oracle_momentum_tendencies_var(_p(u), _p(v), _p(fl), nx, ny, dx, dy, _p(nu), gamma,
+                                         int(bool(periodic)), _p(hu), _p(hv))
 
 
 def obstacle_bc(u, v, fluid, nx, ny, periodic) -> None:
@@ -431,9 +446,10 @@ class OScalar:
     """Explicit-Euler TVD transport of one cell field (models.py:73-250)."""
 
     def __init__(self, name, diffusivity, bcs=None, obstacle=("zerogradient", 0.0),
-                 limiter="vanleer", initial=0.0):
+                 limiter="vanleer", initial=0.0, source=None, lower=None, upper=None):
         self.name, self.D0, self.bcs_in = name, float(diffusivity), bcs or {}
         self.obstacle, self.limiter, self.initial = obstacle, limiter, initial
+        self.source, self.lower, self.upper = source, lower, upper
 
     def attach(self, sim):
         self.sim = sim
@@ -514,7 +530,15 @@ class OScalar:
                          self.bt["bottom"], self.bv["bottom"], self.bt["top"], self.bv["top"],
                          1 if fixed else 0, float(self.obstacle[1]) if fixed else 0.0, self.rhs)
         inner = self.phi[1:-1, 1:-1]
-        self.phi[1:-1, 1:-1] = np.where(g.interior, inner + dt * self.rhs[1:-1, 1:-1], inner)
+        new = inner + dt * self.rhs[1:-1, 1:-1]
+        if self.source is not None:       # S = S_exp - S_imp phi, implicit part pointwise
+            s_exp, s_imp = self.source(sim, self)
+            new = (new + dt * s_exp) / (1.0 + dt * np.maximum(s_imp, 0.0))
+        if self.lower is not None:
+            new = np.maximum(new, self.lower)
+        if self.upper is not None:
+            new = np.minimum(new, self.upper)
+        self.phi[1:-1, 1:-1] = np.where(g.interior, new, inner)
         self.fill_ghosts()
 
     def dt_limit(self):
@@ -657,6 +681,9 @@ class OSim:
             self.u_fluid[0, :] = False
         self.body_force = (float(body_force[0]), float(body_force[1]))
         self.flow_force = 0.0
+        self.nu_t = np.zeros((nx + 2, ny + 2))
+        self.nu_eff = np.full((nx + 2, ny + 2), self.nu)
+        self.variable_viscosity = False
         self.scalars, self.models = {}, []
         for m in models or []:
             m.attach(self)
@@ -700,7 +727,8 @@ class OSim:
         g = self.grid
         umax = max(np.abs(self.u).max(), 1e-12)
         vmax = max(np.abs(self.v).max(), 1e-12)
-        dt_visc = 0.5 / (self.nu * (1.0 / g.dx ** 2 + 1.0 / g.dy ** 2))
+        nu_max = float(self.nu_eff.max()) if self.variable_viscosity else self.nu
+        dt_visc = 0.5 / (nu_max * (1.0 / g.dx ** 2 + 1.0 / g.dy ** 2))
         dt_conv = min(g.dx / umax, g.dy / vmax)
         if self.scheme == "ab2":
             dt_visc *= 0.5
@@ -719,7 +747,8 @@ class OSim:
         if self.scheme == "euler":
             cfl = max(umax * dt / g.dx, vmax * dt / g.dy)
             return float(min(1.0, max(0.05, 1.2 * cfl)))
-        re_cell = max(umax * g.dx, vmax * g.dy) / self.nu
+        nu_ref = float(self.nu_eff[g.fluid].min()) if self.variable_viscosity else self.nu
+        re_cell = max(umax * g.dx, vmax * g.dy) / nu_ref
         return float(min(1.0, max(0.05, 0.015 * re_cell)))
 
     def solve_pressure_and_correct(self, dt):
@@ -761,11 +790,20 @@ class OSim:
     def step(self):
         g = self.grid
         self.apply_bc()
+        for m in self.models:
+            if hasattr(m, "pre_momentum"):
+                m.pre_momentum(self)
+        if self.variable_viscosity:
+            np.add(self.nu, self.nu_t, out=self.nu_eff)
         dt = self.compute_dt()
         self.gamma = (self.auto_gamma(dt) if self.gamma_setting == "auto"
                       else float(self.gamma_setting))
-        momentum_tendencies(self.u, self.v, g.fluid, g.nx, g.ny, g.dx, g.dy, self.nu,
-                            self.gamma, self.periodic, self.hu, self.hv)
+        if self.variable_viscosity:
+            momentum_tendencies_var(self.u, self.v, g.fluid, g.nx, g.ny, g.dx, g.dy, self.nu_eff,
+                                    self.gamma, self.periodic, self.hu, self.hv)
+        else:
+            momentum_tendencies(self.u, self.v, g.fluid, g.nx, g.ny, g.dx, g.dy, self.nu,
+                                self.gamma, self.periodic, self.hu, self.hv)
         for m in self.models:
             m.momentum_source(self, self.hu, self.hv)
         if self.scheme == "ab2" and self.hu_old is not None:

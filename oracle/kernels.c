@@ -15,6 +15,9 @@
  * Follows cfd2d/kernels.py:
  *   oracle_momentum_tendencies  kernels.py:44-155 (constant-viscosity branch,
  *                               lines 104-105; no variable density)
+ *   oracle_momentum_tendencies_var  the same with the stress-form diffusion of
+ *                               a cell viscosity field (kernels.py:96-103,
+ *                               131-138)
  *   oracle_obstacle_bc          kernels.py:158-209
  *   limiter / face_value        kernels.py:215-251
  *   oracle_scalar_transport     kernels.py:254-391
@@ -61,6 +64,67 @@ void oracle_momentum_tendencies(const double *u, const double *v, const uint8_t 
             const double vc = V_(i, j), ve = V_(i + 1, j), vw = V_(i - 1, j);
             const double vn = V_(i, j + 1), vs = V_(i, j - 1);
             const double diff = nu0 * ((ve - 2.0 * vc + vw) * idx2 + (vn - 2.0 * vc + vs) * idy2);
+            const double b_n = 0.5 * (vc + vn), b_s = 0.5 * (vs + vc);
+            const double dv2dy = idy * (b_n * b_n - b_s * b_s)
+                + gamma * idy * (fabs(b_n) * 0.5 * (vc - vn) - fabs(b_s) * 0.5 * (vs - vc));
+            const double u_e = 0.5 * (U_(i, j) + U_(i, j + 1));
+            const double u_w = 0.5 * (U_(i - 1, j) + U_(i - 1, j + 1));
+            const double duvdx = idx * (u_e * 0.5 * (vc + ve) - u_w * 0.5 * (vw + vc))
+                + gamma * idx * (fabs(u_e) * 0.5 * (vc - ve) - fabs(u_w) * 0.5 * (vw - vc));
+            hv[(long)i * (ny + 1) + j] = diff - duvdx - dv2dy;
+        }
+    }
+}
+
+/* Stress-form diffusion div(nu (grad u + grad u^T)) with the cell field nu
+   (nu_eff), kernels.py:96-103 for u faces and 131-138 for v faces; the
+   convection terms are those of the constant branch. */
+void oracle_momentum_tendencies_var(const double *u, const double *v, const uint8_t *fluid,
+                                    int nx, int ny, double dx, double dy, const double *nu,
+                                    double gamma, int periodic, double *hu, double *hv)
+{
+    const double idx2 = 1.0 / (dx * dx), idy2 = 1.0 / (dy * dy);
+    const double idx = 1.0 / dx, idy = 1.0 / dy;
+    for (long k = 0; k < (long)(nx + 1) * (ny + 2); ++k) hu[k] = 0.0;
+    for (long k = 0; k < (long)(nx + 2) * (ny + 1); ++k) hv[k] = 0.0;
+    const int imax = periodic ? nx : nx - 1;
+    for (int i = 1; i <= imax; ++i) {
+        const int ip = i < nx ? i + 1 : 1;
+        for (int j = 1; j <= ny; ++j) {
+            if (!(C_(fluid, i, j) && C_(fluid, i + 1, j))) continue;
+            const double uc = U_(i, j), ue = U_(ip, j), uw = U_(i - 1, j);
+            const double un = U_(i, j + 1), us = U_(i, j - 1);
+            const double nu_e = C_(nu, i + 1, j), nu_w = C_(nu, i, j);
+            const double nu_n = 0.25 * (C_(nu, i, j) + C_(nu, i + 1, j) + C_(nu, i, j + 1)
+                                        + C_(nu, i + 1, j + 1));
+            const double nu_s = 0.25 * (C_(nu, i, j) + C_(nu, i + 1, j) + C_(nu, i, j - 1)
+                                        + C_(nu, i + 1, j - 1));
+            const double diff = (2.0 * nu_e * (ue - uc) - 2.0 * nu_w * (uc - uw)) * idx2
+                + idy * (nu_n * ((un - uc) * idy + (V_(i + 1, j) - V_(i, j)) * idx)
+                         - nu_s * ((uc - us) * idy + (V_(i + 1, j - 1) - V_(i, j - 1)) * idx));
+            const double a_e = 0.5 * (uc + ue), a_w = 0.5 * (uw + uc);
+            const double du2dx = idx * (a_e * a_e - a_w * a_w)
+                + gamma * idx * (fabs(a_e) * 0.5 * (uc - ue) - fabs(a_w) * 0.5 * (uw - uc));
+            const double v_n = 0.5 * (V_(i, j) + V_(i + 1, j));
+            const double v_s = 0.5 * (V_(i, j - 1) + V_(i + 1, j - 1));
+            const double duvdy = idy * (v_n * 0.5 * (uc + un) - v_s * 0.5 * (us + uc))
+                + gamma * idy * (fabs(v_n) * 0.5 * (uc - un) - fabs(v_s) * 0.5 * (us - uc));
+            hu[(long)i * (ny + 2) + j] = diff - du2dx - duvdy;
+        }
+    }
+    for (int i = 1; i <= nx; ++i) {
+        for (int j = 1; j < ny; ++j) {
+            if (!(C_(fluid, i, j) && C_(fluid, i, j + 1))) continue;
+            const double vc = V_(i, j), ve = V_(i + 1, j), vw = V_(i - 1, j);
+            const double vn = V_(i, j + 1), vs = V_(i, j - 1);
+            const double nu_n = C_(nu, i, j + 1), nu_s = C_(nu, i, j);
+            const double nu_e = 0.25 * (C_(nu, i, j) + C_(nu, i + 1, j) + C_(nu, i, j + 1)
+                                        + C_(nu, i + 1, j + 1));
+            const double nu_w = 0.25 * (C_(nu, i, j) + C_(nu, i - 1, j) + C_(nu, i, j + 1)
+                                        + C_(nu, i - 1, j + 1));
+            const double diff = (2.0 * nu_n * (vn - vc) - 2.0 * nu_s * (vc - vs)) * idy2
+                + idx * (nu_e * ((ve - vc) * idx + (U_(i, j + 1) - U_(i, j)) * idy)
+                         - nu_w * ((vc - vw) * idx + (U_(i - 1, j + 1) - U_(i - 1, j)) * idy));
             const double b_n = 0.5 * (vc + vn), b_s = 0.5 * (vs + vc);
             const double dv2dy = idy * (b_n * b_n - b_s * b_s)
                 + gamma * idy * (fabs(b_n) * 0.5 * (vc - vn) - fabs(b_s) * 0.5 * (vs - vc));

@@ -33,6 +33,12 @@ Fixtures (all seeds use ``numpy.random.default_rng``):
   1e-10), 30 steps.
 * ``autogamma32.npz``   cavity Re=1000 32x32 with gamma "auto" and adaptive
   dt, ``run(t_end=0.05)`` (last-step shortening), direct solver.
+* ``turbulence.npz``    the stress-form ``momentum_tendencies`` (``variable =
+  True``) on random fields with a random viscosity field (periodic and
+  not); a Smagorinsky cavity (Re=1000, 32x32, CG, 30 steps, ``l^2`` and
+  ``nu_t`` after steps 1 and 30); k-omega SST on the backward-facing step
+  (Re=2000, length 6, ny 16, CG, 20 steps: wall distance, wall omega, k,
+  omega and ``nu_t`` after the attach and after steps 1 and 20).
 """
 
 from __future__ import annotations
@@ -254,6 +260,63 @@ def autogamma(R) -> dict:
     return out
 
 
+def turbulence(R) -> dict:
+    kernels, Grid, cases = R["kernels"], R["Grid"], R["cases"]
+    rng = np.random.default_rng(20240917)
+    nx, ny = 16, 12
+    dx, dy = 1.0 / nx, 0.75 / ny
+    out = {}
+    for periodic in (False, True):
+        tag = "p" if periodic else "np"
+        g = Grid(nx, ny, 1.0, 0.75)
+        m = np.zeros((nx, ny), dtype=bool)
+        m[4:7, 3:6] = True
+        m[10, 2:9] = True
+        g.add_obstacle(m)
+        if periodic:
+            g.set_periodic_x()
+        fluid = g.fluid.copy()
+        u = rng.standard_normal((nx + 1, ny + 2))
+        v = rng.standard_normal((nx + 2, ny + 1))
+        nu = rng.uniform(0.001, 0.05, (nx + 2, ny + 2))
+        hu, hv = np.full_like(u, 7.0), np.full_like(v, 7.0)
+        kernels.momentum_tendencies(u, v, fluid, nx, ny, dx, dy, nu, True, 0.05, periodic, hu, hv,
+                                    np.ones((nx + 2, ny + 2)), False)
+        out.update({f"var_fluid_{tag}": fluid, f"var_u_{tag}": u, f"var_v_{tag}": v,
+                    f"var_nu_{tag}": nu, f"var_hu_{tag}": hu, f"var_hv_{tag}": hv})
+
+    case = cases.add_turbulence(cases.lid_driven_cavity(re=1000, n=32, pressure_solver="cg"),
+                                "Smagorinsky")
+    sim = case.sim
+    smag = sim.model("turbulence")
+    out["smag_l2"] = smag.l2.copy()
+    out["smag_nut_0"] = sim.nu_t.copy()
+    for n in range(1, 31):
+        sim.step()
+        if n in (1, 30):
+            out.update({f"smag_u_{n}": sim.u.copy(), f"smag_v_{n}": sim.v.copy(),
+                        f"smag_p_{n}": sim.p.copy(), f"smag_nut_{n}": sim.nu_t.copy()})
+    out.update({"smag_" + k: v for k, v in _hist(sim).items()})
+
+    case = cases.backward_facing_step(re=2000, length=6, ny=16, t_end=1.0, pressure_solver="cg")
+    cases.add_turbulence(case, "kOmegaSST")
+    sim = case.sim
+    sst = sim.model("turbulence")
+    out.update({"sst_d": sst.d.copy(), "sst_wall_adj": sst.wall_adj.copy(),
+                "sst_w_wall": sst.w_wall.copy(), "sst_k_0": sst.k.phi.copy(),
+                "sst_w_0": sst.w.phi.copy(), "sst_nut_0": sim.nu_t.copy(),
+                "sst_fluid": sim.grid.fluid.copy()})
+    for n in range(1, 21):
+        sim.step()
+        if n in (1, 20):
+            out.update({f"sst_u_{n}": sim.u.copy(), f"sst_v_{n}": sim.v.copy(),
+                        f"sst_p_{n}": sim.p.copy(), f"sst_nut_{n}": sim.nu_t.copy(),
+                        f"sst_k_{n}": sst.k.phi.copy(), f"sst_w_{n}": sst.w.phi.copy(),
+                        f"sst_kdiff_{n}": sst.k.diff.copy(), f"sst_wdiff_{n}": sst.w.diff.copy()})
+    out.update({"sst_" + k: v for k, v in _hist(sim).items()})
+    return out
+
+
 def main() -> None:
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     R = _import_reference(root)
@@ -266,7 +329,8 @@ def main() -> None:
             ("periodic64x8", lambda: periodic(R)),
             ("euler16", lambda: euler16(R)),
             ("autogamma32", lambda: autogamma(R)),
-            ("cavity64_c1", lambda: cavity_c1(R))]
+            ("cavity64_c1", lambda: cavity_c1(R)),
+            ("turbulence", lambda: turbulence(R))]
     only = set(sys.argv[2:])
     for name, fn in jobs:
         if only and name not in only:
