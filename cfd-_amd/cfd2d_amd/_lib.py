@@ -24,7 +24,8 @@ CFD_ERR_LIMIT = -4
 
 BC_KINDS = {"noslip": 0, "slip": 1, "inflow": 2, "outflow": 3, "periodic": 4}
 FIELDS = {"u": 0, "v": 1, "p": 2, "F": 3, "G": 4, "hu": 5, "hv": 6, "hu_old": 7, "hv_old": 8,
-          "x": 9}
+          "x": 9, "nut": 10, "diff_k": 11, "diff_omega": 12}
+REC_NUT_MAX = 30     # record slot of the step's largest nu_t (10 + 5 * 4 scalar sums before it)
 SOLVERS = {"pcg64": 0, "mgcg": 1, "learned": 2}
 LEARNED_MODES = {"net": 0, "none": 1, "teacher": 2}
 
@@ -77,6 +78,12 @@ class ProbeDesc(ctypes.Structure):
                 ("src", _vp), ("corner", _vp), ("frac", _vp)]
 
 
+class TurbDesc(ctypes.Structure):
+    _fields_ = [("model", _i32), ("sid_k", _i32), ("sid_omega", _i32), ("pad", _i32),
+                ("nu", _d), ("l2", _vp), ("wall_distance", _vp), ("wall_adjacent", _vp),
+                ("omega_wall", _vp)]
+
+
 class LearnedStats(ctypes.Structure):
     _fields_ = [("harvested", _i64), ("solves", _i64), ("iterations", _i64), ("fallbacks", _i64),
                 ("last_iters", _i32), ("last_info", _i32), ("last_relres", _d),
@@ -115,7 +122,7 @@ class State(ctypes.Structure):
 
 class Record(ctypes.Structure):
     _fields_ = [("t", _d), ("dt", _d), ("gamma", _d), ("residual", _d), ("divergence", _d),
-                ("kinetic_energy", _d), ("cg_relres", _d), ("mon", _d * 24), ("n", _i64),
+                ("kinetic_energy", _d), ("cg_relres", _d), ("mon", _d * 32), ("n", _i64),
                 ("cg_iters", _i32), ("cg_info", _i32), ("cg_outer", _i32), ("nmon", _i32)]
 
 
@@ -168,6 +175,9 @@ def lib() -> ctypes.CDLL:
         "cfd_favg_set": ([_vp, _i32, _d], _i32),
         "cfd_favg_read": ([_vp, P(_d), _vp, _vp], _i32),
         "cfd_download_derived": ([_vp, _i32, _vp], _i32),
+        "cfd_set_turbulence": ([_vp, P(TurbDesc)], _i32),
+        "cfd_momentum_tendencies_var": ([_vp, _vp, _vp, _i32, _i32, _d, _d, _vp, _d, _i32, _vp,
+                                         _vp, _vp], _i32),
         "cfd_set_learned": ([_vp, P(LearnedDesc)], _i32),
         "cfd_learned_stats": ([_vp, P(LearnedStats)], _i32),
         "cfd_learned_reset": ([_vp], _i32),
@@ -210,7 +220,7 @@ def exported_symbols() -> list[str]:
                         "cfd_dev_free", "cfd_memcpy", "cfd_momentum_tendencies",
                         "cfd_obstacle_bc", "cfd_scalar_transport", "cfd_probes_set",
                         "cfd_probes_read", "cfd_favg_set", "cfd_favg_read", "cfd_download_derived",
-                        "cfd_set_learned",
+                        "cfd_set_turbulence", "cfd_momentum_tendencies_var", "cfd_set_learned",
                         "cfd_learned_stats", "cfd_learned_reset", "cfd_ens_create",
                         "cfd_ens_destroy", "cfd_ens_set_net", "cfd_ens_step", "cfd_ens_download",
                         "cfd_ens_stats", "cfd_unet_param_count",

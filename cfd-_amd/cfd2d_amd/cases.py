@@ -188,9 +188,20 @@ def _unsupported(name):
 
 
 def add_turbulence(case: Case, model: str = "kOmegaSST", **kw) -> Case:
-    if model in (None, "laminar"):
+    """Adds a turbulence model, ``kOmegaSST`` or ``Smagorinsky`` (cases.py:326-337)."""
+    from .turbulence import TURBULENCE_MODELS
+
+    if model not in TURBULENCE_MODELS:
+        raise ValueError(f"Unbekanntes Turbulenzmodell {model!r}, erlaubt: "
+                         f"{list(TURBULENCE_MODELS)}")
+    cls = TURBULENCE_MODELS[model]
+    if cls is None:
         return case
-    raise ValueError("turbulence models are not part of the GPU projection step")
+    if model == "kOmegaSST":
+        kw.setdefault("U_ref", case.U_ref)
+    case.sim.add_model(cls(**kw))
+    case.info["turbulenz"] = model
+    return case
 
 
 def add_temperature(case: Case, alpha: float, T_inflow: float = 0.0,
@@ -249,7 +260,7 @@ def build_case(spec: dict) -> Case:
     case = CASES[key](**params, **num)
     turb = spec.get("turbulenz")
     if turb and turb.get("modell", "laminar") != "laminar":
-        add_turbulence(case, turb["modell"])
+        add_turbulence(case, turb["modell"], **{k: v for k, v in turb.items() if k != "modell"})
     if spec.get("temperatur"):
         add_temperature(case, **spec["temperatur"])
     if spec.get("skalar"):

@@ -156,6 +156,23 @@ class Simulation:
         model.attach(self)
         self.models.append(model)
 
+    @property
+    def nu_t(self) -> np.ndarray:
+        """Turbulent viscosity (nx+2, ny+2) as the model set it for the last
+        step (zero when laminar); a host copy."""
+        g = self.grid
+        if not self.variable_viscosity:
+            return np.zeros((g.nx + 2, g.ny + 2))
+        self._flush()
+        return self._ctx.download("nut")
+
+    @property
+    def nu_eff(self) -> np.ndarray:
+        """nu + nu_t (solver.py:332-333)."""
+        if not self.variable_viscosity:
+            return np.full((self.grid.nx + 2, self.grid.ny + 2), self.nu)
+        return np.add(self.nu, self.nu_t)
+
     def model(self, kind: str):
         for m in self.models:
             if getattr(m, "kind", None) == kind:
@@ -244,7 +261,7 @@ class Simulation:
         p.gamma_auto = 1 if self.gamma_setting == "auto" else 0
         p.gamma_fixed = 0.0 if self.gamma_setting == "auto" else float(self.gamma_setting)
         p.dt_fixed = float(self.dt_fixed) if self.dt_fixed else 0.0
-        lims = [m.dt_limit(self) for m in self.models]
+        lims = [m.dt_limit(self) for m in self.models if not getattr(m, "device_dt_limit", False)]
         lims = [x for x in lims if x is not None]
         p.dt_model = min(lims) if lims else 1e308
         p.fx = self.body_force[0]
@@ -277,7 +294,8 @@ class Simulation:
         g = self.grid
         umax = max(np.abs(self.u).max(), 1e-12)
         vmax = max(np.abs(self.v).max(), 1e-12)
-        dt_visc = 0.5 / (self.nu * (1.0 / g.dx ** 2 + 1.0 / g.dy ** 2))
+        nu_max = float(self.nu_eff.max()) if self.variable_viscosity else self.nu
+        dt_visc = 0.5 / (nu_max * (1.0 / g.dx ** 2 + 1.0 / g.dy ** 2))
         dt_conv = min(g.dx / umax, g.dy / vmax)
         if self.scheme == "ab2":
             dt_visc *= 0.5

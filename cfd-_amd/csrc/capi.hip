@@ -18,6 +18,7 @@
 #include "lpcg.hip"
 #include "ens.hip"
 #include "monitors.hip"
+#include "turb.hip"
 
 namespace {
 
@@ -98,6 +99,15 @@ struct cfd_ctx {
     } favg;
     long long* mon_last_n = nullptr;
     double* derived = nullptr;   // (nx+1) x (ny+1) node field for reports
+    // turbulence model (turb.hip); cell arrays (nx+2) x P
+    struct {
+        int model = TURB_NONE;
+        double nu = 0.0;
+        double *nut = nullptr, *l2 = nullptr, *d = nullptr, *F1 = nullptr, *cross = nullptr;
+        double *dk = nullptr, *dw = nullptr, *wwall = nullptr;
+        uint8_t* wadj = nullptr;
+        int sid_k = -1, sid_w = -1;
+    } turb;
     double* rold = nullptr;
     MGDev* mg_dev = nullptr;     // device copy of the hierarchy description
     MGDev mg_host;               // host copy (level sizes and array pointers)
@@ -166,6 +176,8 @@ int shape_of(const cfd_ctx* c, int field, int* rows, int* cols, int* off_i, int*
         *rows = c->nx + 2; *cols = c->ny + 2; return 0;
     case CFD_FIELD_X:
         *rows = c->nx; *cols = c->ny; *off_i = 1; *off_j = 1; return 0;
+    case CFD_FIELD_NUT: case CFD_FIELD_DIFF_K: case CFD_FIELD_DIFF_OMEGA:
+        *rows = c->nx + 2; *cols = c->ny + 2; return 0;
     default:
         return set_err(CFD_ERR_ARG, "unknown field id %d", field);
     }
@@ -190,6 +202,9 @@ double* field_ptr(cfd_ctx* c, int field, int par) {
     case CFD_FIELD_HU_OLD: return c->H[0][par ^ 1];
     case CFD_FIELD_HV_OLD: return c->H[1][par ^ 1];
     case CFD_FIELD_X: return c->x;
+    case CFD_FIELD_NUT: return c->turb.nut;
+    case CFD_FIELD_DIFF_K: return c->turb.dk;
+    case CFD_FIELD_DIFF_OMEGA: return c->turb.dw;
     default: return nullptr;
     }
 }
@@ -245,6 +260,7 @@ PredArgs pred_args(cfd_ctx* c, const cfd_step_params* prm, int par) {
         }
     }
     a.nu0 = prm ? prm->nu : 0.0;
+    a.nut = c->turb.model != TURB_NONE ? c->turb.nut : nullptr;
     a.fx = prm ? prm->fx : 0.0;
     a.fy = prm ? prm->fy : 0.0;
     a.st = c->st;
@@ -354,6 +370,71 @@ ScalarArgs scalar_args(cfd_ctx* c, ScalarDev& s, int par) {
     }
     a.st = c->st;
     return a;
+}
+
+TurbArgs turb_args(cfd_ctx* c, int par, double nu, bool check_done) {
+    TurbArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.model = c->turb.model;
+    a.nu = nu;
+    a.u = c->u;
+    a.v = c->v;
+    a.fl = c->fl;
+    a.nut = c->turb.nut;
+    a.l2 = c->turb.l2;
+    if (c->turb.model == TURB_SST) {
+        a.kphi = c->scalars[c->turb.sid_k].phi[par];
+        a.wphi = c->scalars[c->turb.sid_w].phi[par];
+    }
+    a.d = c->turb.d;
+    a.F1 = c->turb.F1;
+    a.cross = c->turb.cross;
+    a.dk = c->turb.dk;
+    a.dw = c->turb.dw;
+    for (int k = 0; k < 4; ++k) a.kind[k] = c->kind[k];
+    a.st = check_done ? c->st : nullptr;
+    return a;
+}
+
+// nu_t (and the SST diffusivities) from the velocity and k, omega of parity
+// par: inner cells, then the ghost ring
+int launch_turb_pre(cfd_ctx* c, int par, double nu, bool check_done) {
+    const TurbArgs a = turb_args(c, par, nu, check_done);
+    hipLaunchKernelGGL(k_turb_cells, dim3((c->ny + 255) / 256, c->nx), dim3(256), 0, c->stream, a,
+                       c->geo);
+    HIPC(hipGetLastError());
+    const int ng = 2 * (c->nx + 2) + 2 * c->ny;
+    hipLaunchKernelGGL(k_turb_ghosts, dim3((ng + 255) / 256), dim3(256), 0, c->stream, a, c->geo);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+// k and omega after the step (parity par -> par ^ 1), ghost fills included
+int launch_sst_post(cfd_ctx* c, int par) {
+    ScalarDev& K = c->scalars[c->turb.sid_k];
+    ScalarDev& W = c->scalars[c->turb.sid_w];
+    SstArgs s;
+    s.k = scalar_args(c, K, par);
+    s.w = scalar_args(c, W, par);
+    s.k.diff = c->turb.dk;
+    s.w.diff = c->turb.dw;
+    s.nut = c->turb.nut;
+    s.F1 = c->turb.F1;
+    s.cross = c->turb.cross;
+    hipLaunchKernelGGL(k_sst_update, grid2d(c), dim3(BS2D), 0, c->stream, s, c->geo);
+    HIPC(hipGetLastError());
+    hipLaunchKernelGGL(k_fill_ghosts, grid2d(c), dim3(BS2D), 0, c->stream, s.k, c->geo);
+    HIPC(hipGetLastError());
+    hipLaunchKernelGGL(k_fill_ghosts, grid2d(c), dim3(BS2D), 0, c->stream, s.w, c->geo);
+    HIPC(hipGetLastError());
+    // the old omega buffer keeps the first fill; the wall cells get omega_wall
+    hipLaunchKernelGGL(k_sst_wall, grid2d(c), dim3(BS2D), 0, c->stream, s.w.phi_out,
+                       (double*)s.w.phi_in, (const uint8_t*)c->turb.wadj,
+                       (const double*)c->turb.wwall, c->geo, (const DevState*)c->st);
+    HIPC(hipGetLastError());
+    hipLaunchKernelGGL(k_fill_ghosts, grid2d(c), dim3(BS2D), 0, c->stream, s.w, c->geo);
+    HIPC(hipGetLastError());
+    return 0;
 }
 
 int host_parity(cfd_ctx* c, int* par) {
@@ -710,7 +791,7 @@ int cfd_ctx_create(const cfd_grid_desc* d, cfd_ctx** out) {
         (rc = dalloc(c, &c->comp, N)) || (rc = dalloc(c, &c->lut, 128)) ||
         (rc = dalloc(c, &c->comp_count, CFD_MAX_COMP)) || (rc = dalloc(c, &c->st, 1)) ||
         (rc = dalloc(c, &c->ring, RING)) || (rc = dalloc(c, &c->bar, 4)) ||
-        (rc = dalloc(c, &c->ticket, 4)) || (rc = dalloc(c, &c->acc, 4)))
+        (rc = dalloc(c, &c->ticket, 4)) || (rc = dalloc(c, &c->acc, 8)))
         return fail(rc);
     // persistent projection launch shape: one block per CU (or per row on small grids)
     c->proj_bs = std::min(PROJ_BS, ((c->ny + 63) / 64) * 64);
@@ -1005,6 +1086,8 @@ int cfd_upload(cfd_ctx* c, int32_t field, const double* host) {
     TRY(shape_of(c, field, &rows, &cols, &oi, &oj));
     int par = 0;
     TRY(host_parity(c, &par));
+    if (field >= CFD_FIELD_NUT)
+        return set_err(CFD_ERR_ARG, "field %d is computed by the turbulence model", field);
     double* dst = field_ptr(c, field, par);
     if (field == CFD_FIELD_X) c->l_hist = 0;   // a new iterate starts a new history
     HIPC(hipMemcpy2DAsync(dst + (size_t)oi * c->P + oj, c->P * sizeof(double), host,
@@ -1021,6 +1104,7 @@ int cfd_download(cfd_ctx* c, int32_t field, double* host) {
     int par = 0;
     TRY(host_parity(c, &par));
     const double* src = field_ptr(c, field, par);
+    if (!src) return set_err(CFD_ERR_STATE, "field %d: no turbulence model set", field);
     HIPC(hipMemcpy2DAsync(host, cols * sizeof(double), src + (size_t)oi * c->P + oj,
                           c->P * sizeof(double), cols * sizeof(double), rows,
                           hipMemcpyDeviceToHost, c->stream));
@@ -1213,22 +1297,33 @@ int cfd_step(cfd_ctx* c, const cfd_step_params* prm, int32_t count) {
         // batch is issued with the parity it will have if all steps run.
         const int pk = par ^ (k & 1);
         TRY(launch_bc(c, true));
+        TurbDt td{nullptr, nullptr, nullptr, c->fl};
+        if (c->turb.model != TURB_NONE) {
+            TRY(launch_turb_pre(c, pk, prm->nu, true));
+            td.nut = c->turb.nut;
+            if (c->turb.model == TURB_SST) {
+                td.dk = c->turb.dk;
+                td.dw = c->turb.dw;
+            }
+        }
         hipLaunchKernelGGL(k_dt_gamma, dim3(c->red_blocks), dim3(256), 0,
-                           c->stream, c->u, c->v, c->geo, sp, c->st, c->acc, c->ticket);
+                           c->stream, c->u, c->v, c->geo, sp, td, c->st, c->acc, c->ticket);
         HIPC(hipGetLastError());
         PredArgs pa = pred_args(c, prm, pk);
         hipLaunchKernelGGL(k_predictor, grid2d(c), dim3(BS2D), 0, c->stream, pa, c->geo);
         HIPC(hipGetLastError());
         TRY(launch_project(c, proj_args(c, prm->solver, prm->rtol, prm->maxiter)));
         TRY(launch_bc(c, true));
-        for (auto& s : c->scalars) {
-            ScalarArgs a = scalar_args(c, s, pk);
+        for (int si = 0; si < (int)c->scalars.size(); ++si) {
+            if (si == c->turb.sid_k || si == c->turb.sid_w) continue;   // SST below
+            ScalarArgs a = scalar_args(c, c->scalars[si], pk);
             hipLaunchKernelGGL((k_scalar_transport<true, 1>), grid2d(c), dim3(BS2D), 0, c->stream,
                                a, c->geo);
             HIPC(hipGetLastError());
             hipLaunchKernelGGL(k_fill_ghosts, grid2d(c), dim3(BS2D), 0, c->stream, a, c->geo);
             HIPC(hipGetLastError());
         }
+        if (c->turb.model == TURB_SST) TRY(launch_sst_post(c, pk));
         RecordArgs ra;
         std::memset(&ra, 0, sizeof(ra));
         ra.u = c->u;
@@ -1240,7 +1335,7 @@ int cfd_step(cfd_ctx* c, const cfd_step_params* prm, int32_t count) {
         for (int s = 0; s < ra.nscalar; ++s) {
             ScalarDev& sd = c->scalars[s];
             ra.phi[s] = sd.phi[pk ^ 1];
-            ra.diffs[s] = nullptr;
+            ra.diffs[s] = s == c->turb.sid_k ? c->turb.dk : (s == c->turb.sid_w ? c->turb.dw : nullptr);
             ra.D0[s] = sd.desc.diffusivity;
             ra.wallflux[s] = sd.desc.wall_flux;
             for (int k2 = 0; k2 < 4; ++k2) {
@@ -1388,8 +1483,22 @@ int cfd_momentum_tendencies(const double* u, const double* v, const uint8_t* flu
     if (!u || !v || !fluid || !hu || !hv) return set_err(CFD_ERR_ARG, "null array");
     if (nx < 4 || ny < 4) return set_err(CFD_ERR_ARG, "grid too small");
     Geo g{nx, ny, ny + 2, ny + 1, ny + 2, periodic ? 1 : 0, dx, dy};
-    hipLaunchKernelGGL(k_momentum_tendencies, dim3((ny + 2 + BS2D - 1) / BS2D, nx + 2),
-                       dim3(BS2D), 0, (hipStream_t)stream, u, v, fluid, g, nu0, gamma, hu, hv);
+    hipLaunchKernelGGL(k_momentum_tendencies<0>, dim3((ny + 2 + BS2D - 1) / BS2D, nx + 2),
+                       dim3(BS2D), 0, (hipStream_t)stream, u, v, fluid, g, nu0,
+                       (const double*)nullptr, gamma, hu, hv);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+int cfd_momentum_tendencies_var(const double* u, const double* v, const uint8_t* fluid,
+                                int32_t nx, int32_t ny, double dx, double dy, const double* nu,
+                                double gamma, int32_t periodic, double* hu, double* hv,
+                                void* stream) {
+    if (!u || !v || !fluid || !nu || !hu || !hv) return set_err(CFD_ERR_ARG, "null array");
+    if (nx < 4 || ny < 4) return set_err(CFD_ERR_ARG, "grid too small");
+    Geo g{nx, ny, ny + 2, ny + 1, ny + 2, periodic ? 1 : 0, dx, dy};
+    hipLaunchKernelGGL(k_momentum_tendencies<1>, dim3((ny + 2 + BS2D - 1) / BS2D, nx + 2),
+                       dim3(BS2D), 0, (hipStream_t)stream, u, v, fluid, g, 0.0, nu, gamma, hu, hv);
     HIPC(hipGetLastError());
     return 0;
 }
@@ -1580,6 +1689,65 @@ int cfd_download_derived(cfd_ctx* c, int32_t kind, double* host) {
     }
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(host, c->derived, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// ---- turbulence models (turbulence.py) -------------------------------------
+int cfd_set_turbulence(cfd_ctx* c, const cfd_turb_desc* d) {
+    if (!c || !d) return set_err(CFD_ERR_ARG, "null argument");
+    if (d->model != CFD_TURB_SMAGORINSKY && d->model != CFD_TURB_SST)
+        return set_err(CFD_ERR_ARG, "turbulence model %d", d->model);
+    if (c->turb.model != TURB_NONE) return set_err(CFD_ERR_STATE, "a turbulence model is set");
+    if (!(d->nu > 0.0)) return set_err(CFD_ERR_ARG, "viscosity must be positive");
+    const int nx = c->nx, ny = c->ny, P = c->P;
+    const int nsc = (int)c->scalars.size();
+    if (d->model == CFD_TURB_SMAGORINSKY && !d->l2) return set_err(CFD_ERR_ARG, "l2 missing");
+    if (d->model == CFD_TURB_SST) {
+        if (!d->wall_distance || !d->wall_adjacent || !d->omega_wall)
+            return set_err(CFD_ERR_ARG, "SST wall arrays missing");
+        if (d->sid_k < 0 || d->sid_k >= nsc || d->sid_omega < 0 || d->sid_omega >= nsc ||
+            d->sid_k == d->sid_omega)
+            return set_err(CFD_ERR_ARG, "SST scalars %d, %d (of %d)", d->sid_k, d->sid_omega, nsc);
+    }
+    HIPC(hipStreamSynchronize(c->stream));
+    const size_t N = cells(c);
+    // inner nx x ny host arrays into the padded device layout
+    auto put = [&](double** dst, const double* src) -> int {
+        TRY(dalloc(c, dst, N));
+        std::vector<double> h(N, 0.0);
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j) h[(size_t)(i + 1) * P + j + 1] = src[(size_t)i * ny + j];
+        HIPC(hipMemcpy(*dst, h.data(), N * sizeof(double), hipMemcpyHostToDevice));
+        return 0;
+    };
+    TRY(dalloc(c, &c->turb.nut, N));
+    if (d->model == CFD_TURB_SMAGORINSKY) {
+        TRY(put(&c->turb.l2, d->l2));
+    } else {
+        TRY(put(&c->turb.d, d->wall_distance));
+        TRY(put(&c->turb.wwall, d->omega_wall));
+        TRY(dalloc(c, &c->turb.wadj, N));
+        std::vector<uint8_t> h(N, 0);
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j)
+                h[(size_t)(i + 1) * P + j + 1] = d->wall_adjacent[(size_t)i * ny + j] ? 1 : 0;
+        HIPC(hipMemcpy(c->turb.wadj, h.data(), N, hipMemcpyHostToDevice));
+        TRY(dalloc(c, &c->turb.F1, N));
+        TRY(dalloc(c, &c->turb.cross, N));
+        TRY(dalloc(c, &c->turb.dk, N));
+        TRY(dalloc(c, &c->turb.dw, N));
+        c->turb.sid_k = d->sid_k;
+        c->turb.sid_w = d->sid_omega;
+    }
+    const unsigned long long inf_bits = 0x7ff0000000000000ull;
+    HIPC(hipMemcpy(c->acc + 3, &inf_bits, sizeof(inf_bits), hipMemcpyHostToDevice));
+    c->turb.model = d->model == CFD_TURB_SMAGORINSKY ? TURB_SMAGORINSKY : TURB_SST;
+    c->turb.nu = d->nu;
+    // the model's first evaluation at attach (turbulence.py:144, 239)
+    int par = 0;
+    TRY(host_parity(c, &par));
+    TRY(launch_turb_pre(c, par, d->nu, false));
     HIPC(hipStreamSynchronize(c->stream));
     return 0;
 }

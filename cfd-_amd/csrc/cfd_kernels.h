@@ -6,7 +6,7 @@
 // Boundary-condition kinds (boundary.py:19-25)
 enum { BC_NOSLIP = 0, BC_SLIP = 1, BC_INFLOW = 2, BC_OUTFLOW = 3, BC_PERIODIC = 4 };
 
-#define CFD_MAX_SCALARS 2
+#define CFD_MAX_SCALARS 4
 #define REC_NV (10 + 5 * CFD_MAX_SCALARS)
 #define CFD_MAX_COMP 8
 #define PROJ_BS 1024
@@ -35,6 +35,7 @@ struct PredArgs {
     const double* T;         // Boussinesq temperature or nullptr
     double gx, gy, Tref;
     double nu0;
+    const double* nut;       // turbulent viscosity (cells): stress-form diffusion of nu0 + nut
     double fx, fy;
     const DevState* st;
     int scheme_ab2;
@@ -172,15 +173,26 @@ struct FcgState {
     unsigned ticket[4];
 };
 
+// Turbulent viscosity and diffusivities the time-step control reads
+// (solver.py:216-218, 249-251, models.py:204-208); nut == nullptr: laminar.
+struct TurbDt {
+    const double* nut;
+    const double* dk;        // k-omega SST diffusivities of k and omega, or nullptr
+    const double* dw;
+    const uint8_t* fl;
+};
+
 // kernels (stencil.hip)
+template <int VIS>
 __global__ void k_momentum_tendencies(const double* u, const double* v, const uint8_t* fl, Geo g,
-                                      double nu0, double gamma, double* hu, double* hv);
+                                      double nu0, const double* nuf, double gamma, double* hu,
+                                      double* hv);
 __global__ void k_predictor(PredArgs a, Geo g);
 __global__ void k_copy_predictor(PredArgs a, Geo g);
 __global__ void k_bc_domain(BCArgs a, Geo g);
 __global__ void k_obstacle_bc(double* u, double* v, const uint8_t* fl, Geo g, const DevState* st);
-__global__ void k_dt_gamma(const double* u, const double* v, Geo g, StepParams prm, DevState* st,
-                           unsigned long long* acc, unsigned* ticket);
+__global__ void k_dt_gamma(const double* u, const double* v, Geo g, StepParams prm, TurbDt td,
+                           DevState* st, unsigned long long* acc, unsigned* ticket);
 template <bool CONST_D, int MODE>
 __global__ void k_scalar_transport(ScalarArgs a, Geo g);
 __global__ void k_fill_ghosts(ScalarArgs a, Geo g);

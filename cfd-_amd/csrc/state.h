@@ -18,9 +18,10 @@ struct DevState {
     int cg_outer;          // mixed precision: true-residual replacements
     int done;              // run finished (t_end, max_steps, steady_tol, error): later steps are no-ops
     int error;             // instability detected (solver.py:395-399)
+    double nut_max;        // max of the turbulent viscosity this step (0 when laminar)
 };
 
-#define CFD_MAX_MON 24
+#define CFD_MAX_MON 32
 // One record per completed step (cfd2d/solver.py:427-443 plus solver data).
 struct StepRecord {
     double t, dt, gamma, residual, divergence, ke;

@@ -12,15 +12,43 @@
 #include "cfd_kernels.h"
 
 // ---------------------------------------------------------------------------
-// Momentum tendencies at one face (kernels.py:84-155, constant viscosity)
+// Momentum tendencies at one face (kernels.py:84-155).  VIS selects the
+// diffusion term: 0 the constant viscosity nu0 (Laplacian form, lines
+// 104-105); 1 the stress form div(nu (grad u + grad u^T)) of the cell field
+// nuf (lines 96-103, 131-138); 2 the stress form of nu0 + nuf (nu_eff =
+// nu + nu_t, solver.py:332-333, formed where it is read).
 // ---------------------------------------------------------------------------
+template <int VIS>
+__device__ __forceinline__ double nu_at(const double* __restrict__ nuf, double nu0, int pc, int i,
+                                        int j) {
+    if (VIS == 1) return nuf[ix(i, j, pc)];
+    return nu0 + nuf[ix(i, j, pc)];
+}
+
+template <int VIS>
 __device__ __forceinline__ double mom_u_face(const double* __restrict__ u,
                                              const double* __restrict__ v, int pu, int pv,
-                                             int i, int ip, int j, double nu0, double gamma,
+                                             int i, int ip, int j, double nu0,
+                                             const double* __restrict__ nuf, int pc, double gamma,
                                              double idx2, double idy2, double idx, double idy) {
     const double uc = u[ix(i, j, pu)], ue = u[ix(ip, j, pu)], uw = u[ix(i - 1, j, pu)];
     const double un = u[ix(i, j + 1, pu)], us = u[ix(i, j - 1, pu)];
-    const double diff = nu0 * ((ue - 2.0 * uc + uw) * idx2 + (un - 2.0 * uc + us) * idy2);
+    double diff;
+    if (VIS == 0) {
+        diff = nu0 * ((ue - 2.0 * uc + uw) * idx2 + (un - 2.0 * uc + us) * idy2);
+    } else {
+        const double nc = nu_at<VIS>(nuf, nu0, pc, i, j), ne = nu_at<VIS>(nuf, nu0, pc, i + 1, j);
+        const double nn = nu_at<VIS>(nuf, nu0, pc, i, j + 1);
+        const double nne = nu_at<VIS>(nuf, nu0, pc, i + 1, j + 1);
+        const double ns = nu_at<VIS>(nuf, nu0, pc, i, j - 1);
+        const double nse = nu_at<VIS>(nuf, nu0, pc, i + 1, j - 1);
+        const double nu_n = 0.25 * (nc + ne + nn + nne);
+        const double nu_s = 0.25 * (nc + ne + ns + nse);
+        diff = (2.0 * ne * (ue - uc) - 2.0 * nc * (uc - uw)) * idx2 +
+               idy * (nu_n * ((un - uc) * idy + (v[ix(i + 1, j, pv)] - v[ix(i, j, pv)]) * idx) -
+                      nu_s * ((uc - us) * idy +
+                              (v[ix(i + 1, j - 1, pv)] - v[ix(i, j - 1, pv)]) * idx));
+    }
     const double a_e = 0.5 * (uc + ue), a_w = 0.5 * (uw + uc);
     const double du2dx = idx * (a_e * a_e - a_w * a_w)
         + gamma * idx * (fabs(a_e) * 0.5 * (uc - ue) - fabs(a_w) * 0.5 * (uw - uc));
@@ -31,13 +59,30 @@ __device__ __forceinline__ double mom_u_face(const double* __restrict__ u,
     return diff - du2dx - duvdy;
 }
 
+template <int VIS>
 __device__ __forceinline__ double mom_v_face(const double* __restrict__ u,
                                              const double* __restrict__ v, int pu, int pv,
-                                             int i, int j, double nu0, double gamma,
+                                             int i, int j, double nu0,
+                                             const double* __restrict__ nuf, int pc, double gamma,
                                              double idx2, double idy2, double idx, double idy) {
     const double vc = v[ix(i, j, pv)], ve = v[ix(i + 1, j, pv)], vw = v[ix(i - 1, j, pv)];
     const double vn = v[ix(i, j + 1, pv)], vs = v[ix(i, j - 1, pv)];
-    const double diff = nu0 * ((ve - 2.0 * vc + vw) * idx2 + (vn - 2.0 * vc + vs) * idy2);
+    double diff;
+    if (VIS == 0) {
+        diff = nu0 * ((ve - 2.0 * vc + vw) * idx2 + (vn - 2.0 * vc + vs) * idy2);
+    } else {
+        const double nc = nu_at<VIS>(nuf, nu0, pc, i, j), nn = nu_at<VIS>(nuf, nu0, pc, i, j + 1);
+        const double ne = nu_at<VIS>(nuf, nu0, pc, i + 1, j);
+        const double nne = nu_at<VIS>(nuf, nu0, pc, i + 1, j + 1);
+        const double nw = nu_at<VIS>(nuf, nu0, pc, i - 1, j);
+        const double nnw = nu_at<VIS>(nuf, nu0, pc, i - 1, j + 1);
+        const double nu_e = 0.25 * (nc + ne + nn + nne);
+        const double nu_w = 0.25 * (nc + nw + nn + nnw);
+        diff = (2.0 * nn * (vn - vc) - 2.0 * nc * (vc - vs)) * idy2 +
+               idx * (nu_e * ((ve - vc) * idx + (u[ix(i, j + 1, pu)] - u[ix(i, j, pu)]) * idy) -
+                      nu_w * ((vc - vw) * idx +
+                              (u[ix(i - 1, j + 1, pu)] - u[ix(i - 1, j, pu)]) * idy));
+    }
     const double b_n = 0.5 * (vc + vn), b_s = 0.5 * (vs + vc);
     const double dv2dy = idy * (b_n * b_n - b_s * b_s)
         + gamma * idy * (fabs(b_n) * 0.5 * (vc - vn) - fabs(b_s) * 0.5 * (vs - vc));
@@ -51,10 +96,12 @@ __device__ __forceinline__ double mom_v_face(const double* __restrict__ u,
 // Standalone momentum_tendencies (kernels.py:44-155): one thread per entry of
 // the (nx+2) x (ny+2) index box; writes every entry of hu and hv (zero off the
 // fluid faces, as kernels.py:74-79 does).
+// VIS 0: constant nu0; VIS 1: the stress form of the cell field nuf (nu_eff).
+template <int VIS>
 __global__ void k_momentum_tendencies(const double* __restrict__ u, const double* __restrict__ v,
                                       const uint8_t* __restrict__ fl, Geo g, double nu0,
-                                      double gamma, double* __restrict__ hu,
-                                      double* __restrict__ hv) {
+                                      const double* __restrict__ nuf, double gamma,
+                                      double* __restrict__ hu, double* __restrict__ hv) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y;
     if (j > g.ny + 1) return;
@@ -67,7 +114,8 @@ __global__ void k_momentum_tendencies(const double* __restrict__ u, const double
         if (i >= 1 && i <= imax && j >= 1 && j <= ny && fl[ix(i, j, g.pc)] & 1 &&
             fl[ix(i + 1, j, g.pc)] & 1) {
             const int ip = i < nx ? i + 1 : 1;
-            h = mom_u_face(u, v, g.pu, g.pv, i, ip, j, nu0, gamma, idx2, idy2, idx, idy);
+            h = mom_u_face<VIS>(u, v, g.pu, g.pv, i, ip, j, nu0, nuf, g.pc, gamma, idx2, idy2,
+                                idx, idy);
         }
         hu[ix(i, j, g.pu)] = h;
     }
@@ -75,7 +123,8 @@ __global__ void k_momentum_tendencies(const double* __restrict__ u, const double
         double h = 0.0;
         if (i >= 1 && i <= nx && j >= 1 && j < ny && fl[ix(i, j, g.pc)] & 1 &&
             fl[ix(i, j + 1, g.pc)] & 1)
-            h = mom_v_face(u, v, g.pu, g.pv, i, j, nu0, gamma, idx2, idy2, idx, idy);
+            h = mom_v_face<VIS>(u, v, g.pu, g.pv, i, j, nu0, nuf, g.pc, gamma, idx2, idy2, idx,
+                                idy);
         hv[ix(i, j, g.pv)] = h;
     }
 }
@@ -106,7 +155,10 @@ __global__ void k_predictor(PredArgs a, Geo g) {
         double h = 0.0;
         if (uf) {   // implies 1 <= i <= (periodic ? nx : nx-1) and 1 <= j <= ny
             const int ip = i < nx ? i + 1 : 1;
-            h = mom_u_face(a.u, a.v, g.pu, g.pv, i, ip, j, a.nu0, gamma, idx2, idy2, idx, idy);
+            h = a.nut ? mom_u_face<2>(a.u, a.v, g.pu, g.pv, i, ip, j, a.nu0, a.nut, g.pc, gamma,
+                                      idx2, idy2, idx, idy)
+                      : mom_u_face<0>(a.u, a.v, g.pu, g.pv, i, ip, j, a.nu0, a.nut, g.pc, gamma,
+                                      idx2, idy2, idx, idy);
         }
         if (uf && a.T && a.gx != 0.0) {
             const double Tf = 0.5 * (a.T[ix(i, j, g.pc)] + a.T[ix(i + 1, j, g.pc)]) - a.Tref;
@@ -136,7 +188,10 @@ __global__ void k_predictor(PredArgs a, Geo g) {
         const bool vf = (fl[ix(i, j, g.pc)] & 1) && (fl[ix(i, j + 1, g.pc)] & 1);
         double h = 0.0;
         if (vf && i >= 1 && i <= nx && j >= 1 && j < ny)
-            h = mom_v_face(a.u, a.v, g.pu, g.pv, i, j, a.nu0, gamma, idx2, idy2, idx, idy);
+            h = a.nut ? mom_v_face<2>(a.u, a.v, g.pu, g.pv, i, j, a.nu0, a.nut, g.pc, gamma, idx2,
+                                      idy2, idx, idy)
+                      : mom_v_face<0>(a.u, a.v, g.pu, g.pv, i, j, a.nu0, a.nut, g.pc, gamma, idx2,
+                                      idy2, idx, idy);
         if (vf && a.T && a.gy != 0.0) {
             const double Tf = 0.5 * (a.T[ix(i, j, g.pc)] + a.T[ix(i, j + 1, g.pc)]) - a.Tref;
             h += -a.gy * Tf;
@@ -333,12 +388,22 @@ __device__ __forceinline__ unsigned long long dbits(double x) {
     return (unsigned long long)__double_as_longlong(x);
 }
 
+// With turbulence (td.nut set) the viscous limit and the cell Reynolds number
+// use nu_eff = nu + nu_t: its maximum over every cell and its minimum over
+// the fluid cells (solver.py:216-218, 249-251; max(nu + x) = nu + max(x)
+// exactly, rounding being monotonic), and k-omega SST adds the explicit
+// diffusion limits of k and omega from their largest diffusivity on the
+// fluid cells (models.py:204-208, turbulence.py:314-315).  acc[2..5] hold
+// max nu_t, min nu_t (fluid), max D_k, max D_omega as the bits of
+// non-negative doubles, whose integer order is their numeric order.
 __global__ void k_dt_gamma(const double* __restrict__ u, const double* __restrict__ v, Geo g,
-                           StepParams prm, DevState* st, unsigned long long* acc,
+                           StepParams prm, TurbDt td, DevState* st, unsigned long long* acc,
                            unsigned* ticket) {
-    __shared__ double lds[2 * 16];
+    __shared__ double lds[5 * 16];
     const int nx = g.nx, ny = g.ny;
     double m[2] = {0.0, 0.0};
+    // turbulence: max nu_t, -min nu_t (fluid), max D_k, max D_omega (fluid)
+    double q[4] = {0.0, -__builtin_inf(), 0.0, 0.0};
     if (!st->done) {
         // row band of this block over rows 0..nx+1; threads along j
         const int r0 = (int)(((long)(nx + 2) * blockIdx.x) / gridDim.x);
@@ -354,11 +419,39 @@ __global__ void k_dt_gamma(const double* __restrict__ u, const double* __restric
                 m[1] = j <= ny ? nanmax(m[1], av) : m[1];
             }
         }
+        if (td.nut) {
+            for (int i = r0; i < r1; ++i)
+                for (int j = threadIdx.x; j <= ny + 1; j += blockDim.x) {
+                    const int c = ix(i, j, g.pc);
+                    const double nt = td.nut[c];
+                    const bool f = td.fl[c] & 1;
+                    q[0] = nanmax(q[0], nt);
+                    q[1] = f ? nanmax(q[1], -nt) : q[1];
+                    if (td.dk) {
+                        q[2] = f ? nanmax(q[2], td.dk[c]) : q[2];
+                        q[3] = f ? nanmax(q[3], td.dw[c]) : q[3];
+                    }
+                }
+        }
     }
     block_max<2>(m, lds);
+    if (td.nut) {
+        __syncthreads();
+        block_max<4>(q, lds);
+    }
     if (threadIdx.x == 0) {
         __hip_atomic_fetch_max(&acc[0], dbits(m[0]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_fetch_max(&acc[1], dbits(m[1]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (td.nut) {
+            __hip_atomic_fetch_max(&acc[2], dbits(q[0]), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_min(&acc[3], dbits(-q[1]), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_max(&acc[4], dbits(q[2]), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_max(&acc[5], dbits(q[3]), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
         const unsigned tk =
             __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         if (tk == gridDim.x - 1) {
@@ -368,6 +461,27 @@ __global__ void k_dt_gamma(const double* __restrict__ u, const double* __restric
                 (long long)__hip_atomic_load(&acc[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
             acc[0] = 0ull;
             acc[1] = 0ull;
+            double nu_max = prm.nu, nu_ref = prm.nu, lim_turb = __builtin_inf();
+            if (td.nut) {
+                auto rd = [&](int k) {
+                    return __longlong_as_double((long long)__hip_atomic_load(
+                        &acc[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                };
+                const double nt_max = rd(2), nt_min = rd(3), dk = rd(4), dw = rd(5);
+                acc[2] = 0ull;
+                acc[3] = dbits(__builtin_inf());
+                acc[4] = 0ull;
+                acc[5] = 0ull;
+                nu_max = prm.nu + nt_max;
+                nu_ref = prm.nu + nt_min;
+                if (!st->done) st->nut_max = nt_max;
+                if (td.dk) {
+                    const double h2 = 1.0 / (g.dx * g.dx) + 1.0 / (g.dy * g.dy);
+                    const double lk = dk > 0.0 ? 0.5 / (dk * h2) : __builtin_inf();
+                    const double lw = dw > 0.0 ? 0.5 / (dw * h2) : __builtin_inf();
+                    lim_turb = fmin(lk, lw);
+                }
+            }
             *ticket = 0u;
             if (!st->done) {
                 double dt;
@@ -375,7 +489,7 @@ __global__ void k_dt_gamma(const double* __restrict__ u, const double* __restric
                     dt = prm.dt_fixed;
                 } else {
                     const double umax = fmax(umax_raw, 1e-12), vmax = fmax(vmax_raw, 1e-12);
-                    double dt_visc = 0.5 / (prm.nu * (1.0 / (g.dx * g.dx) + 1.0 / (g.dy * g.dy)));
+                    double dt_visc = 0.5 / (nu_max * (1.0 / (g.dx * g.dx) + 1.0 / (g.dy * g.dy)));
                     double dt_conv = fmin(g.dx / umax, g.dy / vmax);
                     if (prm.scheme_ab2) {
                         dt_visc *= 0.5;
@@ -383,6 +497,7 @@ __global__ void k_dt_gamma(const double* __restrict__ u, const double* __restric
                     }
                     dt = prm.tau * fmin(dt_visc, dt_conv);
                     if (prm.dt_model < 1e300) dt = fmin(dt, prm.tau * prm.dt_model);
+                    if (lim_turb < 1e300) dt = fmin(dt, prm.tau * lim_turb);
                     if (prm.clamp_t_end && st->t + dt > prm.t_end) dt = prm.t_end - st->t;
                 }
                 double gamma;
@@ -391,7 +506,7 @@ __global__ void k_dt_gamma(const double* __restrict__ u, const double* __restric
                         const double cfl = fmax(umax_raw * dt / g.dx, vmax_raw * dt / g.dy);
                         gamma = fmin(1.0, fmax(0.05, 1.2 * cfl));
                     } else {
-                        const double re_cell = fmax(umax_raw * g.dx, vmax_raw * g.dy) / prm.nu;
+                        const double re_cell = fmax(umax_raw * g.dx, vmax_raw * g.dy) / nu_ref;
                         gamma = fmin(1.0, fmax(0.05, 0.015 * re_cell));
                     }
                 } else {
@@ -653,6 +768,7 @@ __global__ void k_fill_ghosts(ScalarArgs a, Geo g) {
 // `part`; the last block adds them in block order (deterministic), writes the
 // record and advances t, n, dt_old.
 // ---------------------------------------------------------------------------
+static_assert(REC_NV < CFD_MAX_MON, "the record holds the sums and nu_t's maximum");
 __global__ void k_record(RecordArgs a, Geo g) {
     __shared__ double lds[REC_NV * 16];
     const int nx = g.nx, ny = g.ny, pc = g.pc;
@@ -794,6 +910,7 @@ __global__ void k_record(RecordArgs a, Geo g) {
         r.cg_outer = st->cg_outer;
         r.nmon = 10 + 5 * a.nscalar;
         for (int q = 0; q < CFD_MAX_MON; ++q) r.mon[q] = q < REC_NV ? tot[q] : 0.0;
+        r.mon[REC_NV] = st->nut_max;   // turbulence.py:158-159, 322-324
         const double res = r.residual;
         if (!(res == res) || res > 1e8) {
             st->error = 1;          // solver.py:395-399: no record for this step

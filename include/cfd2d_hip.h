@@ -42,7 +42,10 @@ enum { CFD_LIM_UPWIND = 0, CFD_LIM_VANLEER = 1, CFD_LIM_MINMOD = 2, CFD_LIM_SUPE
    solver's warm start (pressure.py:140, 177) as an nx x ny array */
 enum { CFD_FIELD_U = 0, CFD_FIELD_V = 1, CFD_FIELD_P = 2, CFD_FIELD_F = 3, CFD_FIELD_G = 4,
        CFD_FIELD_HU = 5, CFD_FIELD_HV = 6, CFD_FIELD_HU_OLD = 7, CFD_FIELD_HV_OLD = 8,
-       CFD_FIELD_X = 9 };
+       CFD_FIELD_X = 9,
+       /* turbulence model outputs, (nx+2) x (ny+2), download only: nu_t and the
+          SST diffusivities of k and omega */
+       CFD_FIELD_NUT = 10, CFD_FIELD_DIFF_K = 11, CFD_FIELD_DIFF_OMEGA = 12 };
 /* pressure solvers: Jacobi-PCG in FP64 (the reference's "cg", pressure.py:171),
    and flexible CG with FP64 vectors and dot products preconditioned by an FP32
    aggregation-multigrid V-cycle (set up with cfd_set_multigrid), the solve
@@ -122,12 +125,13 @@ typedef struct cfd_state {
    cells, [2..5] sums of p over fluid cells with a solid east / west / north /
    south neighbour, [6..7] uc summed over the north / south ones, [8..9] vc
    over the east / west ones (obstacle forces, solver.py:542-567), then per
-   scalar s (5 values from index 10+5s): sum of phi over fluid cells and the
-   wall-flux sums of the left, right, bottom, top sides (models.py:228-250). */
+   scalar s (5 values from index 10+5s, s < 4): sum of phi over fluid cells and
+   the wall-flux sums of the left, right, bottom, top sides (models.py:228-250);
+   [30] the largest nu_t of the step (turbulence models, else 0). */
 typedef struct cfd_record {
     double t, dt, gamma, residual, divergence, kinetic_energy;
     double cg_relres;
-    double mon[24];
+    double mon[32];
     int64_t n;
     int32_t cg_iters, cg_info, cg_outer, nmon;
 } cfd_record;
@@ -317,6 +321,34 @@ int cfd_favg_read(cfd_ctx* ctx, double* T, double* sum, double* sum2);
    cumsum does (solver.py:580-596) */
 enum { CFD_DERIVED_VORTICITY = 0, CFD_DERIVED_STREAMFUNCTION = 1 };
 int cfd_download_derived(cfd_ctx* ctx, int32_t kind, double* host);
+
+/* ---- turbulence models (turbulence.py:130-325); no counterpart in a C
+   interface of the reference, which runs them in NumPy ----
+   Smagorinsky: nu_t = l2 |S| on fluid cells.  k-omega SST (Menter 2003, low
+   Re): k and omega are two scalars added with cfd_add_scalar (diffusivity nu,
+   the reference's boundary conditions, upwind or another limiter); the step
+   then transports them with the SST diffusivities and sources instead of as
+   passive scalars.  Arrays are nx x ny host layout.  Call once, after the
+   scalars are added and uploaded; it evaluates nu_t for the current fields. */
+enum { CFD_TURB_SMAGORINSKY = 1, CFD_TURB_SST = 2 };
+typedef struct cfd_turb_desc {
+    int32_t model;
+    int32_t sid_k, sid_omega;        /* SST */
+    int32_t pad;
+    double nu;                       /* molecular viscosity */
+    const double* l2;                /* Smagorinsky: min(C_s sqrt(dx dy), 0.41 d)^2 */
+    const double* wall_distance;     /* SST: d >= 1e-12 (turbulence.py:42-69, 237-238) */
+    const uint8_t* wall_adjacent;    /* SST: fluid cells next to a wall */
+    const double* omega_wall;        /* SST: 6 nu / (beta1 y^2) there (turbulence.py:239-243) */
+} cfd_turb_desc;
+int cfd_set_turbulence(cfd_ctx* ctx, const cfd_turb_desc* desc);
+/* kernels.momentum_tendencies with variable = True (kernels.py:96-103,
+   131-138): stress-form diffusion of the cell viscosity field nu
+   ((nx+2) x (ny+2), nu_eff) */
+int cfd_momentum_tendencies_var(const double* u, const double* v, const uint8_t* fluid,
+                                int32_t nx, int32_t ny, double dx, double dy, const double* nu,
+                                double gamma, int32_t periodic, double* hu, double* hv,
+                                void* stream);
 
 /* ---- learned preconditioner network (no counterpart in the reference: the
    U-Net of the learned flexible PCG, trained in-process on residual ->
