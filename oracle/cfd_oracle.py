@@ -658,7 +658,7 @@ class OSim:
 
     def __init__(self, grid, bcs, nu, scheme="ab2", gamma="auto", tau=0.5, dt=None,
                  pressure_solver="direct", initial_velocity=(0.0, 0.0), models=None,
-                 body_force=(0.0, 0.0), pressure_tol=1e-10):
+                 body_force=(0.0, 0.0), pressure_tol=1e-10, flow_rate=None):
         self.removed_cells = grid.finalize()
         self.grid, self.nu, self.scheme = grid, float(nu), scheme
         self.gamma_setting, self.tau, self.dt_fixed = gamma, float(tau), dt
@@ -680,6 +680,7 @@ class OSim:
         if self.periodic:
             self.u_fluid[0, :] = False
         self.body_force = (float(body_force[0]), float(body_force[1]))
+        self.flow_rate = flow_rate
         self.flow_force = 0.0
         self.nu_t = np.zeros((nx + 2, ny + 2))
         self.nu_eff = np.full((nx + 2, ny + 2), self.nu)
@@ -827,6 +828,8 @@ class OSim:
         self.boundary.extrapolate_outflow_predictor(self.F, self.G)
         self.solve_pressure_and_correct(dt)
         self.apply_bc()
+        if self.flow_rate is not None:
+            self.control_flow_rate(dt)
         for m in self.models:
             m.post_step(self, dt)
         if self.scheme == "ab2":
@@ -849,9 +852,34 @@ class OSim:
         h["residual"].append(self.residual)
         h["divergence"].append(self.max_divergence())
         h["kinetic_energy"].append(self.kinetic_energy())
+        if self.flow_rate is not None:
+            self.extra.setdefault("antriebskraft", []).append(self.flow_force)
         for m in self.models:
             for k, val in m.monitors(self).items():
                 self.extra.setdefault(k, []).append(val)
+
+    def bulk_velocity(self):
+        """Mean u over the fluid faces 1..nx (solver.py:405-408)."""
+        nx = self.grid.nx
+        return float(self.u[1:nx + 1, 1:-1][self.u_fluid[1:nx + 1, 1:-1]].mean())
+
+    def control_flow_rate(self, dt):
+        """Shift u on the fluid faces to the target bulk velocity and add the
+        shift / dt to the driving force; with obstacles the shifted field is
+        projected again with dt = 1, keeping p (solver.py:410-425)."""
+        du = self.flow_rate - self.bulk_velocity()
+        self.u[self.u_fluid] += du
+        if self.periodic:
+            self.u[0, :] = self.u[self.grid.nx, :]
+        self.flow_force += du / dt
+        if (~self.grid.interior).any():
+            self.F[:] = self.u
+            self.G[:] = self.v
+            self.boundary.extrapolate_outflow_predictor(self.F, self.G)
+            keep = self.p.copy()
+            self.solve_pressure_and_correct(1.0)
+            self.p[:] = keep
+        self.apply_bc()
 
     def run(self, t_end, max_steps=None, steady_tol=None):
         reason = "t_end"

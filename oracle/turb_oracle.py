@@ -14,6 +14,7 @@ variable-viscosity path (cfd2d/turbulence.py) on top of ``cfd_oracle.OSim``:
 ``OSmagorinsky``               turbulence.py:130-162
 ``OSST``                       turbulence.py:168-325 (Menter 2003, low-Re)
 ``backward_facing_step``       cases.py:133-155
+``turbulent_channel``          cases.py:258-288
 ``add_turbulence``             cases.py:326-337
 =============================  ==============================================
 
@@ -277,6 +278,29 @@ def backward_facing_step(re=400.0, length=15.0, ny=40, nx=None, inlet_length=1.0
     bcs = {"left": OBC("inflow", 1.0, "parabolic"), "right": OBC("outflow"),
            "bottom": OBC("noslip"), "top": OBC("noslip")}
     return OSim(g, bcs, nu=1.0 * H / re, **num)
+
+
+def dean_cf(re_b):
+    """Dean (1978): C_f = 0.073 Re_b^(-1/4) (cases.py:258-260)."""
+    return 0.073 * re_b ** -0.25
+
+
+def turbulent_channel(re_tau=180.0, ny=None, y_plus_1=1.0, model="kOmegaSST", **num):
+    """Periodic channel of half height 1 at bulk velocity 1, Re_b from Dean's
+    Re_tau = 0.09 Re_b^0.88, ny from the first cell's y+ (cases.py:263-288).
+    Returns (sim, re_b)."""
+    re_b = (re_tau / 0.09) ** (1 / 0.88)
+    if ny is None:
+        ny = int(np.ceil(re_tau / y_plus_1))
+        ny += ny % 2
+    g = OGrid(4, ny, 0.5, 2.0)
+    bcs = {"left": OBC("periodic"), "right": OBC("periodic"), "bottom": OBC("noslip"),
+           "top": OBC("noslip")}
+    models = ([OSST(intensity=0.05, viscosity_ratio=50.0)] if model == "kOmegaSST" else
+              [OSmagorinsky()] if model == "Smagorinsky" else [])
+    sim = OSim(g, bcs, nu=2.0 / re_b, flow_rate=1.0, initial_velocity=(1.0, 0.0), models=models,
+               **num)
+    return sim, re_b
 
 
 def add_turbulence(sim: OSim, model="kOmegaSST", U_ref=1.0, **kw) -> OSim:

@@ -39,6 +39,10 @@ Fixtures (all seeds use ``numpy.random.default_rng``):
   ``nu_t`` after steps 1 and 30); k-omega SST on the backward-facing step
   (Re=2000, length 6, ny 16, CG, 20 steps: wall distance, wall omega, k,
   omega and ``nu_t`` after the attach and after steps 1 and 20).
+* ``flowrate.npz``      flow-rate control (solver.py:405-425): the SST
+  turbulent channel (Re_tau 120, first cell at y+ 2, 40 steps) and a periodic
+  channel with a circle (32x32, nu 0.02, 30 steps, re-projection), fields,
+  flow_force and histories.
 """
 
 from __future__ import annotations
@@ -317,6 +321,40 @@ def turbulence(R) -> dict:
     return out
 
 
+def flowrate(R) -> dict:
+    cases, Grid, BC, Simulation = R["cases"], R["Grid"], R["BC"], R["Simulation"]
+    out = {}
+    case = cases.turbulent_channel(re_tau=120, y_plus_1=2.0, t_end=250, pressure_solver="cg")
+    sim = case.sim
+    sst = sim.model("turbulence")
+    out["tc_nut_0"] = sim.nu_t.copy()
+    forces = []
+    for n in range(1, 41):
+        sim.step()
+        forces.append(sim.flow_force)
+        if n in (1, 40):
+            out.update({f"tc_u_{n}": sim.u.copy(), f"tc_v_{n}": sim.v.copy(),
+                        f"tc_p_{n}": sim.p.copy(), f"tc_nut_{n}": sim.nu_t.copy(),
+                        f"tc_k_{n}": sst.k.phi.copy(), f"tc_w_{n}": sst.w.phi.copy()})
+    out["tc_flow_force"] = np.asarray(forces)
+    out.update({"tc_" + k: v for k, v in _hist(sim).items()})
+
+    import cfd2d.geometry as geometry
+
+    g = Grid(32, 32, 1.0, 1.0)
+    g.add_obstacle(geometry.circle(g, 0.5, 0.5, 0.15))
+    bcs = dict(left=BC("periodic"), right=BC("periodic"), bottom=BC("slip"), top=BC("slip"))
+    sim = Simulation(g, bcs, nu=0.02, flow_rate=1.0, pressure_solver="cg")
+    forces = []
+    for n in range(1, 31):
+        sim.step()
+        forces.append(sim.flow_force)
+    out.update({"po_u": sim.u.copy(), "po_v": sim.v.copy(), "po_p": sim.p.copy(),
+                "po_flow_force": np.asarray(forces), "po_fluid": sim.grid.fluid.copy()})
+    out.update({"po_" + k: v for k, v in _hist(sim).items()})
+    return out
+
+
 def main() -> None:
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     R = _import_reference(root)
@@ -330,7 +368,8 @@ def main() -> None:
             ("euler16", lambda: euler16(R)),
             ("autogamma32", lambda: autogamma(R)),
             ("cavity64_c1", lambda: cavity_c1(R)),
-            ("turbulence", lambda: turbulence(R))]
+            ("turbulence", lambda: turbulence(R)),
+            ("flowrate", lambda: flowrate(R))]
     only = set(sys.argv[2:])
     for name, fn in jobs:
         if only and name not in only:

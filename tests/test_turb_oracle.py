@@ -76,3 +76,56 @@ def test_sst_backward_facing_step(gold):
     assert np.array_equal(sim.hist["dt"], gold["sst_h_dt"])
     assert np.array_equal(sim.extra["k_mittel"], gold["sst_hx_k_mittel"])
     assert m.k.phi[1:-1, 1:-1][sim.grid.interior].min() >= 0
+
+
+@pytest.fixture(scope="module")
+def flow():
+    return np.load(GOLDEN / "flowrate.npz")
+
+
+def test_sst_turbulent_channel_with_flow_control(flow):
+    sim, _ = T.turbulent_channel(re_tau=120, y_plus_1=2.0, pressure_solver="cg")
+    m = sim.models[-1]
+    assert np.array_equal(sim.nu_t, flow["tc_nut_0"])
+    forces = []
+    for n in range(1, 41):
+        sim.step()
+        forces.append(sim.flow_force)
+        if n in (1, 40):
+            for k, a in (("u", sim.u), ("v", sim.v), ("p", sim.p), ("nut", sim.nu_t),
+                         ("k", m.k.phi), ("w", m.w.phi)):
+                assert np.array_equal(a, flow[f"tc_{k}_{n}"]), (n, k)
+    assert np.array_equal(forces, flow["tc_flow_force"])
+    assert np.array_equal(sim.hist["residual"], flow["tc_h_residual"])
+    assert np.array_equal(sim.extra["antriebskraft"], flow["tc_hx_antriebskraft"])
+
+
+def test_periodic_obstacle_flow_control_reprojects(flow):
+    g = O.OGrid(32, 32, 1.0, 1.0)
+    g.add_obstacle(g.circle(0.5, 0.5, 0.15))
+    bcs = dict(left=O.OBC("periodic"), right=O.OBC("periodic"), bottom=O.OBC("slip"),
+               top=O.OBC("slip"))
+    sim = O.OSim(g, bcs, nu=0.02, flow_rate=1.0, pressure_solver="cg")
+    forces = []
+    for _ in range(30):
+        sim.step()
+        forces.append(sim.flow_force)
+    for k, a in (("u", sim.u), ("v", sim.v), ("p", sim.p)):
+        assert np.array_equal(a, flow[f"po_{k}"]), k
+    assert np.array_equal(forces, flow["po_flow_force"])
+    assert sim.max_divergence() < 1e-10 and sim.flow_force > 0
+
+
+def test_flow_control_reaches_poiseuille():
+    """Closed form (the reference's tests/test_models.py:17-25): the
+    controlled laminar channel settles on u = 6 U y (1 - y) with the driving
+    force dp/dx = 12 nu U / H^2."""
+    g = O.OGrid(8, 20, 1.0, 1.0)
+    bcs = dict(left=O.OBC("periodic"), right=O.OBC("periodic"), bottom=O.OBC("noslip"),
+               top=O.OBC("noslip"))
+    sim = O.OSim(g, bcs, nu=0.05, flow_rate=1.0)
+    sim.run(20, steady_tol=1e-7)
+    uc, _ = sim.cell_velocity()
+    assert np.abs(uc[3] - 6 * g.yc * (1 - g.yc)).max() < 0.01
+    assert sim.flow_force == pytest.approx(12 * 0.05, rel=0.01)
+    assert sim.bulk_velocity() == pytest.approx(1.0, abs=1e-10)
