@@ -26,6 +26,7 @@ BC_KINDS = {"noslip": 0, "slip": 1, "inflow": 2, "outflow": 3, "periodic": 4}
 FIELDS = {"u": 0, "v": 1, "p": 2, "F": 3, "G": 4, "hu": 5, "hv": 6, "hu_old": 7, "hv_old": 8,
           "x": 9, "nut": 10, "diff_k": 11, "diff_omega": 12}
 REC_NUT_MAX = 30     # record slot of the step's largest nu_t (10 + 5 * 4 scalar sums before it)
+REC_FLOW_FORCE = 31  # record slot of the driving force after the step
 SOLVERS = {"pcg64": 0, "mgcg": 1, "learned": 2}
 LEARNED_MODES = {"net": 0, "none": 1, "teacher": 2}
 
@@ -53,7 +54,8 @@ class StepParams(ctypes.Structure):
                 ("gamma_fixed", _d), ("dt_fixed", _d), ("dt_model", _d), ("t_end", _d),
                 ("clamp_t_end", _i32), ("solver", _i32), ("fx", _d), ("fy", _d), ("rtol", _d),
                 ("maxiter", _i32), ("check_t_end", _i32), ("reliable_delta", _d), ("steady_tol", _d),
-                ("max_steps", _i64)]
+                ("max_steps", _i64), ("flow_control", _i32), ("pad_flow", _i32),
+                ("flow_rate", _d)]
 
 
 class MGLevelDesc(ctypes.Structure):

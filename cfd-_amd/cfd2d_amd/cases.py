@@ -181,6 +181,41 @@ def rayleigh_benard(ra: float = 1e4, pr: float = 0.71, aspect: float = 4.032, ny
                 1.0, float(np.sqrt(ra / pr)), info)
 
 
+def dean_cf(re_b: float) -> float:
+    """Friction coefficient after Dean (1978), C_f = 0.073 Re_b^(-1/4) with
+    Re_b = U_b 2h / nu (cases.py:258-260)."""
+    return 0.073 * re_b ** -0.25
+
+
+def turbulent_channel(re_tau: float = 180.0, ny: int | None = None, y_plus_1: float = 1.0,
+                      model: str = "kOmegaSST", t_end: float = 400.0,
+                      steady_tol: float | None = 1e-6, **num) -> Case:
+    """Periodic channel of half height h = 1 (walls at y = 0 and 2) at bulk
+    velocity 1 held by the flow-rate control; Re_b from Re_tau by Dean's
+    Re_tau = 0.09 Re_b^0.88, ny such that the first cell sits at y+ ~ y_plus_1
+    (cases.py:263-288)."""
+    from .turbulence import TURBULENCE_MODELS
+
+    re_b = (re_tau / 0.09) ** (1 / 0.88)
+    nu = 2.0 / re_b
+    if ny is None:
+        ny = int(np.ceil(re_tau / y_plus_1))
+        ny += ny % 2
+    grid = Grid(4, ny, 0.5, 2.0)
+    bcs = {"left": BC("periodic"), "right": BC("periodic"), "bottom": BC("noslip"),
+           "top": BC("noslip")}
+    cls = TURBULENCE_MODELS[model]
+    if model == "kOmegaSST":
+        models = [cls(intensity=0.05, viscosity_ratio=50.0)]
+    else:
+        models = [cls()] if cls else []
+    sim = Simulation(grid, bcs, nu=nu, flow_rate=1.0, initial_velocity=(1.0, 0.0),
+                     models=models, **_numerics(num))
+    info = {"re_tau_soll": re_tau, "re_b": re_b, "cf_dean": dean_cf(re_b), "modell": model}
+    return Case("turbulent_channel", "Turbulente Kanalströmung (RANS)", sim, t_end, steady_tol,
+                1.0, 2.0, re_b, info)
+
+
 def _unsupported(name):
     def f(*a, **k):
         raise ValueError(f"case {name!r} needs physics outside the GPU projection step")
@@ -245,7 +280,7 @@ CASES = {
     "custom": custom_obstacles,
     "heated_cavity": heated_cavity,
     "rayleigh_benard": rayleigh_benard,
-    "turbulent_channel": _unsupported("turbulent_channel"),
+    "turbulent_channel": turbulent_channel,
     "dam_break": _unsupported("dam_break"),
 }
 

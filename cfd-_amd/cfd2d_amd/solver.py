@@ -82,8 +82,6 @@ class Simulation:
             raise ValueError("tau muss in (0, 1] liegen.")
         if gamma != "auto" and not (0.0 <= float(gamma) <= 1.0):
             raise ValueError("gamma muss in [0, 1] liegen oder 'auto' sein.")
-        if flow_rate is not None:
-            raise ValueError("flow_rate control (periodic channels) is not part of the GPU step")
 
         self.removed_cells = grid.finalize()
         self.grid = grid
@@ -111,13 +109,16 @@ class Simulation:
         self.variable_viscosity = False
         self.variable_density = False
         self.body_force = (float(body_force[0]), float(body_force[1]))
-        self.flow_rate = None
+        self.flow_rate = None if flow_rate is None else float(flow_rate)
         self.flow_force = 0.0
         self.scalars: dict = {}
         self.function_objects: list = []
         self._fo_key = None
         self._fo_host: list = []
         self._probe_set = None
+        self._recorded = 0
+        self._since_sync = 0
+        self._pending = False
         self.models = []
         for m in models or []:
             self.add_model(m)
@@ -144,9 +145,6 @@ class Simulation:
         self.cg_iterations: list[int] = []
         self.cg_info: list[int] = []      # per step: 0 converged, >0 iterations without, <0 breakdown
         self.cg_relres: list[float] = []
-        self._recorded = 0
-        self._since_sync = 0
-        self._pending = False
 
         self._apply_bc()
         self._project_initial()
@@ -270,6 +268,9 @@ class Simulation:
         p.rtol = self.pressure.rtol
         p.maxiter = self.pressure.maxiter
         p.reliable_delta = 0.1
+        if self.flow_rate is not None:
+            p.flow_control = 1
+            p.flow_rate = self.flow_rate
         if run:
             p.check_t_end = 1
             p.t_end = float(t_end)
@@ -325,6 +326,7 @@ class Simulation:
         if self._probe_set is not None:
             self._probe_set.collect(self)
         self.t, self.n, self.dt = float(st.t), int(st.n), float(st.dt)
+        self.flow_force = float(st.flow_force)
         self.dt_old = float(st.dt_old) if st.n > 0 else None
         self.gamma = float(st.gamma)
         if recs:
@@ -360,6 +362,8 @@ class Simulation:
             fx, fy = self._forces_from(r)
             h.fx.append(fx)
             h.fy.append(fy)
+        if self.flow_rate is not None:
+            h.extra.setdefault("antriebskraft", []).append(float(r.mon[_lib.REC_FLOW_FORCE]))
         for m in self.models:
             for k, val in (m.record_monitors(self, r) or {}).items():
                 h.extra.setdefault(k, []).append(val)
@@ -472,6 +476,11 @@ class Simulation:
     # ------------------------------------------------------------------ #
     # evaluation on host copies (solver.py:517-615)
     # ------------------------------------------------------------------ #
+    def bulk_velocity(self) -> float:
+        """Mean u over the fluid faces 1..nx (solver.py:405-408)."""
+        nx = self.grid.nx
+        return float(self.u[1:nx + 1, 1:-1][self.u_fluid[1:nx + 1, 1:-1]].mean())
+
     def divergence(self) -> np.ndarray:
         g = self.grid
         u, v = self.u, self.v

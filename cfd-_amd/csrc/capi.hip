@@ -19,6 +19,7 @@
 #include "ens.hip"
 #include "monitors.hip"
 #include "turb.hip"
+#include "flow.hip"
 
 namespace {
 
@@ -99,6 +100,13 @@ struct cfd_ctx {
     } favg;
     long long* mon_last_n = nullptr;
     double* derived = nullptr;   // (nx+1) x (ny+1) node field for reports
+    // flow-rate control (flow.hip): start-of-step velocity, kept pressure
+    struct {
+        double *u0 = nullptr, *v0 = nullptr, *pkeep = nullptr;
+        unsigned* ticket = nullptr;
+        unsigned long long* acc = nullptr;
+        double nfaces = 0.0;
+    } flow;
     // turbulence model (turb.hip); cell arrays (nx+2) x P
     struct {
         int model = TURB_NONE;
@@ -433,6 +441,46 @@ int launch_sst_post(cfd_ctx* c, int par) {
                        (const double*)c->turb.wwall, c->geo, (const DevState*)c->st);
     HIPC(hipGetLastError());
     hipLaunchKernelGGL(k_fill_ghosts, grid2d(c), dim3(BS2D), 0, c->stream, s.w, c->geo);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+// flow-rate control after the correction and the boundary conditions
+// (solver.py:372-373, 410-425); u0/v0 hold the velocity of the step's start
+int launch_flow_control(cfd_ctx* c, const cfd_step_params* prm) {
+    FlowArgs a;
+    a.u = c->u;
+    a.v = c->v;
+    a.fl = c->fl;
+    a.st = c->st;
+    a.flow_rate = prm->flow_rate;
+    a.nfaces = c->flow.nfaces;
+    a.part = c->part;
+    a.ticket = c->flow.ticket;
+    a.acc = c->flow.acc;
+    a.u0 = c->flow.u0;
+    a.v0 = c->flow.v0;
+    hipLaunchKernelGGL(k_flow_bulk, dim3(c->red_blocks), dim3(256), 0, c->stream, a, c->geo);
+    HIPC(hipGetLastError());
+    hipLaunchKernelGGL(k_flow_shift, grid2d(c), dim3(BS2D), 0, c->stream, a, c->geo);
+    HIPC(hipGetLastError());
+    if (c->has_solid) {
+        // the shifted field is not divergence-free around obstacles: project
+        // it again with dt = 1 and keep the step's pressure
+        PredArgs pa = pred_args(c, prm, 0);
+        hipLaunchKernelGGL(k_copy_predictor, grid2d(c), dim3(BS2D), 0, c->stream, pa, c->geo);
+        HIPC(hipGetLastError());
+        HIPC(hipMemcpyAsync(c->flow.pkeep, c->p, cells(c) * sizeof(double),
+                            hipMemcpyDeviceToDevice, c->stream));
+        ProjArgs pj = proj_args(c, prm->solver, prm->rtol, prm->maxiter);
+        pj.dt_override = 1.0;
+        TRY(launch_project(c, pj));
+        HIPC(hipMemcpyAsync(c->p, c->flow.pkeep, cells(c) * sizeof(double),
+                            hipMemcpyDeviceToDevice, c->stream));
+        c->l_hist = 0;   // the solves no longer form one time series
+    }
+    TRY(launch_bc(c, true));
+    hipLaunchKernelGGL(k_flow_residual, dim3(c->red_blocks), dim3(256), 0, c->stream, a, c->geo);
     HIPC(hipGetLastError());
     return 0;
 }
@@ -817,6 +865,10 @@ int cfd_ctx_create(const cfd_grid_desc* d, cfd_ctx** out) {
             if (cm > d->ncomp) return fail(set_err(CFD_ERR_ARG, "component label %d > ncomp", cm));
             comph[(size_t)(i + 1) * P + j + 1] = (uint8_t)cm;
         }
+    c->flow.nfaces = 0.0;
+    for (int i = 1; i <= nx; ++i)
+        for (int j = 1; j <= ny; ++j)
+            if (flh[(size_t)i * P + j] && flh[(size_t)(i + 1) * P + j]) c->flow.nfaces += 1.0;
     c->has_solid = 0;
     for (int i = 1; i <= nx && !c->has_solid; ++i)
         for (int j = 1; j <= ny; ++j)
@@ -1290,6 +1342,17 @@ int cfd_step(cfd_ctx* c, const cfd_step_params* prm, int32_t count) {
     sp.clamp_t_end = prm->clamp_t_end;
     sp.fx = prm->fx;
     sp.fy = prm->fy;
+    if (prm->flow_control) {
+        if (!(c->flow.nfaces > 0.0)) return set_err(CFD_ERR_ARG, "no fluid faces for flow control");
+        if (!c->flow.u0) {
+            const size_t N = cells(c);
+            TRY(dalloc(c, &c->flow.u0, N));
+            TRY(dalloc(c, &c->flow.v0, N));
+            TRY(dalloc(c, &c->flow.pkeep, N));
+            TRY(dalloc(c, &c->flow.ticket, 2));
+            TRY(dalloc(c, &c->flow.acc, 1));
+        }
+    }
     HIPC(hipEventRecord(c->ev0, c->stream));
     for (int k = 0; k < count; ++k) {
         // steps that the device skips (run finished) keep the parity: the host
@@ -1306,6 +1369,12 @@ int cfd_step(cfd_ctx* c, const cfd_step_params* prm, int32_t count) {
                 td.dw = c->turb.dw;
             }
         }
+        if (prm->flow_control) {
+            HIPC(hipMemcpyAsync(c->flow.u0, c->u, cells(c) * sizeof(double),
+                                hipMemcpyDeviceToDevice, c->stream));
+            HIPC(hipMemcpyAsync(c->flow.v0, c->v, cells(c) * sizeof(double),
+                                hipMemcpyDeviceToDevice, c->stream));
+        }
         hipLaunchKernelGGL(k_dt_gamma, dim3(c->red_blocks), dim3(256), 0,
                            c->stream, c->u, c->v, c->geo, sp, td, c->st, c->acc, c->ticket);
         HIPC(hipGetLastError());
@@ -1314,6 +1383,7 @@ int cfd_step(cfd_ctx* c, const cfd_step_params* prm, int32_t count) {
         HIPC(hipGetLastError());
         TRY(launch_project(c, proj_args(c, prm->solver, prm->rtol, prm->maxiter)));
         TRY(launch_bc(c, true));
+        if (prm->flow_control) TRY(launch_flow_control(c, prm));
         for (int si = 0; si < (int)c->scalars.size(); ++si) {
             if (si == c->turb.sid_k || si == c->turb.sid_w) continue;   // SST below
             ScalarArgs a = scalar_args(c, c->scalars[si], pk);

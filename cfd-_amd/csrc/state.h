@@ -19,6 +19,7 @@ struct DevState {
     int done;              // run finished (t_end, max_steps, steady_tol, error): later steps are no-ops
     int error;             // instability detected (solver.py:395-399)
     double nut_max;        // max of the turbulent viscosity this step (0 when laminar)
+    double flow_du;        // flow-rate control: this step's velocity shift
 };
 
 #define CFD_MAX_MON 32

@@ -768,7 +768,7 @@ __global__ void k_fill_ghosts(ScalarArgs a, Geo g) {
 // `part`; the last block adds them in block order (deterministic), writes the
 // record and advances t, n, dt_old.
 // ---------------------------------------------------------------------------
-static_assert(REC_NV < CFD_MAX_MON, "the record holds the sums and nu_t's maximum");
+static_assert(REC_NV + 1 < CFD_MAX_MON, "the record holds the sums, max nu_t and the force");
 __global__ void k_record(RecordArgs a, Geo g) {
     __shared__ double lds[REC_NV * 16];
     const int nx = g.nx, ny = g.ny, pc = g.pc;
@@ -911,6 +911,7 @@ __global__ void k_record(RecordArgs a, Geo g) {
         r.nmon = 10 + 5 * a.nscalar;
         for (int q = 0; q < CFD_MAX_MON; ++q) r.mon[q] = q < REC_NV ? tot[q] : 0.0;
         r.mon[REC_NV] = st->nut_max;   // turbulence.py:158-159, 322-324
+        r.mon[REC_NV + 1] = st->flow_force;   // solver.py:439-440
         const double res = r.residual;
         if (!(res == res) || res > 1e8) {
             st->error = 1;          // solver.py:395-399: no record for this step

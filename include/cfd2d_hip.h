@@ -107,6 +107,9 @@ typedef struct cfd_step_params {
     double reliable_delta;       /* mixed solver: residual-replacement trigger factor */
     double steady_tol;           /* run(): stop once n > 10 and residual < steady_tol (0: off) */
     int64_t max_steps;           /* run(): stop at this step count (0: off) */
+    int32_t flow_control;        /* 1: hold the bulk velocity at flow_rate (solver.py:405-425) */
+    int32_t pad_flow;
+    double flow_rate;
 } cfd_step_params;
 
 /* Scalars of a running simulation (Simulation attributes t, n, dt, dt_old,
@@ -127,7 +130,8 @@ typedef struct cfd_state {
    over the east / west ones (obstacle forces, solver.py:542-567), then per
    scalar s (5 values from index 10+5s, s < 4): sum of phi over fluid cells and
    the wall-flux sums of the left, right, bottom, top sides (models.py:228-250);
-   [30] the largest nu_t of the step (turbulence models, else 0). */
+   [30] the largest nu_t of the step (turbulence models, else 0), [31] the
+   driving force after the step (flow_force). */
 typedef struct cfd_record {
     double t, dt, gamma, residual, divergence, kinetic_energy;
     double cg_relres;
