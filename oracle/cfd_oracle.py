@@ -73,7 +73,8 @@ def lib():
         i = ctypes.c_int
         vp = ctypes.c_void_p
         L.oracle_momentum_tendencies.argtypes = [vp, vp, vp, i, i, d, d, d, d, i, vp, vp]
-        L.oracle_momentum_tendencies_var.argtypes = [vp, vp, vp, i, i, d, d, vp, d, i, vp, vp]
+        L.oracle_momentum_tendencies_var.argtypes = [vp, vp, vp, i, i, d, d, vp, d, i, vp, vp, vp]
+        L.oracle_vof_advect.argtypes = [vp, vp, vp, vp, i, i, d, d, d, d, i, d, vp]
         L.oracle_obstacle_bc.argtypes = [vp, vp, vp, i, i, i]
         L.oracle_scalar_transport.argtypes = ([vp, vp, vp, vp, i, i, d, d, vp, i, i]
                                               + [vp] * 8 + [i, d, vp])
@@ -104,17 +105,33 @@ def momentum_tendencies(u, v, fluid, nx, ny, dx, dy, nu0, gamma, periodic, hu, h
                                      int(bool(periodic)), _p(hu), _p(hv))
 
 
-def momentum_tendencies_var(u, v, fluid, nx, ny, dx, dy, nu, gamma, periodic, hu, hv) -> None:
+def momentum_tendencies_var(u, v, fluid, nx, ny, dx, dy, nu, gamma, periodic, hu, hv,
+                            rho=None) -> None:
     """kernels.py:44-155 with ``variable = True``: stress-form diffusion of the
-    cell viscosity field ``nu`` (nu_eff, (nx+2, ny+2))."""
+    cell viscosity field ``nu`` (nu_eff, (nx+2, ny+2)); with ``rho`` (variable
+    density) ``nu`` is the dynamic viscosity and the diffusion is divided by
+    the face density."""
     fl = np.ascontiguousarray(fluid, dtype=np.uint8)
     _chk(u, (nx + 1, ny + 2), np.float64)
     _chk(v, (nx + 2, ny + 1), np.float64)
     _chk(nu, (nx + 2, ny + 2), np.float64)
     _chk(hu, (nx + 1, ny + 2), np.float64)
     _chk(hv, (nx + 2, ny + 1), np.float64)
+    if rho is not None:
+        _chk(rho, (nx + 2, ny + 2), np.float64)
     lib().oracle_momentum_tendencies_var(_p(u), _p(v), _p(fl), nx, ny, dx, dy, _p(nu), gamma,
-                                         int(bool(periodic)), _p(hu), _p(hv))
+                                         int(bool(periodic)), _p(hu), _p(hv),
+                                         None if rho is None else _p(rho))
+
+
+def vof_advect(a, u, v, fluid, nx, ny, dx, dy, dt, c_alpha, periodic, a_in, out) -> None:
+    """vof.py:34-133, one explicit alpha step into ``out``."""
+    fl = np.ascontiguousarray(fluid, dtype=np.uint8)
+    for arr, shape in ((a, (nx + 2, ny + 2)), (out, (nx + 2, ny + 2)), (u, (nx + 1, ny + 2)),
+                       (v, (nx + 2, ny + 1))):
+        _chk(arr, shape, np.float64)
+    lib().oracle_vof_advect(_p(a), _p(u), _p(v), _p(fl), nx, ny, dx, dy, dt, c_alpha,
+                            int(bool(periodic)), a_in, _p(out))
 
 
 def obstacle_bc(u, v, fluid, nx, ny, periodic) -> None:
@@ -335,6 +352,7 @@ class OPressure:
                  maxiter=5000):
         nx, ny = grid.nx, grid.ny
         ax, ay = 1.0 / grid.dx ** 2, 1.0 / grid.dy ** 2
+        self._ax, self._ay = ax, ay
         fl = grid.interior
         self.fluid, self.method, self.tol, self.maxiter = fl, method, tol, maxiter
         n = int(fl.sum())
@@ -395,6 +413,8 @@ class OPressure:
             self.pinned.append((int(pi), int(pj)))
         d = diag[fl]
         self.diag = d
+        self._pairs = (px, py, (fl[-1, :] & fl[0, :]) if boundary.periodic else None)
+        self._rc = (np.concatenate(R + [np.arange(n)]), np.concatenate(C + [np.arange(n)]))
         A = sp.csc_matrix((np.concatenate(V + [d]),
                            (np.concatenate(R + [np.arange(n)]), np.concatenate(C + [np.arange(n)]))),
                           shape=(n, n))
@@ -432,6 +452,52 @@ class OPressure:
         p[self.fluid] = x
         for c in self.neumann:
             m = self.labels == c
+            p[m] -= p[m].mean()
+        return p
+
+
+    def solve_variable(self, rhs, bx, by):
+        """div(b grad p) = rhs with face coefficients b = 1/rho: bx (nx+1, ny)
+        on the x faces, by (nx, ny+1); assembled anew and solved directly
+        (pressure.py:187-244)."""
+        fl = self.fluid
+        nx, ny = fl.shape
+        dx2, dy2 = self._ax, self._ay
+        px, py, pp = self._pairs
+        diag = np.zeros((nx, ny))
+        vals = []
+        c = (bx[1:nx, :] * dx2)[px]
+        vals += [-c, -c]
+        diag[:-1, :][px] += c
+        diag[1:, :][px] += c
+        c = (by[:, 1:ny] * dy2)[py]
+        vals += [-c, -c]
+        diag[:, :-1][py] += c
+        diag[:, 1:][py] += c
+        if pp is not None:
+            c = (bx[nx, :] * dx2)[pp]
+            vals += [-c, -c]
+            diag[-1, :][pp] += c
+            diag[0, :][pp] += c
+        d = self.dirichlet
+        diag[0, :][d["left"]] += 2 * dx2 * bx[0, :][d["left"]]
+        diag[-1, :][d["right"]] += 2 * dx2 * bx[nx, :][d["right"]]
+        diag[:, 0][d["bottom"]] += 2 * dy2 * by[:, 0][d["bottom"]]
+        diag[:, -1][d["top"]] += 2 * dy2 * by[:, ny][d["top"]]
+        for fi, fj in self.pinned:
+            diag[fi, fj] += dx2 * max(float(bx.max()), 1e-30)
+        A = sp.csc_matrix((np.concatenate(vals + [diag[fl]]), self._rc), shape=(self.n, self.n))
+        b = rhs[fl].astype(float)
+        if self.neumann:
+            lab = self.labels[fl]
+            for comp in self.neumann:
+                m = lab == comp
+                b[m] -= b[m].mean()
+        x = spla.spsolve(A, -b)
+        p = np.zeros(fl.shape)
+        p[fl] = x
+        for comp in self.neumann:
+            m = self.labels == comp
             p[m] -= p[m].mean()
         return p
 
@@ -685,6 +751,8 @@ class OSim:
         self.nu_t = np.zeros((nx + 2, ny + 2))
         self.nu_eff = np.full((nx + 2, ny + 2), self.nu)
         self.variable_viscosity = False
+        self.variable_density = False
+        self.rho = np.ones((nx + 2, ny + 2))
         self.scalars, self.models = {}, []
         for m in models or []:
             m.attach(self)
@@ -728,7 +796,10 @@ class OSim:
         g = self.grid
         umax = max(np.abs(self.u).max(), 1e-12)
         vmax = max(np.abs(self.v).max(), 1e-12)
-        nu_max = float(self.nu_eff.max()) if self.variable_viscosity else self.nu
+        if self.variable_density:
+            nu_max = float((self.nu_eff / self.rho)[self.grid.fluid].max())
+        else:
+            nu_max = float(self.nu_eff.max()) if self.variable_viscosity else self.nu
         dt_visc = 0.5 / (nu_max * (1.0 / g.dx ** 2 + 1.0 / g.dy ** 2))
         dt_conv = min(g.dx / umax, g.dy / vmax)
         if self.scheme == "ab2":
@@ -748,7 +819,10 @@ class OSim:
         if self.scheme == "euler":
             cfl = max(umax * dt / g.dx, vmax * dt / g.dy)
             return float(min(1.0, max(0.05, 1.2 * cfl)))
-        nu_ref = float(self.nu_eff[g.fluid].min()) if self.variable_viscosity else self.nu
+        if self.variable_density:
+            nu_ref = float((self.nu_eff / self.rho)[g.fluid].min())
+        else:
+            nu_ref = float(self.nu_eff[g.fluid].min()) if self.variable_viscosity else self.nu
         re_cell = max(umax * g.dx, vmax * g.dy) / nu_ref
         return float(min(1.0, max(0.05, 0.015 * re_cell)))
 
@@ -759,7 +833,22 @@ class OSim:
         if self.periodic:
             F[0, :] = F[nx, :]
         div = (F[1:, 1:-1] - F[:-1, 1:-1]) / dx + (G[1:-1, 1:] - G[1:-1, :-1]) / dy
-        pin = self.pressure.solve(div / dt)
+        if self.variable_density:     # face coefficients 1/rho (solver.py:267-273)
+            bx = 2.0 / (self.rho[:-1, :] + self.rho[1:, :])
+            by = 2.0 / (self.rho[:, :-1] + self.rho[:, 1:])
+            if self.periodic:
+                bx[nx, :] = 2.0 / (self.rho[nx, :] + self.rho[1, :])
+            pin = self.pressure.solve_variable(div / dt, bx[:, 1:-1], by[1:-1, :])
+        else:
+            bx = by = None
+            pin = self.pressure.solve(div / dt)
+
+        def cx(sl):
+            return 1.0 if bx is None else bx[sl]
+
+        def cy(sl):
+            return 1.0 if by is None else by[sl]
+
         p = self.p
         p[:] = 0.0
         p[1:-1, 1:-1] = pin
@@ -767,26 +856,33 @@ class OSim:
             p[0, :] = p[nx, :]
             p[nx + 1, :] = p[1, :]
         uf = self.u_fluid[1:nx, 1:ny + 1]
-        corr = F[1:nx, 1:ny + 1] - dt * (p[2:nx + 1, 1:ny + 1] - p[1:nx, 1:ny + 1]) / dx
+        corr = F[1:nx, 1:ny + 1] - dt * cx((slice(1, nx), slice(1, ny + 1))) * (
+            p[2:nx + 1, 1:ny + 1] - p[1:nx, 1:ny + 1]) / dx
         self.u[1:nx, 1:ny + 1] = np.where(uf, corr, F[1:nx, 1:ny + 1])
         vf = self.v_fluid[1:nx + 1, 1:ny]
-        corr = G[1:nx + 1, 1:ny] - dt * (p[1:nx + 1, 2:ny + 1] - p[1:nx + 1, 1:ny]) / dy
+        corr = G[1:nx + 1, 1:ny] - dt * cy((slice(1, nx + 1), slice(1, ny))) * (
+            p[1:nx + 1, 2:ny + 1] - p[1:nx + 1, 1:ny]) / dy
         self.v[1:nx + 1, 1:ny] = np.where(vf, corr, G[1:nx + 1, 1:ny])
         if self.periodic:
             uf = self.u_fluid[nx, 1:ny + 1]
-            corr = F[nx, 1:ny + 1] - dt * (p[1, 1:ny + 1] - p[nx, 1:ny + 1]) / dx
+            corr = F[nx, 1:ny + 1] - dt * cx((nx, slice(1, ny + 1))) * (
+                p[1, 1:ny + 1] - p[nx, 1:ny + 1]) / dx
             self.u[nx, 1:ny + 1] = np.where(uf, corr, F[nx, 1:ny + 1])
             self.u[0, :] = self.u[nx, :]
             return
         d = self.pressure.dirichlet
-        self.u[0, 1:ny + 1] = np.where(d["left"], F[0, 1:ny + 1] - 2 * dt * p[1, 1:ny + 1] / dx,
-                                       F[0, 1:ny + 1])
-        self.u[nx, 1:ny + 1] = np.where(d["right"], F[nx, 1:ny + 1] + 2 * dt * p[nx, 1:ny + 1] / dx,
-                                        F[nx, 1:ny + 1])
-        self.v[1:nx + 1, 0] = np.where(d["bottom"], G[1:nx + 1, 0] - 2 * dt * p[1:nx + 1, 1] / dy,
-                                       G[1:nx + 1, 0])
-        self.v[1:nx + 1, ny] = np.where(d["top"], G[1:nx + 1, ny] + 2 * dt * p[1:nx + 1, ny] / dy,
-                                        G[1:nx + 1, ny])
+        self.u[0, 1:ny + 1] = np.where(
+            d["left"], F[0, 1:ny + 1] - 2 * dt * cx((0, slice(1, ny + 1))) * p[1, 1:ny + 1] / dx,
+            F[0, 1:ny + 1])
+        self.u[nx, 1:ny + 1] = np.where(
+            d["right"], F[nx, 1:ny + 1] + 2 * dt * cx((nx, slice(1, ny + 1))) * p[nx, 1:ny + 1] / dx,
+            F[nx, 1:ny + 1])
+        self.v[1:nx + 1, 0] = np.where(
+            d["bottom"], G[1:nx + 1, 0] - 2 * dt * cy((slice(1, nx + 1), 0)) * p[1:nx + 1, 1] / dy,
+            G[1:nx + 1, 0])
+        self.v[1:nx + 1, ny] = np.where(
+            d["top"], G[1:nx + 1, ny] + 2 * dt * cy((slice(1, nx + 1), ny)) * p[1:nx + 1, ny] / dy,
+            G[1:nx + 1, ny])
 
     def step(self):
         g = self.grid
@@ -794,14 +890,15 @@ class OSim:
         for m in self.models:
             if hasattr(m, "pre_momentum"):
                 m.pre_momentum(self)
-        if self.variable_viscosity:
+        if self.variable_viscosity and not self.variable_density:
             np.add(self.nu, self.nu_t, out=self.nu_eff)
         dt = self.compute_dt()
         self.gamma = (self.auto_gamma(dt) if self.gamma_setting == "auto"
                       else float(self.gamma_setting))
         if self.variable_viscosity:
             momentum_tendencies_var(self.u, self.v, g.fluid, g.nx, g.ny, g.dx, g.dy, self.nu_eff,
-                                    self.gamma, self.periodic, self.hu, self.hv)
+                                    self.gamma, self.periodic, self.hu, self.hv,
+                                    self.rho if self.variable_density else None)
         else:
             momentum_tendencies(self.u, self.v, g.fluid, g.nx, g.ny, g.dx, g.dy, self.nu,
                                 self.gamma, self.periodic, self.hu, self.hv)

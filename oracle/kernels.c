@@ -17,7 +17,10 @@
  *                               lines 104-105; no variable density)
  *   oracle_momentum_tendencies_var  the same with the stress-form diffusion of
  *                               a cell viscosity field (kernels.py:96-103,
- *                               131-138)
+ *                               131-138), divided by the face density when
+ *                               rho is given (variable density, lines 104-105
+ *                               and 139-140)
+ *   oracle_vof_advect           vof.py:35-133 (van Leer + interface compression)
  *   oracle_obstacle_bc          kernels.py:158-209
  *   limiter / face_value        kernels.py:215-251
  *   oracle_scalar_transport     kernels.py:254-391
@@ -81,7 +84,8 @@ void oracle_momentum_tendencies(const double *u, const double *v, const uint8_t 
    convection terms are those of the constant branch. */
 void oracle_momentum_tendencies_var(const double *u, const double *v, const uint8_t *fluid,
                                     int nx, int ny, double dx, double dy, const double *nu,
-                                    double gamma, int periodic, double *hu, double *hv)
+                                    double gamma, int periodic, double *hu, double *hv,
+                                    const double *rho)
 {
     const double idx2 = 1.0 / (dx * dx), idy2 = 1.0 / (dy * dy);
     const double idx = 1.0 / dx, idy = 1.0 / dy;
@@ -99,9 +103,10 @@ void oracle_momentum_tendencies_var(const double *u, const double *v, const uint
                                         + C_(nu, i + 1, j + 1));
             const double nu_s = 0.25 * (C_(nu, i, j) + C_(nu, i + 1, j) + C_(nu, i, j - 1)
                                         + C_(nu, i + 1, j - 1));
-            const double diff = (2.0 * nu_e * (ue - uc) - 2.0 * nu_w * (uc - uw)) * idx2
+            double diff = (2.0 * nu_e * (ue - uc) - 2.0 * nu_w * (uc - uw)) * idx2
                 + idy * (nu_n * ((un - uc) * idy + (V_(i + 1, j) - V_(i, j)) * idx)
                          - nu_s * ((uc - us) * idy + (V_(i + 1, j - 1) - V_(i, j - 1)) * idx));
+            if (rho) diff /= 0.5 * (C_(rho, i, j) + C_(rho, i + 1, j));
             const double a_e = 0.5 * (uc + ue), a_w = 0.5 * (uw + uc);
             const double du2dx = idx * (a_e * a_e - a_w * a_w)
                 + gamma * idx * (fabs(a_e) * 0.5 * (uc - ue) - fabs(a_w) * 0.5 * (uw - uc));
@@ -122,9 +127,10 @@ void oracle_momentum_tendencies_var(const double *u, const double *v, const uint
                                         + C_(nu, i + 1, j + 1));
             const double nu_w = 0.25 * (C_(nu, i, j) + C_(nu, i - 1, j) + C_(nu, i, j + 1)
                                         + C_(nu, i - 1, j + 1));
-            const double diff = (2.0 * nu_n * (vn - vc) - 2.0 * nu_s * (vc - vs)) * idy2
+            double diff = (2.0 * nu_n * (vn - vc) - 2.0 * nu_s * (vc - vs)) * idy2
                 + idx * (nu_e * ((ve - vc) * idx + (U_(i, j + 1) - U_(i, j)) * idy)
                          - nu_w * ((vc - vw) * idx + (U_(i - 1, j + 1) - U_(i - 1, j)) * idy));
+            if (rho) diff /= 0.5 * (C_(rho, i, j) + C_(rho, i, j + 1));
             const double b_n = 0.5 * (vc + vn), b_s = 0.5 * (vs + vc);
             const double dv2dy = idy * (b_n * b_n - b_s * b_s)
                 + gamma * idy * (fabs(b_n) * 0.5 * (vc - vn) - fabs(b_s) * 0.5 * (vs - vc));
@@ -291,4 +297,105 @@ void oracle_scalar_transport(const double *phi, const double *u, const double *v
         for (int j = 0; j < ny + 2; ++j)
             if (!C_(fluid, i, j) || i == 0 || i == nx + 1 || j == 0 || j == ny + 1)
                 C_(out, i, j) = 0.0;
+}
+
+/* vof.py:22-31: van Leer face value from the upwind, central and downwind cells */
+static double vl_face(double pu, double pc, double pd, int has_u)
+{
+    if (!has_u) return pc;
+    const double den = pd - pc;
+    if (fabs(den) < 1e-300) return pc;
+    const double r = (pc - pu) / den;
+    const double psi = r <= 0.0 ? 0.0 : 2.0 * r / (1.0 + r);
+    return pc + 0.5 * psi * den;
+}
+
+/* vof.py:34-133: one explicit step of alpha; faces in the reference's loop
+   order, each adding to and taking from its two cells; inflow faces bring
+   a_in; the compression flux u_r a_up (1 - a_down) with u_r = c_alpha |u| n;
+   alpha clipped to [0, 1] on fluid cells at the end. */
+void oracle_vof_advect(const double *a, const double *u, const double *v, const uint8_t *fluid,
+                       int nx, int ny, double dx, double dy, double dt, double c_alpha,
+                       int periodic, double a_in, double *out)
+{
+    for (long k = 0; k < (long)(nx + 2) * (ny + 2); ++k) out[k] = a[k];
+    for (int i = 0; i <= nx; ++i)
+        for (int j = 1; j <= ny; ++j) {
+            const double uf = U_(i, j);
+            if (i == 0 || (i == nx && !periodic)) {
+                if (i == 0 && periodic) continue;
+                const int P = i == 0 ? 1 : nx;
+                if (!C_(fluid, P, j)) continue;
+                const double into = i == 0 ? uf : -uf;
+                const double ab = into > 0 ? a_in : C_(a, P, j);
+                C_(out, P, j) += dt * into * ab / dx;
+                continue;
+            }
+            const int L = i, R = i < nx ? i + 1 : 1;
+            if (!(C_(fluid, L, j) && C_(fluid, R, j))) continue;
+            int Cc, D, Ui;
+            if (uf >= 0.0) {
+                Cc = L; D = R; Ui = L - 1;
+                if (Ui < 1) Ui = periodic ? nx : -1;
+            } else {
+                Cc = R; D = L; Ui = R + 1;
+                if (Ui > nx) Ui = periodic ? 1 : -1;
+            }
+            const int has_u = Ui >= 1 && C_(fluid, Ui, j);
+            const double af = vl_face(has_u ? C_(a, Ui, j) : 0.0, C_(a, Cc, j), C_(a, D, j), has_u);
+            double flux = uf * af;
+            const double gx = (C_(a, R, j) - C_(a, L, j)) / dx;
+            const int jn = j < ny ? j + 1 : j, js = j > 1 ? j - 1 : j;
+            const double gy = 0.25 * ((C_(a, L, jn) - C_(a, L, js)) + (C_(a, R, jn) - C_(a, R, js))) / dy;
+            const double gm = sqrt(gx * gx + gy * gy);
+            if (gm > 1e-12 && c_alpha > 0.0) {
+                const double ur = c_alpha * fabs(uf) * gx / gm;
+                const double ac = ur >= 0.0 ? C_(a, L, j) : C_(a, R, j);
+                const double ad = ur >= 0.0 ? C_(a, R, j) : C_(a, L, j);
+                flux += ur * ac * (1.0 - ad);
+            }
+            C_(out, L, j) -= dt * flux / dx;
+            C_(out, R, j) += dt * flux / dx;
+        }
+    for (int i = 1; i <= nx; ++i)
+        for (int j = 0; j <= ny; ++j) {
+            const double vf = V_(i, j);
+            if (j == 0 || j == ny) {
+                const int P = j == 0 ? 1 : ny;
+                if (!C_(fluid, i, P)) continue;
+                const double into = j == 0 ? vf : -vf;
+                const double ab = into > 0 ? a_in : C_(a, i, P);
+                C_(out, i, P) += dt * into * ab / dy;
+                continue;
+            }
+            const int B = j, T = j + 1;
+            if (!(C_(fluid, i, B) && C_(fluid, i, T))) continue;
+            int Cc, D, Uj;
+            if (vf >= 0.0) { Cc = B; D = T; Uj = B - 1; }
+            else { Cc = T; D = B; Uj = T + 1; }
+            const int has_u = Uj >= 1 && Uj <= ny && C_(fluid, i, Uj);
+            const double af = vl_face(has_u ? C_(a, i, Uj) : 0.0, C_(a, i, Cc), C_(a, i, D), has_u);
+            double flux = vf * af;
+            const double gy = (C_(a, i, T) - C_(a, i, B)) / dy;
+            int ie = (i < nx || periodic) ? i + 1 : i;
+            int iw = (i > 1 || periodic) ? i - 1 : i;
+            if (ie > nx) ie = 1;
+            if (iw < 1) iw = nx;
+            const double gx = 0.25 * ((C_(a, ie, B) - C_(a, iw, B)) + (C_(a, ie, T) - C_(a, iw, T))) / dx;
+            const double gm = sqrt(gx * gx + gy * gy);
+            if (gm > 1e-12 && c_alpha > 0.0) {
+                const double ur = c_alpha * fabs(vf) * gy / gm;
+                const double ac = ur >= 0.0 ? C_(a, i, B) : C_(a, i, T);
+                const double ad = ur >= 0.0 ? C_(a, i, T) : C_(a, i, B);
+                flux += ur * ac * (1.0 - ad);
+            }
+            C_(out, i, B) -= dt * flux / dy;
+            C_(out, i, T) += dt * flux / dy;
+        }
+    for (int i = 1; i <= nx; ++i)
+        for (int j = 1; j <= ny; ++j)
+            if (C_(fluid, i, j)) {
+                if (C_(out, i, j) < 0.0) C_(out, i, j) = 0.0;
+                else if (C_(out, i, j) > 1.0) C_(out, i, j) = 1.0;
+            }
 }

@@ -43,6 +43,10 @@ Fixtures (all seeds use ``numpy.random.default_rng``):
   turbulent channel (Re_tau 120, first cell at y+ 2, 40 steps) and a periodic
   channel with a circle (32x32, nu 0.02, 30 steps, re-projection), fields,
   flow_force and histories.
+* ``vof.npz``           two-phase VOF (vof.py): ``vof_advect`` on random fields
+  (periodic and not, with obstacles, compression on and off), the CSF
+  surface-tension source on a smoothed column, and the dam break (32x32,
+  with the obstacle, 25 steps): alpha, rho, u, v, p and the monitors.
 """
 
 from __future__ import annotations
@@ -355,6 +359,53 @@ def flowrate(R) -> dict:
     return out
 
 
+def vof(R) -> dict:
+    cases, Grid, BC, Simulation = R["cases"], R["Grid"], R["BC"], R["Simulation"]
+    import cfd2d.vof as vofmod
+
+    rng = np.random.default_rng(20241002)
+    out = {}
+    nx, ny = 16, 12
+    dx, dy = 1.0 / nx, 0.75 / ny
+    for periodic in (False, True):
+        tag = "p" if periodic else "np"
+        g = Grid(nx, ny, 1.0, 0.75)
+        m = np.zeros((nx, ny), dtype=bool)
+        m[4:7, 3:6] = True
+        m[10, 2:9] = True
+        g.add_obstacle(m)
+        if periodic:
+            g.set_periodic_x()
+        fluid = g.fluid.copy()
+        a = np.clip(rng.uniform(-0.2, 1.2, (nx + 2, ny + 2)), 0.0, 1.0)
+        u = rng.standard_normal((nx + 1, ny + 2))
+        v = rng.standard_normal((nx + 2, ny + 1))
+        out.update({f"adv_fluid_{tag}": fluid, f"adv_a_{tag}": a, f"adv_u_{tag}": u,
+                    f"adv_v_{tag}": v})
+        for ci, c_alpha in enumerate((0.0, 1.0)):
+            o = np.full_like(a, 5.0)
+            vofmod.vof_advect(a, u, v, fluid, nx, ny, dx, dy, 0.013, c_alpha, periodic, 0.3, o)
+            out[f"adv_out_{tag}_c{ci}"] = o
+
+    case = cases.dam_break(n=32, t_end=1.0)
+    sim = case.sim
+    vm = sim.model("multiphase")
+    out.update({"db_alpha_0": vm.alpha.copy(), "db_rho_0": sim.rho.copy(),
+                "db_mu_0": sim.nu_eff.copy(), "db_fluid": sim.grid.fluid.copy(),
+                "db_u_0": sim.u.copy(), "db_v_0": sim.v.copy()})
+    hu, hv = np.zeros_like(sim.hu), np.zeros_like(sim.hv)
+    vm.momentum_source(sim, hu, hv)
+    out["db_csf_hu_0"], out["db_csf_hv_0"] = hu, hv
+    for n in range(1, 26):
+        sim.step()
+        if n in (1, 5, 25):
+            out.update({f"db_alpha_{n}": vm.alpha.copy(), f"db_rho_{n}": sim.rho.copy(),
+                        f"db_u_{n}": sim.u.copy(), f"db_v_{n}": sim.v.copy(),
+                        f"db_p_{n}": sim.p.copy()})
+    out.update({"db_" + k: v for k, v in _hist(sim).items()})
+    return out
+
+
 def main() -> None:
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     R = _import_reference(root)
@@ -369,7 +420,8 @@ def main() -> None:
             ("autogamma32", lambda: autogamma(R)),
             ("cavity64_c1", lambda: cavity_c1(R)),
             ("turbulence", lambda: turbulence(R)),
-            ("flowrate", lambda: flowrate(R))]
+            ("flowrate", lambda: flowrate(R)),
+            ("vof", lambda: vof(R))]
     only = set(sys.argv[2:])
     for name, fn in jobs:
         if only and name not in only:
