@@ -24,9 +24,10 @@ CFD_ERR_LIMIT = -4
 
 BC_KINDS = {"noslip": 0, "slip": 1, "inflow": 2, "outflow": 3, "periodic": 4}
 FIELDS = {"u": 0, "v": 1, "p": 2, "F": 3, "G": 4, "hu": 5, "hv": 6, "hu_old": 7, "hv_old": 8,
-          "x": 9, "nut": 10, "diff_k": 11, "diff_omega": 12}
+          "x": 9, "nut": 10, "diff_k": 11, "diff_omega": 12, "alpha": 13, "rho": 14, "mu": 15}
 REC_NUT_MAX = 30     # record slot of the step's largest nu_t (10 + 5 * 4 scalar sums before it)
 REC_FLOW_FORCE = 31  # record slot of the driving force after the step
+REC_VOF = 32         # VOF: alpha sum over fluid cells, front cell, column-height cell
 SOLVERS = {"pcg64": 0, "mgcg": 1, "learned": 2}
 LEARNED_MODES = {"net": 0, "none": 1, "teacher": 2}
 
@@ -86,6 +87,12 @@ class TurbDesc(ctypes.Structure):
                 ("omega_wall", _vp)]
 
 
+class VofDesc(ctypes.Structure):
+    _fields_ = [("rho1", _d), ("rho2", _d), ("mu1", _d), ("mu2", _d), ("sigma", _d),
+                ("c_alpha", _d), ("cfl_alpha", _d), ("alpha_inflow", _d), ("alpha0", _vp),
+                ("npinned", _i32), ("pad", _i32), ("pinned", _vp)]
+
+
 class LearnedStats(ctypes.Structure):
     _fields_ = [("harvested", _i64), ("solves", _i64), ("iterations", _i64), ("fallbacks", _i64),
                 ("last_iters", _i32), ("last_info", _i32), ("last_relres", _d),
@@ -124,7 +131,7 @@ class State(ctypes.Structure):
 
 class Record(ctypes.Structure):
     _fields_ = [("t", _d), ("dt", _d), ("gamma", _d), ("residual", _d), ("divergence", _d),
-                ("kinetic_energy", _d), ("cg_relres", _d), ("mon", _d * 32), ("n", _i64),
+                ("kinetic_energy", _d), ("cg_relres", _d), ("mon", _d * 36), ("n", _i64),
                 ("cg_iters", _i32), ("cg_info", _i32), ("cg_outer", _i32), ("nmon", _i32)]
 
 
@@ -178,6 +185,7 @@ def lib() -> ctypes.CDLL:
         "cfd_favg_read": ([_vp, P(_d), _vp, _vp], _i32),
         "cfd_download_derived": ([_vp, _i32, _vp], _i32),
         "cfd_set_turbulence": ([_vp, P(TurbDesc)], _i32),
+        "cfd_set_vof": ([_vp, P(VofDesc)], _i32),
         "cfd_momentum_tendencies_var": ([_vp, _vp, _vp, _i32, _i32, _d, _d, _vp, _d, _i32, _vp,
                                          _vp, _vp], _i32),
         "cfd_set_learned": ([_vp, P(LearnedDesc)], _i32),
@@ -222,7 +230,8 @@ def exported_symbols() -> list[str]:
                         "cfd_dev_free", "cfd_memcpy", "cfd_momentum_tendencies",
                         "cfd_obstacle_bc", "cfd_scalar_transport", "cfd_probes_set",
                         "cfd_probes_read", "cfd_favg_set", "cfd_favg_read", "cfd_download_derived",
-                        "cfd_set_turbulence", "cfd_momentum_tendencies_var", "cfd_set_learned",
+                        "cfd_set_turbulence", "cfd_momentum_tendencies_var", "cfd_set_vof",
+                        "cfd_set_learned",
                         "cfd_learned_stats", "cfd_learned_reset", "cfd_ens_create",
                         "cfd_ens_destroy", "cfd_ens_set_net", "cfd_ens_step", "cfd_ens_download",
                         "cfd_ens_stats", "cfd_unet_param_count",

@@ -216,10 +216,28 @@ def turbulent_channel(re_tau: float = 180.0, ny: int | None = None, y_plus_1: fl
                 1.0, 2.0, re_b, info)
 
 
-def _unsupported(name):
-    def f(*a, **k):
-        raise ValueError(f"case {name!r} needs physics outside the GPU projection step")
-    return f
+def dam_break(n: int = 96, size: float = 0.584, column_width: float = 0.1461,
+              column_height: float = 0.292, obstacle: bool = True, sigma: float = 0.07,
+              t_end: float = 1.0, open_top: bool = True, **num) -> Case:
+    """Water column collapsing in a square tank (SI units, water and air;
+    Martin & Moyce 1952), optionally with a small block on the floor; open top
+    (p = 0) or closed (cases.py:294-320)."""
+    from .vof import VOF
+
+    grid = Grid(n, n, size, size)
+    if obstacle:
+        x0 = 0.5 * size - 0.012
+        grid.add_obstacle(geometry.rectangle(grid, x0, 0.0, x0 + 0.024, 0.048))
+    X, Y = grid.cell_centers()
+    alpha0 = ((X <= column_width) & (Y <= column_height)).astype(float)
+    bcs = {"left": BC("noslip"), "right": BC("noslip"), "bottom": BC("noslip"),
+           "top": BC("outflow") if open_top else BC("noslip")}
+    num.setdefault("gamma", 0.5)
+    sim = Simulation(grid, bcs, nu=1e-6, models=[VOF(alpha0, sigma=sigma)], **_numerics(num))
+    info = {"a": column_width, "h0": column_height, "g": 9.81}
+    U = np.sqrt(2 * 9.81 * column_height)
+    return Case("dam_break", "Dammbruch (VOF, Wasser/Luft)", sim, t_end, None, U, column_width,
+                float(U * column_width / 1e-6), info)
 
 
 def add_turbulence(case: Case, model: str = "kOmegaSST", **kw) -> Case:
@@ -281,7 +299,7 @@ CASES = {
     "heated_cavity": heated_cavity,
     "rayleigh_benard": rayleigh_benard,
     "turbulent_channel": turbulent_channel,
-    "dam_break": _unsupported("dam_break"),
+    "dam_break": dam_break,
 }
 
 

@@ -68,7 +68,7 @@ class DeviceContext:
             return (nx + 1, ny + 2)
         if name in ("v", "G", "hv", "hv_old"):
             return (nx + 2, ny + 1)
-        if name in ("p", "nut", "diff_k", "diff_omega"):
+        if name in ("p", "nut", "diff_k", "diff_omega", "alpha", "rho", "mu"):
             return (nx + 2, ny + 2)
         if name == "x":
             return (nx, ny)

@@ -166,10 +166,22 @@ class Simulation:
 
     @property
     def nu_eff(self) -> np.ndarray:
-        """nu + nu_t (solver.py:332-333)."""
+        """nu + nu_t (solver.py:332-333); with two phases the mixture's
+        dynamic viscosity (vof.py:205-210)."""
+        if self.variable_density:
+            self._flush()
+            return self._ctx.download("mu")
         if not self.variable_viscosity:
             return np.full((self.grid.nx + 2, self.grid.ny + 2), self.nu)
         return np.add(self.nu, self.nu_t)
+
+    @property
+    def rho(self) -> np.ndarray:
+        """Density (nx+2, ny+2): 1, or the two-phase mixture's."""
+        if not self.variable_density:
+            return np.ones((self.grid.nx + 2, self.grid.ny + 2))
+        self._flush()
+        return self._ctx.download("rho")
 
     def model(self, kind: str):
         for m in self.models:
@@ -295,7 +307,10 @@ class Simulation:
         g = self.grid
         umax = max(np.abs(self.u).max(), 1e-12)
         vmax = max(np.abs(self.v).max(), 1e-12)
-        nu_max = float(self.nu_eff.max()) if self.variable_viscosity else self.nu
+        if self.variable_density:
+            nu_max = float((self.nu_eff / self.rho)[g.fluid].max())
+        else:
+            nu_max = float(self.nu_eff.max()) if self.variable_viscosity else self.nu
         dt_visc = 0.5 / (nu_max * (1.0 / g.dx ** 2 + 1.0 / g.dy ** 2))
         dt_conv = min(g.dx / umax, g.dy / vmax)
         if self.scheme == "ab2":

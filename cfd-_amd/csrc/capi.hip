@@ -20,6 +20,7 @@
 #include "monitors.hip"
 #include "turb.hip"
 #include "flow.hip"
+#include "vof.hip"
 
 namespace {
 
@@ -107,6 +108,16 @@ struct cfd_ctx {
         unsigned long long* acc = nullptr;
         double nfaces = 0.0;
     } flow;
+    // two-phase VOF (vof.hip); cell arrays (nx+2) x P; alpha lives in alpha[0]
+    // between steps (the sub-step launches come in an even number)
+    struct {
+        int on = 0;
+        double *alpha[2] = {nullptr, nullptr}, *rho = nullptr, *mu = nullptr;
+        double *kappa = nullptr, *cE = nullptr, *cN = nullptr, *diag = nullptr;
+        int* pinned = nullptr;
+        int npinned = 0;
+        double rho1 = 1, rho2 = 1, mu1 = 0, mu2 = 0, sigma = 0, c_alpha = 0, cfl = 0.25, a_in = 0;
+    } vof;
     // turbulence model (turb.hip); cell arrays (nx+2) x P
     struct {
         int model = TURB_NONE;
@@ -185,6 +196,7 @@ int shape_of(const cfd_ctx* c, int field, int* rows, int* cols, int* off_i, int*
     case CFD_FIELD_X:
         *rows = c->nx; *cols = c->ny; *off_i = 1; *off_j = 1; return 0;
     case CFD_FIELD_NUT: case CFD_FIELD_DIFF_K: case CFD_FIELD_DIFF_OMEGA:
+    case CFD_FIELD_ALPHA: case CFD_FIELD_RHO: case CFD_FIELD_MU:
         *rows = c->nx + 2; *cols = c->ny + 2; return 0;
     default:
         return set_err(CFD_ERR_ARG, "unknown field id %d", field);
@@ -213,6 +225,9 @@ double* field_ptr(cfd_ctx* c, int field, int par) {
     case CFD_FIELD_NUT: return c->turb.nut;
     case CFD_FIELD_DIFF_K: return c->turb.dk;
     case CFD_FIELD_DIFF_OMEGA: return c->turb.dw;
+    case CFD_FIELD_ALPHA: return c->vof.alpha[0];
+    case CFD_FIELD_RHO: return c->vof.rho;
+    case CFD_FIELD_MU: return c->vof.mu;
     default: return nullptr;
     }
 }
@@ -269,6 +284,15 @@ PredArgs pred_args(cfd_ctx* c, const cfd_step_params* prm, int par) {
     }
     a.nu0 = prm ? prm->nu : 0.0;
     a.nut = c->turb.model != TURB_NONE ? c->turb.nut : nullptr;
+    if (c->vof.on) {
+        a.nut = c->vof.mu;
+        a.rho = c->vof.rho;
+        if (c->vof.sigma > 0.0) {
+            a.kappa = c->vof.kappa;
+            a.alpha = c->vof.alpha[0];
+            a.sigma = c->vof.sigma;
+        }
+    }
     a.fx = prm ? prm->fx : 0.0;
     a.fy = prm ? prm->fy : 0.0;
     a.st = c->st;
@@ -345,7 +369,56 @@ int launch_coop(cfd_ctx* c, ProjArgs a, bool prof = true) {
     return 0;
 }
 
+constexpr int VOF_NSUB_MAX = 8;   // even: alpha returns to alpha[0]
+
+VofArgs vof_args(cfd_ctx* c, const double* in, double* out, int sub) {
+    VofArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.a = in;
+    a.out = out;
+    a.u = c->u;
+    a.v = c->v;
+    a.fl = c->fl;
+    a.rho = c->vof.rho;
+    a.mu = c->vof.mu;
+    a.kappa = c->vof.kappa;
+    a.st = c->st;
+    a.rho1 = c->vof.rho1;
+    a.rho2 = c->vof.rho2;
+    a.mu1 = c->vof.mu1;
+    a.mu2 = c->vof.mu2;
+    a.c_alpha = c->vof.c_alpha;
+    a.cfl = c->vof.cfl;
+    a.a_in = c->vof.a_in;
+    a.sub = sub;
+    a.nsub_max = VOF_NSUB_MAX;
+    return a;
+}
+
+// the variable-coefficient pressure solve and correction, one cooperative launch
+int launch_coop_var(cfd_ctx* c, ProjArgs a) {
+    const int words = (c->proj_blocks + 2) * CFD_SLOT * 2;
+    hipLaunchKernelGGL(k_zero_u32, dim3((words + 255) / 256), dim3(256), 0, c->stream,
+                       (unsigned*)c->sync, words);
+    HIPC(hipGetLastError());
+    VarProjArgs va;
+    va.p = a;
+    va.p.rtol = std::min(a.rtol, 1e-12);   // the reference solves this system directly
+    va.rho = c->vof.rho;
+    va.cE = c->vof.cE;
+    va.cN = c->vof.cN;
+    va.diag = c->vof.diag;
+    va.pinned = c->vof.pinned;
+    va.npinned = c->vof.npinned;
+    Geo g = c->geo;
+    void* args[] = {(void*)&va, (void*)&g};
+    HIPC(hipLaunchCooperativeKernel((const void*)k_project_var, dim3(c->proj_blocks),
+                                    dim3(c->proj_bs), args, 0, c->stream));
+    return 0;
+}
+
 int launch_project(cfd_ctx* c, ProjArgs a) {
+    if (c->vof.on) return launch_coop_var(c, a);
     if (a.method != PROJ_PCG64 && a.method != PROJ_MG && a.method != PROJ_LEARNED)
         return set_err(CFD_ERR_ARG, "pressure solver %d is not available", a.method);
     if (a.method == PROJ_MG && !c->mg_ready)
@@ -481,6 +554,38 @@ int launch_flow_control(cfd_ctx* c, const cfd_step_params* prm) {
     }
     TRY(launch_bc(c, true));
     hipLaunchKernelGGL(k_flow_residual, dim3(c->red_blocks), dim3(256), 0, c->stream, a, c->geo);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+
+// fill the ghost and solid cells of alpha[k] and set rho, mu (sub -1: always)
+int launch_vof_fill(cfd_ctx* c, int k, int sub) {
+    VofArgs a = vof_args(c, c->vof.alpha[k], c->vof.alpha[k], sub);
+    const int ng = 2 * (c->nx + 2) + 2 * c->ny;
+    hipLaunchKernelGGL(k_vof_ghosts, dim3((ng + 255) / 256), dim3(256), 0, c->stream, a, c->geo);
+    HIPC(hipGetLastError());
+    hipLaunchKernelGGL(k_vof_solids, grid2d(c), dim3(BS2D), 0, c->stream, a, c->geo);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+// alpha transport after the step (vof.py:243-254) and its monitors
+int launch_vof_post(cfd_ctx* c) {
+    VofArgs a0 = vof_args(c, c->vof.alpha[0], c->vof.alpha[1], 0);
+    hipLaunchKernelGGL(k_vof_nsub, dim3(1), dim3(1024), 0, c->stream, a0, c->geo);
+    HIPC(hipGetLastError());
+    for (int s = 0; s < VOF_NSUB_MAX; ++s) {
+        VofArgs a = vof_args(c, c->vof.alpha[s & 1], c->vof.alpha[(s & 1) ^ 1], s);
+        hipLaunchKernelGGL(k_vof_advect, grid2d(c), dim3(BS2D), 0, c->stream, a, c->geo);
+        HIPC(hipGetLastError());
+        TRY(launch_vof_fill(c, (s & 1) ^ 1, s));
+    }
+    // the properties update fills the ghosts once more (vof.py:205-206): a
+    // ghost next to a solid edge cell then takes that cell's new value
+    TRY(launch_vof_fill(c, 0, -1));
+    VofArgs m = vof_args(c, c->vof.alpha[0], c->vof.alpha[0], -1);
+    hipLaunchKernelGGL(k_vof_monitors, dim3(1), dim3(1024), 0, c->stream, m, c->geo);
     HIPC(hipGetLastError());
     return 0;
 }
@@ -1138,13 +1243,15 @@ int cfd_upload(cfd_ctx* c, int32_t field, const double* host) {
     TRY(shape_of(c, field, &rows, &cols, &oi, &oj));
     int par = 0;
     TRY(host_parity(c, &par));
-    if (field >= CFD_FIELD_NUT)
-        return set_err(CFD_ERR_ARG, "field %d is computed by the turbulence model", field);
+    if (field >= CFD_FIELD_NUT && field != CFD_FIELD_ALPHA)
+        return set_err(CFD_ERR_ARG, "field %d is computed by a model", field);
+    if (field == CFD_FIELD_ALPHA && !c->vof.on) return set_err(CFD_ERR_STATE, "no VOF model set");
     double* dst = field_ptr(c, field, par);
     if (field == CFD_FIELD_X) c->l_hist = 0;   // a new iterate starts a new history
     HIPC(hipMemcpy2DAsync(dst + (size_t)oi * c->P + oj, c->P * sizeof(double), host,
                           cols * sizeof(double), cols * sizeof(double), rows,
                           hipMemcpyHostToDevice, c->stream));
+    if (field == CFD_FIELD_ALPHA) TRY(launch_vof_fill(c, 0, -1));   // vof.py:283-287
     HIPC(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1360,7 +1467,17 @@ int cfd_step(cfd_ctx* c, const cfd_step_params* prm, int32_t count) {
         // batch is issued with the parity it will have if all steps run.
         const int pk = par ^ (k & 1);
         TRY(launch_bc(c, true));
-        TurbDt td{nullptr, nullptr, nullptr, c->fl};
+        TurbDt td{nullptr, nullptr, nullptr, c->fl, nullptr};
+        if (c->vof.on) {
+            td.nut = c->vof.mu;
+            td.rho = c->vof.rho;
+            if (c->vof.sigma > 0.0) {
+                VofArgs va = vof_args(c, c->vof.alpha[0], c->vof.alpha[0], -1);
+                hipLaunchKernelGGL(k_vof_kappa, dim3((c->ny + 255) / 256, c->nx), dim3(256), 0,
+                                   c->stream, va, c->geo);
+                HIPC(hipGetLastError());
+            }
+        }
         if (c->turb.model != TURB_NONE) {
             TRY(launch_turb_pre(c, pk, prm->nu, true));
             td.nut = c->turb.nut;
@@ -1394,6 +1511,7 @@ int cfd_step(cfd_ctx* c, const cfd_step_params* prm, int32_t count) {
             HIPC(hipGetLastError());
         }
         if (c->turb.model == TURB_SST) TRY(launch_sst_post(c, pk));
+        if (c->vof.on) TRY(launch_vof_post(c));
         RecordArgs ra;
         std::memset(&ra, 0, sizeof(ra));
         ra.u = c->u;
@@ -1818,6 +1936,61 @@ int cfd_set_turbulence(cfd_ctx* c, const cfd_turb_desc* d) {
     int par = 0;
     TRY(host_parity(c, &par));
     TRY(launch_turb_pre(c, par, d->nu, false));
+    HIPC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// ---- two-phase VOF (vof.py) ---------------------------------------------------
+int cfd_set_vof(cfd_ctx* c, const cfd_vof_desc* d) {
+    if (!c || !d || !d->alpha0) return set_err(CFD_ERR_ARG, "null argument");
+    if (c->vof.on) return set_err(CFD_ERR_STATE, "a VOF model is set");
+    if (c->turb.model != TURB_NONE)
+        return set_err(CFD_ERR_STATE, "VOF and a turbulence model together are not supported");
+    if (!(d->rho1 > 0.0 && d->rho2 > 0.0) || !(d->cfl_alpha > 0.0))
+        return set_err(CFD_ERR_ARG, "densities and cfl_alpha must be positive");
+    if (d->npinned < 0 || d->npinned > CFD_MAX_COMP || (d->npinned && !d->pinned))
+        return set_err(CFD_ERR_ARG, "pinned cells");
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_project_var, c->proj_bs, 0) !=
+            hipSuccess || per_cu < 1)
+        return set_err(CFD_ERR_HIP, "variable-density projection kernel does not fit on a CU");
+    HIPC(hipStreamSynchronize(c->stream));
+    const int nx = c->nx, ny = c->ny, P = c->P;
+    const size_t N = cells(c);
+    TRY(dalloc(c, &c->vof.alpha[0], N));
+    TRY(dalloc(c, &c->vof.alpha[1], N));
+    TRY(dalloc(c, &c->vof.rho, N));
+    TRY(dalloc(c, &c->vof.mu, N));
+    TRY(dalloc(c, &c->vof.kappa, N));
+    TRY(dalloc(c, &c->vof.cE, N));
+    TRY(dalloc(c, &c->vof.cN, N));
+    TRY(dalloc(c, &c->vof.diag, N));
+    std::vector<double> h(N, 0.0);
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) h[(size_t)(i + 1) * P + j + 1] = d->alpha0[(size_t)i * ny + j];
+    HIPC(hipMemcpy(c->vof.alpha[0], h.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<int> pin(std::max(1, (int)d->npinned), -1);
+    for (int k = 0; k < d->npinned; ++k) {
+        const int pi = d->pinned[2 * k], pj = d->pinned[2 * k + 1];
+        if (pi < 0 || pi >= nx || pj < 0 || pj >= ny) return set_err(CFD_ERR_ARG, "pinned cell");
+        pin[k] = (pi + 1) * P + pj + 1;
+    }
+    TRY(dalloc(c, &c->vof.pinned, pin.size()));
+    HIPC(hipMemcpy(c->vof.pinned, pin.data(), pin.size() * sizeof(int), hipMemcpyHostToDevice));
+    c->vof.npinned = d->npinned;
+    c->vof.rho1 = d->rho1;
+    c->vof.rho2 = d->rho2;
+    c->vof.mu1 = d->mu1;
+    c->vof.mu2 = d->mu2;
+    c->vof.sigma = d->sigma;
+    c->vof.c_alpha = d->c_alpha;
+    c->vof.cfl = d->cfl_alpha;
+    c->vof.a_in = d->alpha_inflow;
+    c->vof.on = 1;
+    TRY(launch_vof_fill(c, 0, -1));
+    VofArgs m = vof_args(c, c->vof.alpha[0], c->vof.alpha[0], -1);
+    hipLaunchKernelGGL(k_vof_monitors, dim3(1), dim3(1024), 0, c->stream, m, c->geo);
+    HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(c->stream));
     return 0;
 }

@@ -35,7 +35,12 @@ struct PredArgs {
     const double* T;         // Boussinesq temperature or nullptr
     double gx, gy, Tref;
     double nu0;
-    const double* nut;       // turbulent viscosity (cells): stress-form diffusion of nu0 + nut
+    const double* nut;       // turbulent viscosity (cells): stress-form diffusion of nu0 + nut;
+                             // with rho: the dynamic viscosity of the two-phase mixture
+    const double* rho;       // variable density (VOF) or nullptr
+    const double* kappa;     // VOF surface tension: curvature (cells), alpha, sigma
+    const double* alpha;
+    double sigma;
     double fx, fy;
     const DevState* st;
     int scheme_ab2;
@@ -180,6 +185,7 @@ struct TurbDt {
     const double* dk;        // k-omega SST diffusivities of k and omega, or nullptr
     const double* dw;
     const uint8_t* fl;
+    const double* rho;       // variable density: nut holds mu, the limits use mu / rho
 };
 
 // kernels (stencil.hip)

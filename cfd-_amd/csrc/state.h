@@ -20,9 +20,11 @@ struct DevState {
     int error;             // instability detected (solver.py:395-399)
     double nut_max;        // max of the turbulent viscosity this step (0 when laminar)
     double flow_du;        // flow-rate control: this step's velocity shift
+    double vof_nsub;       // VOF: sub-steps of this step's alpha transport
+    double vof_sum, vof_front, vof_height;   // VOF monitors of the step
 };
 
-#define CFD_MAX_MON 32
+#define CFD_MAX_MON 36
 // One record per completed step (cfd2d/solver.py:427-443 plus solver data).
 struct StepRecord {
     double t, dt, gamma, residual, divergence, ke;

@@ -30,7 +30,8 @@ __device__ __forceinline__ double mom_u_face(const double* __restrict__ u,
                                              const double* __restrict__ v, int pu, int pv,
                                              int i, int ip, int j, double nu0,
                                              const double* __restrict__ nuf, int pc, double gamma,
-                                             double idx2, double idy2, double idx, double idy) {
+                                             double idx2, double idy2, double idx, double idy,
+                                             const double* __restrict__ rho = nullptr) {
     const double uc = u[ix(i, j, pu)], ue = u[ix(ip, j, pu)], uw = u[ix(i - 1, j, pu)];
     const double un = u[ix(i, j + 1, pu)], us = u[ix(i, j - 1, pu)];
     double diff;
@@ -48,6 +49,8 @@ __device__ __forceinline__ double mom_u_face(const double* __restrict__ u,
                idy * (nu_n * ((un - uc) * idy + (v[ix(i + 1, j, pv)] - v[ix(i, j, pv)]) * idx) -
                       nu_s * ((uc - us) * idy +
                               (v[ix(i + 1, j - 1, pv)] - v[ix(i, j - 1, pv)]) * idx));
+        // variable density: nuf is the dynamic viscosity (kernels.py:104-105)
+        if (rho) diff /= 0.5 * (rho[ix(i, j, pc)] + rho[ix(i + 1, j, pc)]);
     }
     const double a_e = 0.5 * (uc + ue), a_w = 0.5 * (uw + uc);
     const double du2dx = idx * (a_e * a_e - a_w * a_w)
@@ -64,7 +67,8 @@ __device__ __forceinline__ double mom_v_face(const double* __restrict__ u,
                                              const double* __restrict__ v, int pu, int pv,
                                              int i, int j, double nu0,
                                              const double* __restrict__ nuf, int pc, double gamma,
-                                             double idx2, double idy2, double idx, double idy) {
+                                             double idx2, double idy2, double idx, double idy,
+                                             const double* __restrict__ rho = nullptr) {
     const double vc = v[ix(i, j, pv)], ve = v[ix(i + 1, j, pv)], vw = v[ix(i - 1, j, pv)];
     const double vn = v[ix(i, j + 1, pv)], vs = v[ix(i, j - 1, pv)];
     double diff;
@@ -82,6 +86,7 @@ __device__ __forceinline__ double mom_v_face(const double* __restrict__ u,
                idx * (nu_e * ((ve - vc) * idx + (u[ix(i, j + 1, pu)] - u[ix(i, j, pu)]) * idy) -
                       nu_w * ((vc - vw) * idx +
                               (u[ix(i - 1, j + 1, pu)] - u[ix(i - 1, j, pu)]) * idy));
+        if (rho) diff /= 0.5 * (rho[ix(i, j, pc)] + rho[ix(i, j + 1, pc)]);   // lines 139-140
     }
     const double b_n = 0.5 * (vc + vn), b_s = 0.5 * (vs + vc);
     const double dv2dy = idy * (b_n * b_n - b_s * b_s)
@@ -155,10 +160,20 @@ __global__ void k_predictor(PredArgs a, Geo g) {
         double h = 0.0;
         if (uf) {   // implies 1 <= i <= (periodic ? nx : nx-1) and 1 <= j <= ny
             const int ip = i < nx ? i + 1 : 1;
-            h = a.nut ? mom_u_face<2>(a.u, a.v, g.pu, g.pv, i, ip, j, a.nu0, a.nut, g.pc, gamma,
-                                      idx2, idy2, idx, idy)
-                      : mom_u_face<0>(a.u, a.v, g.pu, g.pv, i, ip, j, a.nu0, a.nut, g.pc, gamma,
-                                      idx2, idy2, idx, idy);
+            if (a.rho)
+                h = mom_u_face<1>(a.u, a.v, g.pu, g.pv, i, ip, j, 0.0, a.nut, g.pc, gamma, idx2,
+                                  idy2, idx, idy, a.rho);
+            else
+                h = a.nut ? mom_u_face<2>(a.u, a.v, g.pu, g.pv, i, ip, j, a.nu0, a.nut, g.pc,
+                                          gamma, idx2, idy2, idx, idy)
+                          : mom_u_face<0>(a.u, a.v, g.pu, g.pv, i, ip, j, a.nu0, a.nut, g.pc,
+                                          gamma, idx2, idy2, idx, idy);
+        }
+        if (uf && a.kappa) {   // CSF surface tension over the face density (vof.py:235-240)
+            const int c0 = ix(i, j, g.pc), c1 = ix(i + 1, j, g.pc);
+            const double fxs = a.sigma * 0.5 * (a.kappa[c0] + a.kappa[c1]) *
+                               (a.alpha[c1] - a.alpha[c0]) / g.dx;
+            h = h + fxs / (0.5 * (a.rho[c0] + a.rho[c1]));
         }
         if (uf && a.T && a.gx != 0.0) {
             const double Tf = 0.5 * (a.T[ix(i, j, g.pc)] + a.T[ix(i + 1, j, g.pc)]) - a.Tref;
@@ -188,10 +203,18 @@ __global__ void k_predictor(PredArgs a, Geo g) {
         const bool vf = (fl[ix(i, j, g.pc)] & 1) && (fl[ix(i, j + 1, g.pc)] & 1);
         double h = 0.0;
         if (vf && i >= 1 && i <= nx && j >= 1 && j < ny)
-            h = a.nut ? mom_v_face<2>(a.u, a.v, g.pu, g.pv, i, j, a.nu0, a.nut, g.pc, gamma, idx2,
-                                      idy2, idx, idy)
-                      : mom_v_face<0>(a.u, a.v, g.pu, g.pv, i, j, a.nu0, a.nut, g.pc, gamma, idx2,
-                                      idy2, idx, idy);
+            h = a.rho ? mom_v_face<1>(a.u, a.v, g.pu, g.pv, i, j, 0.0, a.nut, g.pc, gamma, idx2,
+                                      idy2, idx, idy, a.rho)
+                : a.nut ? mom_v_face<2>(a.u, a.v, g.pu, g.pv, i, j, a.nu0, a.nut, g.pc, gamma,
+                                        idx2, idy2, idx, idy)
+                        : mom_v_face<0>(a.u, a.v, g.pu, g.pv, i, j, a.nu0, a.nut, g.pc, gamma,
+                                        idx2, idy2, idx, idy);
+        if (vf && a.kappa) {
+            const int c0 = ix(i, j, g.pc), c1 = ix(i, j + 1, g.pc);
+            const double fys = a.sigma * 0.5 * (a.kappa[c0] + a.kappa[c1]) *
+                               (a.alpha[c1] - a.alpha[c0]) / g.dy;
+            h = h + fys / (0.5 * (a.rho[c0] + a.rho[c1]));
+        }
         if (vf && a.T && a.gy != 0.0) {
             const double Tf = 0.5 * (a.T[ix(i, j, g.pc)] + a.T[ix(i, j + 1, g.pc)]) - a.Tref;
             h += -a.gy * Tf;
@@ -423,9 +446,10 @@ __global__ void k_dt_gamma(const double* __restrict__ u, const double* __restric
             for (int i = r0; i < r1; ++i)
                 for (int j = threadIdx.x; j <= ny + 1; j += blockDim.x) {
                     const int c = ix(i, j, g.pc);
-                    const double nt = td.nut[c];
                     const bool f = td.fl[c] & 1;
-                    q[0] = nanmax(q[0], nt);
+                    // variable density: nu = mu / rho over the fluid cells
+                    const double nt = td.rho ? td.nut[c] / td.rho[c] : td.nut[c];
+                    q[0] = (f || !td.rho) ? nanmax(q[0], nt) : q[0];
                     q[1] = f ? nanmax(q[1], -nt) : q[1];
                     if (td.dk) {
                         q[2] = f ? nanmax(q[2], td.dk[c]) : q[2];
@@ -472,8 +496,8 @@ __global__ void k_dt_gamma(const double* __restrict__ u, const double* __restric
                 acc[3] = dbits(__builtin_inf());
                 acc[4] = 0ull;
                 acc[5] = 0ull;
-                nu_max = prm.nu + nt_max;
-                nu_ref = prm.nu + nt_min;
+                nu_max = td.rho ? nt_max : prm.nu + nt_max;
+                nu_ref = td.rho ? nt_min : prm.nu + nt_min;
                 if (!st->done) st->nut_max = nt_max;
                 if (td.dk) {
                     const double h2 = 1.0 / (g.dx * g.dx) + 1.0 / (g.dy * g.dy);
@@ -768,7 +792,7 @@ __global__ void k_fill_ghosts(ScalarArgs a, Geo g) {
 // `part`; the last block adds them in block order (deterministic), writes the
 // record and advances t, n, dt_old.
 // ---------------------------------------------------------------------------
-static_assert(REC_NV + 1 < CFD_MAX_MON, "the record holds the sums, max nu_t and the force");
+static_assert(REC_NV + 4 < CFD_MAX_MON, "the record holds the sums, nu_t, force, VOF values");
 __global__ void k_record(RecordArgs a, Geo g) {
     __shared__ double lds[REC_NV * 16];
     const int nx = g.nx, ny = g.ny, pc = g.pc;
@@ -912,6 +936,9 @@ __global__ void k_record(RecordArgs a, Geo g) {
         for (int q = 0; q < CFD_MAX_MON; ++q) r.mon[q] = q < REC_NV ? tot[q] : 0.0;
         r.mon[REC_NV] = st->nut_max;   // turbulence.py:158-159, 322-324
         r.mon[REC_NV + 1] = st->flow_force;   // solver.py:439-440
+        r.mon[REC_NV + 2] = st->vof_sum;      // vof.py:264-281: alpha sum, front, height
+        r.mon[REC_NV + 3] = st->vof_front;
+        r.mon[REC_NV + 4] = st->vof_height;
         const double res = r.residual;
         if (!(res == res) || res > 1e8) {
             st->error = 1;          // solver.py:395-399: no record for this step

@@ -45,7 +45,11 @@ enum { CFD_FIELD_U = 0, CFD_FIELD_V = 1, CFD_FIELD_P = 2, CFD_FIELD_F = 3, CFD_F
        CFD_FIELD_X = 9,
        /* turbulence model outputs, (nx+2) x (ny+2), download only: nu_t and the
           SST diffusivities of k and omega */
-       CFD_FIELD_NUT = 10, CFD_FIELD_DIFF_K = 11, CFD_FIELD_DIFF_OMEGA = 12 };
+       CFD_FIELD_NUT = 10, CFD_FIELD_DIFF_K = 11, CFD_FIELD_DIFF_OMEGA = 12,
+       /* two-phase VOF, (nx+2) x (ny+2): the phase fraction (upload sets the
+          properties from it) and the mixture density and dynamic viscosity
+          (download only) */
+       CFD_FIELD_ALPHA = 13, CFD_FIELD_RHO = 14, CFD_FIELD_MU = 15 };
 /* pressure solvers: Jacobi-PCG in FP64 (the reference's "cg", pressure.py:171),
    and flexible CG with FP64 vectors and dot products preconditioned by an FP32
    aggregation-multigrid V-cycle (set up with cfd_set_multigrid), the solve
@@ -131,11 +135,13 @@ typedef struct cfd_state {
    scalar s (5 values from index 10+5s, s < 4): sum of phi over fluid cells and
    the wall-flux sums of the left, right, bottom, top sides (models.py:228-250);
    [30] the largest nu_t of the step (turbulence models, else 0), [31] the
-   driving force after the step (flow_force). */
+   driving force after the step (flow_force), [32..34] VOF: the sum of alpha
+   over fluid cells, the last cell of the bottom row and of the left column
+   with alpha > 0.5 (1-based, -1 for none). */
 typedef struct cfd_record {
     double t, dt, gamma, residual, divergence, kinetic_energy;
     double cg_relres;
-    double mon[32];
+    double mon[36];
     int64_t n;
     int32_t cg_iters, cg_info, cg_outer, nmon;
 } cfd_record;
@@ -346,6 +352,20 @@ typedef struct cfd_turb_desc {
     const double* omega_wall;        /* SST: 6 nu / (beta1 y^2) there (turbulence.py:239-243) */
 } cfd_turb_desc;
 int cfd_set_turbulence(cfd_ctx* ctx, const cfd_turb_desc* desc);
+/* Two-phase VOF (vof.py:136-306, cases.py:294-320); no counterpart in a C
+   interface of the reference.  alpha0 is nx x ny (clipped to [0, 1], 0 off
+   the fluid); the pressure system becomes div((1/rho) grad p) = div(u*)/dt,
+   solved by a Jacobi-PCG to 1e-12 (the reference solves it directly) with
+   the pure-Neumann components' pinned cells (i, j) from the pressure set-up.
+   Gravity is the caller's body force; sigma > 0 adds CSF surface tension. */
+typedef struct cfd_vof_desc {
+    double rho1, rho2, mu1, mu2, sigma, c_alpha, cfl_alpha, alpha_inflow;
+    const double* alpha0;
+    int32_t npinned;
+    int32_t pad;
+    const int32_t* pinned;
+} cfd_vof_desc;
+int cfd_set_vof(cfd_ctx* ctx, const cfd_vof_desc* desc);
 /* kernels.momentum_tendencies with variable = True (kernels.py:96-103,
    131-138): stress-form diffusion of the cell viscosity field nu
    ((nx+2) x (ny+2), nu_eff) */
