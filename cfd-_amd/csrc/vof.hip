@@ -437,7 +437,6 @@ __global__ void __launch_bounds__(PROJ_BS) k_project_var(VarProjArgs va, Geo g) 
         const double rv = bv - apply(a.x, i, j, dg);
         a.r[c] = rv;
         const double z = rv / dg;
-        a.d0[c] = z;
         s3[0] += bv * bv;
         s3[1] += rv * rv;
         s3[2] += rv * z;
@@ -453,19 +452,40 @@ __global__ void __launch_bounds__(PROJ_BS) k_project_var(VarProjArgs va, Geo g) 
         rr = 0.0;
     } else {
         bool conv = false;
+        double rz_prev = 1.0;
         for (int k = 0; k < a.maxiter; ++k) {
             if (sqrt(rr) < atol || !(rr == rr)) {
                 conv = sqrt(rr) < atol;
                 break;
             }
-            // q = A d, d.q
+            // new direction d = z + beta d_old, formed for the neighbours too
+            // (from their residual and old direction), so that the update and
+            // the product q = A d need no grid barrier between them
+            const double beta = k > 0 ? rz / rz_prev : 0.0;
+            double* dnew = (k & 1) ? a.d1 : a.d0;
+            const double* dold = (k & 1) ? a.d0 : a.d1;
+            auto dirv = [&](long cc) -> double {
+                const double z = a.r[cc] / va.diag[cc];
+                return k > 0 ? z + beta * dold[cc] : z;
+            };
             double dq[1] = {0.0};
             FOR_CELLS {
                 const long c = ix(i, j, pc);
                 if (!fluid(i, j)) continue;
-                const double q = apply(a.d0, i, j, va.diag[c]);
+                const int iE = (periodic && i == nx) ? 1 : i + 1;
+                const int iW = (periodic && i == 1) ? nx : i - 1;
+                const double ce = va.cE[c];
+                const double cw = (periodic && i == 1) ? va.cE[ix(nx, j, pc)] : cW(i, j);
+                const double cn = va.cN[c], csv = j > 1 ? va.cN[c - 1] : 0.0;
+                const double dc = dirv(c);
+                double q = va.diag[c] * dc;
+                if (ce != 0.0) q -= ce * dirv(ix(iE, j, pc));
+                if (cw != 0.0) q -= cw * dirv(ix(iW, j, pc));
+                if (cn != 0.0) q -= cn * dirv(c + 1);
+                if (csv != 0.0) q -= csv * dirv(c - 1);
+                dnew[c] = dc;
                 a.q[c] = q;
-                dq[0] += a.d0[c] * q;
+                dq[0] += dc * q;
             }
             grid_sum<1>(dq, gs, lds, epoch);
             const double alpha = rz / dq[0];
@@ -473,24 +493,16 @@ __global__ void __launch_bounds__(PROJ_BS) k_project_var(VarProjArgs va, Geo g) 
             FOR_CELLS {
                 const long c = ix(i, j, pc);
                 if (!fluid(i, j)) continue;
-                a.x[c] += alpha * a.d0[c];
+                a.x[c] += alpha * dnew[c];
                 const double rv = a.r[c] - alpha * a.q[c];
                 a.r[c] = rv;
-                const double z = rv / va.diag[c];
-                a.d1[c] = z;
                 s2[0] += rv * rv;
-                s2[1] += rv * z;
+                s2[1] += rv * (rv / va.diag[c]);
             }
             grid_sum<2>(s2, gs, lds, epoch);
-            const double beta = s2[1] / rz;
             rr = s2[0];
+            rz_prev = rz;
             rz = s2[1];
-            FOR_CELLS {
-                const long c = ix(i, j, pc);
-                if (!fluid(i, j)) continue;
-                a.d0[c] = a.d1[c] + beta * a.d0[c];
-            }
-            grid_sync(gs, lds, epoch);
             ++iters;
         }
         if (!conv) info = iters > 0 ? iters : a.maxiter;
