@@ -104,7 +104,39 @@ def build_case(leg: str, n: int = N):
     return case
 
 
+VOF_N = 256         # the two-phase leg: dam break on 256 x 256 (variable-density solve)
+
+
+def run_vof_leg(steps: int, warmup: int) -> dict:
+    """Dam break (water column, obstacle, CSF, open top) on VOF_N^2: a few
+    steps, the variable-density pressure solve dominating."""
+    from cfd2d_amd import cases
+
+    sim = cases.dam_break(n=VOF_N, t_end=1.0).sim
+    ctx = sim._ctx
+    prm = sim._params()
+    steps, warmup = min(steps, 40), min(warmup, 10)
+    ctx.step(prm, warmup)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(prm, steps)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    st = ctx.get_state()
+    recs = ctx.records(st.n - steps + 1, steps)
+    iters = np.array([r.cg_iters for r in recs], dtype=float)
+    out = {"solver": "vof_dam_break", "grid": [VOF_N, VOF_N], "steps": steps,
+           "steps_per_s": steps / wall, "ms_per_step": 1e3 * wall / steps,
+           "cg_iters_mean": float(iters.mean()),
+           "not_converged_steps": int(sum(1 for r in recs if r.cg_info != 0)),
+           "max_div": float(max(r.divergence for r in recs))}
+    sim.close()
+    return out
+
+
 def run_leg(solver: str, steps: int, warmup: int) -> dict:
+    if solver == "vof_dam_break":
+        return run_vof_leg(steps, warmup)
     case = build_case(solver)
     sim = case.sim
     ctx = sim._ctx
@@ -291,7 +323,7 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--legs", default="cg,cg_learned,cg_learned_none,cg_learned_x0prev,cg_mixed,"
-                                      "cg_learned_smagorinsky,cg_learned_sst")
+                                      "cg_learned_smagorinsky,cg_learned_sst,vof_dam_break")
     ap.add_argument("--cpu-child", nargs=2, default=None)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device-to-device (a known-size kernel for "
@@ -355,7 +387,7 @@ def main() -> None:
         "data": "synthetic",
         "config": {"workload": WORKLOAD, "grid": [N, N], "solver": head["solver"],
                    "rtol": RTOL},
-        "roofline": roofline_for(head),
+        "roofline": roofline_for(head) if "proj_ms_mean" in head else None,
         "cpu_baseline": cpu,
         "fp64_cg_steps_per_s": round(base["steps_per_s"], 3) if base and "error" not in base
         else None,
