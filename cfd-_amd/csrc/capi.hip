@@ -1541,7 +1541,16 @@ int cfd_step(cfd_ctx* c, const cfd_step_params* prm, int32_t count) {
         ra.t_end = prm->t_end;
         ra.steady_tol = prm->steady_tol;
         ra.max_steps = prm->max_steps;
-        hipLaunchKernelGGL(k_record, dim3(c->red_blocks), dim3(256), 0, c->stream, ra, c->geo);
+        {
+            const dim3 gr(c->red_blocks), bl(256);
+            switch (ra.nscalar) {
+            case 0: hipLaunchKernelGGL(k_record<0>, gr, bl, 0, c->stream, ra, c->geo); break;
+            case 1: hipLaunchKernelGGL(k_record<1>, gr, bl, 0, c->stream, ra, c->geo); break;
+            case 2: hipLaunchKernelGGL(k_record<2>, gr, bl, 0, c->stream, ra, c->geo); break;
+            case 3: hipLaunchKernelGGL(k_record<3>, gr, bl, 0, c->stream, ra, c->geo); break;
+            default: hipLaunchKernelGGL(k_record<4>, gr, bl, 0, c->stream, ra, c->geo); break;
+            }
+        }
         HIPC(hipGetLastError());
         TRY(launch_monitors(c, ra));
     }

@@ -202,6 +202,7 @@ __global__ void k_dt_gamma(const double* u, const double* v, Geo g, StepParams p
 template <bool CONST_D, int MODE>
 __global__ void k_scalar_transport(ScalarArgs a, Geo g);
 __global__ void k_fill_ghosts(ScalarArgs a, Geo g);
+template <int NS>
 __global__ void k_record(RecordArgs a, Geo g);
 __global__ void k_zero_u32(unsigned* p, int n);
 // pcg.hip

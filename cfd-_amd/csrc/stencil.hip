@@ -793,12 +793,16 @@ __global__ void k_fill_ghosts(ScalarArgs a, Geo g) {
 // record and advances t, n, dt_old.
 // ---------------------------------------------------------------------------
 static_assert(REC_NV + 4 < CFD_MAX_MON, "the record holds the sums, nu_t, force, VOF values");
+// NS: the number of transported fields (a compile-time count keeps the
+// per-thread sums and the block reductions as small as the case needs)
+template <int NS>
 __global__ void k_record(RecordArgs a, Geo g) {
-    __shared__ double lds[REC_NV * 16];
+    constexpr int NV = 10 + 5 * NS;
+    __shared__ double lds[NV * 16];
     const int nx = g.nx, ny = g.ny, pc = g.pc;
-    double val[REC_NV];
+    double val[NV];
 #pragma unroll
-    for (int q = 0; q < REC_NV; ++q) val[q] = 0.0;
+    for (int q = 0; q < NV; ++q) val[q] = 0.0;
     const bool active = !a.st->done;
     if (active) {
         const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -819,8 +823,7 @@ __global__ void k_record(RecordArgs a, Geo g) {
             val[1] = f ? val[1] + (uc * uc + vc * vc) : val[1];
             // compile-time scalar indices keep val[] in registers
 #pragma unroll
-            for (int s = 0; s < CFD_MAX_SCALARS; ++s) {
-                if (s >= a.nscalar) break;
+            for (int s = 0; s < NS; ++s) {
                 const double ph = a.phi[s][ix(i, j, pc)];
                 val[10 + 5 * s] = f ? val[10 + 5 * s] + ph : val[10 + 5 * s];
             }
@@ -839,8 +842,8 @@ __global__ void k_record(RecordArgs a, Geo g) {
         }
         // wall fluxes of scalars with fixed-value or fixed-flux sides (models.py:228-250)
 #pragma unroll
-        for (int s = 0; s < CFD_MAX_SCALARS; ++s) {
-            if (s >= a.nscalar || !a.wallflux[s]) continue;
+        for (int s = 0; s < NS; ++s) {
+            if (!a.wallflux[s]) continue;
             const double* phi = a.phi[s];
 #pragma unroll
             for (int side = 0; side < 4; ++side) {
@@ -871,17 +874,17 @@ __global__ void k_record(RecordArgs a, Geo g) {
         double mx[1] = {val[0]};
         block_max<1>(mx, lds);
         val[0] = mx[0];
-        double sm[REC_NV - 1];
+        double sm[NV - 1];
 #pragma unroll
-        for (int q = 1; q < REC_NV; ++q) sm[q - 1] = val[q];
-        block_sum<REC_NV - 1>(sm, lds);
+        for (int q = 1; q < NV; ++q) sm[q - 1] = val[q];
+        block_sum<NV - 1>(sm, lds);
 #pragma unroll
-        for (int q = 1; q < REC_NV; ++q) val[q] = sm[q - 1];
+        for (int q = 1; q < NV; ++q) val[q] = sm[q - 1];
     }
     __shared__ int s_last;
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < REC_NV; ++q) a.part[REC_NV * blockIdx.x + q] = val[q];
+        for (int q = 0; q < NV; ++q) a.part[NV * blockIdx.x + q] = val[q];
         const unsigned tk =
             __hip_atomic_fetch_add(a.ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         s_last = tk == gridDim.x - 1;
@@ -889,27 +892,27 @@ __global__ void k_record(RecordArgs a, Geo g) {
     __syncthreads();
     if (!s_last) return;
     // last block: add the block partials in block order (fixed tree per thread)
-    double tot[REC_NV];
+    double tot[NV];
 #pragma unroll
-    for (int q = 0; q < REC_NV; ++q) tot[q] = 0.0;
+    for (int q = 0; q < NV; ++q) tot[q] = 0.0;
     for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-        tot[0] = nanmax(tot[0], __hip_atomic_load(&a.part[REC_NV * b], __ATOMIC_RELAXED,
+        tot[0] = nanmax(tot[0], __hip_atomic_load(&a.part[NV * b], __ATOMIC_RELAXED,
                                                   __HIP_MEMORY_SCOPE_AGENT));
 #pragma unroll
-        for (int q = 1; q < REC_NV; ++q)
-            tot[q] += __hip_atomic_load(&a.part[REC_NV * b + q], __ATOMIC_RELAXED,
+        for (int q = 1; q < NV; ++q)
+            tot[q] += __hip_atomic_load(&a.part[NV * b + q], __ATOMIC_RELAXED,
                                         __HIP_MEMORY_SCOPE_AGENT);
     }
     {
         double mx[1] = {tot[0]};
         block_max<1>(mx, lds);
         tot[0] = mx[0];
-        double sm[REC_NV - 1];
+        double sm[NV - 1];
 #pragma unroll
-        for (int q = 1; q < REC_NV; ++q) sm[q - 1] = tot[q];
-        block_sum<REC_NV - 1>(sm, lds);
+        for (int q = 1; q < NV; ++q) sm[q - 1] = tot[q];
+        block_sum<NV - 1>(sm, lds);
 #pragma unroll
-        for (int q = 1; q < REC_NV; ++q) tot[q] = sm[q - 1];
+        for (int q = 1; q < NV; ++q) tot[q] = sm[q - 1];
     }
     if (threadIdx.x != 0) return;
     *a.ticket = 0u;
@@ -933,7 +936,7 @@ __global__ void k_record(RecordArgs a, Geo g) {
         r.cg_info = st->cg_info;
         r.cg_outer = st->cg_outer;
         r.nmon = 10 + 5 * a.nscalar;
-        for (int q = 0; q < CFD_MAX_MON; ++q) r.mon[q] = q < REC_NV ? tot[q] : 0.0;
+        for (int q = 0; q < CFD_MAX_MON; ++q) r.mon[q] = q < NV ? tot[q] : 0.0;
         r.mon[REC_NV] = st->nut_max;   // turbulence.py:158-159, 322-324
         r.mon[REC_NV + 1] = st->flow_force;   // solver.py:439-440
         r.mon[REC_NV + 2] = st->vof_sum;      // vof.py:264-281: alpha sum, front, height
