@@ -40,7 +40,6 @@ def test_vof_advect_conserves_in_a_closed_periodic_box():
     a = np.zeros((nx + 2, ny + 2))
     a[1:-1, 1:-1] = 0.5 + 0.3 * np.sin(2 * np.pi * X) * np.sin(np.pi * Y)
     a[0], a[-1] = a[nx], a[1]
-    xs = np.arange(nx + 1) / nx
     ys = (np.arange(ny + 2) - 0.5) / ny
     u = np.sin(np.pi * np.clip(ys, 0, 1))[None, :] * np.ones((nx + 1, 1)) * 0.7
     u[:, 0] = u[:, -1] = 0.0
@@ -48,7 +47,6 @@ def test_vof_advect_conserves_in_a_closed_periodic_box():
     out = np.zeros_like(a)
     O.vof_advect(a, u, v, g.fluid, nx, ny, 1.0 / nx, 1.0 / ny, 0.01, 1.0, True, 0.0, out)
     assert abs(out[1:-1, 1:-1].sum() - a[1:-1, 1:-1].sum()) < 1e-12
-    del xs
 
 
 def test_dam_break(gold):
