@@ -90,7 +90,7 @@ class TurbDesc(ctypes.Structure):
 class VofDesc(ctypes.Structure):
     _fields_ = [("rho1", _d), ("rho2", _d), ("mu1", _d), ("mu2", _d), ("sigma", _d),
                 ("c_alpha", _d), ("cfl_alpha", _d), ("alpha_inflow", _d), ("alpha0", _vp),
-                ("npinned", _i32), ("pad", _i32), ("pinned", _vp)]
+                ("npinned", _i32), ("jacobi_only", _i32), ("pinned", _vp)]
 
 
 class LearnedStats(ctypes.Structure):

@@ -28,7 +28,8 @@ class VOF(Model):
 
     def __init__(self, alpha0, rho1: float = 1000.0, rho2: float = 1.0, nu1: float = 1e-6,
                  nu2: float = 1.48e-5, sigma: float = 0.07, gravity=(0.0, -9.81),
-                 c_alpha: float = 1.0, cfl_alpha: float = 0.25, alpha_inflow: float = 0.0):
+                 c_alpha: float = 1.0, cfl_alpha: float = 0.25, alpha_inflow: float = 0.0,
+                 multigrid: bool = True):
         self.alpha0 = np.asarray(alpha0, dtype=float)
         self.rho1, self.rho2 = float(rho1), float(rho2)
         self.mu1, self.mu2 = float(nu1 * rho1), float(nu2 * rho2)
@@ -37,6 +38,7 @@ class VOF(Model):
         self.c_alpha = float(c_alpha)
         self.cfl = float(cfl_alpha)
         self.alpha_inflow = float(alpha_inflow)
+        self.multigrid = bool(multigrid)   # preconditioner of the variable-density CG
         self.title = (f"VOF zweiphasig: rho1 = {rho1:g}, rho2 = {rho2:g}, sigma = {sigma:g}, "
                       f"g = {self.gravity}, c_alpha = {c_alpha:g}")
 
@@ -53,6 +55,7 @@ class VOF(Model):
         d.alpha_inflow = self.alpha_inflow
         d.alpha0 = _lib.ptr(a0)
         d.npinned = len(sim.pressure._pinned)
+        d.jacobi_only = 0 if self.multigrid else 1
         d.pinned = _lib.ptr(pinned) if d.npinned else None
         sim._flush()
         sim._ctx.call("cfd_set_vof", ctypes.byref(d))

@@ -114,6 +114,7 @@ struct cfd_ctx {
         int on = 0;
         double *alpha[2] = {nullptr, nullptr}, *rho = nullptr, *mu = nullptr;
         double *kappa = nullptr, *cE = nullptr, *cN = nullptr, *diag = nullptr;
+        VMG mg;                  // multigrid levels of the variable solve (nlev 0: Jacobi)
         int* pinned = nullptr;
         int npinned = 0;
         double rho1 = 1, rho2 = 1, mu1 = 0, mu2 = 0, sigma = 0, c_alpha = 0, cfl = 0.25, a_in = 0;
@@ -410,6 +411,11 @@ int launch_coop_var(cfd_ctx* c, ProjArgs a) {
     va.diag = c->vof.diag;
     va.pinned = c->vof.pinned;
     va.npinned = c->vof.npinned;
+    va.mg = c->vof.mg;
+    va.mg.lev[0].cE = c->vof.cE;
+    va.mg.lev[0].cN = c->vof.cN;
+    va.mg.lev[0].diag = c->vof.diag;
+    va.mg.lev[0].b = a.r;
     Geo g = c->geo;
     void* args[] = {(void*)&va, (void*)&g};
     HIPC(hipLaunchCooperativeKernel((const void*)k_project_var, dim3(c->proj_blocks),
@@ -1987,6 +1993,35 @@ int cfd_set_vof(cfd_ctx* c, const cfd_vof_desc* d) {
     TRY(dalloc(c, &c->vof.pinned, pin.size()));
     HIPC(hipMemcpy(c->vof.pinned, pin.data(), pin.size() * sizeof(int), hipMemcpyHostToDevice));
     c->vof.npinned = d->npinned;
+    // multigrid levels: 2x2 aggregation (2x1 once a side is one cell) down to
+    // at most VMG_COARSE cells; a grid too large for VMG_MAX levels keeps the
+    // Jacobi preconditioner
+    std::memset(&c->vof.mg, 0, sizeof(c->vof.mg));
+    {
+        VMG& mg = c->vof.mg;
+        mg.lev[0].nx = nx;
+        mg.lev[0].ny = ny;
+        mg.lev[0].P = P;
+        TRY(dalloc(c, &mg.lev[0].x, N));
+        TRY(dalloc(c, &mg.lev[0].t, N));
+        int L = 1;
+        while (L < VMG_MAX && (long)mg.lev[L - 1].nx * mg.lev[L - 1].ny > VMG_COARSE) {
+            VLev& V = mg.lev[L];
+            V.nx = (mg.lev[L - 1].nx + 1) / 2;
+            V.ny = (mg.lev[L - 1].ny + 1) / 2;
+            V.P = V.ny + 2;
+            const size_t M = (size_t)(V.nx + 2) * V.P;
+            TRY(dalloc(c, &V.cE, M));
+            TRY(dalloc(c, &V.cN, M));
+            TRY(dalloc(c, &V.diag, M));
+            TRY(dalloc(c, &V.x, M));
+            TRY(dalloc(c, &V.t, M));
+            TRY(dalloc(c, &V.b, M));
+            ++L;
+        }
+        const bool fits = (long)mg.lev[L - 1].nx * mg.lev[L - 1].ny <= VMG_COARSE;
+        mg.nlev = (L >= 2 && fits && !d->jacobi_only) ? L : 0;
+    }
     c->vof.rho1 = d->rho1;
     c->vof.rho2 = d->rho2;
     c->vof.mu1 = d->mu1;

@@ -309,6 +309,25 @@ __global__ void __launch_bounds__(1024) k_vof_monitors(VofArgs a, Geo g) {
 }
 
 // ---- variable-density pressure solve ----------------------------------------
+// Aggregation multigrid for the variable-coefficient system, rebuilt every
+// solve: level l+1 is 0.5 P^T A_l P for 2x2-cell aggregates (coupling of two
+// aggregates = half the fine couplings between them; diagonal = half the
+// fine diagonals minus twice the couplings inside), damped Jacobi (0.8) two
+// sweeps before and after, an exact solve on the coarsest level of at most
+// VMG_COARSE cells.  The V-cycle is symmetric, so it preconditions the CG
+// directly.
+constexpr int VMG_MAX = 12;
+constexpr int VMG_COARSE = 64;
+struct VLev {
+    int nx, ny, P;           // cells, pitch of the padded (nx+2) x (ny+2) arrays
+    double *cE, *cN, *diag;  // operator
+    double *x, *t, *b;       // two iterates and the right-hand side
+};
+struct VMG {
+    int nlev;                // 0: plain Jacobi-PCG
+    VLev lev[VMG_MAX];       // lev[0] is the fine grid (x, t work vectors, b = r)
+};
+
 struct VarProjArgs {
     ProjArgs p;              // F, G, u, v, p, code, comp, comp_count, ncomp, x, b, r, q, d0,
                              // sync, st, rtol, maxiter, dt_override
@@ -318,6 +337,7 @@ struct VarProjArgs {
     double* diag;
     const int* pinned;       // cell offsets pinned in pure-Neumann components
     int npinned;
+    VMG mg;
 };
 
 __device__ __forceinline__ double bx_face(const double* rho, const Geo& g, int f, int j) {
@@ -334,6 +354,8 @@ __device__ __forceinline__ double by_face(const double* rho, const Geo& g, int i
 
 __global__ void __launch_bounds__(PROJ_BS) k_project_var(VarProjArgs va, Geo g) {
     __shared__ double lds[CFD_MAX_COMP * (PROJ_BS / 64) + 16];
+    __shared__ double cinv[VMG_COARSE * VMG_COARSE];   // inverse of the coarsest operator
+    __shared__ double cb[VMG_COARSE];
     const ProjArgs& a = va.p;
     DevState* st = a.st;
     if (st->done) return;
@@ -450,6 +472,238 @@ __global__ void __launch_bounds__(PROJ_BS) k_project_var(VarProjArgs va, Geo g) 
             if (fluid(i, j)) a.x[ix(i, j, pc)] = 0.0;
         }
         rr = 0.0;
+    } else if (va.mg.nlev >= 2) {
+        // ---- multigrid-preconditioned CG ----
+        const VMG& mg = va.mg;
+        const int L = mg.nlev;
+        constexpr double om = 0.8;
+        // a level of at most four cells per thread of one block is handled by
+        // block 0 alone, behind block barriers instead of grid barriers
+        auto is_small = [&](int l) -> bool { return (long)mg.lev[l].nx * mg.lev[l].ny <= 4L * BS; };
+        auto lsync = [&](bool small) {
+            if (small) __syncthreads();
+            else grid_sync(gs, lds, epoch);
+        };
+        // body(i, j, c) for the cells of level l; consecutive threads take
+        // consecutive j, the contiguous index
+        auto each = [&](int l, auto&& body) {
+            const VLev& V = mg.lev[l];
+            const int n = V.nx * V.ny, vny = V.ny;
+            int q = tid, qs = BS;
+            if (is_small(l)) {
+                if (blockIdx.x != 0) return;
+            } else {
+                q += blockIdx.x * BS;
+                qs *= gridDim.x;
+            }
+            for (; q < n; q += qs) {
+                const int i = q / vny, j = q - i * vny;
+                body(i + 1, j + 1, ix(i + 1, j + 1, V.P));
+            }
+        };
+        // (A xv)(i, j) on level V, xv(i, j, c) giving the iterate; couplings
+        // that are zero do not read their neighbour
+        auto Aop = [&](const VLev& V, int i, int j, long c, auto&& xv) -> double {
+            const int iE = (periodic && i == V.nx) ? 1 : i + 1;
+            const int iW = (periodic && i == 1) ? V.nx : i - 1;
+            const double ce = V.cE[c];
+            const double cw = (periodic && i == 1) ? V.cE[ix(V.nx, j, V.P)]
+                                                   : (i > 1 ? V.cE[c - V.P] : 0.0);
+            const double cn = V.cN[c], cs = j > 1 ? V.cN[c - 1] : 0.0;
+            double r = V.diag[c] * xv(i, j, c);
+            if (ce != 0.0) r -= ce * xv(iE, j, ix(iE, j, V.P));
+            if (cw != 0.0) r -= cw * xv(iW, j, ix(iW, j, V.P));
+            if (cn != 0.0) r -= cn * xv(i, j + 1, c + 1);
+            if (cs != 0.0) r -= cs * xv(i, j - 1, c - 1);
+            return r;
+        };
+        // coarse operators
+        for (int l = 0; l + 1 < L; ++l) {
+            const VLev& F = mg.lev[l];
+            const VLev& C = mg.lev[l + 1];
+            each(l + 1, [&](int i, int j, long c) {
+                const int i0 = 2 * i - 1, j0 = 2 * j - 1;
+                const bool hi = 2 * i <= F.nx, hj = 2 * j <= F.ny;
+                const int i1 = hi ? i0 + 1 : i0, j1 = hj ? j0 + 1 : j0;
+                auto fe = [&](int fi, int fj) { return F.cE[ix(fi, fj, F.P)]; };
+                auto fn = [&](int fi, int fj) { return F.cN[ix(fi, fj, F.P)]; };
+                auto fd = [&](int fi, int fj) { return F.diag[ix(fi, fj, F.P)]; };
+                double ds = fd(i0, j0), inner = 0.0;
+                if (hi) { ds += fd(i1, j0); inner += fe(i0, j0); }
+                if (hj) { ds += fd(i0, j1); inner += fn(i0, j0); }
+                if (hi && hj) { ds += fd(i1, j1); inner += fe(i0, j1) + fn(i1, j0); }
+                C.diag[c] = 0.5 * (ds - 2.0 * inner);
+                C.cE[c] = 0.5 * (fe(i1, j0) + (hj ? fe(i1, j1) : 0.0));
+                C.cN[c] = 0.5 * (fn(i0, j1) + (hi ? fn(i1, j1) : 0.0));
+            });
+            lsync(is_small(l + 1) && (l + 2 >= L || is_small(l + 2)));
+        }
+        grid_sync(gs, lds, epoch);
+        // the coarsest level (at most VMG_COARSE cells) is solved exactly: its
+        // dense matrix, in the cell order q = (i-1) ny + j-1, is inverted in
+        // LDS by Gauss-Jordan elimination without pivoting (the matrix is
+        // symmetric positive definite; all-solid aggregates get a unit row)
+        const VLev& Vc = mg.lev[L - 1];
+        const int nco = Vc.nx * Vc.ny;
+        if (blockIdx.x == 0) {
+            for (int e = tid; e < VMG_COARSE * VMG_COARSE; e += BS) cinv[e] = 0.0;
+            __syncthreads();
+            for (int q = tid; q < nco; q += BS) {
+                const int i = 1 + q / Vc.ny, j = 1 + q % Vc.ny;
+                const long c = ix(i, j, Vc.P);
+                double* row = cinv + q * VMG_COARSE;
+                if (!(Vc.diag[c] > 0.0)) {
+                    row[q] = 1.0;
+                    continue;
+                }
+                const int iE = (periodic && i == Vc.nx) ? 1 : i + 1;
+                const int iW = (periodic && i == 1) ? Vc.nx : i - 1;
+                const double cw = (periodic && i == 1) ? Vc.cE[ix(Vc.nx, j, Vc.P)]
+                                                       : (i > 1 ? Vc.cE[c - Vc.P] : 0.0);
+                row[q] += Vc.diag[c];
+                if (Vc.cE[c] != 0.0) row[(iE - 1) * Vc.ny + j - 1] -= Vc.cE[c];
+                if (cw != 0.0) row[(iW - 1) * Vc.ny + j - 1] -= cw;
+                if (Vc.cN[c] != 0.0) row[q + 1] -= Vc.cN[c];
+                if (j > 1 && Vc.cN[c - 1] != 0.0) row[q - 1] -= Vc.cN[c - 1];
+            }
+            __syncthreads();
+            for (int k = 0; k < nco; ++k) {
+                const double piv = cinv[k * VMG_COARSE + k];
+                for (int e = tid; e < nco * VMG_COARSE; e += BS) {
+                    const int r = e / VMG_COARSE, s = e % VMG_COARSE;
+                    if (r != k && s < nco && s != k)
+                        cinv[e] -= cinv[r * VMG_COARSE + k] * cinv[k * VMG_COARSE + s] / piv;
+                }
+                __syncthreads();
+                for (int e = tid; e < nco; e += BS) {
+                    if (e == k) continue;
+                    cinv[e * VMG_COARSE + k] = -cinv[e * VMG_COARSE + k] / piv;
+                    cinv[k * VMG_COARSE + e] = cinv[k * VMG_COARSE + e] / piv;
+                }
+                if (tid == 0) cinv[k * VMG_COARSE + k] = 1.0 / piv;
+                __syncthreads();
+            }
+        }
+        // z = M r by one V-cycle from zero; z is lev[0].t and the return value
+        // is this thread's share of r . z
+        auto vcycle = [&]() -> double {
+            for (int l = 0; l + 1 < L; ++l) {
+                const VLev& V = mg.lev[l];
+                const VLev& C = mg.lev[l + 1];
+                if (l > 0) lsync(is_small(l));   // b of level l complete
+                // two damped Jacobi sweeps from zero, the first formed for
+                // the neighbours on the fly
+                auto x1 = [&](int, int, long cc) -> double {
+                    const double d = V.diag[cc];
+                    return d > 0.0 ? om * V.b[cc] / d : 0.0;
+                };
+                each(l, [&](int i, int j, long c) {
+                    const double d = V.diag[c];
+                    V.t[c] = d > 0.0 ? x1(i, j, c) + om * (V.b[c] - Aop(V, i, j, c, x1)) / d : 0.0;
+                });
+                lsync(is_small(l) && is_small(l + 1));   // t of level l complete
+                auto tv = [&](int, int, long cc) -> double { return V.t[cc]; };
+                each(l + 1, [&](int i, int j, long c) {   // restriction of the residual
+                    const int i0 = 2 * i - 1, j0 = 2 * j - 1;
+                    double s = 0.0;
+                    for (int di = 0; di < 2; ++di)
+                        for (int dj = 0; dj < 2; ++dj) {
+                            const int fi = i0 + di, fj = j0 + dj;
+                            if (fi > V.nx || fj > V.ny) continue;
+                            const long fc = ix(fi, fj, V.P);
+                            if (V.diag[fc] > 0.0) s += V.b[fc] - Aop(V, fi, fj, fc, tv);
+                        }
+                    C.b[c] = s;
+                });
+            }
+            lsync(true);   // b of the coarsest level complete (block 0 wrote it)
+            if (blockIdx.x == 0) {
+                for (int q = tid; q < nco; q += BS) cb[q] = Vc.b[ix(1 + q / Vc.ny, 1 + q % Vc.ny, Vc.P)];
+                __syncthreads();
+                for (int q = tid; q < nco; q += BS) {
+                    double s = 0.0;
+                    for (int k = 0; k < nco; ++k) s += cinv[k * VMG_COARSE + q] * cb[k];
+                    const long c = ix(1 + q / Vc.ny, 1 + q % Vc.ny, Vc.P);
+                    Vc.t[c] = Vc.diag[c] > 0.0 ? s : 0.0;
+                }
+            }
+            double s1 = 0.0;
+            for (int l = L - 2; l >= 0; --l) {
+                const VLev& V = mg.lev[l];
+                const VLev& C = mg.lev[l + 1];
+                lsync(is_small(l) && is_small(l + 1));   // t of level l+1 complete
+                // prolongation (add the aggregate's correction) fused with the
+                // first of two damped Jacobi sweeps
+                auto tp = [&](int ii, int jj, long cc) -> double {
+                    return V.diag[cc] > 0.0 ? V.t[cc] + C.t[ix((ii + 1) / 2, (jj + 1) / 2, C.P)] : 0.0;
+                };
+                each(l, [&](int i, int j, long c) {
+                    const double d = V.diag[c];
+                    V.x[c] = d > 0.0 ? tp(i, j, c) + om * (V.b[c] - Aop(V, i, j, c, tp)) / d : 0.0;
+                });
+                lsync(is_small(l));
+                auto xv = [&](int, int, long cc) -> double { return V.x[cc]; };
+                each(l, [&](int i, int j, long c) {
+                    const double d = V.diag[c];
+                    const double z = d > 0.0 ? V.x[c] + om * (V.b[c] - Aop(V, i, j, c, xv)) / d : 0.0;
+                    V.t[c] = z;
+                    if (l == 0) s1 += V.b[c] * z;
+                });
+            }
+            return s1;
+        };
+        const double* z = mg.lev[0].t;
+        bool conv = false;
+        double rz_prev = 1.0;
+        rz = 0.0;
+        for (int k = 0; k < a.maxiter; ++k) {
+            if (sqrt(rr) < atol || !(rr == rr)) {
+                conv = sqrt(rr) < atol;
+                break;
+            }
+            double s1[1] = {vcycle()};
+            grid_sum<1>(s1, gs, lds, epoch);   // also completes z
+            rz_prev = rz;
+            rz = s1[0];
+            const double beta = k > 0 ? rz / rz_prev : 0.0;
+            double* dnew = (k & 1) ? a.d1 : a.d0;
+            const double* dold = (k & 1) ? a.d0 : a.d1;
+            auto dirv = [&](long cc) -> double { return k > 0 ? z[cc] + beta * dold[cc] : z[cc]; };
+            double dq[1] = {0.0};
+            FOR_CELLS {
+                const long c = ix(i, j, pc);
+                if (!fluid(i, j)) continue;
+                const int iE = (periodic && i == nx) ? 1 : i + 1;
+                const int iW = (periodic && i == 1) ? nx : i - 1;
+                const double ce = va.cE[c];
+                const double cw = (periodic && i == 1) ? va.cE[ix(nx, j, pc)] : cW(i, j);
+                const double cn = va.cN[c], csv = j > 1 ? va.cN[c - 1] : 0.0;
+                const double dc = dirv(c);
+                double q = va.diag[c] * dc;
+                if (ce != 0.0) q -= ce * dirv(ix(iE, j, pc));
+                if (cw != 0.0) q -= cw * dirv(ix(iW, j, pc));
+                if (cn != 0.0) q -= cn * dirv(c + 1);
+                if (csv != 0.0) q -= csv * dirv(c - 1);
+                dnew[c] = dc;
+                a.q[c] = q;
+                dq[0] += dc * q;
+            }
+            grid_sum<1>(dq, gs, lds, epoch);
+            const double alpha = rz / dq[0];
+            double s2[1] = {0.0};
+            FOR_CELLS {
+                const long c = ix(i, j, pc);
+                if (!fluid(i, j)) continue;
+                a.x[c] += alpha * dnew[c];
+                const double rv = a.r[c] - alpha * a.q[c];
+                a.r[c] = rv;
+                s2[0] += rv * rv;
+            }
+            grid_sum<1>(s2, gs, lds, epoch);
+            rr = s2[0];
+            ++iters;
+        }
+        if (!conv) info = iters > 0 ? iters : a.maxiter;
     } else {
         bool conv = false;
         double rz_prev = 1.0;

@@ -355,14 +355,15 @@ int cfd_set_turbulence(cfd_ctx* ctx, const cfd_turb_desc* desc);
 /* Two-phase VOF (vof.py:136-306, cases.py:294-320); no counterpart in a C
    interface of the reference.  alpha0 is nx x ny (clipped to [0, 1], 0 off
    the fluid); the pressure system becomes div((1/rho) grad p) = div(u*)/dt,
-   solved by a Jacobi-PCG to 1e-12 (the reference solves it directly) with
-   the pure-Neumann components' pinned cells (i, j) from the pressure set-up.
+   solved by CG to 1e-12 (the reference solves it directly) with
+   the pure-Neumann components' pinned cells (i, j) from the pressure set-up;
+   the CG is preconditioned by an aggregation multigrid rebuilt every solve.
    Gravity is the caller's body force; sigma > 0 adds CSF surface tension. */
 typedef struct cfd_vof_desc {
     double rho1, rho2, mu1, mu2, sigma, c_alpha, cfl_alpha, alpha_inflow;
     const double* alpha0;
     int32_t npinned;
-    int32_t pad;
+    int32_t jacobi_only;             /* 1: Jacobi-PCG instead of the multigrid-preconditioned CG */
     const int32_t* pinned;
 } cfd_vof_desc;
 int cfd_set_vof(cfd_ctx* ctx, const cfd_vof_desc* desc);
