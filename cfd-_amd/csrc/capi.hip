@@ -117,6 +117,9 @@ struct cfd_ctx {
         VMG mg;                  // multigrid levels of the variable solve (nlev 0: Jacobi)
         int* pinned = nullptr;
         int npinned = 0;
+        double* part = nullptr;              // monitors: 3 partials per block
+        unsigned* ticket = nullptr;          // last-block tickets of k_vof_nsub, k_vof_monitors
+        unsigned long long* acc = nullptr;   // max |u|, max |v|
         double rho1 = 1, rho2 = 1, mu1 = 0, mu2 = 0, sigma = 0, c_alpha = 0, cfl = 0.25, a_in = 0;
     } vof;
     // turbulence model (turb.hip); cell arrays (nx+2) x P
@@ -393,6 +396,9 @@ VofArgs vof_args(cfd_ctx* c, const double* in, double* out, int sub) {
     a.a_in = c->vof.a_in;
     a.sub = sub;
     a.nsub_max = VOF_NSUB_MAX;
+    a.part = c->vof.part;
+    a.ticket = c->vof.ticket;
+    a.acc = c->vof.acc;
     return a;
 }
 
@@ -579,7 +585,7 @@ int launch_vof_fill(cfd_ctx* c, int k, int sub) {
 // alpha transport after the step (vof.py:243-254) and its monitors
 int launch_vof_post(cfd_ctx* c) {
     VofArgs a0 = vof_args(c, c->vof.alpha[0], c->vof.alpha[1], 0);
-    hipLaunchKernelGGL(k_vof_nsub, dim3(1), dim3(1024), 0, c->stream, a0, c->geo);
+    hipLaunchKernelGGL(k_vof_nsub, dim3(c->red_blocks), dim3(256), 0, c->stream, a0, c->geo);
     HIPC(hipGetLastError());
     for (int s = 0; s < VOF_NSUB_MAX; ++s) {
         VofArgs a = vof_args(c, c->vof.alpha[s & 1], c->vof.alpha[(s & 1) ^ 1], s);
@@ -591,7 +597,7 @@ int launch_vof_post(cfd_ctx* c) {
     // ghost next to a solid edge cell then takes that cell's new value
     TRY(launch_vof_fill(c, 0, -1));
     VofArgs m = vof_args(c, c->vof.alpha[0], c->vof.alpha[0], -1);
-    hipLaunchKernelGGL(k_vof_monitors, dim3(1), dim3(1024), 0, c->stream, m, c->geo);
+    hipLaunchKernelGGL(k_vof_monitors, dim3(c->red_blocks), dim3(256), 0, c->stream, m, c->geo);
     HIPC(hipGetLastError());
     return 0;
 }
@@ -1991,6 +1997,9 @@ int cfd_set_vof(cfd_ctx* c, const cfd_vof_desc* d) {
         pin[k] = (pi + 1) * P + pj + 1;
     }
     TRY(dalloc(c, &c->vof.pinned, pin.size()));
+    TRY(dalloc(c, &c->vof.part, (size_t)3 * c->red_blocks));
+    TRY(dalloc(c, &c->vof.ticket, 2));
+    TRY(dalloc(c, &c->vof.acc, 2));
     HIPC(hipMemcpy(c->vof.pinned, pin.data(), pin.size() * sizeof(int), hipMemcpyHostToDevice));
     c->vof.npinned = d->npinned;
     // multigrid levels: 2x2 aggregation (2x1 once a side is one cell) down to
@@ -2033,7 +2042,7 @@ int cfd_set_vof(cfd_ctx* c, const cfd_vof_desc* d) {
     c->vof.on = 1;
     TRY(launch_vof_fill(c, 0, -1));
     VofArgs m = vof_args(c, c->vof.alpha[0], c->vof.alpha[0], -1);
-    hipLaunchKernelGGL(k_vof_monitors, dim3(1), dim3(1024), 0, c->stream, m, c->geo);
+    hipLaunchKernelGGL(k_vof_monitors, dim3(c->red_blocks), dim3(256), 0, c->stream, m, c->geo);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(c->stream));
     return 0;

@@ -9,7 +9,8 @@
 //                     CSF force divided by the face density
 //   k_project_var     cooperative Jacobi-PCG on the variable-coefficient
 //                     system and the correction u = F - dt (1/rho_f) grad p
-//   k_vof_nsub        sub-step count of the alpha transport (vof.py:243-248)
+//   k_vof_nsub        sub-step count of the alpha transport (vof.py:243-248),
+//                     a grid-wide maximum finished by the last block
 //   k_vof_advect      one alpha step in gather form: every cell adds its
 //                     faces' fluxes in the order the reference's face loops
 //                     reach it, then clips to [0, 1] (vof.py:34-133)
@@ -35,6 +36,9 @@ struct VofArgs {
     double c_alpha, cfl, a_in;
     int sub;                 // k_vof_advect: this launch's sub-step index
     int nsub_max;            // sub-step launches per step
+    double* part;            // per-block partials of the monitors (3 per block)
+    unsigned* ticket;        // [0] k_vof_nsub, [1] k_vof_monitors
+    unsigned long long* acc; // max |u|, max |v| (bits of non-negative doubles)
 };
 
 // ---- surface tension: curvature of the smoothed alpha ----------------------
@@ -145,23 +149,38 @@ __device__ double vof_yflux(const VofArgs& a, const Geo& g, int i, int B) {
 
 // sub-step count from the corrected velocity: max |u| / dx, |v| / dy over all
 // faces (one work-group; tiny next to the step)
-__global__ void __launch_bounds__(1024) k_vof_nsub(VofArgs a, Geo g) {
-    __shared__ double lds[16];
+__global__ void __launch_bounds__(256) k_vof_nsub(VofArgs a, Geo g) {
+    __shared__ double lds[2 * 16];
     if (a.st->done) return;
-    double m[1] = {0.0};
     const long nu = (long)(g.nx + 1) * g.pu, nv = (long)(g.nx + 2) * g.pv;
-    double mu = 0.0, mv = 0.0;
-    for (long k = threadIdx.x; k < nu; k += blockDim.x)
-        if (k % g.pu <= g.ny + 1) mu = nanmax(mu, fabs(a.u[k]));
-    for (long k = threadIdx.x; k < nv; k += blockDim.x)
-        if (k % g.pv <= g.ny) mv = nanmax(mv, fabs(a.v[k]));
-    m[0] = mu;
-    block_max<1>(m, lds);
-    const double umax_u = m[0];
-    m[0] = mv;
-    block_max<1>(m, lds);
-    if (threadIdx.x != 0) return;
-    const double rate = fmax(fmax(umax_u / g.dx, m[0] / g.dy), 1e-12);
+    const long k0 = (long)blockIdx.x * blockDim.x + threadIdx.x, ks = (long)gridDim.x * blockDim.x;
+    double m[2] = {0.0, 0.0};
+    for (long k = k0; k < nu; k += ks)
+        if (k % g.pu <= g.ny + 1) m[0] = nanmax(m[0], fabs(a.u[k]));
+    for (long k = k0; k < nv; k += ks)
+        if (k % g.pv <= g.ny) m[1] = nanmax(m[1], fabs(a.v[k]));
+    block_max<2>(m, lds);
+    __shared__ int s_last;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+            __hip_atomic_fetch_max(&a.acc[q], (unsigned long long)__double_as_longlong(m[q]),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned tk =
+            __hip_atomic_fetch_add(&a.ticket[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = tk == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last || threadIdx.x != 0) return;
+    double mx[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        mx[q] = __longlong_as_double(
+            (long long)__hip_atomic_load(&a.acc[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        a.acc[q] = 0ull;
+    }
+    a.ticket[0] = 0u;
+    const double rate = fmax(fmax(mx[0] / g.dx, mx[1] / g.dy), 1e-12);
     const double dt = a.st->dt;
     const double nsub = fmax(1.0, ceil(dt * rate / a.cfl));
     a.st->vof_nsub = nsub;
@@ -287,25 +306,49 @@ __global__ void __launch_bounds__(256) k_vof_solids(VofArgs a, Geo g) {
 
 // phase volume sum over fluid cells, front (last cell of the bottom row with
 // alpha > 0.5) and height (last cell of the left column): one work-group
-__global__ void __launch_bounds__(1024) k_vof_monitors(VofArgs a, Geo g) {
+__global__ void __launch_bounds__(256) k_vof_monitors(VofArgs a, Geo g) {
     __shared__ double lds[3 * 16];
     if (a.st->done) return;
     const int nx = g.nx, ny = g.ny, P = g.pc;
+    const int r0 = 1 + (int)(((long)nx * blockIdx.x) / gridDim.x);
+    const int r1 = 1 + (int)(((long)nx * (blockIdx.x + 1)) / gridDim.x);
     double s[1] = {0.0};
     double m[2] = {-1.0, -1.0};
-    for (long k = threadIdx.x; k < (long)nx * ny; k += blockDim.x) {
-        const int i = 1 + (int)(k / ny), j = 1 + (int)(k % ny);
-        const long c = ix(i, j, P);
-        if (a.fl[c] & 1) s[0] += a.out[c];
-        if (j == 1 && a.out[c] > 0.5) m[0] = fmax(m[0], (double)i);
-        if (i == 1 && a.out[c] > 0.5) m[1] = fmax(m[1], (double)j);
-    }
+    for (int i = r0; i < r1; ++i)
+        for (int j = 1 + threadIdx.x; j <= ny; j += blockDim.x) {
+            const long c = ix(i, j, P);
+            if (a.fl[c] & 1) s[0] += a.out[c];
+            if (j == 1 && a.out[c] > 0.5) m[0] = fmax(m[0], (double)i);
+            if (i == 1 && a.out[c] > 0.5) m[1] = fmax(m[1], (double)j);
+        }
     block_sum<1>(s, lds);
     block_max<2>(m, lds);
+    __shared__ int s_last;
+    if (threadIdx.x == 0) {
+        a.part[3 * blockIdx.x] = s[0];
+        a.part[3 * blockIdx.x + 1] = m[0];
+        a.part[3 * blockIdx.x + 2] = m[1];
+        const unsigned tk =
+            __hip_atomic_fetch_add(&a.ticket[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = tk == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    // the last block adds the partials in a fixed order
+    double t[1] = {0.0};
+    double n[2] = {-1.0, -1.0};
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+        t[0] += __hip_atomic_load(&a.part[3 * b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        n[0] = fmax(n[0], __hip_atomic_load(&a.part[3 * b + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        n[1] = fmax(n[1], __hip_atomic_load(&a.part[3 * b + 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    block_sum<1>(t, lds);
+    block_max<2>(n, lds);
     if (threadIdx.x != 0) return;
-    a.st->vof_sum = s[0];
-    a.st->vof_front = m[0];
-    a.st->vof_height = m[1];
+    a.ticket[1] = 0u;
+    a.st->vof_sum = t[0];
+    a.st->vof_front = n[0];
+    a.st->vof_height = n[1];
 }
 
 // ---- variable-density pressure solve ----------------------------------------

@@ -80,3 +80,47 @@ def test_reference_dam_break_conserves_volume(gpu):
     assert a.min() >= 0 and a.max() <= 1
     assert case.sim.history.extra["front_x"][-1] > case.info["a"] * 1.3
     assert case.sim.max_divergence() < 1e-8
+
+
+def _slosh(G, BC, **kw):
+    g = G(50, 30, 1.0, 0.6)
+    X, Y = g.cell_centers()
+    g.add_obstacle((X - 0.5) ** 2 + (Y - 0.18) ** 2 < 0.08 ** 2)
+    alpha0 = (Y < 0.3 + 0.05 * np.sin(2 * np.pi * X)).astype(float)
+    bcs = dict(left=BC("periodic"), right=BC("periodic"), bottom=BC("noslip"), top=BC("outflow"))
+    return g, alpha0, bcs
+
+
+@pytest.mark.parametrize("multigrid", [True, False])
+def test_periodic_slosh_matches_the_oracle(gpu, multigrid):
+    """A periodic two-layer slosh past a cylinder, against the oracle (sparse
+    direct solve of the variable-density system): the multigrid hierarchy over
+    odd level sizes (50x30 -> 25x15 -> 13x8 -> 7x4, the periodic wrap on every
+    level) and the Jacobi-PCG."""
+    from cfd2d_amd.boundary import BoundaryCondition as BC
+    from cfd2d_amd.grid import Grid
+    from cfd2d_amd.solver import Simulation
+    from cfd2d_amd.vof import VOF
+    from oracle import cfd_oracle as O
+    from oracle import vof_oracle as V
+
+    g, a0, bcs = _slosh(Grid, BC)
+    sim = Simulation(g, bcs, nu=1e-6, gamma=0.5,
+                     models=[VOF(a0, sigma=0.07, multigrid=multigrid)])
+    og, oa0, obcs = _slosh(O.OGrid, O.OBC)
+    osim = O.OSim(og, obcs, nu=1e-6, gamma=0.5, models=[V.OVOF(oa0, sigma=0.07)])
+    assert np.array_equal(sim.grid.fluid, osim.grid.fluid)
+    worst = {}
+    for _ in range(10):
+        sim.step()
+        osim.step()
+    m, om = sim.model("multiphase"), osim.models[-1]
+    for k, a, b in (("alpha", m.alpha, om.alpha), ("u", sim.u, osim.u), ("v", sim.v, osim.v),
+                    ("p", sim.p, osim.p)):
+        worst[k] = _rel(a, b)
+    print("periodic slosh, multigrid", multigrid, worst, "iterations", sim.cg_iterations,
+          flush=True)
+    # PCG to a relative residual of 1e-12 against the direct solve
+    assert max(worst.values()) < 1e-8
+    assert _rel(sim.history.dt, osim.hist["dt"]) < 1e-10
+    assert max(sim.cg_info) == 0
