@@ -81,6 +81,7 @@ struct cfd_ctx {
     uint8_t* open[4] = {nullptr, nullptr, nullptr, nullptr};
     double *sync = nullptr, *part = nullptr;
     unsigned *bar = nullptr, *ticket = nullptr;
+    unsigned* flags = nullptr;   // per-block publish flags of the one-pass reductions
     unsigned long long* acc = nullptr;
     DevState* st = nullptr;
     StepRecord* ring = nullptr;
@@ -167,11 +168,22 @@ struct cfd_ctx {
 
 namespace {
 
+// zeroed device array; the fill runs on the context stream and is complete on
+// return: a plain hipMemset goes to the null stream, which the context's
+// non-blocking stream does not wait for, so a kernel issued right after a
+// lazy allocation could see its output cleared behind it
 template <typename T>
 int dalloc(cfd_ctx* c, T** p, size_t n) {
     void* q = nullptr;
-    HIPC(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
-    HIPC(hipMemset(q, 0, std::max<size_t>(n, 1) * sizeof(T)));
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    HIPC(hipMalloc(&q, bytes));
+    if (c->stream) {
+        HIPC(hipMemsetAsync(q, 0, bytes, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+    } else {
+        HIPC(hipMemset(q, 0, bytes));
+        HIPC(hipDeviceSynchronize());
+    }
     c->allocs.push_back(q);
     *p = (T*)q;
     return 0;
@@ -967,7 +979,8 @@ int cfd_ctx_create(const cfd_grid_desc* d, cfd_ctx** out) {
     c->proj_blocks = std::max(1, std::min(c->num_cu, c->nx));
     c->red_blocks = std::max(1, std::min(c->num_cu, c->nx + 2));
     if ((rc = dalloc(c, &c->sync, (size_t)(c->proj_blocks + 2) * CFD_SLOT)) ||
-        (rc = dalloc(c, &c->part, (size_t)REC_NV * c->red_blocks)))
+        (rc = dalloc(c, &c->part, (size_t)REC_NV * c->red_blocks)) ||
+        (rc = dalloc(c, &c->flags, (size_t)c->red_blocks)))
         return fail(rc);
 
     // masks and tables
@@ -1505,7 +1518,7 @@ int cfd_step(cfd_ctx* c, const cfd_step_params* prm, int32_t count) {
                                 hipMemcpyDeviceToDevice, c->stream));
         }
         hipLaunchKernelGGL(k_dt_gamma, dim3(c->red_blocks), dim3(256), 0,
-                           c->stream, c->u, c->v, c->geo, sp, td, c->st, c->acc, c->ticket);
+                           c->stream, c->u, c->v, c->geo, sp, td, c->st, c->part, c->flags);
         HIPC(hipGetLastError());
         PredArgs pa = pred_args(c, prm, pk);
         hipLaunchKernelGGL(k_predictor, grid2d(c), dim3(BS2D), 0, c->stream, pa, c->geo);
@@ -1545,7 +1558,7 @@ int cfd_step(cfd_ctx* c, const cfd_step_params* prm, int32_t count) {
         }
         ra.st = c->st;
         ra.part = c->part;
-        ra.ticket = c->ticket + 1;
+        ra.flags = c->flags;
         ra.ring = c->ring;
         ra.ring_len = RING;
         ra.scheme_ab2 = prm->scheme_ab2;
@@ -1797,7 +1810,8 @@ int cfd_probes_set(cfd_ctx* c, const cfd_probe_desc* d) {
     release();   // a rejected descriptor leaves the previous probes in place
     auto alloc = [&](void** q, size_t bytes) -> int {
         HIPC(hipMalloc(q, std::max<size_t>(bytes, 8)));
-        HIPC(hipMemset(*q, 0, std::max<size_t>(bytes, 8)));
+        HIPC(hipMemsetAsync(*q, 0, std::max<size_t>(bytes, 8), c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
         c->probes.mem.push_back(*q);
         return 0;
     };
@@ -1856,9 +1870,10 @@ int cfd_favg_set(cfd_ctx* c, int32_t enable, double t_start) {
         TRY(dalloc(c, &c->favg.sum2, n));
         TRY(dalloc(c, &c->favg.T, 1));
     }
-    HIPC(hipMemset(c->favg.sum, 0, n * sizeof(double)));
-    HIPC(hipMemset(c->favg.sum2, 0, n * sizeof(double)));
-    HIPC(hipMemset(c->favg.T, 0, sizeof(double)));
+    // on the context stream, ordered before the accumulating kernels
+    HIPC(hipMemsetAsync(c->favg.sum, 0, n * sizeof(double), c->stream));
+    HIPC(hipMemsetAsync(c->favg.sum2, 0, n * sizeof(double), c->stream));
+    HIPC(hipMemsetAsync(c->favg.T, 0, sizeof(double), c->stream));
     if (!c->mon_last_n) TRY(dalloc(c, &c->mon_last_n, 1));
     DevState h;
     TRY(read_state(c, &h));
@@ -1949,8 +1964,6 @@ int cfd_set_turbulence(cfd_ctx* c, const cfd_turb_desc* d) {
         c->turb.sid_k = d->sid_k;
         c->turb.sid_w = d->sid_omega;
     }
-    const unsigned long long inf_bits = 0x7ff0000000000000ull;
-    HIPC(hipMemcpy(c->acc + 3, &inf_bits, sizeof(inf_bits), hipMemcpyHostToDevice));
     c->turb.model = d->model == CFD_TURB_SMAGORINSKY ? TURB_SMAGORINSKY : TURB_SST;
     c->turb.nu = d->nu;
     // the model's first evaluation at attach (turbulence.py:144, 239)

@@ -91,7 +91,7 @@ struct RecordArgs {
     const double* bv[CFD_MAX_SCALARS][4];
     DevState* st;
     double* part;
-    unsigned* ticket;
+    unsigned* flags;         // per-block publish flags (gather_block0)
     StepRecord* ring;
     int ring_len;
     int scheme_ab2;
@@ -198,7 +198,7 @@ __global__ void k_copy_predictor(PredArgs a, Geo g);
 __global__ void k_bc_domain(BCArgs a, Geo g);
 __global__ void k_obstacle_bc(double* u, double* v, const uint8_t* fl, Geo g, const DevState* st);
 __global__ void k_dt_gamma(const double* u, const double* v, Geo g, StepParams prm, TurbDt td,
-                           DevState* st, unsigned long long* acc, unsigned* ticket);
+                           DevState* st, double* part, unsigned* flags);
 template <bool CONST_D, int MODE>
 __global__ void k_scalar_transport(ScalarArgs a, Geo g);
 __global__ void k_fill_ghosts(ScalarArgs a, Geo g);

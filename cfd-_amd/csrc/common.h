@@ -10,6 +10,59 @@ __host__ __device__ __forceinline__ long ix(int i, int j, int pitch) {
     return (long)i * pitch + j;
 }
 
+// One-pass reduction over the w-wide block of n (< 2^31) entries starting at
+// (i0, j0), spread over the whole grid, U entries per thread at a time:
+// load(i, j) reads an entry's values (in-range indices: past the end the last
+// entry is read again), then acc(values, ok) folds them in, ok false past the
+// end.  The empty asm statement between the phases keeps every load of a
+// batch ahead of the first use, so the U entries' loads are in flight
+// together instead of one HBM round trip after the other (the compiler turns
+// the guarded accumulations into branches and does not move loads across
+// them).
+template <int U, typename L, typename A>
+__device__ __forceinline__ void flat_reduce(int n, int w, int i0, int j0, L load, A acc) {
+    using T = decltype(load(0, 0));
+    const int stride = gridDim.x * blockDim.x;
+    for (int k0 = blockIdx.x * blockDim.x + threadIdx.x; k0 < n; k0 += U * stride) {
+        T val[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u * stride;
+            ok[u] = k < n;
+            const int kk = ok[u] ? k : n - 1;
+            const int i = kk / w;
+            val[u] = load(i0 + i, j0 + (kk - i * w));
+        }
+        __asm__ volatile("" ::: "memory");
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc(val[u], ok[u]);
+    }
+}
+
+// Hands the per-block partials of a one-pass grid reduction to block 0 with no
+// read-modify-write on a shared address (those serialise across the chip):
+// every other block publishes with one release store of its own flag, block 0
+// waits for all the flags, clears them for the next launch and returns true
+// once all its threads see the partials.  Thread 0 of each block writes its
+// partials before the call.  Only block 0 waits, holding one work-group slot,
+// so the other blocks can always be scheduled.
+__device__ __forceinline__ bool gather_block0(unsigned* flags) {
+    if (blockIdx.x != 0) {
+        if (threadIdx.x == 0)
+            __hip_atomic_store(&flags[blockIdx.x], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        return false;
+    }
+    for (int b = 1 + threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+        while (__hip_atomic_load(&flags[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+            __builtin_amdgcn_s_sleep(1);
+        __hip_atomic_store(&flags[b], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __syncthreads();
+    return true;
+}
+
 // ---------------------------------------------------------------------------
 // Wave / block reductions (deterministic: fixed shuffle tree, fixed wave order)
 // ---------------------------------------------------------------------------

@@ -416,46 +416,64 @@ __device__ __forceinline__ unsigned long long dbits(double x) {
 // the fluid cells (solver.py:216-218, 249-251; max(nu + x) = nu + max(x)
 // exactly, rounding being monotonic), and k-omega SST adds the explicit
 // diffusion limits of k and omega from their largest diffusivity on the
-// fluid cells (models.py:204-208, turbulence.py:314-315).  acc[2..5] hold
-// max nu_t, min nu_t (fluid), max D_k, max D_omega as the bits of
-// non-negative doubles, whose integer order is their numeric order.
+// fluid cells (models.py:204-208, turbulence.py:314-315).  Each block leaves
+// max |u|, max |v|, max nu_t, -min nu_t (fluid), max D_k, max D_omega in
+// part[6 b ..]; block 0 takes the maxima and sets the step.
 __global__ void k_dt_gamma(const double* __restrict__ u, const double* __restrict__ v, Geo g,
-                           StepParams prm, TurbDt td, DevState* st, unsigned long long* acc,
-                           unsigned* ticket) {
+                           StepParams prm, TurbDt td, DevState* st, double* part,
+                           unsigned* flags) {
     __shared__ double lds[5 * 16];
     const int nx = g.nx, ny = g.ny;
     double m[2] = {0.0, 0.0};
     // turbulence: max nu_t, -min nu_t (fluid), max D_k, max D_omega (fluid)
     double q[4] = {0.0, -__builtin_inf(), 0.0, 0.0};
     if (!st->done) {
-        // row band of this block over rows 0..nx+1; threads along j
-        const int r0 = (int)(((long)(nx + 2) * blockIdx.x) / gridDim.x);
-        const int r1 = (int)(((long)(nx + 2) * (blockIdx.x + 1)) / gridDim.x);
-        // branch-free body (clamped indices, selects) so that the unrolled
-        // iterations' loads are in flight together
-        for (int i = r0; i < r1; ++i) {
-#pragma unroll 4
-            for (int j = threadIdx.x; j <= ny + 1; j += blockDim.x) {
-                const double au = fabs(u[ix(i <= nx ? i : nx, j, g.pu)]);
-                const double av = fabs(v[ix(i, j <= ny ? j : ny, g.pv)]);
-                m[0] = i <= nx ? nanmax(m[0], au) : m[0];
-                m[1] = j <= ny ? nanmax(m[1], av) : m[1];
-            }
-        }
+        // rows 0..nx+1 x columns 0..ny+1 (u has rows 0..nx, v columns 0..ny)
+        const int w = ny + 2, n = (nx + 2) * w;
+        struct UV {
+            double au, av;
+            bool iu, jv;
+        };
+        flat_reduce<8>(
+            n, w, 0, 0,
+            [&](int i, int j) {
+                UV e;
+                e.au = fabs(u[ix(i <= nx ? i : nx, j, g.pu)]);
+                e.av = fabs(v[ix(i, j <= ny ? j : ny, g.pv)]);
+                e.iu = i <= nx;
+                e.jv = j <= ny;
+                return e;
+            },
+            [&](const UV& e, bool ok) {
+                m[0] = ok && e.iu ? nanmax(m[0], e.au) : m[0];
+                m[1] = ok && e.jv ? nanmax(m[1], e.av) : m[1];
+            });
         if (td.nut) {
-            for (int i = r0; i < r1; ++i)
-                for (int j = threadIdx.x; j <= ny + 1; j += blockDim.x) {
-                    const int c = ix(i, j, g.pc);
-                    const bool f = td.fl[c] & 1;
+            struct Turb {
+                double nt, dk, dw;
+                bool f;
+            };
+            flat_reduce<4>(
+                n, w, 0, 0,
+                [&](int i, int j) {
+                    const long c = ix(i, j, g.pc);
+                    Turb e;
+                    e.f = td.fl[c] & 1;
                     // variable density: nu = mu / rho over the fluid cells
-                    const double nt = td.rho ? td.nut[c] / td.rho[c] : td.nut[c];
-                    q[0] = (f || !td.rho) ? nanmax(q[0], nt) : q[0];
-                    q[1] = f ? nanmax(q[1], -nt) : q[1];
+                    e.nt = td.rho ? td.nut[c] / td.rho[c] : td.nut[c];
+                    e.dk = td.dk ? td.dk[c] : 0.0;
+                    e.dw = td.dk ? td.dw[c] : 0.0;
+                    return e;
+                },
+                [&](const Turb& e, bool ok) {
+                    const bool f = ok && e.f;
+                    q[0] = (f || (ok && !td.rho)) ? nanmax(q[0], e.nt) : q[0];
+                    q[1] = f ? nanmax(q[1], -e.nt) : q[1];
                     if (td.dk) {
-                        q[2] = f ? nanmax(q[2], td.dk[c]) : q[2];
-                        q[3] = f ? nanmax(q[3], td.dw[c]) : q[3];
+                        q[2] = f ? nanmax(q[2], e.dk) : q[2];
+                        q[3] = f ? nanmax(q[3], e.dw) : q[3];
                     }
-                }
+                });
         }
     }
     block_max<2>(m, lds);
@@ -464,38 +482,25 @@ __global__ void k_dt_gamma(const double* __restrict__ u, const double* __restric
         block_max<4>(q, lds);
     }
     if (threadIdx.x == 0) {
-        __hip_atomic_fetch_max(&acc[0], dbits(m[0]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_max(&acc[1], dbits(m[1]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (td.nut) {
-            __hip_atomic_fetch_max(&acc[2], dbits(q[0]), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_min(&acc[3], dbits(-q[1]), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_max(&acc[4], dbits(q[2]), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_max(&acc[5], dbits(q[3]), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        }
-        const unsigned tk =
-            __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (tk == gridDim.x - 1) {
-            const double umax_raw = __longlong_as_double(
-                (long long)__hip_atomic_load(&acc[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            const double vmax_raw = __longlong_as_double(
-                (long long)__hip_atomic_load(&acc[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            acc[0] = 0ull;
-            acc[1] = 0ull;
+        double* my = part + 6 * blockIdx.x;
+        my[0] = m[0];
+        my[1] = m[1];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) my[2 + k] = q[k];
+    }
+    if (!gather_block0(flags)) return;
+    // block 0: the maxima over the blocks
+    double t[6] = {0.0, 0.0, 0.0, -__builtin_inf(), 0.0, 0.0};
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) t[k] = nanmax(t[k], part[6 * b + k]);
+    block_max<6>(t, lds);
+    {
+        if (threadIdx.x == 0) {
+            const double umax_raw = t[0], vmax_raw = t[1];
             double nu_max = prm.nu, nu_ref = prm.nu, lim_turb = __builtin_inf();
             if (td.nut) {
-                auto rd = [&](int k) {
-                    return __longlong_as_double((long long)__hip_atomic_load(
-                        &acc[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-                };
-                const double nt_max = rd(2), nt_min = rd(3), dk = rd(4), dw = rd(5);
-                acc[2] = 0ull;
-                acc[3] = dbits(__builtin_inf());
-                acc[4] = 0ull;
-                acc[5] = 0ull;
+                const double nt_max = t[2], nt_min = -t[3], dk = t[4], dw = t[5];
                 nu_max = td.rho ? nt_max : prm.nu + nt_max;
                 nu_ref = td.rho ? nt_min : prm.nu + nt_min;
                 if (!st->done) st->nut_max = nt_max;
@@ -506,7 +511,6 @@ __global__ void k_dt_gamma(const double* __restrict__ u, const double* __restric
                     lim_turb = fmin(lk, lw);
                 }
             }
-            *ticket = 0u;
             if (!st->done) {
                 double dt;
                 if (prm.dt_fixed > 0.0) {
@@ -807,32 +811,46 @@ __global__ void k_record(RecordArgs a, Geo g) {
     if (active) {
         const int tid = blockIdx.x * blockDim.x + threadIdx.x;
         const int nth = gridDim.x * blockDim.x;
-        const int r0 = 1 + (int)(((long)nx * blockIdx.x) / gridDim.x);
-        const int r1 = 1 + (int)(((long)nx * (blockIdx.x + 1)) / gridDim.x);
-        // the body loads unconditionally and accumulates through selects, so
-        // that the unrolled iterations' loads are in flight together
-        for (int i = r0; i < r1; ++i)
-#pragma unroll 4
-            for (int j = 1 + threadIdx.x; j <= ny; j += blockDim.x) {
-            const bool f = a.fl[ix(i, j, pc)] & 1;
-            const double div = (a.u[ix(i, j, g.pu)] - a.u[ix(i - 1, j, g.pu)]) / g.dx +
-                               (a.v[ix(i, j, g.pv)] - a.v[ix(i, j - 1, g.pv)]) / g.dy;
-            val[0] = f ? nanmax(val[0], fabs(div)) : val[0];
-            const double uc = 0.5 * (a.u[ix(i - 1, j, g.pu)] + a.u[ix(i, j, g.pu)]);
-            const double vc = 0.5 * (a.v[ix(i, j - 1, g.pv)] + a.v[ix(i, j, g.pv)]);
-            val[1] = f ? val[1] + (uc * uc + vc * vc) : val[1];
-            // compile-time scalar indices keep val[] in registers
+        struct Cell {
+            double uw, ue, vs, vn, ph[NS > 0 ? NS : 1];
+            bool f;
+        };
+        flat_reduce<8>(
+            nx * ny, ny, 1, 1,
+            [&](int i, int j) {
+                Cell e;
+                e.f = a.fl[ix(i, j, pc)] & 1;
+                e.uw = a.u[ix(i - 1, j, g.pu)];
+                e.ue = a.u[ix(i, j, g.pu)];
+                e.vs = a.v[ix(i, j - 1, g.pv)];
+                e.vn = a.v[ix(i, j, g.pv)];
 #pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                const double ph = a.phi[s][ix(i, j, pc)];
-                val[10 + 5 * s] = f ? val[10 + 5 * s] + ph : val[10 + 5 * s];
-            }
-            if (a.has_solid && f) {
-                // fluid cells next to interior solid cells (solver.py:542-567)
+                for (int s = 0; s < NS; ++s) e.ph[s] = a.phi[s][ix(i, j, pc)];
+                return e;
+            },
+            [&](const Cell& e, bool ok) {
+                const bool f = ok && e.f;
+                const double div = (e.ue - e.uw) / g.dx + (e.vn - e.vs) / g.dy;
+                val[0] = f ? nanmax(val[0], fabs(div)) : val[0];
+                const double uc = 0.5 * (e.uw + e.ue);
+                const double vc = 0.5 * (e.vs + e.vn);
+                val[1] = f ? val[1] + (uc * uc + vc * vc) : val[1];
+                // compile-time scalar indices keep val[] in registers
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    val[10 + 5 * s] = f ? val[10 + 5 * s] + e.ph[s] : val[10 + 5 * s];
+            });
+        if (a.has_solid) {
+            // fluid cells next to interior solid cells (solver.py:542-567)
+            for (int k = tid; k < nx * ny; k += nth) {
+                const int i = 1 + k / ny, j = 1 + k % ny;
+                if (!(a.fl[ix(i, j, pc)] & 1)) continue;
                 auto solid = [&](int si, int sj) -> bool {
                     return si >= 1 && si <= nx && sj >= 1 && sj <= ny &&
                            !(a.fl[ix(si, sj, pc)] & 1);
                 };
+                const double uc = 0.5 * (a.u[ix(i - 1, j, g.pu)] + a.u[ix(i, j, g.pu)]);
+                const double vc = 0.5 * (a.v[ix(i, j - 1, g.pv)] + a.v[ix(i, j, g.pv)]);
                 const double pv = a.p[ix(i, j, pc)];
                 if (solid(i + 1, j)) { val[2] += pv; val[8] += vc; }
                 if (solid(i - 1, j)) { val[3] += pv; val[9] += vc; }
@@ -881,27 +899,19 @@ __global__ void k_record(RecordArgs a, Geo g) {
 #pragma unroll
         for (int q = 1; q < NV; ++q) val[q] = sm[q - 1];
     }
-    __shared__ int s_last;
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) a.part[NV * blockIdx.x + q] = val[q];
-        const unsigned tk =
-            __hip_atomic_fetch_add(a.ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = tk == gridDim.x - 1;
     }
-    __syncthreads();
-    if (!s_last) return;
-    // last block: add the block partials in block order (fixed tree per thread)
+    if (!gather_block0(a.flags)) return;
+    // block 0: add the block partials in block order (fixed tree per thread)
     double tot[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) tot[q] = 0.0;
     for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-        tot[0] = nanmax(tot[0], __hip_atomic_load(&a.part[NV * b], __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT));
+        tot[0] = nanmax(tot[0], a.part[NV * b]);
 #pragma unroll
-        for (int q = 1; q < NV; ++q)
-            tot[q] += __hip_atomic_load(&a.part[NV * b + q], __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
+        for (int q = 1; q < NV; ++q) tot[q] += a.part[NV * b + q];
     }
     {
         double mx[1] = {tot[0]};
@@ -915,7 +925,6 @@ __global__ void k_record(RecordArgs a, Geo g) {
         for (int q = 1; q < NV; ++q) tot[q] = sm[q - 1];
     }
     if (threadIdx.x != 0) return;
-    *a.ticket = 0u;
     DevState* st = a.st;
     if (!st->done) {
         const double dt = st->dt;

@@ -91,6 +91,7 @@ int ualloc(cfd_unet* u, T** p, size_t n) {
     void* q = nullptr;
     HIPC(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
     HIPC(hipMemset(q, 0, std::max<size_t>(n, 1) * sizeof(T)));
+    HIPC(hipDeviceSynchronize());   // the fill is done before any stream uses the array
     u->allocs.push_back(q);
     *p = (T*)q;
     return 0;
