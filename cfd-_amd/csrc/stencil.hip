@@ -290,6 +290,47 @@ __device__ __forceinline__ double bc_tangential(int kind, double inner, double w
     }
 }
 
+// indices first..last over the work-group, R per thread at a time: the R
+// entries' loads all come before any of their stores, which the compiler could
+// not reorder itself (the stores might alias the later loads); no index of a
+// phase reads what another index of it writes
+template <int R, typename L, typename S>
+__device__ __forceinline__ void bc_batch(int first, int last, L load, S store) {
+    using V = decltype(load(first));
+    const int T = blockDim.x;
+    for (int k0 = first + (int)threadIdx.x; k0 <= last; k0 += R * T) {
+        V val[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (k0 + r * T <= last) val[r] = load(k0 + r * T);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (k0 + r * T <= last) store(k0 + r * T, val[r]);
+    }
+}
+
+// the normal component's raw inputs at boundary entry k of side s
+struct BCNormalIn {
+    double cur, in;
+    int open;
+};
+__device__ __forceinline__ BCNormalIn bc_normal_in(const BCArgs& a, int s, double cur, int k) {
+    BCNormalIn e;
+    e.cur = cur;
+    e.in = a.kind[s] == BC_INFLOW ? a.inflow[s][k] : 0.0;
+    e.open = a.kind[s] == BC_OUTFLOW ? a.open[s][k] : 1;
+    return e;
+}
+__device__ __forceinline__ double bc_normal_of(int kind, const BCNormalIn& e) {
+    switch (kind) {
+    case BC_NOSLIP:
+    case BC_SLIP: return 0.0;
+    case BC_INFLOW: return e.in;
+    case BC_OUTFLOW: return e.open ? e.cur : 0.0;
+    default: return e.cur;
+    }
+}
+
 __global__ void k_bc_domain(BCArgs a, Geo g) {
     const int nx = g.nx, ny = g.ny, pu = g.pu, pv = g.pv;
     double* u = a.u;
@@ -305,38 +346,77 @@ __global__ void k_bc_domain(BCArgs a, Geo g) {
     }
     __syncthreads();
     // left / right
-    for (int j = t; j <= ny; j += T) {
-        if (j >= 1) {
-            u[ix(0, j, pu)] = bc_normal(a.kind[0], u[ix(0, j, pu)], a.inflow[0], a.open[0], j - 1);
-            u[ix(nx, j, pu)] = bc_normal(a.kind[1], u[ix(nx, j, pu)], a.inflow[1], a.open[1], j - 1);
-        }
-        v[ix(0, j, pv)] = bc_tangential(a.kind[0], v[ix(1, j, pv)], a.vel[0], v[ix(0, j, pv)]);
-        v[ix(nx + 1, j, pv)] = bc_tangential(a.kind[1], v[ix(nx, j, pv)], a.vel[1],
-                                             v[ix(nx + 1, j, pv)]);
-    }
+    struct LR {
+        BCNormalIn w, e;
+        double v0, v1, vn, vn1;
+    };
+    bc_batch<2>(
+        0, ny,
+        [&](int j) {
+            LR r;
+            const int k = j >= 1 ? j - 1 : 0;
+            r.w = bc_normal_in(a, 0, u[ix(0, j, pu)], k);
+            r.e = bc_normal_in(a, 1, u[ix(nx, j, pu)], k);
+            r.v0 = v[ix(0, j, pv)];
+            r.v1 = v[ix(1, j, pv)];
+            r.vn = v[ix(nx, j, pv)];
+            r.vn1 = v[ix(nx + 1, j, pv)];
+            return r;
+        },
+        [&](int j, const LR& r) {
+            if (j >= 1) {
+                u[ix(0, j, pu)] = bc_normal_of(a.kind[0], r.w);
+                u[ix(nx, j, pu)] = bc_normal_of(a.kind[1], r.e);
+            }
+            v[ix(0, j, pv)] = bc_tangential(a.kind[0], r.v1, a.vel[0], r.v0);
+            v[ix(nx + 1, j, pv)] = bc_tangential(a.kind[1], r.vn, a.vel[1], r.vn1);
+        });
     __syncthreads();
     // bottom / top
-    for (int i = t; i <= nx; i += T) {
-        if (i >= 1) {
-            v[ix(i, 0, pv)] = bc_normal(a.kind[2], v[ix(i, 0, pv)], a.inflow[2], a.open[2], i - 1);
-            v[ix(i, ny, pv)] = bc_normal(a.kind[3], v[ix(i, ny, pv)], a.inflow[3], a.open[3], i - 1);
-        }
-        u[ix(i, 0, pu)] = bc_tangential(a.kind[2], u[ix(i, 1, pu)], a.vel[2], u[ix(i, 0, pu)]);
-        u[ix(i, ny + 1, pu)] = bc_tangential(a.kind[3], u[ix(i, ny, pu)], a.vel[3],
-                                             u[ix(i, ny + 1, pu)]);
-    }
+    struct BT {
+        BCNormalIn s, n;
+        double u0, u1, un, un1;
+    };
+    bc_batch<2>(
+        0, nx,
+        [&](int i) {
+            BT r;
+            const int k = i >= 1 ? i - 1 : 0;
+            r.s = bc_normal_in(a, 2, v[ix(i, 0, pv)], k);
+            r.n = bc_normal_in(a, 3, v[ix(i, ny, pv)], k);
+            r.u0 = u[ix(i, 0, pu)];
+            r.u1 = u[ix(i, 1, pu)];
+            r.un = u[ix(i, ny, pu)];
+            r.un1 = u[ix(i, ny + 1, pu)];
+            return r;
+        },
+        [&](int i, const BT& r) {
+            if (i >= 1) {
+                v[ix(i, 0, pv)] = bc_normal_of(a.kind[2], r.s);
+                v[ix(i, ny, pv)] = bc_normal_of(a.kind[3], r.n);
+            }
+            u[ix(i, 0, pu)] = bc_tangential(a.kind[2], r.u1, a.vel[2], r.u0);
+            u[ix(i, ny + 1, pu)] = bc_tangential(a.kind[3], r.un, a.vel[3], r.un1);
+        });
     __syncthreads();
     // faces closed by an obstacle behind the boundary (boundary.py:219-223)
+    struct Open2 {
+        int lo, hi;
+    };
     if (!g.periodic) {
-        for (int j = 1 + t; j <= ny; j += T) {
-            if (!a.open[0][j - 1]) u[ix(0, j, pu)] = 0.0;
-            if (!a.open[1][j - 1]) u[ix(nx, j, pu)] = 0.0;
-        }
+        bc_batch<2>(
+            1, ny, [&](int j) { return Open2{a.open[0][j - 1], a.open[1][j - 1]}; },
+            [&](int j, const Open2& o) {
+                if (!o.lo) u[ix(0, j, pu)] = 0.0;
+                if (!o.hi) u[ix(nx, j, pu)] = 0.0;
+            });
     }
-    for (int i = 1 + t; i <= nx; i += T) {
-        if (!a.open[2][i - 1]) v[ix(i, 0, pv)] = 0.0;
-        if (!a.open[3][i - 1]) v[ix(i, ny, pv)] = 0.0;
-    }
+    bc_batch<2>(
+        1, nx, [&](int i) { return Open2{a.open[2][i - 1], a.open[3][i - 1]}; },
+        [&](int i, const Open2& o) {
+            if (!o.lo) v[ix(i, 0, pv)] = 0.0;
+            if (!o.hi) v[ix(i, ny, pv)] = 0.0;
+        });
 }
 
 // ---------------------------------------------------------------------------
