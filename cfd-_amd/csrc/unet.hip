@@ -766,23 +766,40 @@ __device__ __forceinline__ void unet_img32(const float* xin, unsigned short* S0,
     unsigned short* W0b = S0;                        // 1024 x 16 (after S0 is dead)
     float* red = reinterpret_cast<float*>(W1a);      // split-K partials (levels 3)
     static_assert(256 * P32 + 2 * 64 * P64 + 2 * 16 * P64 <= IMG_LDS_S0, "W0a holds levels 1-3");
-    // input layer 1 -> 16 (scalar), ReLU
+    // LDS and global views of the tensors and weights for the layer functions
+    auto L = [](unsigned short* p) { return (lds_u16*)p; };
+    // input layer 1 -> 16 (scalar), ReLU: one pixel per thread, all 16
+    // channels (the weights are the same for every thread), each output
+    // summed in tap order as before
     {
         const float* W = w + woff(0);
         const float* b = w + boff(0);
-        for (int k = threadIdx.x; k < 1024 * 16; k += blockDim.x) {
-            const int c = k % 16, p = k / 16, h = p / 32, ww = p % 32;
-            float s = b[c];
+        const lds_f32* xl = (const lds_f32*)xin;
+        for (int p = threadIdx.x; p < 1024; p += blockDim.x) {
+            const int h = p / 32, ww = p % 32;
+            float xv[9];
+            bool in[9];
+#pragma unroll
             for (int t = 0; t < 9; ++t) {
                 const int hh = h + t / 3 - 1, w2 = ww + t % 3 - 1;
-                if (hh >= 0 && hh < 32 && w2 >= 0 && w2 < 32) s += W[t * 16 + c] * xin[hh * 32 + w2];
+                in[t] = hh >= 0 && hh < 32 && w2 >= 0 && w2 < 32;
+                xv[t] = xl[in[t] ? hh * 32 + w2 : p];
             }
-            W0a[p * P16 + c] = f2bf(fmaxf(s, 0.f));
+            bf16x4 o[4];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                float s = b[c];
+#pragma unroll
+                for (int t = 0; t < 9; ++t)
+                    if (in[t]) s += W[t * 16 + c] * xv[t];
+                o[c / 4][c % 4] = (short)f2bf(fmaxf(s, 0.f));
+            }
+            lds_bf16x4* dst = (lds_bf16x4*)(L(W0a) + p * P16);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) dst[q] = o[q];
         }
         __syncthreads();
     }
-    // LDS and global views of the tensors and weights for the layer functions
-    auto L = [](unsigned short* p) { return (lds_u16*)p; };
     auto Wt = [&](int l) { return (const glb_bf16x4*)(wsw + woff(l)); };
     auto Bs = [&](int l) { return (const glb_f32*)(w + boff(l)); };
     lds_f32* rd = (lds_f32*)red;
@@ -813,7 +830,8 @@ __device__ __forceinline__ void unet_img32(const float* xin, unsigned short* S0,
             for (int t = 0; t < 9; ++t) {
                 const int hh = h + t / 3 - 1, w2 = ww + t % 3 - 1;
                 if (hh < 0 || hh >= 32 || w2 < 0 || w2 >= 32) continue;
-                const unsigned short* xp = W0b + (hh * 32 + w2) * P16;
+                const lds_u16* xp = L(W0b) + (hh * 32 + w2) * P16;
+#pragma unroll
                 for (int c = 0; c < 16; ++c) s += W[t * 16 + c] * bf2f(xp[c]);
             }
             out(p, s);
