@@ -436,8 +436,11 @@ int launch_coop_var(cfd_ctx* c, ProjArgs a) {
     va.mg.lev[0].b = a.r;
     Geo g = c->geo;
     void* args[] = {(void*)&va, (void*)&g};
-    HIPC(hipLaunchCooperativeKernel((const void*)k_project_var, dim3(c->proj_blocks),
-                                    dim3(c->proj_bs), args, 0, c->stream));
+    // the multigrid solve runs the levels of up to 4 cells per thread in block
+    // 0 behind block barriers: full 1024-thread blocks keep more levels there
+    const int bs = va.mg.nlev >= 2 ? PROJ_BS : c->proj_bs;
+    HIPC(hipLaunchCooperativeKernel((const void*)k_project_var, dim3(c->proj_blocks), dim3(bs),
+                                    args, 0, c->stream));
     return 0;
 }
 
