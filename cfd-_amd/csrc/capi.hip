@@ -80,9 +80,7 @@ struct cfd_ctx {
     double* inflow[4] = {nullptr, nullptr, nullptr, nullptr};
     uint8_t* open[4] = {nullptr, nullptr, nullptr, nullptr};
     double *sync = nullptr, *part = nullptr;
-    unsigned *bar = nullptr, *ticket = nullptr;
     unsigned* flags = nullptr;   // per-block publish flags of the one-pass reductions
-    unsigned long long* acc = nullptr;
     DevState* st = nullptr;
     StepRecord* ring = nullptr;
     std::vector<ScalarDev> scalars;
@@ -970,8 +968,7 @@ int cfd_ctx_create(const cfd_grid_desc* d, cfd_ctx** out) {
         (rc = dalloc(c, &c->fl, N)) || (rc = dalloc(c, &c->code, N)) ||
         (rc = dalloc(c, &c->comp, N)) || (rc = dalloc(c, &c->lut, 128)) ||
         (rc = dalloc(c, &c->comp_count, CFD_MAX_COMP)) || (rc = dalloc(c, &c->st, 1)) ||
-        (rc = dalloc(c, &c->ring, RING)) || (rc = dalloc(c, &c->bar, 4)) ||
-        (rc = dalloc(c, &c->ticket, 4)) || (rc = dalloc(c, &c->acc, 8)))
+        (rc = dalloc(c, &c->ring, RING)))
         return fail(rc);
     // persistent projection launch shape: one block per CU (or per row on small grids)
     c->proj_bs = std::min(PROJ_BS, ((c->ny + 63) / 64) * 64);
