@@ -163,3 +163,40 @@ def test_closed_long_tank_matches_the_oracle(gpu, multigrid):
     assert max(worst.values()) < 1e-8
     assert max(sim.cg_info) == 0
     assert abs(m.volume() / m.volume0 - 1) < 1e-12
+
+
+def test_inflow_of_the_heavy_phase_matches_the_oracle(gpu):
+    """Water entering through an inflow side (alpha_inflow = 1) into a
+    channel with a water layer, outflow on the right: the inflow face of the
+    alpha transport, the inflow and outflow faces of the variable-density
+    system and the mixture properties, against the oracle."""
+    from cfd2d_amd.boundary import BoundaryCondition as BC
+    from cfd2d_amd.grid import Grid
+    from cfd2d_amd.solver import Simulation
+    from cfd2d_amd.vof import VOF
+    from oracle import cfd_oracle as O
+    from oracle import vof_oracle as V
+
+    def setup(G, B):
+        g = G(40, 16, 1.0, 0.4)
+        X, Y = g.cell_centers()
+        a0 = (Y < 0.15).astype(float)
+        bcs = dict(left=B("inflow", 0.5), right=B("outflow"), bottom=B("noslip"),
+                   top=B("noslip"))
+        return g, a0, bcs
+
+    g, a0, bcs = setup(Grid, BC)
+    sim = Simulation(g, bcs, nu=1e-6, gamma=0.5,
+                     models=[VOF(a0, sigma=0.0, alpha_inflow=1.0)])
+    og, oa0, obcs = setup(O.OGrid, O.OBC)
+    osim = O.OSim(og, obcs, nu=1e-6, gamma=0.5, models=[V.OVOF(oa0, sigma=0.0, alpha_inflow=1.0)])
+    for _ in range(10):
+        sim.step()
+        osim.step()
+    m, om = sim.model("multiphase"), osim.models[-1]
+    worst = {k: _rel(a, b) for k, a, b in (("alpha", m.alpha, om.alpha), ("u", sim.u, osim.u),
+                                            ("v", sim.v, osim.v), ("p", sim.p, osim.p))}
+    print("inflow of water, vs oracle", worst, "iterations", sim.cg_iterations, flush=True)
+    assert max(worst.values()) < 1e-8
+    assert m.alpha[1, 1:-1].min() > 0.5          # the inflow column fills with water
+    assert _rel(sim.history.dt, osim.hist["dt"]) < 1e-10
