@@ -200,3 +200,23 @@ def test_inflow_of_the_heavy_phase_matches_the_oracle(gpu):
     assert max(worst.values()) < 1e-8
     assert m.alpha[1, 1:-1].min() > 0.5          # the inflow column fills with water
     assert _rel(sim.history.dt, osim.hist["dt"]) < 1e-10
+
+
+def test_dam_break_96_matches_the_oracle(gpu):
+    """The dam break at its default 96 x 96 (five multigrid levels: a grid-wide
+    fine level, then levels held by block 0 down to the 6 x 6 exact solve)
+    against the oracle for 10 steps."""
+    from cfd2d_amd import cases
+    from oracle import vof_oracle as V
+
+    sim = cases.dam_break(n=96, t_end=1.0).sim
+    osim = V.dam_break(n=96)
+    for _ in range(10):
+        sim.step()
+        osim.step()
+    m, om = sim.model("multiphase"), osim.models[-1]
+    worst = {k: _rel(a, b) for k, a, b in (("alpha", m.alpha, om.alpha), ("u", sim.u, osim.u),
+                                            ("v", sim.v, osim.v), ("p", sim.p, osim.p))}
+    print("dam break 96 vs oracle", worst, "iterations", sim.cg_iterations, flush=True)
+    assert max(worst.values()) < 1e-8
+    assert max(sim.cg_info) == 0
