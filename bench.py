@@ -69,13 +69,21 @@ def gaussian_blobs(n: int, seed: int = 0, count: int = 8) -> np.ndarray:
     return phi
 
 
-# the learned legs start each solve from the linear extrapolation in time of
-# the last two solutions; cg_learned_x0prev is the headline solver with the
-# reference's warm start (the last solution), cg and cg_mixed keep that start
-LEGS = {"cg": ("cg", None), "cg_mixed": ("cg_mixed", None),
-        "cg_learned": ("cg_learned", {"mode": "net", "x0": "extrapolate"}),
-        "cg_learned_none": ("cg_learned", {"mode": "none", "x0": "extrapolate"}),
-        "cg_learned_x0prev": ("cg_learned", {"mode": "net", "x0": "previous"})}
+# leg -> (pressure solver, learned-solver settings, initial iterate).  The
+# learned legs start each solve from the linear extrapolation in time of the
+# last two solutions; cg_learned_x0prev is the headline solver with the
+# reference's warm start (the last solution); cg and cg_mixed keep that start,
+# and cg_x0extrap is the FP64 baseline with the learned legs' start (the
+# like-for-like ratio).  cg_learned_vcycle replaces the network by the rest of
+# the V-cycle down to the dense coarsest solve (the method without the
+# network); cg_learned_none drops the coarse solve below 32x32 altogether.
+LEGS = {"cg": ("cg", None, "previous"),
+        "cg_x0extrap": ("cg", None, "extrapolate"),
+        "cg_mixed": ("cg_mixed", None, "previous"),
+        "cg_learned": ("cg_learned", {"mode": "net"}, "extrapolate"),
+        "cg_learned_vcycle": ("cg_learned", {"mode": "vcycle"}, "extrapolate"),
+        "cg_learned_none": ("cg_learned", {"mode": "none"}, "extrapolate"),
+        "cg_learned_x0prev": ("cg_learned", {"mode": "net"}, "previous")}
 # the same workload with a turbulence model on the device (variable viscosity,
 # nu_t per step; SST also transports k and omega): the cost of the model
 TURB_LEGS = {"cg_learned_smagorinsky": ("cg_learned", "Smagorinsky"),
@@ -91,13 +99,13 @@ def build_case(leg: str, n: int = N):
     turb = None
     if leg in TURB_LEGS:
         base, turb = TURB_LEGS[leg]
-        solver, learned = LEGS[base]
+        solver, learned, x0 = LEGS[base]
     else:
-        solver, learned = LEGS[leg]
+        solver, learned, x0 = LEGS[leg]
     if learned is not None:
         learned = dict(learned, use_graph=USE_GRAPH)
     case = cases.lid_driven_cavity(re=RE, n=n, pressure_solver=solver, pressure_tol=RTOL,
-                                   pressure_maxiter=50000, learned=learned)
+                                   pressure_maxiter=50000, learned=learned, pressure_x0=x0)
     if turb is not None:
         cases.add_turbulence(case, turb)
     cases.add_scalar(case, "c", diffusivity=1e-3, limiter="vanleer", initial=gaussian_blobs(n))
@@ -161,8 +169,7 @@ def run_leg(solver: str, steps: int, warmup: int) -> dict:
     divmax = max(r.divergence for r in recs)
     out = {
         "solver": solver,
-        "x0": (LEGS[TURB_LEGS[solver][0] if solver in TURB_LEGS else solver][1] or {}).get(
-            "x0", "previous"),
+        "x0": LEGS[TURB_LEGS[solver][0] if solver in TURB_LEGS else solver][2],
         "turbulence": TURB_LEGS[solver][1] if solver in TURB_LEGS else None,
         "steps_per_s": steps / wall,
         "ms_per_step": 1e3 * wall / steps,
@@ -322,7 +329,8 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
-    ap.add_argument("--legs", default="cg,cg_learned,cg_learned_none,cg_learned_x0prev,cg_mixed,"
+    ap.add_argument("--legs", default="cg,cg_x0extrap,cg_learned,cg_learned_vcycle,"
+                                      "cg_learned_none,cg_learned_x0prev,cg_mixed,"
                                       "cg_learned_smagorinsky,cg_learned_sst,vof_dam_break")
     ap.add_argument("--cpu-child", nargs=2, default=None)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -361,7 +369,9 @@ def main() -> None:
         print("no solver leg ran", file=sys.stderr)
         sys.exit(1)
     base = good("cg")
+    base_x0 = good("cg_x0extrap")
     none = good("cg_learned_none")
+    vcyc = good("cg_learned_vcycle")
     ensemble = None
     if not args.no_ensemble:
         try:
@@ -379,6 +389,7 @@ def main() -> None:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "timed_steps": [args.warmup + 1, args.warmup + args.steps],
         "ms_per_step": round(head["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
@@ -393,10 +404,18 @@ def main() -> None:
         else None,
         "speedup_vs_fp64_cg": (round(head["steps_per_s"] / base["steps_per_s"], 3)
                                if base and "error" not in base else None),
-        "network_gain": ({"iters_without": round(none["cg_iters_mean"], 2),
-                          "iters_with": round(head["cg_iters_mean"], 2),
-                          "steps_per_s_without": round(none["steps_per_s"], 3)}
-                         if none and head["solver"] == "cg_learned" else None),
+        "speedup_same_x0": (round(head["steps_per_s"] / base_x0["steps_per_s"], 3)
+                            if base_x0 and head.get("x0") == base_x0["x0"] else None),
+        # the network against the same solver with the V-cycle carried on to
+        # the dense coarsest solve (the complete method without the network),
+        # and against no coarse solve below 32x32 at all
+        "network_gain": ({"iters_with": round(head["cg_iters_mean"], 2),
+                          "steps_per_s_with": round(head["steps_per_s"], 3),
+                          "iters_vcycle": round(vcyc["cg_iters_mean"], 2) if vcyc else None,
+                          "steps_per_s_vcycle": round(vcyc["steps_per_s"], 3) if vcyc else None,
+                          "iters_none": round(none["cg_iters_mean"], 2) if none else None,
+                          "steps_per_s_none": round(none["steps_per_s"], 3) if none else None}
+                         if head["solver"] == "cg_learned" and (none or vcyc) else None),
         "ensemble": ensemble,
         "legs": legs,
     }

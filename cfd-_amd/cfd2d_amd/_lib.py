@@ -8,8 +8,12 @@ missing.
 
 from __future__ import annotations
 
+import atexit
 import ctypes
+import itertools
+import sys
 import warnings
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -24,12 +28,13 @@ CFD_ERR_LIMIT = -4
 
 BC_KINDS = {"noslip": 0, "slip": 1, "inflow": 2, "outflow": 3, "periodic": 4}
 FIELDS = {"u": 0, "v": 1, "p": 2, "F": 3, "G": 4, "hu": 5, "hv": 6, "hu_old": 7, "hv_old": 8,
-          "x": 9, "nut": 10, "diff_k": 11, "diff_omega": 12, "alpha": 13, "rho": 14, "mu": 15}
+          "x": 9, "nut": 10, "diff_k": 11, "diff_omega": 12, "alpha": 13, "rho": 14, "mu": 15,
+          "xprev": 16}
 REC_NUT_MAX = 30     # record slot of the step's largest nu_t (10 + 5 * 4 scalar sums before it)
 REC_FLOW_FORCE = 31  # record slot of the driving force after the step
 REC_VOF = 32         # VOF: alpha sum over fluid cells, front cell, column-height cell
 SOLVERS = {"pcg64": 0, "mgcg": 1, "learned": 2}
-LEARNED_MODES = {"net": 0, "none": 1, "teacher": 2}
+LEARNED_MODES = {"net": 0, "none": 1, "teacher": 2, "vcycle": 3}
 
 _d = ctypes.c_double
 _i32 = ctypes.c_int32
@@ -136,6 +141,42 @@ class Record(ctypes.Structure):
 
 
 _LIB = None
+# objects holding device resources (anything with close()), by creation order
+_LIVE: "weakref.WeakValueDictionary[int, object]" = weakref.WeakValueDictionary()
+_SEQ = itertools.count()
+_SHUT = False
+
+
+def track(obj) -> None:
+    """Register an object that owns device resources; whatever is still open
+    at interpreter exit is closed, newest first, by :func:`_at_exit`."""
+    _LIVE[next(_SEQ)] = obj
+
+
+def closing_allowed() -> bool:
+    """False once the library has shut down or Python is finalising: a
+    ``__del__`` then makes no HIP call (the runtime may be half torn down)."""
+    return not _SHUT and not sys.is_finalizing()
+
+
+def _at_exit() -> None:
+    """Close every tracked object, then ``cfd_shutdown`` (waits for the
+    device).  Runs from ``atexit``, i.e. before the process's C exit handlers
+    tear the HIP runtime and any profiler down."""
+    global _SHUT
+    if _LIB is None or _SHUT:
+        return
+    for key in sorted(_LIVE.keys(), reverse=True):
+        obj = _LIVE.get(key)
+        if obj is not None:
+            try:
+                obj.close()
+            except Exception:
+                pass
+    try:
+        _LIB.cfd_shutdown()
+    finally:
+        _SHUT = True
 
 
 def lib() -> ctypes.CDLL:
@@ -143,6 +184,8 @@ def lib() -> ctypes.CDLL:
     global _LIB
     if _LIB is not None:
         return _LIB
+    if _SHUT:
+        raise RuntimeError("the cfd2d_amd library has been shut down (interpreter exit)")
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python cfd-_amd/build.py` "
                            "(there is no CPU fallback)")
@@ -150,6 +193,9 @@ def lib() -> ctypes.CDLL:
     P = ctypes.POINTER
     sig = {
         "cfd_last_error": ([], ctypes.c_char_p),
+        "cfd_shutdown": ([], _i32),
+        "cfd_set_initial_iterate": ([_vp, _i32], _i32),
+        "cfd_initial_iterate": ([_vp, P(_i32), P(_i32)], _i32),
         "cfd_device_count": ([P(_i32)], _i32),
         "cfd_ctx_create": ([P(GridDesc), P(_vp)], _i32),
         "cfd_ctx_destroy": ([_vp], _i32),
@@ -216,11 +262,13 @@ def lib() -> ctypes.CDLL:
         f.argtypes = args
         f.restype = res
     _LIB = L
+    atexit.register(_at_exit)
     return L
 
 
 def exported_symbols() -> list[str]:
-    return [n for n in ("cfd_last_error", "cfd_device_count", "cfd_ctx_create",
+    return [n for n in ("cfd_last_error", "cfd_device_count", "cfd_shutdown",
+                        "cfd_set_initial_iterate", "cfd_initial_iterate", "cfd_ctx_create",
                         "cfd_ctx_destroy", "cfd_ctx_set_stream", "cfd_add_scalar", "cfd_upload",
                         "cfd_download", "cfd_upload_scalar", "cfd_download_scalar",
                         "cfd_set_state", "cfd_get_state", "cfd_set_multigrid", "cfd_apply_bc",
@@ -273,6 +321,7 @@ class DeviceArray:
         p = _vp()
         check(lib().cfd_dev_alloc(self.nbytes, ctypes.byref(p)), "device alloc")
         self.ptr = p
+        track(self)
         self.upload(host)
 
     def upload(self, host: np.ndarray) -> None:
@@ -296,11 +345,14 @@ class DeviceArray:
             lib().cfd_dev_free(self.ptr)
             self.ptr = None
 
+    close = free
+
     def __del__(self):  # pragma: no cover
-        try:
-            self.free()
-        except Exception:
-            pass
+        if closing_allowed():
+            try:
+                self.free()
+            except Exception:
+                pass
 
 
 class DeviceImage:

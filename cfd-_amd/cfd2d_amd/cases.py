@@ -2,14 +2,16 @@
 
 Dimensionless cases with density 1; ``Re = U_ref L_ref / nu`` sets the
 viscosity.  The factories mirror the reference's (same parameters, defaults
-and ``case.json`` schema).  Cases that need physics outside the GPU step
-(k-omega SST / Smagorinsky turbulence, two-phase VOF, periodic flow-rate
-control) raise ``ValueError``.
+and ``case.json`` schema): their bodies are the reference's case definitions
+(grid, boundary conditions, viscosity from Re), kept so that each factory
+builds the same case.  Every case runs on the GPU step, including the
+turbulence models (Smagorinsky, k-omega SST), two-phase VOF and flow-rate
+control.
 
 ``numerik`` keys understood in addition to the reference's (``scheme``,
-``gamma``, ``tau``, ``dt``, ``pressure_solver``): ``pressure_tol`` and
-``pressure_maxiter``.  The reference ignores unknown ``numerik`` keys, so a
-case file using them stays loadable there.
+``gamma``, ``tau``, ``dt``, ``pressure_solver``): ``pressure_tol``,
+``pressure_maxiter``, ``pressure_x0`` and ``learned``.  The reference ignores
+unknown ``numerik`` keys, so a case file using them stays loadable there.
 """
 
 from __future__ import annotations
@@ -27,7 +29,7 @@ from .models import BoussinesqTemperature, PassiveScalar, ScalarBC, default_scal
 from .solver import Simulation
 
 NUMERIK_KEYS = ("scheme", "gamma", "tau", "dt", "pressure_solver", "pressure_tol",
-                "pressure_maxiter", "learned")
+                "pressure_maxiter", "pressure_x0", "learned")
 
 
 @dataclass

@@ -49,6 +49,7 @@ class DeviceContext:
         _lib.check(L.cfd_ctx_create(ctypes.byref(d), ctypes.byref(h)), "context creation")
         self.handle = h
         self._keep = []
+        _lib.track(self)
 
     def close(self) -> None:
         if getattr(self, "handle", None):
@@ -56,10 +57,11 @@ class DeviceContext:
             self.handle = None
 
     def __del__(self):  # pragma: no cover - interpreter shutdown order varies
-        try:
-            self.close()
-        except Exception:
-            pass
+        if _lib.closing_allowed():
+            try:
+                self.close()
+            except Exception:
+                pass
 
     # fields ------------------------------------------------------------------
     def shape(self, name: str) -> tuple[int, int]:
@@ -70,7 +72,7 @@ class DeviceContext:
             return (nx + 2, ny + 1)
         if name in ("p", "nut", "diff_k", "diff_omega", "alpha", "rho", "mu"):
             return (nx + 2, ny + 2)
-        if name == "x":
+        if name in ("x", "xprev"):
             return (nx, ny)
         raise KeyError(name)
 
@@ -113,6 +115,14 @@ class DeviceContext:
 
     def set_state(self, s: _lib.State) -> None:
         _lib.check(_lib.lib().cfd_set_state(self.handle, ctypes.byref(s)), "set state")
+
+    def initial_iterate(self) -> tuple[int, int]:
+        """(mode, has_history) of the pressure solves' initial iterate: mode 1
+        extrapolates in time, and has_history says whether ``xprev`` holds the
+        solution before the last one."""
+        m, h = ctypes.c_int32(0), ctypes.c_int32(0)
+        self.call("cfd_initial_iterate", ctypes.byref(m), ctypes.byref(h))
+        return int(m.value), int(h.value)
 
     def apply_bc(self) -> None:
         _lib.check(_lib.lib().cfd_apply_bc(self.handle), "apply bc")

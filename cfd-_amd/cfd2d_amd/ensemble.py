@@ -140,6 +140,7 @@ class Ensemble:
         hdl = ctypes.c_void_p()
         _lib.check(_lib.lib().cfd_ens_create(ctypes.byref(d), ctypes.byref(hdl)), "ensemble create")
         self._h = hdl
+        _lib.track(self)
         self.net = None
         if net:
             if params is None:
@@ -183,7 +184,8 @@ class Ensemble:
             self.net = None
 
     def __del__(self):  # pragma: no cover
-        try:
-            self.close()
-        except Exception:
-            pass
+        if _lib.closing_allowed():
+            try:
+                self.close()
+            except Exception:
+                pass

@@ -12,8 +12,10 @@
   [hu_old, hv_old], skalar_<name>, modell<k>_<key>, history_json -- identical
   to the reference's, so files move both ways.  One extra key, ``druck_x0``,
   stores the pressure solver's warm start (the reference keeps it in memory
-  only, pressure.py:140), which makes a CG restart continue bit for bit; the
-  reference's loader ignores it.
+  only, pressure.py:140), which makes a CG restart continue bit for bit; with
+  the extrapolated initial iterate a second one, ``druck_xprev``, stores the
+  solution before the last, so that restart is exact too.  The reference's
+  loader ignores both.
 """
 
 from __future__ import annotations
@@ -523,6 +525,10 @@ def save_checkpoint(sim, path) -> Path:
         {k: list(map(float, v)) for k, v in hist.__dict__.items() if k != "extra"}
         | {"extra": {k: list(map(float, v)) for k, v in hist.extra.items()}})
     data["druck_x0"] = sim._ctx.download("x")
+    if sim._ctx.initial_iterate()[1]:
+        # the solution before the last one: an extrapolated initial iterate
+        # continues from it after a restart
+        data["druck_xprev"] = sim._ctx.download("xprev")
     np.savez_compressed(path, **data)
     return path
 
@@ -561,6 +567,9 @@ def load_checkpoint(sim, path) -> None:
                 m.set_state(sim, st)
     if "druck_x0" in d and d["druck_x0"].shape == (g.nx, g.ny):
         sim._ctx.upload("x", d["druck_x0"])
+        if ("druck_xprev" in d and d["druck_xprev"].shape == (g.nx, g.ny)
+                and sim._ctx.initial_iterate()[0] == 1):
+            sim._ctx.upload("xprev", d["druck_xprev"])
     h = json.loads(str(d["history_json"]))
     extra = h.pop("extra", {})
     for k, v in h.items():

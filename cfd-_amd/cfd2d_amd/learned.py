@@ -37,8 +37,10 @@ WEIGHTS = Path(__file__).resolve().parent / "weights" / "unet_pressure.npz"
 class LearnedConfig:
     """How the learned solver runs.
 
-    mode: "net" (the network), "none" (constant-vector correction only: the
-    method without the network) or "teacher" (level K solved by
+    mode: "net" (the network), "none" (constant-vector correction only),
+    "vcycle" (no network: the V-cycle continues below level K to the dense
+    coarsest level, the complete multigrid preconditioner the network is
+    measured against) or "teacher" (level K solved by
     ``teacher_cycles`` V-cycles; with ``harvest_cap`` > 0 the training pairs
     are stored).  ``params``/``sigma`` default to the weights shipped in
     ``cfd2d_amd/weights``; ``level`` None picks the level automatically.
@@ -118,7 +120,7 @@ def load_weights(path=None) -> tuple[np.ndarray, float]:
 class Attached:
     """A learned solver attached to a device context (keeps its buffers)."""
 
-    def __init__(self, ctx, hierarchy, cfg: LearnedConfig):
+    def __init__(self, ctx, hierarchy, cfg: LearnedConfig, x0: str = "previous"):
         self.cfg = cfg
         self.level = choose_level(hierarchy, cfg.side) if cfg.level is None else int(cfg.level)
         self.region = (hierarchy.levels[self.level]["nx"], hierarchy.levels[self.level]["ny"])
@@ -151,7 +153,9 @@ class Attached:
         d.coarse_fused = 1 if cfg.coarse_fused else 0
         if cfg.x0 not in ("extrapolate", "previous"):
             raise ValueError(f"x0 must be 'extrapolate' or 'previous', got {cfg.x0!r}")
-        d.x0_extrapolate = 1 if cfg.x0 == "extrapolate" else 0
+        # either setting asks for the extrapolated start (the solver's x0 or
+        # the pressure solver's, which applies to every method)
+        d.x0_extrapolate = 1 if "extrapolate" in (cfg.x0, x0) else 0
         d.sigma = self.sigma
         d.one_a_one = one_a_one(hierarchy, self.level)
         d.harvest_in = self.hin.ptr if self.hin else None

@@ -39,11 +39,16 @@ MAX_DIAG_VALUES = 63
 
 class PressureSolver:
     METHODS = ("direct", "cg", "cg_mixed", "cg_learned")
+    X0 = ("previous", "extrapolate")
 
     def __init__(self, grid: Grid, boundary: DomainBoundary, method: str = "direct",
-                 tol: float = 1e-10, maxiter: int | None = None, learned=None):
+                 tol: float = 1e-10, maxiter: int | None = None, learned=None,
+                 x0: str = "previous"):
         if method not in self.METHODS:
             raise ValueError(f"Unbekannter Drucklöser {method!r}, erlaubt: {self.METHODS}")
+        if x0 not in self.X0:
+            raise ValueError(f"x0 must be one of {self.X0}, got {x0!r}")
+        self.x0 = x0
         self.grid, self.method = grid, method
         self.tol = float(tol)
         nx, ny = grid.nx, grid.ny
@@ -147,6 +152,8 @@ class PressureSolver:
 
     def attach(self, ctx) -> None:
         self._ctx = ctx
+        if self.x0 == "extrapolate":
+            ctx.call("cfd_set_initial_iterate", 1)
         if self.method in ("cg_mixed", "cg_learned"):
             from .multigrid import Hierarchy
 
@@ -157,7 +164,8 @@ class PressureSolver:
         if self.method == "cg_learned":
             from .learned import Attached, LearnedConfig
 
-            self.learned = Attached(ctx, self.hierarchy, LearnedConfig.from_spec(self.learned_spec))
+            self.learned = Attached(ctx, self.hierarchy, LearnedConfig.from_spec(self.learned_spec),
+                                    x0=self.x0)
 
     def solve(self, rhs: np.ndarray) -> np.ndarray:
         """Solve ``Lap(p) = rhs`` for rhs of shape (nx, ny) on the device

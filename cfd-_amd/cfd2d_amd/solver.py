@@ -62,9 +62,12 @@ class Simulation:
     """2D incompressible flow on the GPU.
 
     Parameters as cfd2d.solver.Simulation (solver.py:58-100), plus
-    ``pressure_tol`` / ``pressure_maxiter`` (the reference fixes 1e-10 / 5000)
-    and ``sync_every``.  ``pressure_solver`` is ``"direct"`` (default),
-    ``"cg"`` or ``"cg_mixed"``.
+    ``pressure_tol`` / ``pressure_maxiter`` (the reference fixes 1e-10 / 5000),
+    ``pressure_x0`` (``"previous"``: each solve starts from the last solution,
+    as the reference's CG does; ``"extrapolate"``: from the linear
+    extrapolation in time of the last two solutions), ``sync_every`` and
+    ``learned`` (settings of the learned solver).  ``pressure_solver`` is
+    ``"direct"`` (default), ``"cg"``, ``"cg_mixed"`` or ``"cg_learned"``.
     """
 
     def __init__(self, grid: Grid, bcs: dict[str, BoundaryCondition], nu: float,
@@ -73,7 +76,8 @@ class Simulation:
                  compute_forces: bool = False, initial_velocity: tuple = (0.0, 0.0),
                  models: list | None = None, body_force: tuple = (0.0, 0.0),
                  flow_rate: float | None = None, pressure_tol: float | None = None,
-                 pressure_maxiter: int | None = None, sync_every: int = 1, learned=None):
+                 pressure_maxiter: int | None = None, sync_every: int = 1, learned=None,
+                 pressure_x0: str = "previous"):
         if nu <= 0:
             raise ValueError("Die Viskosität muss positiv sein.")
         if scheme not in ("ab2", "euler"):
@@ -96,7 +100,8 @@ class Simulation:
         self.bcs = bcs
         self.pressure = PressureSolver(grid, self.boundary, method=pressure_solver,
                                        tol=1e-10 if pressure_tol is None else pressure_tol,
-                                       maxiter=pressure_maxiter, learned=learned)
+                                       maxiter=pressure_maxiter, learned=learned,
+                                       x0=pressure_x0)
         self._ctx = DeviceContext(grid, self.boundary, self.pressure)
         self.pressure.attach(self._ctx)
 

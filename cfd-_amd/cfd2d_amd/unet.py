@@ -54,6 +54,7 @@ class UNet:
         h = ctypes.c_void_p()
         _lib.check(L.cfd_unet_create(self.H, self.W, ctypes.byref(h)), "U-Net create")
         self._h = h
+        _lib.track(self)
         self.set_params(init_params(seed) if params is None else params)
 
     # ---- parameters ----
@@ -123,10 +124,11 @@ class UNet:
             self._h = None
 
     def __del__(self):  # pragma: no cover
-        try:
-            self.close()
-        except Exception:
-            pass
+        if _lib.closing_allowed():
+            try:
+                self.close()
+            except Exception:
+                pass
 
 
 def mfma_layout_check(A: np.ndarray, B: np.ndarray) -> np.ndarray:

@@ -24,15 +24,14 @@
 
 namespace {
 
-thread_local std::string g_err;
+// last error message of the calling thread (a plain array: nothing to destroy at exit)
+thread_local char g_err[512];
 
 int set_err(int code, const char* fmt, ...) {
-    char buf[512];
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
     va_end(ap);
-    g_err = buf;
     return code;
 }
 
@@ -207,7 +206,7 @@ int shape_of(const cfd_ctx* c, int field, int* rows, int* cols, int* off_i, int*
         *rows = c->nx + 2; *cols = c->ny + 1; return 0;
     case CFD_FIELD_P:
         *rows = c->nx + 2; *cols = c->ny + 2; return 0;
-    case CFD_FIELD_X:
+    case CFD_FIELD_X: case CFD_FIELD_XPREV:
         *rows = c->nx; *cols = c->ny; *off_i = 1; *off_j = 1; return 0;
     case CFD_FIELD_NUT: case CFD_FIELD_DIFF_K: case CFD_FIELD_DIFF_OMEGA:
     case CFD_FIELD_ALPHA: case CFD_FIELD_RHO: case CFD_FIELD_MU:
@@ -236,6 +235,7 @@ double* field_ptr(cfd_ctx* c, int field, int par) {
     case CFD_FIELD_HU_OLD: return c->H[0][par ^ 1];
     case CFD_FIELD_HV_OLD: return c->H[1][par ^ 1];
     case CFD_FIELD_X: return c->x;
+    case CFD_FIELD_XPREV: return c->xprev;
     case CFD_FIELD_NUT: return c->turb.nut;
     case CFD_FIELD_DIFF_K: return c->turb.dk;
     case CFD_FIELD_DIFF_OMEGA: return c->turb.dw;
@@ -442,6 +442,19 @@ int launch_coop_var(cfd_ctx* c, ProjArgs a) {
     return 0;
 }
 
+// initial iterate of a time step's constant-density solve: x <- x + theta
+// (x - x_prev) when the extrapolated start is set (not for the initial
+// projection, which passes dt_override); x_prev keeps the last solution
+int x0_extrapolate(cfd_ctx* c, const ProjArgs& a) {
+    if (!c->l_x0 || a.dt_override != 0.0) return 0;
+    const long n = (long)(c->nx + 2) * c->P;
+    hipLaunchKernelGGL(k_x0_extrap, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       c->x, c->xprev, n, (const DevState*)c->st, c->l_hist);
+    HIPC(hipGetLastError());
+    c->l_hist = 1;
+    return 0;
+}
+
 int launch_project(cfd_ctx* c, ProjArgs a) {
     if (c->vof.on) return launch_coop_var(c, a);
     if (a.method != PROJ_PCG64 && a.method != PROJ_MG && a.method != PROJ_LEARNED)
@@ -453,6 +466,7 @@ int launch_project(cfd_ctx* c, ProjArgs a) {
             return set_err(CFD_ERR_STATE, "learned solver selected before cfd_set_learned");
         return run_learned(c, a);
     }
+    TRY(x0_extrapolate(c, a));
     return launch_coop(c, a);
 }
 
@@ -711,6 +725,7 @@ int learned_iteration(cfd_ctx* c) {
             cn.w = u->w;
             cn.use_net = 1;
         }
+        cn.vcycle = c->l_mode == LEARNED_VCYCLE ? 1 : 0;
         cn.gain = c->l_sigma / c->l_D;
         cn.one_a_one = c->l_1a1;
         hipLaunchKernelGGL(k_lv_coarse, dim3(1), dim3(1024), 0, c->stream, a, Lc, K, cn);
@@ -721,6 +736,18 @@ int learned_iteration(cfd_ctx* c) {
         LV_LAUNCH512(k_fcg_update, nb0, a, v, nb0);
         LV_LAUNCH512(k_fcg_true, nb0, a, v, nb0);
         LV_LAUNCH(k_fcg_true_fin, 1, a, nb0, c->l_mode == LEARNED_NET ? 1 : 0);
+        return 0;
+    }
+    if (c->l_mode == LEARNED_VCYCLE) {   // the complete V-cycle, dense coarsest solve
+        for (int l = 0; l < NL - 1; ++l) TRY(down(l));
+        LV_LAUNCH(k_lv_dense, 1, a);
+        for (int l = NL - 2; l >= 1; --l) TRY(up(l));
+        if (NL >= 2) TRY(up0());
+        else LV_LAUNCH512(k_fcg_dots, nb0, a, v);
+        LV_LAUNCH512(k_fcg_dir, nb0, a, v, nb0);
+        LV_LAUNCH512(k_fcg_update, nb0, a, v, nb0);
+        LV_LAUNCH512(k_fcg_true, nb0, a, v, nb0);
+        LV_LAUNCH(k_fcg_true_fin, 1, a, nb0, 0);
         return 0;
     }
     for (int l = 0; l < K; ++l) TRY(down(l));
@@ -845,13 +872,7 @@ int run_learned(cfd_ctx* c, ProjArgs a) {
         }
         HIPC(hipEventRecord(c->prof_ev[c->prof_used], c->stream));
     }
-    if (c->l_x0 && a.dt_override == 0.0) {   // time steps only, not the initial projection
-        const long n = (long)(c->nx + 2) * c->P;
-        hipLaunchKernelGGL(k_x0_extrap, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                           c->x, c->xprev, n, (const DevState*)c->st, c->l_hist);
-        HIPC(hipGetLastError());
-        c->l_hist = 1;
-    }
+    TRY(x0_extrapolate(c, a));
     TRY(learned_start(c, a));
     int batch = std::max(1, c->l_prev_iters - 1);
     int launched = 0;
@@ -898,7 +919,17 @@ int run_learned(cfd_ctx* c, ProjArgs a) {
 // ===========================================================================
 extern "C" {
 
-const char* cfd_last_error(void) { return g_err.c_str(); }
+const char* cfd_last_error(void) { return g_err; }
+
+int cfd_shutdown(void) {
+    // every context, ensemble and network is destroyed by now (the caller's
+    // duty); wait for the device so that no work is in flight when the
+    // process's exit handlers tear the runtime down
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return 0;
+    HIPC(hipDeviceSynchronize());
+    return 0;
+}
 
 int cfd_device_count(int32_t* count) {
     int n = 0;
@@ -1160,7 +1191,8 @@ int cfd_set_learned(cfd_ctx* c, const cfd_learned_desc* d) {
     const MGDev& mg = c->mg_host;
     if (d->level < 0 || d->level >= mg.nlev)
         return set_err(CFD_ERR_ARG, "level %d outside the hierarchy (0..%d)", d->level, mg.nlev - 1);
-    if (d->mode < 0 || d->mode > 2) return set_err(CFD_ERR_ARG, "mode %d", d->mode);
+    if (d->mode < LEARNED_NET || d->mode > LEARNED_VCYCLE)
+        return set_err(CFD_ERR_ARG, "mode %d", d->mode);
     const MGLevelDev& L = mg.lev[d->level];
     const int H = ((L.nx + 7) / 8) * 8, W = ((L.ny + 7) / 8) * 8;
     if (d->mode == LEARNED_NET) {
@@ -1233,6 +1265,23 @@ int cfd_set_learned(cfd_ctx* c, const cfd_learned_desc* d) {
     return 0;
 }
 
+int cfd_set_initial_iterate(cfd_ctx* c, int32_t mode) {
+    if (!c) return set_err(CFD_ERR_ARG, "null argument");
+    if (mode != 0 && mode != 1) return set_err(CFD_ERR_ARG, "initial iterate mode %d", mode);
+    HIPC(hipStreamSynchronize(c->stream));
+    if (mode == 1 && !c->xprev) TRY(dalloc(c, &c->xprev, (size_t)(c->nx + 2) * c->P));
+    if (mode != c->l_x0) c->l_hist = 0;
+    c->l_x0 = mode;
+    return 0;
+}
+
+int cfd_initial_iterate(cfd_ctx* c, int32_t* mode, int32_t* has_history) {
+    if (!c || !mode || !has_history) return set_err(CFD_ERR_ARG, "null argument");
+    *mode = c->l_x0;
+    *has_history = c->l_x0 ? c->l_hist : 0;
+    return 0;
+}
+
 int cfd_learned_stats(cfd_ctx* c, cfd_learned_stats_t* out) {
     if (!c || !out) return set_err(CFD_ERR_ARG, "null argument");
     if (!c->fcg) return set_err(CFD_ERR_STATE, "no learned solver");
@@ -1268,11 +1317,14 @@ int cfd_upload(cfd_ctx* c, int32_t field, const double* host) {
     TRY(shape_of(c, field, &rows, &cols, &oi, &oj));
     int par = 0;
     TRY(host_parity(c, &par));
-    if (field >= CFD_FIELD_NUT && field != CFD_FIELD_ALPHA)
+    if (field >= CFD_FIELD_NUT && field != CFD_FIELD_ALPHA && field != CFD_FIELD_XPREV)
         return set_err(CFD_ERR_ARG, "field %d is computed by a model", field);
     if (field == CFD_FIELD_ALPHA && !c->vof.on) return set_err(CFD_ERR_STATE, "no VOF model set");
     double* dst = field_ptr(c, field, par);
-    if (field == CFD_FIELD_X) c->l_hist = 0;   // a new iterate starts a new history
+    if (field == CFD_FIELD_XPREV && !dst)
+        return set_err(CFD_ERR_STATE, "no extrapolated initial iterate set");
+    if (field == CFD_FIELD_X) c->l_hist = 0;       // a new iterate starts a new history
+    if (field == CFD_FIELD_XPREV) c->l_hist = 1;   // ... which x_prev continues
     HIPC(hipMemcpy2DAsync(dst + (size_t)oi * c->P + oj, c->P * sizeof(double), host,
                           cols * sizeof(double), cols * sizeof(double), rows,
                           hipMemcpyHostToDevice, c->stream));
@@ -1288,6 +1340,8 @@ int cfd_download(cfd_ctx* c, int32_t field, double* host) {
     int par = 0;
     TRY(host_parity(c, &par));
     const double* src = field_ptr(c, field, par);
+    if (!src && field == CFD_FIELD_XPREV)
+        return set_err(CFD_ERR_STATE, "no extrapolated initial iterate set");
     if (!src) return set_err(CFD_ERR_STATE, "field %d: no turbulence model set", field);
     HIPC(hipMemcpy2DAsync(host, cols * sizeof(double), src + (size_t)oi * c->P + oj,
                           c->P * sizeof(double), cols * sizeof(double), rows,

@@ -20,12 +20,14 @@
 // interior diagonal 2 (1/dx^2 + 1/dy^2), sigma the network's output scale, and
 // NN the U-Net of unet.hip (an image of the level-K cells padded to multiples
 // of 8).  Without the network (mode LEARNED_NONE) the coarse correction is
-// c 1 alone; the teacher mode (LEARNED_TEACHER) solves level K to high accuracy
+// c 1 alone; mode LEARNED_VCYCLE continues the V-cycle below level K to the
+// dense coarsest level instead (the complete multigrid preconditioner of
+// cg_mixed, the method the network has to beat); the teacher mode (LEARNED_TEACHER) solves level K to high accuracy
 // (flexible CG preconditioned by V-cycles down to the dense coarsest level)
 // and can store the pair (network input, normalised correction) for training.
 #include "cfd_kernels.h"
 
-enum { LEARNED_NET = 0, LEARNED_NONE = 1, LEARNED_TEACHER = 2 };
+enum { LEARNED_NET = 0, LEARNED_NONE = 1, LEARNED_TEACHER = 2, LEARNED_VCYCLE = 3 };
 
 // iteration after which a network that has not reduced ||r|| tenfold is
 // switched off (the solver then runs as without it)
@@ -773,6 +775,7 @@ struct CoarseNet {
     const float* w;              // FP32 parameters
     float gain;                  // sigma / D
     int use_net;
+    int vcycle;                  // 1: V-cycle levels Lc .. coarsest, dense solve there
     double one_a_one;            // 1^T A_K 1
 };
 
@@ -832,6 +835,35 @@ __device__ __forceinline__ void wg_smooth(const MGCtx& m, int l, const MGLevelDe
         });
 }
 
+// up-leg from level `from` to level Lc by the whole work-group: prolongation
+// of the level below plus the first post-smoothing sweep, then the second
+__device__ __forceinline__ void lv_up_leg(const MGCtx& m, const MGDev& mg, int from, int Lc) {
+    const float w = mg.omega, wc = mg.wc;
+    for (int l = from; l >= Lc; --l) {
+        const MGLevelDev L = mg.lev[l];
+        const MGLevelDev C = mg.lev[l + 1];
+        auto corr = [&](int p, int q) -> float {   // loads in bounds on the ghost ring too
+            const long c = ix(p, q, L.P);
+            const float v = L.t[c] + wc * C.t[ix((p + 1) >> 1, (q + 1) >> 1, C.P)];
+            return mg_diag(m, l, L, c) == 0.0f ? 0.0f : v;
+        };
+        wg_cells_auto(
+            L.nx, L.ny,
+            [&](int i, int j) {
+                const long c = ix(i, j, L.P);
+                CellVal o;
+                o.d = mg_diag(m, l, L, c);
+                const float y = corr(i, j);
+                o.v = y + w * (L.b[c] - mg_apply(m, l, L, i, j, o.d, corr)) / o.d;
+                return o;
+            },
+            [&](int i, int j, const CellVal& o) {
+                if (o.d != 0.0f) L.x[ix(i, j, L.P)] = o.v;
+            });
+        wg_smooth(m, l, L, w, L.x, L.t);
+    }
+}
+
 }  // namespace
 
 __global__ void __launch_bounds__(1024) k_lv_coarse(LvArgs a, int Lc, int K, CoarseNet net) {
@@ -847,8 +879,10 @@ __global__ void __launch_bounds__(1024) k_lv_coarse(LvArgs a, int Lc, int K, Coa
     const MGDev& mg = *a.mg;
     const float w = mg.omega, wc = mg.wc;
     const bool use_net = net.use_net && !fs->net_off;
+    // the last level of the down-leg: K, or the dense coarsest level
+    const int Kd = net.vcycle ? mg.nlev - 1 : K;
     // down-leg: pre-smoothing sweep, restriction of the residual
-    for (int l = Lc; l < K; ++l) {
+    for (int l = Lc; l < Kd; ++l) {
         const MGLevelDev L = mg.lev[l];
         const MGLevelDev C = mg.lev[l + 1];
         wg_smooth(m, l, L, w, L.x, L.t);
@@ -874,6 +908,20 @@ __global__ void __launch_bounds__(1024) k_lv_coarse(LvArgs a, int Lc, int K, Coa
                 const float dc = C.diag[cc];
                 C.x[cc] = dc != 0.0f ? w * bc / dc : 0.0f;
             });
+    }
+    if (net.vcycle) {
+        // coarsest level: t = cinv b (cinv is zero in the columns of solid cells)
+        const MGLevelDev C = mg.lev[Kd];
+        const int n = mg.ncoarse;
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            float acc = 0.0f;
+            for (int q = 0; q < n; ++q)
+                acc += mg.cinv[(long)k * n + q] * C.b[ix(q / C.ny + 1, q % C.ny + 1, C.P)];
+            C.t[ix(k / C.ny + 1, k % C.ny + 1, C.P)] = acc;
+        }
+        __syncthreads();
+        lv_up_leg(m, mg, Kd - 1, Lc);
+        return;
     }
     // level K: c = 1^T b / 1^T A 1 and s = RMS(b - c A 1) over the fluid cells
     const MGLevelDev LK = mg.lev[K];
@@ -929,30 +977,7 @@ __global__ void __launch_bounds__(1024) k_lv_coarse(LvArgs a, int Lc, int K, Coa
     __syncthreads();
     wg_smooth(m, K, LK, w, LK.t, LK.x);
     wg_smooth(m, K, LK, w, LK.x, LK.t);
-    // up-leg: prolongation + post-smoothing sweep, second sweep
-    for (int l = K - 1; l >= Lc; --l) {
-        const MGLevelDev L = mg.lev[l];
-        const MGLevelDev C = mg.lev[l + 1];
-        auto corr = [&](int p, int q) -> float {   // loads in bounds on the ghost ring too
-            const long c = ix(p, q, L.P);
-            const float v = L.t[c] + wc * C.t[ix((p + 1) >> 1, (q + 1) >> 1, C.P)];
-            return mg_diag(m, l, L, c) == 0.0f ? 0.0f : v;
-        };
-        wg_cells_auto(
-            L.nx, L.ny,
-            [&](int i, int j) {
-                const long c = ix(i, j, L.P);
-                CellVal o;
-                o.d = mg_diag(m, l, L, c);
-                const float y = corr(i, j);
-                o.v = y + w * (L.b[c] - mg_apply(m, l, L, i, j, o.d, corr)) / o.d;
-                return o;
-            },
-            [&](int i, int j, const CellVal& o) {
-                if (o.d != 0.0f) L.x[ix(i, j, L.P)] = o.v;
-            });
-        wg_smooth(m, l, L, w, L.x, L.t);
-    }
+    lv_up_leg(m, mg, K - 1, Lc);
 }
 
 // initial iterate of a time step's learned solve: x <- x + theta (x - x_prev)

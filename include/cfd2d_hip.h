@@ -49,7 +49,11 @@ enum { CFD_FIELD_U = 0, CFD_FIELD_V = 1, CFD_FIELD_P = 2, CFD_FIELD_F = 3, CFD_F
        /* two-phase VOF, (nx+2) x (ny+2): the phase fraction (upload sets the
           properties from it) and the mixture density and dynamic viscosity
           (download only) */
-       CFD_FIELD_ALPHA = 13, CFD_FIELD_RHO = 14, CFD_FIELD_MU = 15 };
+       CFD_FIELD_ALPHA = 13, CFD_FIELD_RHO = 14, CFD_FIELD_MU = 15,
+       /* the solution before the last one, nx x ny, kept for the extrapolated
+          initial iterate (cfd_set_initial_iterate mode 1); an upload continues
+          the time series of X (checkpoint restart) */
+       CFD_FIELD_XPREV = 16 };
 /* pressure solvers: Jacobi-PCG in FP64 (the reference's "cg", pressure.py:171),
    and flexible CG with FP64 vectors and dot products preconditioned by an FP32
    aggregation-multigrid V-cycle (set up with cfd_set_multigrid), the solve
@@ -148,6 +152,12 @@ typedef struct cfd_record {
 
 const char* cfd_last_error(void);
 int cfd_device_count(int32_t* count);
+/* End of use (no counterpart in the reference, which holds no device
+   resources): waits for the device.  Callers destroy their contexts,
+   ensembles and networks first and make no further call afterwards; the
+   Python binding does both from an atexit handler, before the process's exit
+   handlers tear the HIP runtime down. */
+int cfd_shutdown(void);
 
 /* Simulation.__init__ device half (solver.py:110-147): allocate and upload. */
 int cfd_ctx_create(const cfd_grid_desc* desc, cfd_ctx** out);
@@ -245,7 +255,9 @@ int cfd_scalar_transport(const double* phi, const double* u, const double* v,
    V-cycle over the multigrid hierarchy (cfd_set_multigrid) that stops at
    `level`, where the U-Net supplies the coarse correction after an exact
    correction in the constant vector.  Modes: 0 network, 1 no network (the
-   constant-vector correction alone), 2 teacher (level solved by
+   constant-vector correction alone), 3 V-cycle (no network: the V-cycle goes on
+   below `level` to the dense coarsest level, the complete multigrid
+   preconditioner), 2 teacher (level solved by
    `teacher_cycles` iterations of flexible CG preconditioned by V-cycles down
    to the dense coarsest level; with harvest
    buffers, each iteration stores the pair (network input, normalised
@@ -256,6 +268,18 @@ int cfd_scalar_transport(const double* phi, const double* u, const double* v,
    switched off for the context (the solver then runs as in mode 1).
    The context keeps `net` (and the harvest buffers) until cfd_set_learned is
    called again or the context is destroyed: the caller keeps them alive. ---- */
+/* Initial iterate of the pressure solves of time steps, for the
+   constant-density solvers (no counterpart in the reference, whose CG starts
+   from the last solution, pressure.py:140, 171-177): mode 0 the last
+   solution, mode 1 the linear extrapolation in time
+   x + (dt/dt_old)(x - x_prev) of the last two solutions (the first solve
+   after the initial projection, an upload of X or a change of mode starts
+   from the last solution).  cfd_learned_desc.x0_extrapolate sets the same
+   mode.  cfd_initial_iterate reports the mode and whether x_prev holds the
+   solution before the last one. */
+int cfd_set_initial_iterate(cfd_ctx* ctx, int32_t mode);
+int cfd_initial_iterate(cfd_ctx* ctx, int32_t* mode, int32_t* has_history);
+
 typedef struct cfd_unet cfd_unet;
 typedef struct cfd_learned_desc {
     cfd_unet* net;
@@ -271,8 +295,8 @@ typedef struct cfd_learned_desc {
     float* harvest_out;
     int64_t harvest_cap;
     int32_t coarse_fused;     /* 1: levels with <= 64 x 64 cells, the network and the
-                                 level-K correction in one work-group launch (modes 0 and
-                                 1 on a 32 x 32 level K; ignored otherwise) */
+                                 level-K correction in one work-group launch (modes 0, 1
+                                 and 3 on a 32 x 32 level K; ignored otherwise) */
     int32_t x0_extrapolate;   /* 1: start each solve of a time step from the linear
                                  extrapolation x + (dt/dt_old)(x - x_previous) of the last
                                  two solutions; 0: from the last solution (SciPy's x0) */

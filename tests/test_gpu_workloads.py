@@ -15,28 +15,34 @@ def _cmp(a, b):
     return float(np.abs(np.asarray(a) - np.asarray(b)).max())
 
 
-def _channel_spec(solver="direct", **extra):
+def _channel_spec(solver="direct", numerik=None, **extra):
     spec = {"fall": "custom",
             "parameter": {"obstacles": [{"typ": "kreis", "x": 0.7, "y": 0.5, "r": 0.12},
                                         {"typ": "kreis", "x": 2.1, "y": 0.4, "r": 0.1}],
                           "lx": 4.0, "ly": 1.0, "nx": 64, "ny": 16, "re": 200, "L_ref": 1.0,
                           "inflow_profile": "parabolic"},
-            "numerik": {"pressure_solver": solver}}
+            "numerik": dict({"pressure_solver": solver}, **(numerik or {}))}
     spec.update(extra)
     return spec
 
 
-@pytest.mark.parametrize("solver,extra", [
-    ("direct", {}),
-    ("cg", {"skalar": {"name": "c", "diffusivity": 1e-3, "inflow_band": [0.25, 0.75]}}),
-    ("cg_mixed", {"skalar": {"name": "c", "diffusivity": 1e-3, "limiter": "superbee"}}),
+@pytest.mark.parametrize("solver,numerik,extra", [
+    ("direct", None, {}),
+    ("cg", None, {"skalar": {"name": "c", "diffusivity": 1e-3, "inflow_band": [0.25, 0.75]}}),
+    ("cg_mixed", None, {"skalar": {"name": "c", "diffusivity": 1e-3, "limiter": "superbee"}}),
+    ("cg", {"pressure_x0": "extrapolate", "pressure_tol": 1e-6}, {}),
+    ("cg_learned", {"pressure_tol": 1e-6, "learned": {"mode": "net", "x0": "extrapolate"}},
+     {"skalar": {"name": "c", "diffusivity": 1e-3}}),
 ])
-def test_checkpoint_restart_is_exact(gpu, tmp_path, solver, extra):
+def test_checkpoint_restart_is_exact(gpu, tmp_path, solver, numerik, extra):
     """tests/test_functions.py:48-70 of the reference: save at step 40 of 80,
-    reload into a fresh case, continue; the CG warm start travels in the file."""
+    reload into a fresh case, continue; the CG warm start travels in the file,
+    and with the extrapolated initial iterate the solution before the last
+    one as well (druck_xprev)."""
     from cfd2d_amd import cases, functions
 
-    spec = _channel_spec(solver, **extra)
+    print(f"restart case: {solver} {numerik}", flush=True)
+    spec = _channel_spec(solver, numerik, **extra)
     ref = cases.build_case(spec)
     for _ in range(80):
         ref.sim.step()
@@ -44,6 +50,8 @@ def test_checkpoint_restart_is_exact(gpu, tmp_path, solver, extra):
     for _ in range(40):
         a.sim.step()
     functions.save_checkpoint(a.sim, tmp_path / "cp.npz")
+    extrapolated = "extrapolate" in json.dumps(numerik or {})
+    assert ("druck_xprev" in np.load(tmp_path / "cp.npz").files) == extrapolated
     b = cases.build_case(spec)
     functions.load_checkpoint(b.sim, tmp_path / "cp.npz")
     for _ in range(40):
@@ -72,7 +80,8 @@ def test_checkpoint_layout_matches_reference(gpu, tmp_path):
     for _ in range(30):
         case.sim.step()
     d = np.load(functions.save_checkpoint(case.sim, tmp_path / "cp.npz"), allow_pickle=False)
-    mine = {k: [str(d[k].dtype), list(d[k].shape)] for k in d.files if k != "druck_x0"}
+    mine = {k: [str(d[k].dtype), list(d[k].shape)] for k in d.files
+            if k not in ("druck_x0", "druck_xprev")}
     # history_json is a unicode scalar whose length depends on the float reprs
     assert mine.pop("history_json")[0].startswith("<U")
     assert layout.pop("history_json")[0].startswith("<U")
