@@ -137,6 +137,8 @@ struct cfd_ctx {
     int l_ready = 0;
     int l_level = 0, l_mode = 0, l_cycles = 0, l_fallback = 1, l_graph = 1, l_maxiter = 200;
     int l_fused = 0, l_lc = 0;   // coarse levels l_lc .. l_level in one k_lv_coarse launch
+    int l_vlds = 0;              // mode V-cycle: levels l_lc .. coarsest in LDS (k_lv_vcoarse)
+    VcArgs l_vargs;
     int l_x0 = 0, l_hist = 0;    // extrapolated initial iterate; x_prev holds the last solution
     double* xprev = nullptr;
     int l_H = 0, l_W = 0;
@@ -714,6 +716,19 @@ int learned_iteration(cfd_ctx* c) {
         LV_LAUNCH512(k_lv_smooth_dots, nb0, a, v);
         return 0;
     };
+    if (c->l_vlds) {   // the complete V-cycle with levels >= l_lc in one LDS work-group
+        const int Lc = c->l_lc;
+        for (int l = 0; l < Lc; ++l) TRY(down(l));
+        hipLaunchKernelGGL(k_lv_vcoarse, dim3(1), dim3(1024), 0, c->stream, c->l_vargs);
+        HIPC(hipGetLastError());
+        for (int l = Lc - 1; l >= 1; --l) TRY(up(l));
+        TRY(up0());
+        LV_LAUNCH512(k_fcg_dir, nb0, a, v, nb0);
+        LV_LAUNCH512(k_fcg_update, nb0, a, v, nb0);
+        LV_LAUNCH512(k_fcg_true, nb0, a, v, nb0);
+        LV_LAUNCH(k_fcg_true_fin, 1, a, nb0, 0);
+        return 0;
+    }
     if (c->l_fused) {
         const int Lc = c->l_lc;
         for (int l = 0; l < Lc; ++l) TRY(down(l));
@@ -1247,7 +1262,47 @@ int cfd_set_learned(cfd_ctx* c, const cfd_learned_desc* d) {
     // work-group sweeps 64 x 64 cells faster than a launch costs, 128 x 128
     // cells slower
     c->l_fused = 0;
-    if (d->coarse_fused && d->mode != LEARNED_TEACHER && d->level >= 1 && L.nx == 32 &&
+    c->l_vlds = 0;
+    if (d->coarse_fused && d->mode == LEARNED_VCYCLE) {
+        // the first level below the fine one with at most 64 x 64 cells, when
+        // it and all coarser levels fit in the LDS pool of k_lv_vcoarse
+        int lc = 1;
+        while (lc < mg.nlev - 1 && (long)mg.lev[lc].nx * mg.lev[lc].ny > 64L * 64) ++lc;
+        if (mg.nlev >= 2 && mg.nlev - lc <= VC_MAXL && mg.ncoarse <= VC_MAXC &&
+            vc_need(mg, lc) <= VC_POOL) {
+            c->l_lc = lc;
+            c->l_vlds = 1;
+            VcArgs& v = c->l_vargs;
+            std::memset(&v, 0, sizeof(v));
+            v.fs = c->fcg;
+            v.cinv = mg.cinv;
+            v.omega = mg.omega;
+            v.wc = mg.wc;
+            v.periodic = c->periodic;
+            v.nl = mg.nlev - lc;
+            v.nc = mg.ncoarse;
+            int off = 0;
+            for (int q = 0; q < v.nl; ++q) {
+                const MGLevelDev& G = mg.lev[lc + q];
+                VcLev& L = v.lev[q];
+                L.nx = G.nx;
+                L.ny = G.ny;
+                L.pw = G.ny + 2;
+                L.sz = (G.nx + 2) * L.pw;
+                L.off = off;
+                L.P = G.P;
+                L.m = ((1ull << 32) + (unsigned long long)G.ny - 1) / (unsigned long long)G.ny;
+                L.gd = G.diag;
+                L.ge = G.cE;
+                L.gn = G.cN;
+                L.gb = G.b;
+                L.gt = G.t;
+                off += VC_ARR * L.sz;
+                v.total += L.sz;
+            }
+        }
+    }
+    if (!c->l_vlds && d->coarse_fused && d->mode != LEARNED_TEACHER && d->level >= 1 && L.nx == 32 &&
         L.ny == 32 && (d->mode != LEARNED_NET || (d->net->H == 32 && d->net->W == 32))) {
         int lc = d->level;
         while (lc - 1 >= 1 && (long)mg.lev[lc - 1].nx * mg.lev[lc - 1].ny <= 64L * 64) --lc;

@@ -980,6 +980,256 @@ __global__ void __launch_bounds__(1024) k_lv_coarse(LvArgs a, int Lc, int K, Coa
     lv_up_leg(m, mg, K - 1, Lc);
 }
 
+// ---- the complete V-cycle's coarse end in LDS ------------------------------
+// Levels Lc .. coarsest (Lc: the first level of at most 64 x 64 cells) of one
+// V-cycle application, held in LDS for the whole launch: per level b, x, t and
+// the coefficients diag, cE, cN at the tight pitch ny + 2 with the zero ghost
+// ring.  The sweeps, restrictions, dense coarsest solve and prolongations are
+// the expressions of k_lv_smooth / k_lv_restrict / k_lv_dense / k_lv_post in
+// the same order, so the result is bit-identical to the launch sequence; only
+// level Lc's b comes from global memory (the previous launch's restriction)
+// and only its t goes back (for the next launch's prolongation).
+constexpr int VC_POOL = 36864;           // LDS floats (144 KB)
+constexpr int VC_ARR = 6;                // b, x, t, diag, cE, cN per level
+constexpr int VC_MAXL = 8;               // levels held in LDS
+constexpr int VC_LOADS = VC_POOL / VC_ARR / 1024;   // coefficient entries per thread
+constexpr int VC_MAXC = 64;              // cells of the dense coarsest level
+
+// LDS floats needed for levels Lc .. nlev-1 (host and device)
+__host__ __device__ inline long vc_need(const MGDev& mg, int Lc) {
+    long s = 0;
+    for (int l = Lc; l < mg.nlev; ++l)
+        s += (long)VC_ARR * (mg.lev[l].nx + 2) * (mg.lev[l].ny + 2);
+    return s;
+}
+
+// one LDS level: sizes, the offset of its six arrays in the pool, the
+// multiplier of the exact division q / ny for q < 2^16 (m = ceil(2^32 / ny)),
+// and the global arrays it is loaded from / written back to
+struct VcLev {
+    int nx, ny, pw, sz, off, P;
+    unsigned long long m;
+    const float *gd, *ge, *gn, *gb;
+    float* gt;
+};
+
+// arguments of k_lv_vcoarse, built on the host from the hierarchy (kernel
+// arguments: no descriptor loads on the device)
+struct VcArgs {
+    FcgState* fs;
+    const float* cinv;       // dense inverse of the coarsest level
+    float omega, wc;
+    int periodic, nl, nc;    // levels in LDS, cells of the coarsest level
+    int total;               // sum of sz over the levels
+    VcLev lev[VC_MAXL];
+};
+
+namespace {
+
+struct VcView {
+    int nx, ny, pw;
+    unsigned long long m;
+    float *b, *x, *t, *d, *e, *n;
+};
+
+__device__ __forceinline__ VcView vc_view(float* pool, const VcLev& L) {
+    VcView v;
+    v.nx = L.nx;
+    v.ny = L.ny;
+    v.pw = L.pw;
+    v.m = L.m;
+    float* p = pool + L.off;
+    v.b = p;
+    v.x = p + L.sz;
+    v.t = p + 2 * L.sz;
+    v.d = p + 3 * L.sz;
+    v.e = p + 4 * L.sz;
+    v.n = p + 5 * L.sz;
+    return v;
+}
+
+// interior cell q (row-major, 0-based) -> (i, j), 1-based
+__device__ __forceinline__ void vc_ij(const VcView& L, int q, int& i, int& j) {
+    const int r = (int)(((unsigned long long)(unsigned)q * L.m) >> 32);
+    i = r + 1;
+    j = q - r * L.ny + 1;
+}
+
+// (A v)(i, j) on an LDS level, mg_apply's expression for levels >= 1
+__device__ __forceinline__ float vc_apply(const VcView& L, int periodic, int i, int j, float dc,
+                                          const float* v) {
+    const int iE = (periodic && i == L.nx) ? 1 : i + 1;
+    const int iW = (periodic && i == 1) ? L.nx : i - 1;
+    const int k = i * L.pw + j;
+    return dc * v[k] - L.e[k] * v[iE * L.pw + j] - L.e[iW * L.pw + j] * v[iW * L.pw + j] -
+           L.n[k] * v[k + 1] - L.n[k - 1] * v[k - 1];
+}
+
+// dst = src + w (b - A src) / d on the fluid cells of an LDS level
+__device__ __forceinline__ void vc_smooth(const VcView& L, int periodic, float w, const float* src,
+                                          float* dst) {
+    const int n = L.nx * L.ny;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        int i, j;
+        vc_ij(L, q, i, j);
+        const int k = i * L.pw + j;
+        const float d = L.d[k];
+        if (d == 0.0f) continue;
+        dst[k] = src[k] + w * (L.b[k] - vc_apply(L, periodic, i, j, d, src)) / d;
+    }
+    __syncthreads();
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(1024) k_lv_vcoarse(VcArgs a) {
+    __shared__ float pool[VC_POOL];
+    __shared__ float s_bc[VC_MAXC];
+    const int nl = a.nl, nc = a.nc, per = a.periodic;
+    const float w = a.omega, wc = a.wc;
+    const int tid = threadIdx.x, bs = blockDim.x;
+    // every global load of the launch in flight at once: the solve's state,
+    // this lane's row of the dense inverse (wave 0), the coefficients of every
+    // level (ghost ring included) and level Lc's right-hand side
+    const int done = a.fs->done;
+    float crow[VC_MAXC];
+    if (tid < nc) {
+#pragma unroll
+        for (int q = 0; q < VC_MAXC; ++q) crow[q] = q < nc ? a.cinv[(long)tid * nc + q] : 0.0f;
+    }
+    float rd[VC_LOADS], re[VC_LOADS], rn[VC_LOADS], rb[VC_LOADS];
+    int dst[VC_LOADS];
+#pragma unroll
+    for (int u = 0; u < VC_LOADS; ++u) {
+        int e = tid + u * bs;
+        dst[u] = -1;
+        rd[u] = re[u] = rn[u] = rb[u] = 0.0f;
+        if (e >= a.total) continue;
+        int q = 0;
+        while (e >= a.lev[q].sz) e -= a.lev[q++].sz;
+        const VcLev& L = a.lev[q];
+        const int i = e / L.pw, j = e - i * L.pw;
+        const long g = ix(i, j, L.P);
+        rd[u] = L.gd[g];
+        re[u] = L.ge[g];
+        rn[u] = L.gn[g];
+        if (q == 0 && i >= 1 && i <= L.nx && j >= 1 && j <= L.ny) rb[u] = L.gb[g];
+        dst[u] = (q << 24) | e;   // level and position within its arrays
+    }
+    if (done) return;
+    // zero b, x, t of every level (ghost rings, solid cells) meanwhile
+    for (int q = 0; q < nl; ++q) {
+        const VcLev& L = a.lev[q];
+        for (int k = tid; k < 3 * L.sz; k += bs) pool[L.off + k] = 0.0f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < VC_LOADS; ++u) {
+        if (dst[u] < 0) continue;
+        const int q = dst[u] >> 24, e = dst[u] & 0xFFFFFF;
+        const VcView V = vc_view(pool, a.lev[q]);
+        V.d[e] = rd[u];
+        V.e[e] = re[u];
+        V.n[e] = rn[u];
+        if (q == 0) {   // level Lc: b and the first sweep from zero, x = w b / d
+            V.b[e] = rb[u];
+            V.x[e] = rd[u] != 0.0f ? w * rb[u] / rd[u] : 0.0f;
+        }
+    }
+    __syncthreads();
+    // down-leg: second pre-smoothing sweep, residual, restriction
+    for (int q = 0; q < nl - 1; ++q) {
+        const VcView L = vc_view(pool, a.lev[q]);
+        const VcView C = vc_view(pool, a.lev[q + 1]);
+        vc_smooth(L, per, w, L.x, L.t);
+        // r = b - A t into the x slot: the pass reads t, and x is not read again
+        // before the up-leg rewrites it (vc_smooth's barrier ended its reads)
+        const int n = L.nx * L.ny;
+        for (int c = tid; c < n; c += bs) {
+            int i, j;
+            vc_ij(L, c, i, j);
+            const int k = i * L.pw + j;
+            const float d = L.d[k];
+            if (d != 0.0f) L.x[k] = L.b[k] - vc_apply(L, per, i, j, d, L.t);
+        }
+        __syncthreads();
+        const int ncl = C.nx * C.ny;
+        for (int c = tid; c < ncl; c += bs) {
+            int ci, cj;
+            vc_ij(C, c, ci, cj);
+            const int kc = ci * C.pw + cj;
+            float bc = 0.0f;
+            for (int di = 0; di < 2; ++di)
+                for (int dj = 0; dj < 2; ++dj) {
+                    const int i = 2 * ci - 1 + di, j = 2 * cj - 1 + dj;
+                    if (i > L.nx || j > L.ny) continue;
+                    const int k = i * L.pw + j;
+                    if (L.d[k] != 0.0f) bc += L.x[k];
+                }
+            C.b[kc] = bc;
+            if (q + 1 < nl - 1) {   // (the coarsest level's x is not used)
+                const float dc = C.d[kc];
+                C.x[kc] = dc != 0.0f ? w * bc / dc : 0.0f;
+            } else if (c < VC_MAXC) {
+                s_bc[c] = bc;       // the coarsest right-hand side, compact
+            }
+        }
+        __syncthreads();
+    }
+    // coarsest level: t = cinv b, the sum in k_lv_dense's order
+    {
+        const VcView C = vc_view(pool, a.lev[nl - 1]);
+        if (tid < nc) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int q = 0; q < VC_MAXC; ++q)
+                if (q < nc) acc += crow[q] * s_bc[q];
+            int i, j;
+            vc_ij(C, tid, i, j);
+            C.t[i * C.pw + j] = acc;
+        }
+        __syncthreads();
+    }
+    // up-leg: prolongation + first post-smoothing sweep into x, second into t
+    for (int q = nl - 2; q >= 0; --q) {
+        const VcView L = vc_view(pool, a.lev[q]);
+        const VcView C = vc_view(pool, a.lev[q + 1]);
+        const int n = L.nx * L.ny;
+        for (int c = tid; c < n; c += bs) {
+            int i, j;
+            vc_ij(L, c, i, j);
+            const int k = i * L.pw + j;
+            const float d = L.d[k];
+            if (d == 0.0f) continue;
+            auto corr = [&](int p, int r) -> float {
+                const int kk = p * L.pw + r;
+                const float v = L.t[kk] + wc * C.t[((p + 1) >> 1) * C.pw + ((r + 1) >> 1)];
+                return L.d[kk] == 0.0f ? 0.0f : v;
+            };
+            const int iE = (per && i == L.nx) ? 1 : i + 1;
+            const int iW = (per && i == 1) ? L.nx : i - 1;
+            const float y = corr(i, j);
+            const float Ay = d * y - L.e[k] * corr(iE, j) - L.e[iW * L.pw + j] * corr(iW, j) -
+                             L.n[k] * corr(i, j + 1) - L.n[k - 1] * corr(i, j - 1);
+            L.x[k] = y + w * (L.b[k] - Ay) / d;
+        }
+        __syncthreads();
+        vc_smooth(L, per, w, L.x, L.t);
+    }
+    // level Lc's t for the prolongation launch of level Lc - 1
+    {
+        const VcLev& G = a.lev[0];
+        const VcView L = vc_view(pool, G);
+        const int n = L.nx * L.ny;
+        for (int c = tid; c < n; c += bs) {
+            int i, j;
+            vc_ij(L, c, i, j);
+            const int k = i * L.pw + j;
+            if (L.d[k] != 0.0f) G.gt[ix(i, j, G.P)] = L.t[k];
+        }
+    }
+}
+
 // initial iterate of a time step's learned solve: x <- x + theta (x - x_prev)
 // with theta = dt / dt_old, and x_prev <- the old x (hist = 0: the copy only)
 __global__ void __launch_bounds__(256) k_x0_extrap(double* __restrict__ x, double* __restrict__ xprev,
