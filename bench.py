@@ -4,14 +4,21 @@ Workload (one GPU): lid-driven cavity Re=1000 on 1024x1024, AB2, gamma "auto",
 tau 0.5, with a passive van Leer scalar (D = 1e-3) initialised as 8 Gaussian
 blobs (seed 0).  Every step is the whole projection step (BCs, dt/gamma,
 predictor, pressure solve to relative residual 1e-6, correction, scalar
-transport, monitors), all inputs resident in HBM.  Solver legs:
+transport, monitors), all inputs resident in HBM.  The timed steps are
+warmup+1 .. warmup+steps from rest (``timed_steps``).  Solver legs:
 
-* ``cg``               plain FP64 Jacobi-PCG (the reference's method, the baseline);
-* ``cg_learned``       the learned flexible CG (FP64 outer, FP32 V-cycle, BF16
-                       U-Net coarse correction; the shipped trained weights): the
-                       headline ``value``;
-* ``cg_learned_none``  the same solver without the network (the gain it brings);
-* ``cg_mixed``         FP64 flexible CG with the FP32 multigrid V-cycle.
+* ``cg``                 plain FP64 Jacobi-PCG (the reference's method, the
+                         baseline), warm-started from the last solution as the
+                         reference's CG is; ``cg_x0extrap`` the same from the
+                         extrapolated start of the mixed-precision legs
+                         (``speedup_same_x0``: the like-for-like ratio);
+* ``cg_learned_vcycle``  flexible CG with FP64 vectors and dots and an FP32
+                         aggregation V-cycle down to the dense coarsest solve
+                         (coarse levels in LDS): the headline ``value``;
+* ``cg_learned``         the same with the BF16 U-Net coarse correction on the
+                         32x32 level (shipped trained weights) instead of the
+                         levels below it; ``cg_learned_none`` with neither;
+* ``cg_mixed``           the V-cycle FCG as one persistent cooperative launch.
 
 ``ensemble`` adds the batched-ensemble workload (256 lid / heated cavities of
 sides 64-512, FP32 bulk, shared learned PCG) as aggregate cell updates per
@@ -22,8 +29,11 @@ headline leg (it takes nearly all of the step): algorithmic bytes per
 projection (per cell: 80 B for the right-hand side, pressure write-back and
 correction, plus per iteration the FP64 outer vectors and the FP32 V-cycle,
 counted per kernel below) over its mean duration, timed with HIP events on
-the launch stream.  ``cpu_baseline`` times the CPU FP64 oracle (SciPy CG) on
-one host core for a bounded number of steps of the same workload.
+the launch stream.  ``errors``: max-abs u, v, p, phi of the headline, network
+and FP64 legs after the same steps from rest against the FP64 oracle (exact
+DCT solves).  ``cpu_baseline`` times the CPU FP64 oracle: one pinned core with
+SciPy's CG (``value``) and every core with the OpenMP C CG, on a sample of
+the 1024^2 workload and on config C1 (64^2, 2000 steps) in full.
 """
 
 from __future__ import annotations
@@ -47,8 +57,11 @@ N = 1024
 RE = 1000.0
 RTOL = 1e-6
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_RATIO_MGCG = 1.10
-TRAFFIC_RATIO_LEARNED = 1.20
+# measured HBM traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a 256 MiB
+# copy) over the algorithmic bytes of the same projection kernels
+TRAFFIC_RATIO_MGCG = 1.10        # profiles/k_project_traffic_cavity1024.md
+TRAFFIC_RATIO_LEARNED = 1.20     # network leg, profiles/learned_projection_cavity1024.md
+TRAFFIC_RATIO_VCYCLE = 1.154     # steps 6-25, profiles/learned_vcycle_projection_cavity1024.md
 WORKLOAD = ("lid-driven cavity Re=1000, 1024x1024 MAC grid, AB2, gamma auto, tau 0.5, "
             "passive van Leer scalar (D=1e-3, 8 Gaussian blobs, seed 0), "
             "pressure solve to relative residual 1e-6")
@@ -91,6 +104,13 @@ TURB_LEGS = {"cg_learned_smagorinsky": ("cg_learned", "Smagorinsky"),
 
 
 USE_GRAPH = True   # learned legs: replay each iteration as a captured HIP graph
+LEG_DETAIL = {
+    "cg_learned_vcycle": ("flexible CG, FP64 vectors and dots, FP32 aggregation V-cycle whose "
+                          "coarse end (levels <= 64x64 to the dense coarsest solve) runs in LDS; "
+                          "U-Net not used: it loses to this complete coarse solve"),
+    "cg_learned": ("flexible CG, FP64 vectors and dots, FP32 V-cycle with the BF16 U-Net "
+                   "coarse correction on the 32x32 level"),
+}
 
 
 def build_case(leg: str, n: int = N):
@@ -214,7 +234,13 @@ def roofline_for(leg: dict) -> dict:
     learned = leg["solver"].startswith("cg_learned")
     # measured HBM traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated against a
     # 256 MiB copy) over the algorithmic bytes of the same launches
-    if learned:
+    if leg["solver"] == "cg_learned_vcycle":
+        traffic = round(TRAFFIC_RATIO_VCYCLE * bytes_per_launch)
+        source = ("rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE summed over the projection kernels "
+                  "of steps 6-25 (the --warmup 5 --steps 20 window): 1216.3 MB per step at "
+                  "4.7 iterations, 1.154 times the algorithmic bytes; "
+                  "profiles/learned_vcycle_projection_cavity1024.md")
+    elif learned:
         traffic = round(TRAFFIC_RATIO_LEARNED * bytes_per_launch)
         source = ("rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE summed over the projection kernels "
                   "of 10 steps (ratio 1.20 to algorithmic), "
@@ -265,43 +291,195 @@ def run_ensemble(steps: int = 10, warmup: int = 3, net: bool = True) -> dict:
 
 
 # --------------------------------------------------------------------------- #
-# CPU baseline: the FP64 oracle (NumPy/SciPy + serial C kernels) on one core
+# CPU baseline and accuracy: the FP64 oracle (NumPy/SciPy + C kernels)
 # --------------------------------------------------------------------------- #
-def cpu_baseline_child(budget_s: float, max_steps: int) -> dict:
+# config C1: 64^2 cavity Re 100, gamma 0, AB2, tau 0.5, CG rtol 1e-8, 2000 steps
+C1 = {"re": 100.0, "n": 64, "gamma": 0.0, "scheme": "ab2", "tau": 0.5, "tol": 1e-8,
+      "steps": 2000}
+ERR_STEPS = 25      # steps from rest after which the per-field errors are taken
+
+
+def host_cores() -> int:
+    return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+
+
+def _oracle_c2(solver: str):
     from oracle import cfd_oracle as O
 
-    t0 = time.perf_counter()
-    sim = O.cavity(re=RE, n=N, pressure_solver="cg", pressure_tol=RTOL)
+    sim = O.cavity(re=RE, n=N, pressure_solver=solver, pressure_tol=RTOL)
     sim.pressure.maxiter = 50000
     O.add_scalar(sim, "c", 1e-3, limiter="vanleer", initial=gaussian_blobs(N))
-    setup = time.perf_counter() - t0
+    return sim
+
+
+def _oracle_c1(solver: str):
+    from oracle import cfd_oracle as O
+
+    sim = O.cavity(re=C1["re"], n=C1["n"], gamma=C1["gamma"], scheme=C1["scheme"],
+                   tau=C1["tau"], pressure_solver=solver, pressure_tol=C1["tol"])
+    sim.pressure.maxiter = 50000
+    return sim
+
+
+def cpu_child(mode: str, arg: str, budget_s: float) -> dict:
+    """One CPU measurement in a child process (its thread count and pinning
+    set by the parent).  c2: steps of the 1024^2 workload from rest until the
+    budget (at least `arg` steps); c1: config C1 in full, then the max-abs
+    error of the GPU's C1 fields (npz at `arg`, if given) against it;
+    errors: the oracle with exact (DCT) solves for the steps of the npz at
+    `arg`, and each GPU leg's max-abs field errors against it."""
     t0 = time.perf_counter()
-    k = 0
-    while k < max_steps and time.perf_counter() - t0 < budget_s:
-        sim.step()
-        k += 1
-    wall = time.perf_counter() - t0
-    return {"steps": k, "wall_s": wall, "setup_s": setup, "cg_iters": sim.cg_iters}
+    if mode in ("c2_1", "c2_all"):
+        sim = _oracle_c2("cg" if mode == "c2_1" else "cg_omp")
+        setup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        k = 0
+        while k < int(arg) or time.perf_counter() - t0 < budget_s:
+            sim.step()
+            k += 1
+        return {"steps": k, "wall_s": time.perf_counter() - t0, "setup_s": setup,
+                "cg_iters": sim.cg_iters}
+    if mode in ("c1_1", "c1_all"):
+        sim = _oracle_c1("cg" if mode == "c1_1" else "cg_omp")
+        t0 = time.perf_counter()
+        for _ in range(C1["steps"]):
+            sim.step()
+        out = {"steps": C1["steps"], "wall_s": time.perf_counter() - t0,
+               "cg_iters_mean": float(np.mean(sim.cg_iters))}
+        if arg != "-":
+            g = np.load(arg)
+            out["errors"] = {f: float(np.abs(g[f] - getattr(sim, f)).max())
+                             for f in ("u", "v", "p")}
+        return out
+    if mode == "errors":
+        g = np.load(arg)
+        steps = int(g["steps"])
+        sim = _oracle_c2("dct")
+        for _ in range(steps):
+            sim.step()
+        ref = {"u": sim.u, "v": sim.v, "p": sim.p, "phi": sim.scalars["c"].phi}
+        out = {"steps": steps, "wall_s": time.perf_counter() - t0}
+        for leg in str(g["legs"]).split(","):
+            out[leg] = {f: float(np.abs(g[f"{leg}_{f}"] - ref[f]).max()) for f in ref}
+        return out
+    raise ValueError(mode)
 
 
-def cpu_baseline(budget_s: float = 20.0, max_steps: int = 30) -> dict | None:
+def run_cpu_child(mode: str, arg: str, budget_s: float, threads: int, pin: bool,
+                  timeout: float = 900.0) -> dict | None:
     env = dict(os.environ)
-    env.update({"OMP_NUM_THREADS": "1", "OPENBLAS_NUM_THREADS": "1", "MKL_NUM_THREADS": "1"})
-    cmd = [sys.executable, str(Path(__file__).resolve()), "--cpu-child", str(budget_s),
-           str(max_steps)]
+    t = str(threads)
+    env.update({"OMP_NUM_THREADS": t, "OPENBLAS_NUM_THREADS": t, "MKL_NUM_THREADS": t})
+    cmd = [sys.executable, str(Path(__file__).resolve()), "--cpu-child", mode, arg, str(budget_s),
+           "1" if pin else "0"]
     try:
-        res = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=600)
+        res = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=timeout)
         line = [l for l in res.stdout.splitlines() if l.startswith("{")][-1]
-        r = json.loads(line)
+        return json.loads(line)
     except Exception as e:  # pragma: no cover - reported, not fatal
-        print(f"cpu baseline failed: {e}", file=sys.stderr)
+        print(f"cpu child {mode} failed: {e}", file=sys.stderr, flush=True)
         return None
-    return {"value": round(r["steps"] / r["wall_s"], 4), "unit": "steps/s", "cores": 1,
-            "kind": "port",
-            "sample": (f"FP64 oracle (serial C kernels + SciPy Jacobi-CG, rtol 1e-6), "
-                       f"first {r['steps']} steps of the same 1024x1024 workload from rest, "
-                       f"{r['wall_s']:.1f} s, one thread; CG iterations "
-                       f"{r['cg_iters']}")}
+
+
+def gpu_fields(leg: str, steps: int) -> dict:
+    """u, v, p and the scalar of a GPU leg after `steps` steps from rest."""
+    sim = build_case(leg).sim
+    ctx = sim._ctx
+    ctx.step(sim._params(), steps)
+    ctx.sync()
+    sim._flush()
+    out = {"u": sim.u.copy(), "v": sim.v.copy(), "p": sim.p.copy(),
+           "phi": sim.scalars["c"].phi.copy()}
+    sim.close()
+    return out
+
+
+def gpu_c1() -> tuple[dict, float]:
+    """Config C1 on the GPU (FP64 Jacobi-PCG, the reference's method): the
+    final fields and the steps per second."""
+    from cfd2d_amd import cases
+
+    sim = cases.lid_driven_cavity(re=C1["re"], n=C1["n"], gamma=C1["gamma"],
+                                  scheme=C1["scheme"], tau=C1["tau"], pressure_solver="cg",
+                                  pressure_tol=C1["tol"], pressure_maxiter=50000).sim
+    ctx = sim._ctx
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(sim._params(), C1["steps"])
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    sim._flush()
+    out = {"u": sim.u.copy(), "v": sim.v.copy(), "p": sim.p.copy()}
+    sim.close()
+    return out, C1["steps"] / wall
+
+
+def cpu_baseline(gpu_c1_path: str | None, c1_gpu_rate: float | None) -> dict | None:
+    """The FP64 oracle on the host: (i) one pinned core with SciPy's CG (the
+    reference's single-threaded configuration), (ii) every core of this
+    process with the C Jacobi-CG and the gather-form kernels on OpenMP.  C2
+    (the 1024^2 workload) is sampled from rest; C1 runs in full."""
+    cores = host_cores()
+    c2_1 = run_cpu_child("c2_1", "1", 20.0, 1, True)
+    c2_all = run_cpu_child("c2_all", "3", 20.0, cores, False)
+    c1_1 = run_cpu_child("c1_1", gpu_c1_path or "-", 0.0, 1, True)
+    c1_all = run_cpu_child("c1_all", "-", 0.0, cores, False)
+    if c2_1 is None:
+        return None
+
+    def rate(r):
+        return round(r["steps"] / r["wall_s"], 4) if r else None
+
+    out = {"value": rate(c2_1), "unit": "steps/s", "cores": 1, "kind": "port",
+           "sample": (f"FP64 oracle (C kernels + SciPy Jacobi-CG, rtol 1e-6), first "
+                      f"{c2_1['steps']} step(s) of the same 1024x1024 workload from rest, "
+                      f"{c2_1['wall_s']:.1f} s on one pinned core; CG iterations "
+                      f"{c2_1['cg_iters']}.  steps/s = steps / wall time of the sample; "
+                      f"the first steps from rest need the most CG iterations, so the "
+                      f"sample understates the rate of later steps"),
+           "host_cores": cores}
+    if c2_all:
+        out["all_cores"] = {"value": rate(c2_all), "unit": "steps/s", "cores": cores,
+                            "sample": (f"first {c2_all['steps']} steps from rest, "
+                                       f"{c2_all['wall_s']:.1f} s, C Jacobi-CG and gather-form "
+                                       f"kernels on {cores} OpenMP threads; CG iterations "
+                                       f"{c2_all['cg_iters']}")}
+    c1 = {"workload": "config C1: 64x64 cavity Re 100, gamma 0, AB2, tau 0.5, CG rtol 1e-8, "
+                      "2000 steps from rest"}
+    if c1_1:
+        c1["one_core_steps_per_s"] = rate(c1_1)
+        c1["cg_iters_mean"] = round(c1_1["cg_iters_mean"], 2)
+        if "errors" in c1_1:
+            c1["gpu_fp64_errors"] = c1_1["errors"]
+    if c1_all:
+        c1["all_cores_steps_per_s"] = rate(c1_all)
+    if c1_gpu_rate:
+        c1["gpu_fp64_steps_per_s"] = round(c1_gpu_rate, 2)
+    out["config1"] = c1
+    return out
+
+
+def field_errors(legs: list[str], steps: int, tmpdir: Path) -> dict | None:
+    """Max-abs errors of u, v, p, phi after `steps` steps from rest: each GPU
+    leg against the FP64 oracle with exact (DCT) pressure solves, and the
+    first leg against the GPU FP64 Jacobi-PCG leg ("cg")."""
+    data = {"steps": np.int64(steps), "legs": np.array(",".join(legs))}
+    fields = {}
+    for leg in legs:
+        fields[leg] = gpu_fields(leg, steps)
+        for f, a in fields[leg].items():
+            data[f"{leg}_{f}"] = a
+    path = tmpdir / "gpu_fields.npz"
+    np.savez(path, **data)
+    r = run_cpu_child("errors", str(path), 0.0, host_cores(), False)
+    if r is None:
+        return None
+    out = {"steps_from_rest": steps, "reference": "FP64 oracle, exact (DCT) pressure solves",
+           "vs_oracle": {leg: r[leg] for leg in legs}}
+    if "cg" in fields and legs[0] != "cg":
+        out["vs_gpu_fp64_cg"] = {f: float(np.abs(fields[legs[0]][f] - fields["cg"][f]).max())
+                                 for f in fields["cg"]}
+    return out
 
 
 def copy_calibration(nbytes: int = 256 << 20) -> None:
@@ -332,7 +510,8 @@ def main() -> None:
     ap.add_argument("--legs", default="cg,cg_x0extrap,cg_learned,cg_learned_vcycle,"
                                       "cg_learned_none,cg_learned_x0prev,cg_mixed,"
                                       "cg_learned_smagorinsky,cg_learned_sst,vof_dam_break")
-    ap.add_argument("--cpu-child", nargs=2, default=None)
+    ap.add_argument("--cpu-child", nargs=4, default=None,
+                    metavar=("MODE", "ARG", "BUDGET_S", "PIN"))
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device-to-device (a known-size kernel for "
                          "scaling rocprof FETCH_SIZE / WRITE_SIZE)")
@@ -343,7 +522,10 @@ def main() -> None:
     global USE_GRAPH
     USE_GRAPH = not args.no_graph
     if args.cpu_child:
-        print(json.dumps(cpu_baseline_child(float(args.cpu_child[0]), int(args.cpu_child[1]))))
+        mode, arg, budget, pin = args.cpu_child
+        if pin == "1" and hasattr(os, "sched_setaffinity"):
+            os.sched_setaffinity(0, {min(os.sched_getaffinity(0))})
+        print(json.dumps(cpu_child(mode, arg, float(budget))))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path is "
@@ -363,7 +545,10 @@ def main() -> None:
         leg = legs.get(name)
         return leg if leg is not None and "error" not in leg else None
 
-    head = (good("cg_learned") or good("cg_mixed") or good("cg")
+    # the headline: the fastest mixed-precision flexible CG of the learned
+    # solver's family -- with the complete V-cycle coarse end (which beats the
+    # network's approximate coarse solve), else with the network
+    head = (good("cg_learned_vcycle") or good("cg_learned") or good("cg_mixed") or good("cg")
             or next((good(k) for k in legs if good(k)), None))
     if head is None:
         print("no solver leg ran", file=sys.stderr)
@@ -372,6 +557,19 @@ def main() -> None:
     base_x0 = good("cg_x0extrap")
     none = good("cg_learned_none")
     vcyc = good("cg_learned_vcycle")
+    net = good("cg_learned")
+    import tempfile
+
+    tmp = Path(tempfile.mkdtemp(prefix="cfd_bench_"))
+    errors = c1_path = c1_rate = None
+    if not args.no_cpu:
+        err_legs = [k for k in (head["solver"], "cg_learned", "cg") if good(k)]
+        err_legs = list(dict.fromkeys(err_legs))
+        errors = field_errors(err_legs, min(ERR_STEPS, args.warmup + args.steps), tmp)
+        print(f"errors: {json.dumps(errors)}", file=sys.stderr, flush=True)
+        c1_fields, c1_rate = gpu_c1()
+        c1_path = str(tmp / "gpu_c1.npz")
+        np.savez(c1_path, **c1_fields)
     ensemble = None
     if not args.no_ensemble:
         try:
@@ -380,7 +578,8 @@ def main() -> None:
         except (ValueError, RuntimeError) as e:
             ensemble = {"error": str(e)}
         print(f"ensemble: {json.dumps(ensemble)}", file=sys.stderr, flush=True)
-    cpu = None if args.no_cpu else cpu_baseline()
+    cpu = None if args.no_cpu else cpu_baseline(c1_path, c1_rate)
+    print(f"cpu baseline: {json.dumps(cpu)}", file=sys.stderr, flush=True)
     line = {
         "metric": "MAC steps/s @1024² cavity: learned-PCG fp32 vs plain fp64 CG "
                   "(resid 1e-6)",
@@ -394,9 +593,11 @@ def main() -> None:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"cg_learned": "f64+f32+bf16", "cg_mixed": "f64+f32"}.get(head["solver"], "f64"),
+        "dtype": {"cg_learned": "f64+f32+bf16", "cg_learned_vcycle": "f64+f32",
+                  "cg_mixed": "f64+f32"}.get(head["solver"], "f64"),
         "data": "synthetic",
         "config": {"workload": WORKLOAD, "grid": [N, N], "solver": head["solver"],
+                   "solver_detail": LEG_DETAIL.get(head["solver"], head["solver"]),
                    "rtol": RTOL},
         "roofline": roofline_for(head) if "proj_ms_mean" in head else None,
         "cpu_baseline": cpu,
@@ -406,16 +607,20 @@ def main() -> None:
                                if base and "error" not in base else None),
         "speedup_same_x0": (round(head["steps_per_s"] / base_x0["steps_per_s"], 3)
                             if base_x0 and head.get("x0") == base_x0["x0"] else None),
-        # the network against the same solver with the V-cycle carried on to
-        # the dense coarsest solve (the complete method without the network),
-        # and against no coarse solve below 32x32 at all
-        "network_gain": ({"iters_with": round(head["cg_iters_mean"], 2),
-                          "steps_per_s_with": round(head["steps_per_s"], 3),
+        "errors": errors,
+        # the U-Net coarse correction against the same solver with the V-cycle
+        # carried on to the dense coarsest solve (the complete method without
+        # the network) and against no coarse solve below 32x32 at all
+        "network_gain": ({"iters_net": round(net["cg_iters_mean"], 2),
+                          "steps_per_s_net": round(net["steps_per_s"], 3),
+                          "speedup_net_vs_fp64_cg": (round(net["steps_per_s"]
+                                                           / base["steps_per_s"], 3)
+                                                     if base else None),
                           "iters_vcycle": round(vcyc["cg_iters_mean"], 2) if vcyc else None,
                           "steps_per_s_vcycle": round(vcyc["steps_per_s"], 3) if vcyc else None,
                           "iters_none": round(none["cg_iters_mean"], 2) if none else None,
                           "steps_per_s_none": round(none["steps_per_s"], 3) if none else None}
-                         if head["solver"] == "cg_learned" and (none or vcyc) else None),
+                         if net and (none or vcyc) else None),
         "ensemble": ensemble,
         "legs": legs,
     }

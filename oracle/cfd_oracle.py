@@ -80,6 +80,9 @@ def lib():
                                               + [vp] * 8 + [i, d, vp])
         L.oracle_limiter.argtypes = [d, i]
         L.oracle_limiter.restype = d
+        L.oracle_pcg_csr.argtypes = [ctypes.c_long, vp, vp, vp, vp, vp, vp, d, i, vp,
+                                     ctypes.POINTER(i)]
+        L.oracle_pcg_csr.restype = i
         _LIB = L
     return _LIB
 
@@ -346,7 +349,14 @@ class OBoundary:
 # Pressure Poisson solve
 # --------------------------------------------------------------------------- #
 class OPressure:
-    """Assembled 5-point -Laplacian on fluid cells, LU or Jacobi-CG (pressure.py:32-185)."""
+    """Assembled 5-point -Laplacian on fluid cells, LU or Jacobi-CG (pressure.py:32-185).
+
+    Two methods beyond the reference's: ``"cg_omp"`` runs the same Jacobi-CG
+    loop in the C library on OMP_NUM_THREADS threads (the all-cores CPU
+    baseline); ``"dct"`` solves a closed, uniform, obstacle-free box exactly by
+    the DCT-II diagonalisation of the Neumann Laplacian (an FP64 reference
+    for large grids, where splu is slow: it equals the LU solve after the
+    reference's mean removal, tests/test_oracle.py)."""
 
     def __init__(self, grid: OGrid, boundary: OBoundary, method="direct", tol=1e-10,
                  maxiter=5000):
@@ -421,8 +431,24 @@ class OPressure:
         self.A = A
         if method == "direct":
             self._lu = spla.splu(A, permc_spec="COLAMD")
+        elif method == "dct":
+            if (n != nx * ny or boundary.periodic or len(self.neumann) != 1
+                    or any(m.any() for m in self.dirichlet.values())):
+                raise ValueError("the DCT solve needs a closed box without obstacles")
+            lx = (2.0 - 2.0 * np.cos(np.pi * np.arange(nx) / nx)) * ax
+            ly = (2.0 - 2.0 * np.cos(np.pi * np.arange(ny) / ny)) * ay
+            lam = lx[:, None] + ly[None, :]
+            lam[0, 0] = 1.0
+            self._lam = lam
         else:
             self._M = sp.diags(1.0 / d)
+            if method == "cg_omp":
+                csr = A.tocsr()
+                csr.sort_indices()
+                self._csr = (csr.indptr.astype(np.int64), csr.indices.astype(np.int64),
+                             csr.data.astype(np.float64))
+                self._inv_diag = np.ascontiguousarray(1.0 / d)
+                self._work = np.zeros(4 * n)
         self.x0 = np.zeros(n)
         self.last_iterations = 0
 
@@ -437,6 +463,24 @@ class OPressure:
         if self.method == "direct":
             x = self._lu.solve(b)
             self.last_iterations = 1
+        elif self.method == "dct":
+            import scipy.fft
+
+            r = scipy.fft.dctn(b.reshape(self.fluid.shape), type=2, norm="ortho") / self._lam
+            r[0, 0] = 0.0
+            x = scipy.fft.idctn(r, type=2, norm="ortho").ravel()
+            self.last_iterations = 1
+        elif self.method == "cg_omp":
+            x = np.ascontiguousarray(self.x0, dtype=np.float64).copy()
+            bb = np.ascontiguousarray(b)
+            it = ctypes.c_int(0)
+            ptr, ind, val = self._csr
+            info = lib().oracle_pcg_csr(self.n, _p(ptr), _p(ind), _p(val), _p(self._inv_diag),
+                                        _p(bb), _p(x), self.tol, self.maxiter, _p(self._work),
+                                        ctypes.byref(it))
+            self.last_iterations = int(it.value)
+            self.last_info = int(info)
+            self.x0 = x
         else:
             count = [0]
 

@@ -24,6 +24,16 @@
  *   oracle_obstacle_bc          kernels.py:158-209
  *   limiter / face_value        kernels.py:215-251
  *   oracle_scalar_transport     kernels.py:254-391
+ *   oracle_pcg_csr              scipy.sparse.linalg.cg (SciPy 1.15
+ *                               iterative.py:305-423) with the Jacobi
+ *                               preconditioner of pressure.py:138-139, for the
+ *                               all-cores CPU baseline
+ *
+ * Built with OpenMP: the momentum loops are gather-form (every face is written
+ * once from read-only inputs), so their results do not depend on the thread
+ * count; the scatter-form scalar transport stays serial.  The CG's dot
+ * products are OpenMP reductions, summed in a thread-count-dependent order
+ * (its results agree with SciPy's to rounding, not bit for bit).
  */
 #include <math.h>
 #include <stdint.h>
@@ -43,6 +53,7 @@ void oracle_momentum_tendencies(const double *u, const double *v, const uint8_t 
     const int imax = periodic ? nx : nx - 1;
 
     /* u faces: between cells i and i+1 */
+#pragma omp parallel for schedule(static)
     for (int i = 1; i <= imax; ++i) {
         const int ip = i < nx ? i + 1 : 1;
         for (int j = 1; j <= ny; ++j) {
@@ -61,6 +72,7 @@ void oracle_momentum_tendencies(const double *u, const double *v, const uint8_t 
         }
     }
     /* v faces: between cells j and j+1 */
+#pragma omp parallel for schedule(static)
     for (int i = 1; i <= nx; ++i) {
         for (int j = 1; j < ny; ++j) {
             if (!(C_(fluid, i, j) && C_(fluid, i, j + 1))) continue;
@@ -398,4 +410,71 @@ void oracle_vof_advect(const double *a, const double *u, const double *v, const 
                 if (C_(out, i, j) < 0.0) C_(out, i, j) = 0.0;
                 else if (C_(out, i, j) > 1.0) C_(out, i, j) = 1.0;
             }
+}
+
+/* Jacobi-preconditioned CG on a CSR matrix: the loop of
+   scipy.sparse.linalg.cg (SciPy 1.15 iterative.py:305-423) -- stop when
+   ||r|| < rtol ||b|| at the top of an iteration, z = r / diag, p = z (first
+   iteration) or z + beta p, alpha = rho / (p.q) -- with OpenMP loops and
+   reductions.  x holds x0 on entry and the result on return; *iters the
+   iterations done (the callback count); returns 0 converged, maxiter not. */
+static double dot_omp(long n, const double *a, const double *b)
+{
+    double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (long k = 0; k < n; ++k) s += a[k] * b[k];
+    return s;
+}
+
+static void spmv_omp(long n, const int64_t *ptr, const int64_t *ind, const double *val,
+                     const double *x, double *y)
+{
+#pragma omp parallel for schedule(static)
+    for (long r = 0; r < n; ++r) {
+        double s = 0.0;
+        for (int64_t q = ptr[r]; q < ptr[r + 1]; ++q) s += val[q] * x[ind[q]];
+        y[r] = s;
+    }
+}
+
+int oracle_pcg_csr(long n, const int64_t *ptr, const int64_t *ind, const double *val,
+                   const double *inv_diag, const double *b, double *x, double rtol, int maxiter,
+                   double *work, int *iters)
+{
+    double *r = work, *z = work + n, *p = work + 2 * n, *q = work + 3 * n;
+    spmv_omp(n, ptr, ind, val, x, q);
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < n; ++k) r[k] = b[k] - q[k];
+    const double bnrm = sqrt(dot_omp(n, b, b));
+    *iters = 0;
+    if (bnrm == 0.0) {
+        for (long k = 0; k < n; ++k) x[k] = b[k];
+        return 0;
+    }
+    const double atol = rtol * bnrm;
+    double rho_prev = 0.0;
+    for (int it = 0; it < maxiter; ++it) {
+        if (sqrt(dot_omp(n, r, r)) < atol) return 0;
+#pragma omp parallel for schedule(static)
+        for (long k = 0; k < n; ++k) z[k] = inv_diag[k] * r[k];
+        const double rho = dot_omp(n, r, z);
+        if (it > 0) {
+            const double beta = rho / rho_prev;
+#pragma omp parallel for schedule(static)
+            for (long k = 0; k < n; ++k) p[k] = beta * p[k] + z[k];
+        } else {
+#pragma omp parallel for schedule(static)
+            for (long k = 0; k < n; ++k) p[k] = z[k];
+        }
+        spmv_omp(n, ptr, ind, val, p, q);
+        const double alpha = rho / dot_omp(n, p, q);
+#pragma omp parallel for schedule(static)
+        for (long k = 0; k < n; ++k) {
+            x[k] += alpha * p[k];
+            r[k] -= alpha * q[k];
+        }
+        rho_prev = rho;
+        *iters = it + 1;
+    }
+    return maxiter;
 }

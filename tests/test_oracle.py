@@ -107,6 +107,36 @@ def test_pressure_solve_equals_dct_exact(method):
     assert abs(p.mean()) < 1e-13
 
 
+def test_oracle_dct_method_equals_lu_trajectory():
+    """The oracle's "dct" pressure method (the FP64 reference of the 1024^2
+    error figures) follows the reference's LU trajectory: a 32^2 Re 1000
+    cavity with a van Leer scalar over 20 steps, fields to rounding."""
+    a = O.cavity(re=1000, n=32, pressure_solver="direct")
+    b = O.cavity(re=1000, n=32, pressure_solver="dct")
+    for s in (a, b):
+        O.add_scalar(s, "c", 1e-3, limiter="vanleer", initial=O.gaussian_blobs(s.grid))
+        for _ in range(20):
+            s.step()
+    for f in ("u", "v", "p"):
+        assert np.abs(getattr(a, f) - getattr(b, f)).max() < 1e-11, f
+    assert np.abs(a.scalars["c"].phi - b.scalars["c"].phi).max() < 1e-12
+    with pytest.raises(ValueError):
+        O.obstacle_channel([(1.0, 0.5, 0.1)], nx=32, ny=8, pressure_solver="dct")
+
+
+def test_oracle_openmp_cg_follows_scipy_cg():
+    """The all-cores CPU baseline's C Jacobi-CG runs SciPy's loop: the same
+    iteration counts (to one, the dot products are summed in another order)
+    and the same fields to the solve tolerance as scipy.sparse.linalg.cg."""
+    a = O.cavity(re=1000, n=48, pressure_solver="cg", pressure_tol=1e-8)
+    b = O.cavity(re=1000, n=48, pressure_solver="cg_omp", pressure_tol=1e-8)
+    for s in (a, b):
+        for _ in range(10):
+            s.step()
+    assert all(abs(x - y) <= 1 for x, y in zip(a.cg_iters, b.cg_iters))
+    assert np.abs(a.u - b.u).max() < 1e-9 and np.abs(a.p - b.p).max() < 1e-7
+
+
 def test_planted_laplacian_mistake_is_caught():
     rng = np.random.default_rng(3)
     rhs = rng.standard_normal((16, 16))
