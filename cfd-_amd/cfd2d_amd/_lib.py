@@ -230,6 +230,7 @@ def lib() -> ctypes.CDLL:
         "cfd_favg_set": ([_vp, _i32, _d], _i32),
         "cfd_favg_read": ([_vp, P(_d), _vp, _vp], _i32),
         "cfd_download_derived": ([_vp, _i32, _vp], _i32),
+        "cfd_download_derived_noalloc": ([_vp, _i32, _vp], _i32),
         "cfd_set_turbulence": ([_vp, P(TurbDesc)], _i32),
         "cfd_set_vof": ([_vp, P(VofDesc)], _i32),
         "cfd_momentum_tendencies_var": ([_vp, _vp, _vp, _i32, _i32, _d, _d, _vp, _d, _i32, _vp,
@@ -278,6 +279,7 @@ def exported_symbols() -> list[str]:
                         "cfd_dev_free", "cfd_memcpy", "cfd_momentum_tendencies",
                         "cfd_obstacle_bc", "cfd_scalar_transport", "cfd_probes_set",
                         "cfd_probes_read", "cfd_favg_set", "cfd_favg_read", "cfd_download_derived",
+                        "cfd_download_derived_noalloc",
                         "cfd_set_turbulence", "cfd_momentum_tendencies_var", "cfd_set_vof",
                         "cfd_set_learned",
                         "cfd_learned_stats", "cfd_learned_reset", "cfd_ens_create",

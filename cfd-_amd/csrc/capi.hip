@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -22,6 +23,13 @@
 #include "flow.hip"
 #include "vof.hip"
 
+// Staging of cfd_download_derived_noalloc: a fixed device array per GPU
+// (each device holds its own copy until the process ends).  Host code never
+// passes `scratch` itself to a kernel or a copy (in host code the name is a
+// stand-in, not a device address): scratch_on() fetches this device's address
+// once with hipGetSymbolAddress, and every use goes through that pointer.
+static __device__ __align__(256) double scratch[CFD_DERIVED_NOALLOC_MAX];
+
 namespace {
 
 // last error message of the calling thread (a plain array: nothing to destroy at exit)
@@ -34,6 +42,10 @@ int set_err(int code, const char* fmt, ...) {
     va_end(ap);
     return code;
 }
+
+constexpr int SCRATCH_MAX_DEV = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[SCRATCH_MAX_DEV] = {};
 
 #define HIPC(expr)                                                                     \
     do {                                                                               \
@@ -55,6 +67,7 @@ struct ScalarDev {
 }  // namespace
 
 struct cfd_ctx {
+    int device = 0;              // HIP device the context was created on
     int nx, ny, P;
     double dx, dy;
     int periodic;
@@ -216,6 +229,29 @@ int shape_of(const cfd_ctx* c, int field, int* rows, int* cols, int* off_i, int*
     default:
         return set_err(CFD_ERR_ARG, "unknown field id %d", field);
     }
+}
+
+// the address of `scratch` on device `dev` (fetched once per device, with that
+// device current; the caller's current device is restored)
+int scratch_on(int dev, double** out) {
+    if (dev < 0 || dev >= SCRATCH_MAX_DEV)
+        return set_err(CFD_ERR_LIMIT, "device %d: the allocation-free staging serves devices "
+                       "0..%d", dev, SCRATCH_MAX_DEV - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int cur = 0;
+        HIPC(hipGetDevice(&cur));
+        if (cur != dev) HIPC(hipSetDevice(dev));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) HIPC(hipSetDevice(cur));
+        if (e != hipSuccess || !p)
+            return set_err(CFD_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                           hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
 }
 
 int read_state(cfd_ctx* c, DevState* h) {
@@ -993,6 +1029,12 @@ int cfd_ctx_create(const cfd_grid_desc* d, cfd_ctx** out) {
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
         return fail(set_err(CFD_ERR_HIP, "cannot query the HIP device"));
     c->num_cu = prop.multiProcessorCount;
+    c->device = dev;
+    {   // set-up of the allocation-free entry point's staging on this device
+        double* s = nullptr;
+        const int rc = scratch_on(dev, &s);
+        if (rc) return fail(rc);
+    }
     if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess)
         return fail(set_err(CFD_ERR_HIP, "stream creation failed"));
     c->stream = c->own_stream;
@@ -2003,25 +2045,60 @@ int cfd_favg_read(cfd_ctx* c, double* T, double* sum, double* sum2) {
     return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+// the node field `kind` of the current velocity into dst, (nx+1) x (ny+1),
+// on the context's stream
+int launch_derived(cfd_ctx* c, int32_t kind, double* dst) {
+    const size_t n = (size_t)(c->nx + 1) * (c->ny + 1);
+    if (kind == CFD_DERIVED_VORTICITY) {
+        hipLaunchKernelGGL(k_vort_nodes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                           (const double*)c->u, (const double*)c->v, c->geo, c->P, dst);
+    } else {
+        hipLaunchKernelGGL(k_psi_bottom, dim3(1), dim3(1024), 0, c->stream, (const double*)c->v,
+                           (const uint8_t*)c->fl, c->geo, c->P, dst);
+        HIPC(hipGetLastError());
+        hipLaunchKernelGGL(k_psi_cols, dim3((unsigned)((c->nx + 1 + 255) / 256)), dim3(256), 0,
+                           c->stream, (const double*)c->u, (const uint8_t*)c->fl, c->geo, c->P,
+                           dst);
+    }
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
 int cfd_download_derived(cfd_ctx* c, int32_t kind, double* host) {
     if (!c || !host) return set_err(CFD_ERR_ARG, "null argument");
     if (kind != CFD_DERIVED_VORTICITY && kind != CFD_DERIVED_STREAMFUNCTION)
         return set_err(CFD_ERR_ARG, "derived field %d", kind);
     const size_t n = (size_t)(c->nx + 1) * (c->ny + 1);
     if (!c->derived) TRY(dalloc(c, &c->derived, n));
-    if (kind == CFD_DERIVED_VORTICITY) {
-        hipLaunchKernelGGL(k_vort_nodes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                           (const double*)c->u, (const double*)c->v, c->geo, c->P, c->derived);
-    } else {
-        hipLaunchKernelGGL(k_psi_bottom, dim3(1), dim3(1024), 0, c->stream, (const double*)c->v,
-                           (const uint8_t*)c->fl, c->geo, c->P, c->derived);
-        HIPC(hipGetLastError());
-        hipLaunchKernelGGL(k_psi_cols, dim3((unsigned)((c->nx + 1 + 255) / 256)), dim3(256), 0,
-                           c->stream, (const double*)c->u, (const uint8_t*)c->fl, c->geo, c->P,
-                           c->derived);
-    }
-    HIPC(hipGetLastError());
+    TRY(launch_derived(c, kind, c->derived));
     HIPC(hipMemcpyAsync(host, c->derived, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int cfd_download_derived_noalloc(cfd_ctx* c, int32_t kind, double* host) {
+    if (!c || !host) return set_err(CFD_ERR_ARG, "null argument");
+    if (kind != CFD_DERIVED_VORTICITY && kind != CFD_DERIVED_STREAMFUNCTION)
+        return set_err(CFD_ERR_ARG, "derived field %d", kind);
+    const long n = (long)(c->nx + 1) * (c->ny + 1);
+    if (n > CFD_DERIVED_NOALLOC_MAX)
+        return set_err(CFD_ERR_LIMIT, "%d x %d nodes exceed the allocation-free staging of %ld "
+                       "nodes (use cfd_download_derived)", c->nx + 1, c->ny + 1,
+                       (long)CFD_DERIVED_NOALLOC_MAX);
+    // this device's staging (set up when the context was created); both
+    // kernels write every one of the n nodes, so nothing needs clearing
+    double* s = nullptr;
+    TRY(scratch_on(c->device, &s));
+    TRY(launch_derived(c, kind, s));
+    HIPC(hipMemcpyAsync(host, s, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPC(hipStreamSynchronize(c->stream));
     return 0;
 }

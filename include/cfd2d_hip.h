@@ -355,6 +355,16 @@ int cfd_favg_read(cfd_ctx* ctx, double* T, double* sum, double* sum2);
    cumsum does (solver.py:580-596) */
 enum { CFD_DERIVED_VORTICITY = 0, CFD_DERIVED_STREAMFUNCTION = 1 };
 int cfd_download_derived(cfd_ctx* ctx, int32_t kind, double* host);
+/* Allocation-free variant of cfd_download_derived.  It has no counterpart in
+   the reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid it).
+   The node field is staged in a static device array of
+   CFD_DERIVED_NOALLOC_MAX doubles on each GPU (about 34 MB, held until the
+   process ends) instead of a buffer allocated on the first call; grids with
+   more nodes get CFD_ERR_LIMIT.  All calls on one GPU share that array:
+   calls that overlap on one GPU race. */
+#define CFD_DERIVED_NOALLOC_MAX (2049L * 2049L)
+int cfd_download_derived_noalloc(cfd_ctx* ctx, int32_t kind, double* host);
 
 /* ---- turbulence models (turbulence.py:130-325); no counterpart in a C
    interface of the reference, which runs them in NumPy ----
