@@ -212,6 +212,7 @@ def lib() -> ctypes.CDLL:
         "cfd_project_initial": ([_vp, P(StepParams)], _i32),
         "cfd_step": ([_vp, P(StepParams), _i32], _i32),
         "cfd_sync": ([_vp], _i32),
+        "cfd_set_spin_limit": ([_vp, _d], _i32),
         "cfd_read_records": ([_vp, _i64, _i32, P(Record)], _i32),
         "cfd_last_elapsed_ms": ([_vp, P(_d)], _i32),
         "cfd_set_profiling": ([_vp, _i32], _i32),
@@ -273,7 +274,8 @@ def exported_symbols() -> list[str]:
                         "cfd_ctx_destroy", "cfd_ctx_set_stream", "cfd_add_scalar", "cfd_upload",
                         "cfd_download", "cfd_upload_scalar", "cfd_download_scalar",
                         "cfd_set_state", "cfd_get_state", "cfd_set_multigrid", "cfd_apply_bc",
-                        "cfd_project_initial", "cfd_step", "cfd_sync", "cfd_read_records",
+                        "cfd_project_initial", "cfd_step", "cfd_sync", "cfd_set_spin_limit",
+                        "cfd_read_records",
                         "cfd_last_elapsed_ms", "cfd_set_profiling", "cfd_projection_times",
                         "cfd_pressure_solve", "cfd_dev_alloc",
                         "cfd_dev_free", "cfd_memcpy", "cfd_momentum_tendencies",

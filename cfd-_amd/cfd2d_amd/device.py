@@ -141,6 +141,11 @@ class DeviceContext:
     def sync(self) -> None:
         _lib.check(_lib.lib().cfd_sync(self.handle), "sync")
 
+    def set_spin_limit(self, seconds: float) -> None:
+        """Bound of the device's waits on other work-groups (default 2 s);
+        clears an earlier give-up.  0 makes every wait give up at once."""
+        _lib.check(_lib.lib().cfd_set_spin_limit(self.handle, float(seconds)), "spin limit")
+
     def elapsed_ms(self) -> float:
         ms = ctypes.c_double(0.0)
         _lib.check(_lib.lib().cfd_last_elapsed_ms(self.handle, ctypes.byref(ms)), "elapsed")

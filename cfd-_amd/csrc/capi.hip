@@ -68,6 +68,8 @@ struct ScalarDev {
 
 struct cfd_ctx {
     int device = 0;              // HIP device the context was created on
+    unsigned* spin_err = nullptr;        // set by a device wait that exceeded spin_limit
+    long long spin_limit = 200000000;    // 2 s in ticks of the 100 MHz real-time counter
     int nx, ny, P;
     double dx, dy;
     int periodic;
@@ -254,9 +256,27 @@ int scratch_on(int dev, double** out) {
     return 0;
 }
 
+SpinGuard spin_guard(const cfd_ctx* c) { return SpinGuard{c->spin_err, c->spin_limit}; }
+
+// a device wait that gave up leaves the state invalid; e is the error word
+// (read here after synchronising the stream, unless the caller passes it)
+int check_spin(cfd_ctx* c, const unsigned* word = nullptr) {
+    unsigned e = 0;
+    if (word) e = *word;
+    else HIPC(hipMemcpy(&e, c->spin_err, sizeof(e), hipMemcpyDeviceToHost));
+    if (e)
+        return set_err(CFD_ERR_STATE, "a grid synchronisation wait on the device exceeded its "
+                       "limit of %.3g s and gave up: the device state is invalid "
+                       "(cfd_set_spin_limit clears the error)", c->spin_limit * 1e-8);
+    return 0;
+}
+
 int read_state(cfd_ctx* c, DevState* h) {
+    unsigned e = 0;
     HIPC(hipMemcpyAsync(h, c->st, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipMemcpyAsync(&e, c->spin_err, sizeof(e), hipMemcpyDeviceToHost, c->stream));
     HIPC(hipStreamSynchronize(c->stream));
+    TRY(check_spin(c, &e));
     return 0;
 }
 
@@ -380,6 +400,7 @@ ProjArgs proj_args(cfd_ctx* c, int method, double rtol, int maxiter) {
     a.d32_0 = c->e0;
     a.d32_1 = c->e1;
     a.sync = c->sync;
+    a.spin = spin_guard(c);
     a.rold = c->rold;
     a.mg = c->mg_dev;
     a.st = c->st;
@@ -1056,7 +1077,7 @@ int cfd_ctx_create(const cfd_grid_desc* d, cfd_ctx** out) {
         (rc = dalloc(c, &c->fl, N)) || (rc = dalloc(c, &c->code, N)) ||
         (rc = dalloc(c, &c->comp, N)) || (rc = dalloc(c, &c->lut, 128)) ||
         (rc = dalloc(c, &c->comp_count, CFD_MAX_COMP)) || (rc = dalloc(c, &c->st, 1)) ||
-        (rc = dalloc(c, &c->ring, RING)))
+        (rc = dalloc(c, &c->ring, RING)) || (rc = dalloc(c, &c->spin_err, 1)))
         return fail(rc);
     // persistent projection launch shape: one block per CU (or per row on small grids)
     c->proj_bs = std::min(PROJ_BS, ((c->ny + 63) / 64) * 64);
@@ -1669,7 +1690,8 @@ int cfd_step(cfd_ctx* c, const cfd_step_params* prm, int32_t count) {
                                 hipMemcpyDeviceToDevice, c->stream));
         }
         hipLaunchKernelGGL(k_dt_gamma, dim3(c->red_blocks), dim3(256), 0,
-                           c->stream, c->u, c->v, c->geo, sp, td, c->st, c->part, c->flags);
+                           c->stream, c->u, c->v, c->geo, sp, td, c->st, c->part, c->flags,
+                           spin_guard(c));
         HIPC(hipGetLastError());
         PredArgs pa = pred_args(c, prm, pk);
         hipLaunchKernelGGL(k_predictor, grid2d(c), dim3(BS2D), 0, c->stream, pa, c->geo);
@@ -1710,6 +1732,7 @@ int cfd_step(cfd_ctx* c, const cfd_step_params* prm, int32_t count) {
         ra.st = c->st;
         ra.part = c->part;
         ra.flags = c->flags;
+        ra.spin = spin_guard(c);
         ra.ring = c->ring;
         ra.ring_len = RING;
         ra.scheme_ab2 = prm->scheme_ab2;
@@ -1736,6 +1759,20 @@ int cfd_step(cfd_ctx* c, const cfd_step_params* prm, int32_t count) {
 
 int cfd_sync(cfd_ctx* c) {
     if (!c) return set_err(CFD_ERR_ARG, "null context");
+    HIPC(hipStreamSynchronize(c->stream));
+    return check_spin(c);
+}
+
+int cfd_set_spin_limit(cfd_ctx* c, double seconds) {
+    if (!c) return set_err(CFD_ERR_ARG, "null context");
+    if (!(seconds == seconds) || seconds > 3600.0)
+        return set_err(CFD_ERR_ARG, "spin limit %g s (at most 3600)", seconds);
+    HIPC(hipStreamSynchronize(c->stream));
+    c->spin_limit = seconds > 0.0 ? (long long)(seconds * 1e8) : 0;
+    HIPC(hipMemsetAsync(c->spin_err, 0, sizeof(unsigned), c->stream));
+    // flags that blocks raised after an abandoned gather would release the
+    // next one early
+    HIPC(hipMemsetAsync(c->flags, 0, (size_t)c->red_blocks * sizeof(unsigned), c->stream));
     HIPC(hipStreamSynchronize(c->stream));
     return 0;
 }

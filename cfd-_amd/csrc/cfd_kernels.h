@@ -92,6 +92,7 @@ struct RecordArgs {
     DevState* st;
     double* part;
     unsigned* flags;         // per-block publish flags (gather_block0)
+    SpinGuard spin;          // bound of block 0's wait
     StepRecord* ring;
     int ring_len;
     int scheme_ab2;
@@ -149,6 +150,7 @@ struct ProjArgs {
     double* rold;            // flexible CG: previous residual
     const MGDev* mg;         // multigrid hierarchy (device memory) for PROJ_MG
     double* sync;            // grid-sync lines: gridDim.x + 2 lines of CFD_SLOT doubles
+    SpinGuard spin;          // bound of the grid syncs' waits
     DevState* st;
     double dt_override;      // > 0: use this dt (initial projection with dt = 1)
     const double* rhs_in;    // standalone solve: right-hand side (cell layout) instead of div(F,G)/dt
@@ -198,7 +200,7 @@ __global__ void k_copy_predictor(PredArgs a, Geo g);
 __global__ void k_bc_domain(BCArgs a, Geo g);
 __global__ void k_obstacle_bc(double* u, double* v, const uint8_t* fl, Geo g, const DevState* st);
 __global__ void k_dt_gamma(const double* u, const double* v, Geo g, StepParams prm, TurbDt td,
-                           DevState* st, double* part, unsigned* flags);
+                           DevState* st, double* part, unsigned* flags, SpinGuard spin);
 template <bool CONST_D, int MODE>
 __global__ void k_scalar_transport(ScalarArgs a, Geo g);
 __global__ void k_fill_ghosts(ScalarArgs a, Geo g);

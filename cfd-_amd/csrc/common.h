@@ -40,6 +40,35 @@ __device__ __forceinline__ void flat_reduce(int n, int w, int i0, int j0, L load
     }
 }
 
+// Bound of the spin-waits on other work-groups (block 0 of a one-pass
+// reduction waiting for the others' flags, every block of a persistent kernel
+// waiting for block 0's release; see grid_reduce for block 0's own wait).  A
+// wait longer than `limit` ticks of the 100 MHz real-time counter gives up,
+// stores 1 in *err -- the host reads it whenever it reads the device state
+// and reports the error -- and goes on, so the kernel still ends (with
+// meaningless values).  limit <= 0 gives up at once without waiting (the
+// host's test of this path).  The start time and the limit are uniform
+// (scalar registers).
+struct SpinGuard {
+    unsigned* err;
+    long long limit;
+};
+
+__device__ __forceinline__ long long spin_now() {
+    return (long long)__builtin_amdgcn_s_memrealtime();   // a scalar (uniform) read
+}
+
+__device__ __forceinline__ void spin_fail(const SpinGuard& g) {
+    __hip_atomic_store(g.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// a wait that is forced to give up at once (limit <= 0)
+__device__ __forceinline__ bool spin_forced(const SpinGuard& g) {
+    if (g.limit > 0) return false;
+    spin_fail(g);
+    return true;
+}
+
 // Hands the per-block partials of a one-pass grid reduction to block 0 with no
 // read-modify-write on a shared address (those serialise across the chip):
 // every other block publishes with one release store of its own flag, block 0
@@ -47,15 +76,23 @@ __device__ __forceinline__ void flat_reduce(int n, int w, int i0, int j0, L load
 // once all its threads see the partials.  Thread 0 of each block writes its
 // partials before the call.  Only block 0 waits, holding one work-group slot,
 // so the other blocks can always be scheduled.
-__device__ __forceinline__ bool gather_block0(unsigned* flags) {
+__device__ __forceinline__ bool gather_block0(unsigned* flags, const SpinGuard& spin) {
     if (blockIdx.x != 0) {
         if (threadIdx.x == 0)
             __hip_atomic_store(&flags[blockIdx.x], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         return false;
     }
+    const bool forced = spin_forced(spin);
+    const long long t0 = spin_now();
     for (int b = 1 + threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-        while (__hip_atomic_load(&flags[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+        while (!forced &&
+               __hip_atomic_load(&flags[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+            if (spin_now() - t0 >= spin.limit) {
+                spin_fail(spin);
+                break;
+            }
             __builtin_amdgcn_s_sleep(1);
+        }
         __hip_atomic_store(&flags[b], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -147,18 +184,28 @@ __device__ __forceinline__ void block_max(double (&val)[NV], double* lds) {
 struct GridSync {
     double* slots;    // gridDim.x * CFD_SLOT doubles
     double* rel;      // 2 * CFD_SLOT doubles
+    SpinGuard spin;   // bound of the waits
 };
 
 __device__ __forceinline__ unsigned* slot_tag(double* line) {
     return reinterpret_cast<unsigned*>(line + CFD_SLOT - 1);
 }
 
+// Bounds (SpinGuard): every block's wait for block 0's release gives up after
+// the limit (storing the error word) and passes; tags compare with >= so a
+// stale or advanced tag never blocks.  Block 0's wait for the other blocks'
+// tags has no clock: any bound inside that loop makes the persistent kernels
+// spill registers (measured: the FP64 Jacobi-PCG 15-28 % slower).  It waits
+// only for blocks that are resident -- the cooperative launch guarantees it --
+// and whose own waits are bounded, so it ends once they arrive.  A forced
+// limit (<= 0) skips every wait.
 template <int NV, bool IS_MAX>
 __device__ __forceinline__ void grid_reduce(double (&val)[NV], const GridSync& gs, double* lds,
                                             unsigned& epoch) {
     static_assert(NV >= 1 && NV <= CFD_MAXV, "1..14 reduction values");
     if (IS_MAX) block_max<NV>(val, lds);
     else block_sum<NV>(val, lds);
+    const bool forced = spin_forced(gs.spin);
     epoch += 1;
     const unsigned e = epoch;
     if (threadIdx.x == 0) {
@@ -176,7 +223,8 @@ __device__ __forceinline__ void grid_reduce(double (&val)[NV], const GridSync& g
             double* line = gs.slots + (long)b * CFD_SLOT;
             // poll without acquire (an acquire load invalidates the caches on
             // every try); one acquire fence once the tag has been seen
-            while (__hip_atomic_load(slot_tag(line), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != e) {
+            while (!forced && __hip_atomic_load(slot_tag(line), __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT) < e) {
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
@@ -194,8 +242,15 @@ __device__ __forceinline__ void grid_reduce(double (&val)[NV], const GridSync& g
         }
     }
     if (threadIdx.x == 0) {
-        while (__hip_atomic_load(slot_tag(rel), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != e)
+        const long long t0 = spin_now();
+        while (!forced && __hip_atomic_load(slot_tag(rel), __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT) < e) {
+            if (spin_now() - t0 >= gs.spin.limit) {
+                spin_fail(gs.spin);
+                break;
+            }
             __builtin_amdgcn_s_sleep(1);
+        }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
         for (int q = 0; q < NV; ++q)

@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(PROJ_BS) k_project(ProjArgs a, Geo g) {
     const Rows rows = block_rows(nx);
     const int tid = threadIdx.x, BS = blockDim.x;
     const bool lead = blockIdx.x == 0 && tid == 0;
-    const GridSync gs{a.sync, a.sync + (long)gridDim.x * CFD_SLOT};
+    const GridSync gs{a.sync, a.sync + (long)gridDim.x * CFD_SLOT, a.spin};
 
 #define FOR_CELLS                                            \
     for (int i = rows.i0; i < rows.i1; ++i)                  \

@@ -501,7 +501,7 @@ __device__ __forceinline__ unsigned long long dbits(double x) {
 // part[6 b ..]; block 0 takes the maxima and sets the step.
 __global__ void k_dt_gamma(const double* __restrict__ u, const double* __restrict__ v, Geo g,
                            StepParams prm, TurbDt td, DevState* st, double* part,
-                           unsigned* flags) {
+                           unsigned* flags, SpinGuard spin) {
     __shared__ double lds[5 * 16];
     const int nx = g.nx, ny = g.ny;
     double m[2] = {0.0, 0.0};
@@ -568,7 +568,7 @@ __global__ void k_dt_gamma(const double* __restrict__ u, const double* __restric
 #pragma unroll
         for (int k = 0; k < 4; ++k) my[2 + k] = q[k];
     }
-    if (!gather_block0(flags)) return;
+    if (!gather_block0(flags, spin)) return;
     // block 0: the maxima over the blocks
     double t[6] = {0.0, 0.0, 0.0, -__builtin_inf(), 0.0, 0.0};
     for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
@@ -983,7 +983,7 @@ __global__ void k_record(RecordArgs a, Geo g) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) a.part[NV * blockIdx.x + q] = val[q];
     }
-    if (!gather_block0(a.flags)) return;
+    if (!gather_block0(a.flags, a.spin)) return;
     // block 0: add the block partials in block order (fixed tree per thread)
     double tot[NV];
 #pragma unroll

@@ -411,7 +411,7 @@ __global__ void __launch_bounds__(PROJ_BS) k_project_var(VarProjArgs va, Geo g) 
     const int i1 = 1 + (int)(((long)nx * (blockIdx.x + 1)) / gridDim.x);
     const int tid = threadIdx.x, BS = blockDim.x;
     const bool lead = blockIdx.x == 0 && tid == 0;
-    const GridSync gs{a.sync, a.sync + (long)gridDim.x * CFD_SLOT};
+    const GridSync gs{a.sync, a.sync + (long)gridDim.x * CFD_SLOT, a.spin};
     auto fluid = [&](int i, int j) -> bool { return a.code[ix(i, j, pc)] & 63; };
 #define FOR_CELLS                          \
     for (int i = i0; i < i1; ++i)          \

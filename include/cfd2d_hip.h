@@ -205,8 +205,17 @@ int cfd_apply_bc(cfd_ctx* ctx);
 int cfd_project_initial(cfd_ctx* ctx, const cfd_step_params* prm);
 /* Simulation.step (solver.py:326-403): enqueue `count` steps, no host sync. */
 int cfd_step(cfd_ctx* ctx, const cfd_step_params* prm, int32_t count);
-/* Wait for enqueued work. */
+/* Wait for enqueued work.  Returns CFD_ERR_STATE when a device wait gave up
+   (cfd_set_spin_limit), as does every call that reads the device state. */
 int cfd_sync(cfd_ctx* ctx);
+/* Bound of the device's waits on other work-groups (no counterpart in the
+   reference, which has no device): the persistent pressure kernels' grid
+   barriers and the one-pass reductions' gathers.  A wait longer than
+   `seconds` (default 2) gives up, every kernel still ends, and the context
+   reports CFD_ERR_STATE from then on (its fields are invalid); calling this
+   again clears that error.  seconds <= 0 makes every wait give up at once
+   (a test of the error path that hangs nothing). */
+int cfd_set_spin_limit(cfd_ctx* ctx, double seconds);
 /* Copy the records of steps first_n .. first_n+count-1 (1-based step numbers,
    at most the ring length back from the current step). */
 int cfd_read_records(cfd_ctx* ctx, int64_t first_n, int32_t count, cfd_record* out);

@@ -229,3 +229,28 @@ def test_mixed_pressure_solve_equals_dct(gpu):
     exact = scipy.fft.idctn(r, type=2, norm="ortho")
     assert sim.pressure.last_relres <= 1e-10
     assert np.abs(p - exact).max() < 1e-7 * np.abs(exact).max()
+
+
+@pytest.mark.parametrize("solver", ["cg", "cg_learned"])
+def test_device_wait_give_up_is_reported(gpu, solver):
+    """The device's waits on other work-groups are bounded.  A zero limit,
+    set from the host, makes every wait give up at once (nothing waits, so
+    nothing can hang): the kernels still end, and the next read of the device
+    state raises instead of returning meaningless fields.  Setting a limit
+    again clears the error; a fresh context then steps normally."""
+    from cfd2d_amd import cases
+
+    print(f"give-up case: {solver}", flush=True)
+    sim = cases.lid_driven_cavity(re=100, n=32, pressure_solver=solver, pressure_tol=1e-6).sim
+    sim.step()
+    sim._ctx.set_spin_limit(0.0)
+    with pytest.raises(RuntimeError, match="gave up"):
+        sim.step()
+    sim._ctx.set_spin_limit(2.0)
+    sim._ctx.sync()
+    sim.close()
+    fresh = cases.lid_driven_cavity(re=100, n=32, pressure_solver=solver, pressure_tol=1e-6).sim
+    for _ in range(5):
+        fresh.step()
+    assert fresh.history.divergence[-1] < 1e-6
+    fresh.close()
