@@ -242,6 +242,7 @@ def lib() -> ctypes.CDLL:
         "cfd_ens_create": ([P(EnsDesc), P(_vp)], _i32),
         "cfd_ens_destroy": ([_vp], _i32),
         "cfd_ens_set_net": ([_vp, _vp, ctypes.c_float], _i32),
+        "cfd_ens_set_coarse": ([_vp, _i32], _i32),
         "cfd_ens_step": ([_vp, _i32], _i32),
         "cfd_ens_download": ([_vp, _i32, _i32, _i32, _vp], _i32),
         "cfd_ens_stats": ([_vp, _i32, _i32, P(EnsStats)], _i32),
@@ -285,7 +286,8 @@ def exported_symbols() -> list[str]:
                         "cfd_set_turbulence", "cfd_momentum_tendencies_var", "cfd_set_vof",
                         "cfd_set_learned",
                         "cfd_learned_stats", "cfd_learned_reset", "cfd_ens_create",
-                        "cfd_ens_destroy", "cfd_ens_set_net", "cfd_ens_step", "cfd_ens_download",
+                        "cfd_ens_destroy", "cfd_ens_set_net", "cfd_ens_set_coarse",
+                        "cfd_ens_step", "cfd_ens_download",
                         "cfd_ens_stats", "cfd_unet_param_count",
                         "cfd_unet_create", "cfd_unet_destroy", "cfd_unet_set_params",
                         "cfd_unet_get_params", "cfd_unet_get_grad", "cfd_unet_zero_grad",

@@ -10,9 +10,11 @@ FP32; the pressure solve is the learned flexible CG (FP64 vectors and dot
 products, true residual <= rtol), whose coarse corrections for all instances
 come from one batched pass of the shared U-Net per iteration; each solve
 starts from the linear extrapolation in time of the instance's last two
-solutions (``x0="extrapolate"``; "previous": from the last one).  The
-predictor's work-group shape is chosen per class by timing at setup
-(``autotune``).
+solutions (``x0="extrapolate"``; "previous": from the last one).
+``coarse="vcycle"`` replaces the network by the V-cycle carried on to the
+dense coarsest solve (levels of at most 64 x 64 cells in LDS, one work-group
+per instance).  The predictor's work-group shape is chosen per class by
+timing at setup (``autotune``).
 """
 
 from __future__ import annotations
@@ -94,7 +96,7 @@ class Ensemble:
 
     def __init__(self, instances: list[Instance], net: bool | str = True, params=None,
                  sigma=None, rtol: float = 1e-6, maxiter: int = 200, autotune: bool = True,
-                 use_graph: bool = True, x0: str = "extrapolate"):
+                 use_graph: bool = True, x0: str = "extrapolate", coarse: str = "network"):
         self.instances = list(instances)
         sides = sorted({i.n for i in self.instances})
         if any(n not in SIDES for n in sides):
@@ -142,7 +144,12 @@ class Ensemble:
         self._h = hdl
         _lib.track(self)
         self.net = None
-        if net:
+        if coarse not in ("network", "vcycle"):
+            raise ValueError(f"coarse must be 'network' or 'vcycle', got {coarse!r}")
+        self.coarse = coarse
+        if coarse == "vcycle":
+            _lib.check(_lib.lib().cfd_ens_set_coarse(self._h, 1), "ensemble coarse end")
+        elif net:
             if params is None:
                 params, sigma = load_weights(net if isinstance(net, str) else None)
             self.net = UNet(32 * len(self.instances), 32, params)

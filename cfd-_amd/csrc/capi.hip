@@ -5,6 +5,7 @@
 #include "../../include/cfd2d_hip.h"
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1337,7 +1338,8 @@ int cfd_set_learned(cfd_ctx* c, const cfd_learned_desc* d) {
             c->l_vlds = 1;
             VcArgs& v = c->l_vargs;
             std::memset(&v, 0, sizeof(v));
-            v.fs = c->fcg;
+            v.done = reinterpret_cast<const char*>(c->fcg) + offsetof(FcgState, done);
+            v.done_stride = 0;
             v.cinv = mg.cinv;
             v.omega = mg.omega;
             v.wc = mg.wc;

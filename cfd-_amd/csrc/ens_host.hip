@@ -20,6 +20,9 @@ struct cfd_ens {
     int prev_iters = 0;
     int use_graph = 0;
     int x0_extrap = 0, x0_hist = 0;     // extrapolated initial iterates; history present
+    int vcycle = 0;                     // coarse end: 1 the V-cycle to the dense solve in LDS
+    int vlc[CFD_ENS_MAX_CLASSES];       // first level of the LDS coarse end per class
+    VcArgs vargs[CFD_ENS_MAX_CLASSES];
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_x = nullptr;
     long long iters_total = 0, solves = 0;
@@ -55,7 +58,34 @@ int ens_pred(cfd_ens* e, int k, int tx, int ty) {
     return 0;
 }
 
+// one iteration with the complete V-cycle: legs down to level vlc, the LDS
+// coarse end (one work-group per instance), legs up, the FP64 vector kernels
+int ens_iteration_vcycle(cfd_ens* e) {
+    for (int k = 0; k < e->ncls; ++k) {
+        const EnsClass& c = e->h[k];
+        EnsClass* dc = e->d + k;
+        const int Lc = e->vlc[k];
+        for (int l = 0; l < Lc; ++l) {
+            E_LAUNCH(k_e_smooth, dim3(cells1(c.lev[l]), c.B), dim3(256), dc, l, 0);
+            E_LAUNCH(k_e_restrict, dim3(cells1(c.lev[l + 1]), c.B), dim3(256), dc, l);
+        }
+        E_LAUNCH(k_lv_vcoarse, dim3(c.B), dim3(1024), e->vargs[k]);
+        for (int l = Lc - 1; l >= 1; --l) {
+            E_LAUNCH(k_e_post, dim3(cells1(c.lev[l]), c.B), dim3(256), dc, l);
+            E_LAUNCH(k_e_smooth, dim3(cells1(c.lev[l]), c.B), dim3(256), dc, l, 0);
+        }
+        E_LAUNCH(k_e_post, dim3(cells1(c.lev[0]), c.B), dim3(256), dc, 0);
+        E_LAUNCH(k_e_smooth_dots, dim3(ENS_BANDS, c.B), dim3(256), dc);
+        E_LAUNCH(k_e_dir, dim3(ENS_BANDS, c.B), dim3(256), dc);
+        E_LAUNCH(k_e_update, dim3(ENS_BANDS, c.B), dim3(256), dc);
+        E_LAUNCH(k_e_true, dim3(ENS_BANDS, c.B), dim3(256), dc);
+        E_LAUNCH(k_e_true_fin, dim3(1), dim3(256), dc);
+    }
+    return 0;
+}
+
 int ens_iteration(cfd_ens* e) {
+    if (e->vcycle) return ens_iteration_vcycle(e);
     for (int k = 0; k < e->ncls; ++k) {
         const EnsClass& c = e->h[k];
         EnsClass* dc = e->d + k;
@@ -376,6 +406,58 @@ int cfd_ens_set_net(cfd_ens* e, cfd_unet* net, float sigma) {
         c.gain = sigma / (4.0f * c.ax);
     }
     HIPC(hipMemcpy(e->d, e->h, e->ncls * sizeof(EnsClass), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int cfd_ens_set_coarse(cfd_ens* e, int32_t mode) {
+    if (!e) return set_err(CFD_ERR_ARG, "null ensemble");
+    if (mode != 0 && mode != 1) return set_err(CFD_ERR_ARG, "coarse mode %d", mode);
+    HIPC(hipStreamSynchronize(e->stream));
+    ens_drop_graph(e);
+    if (mode == 0) {
+        e->vcycle = 0;
+        return 0;
+    }
+    for (int k = 0; k < e->ncls; ++k) {
+        const EnsClass& c = e->h[k];
+        int lc = 1;
+        while (lc < c.nlev - 1 && (long)c.lev[lc].n * c.lev[lc].n > 64L * 64) ++lc;
+        long need = 0;
+        for (int l = lc; l < c.nlev; ++l) need += (long)VC_ARR * (c.lev[l].n + 2) * (c.lev[l].n + 2);
+        if (c.nlev - lc > VC_MAXL || c.ncoarse > VC_MAXC || need > VC_POOL)
+            return set_err(CFD_ERR_LIMIT, "class %d: the coarse levels do not fit the LDS coarse end", k);
+        VcArgs& v = e->vargs[k];
+        std::memset(&v, 0, sizeof(v));
+        v.done = reinterpret_cast<const char*>(c.st) + offsetof(EnsState, done);
+        v.done_stride = (long)sizeof(EnsState);
+        v.cinv = c.cinv;
+        v.omega = c.omega;
+        v.wc = c.wc;
+        v.periodic = 0;
+        v.nl = c.nlev - lc;
+        v.nc = c.ncoarse;
+        int off = 0;
+        for (int q = 0; q < v.nl; ++q) {
+            const EnsLevel& G = c.lev[lc + q];
+            VcLev& L = v.lev[q];
+            L.nx = L.ny = G.n;
+            L.pw = G.n + 2;
+            L.sz = (G.n + 2) * L.pw;
+            L.off = off;
+            L.P = G.P;
+            L.m = ((1ull << 32) + (unsigned long long)G.n - 1) / (unsigned long long)G.n;
+            L.istride = G.S;
+            L.gd = G.diag;
+            L.ge = G.cE;
+            L.gn = G.cN;
+            L.gb = G.b;
+            L.gt = G.t;
+            off += VC_ARR * L.sz;
+            v.total += L.sz;
+        }
+        e->vlc[k] = lc;
+    }
+    e->vcycle = 1;
     return 0;
 }
 

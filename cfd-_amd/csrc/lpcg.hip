@@ -1005,19 +1005,25 @@ __host__ __device__ inline long vc_need(const MGDev& mg, int Lc) {
 
 // one LDS level: sizes, the offset of its six arrays in the pool, the
 // multiplier of the exact division q / ny for q < 2^16 (m = ceil(2^32 / ny)),
-// and the global arrays it is loaded from / written back to
+// and the global arrays it is loaded from / written back to (the right-hand
+// side and correction of instance blockIdx.x at gb / gt + blockIdx.x * istride;
+// the coefficients are shared)
 struct VcLev {
     int nx, ny, pw, sz, off, P;
     unsigned long long m;
+    long istride;
     const float *gd, *ge, *gn, *gb;
     float* gt;
 };
 
 // arguments of k_lv_vcoarse, built on the host from the hierarchy (kernel
-// arguments: no descriptor loads on the device)
+// arguments: no descriptor loads on the device).  One work-group per instance
+// (the single case: one); an instance whose solve has finished (the int at
+// done + blockIdx.x * done_stride bytes) returns at once.
 struct VcArgs {
-    FcgState* fs;
-    const float* cinv;       // dense inverse of the coarsest level
+    const char* done;
+    long done_stride;
+    const float* cinv;       // dense inverse of the coarsest level (shared)
     float omega, wc;
     int periodic, nl, nc;    // levels in LDS, cells of the coarsest level
     int total;               // sum of sz over the levels
@@ -1091,7 +1097,8 @@ __global__ void __launch_bounds__(1024) k_lv_vcoarse(VcArgs a) {
     // every global load of the launch in flight at once: the solve's state,
     // this lane's row of the dense inverse (wave 0), the coefficients of every
     // level (ghost ring included) and level Lc's right-hand side
-    const int done = a.fs->done;
+    const long inst = blockIdx.x;
+    const int done = *reinterpret_cast<const int*>(a.done + inst * a.done_stride);
     float crow[VC_MAXC];
     if (tid < nc) {
 #pragma unroll
@@ -1113,7 +1120,7 @@ __global__ void __launch_bounds__(1024) k_lv_vcoarse(VcArgs a) {
         rd[u] = L.gd[g];
         re[u] = L.ge[g];
         rn[u] = L.gn[g];
-        if (q == 0 && i >= 1 && i <= L.nx && j >= 1 && j <= L.ny) rb[u] = L.gb[g];
+        if (q == 0 && i >= 1 && i <= L.nx && j >= 1 && j <= L.ny) rb[u] = L.gb[inst * L.istride + g];
         dst[u] = (q << 24) | e;   // level and position within its arrays
     }
     if (done) return;
@@ -1225,7 +1232,7 @@ __global__ void __launch_bounds__(1024) k_lv_vcoarse(VcArgs a) {
             int i, j;
             vc_ij(L, c, i, j);
             const int k = i * L.pw + j;
-            if (L.d[k] != 0.0f) G.gt[ix(i, j, G.P)] = L.t[k];
+            if (L.d[k] != 0.0f) G.gt[inst * G.istride + ix(i, j, G.P)] = L.t[k];
         }
     }
 }

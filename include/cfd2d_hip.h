@@ -507,6 +507,11 @@ int cfd_ens_destroy(cfd_ens* ens);
 /* net: a batch of (total instances) images of 32 x 32 (cfd_unet_set_batch(net,
    32)), or NULL for the constant-vector coarse correction alone */
 int cfd_ens_set_net(cfd_ens* ens, cfd_unet* net, float sigma);
+/* Coarse end of every instance's V-cycle: 0 the network (or the
+   constant-vector correction, cfd_ens_set_net), 1 the V-cycle carried on to
+   the dense coarsest solve, the levels of at most 64 x 64 cells held in LDS
+   by one work-group per instance (the complete multigrid preconditioner) */
+int cfd_ens_set_coarse(cfd_ens* ens, int32_t mode);
 int cfd_ens_step(cfd_ens* ens, int32_t count);
 /* field 0 u, 1 v, 2 p, 3 T of one instance, reference layout, float */
 int cfd_ens_download(cfd_ens* ens, int32_t cls, int32_t inst, int32_t field, float* host);

@@ -119,3 +119,72 @@ def test_extrapolated_start_saves_iterations(gpu):
     assert out["extrapolate"][0] < 0.8 * out["previous"][0]
     for ue, up in zip(out["extrapolate"][1], out["previous"][1]):
         assert np.abs(ue - up).max() < 1e-5
+
+
+def _wide():
+    """All four sides, both families, the ends of both parameter ranges."""
+    from cfd2d_amd.ensemble import Instance
+
+    return [Instance("lid", 64, re=100.0), Instance("heated", 64, ra=1e6),
+            Instance("heated", 128, ra=1e4), Instance("lid", 256, re=5000.0),
+            Instance("lid", 512, re=1000.0), Instance("heated", 512, ra=1e6)]
+
+
+@pytest.mark.parametrize("coarse", ["network", "vcycle"])
+def test_instances_match_fp64_oracle(gpu, coarse):
+    """Every instance of a batch covering sides 64-512, Re 100-5000 and Ra
+    1e4-1e6 against the FP64 oracle (cases.py:60-70, 207-223; pressure by the
+    exact DCT solve of the closed box) after 20 steps: the same t and dt,
+    velocity and temperature within 1e-5 (the FP32 bulk and the 1e-6 solver
+    tolerance), every solve converged."""
+    from cfd2d_amd.ensemble import Ensemble
+    from oracle import cfd_oracle as O
+
+    ins = _wide()
+    ens = Ensemble(ins, coarse=coarse)
+    steps = 20
+    ens.step(steps)
+    for k, inst in enumerate(ins):
+        print(f"ensemble vs oracle ({coarse}): {inst.kind} n={inst.n}", flush=True)
+        if inst.kind == "lid":
+            ref = O.cavity(re=inst.re, n=inst.n, pressure_solver="dct")
+        else:
+            ref = O.heated_cavity(ra=inst.ra, pr=inst.pr, n=inst.n, pressure_solver="dct")
+        for _ in range(steps):
+            ref.step()
+        st = ens.stats(k)
+        assert st["steps"] == steps and st["t"] == pytest.approx(ref.t, rel=1e-12)
+        assert st["relres"] <= 1e-6 and st["info"] == 0
+        du = np.abs(ens.field(k, "u") - ref.u).max()
+        dv = np.abs(ens.field(k, "v") - ref.v).max()
+        msg = f"{inst.kind} n={inst.n}: max|du| {du:.2e} max|dv| {dv:.2e}"
+        assert du < 1e-5 and dv < 1e-5, msg
+        if inst.kind == "heated":
+            T = ref.scalars["T"].phi
+            dT = np.abs(ens.field(k, "T")[1:-1, 1:-1] - T[1:-1, 1:-1]).max()
+            msg += f" max|dT| {dT:.2e}"
+            assert dT < 1e-5, msg
+        print(msg)
+    ens.close()
+
+
+def test_vcycle_coarse_end_is_faster_and_converges(gpu):
+    """The complete V-cycle coarse end against the shared network on the
+    full 256-instance workload: fewer lockstep iterations per step."""
+    from cfd2d_amd.ensemble import Ensemble, sample_instances
+
+    ins = sample_instances(256, 0)
+    out = {}
+    for coarse in ("network", "vcycle"):
+        ens = Ensemble(ins, coarse=coarse)
+        ens.step(3)
+        s0 = ens.stats(0)["sweeps_total"]
+        t0 = time.perf_counter()
+        ens.step(5)
+        wall = time.perf_counter() - t0
+        out[coarse] = ((ens.stats(0)["sweeps_total"] - s0) / 5, 1e3 * wall / 5)
+        assert max(ens.stats(k)["relres"] for k in range(len(ins))) <= 1e-6
+        ens.close()
+    print("256 instances: " + ", ".join(f"{c} {it:.1f} iterations {ms:.1f} ms/step"
+                                        for c, (it, ms) in out.items()))
+    assert out["vcycle"][0] < out["network"][0]

@@ -254,3 +254,32 @@ def test_device_wait_give_up_is_reported(gpu, solver):
         fresh.step()
     assert fresh.history.divergence[-1] < 1e-6
     fresh.close()
+
+
+def test_scalar_dt_limit_binds(gpu):
+    """ScalarField.dt_limit (models.py:210-215) as the binding time-step limit:
+    a scalar whose diffusivity (0.05) is 50 times the viscosity sets
+    dt = tau * 0.5 / (D (1/dx^2 + 1/dy^2)) exactly (solver.py:227-230), and the
+    trajectory follows the oracle's step for step."""
+    from cfd2d_amd import cases
+    from oracle import cfd_oracle as O
+
+    n, D = 32, 0.05
+    case = cases.lid_driven_cavity(re=1000, n=n, pressure_solver="cg", pressure_tol=1e-12,
+                                   pressure_maxiter=20000)
+    rng = np.random.default_rng(4)
+    phi0 = rng.uniform(0.0, 1.0, (n, n))
+    cases.add_scalar(case, "c", diffusivity=D, limiter="superbee", initial=phi0)
+    ref = O.cavity(re=1000, n=n, pressure_solver="cg", pressure_tol=1e-12)
+    ref.pressure.maxiter = 20000
+    O.add_scalar(ref, "c", D, limiter="superbee", initial=phi0)
+    h = 1.0 / n
+    limit = 0.5 * (0.5 / (D * (1.0 / h ** 2 + 1.0 / h ** 2)))
+    for _ in range(10):
+        case.sim.step()
+        ref.step()
+        assert case.sim.dt == ref.dt == limit
+    case.sim._flush()
+    assert np.abs(case.sim.u - ref.u).max() < 1e-11
+    assert np.abs(case.sim.scalars["c"].phi - ref.scalars["c"].phi).max() < 1e-12
+    case.sim.close()
