@@ -261,15 +261,20 @@ def roofline_for(leg: dict) -> dict:
             "launch_ms": round(leg["proj_ms_mean"], 4), "cg_iters_per_launch": round(n_it, 2)}
 
 
-def run_ensemble(steps: int = 10, warmup: int = 3, net: bool = True) -> dict:
+def run_ensemble(steps: int = 10, warmup: int = 3, net: bool = True,
+                 coarse: str = "network", use_graph: bool | None = None) -> dict:
     """The batched-ensemble workload: 256 lid / heated cavities of sides
-    {64, 128, 256, 512} (seed 0), FP32 bulk, shared learned PCG; aggregate
-    cell updates per second over `steps` steps after `warmup`."""
+    {64, 128, 256, 512} (seed 0), FP32 bulk, the learned solver's flexible CG
+    with the coarse end `coarse` ("vcycle": the V-cycle to the dense solve;
+    "network": the shared U-Net, or the constant-vector correction alone
+    without `net`); aggregate cell updates per second over `steps` steps after
+    `warmup`, and the roofline of the step."""
     from cfd2d_amd.ensemble import Ensemble, sample_instances
 
     ins = sample_instances(256, 0)
     t0 = time.perf_counter()
-    ens = Ensemble(ins, net=net, use_graph=USE_GRAPH)
+    graph = USE_GRAPH if use_graph is None else use_graph
+    ens = Ensemble(ins, net=net, use_graph=graph, coarse=coarse)
     setup = time.perf_counter() - t0
     ens.step(warmup)
     s0 = ens.stats(0)["sweeps_total"]
@@ -278,11 +283,25 @@ def run_ensemble(steps: int = 10, warmup: int = 3, net: bool = True) -> dict:
     wall = time.perf_counter() - t0
     sweeps = ens.stats(0)["sweeps_total"] - s0
     rel = max(ens.stats(k)["relres"] for k in range(len(ins)))
-    out = {"instances": len(ins), "cells": ens.cells, "steps": steps, "warmup": warmup,
+    # algorithmic bytes per step: per cell 4-byte bulk fields -- predictor 8,
+    # right-hand side 3, correction 7, temperature 4 (both families are
+    # counted with it) -- plus per lockstep iteration the FP64 outer vectors
+    # and the FP32 V-cycle as in the single case (190 B); the lockstep
+    # iterations bound the per-instance ones from above
+    it = sweeps / steps
+    bytes_step = ens.cells * (4.0 * (8 + 3 + 7 + 4) + it * 190.0)
+    gbs = bytes_step / (wall / steps) / 1e9
+    out = {"coarse": coarse if (net or coarse == "vcycle") else "none",
+           "launch": ("one graph launch per step, iterations controlled on the device"
+                      if graph else "direct launches, the host polls the solves"),
+           "instances": len(ins), "cells": ens.cells, "steps": steps, "warmup": warmup,
            "ms_per_step": round(1e3 * wall / steps, 3),
            "cell_updates_per_s": round(ens.cells * steps / wall, 1),
            "solver_iterations_per_step": round(sweeps / steps, 2),
            "relres_max": rel, "setup_s": round(setup, 2),
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                        "bytes_per_step": bytes_step},
            "predictor_shapes": {str(ins[c[0]].n): [ens.stats(c[0])["pred_tx"],
                                                     ens.stats(c[0])["pred_ty"]]
                                 for c in ens.classes}}
@@ -573,7 +592,10 @@ def main() -> None:
     ensemble = None
     if not args.no_ensemble:
         try:
-            ensemble = run_ensemble(net=True)
+            ensemble = run_ensemble(coarse="vcycle")
+            if USE_GRAPH:
+                ensemble["direct_launches"] = run_ensemble(coarse="vcycle", use_graph=False)
+            ensemble["with_network"] = run_ensemble(net=True)
             ensemble["without_network"] = run_ensemble(net=False)
         except (ValueError, RuntimeError) as e:
             ensemble = {"error": str(e)}

@@ -10,8 +10,10 @@ FP32; the pressure solve is the learned flexible CG (FP64 vectors and dot
 products, true residual <= rtol), whose coarse corrections for all instances
 come from one batched pass of the shared U-Net per iteration; each solve
 starts from the linear extrapolation in time of the instance's last two
-solutions (``x0="extrapolate"``; "previous": from the last one).
-``coarse="vcycle"`` replaces the network by the V-cycle carried on to the
+solutions (``x0="extrapolate"``; "previous": from the last one).  With
+``use_graph`` a step is one graph launch whose iteration count is controlled
+on the device (a step that needs more iterations than the graph holds
+continues in the next launch).  ``coarse="vcycle"`` replaces the network by the V-cycle carried on to the
 dense coarsest solve (levels of at most 64 x 64 cells in LDS, one work-group
 per instance).  The predictor's work-group shape is chosen per class by
 timing at setup (``autotune``).

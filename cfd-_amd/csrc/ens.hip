@@ -36,6 +36,25 @@ struct EnsState {
     int has_old, k, first, done, pending, iters, info, reps;
 };
 
+// Step control shared by all classes (device memory).  A step captured as one
+// graph runs a fixed number of solver iterations (finished instances return
+// at once); when a solve is still unfinished after them, k_e_cap sets `stall`:
+// every pre- and post-solve kernel then returns at once, so the next launch of
+// the graph only continues the iterations, and the step ends in the launch
+// whose k_e_cap finds every solve done.  The buffer parity and the history of
+// the extrapolated start live here too, so a launch depends on no host value.
+struct EnsCtl {
+    int stall;               // a solve of the current step is unfinished
+    int par;                 // parity of the step's hu / hv / T buffers
+    int hist;                // the extrapolated start has its history
+    int x0ext;               // extrapolated initial iterates in use
+    long long iters;         // lockstep solver iterations of all completed steps
+    int last_max;            // lockstep iterations of the last completed step
+    int max_call;            // their largest value since the host cleared it
+    int stalls;              // launches that ended with a solve unfinished
+    int pad;
+};
+
 struct EnsClass {
     int n, P, B, K, nlev;
     long S;
@@ -63,6 +82,7 @@ struct EnsClass {
     int maxiter;
     int use_net;
     int* notdone;             // per class: instances still iterating
+    EnsCtl* ctl;              // shared step control
 };
 
 namespace {
@@ -217,6 +237,7 @@ __device__ __forceinline__ float tvd_face(float phu, float phc, float phd, bool 
 // cavity walls (boundary.py:143-223 for four no-slip walls, lid on top), one
 // work-group per instance, in the reference's statement order
 __global__ void __launch_bounds__(256) k_e_bc(const EnsClass* __restrict__ c) {
+    if (c->ctl->stall) return;
     const int b = blockIdx.x, n = c->n, P = c->P;
     float* u = c->u + b * c->S;
     float* v = c->v + b * c->S;
@@ -242,6 +263,7 @@ __global__ void __launch_bounds__(256) k_e_bc(const EnsClass* __restrict__ c) {
 
 // max |u|, max |v| over the whole arrays (ghosts included), per band
 __global__ void __launch_bounds__(256) k_e_maxuv(const EnsClass* __restrict__ c) {
+    if (c->ctl->stall) return;
     const int b = blockIdx.y, n = c->n, P = c->P;
     const float* u = c->u + b * c->S;
     const float* v = c->v + b * c->S;
@@ -258,6 +280,7 @@ __global__ void __launch_bounds__(256) k_e_maxuv(const EnsClass* __restrict__ c)
 
 // dt, gamma and the AB2 weights of every instance (solver.py:209-253)
 __global__ void __launch_bounds__(256) k_e_dt(const EnsClass* __restrict__ c) {
+    if (c->ctl->stall) return;
     for (int b = threadIdx.x; b < c->B; b += blockDim.x) {
         const double* pa = ens_part(c, ENS_P_A, b);
         double umax_raw = 0.0, vmax_raw = 0.0;
@@ -291,7 +314,9 @@ __global__ void __launch_bounds__(256) k_e_dt(const EnsClass* __restrict__ c) {
 
 // fused predictor: tendencies + Boussinesq buoyancy (gravity (0, -1),
 // T_ref 0) + AB2 + F/G; writes hu/hv of parity par
-__global__ void k_e_pred(const EnsClass* __restrict__ c, int par) {
+__global__ void k_e_pred(const EnsClass* __restrict__ c) {
+    if (c->ctl->stall) return;
+    const int par = c->ctl->par;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y * blockDim.y + threadIdx.y;
     const int b = blockIdx.z, n = c->n, P = c->P;
@@ -340,6 +365,7 @@ __global__ void k_e_pred(const EnsClass* __restrict__ c, int par) {
 
 // right-hand side div(F, G) / dt (FP64) and its partial sums
 __global__ void __launch_bounds__(256) k_e_rhs(const EnsClass* __restrict__ c) {
+    if (c->ctl->stall) return;
     const int b = blockIdx.y, n = c->n, P = c->P;
     const long o = b * c->S;
     const double dt = c->st[b].dt, h = c->hd;
@@ -359,7 +385,9 @@ __global__ void __launch_bounds__(256) k_e_rhs(const EnsClass* __restrict__ c) {
 
 // initial iterate of the instance's solve: x <- x + theta (x - x_prev) with
 // theta = dt / dt_old, x_prev <- the old x (hist = 0: the copy only)
-__global__ void __launch_bounds__(256) k_e_x0(const EnsClass* __restrict__ c, int hist) {
+__global__ void __launch_bounds__(256) k_e_x0(const EnsClass* __restrict__ c) {
+    if (c->ctl->stall) return;
+    const int hist = c->ctl->hist;
     const int b = blockIdx.y, n = c->n, P = c->P;
     const EnsState& st = c->st[b];
     const double th = hist && st.dt_old > 0.0 ? st.dt / st.dt_old : 0.0;
@@ -378,6 +406,7 @@ __global__ void __launch_bounds__(256) k_e_x0(const EnsClass* __restrict__ c, in
 // b = -(rhs - mean) (compatibility), r = b - A x, first V-cycle input;
 // partial b.b and r.r
 __global__ void __launch_bounds__(256) k_e_res(const EnsClass* __restrict__ c) {
+    if (c->ctl->stall) return;
     const int b = blockIdx.y, n = c->n, P = c->P;
     const long o = b * c->S;
     double tot[1];
@@ -409,6 +438,7 @@ __global__ void __launch_bounds__(256) k_e_res(const EnsClass* __restrict__ c) {
 
 // per instance: the solver's initial state
 __global__ void __launch_bounds__(256) k_e_init(const EnsClass* __restrict__ c) {
+    if (c->ctl->stall) return;
     int cnt = 0;
     for (int b = threadIdx.x; b < c->B; b += blockDim.x) {
         double s[2];
@@ -804,6 +834,7 @@ __global__ void __launch_bounds__(256) k_e_true_fin(const EnsClass* __restrict__
 
 // ---- end of the projection ------------------------------------------------
 __global__ void __launch_bounds__(256) k_e_xsum(const EnsClass* __restrict__ c) {
+    if (c->ctl->stall) return;
     const int b = blockIdx.y, n = c->n, P = c->P;
     const long o = b * c->S;
     int i0, i1;
@@ -816,6 +847,7 @@ __global__ void __launch_bounds__(256) k_e_xsum(const EnsClass* __restrict__ c) 
 
 // pressure = x minus its mean (the pure-Neumann component), FP32
 __global__ void __launch_bounds__(256) k_e_pwrite(const EnsClass* __restrict__ c) {
+    if (c->ctl->stall) return;
     const int b = blockIdx.y, n = c->n, P = c->P;
     const long o = b * c->S;
     double tot[1];
@@ -837,6 +869,7 @@ __global__ void __launch_bounds__(256) k_e_pwrite(const EnsClass* __restrict__ c
 
 // u = F - dt grad p on interior faces (F elsewhere), partial max |u_new - u_old|
 __global__ void __launch_bounds__(256) k_e_correct(const EnsClass* __restrict__ c) {
+    if (c->ctl->stall) return;
     const int b = blockIdx.y, n = c->n, P = c->P;
     const long o = b * c->S;
     const float dt = (float)c->st[b].dt, h = c->h;
@@ -876,7 +909,9 @@ __global__ void __launch_bounds__(256) k_e_correct(const EnsClass* __restrict__ 
 // temperature: TVD (van Leer) transport and Euler update on every cell, with
 // fixed values on the left (0.5) and right (-0.5) walls and zero gradient on
 // the bottom and top (cases.py:207-223; kernels.py:254-391 for these codes)
-__global__ void k_e_temp(const EnsClass* __restrict__ c, int par) {
+__global__ void k_e_temp(const EnsClass* __restrict__ c) {
+    if (c->ctl->stall) return;
+    const int par = c->ctl->par;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y * blockDim.y + threadIdx.y;
     const int b = blockIdx.z, n = c->n, P = c->P;
@@ -943,7 +978,9 @@ __global__ void k_e_temp(const EnsClass* __restrict__ c, int par) {
 }
 
 // temperature ghosts (models.py:139-183 for these codes), one work-group per instance
-__global__ void __launch_bounds__(256) k_e_tghost(const EnsClass* __restrict__ c, int par) {
+__global__ void __launch_bounds__(256) k_e_tghost(const EnsClass* __restrict__ c) {
+    if (c->ctl->stall) return;
+    const int par = c->ctl->par;
     const int b = blockIdx.x, n = c->n, P = c->P;
     if (c->kappa[b] <= 0.0) return;
     float* T = c->T[par ^ 1] + b * c->S;
@@ -957,6 +994,7 @@ __global__ void __launch_bounds__(256) k_e_tghost(const EnsClass* __restrict__ c
 
 // monitors: partial max |div u| and sum of cell kinetic energy
 __global__ void __launch_bounds__(256) k_e_record(const EnsClass* __restrict__ c) {
+    if (c->ctl->stall) return;
     const int b = blockIdx.y, n = c->n, P = c->P;
     const long o = b * c->S;
     const float* u = c->u + o;
@@ -980,6 +1018,7 @@ __global__ void __launch_bounds__(256) k_e_record(const EnsClass* __restrict__ c
 
 // per instance: monitors and the state advance (solver.py:378-401)
 __global__ void __launch_bounds__(256) k_e_advance(const EnsClass* __restrict__ c) {
+    if (c->ctl->stall) return;
     for (int b = threadIdx.x; b < c->B; b += blockDim.x) {
         EnsState& st = c->st[b];
         double mx = 0.0, ke = 0.0, res = 0.0;
@@ -999,5 +1038,44 @@ __global__ void __launch_bounds__(256) k_e_advance(const EnsClass* __restrict__ 
         st.dt_old = dt;
         st.t += dt;
         st.nstep += 1;
+    }
+}
+
+// after a step's solver iterations (one work-group): whether every solve of
+// every class has finished -- then the step's lockstep iterations (the most
+// any instance took) are counted and the post-solve kernels run -- or the
+// step stalls until the next launch continues its iterations
+__global__ void __launch_bounds__(256) k_e_cap(const EnsClass* __restrict__ d, int ncls) {
+    __shared__ int s_left, s_max;
+    if (threadIdx.x == 0) {
+        s_left = 0;
+        s_max = 0;
+    }
+    __syncthreads();
+    for (int k = 0; k < ncls; ++k) {
+        const EnsClass* c = d + k;
+        if (threadIdx.x == 0) atomicAdd(&s_left, *c->notdone);
+        for (int b = threadIdx.x; b < c->B; b += blockDim.x) atomicMax(&s_max, c->st[b].k);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        EnsCtl* ctl = d->ctl;
+        if (s_left > 0) {
+            ctl->stall = 1;
+            ctl->stalls += 1;
+        } else {
+            ctl->stall = 0;
+            ctl->iters += s_max;
+            ctl->last_max = s_max;
+            ctl->max_call = max(ctl->max_call, s_max);
+        }
+    }
+}
+
+// end of a completed step: the next step's buffer parity and history
+__global__ void k_e_flip(EnsCtl* ctl) {
+    if (threadIdx.x == 0 && !ctl->stall) {
+        ctl->par ^= 1;
+        ctl->hist = ctl->x0ext;
     }
 }

@@ -1,7 +1,12 @@
 // Host side of the batched cavity ensemble (ens.hip): classes of equal-size
 // instances, one launch sequence per step for all of them, the learned
-// flexible CG with one batched network pass per iteration, and the predictor
-// tile shape chosen per class by timing the candidates at setup.
+// flexible CG with one batched network pass (or the LDS coarse end) per
+// iteration, and the predictor tile shape chosen per class by timing the
+// candidates at setup.  With graphs a step is one graph launch: the
+// pre-solve kernels, `cap` solver iterations, k_e_cap, the post-solve kernels
+// and k_e_flip, all controlled on the device (EnsCtl); the host checks the
+// completed steps once per cfd_ens_step call and adapts `cap` to the
+// iterations the steps took.
 
 struct cfd_ens {
     int ncls = 0;
@@ -15,17 +20,18 @@ struct cfd_ens {
     int use_net = 0;
     int* notdone = nullptr;
     int* notdone_h = nullptr;           // pinned
-    int par = 0;
+    EnsCtl* ctl = nullptr;              // device step control (ens.hip)
+    int cap = 16;                       // solver iterations in the step graph
+    int cap_graph = 0;                  // cap of the captured graph (0: none)
     long long steps = 0;
     int prev_iters = 0;
     int use_graph = 0;
-    int x0_extrap = 0, x0_hist = 0;     // extrapolated initial iterates; history present
+    int x0_extrap = 0;                  // extrapolated initial iterates
     int vcycle = 0;                     // coarse end: 1 the V-cycle to the dense solve in LDS
     int vlc[CFD_ENS_MAX_CLASSES];       // first level of the LDS coarse end per class
     VcArgs vargs[CFD_ENS_MAX_CLASSES];
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_x = nullptr;
-    long long iters_total = 0, solves = 0;
     hipStream_t stream = nullptr;
     std::vector<void*> allocs;
 };
@@ -54,7 +60,7 @@ unsigned cells1(const EnsLevel& L) { return (unsigned)(L.n * ((L.n + 255) / 256)
 int ens_pred(cfd_ens* e, int k, int tx, int ty) {
     const EnsClass& c = e->h[k];
     const dim3 grid((c.n + 2 + tx - 1) / tx, (c.n + 2 + ty - 1) / ty, c.B);
-    E_LAUNCH(k_e_pred, grid, dim3(tx, ty), e->d + k, e->par);
+    E_LAUNCH(k_e_pred, grid, dim3(tx, ty), e->d + k);
     return 0;
 }
 
@@ -130,26 +136,7 @@ void ens_drop_graph(cfd_ens* e) {
     if (e->graph) hipGraphDestroy(e->graph);
     e->graph_x = nullptr;
     e->graph = nullptr;
-}
-
-// one solver iteration of every class, replayed from a captured graph
-int ens_iteration_launch(cfd_ens* e) {
-    if (!e->use_graph) return ens_iteration(e);
-    if (!e->graph_x) {
-        HIPC(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
-        const int rc = ens_iteration(e);
-        hipGraph_t g = nullptr;
-        const hipError_t err = hipStreamEndCapture(e->stream, &g);
-        if (rc) {
-            if (g) hipGraphDestroy(g);
-            return rc;
-        }
-        HIPC(err);
-        e->graph = g;
-        HIPC(hipGraphInstantiate(&e->graph_x, g, nullptr, nullptr, 0));
-    }
-    HIPC(hipGraphLaunch(e->graph_x, e->stream));
-    return 0;
+    e->cap_graph = 0;
 }
 
 int ens_notdone(cfd_ens* e, int* total) {
@@ -162,8 +149,9 @@ int ens_notdone(cfd_ens* e, int* total) {
     return 0;
 }
 
-int ens_step(cfd_ens* e) {
-    const int par = e->par;
+// before the solve: boundary conditions, dt / gamma, predictor, right-hand
+// side, initial iterate and residual of every class
+int ens_pre(cfd_ens* e) {
     for (int k = 0; k < e->ncls; ++k) {
         const EnsClass& c = e->h[k];
         EnsClass* dc = e->d + k;
@@ -172,24 +160,18 @@ int ens_step(cfd_ens* e) {
         E_LAUNCH(k_e_dt, dim3(1), dim3(256), dc);
         TRY(ens_pred(e, k, e->pred_tx[k], e->pred_ty[k]));
         E_LAUNCH(k_e_rhs, dim3(ENS_BANDS, c.B), dim3(256), dc);
-        if (e->x0_extrap) E_LAUNCH(k_e_x0, dim3(ENS_BANDS, c.B), dim3(256), dc, e->x0_hist);
+        if (e->x0_extrap) E_LAUNCH(k_e_x0, dim3(ENS_BANDS, c.B), dim3(256), dc);
         E_LAUNCH(k_e_res, dim3(ENS_BANDS, c.B), dim3(256), dc);
         E_LAUNCH(k_e_init, dim3(1), dim3(256), dc);
     }
-    // iterations until every instance has finished (polled after the
-    // predicted count, then after each iteration)
-    int batch = std::max(1, e->prev_iters - 1), launched = 0, left = 1, maxit = 0;
-    for (int k = 0; k < e->ncls; ++k) maxit = std::max(maxit, e->h[k].maxiter);
-    for (;;) {
-        for (int q = 0; q < batch; ++q) TRY(ens_iteration_launch(e));
-        launched += batch;
-        TRY(ens_notdone(e, &left));
-        if (left == 0 || launched > maxit + 2) break;
-        batch = 1;
-    }
-    e->prev_iters = launched;
-    e->iters_total += launched;
-    e->solves += 1;
+    return 0;
+}
+
+// after the solve: pressure, correction, temperature, monitors, state
+// advance of every class, then the step's end (k_e_cap decided whether the
+// step is complete; k_e_flip moves to the next step's buffers)
+int ens_post(cfd_ens* e) {
+    E_LAUNCH(k_e_cap, dim3(1), dim3(256), e->d, e->ncls);
     for (int k = 0; k < e->ncls; ++k) {
         const EnsClass& c = e->h[k];
         EnsClass* dc = e->d + k;
@@ -199,15 +181,112 @@ int ens_step(cfd_ens* e) {
         E_LAUNCH(k_e_bc, dim3(c.B), dim3(256), dc);
         const dim3 gt((c.n + 2 + e->pred_tx[k] - 1) / e->pred_tx[k],
                       (c.n + 2 + e->pred_ty[k] - 1) / e->pred_ty[k], c.B);
-        E_LAUNCH(k_e_temp, gt, dim3(e->pred_tx[k], e->pred_ty[k]), dc, par);
-        E_LAUNCH(k_e_tghost, dim3(c.B), dim3(256), dc, par);
+        E_LAUNCH(k_e_temp, gt, dim3(e->pred_tx[k], e->pred_ty[k]), dc);
+        E_LAUNCH(k_e_tghost, dim3(c.B), dim3(256), dc);
         E_LAUNCH(k_e_record, dim3(ENS_BANDS, c.B), dim3(256), dc);
         E_LAUNCH(k_e_advance, dim3(1), dim3(256), dc);
     }
-    e->par ^= 1;
-    e->steps += 1;
-    e->x0_hist = e->x0_extrap;
+    E_LAUNCH(k_e_flip, dim3(1), dim3(64), e->ctl);
     return 0;
+}
+
+// a step with direct launches: iterations until every instance has finished
+// (the host polls after the predicted count, then after each iteration)
+int ens_step_direct(cfd_ens* e) {
+    TRY(ens_pre(e));
+    int batch = std::max(1, e->prev_iters - 1), launched = 0, left = 1, maxit = 0;
+    for (int k = 0; k < e->ncls; ++k) maxit = std::max(maxit, e->h[k].maxiter);
+    for (;;) {
+        for (int q = 0; q < batch; ++q) TRY(ens_iteration(e));
+        launched += batch;
+        TRY(ens_notdone(e, &left));
+        if (left == 0 || launched > maxit + 2) break;
+        batch = 1;
+    }
+    e->prev_iters = launched;
+    return ens_post(e);
+}
+
+// the step graph with `cap` iterations (captured once per cap)
+int ens_graph_ready(cfd_ens* e) {
+    if (!e->graph_x || e->cap_graph != e->cap) {
+        ens_drop_graph(e);
+        HIPC(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
+        int rc = ens_pre(e);
+        for (int q = 0; q < e->cap && !rc; ++q) rc = ens_iteration(e);
+        if (!rc) rc = ens_post(e);
+        hipGraph_t g = nullptr;
+        const hipError_t err = hipStreamEndCapture(e->stream, &g);
+        if (rc) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        HIPC(err);
+        e->graph = g;
+        HIPC(hipGraphInstantiate(&e->graph_x, g, nullptr, nullptr, 0));
+        e->cap_graph = e->cap;
+    }
+    return 0;
+}
+
+int ens_step_graph(cfd_ens* e) {
+    TRY(ens_graph_ready(e));
+    HIPC(hipGraphLaunch(e->graph_x, e->stream));
+    return 0;
+}
+
+int ens_read_ctl(cfd_ens* e, EnsCtl* h) {
+    HIPC(hipMemcpyAsync(h, e->ctl, sizeof(EnsCtl), hipMemcpyDeviceToHost, e->stream));
+    HIPC(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+// completed steps on the device (every instance advances in lockstep)
+int ens_device_steps(cfd_ens* e, long long* n) {
+    EnsState s;
+    HIPC(hipMemcpyAsync(&s, e->h[0].st, sizeof(s), hipMemcpyDeviceToHost, e->stream));
+    HIPC(hipStreamSynchronize(e->stream));
+    *n = s.nstep;
+    return 0;
+}
+
+// `count` steps: one graph launch each; launches that only continued a
+// stalled solve are made up until the device has completed them all; the
+// cap then follows the iteration count of the last step (+2)
+int ens_steps(cfd_ens* e, int count) {
+    if (!e->use_graph) {
+        for (int s = 0; s < count; ++s) TRY(ens_step_direct(e));
+        HIPC(hipStreamSynchronize(e->stream));
+        e->steps += count;
+        return 0;
+    }
+    long long n0 = 0;
+    TRY(ens_device_steps(e, &n0));
+    const long long target = n0 + count;
+    for (int s = 0; s < count; ++s) TRY(ens_step_graph(e));
+    long long n = 0;
+    int maxit = 0;
+    for (int k = 0; k < e->ncls; ++k) maxit = std::max(maxit, e->h[k].maxiter);
+    const long long limit = (long long)count * (maxit / std::max(1, e->cap) + 2);
+    long long extra = 0;
+    for (;;) {
+        TRY(ens_device_steps(e, &n));
+        if (n >= target) break;
+        if (++extra > limit)
+            return set_err(CFD_ERR_STATE, "ensemble steps did not complete (%lld of %d)", n - n0,
+                           count);
+        TRY(ens_step_graph(e));
+    }
+    EnsCtl h;
+    TRY(ens_read_ctl(e, &h));
+    // the last step's count + 2: the counts fall as the flow settles, and a
+    // step that needs more only continues in one more launch
+    const int want = std::min(maxit, std::max(1, h.last_max + 2));
+    h.max_call = 0;
+    HIPC(hipMemcpy(&e->ctl->max_call, &h.max_call, sizeof(int), hipMemcpyHostToDevice));
+    e->cap = want;
+    e->steps += count;
+    return ens_graph_ready(e);   // the next call starts without a capture
 }
 
 // Autotuner: the predictor's work-group shape per class, the fastest of the
@@ -358,13 +437,21 @@ int cfd_ens_create(const cfd_ens_desc* d, cfd_ens** out) {
     }
     e->nimg = img;
     if ((rc = ealloc(e, &e->img, (size_t)img * 1024)) || (rc = ealloc(e, &e->d, e->ncls)) ||
-        (rc = ealloc(e, &e->notdone, e->ncls)))
+        (rc = ealloc(e, &e->notdone, e->ncls)) || (rc = ealloc(e, &e->ctl, 1)))
         return fail(rc);
+    {
+        EnsCtl h;
+        std::memset(&h, 0, sizeof(h));
+        h.x0ext = d->x0_extrapolate ? 1 : 0;
+        if (hipMemcpy(e->ctl, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_err(CFD_ERR_HIP, "control upload failed"));
+    }
     if (hipHostMalloc((void**)&e->notdone_h, CFD_ENS_MAX_CLASSES * sizeof(int), 0) != hipSuccess)
         return fail(set_err(CFD_ERR_HIP, "pinned allocation failed"));
     for (int k = 0; k < e->ncls; ++k) {
         e->h[k].img = e->img;
         e->h[k].notdone = e->notdone + k;
+        e->h[k].ctl = e->ctl;
         e->pred_tx[k] = 256;
         e->pred_ty[k] = 1;
     }
@@ -463,9 +550,7 @@ int cfd_ens_set_coarse(cfd_ens* e, int32_t mode) {
 
 int cfd_ens_step(cfd_ens* e, int32_t count) {
     if (!e || count < 0) return set_err(CFD_ERR_ARG, "bad arguments");
-    for (int s = 0; s < count; ++s) TRY(ens_step(e));
-    HIPC(hipStreamSynchronize(e->stream));
-    return 0;
+    return ens_steps(e, count);
 }
 
 // fields of one instance in the reference layout: 0 u ((n+1) x (n+2)),
@@ -480,10 +565,15 @@ int cfd_ens_download(cfd_ens* e, int32_t cls, int32_t inst, int32_t field, float
     case 0: src = c.u; rows = c.n + 1; cols = c.n + 2; break;
     case 1: src = c.v; rows = c.n + 2; cols = c.n + 1; break;
     case 2: src = c.p; rows = c.n + 2; cols = c.n + 2; break;
-    case 3: src = c.T[e->par]; rows = c.n + 2; cols = c.n + 2; break;
+    case 3: src = c.T[0]; rows = c.n + 2; cols = c.n + 2; break;
     default: return set_err(CFD_ERR_ARG, "unknown field %d", field);
     }
     HIPC(hipStreamSynchronize(e->stream));
+    if (field == 3) {   // the current temperature buffer has the device's parity
+        EnsCtl h;
+        TRY(ens_read_ctl(e, &h));
+        src = c.T[h.par];
+    }
     HIPC(hipMemcpy2D(host, cols * sizeof(float), src + inst * c.S, c.P * sizeof(float),
                      cols * sizeof(float), rows, hipMemcpyDeviceToHost));
     return 0;
@@ -508,7 +598,9 @@ int cfd_ens_stats(cfd_ens* e, int32_t cls, int32_t inst, cfd_ens_stats_t* out) {
     out->pred_tx = e->pred_tx[cls];
     out->pred_ty = e->pred_ty[cls];
     out->pred_us = e->pred_us[cls];
-    out->sweeps_total = e->iters_total;
+    EnsCtl h;
+    TRY(ens_read_ctl(e, &h));
+    out->sweeps_total = h.iters;
     return 0;
 }
 

@@ -512,6 +512,11 @@ int cfd_ens_set_net(cfd_ens* ens, cfd_unet* net, float sigma);
    the dense coarsest solve, the levels of at most 64 x 64 cells held in LDS
    by one work-group per instance (the complete multigrid preconditioner) */
 int cfd_ens_set_coarse(cfd_ens* ens, int32_t mode);
+/* `count` steps of every instance.  With use_graph each step is one graph
+   launch whose solver iterations (a cap adapted to the previous steps' count)
+   are controlled on the device; a step whose solves need more continues in
+   the next launch, and the call returns once the device has completed
+   `count` steps.  Without it, the host polls the solves' progress. */
 int cfd_ens_step(cfd_ens* ens, int32_t count);
 /* field 0 u, 1 v, 2 p, 3 T of one instance, reference layout, float */
 int cfd_ens_download(cfd_ens* ens, int32_t cls, int32_t inst, int32_t field, float* host);
