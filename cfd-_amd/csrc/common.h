@@ -10,6 +10,27 @@ __host__ __device__ __forceinline__ long ix(int i, int j, int pitch) {
     return (long)i * pitch + j;
 }
 
+// Work-groups are dispatched round-robin over the 8 XCDs by their flattened
+// index (x fastest), each XCD with its own L2.  xcd_block maps the hardware
+// index to a logical one such that the work-groups of one XCD hold
+// consecutive logical indices: neighbouring rows then share an L2 instead of
+// being fetched again by another XCD.
+__device__ __forceinline__ int xcd_block(int hw, int total) {
+    const int x = hw & 7;
+    int off = 0;
+    for (int k = 0; k < x; ++k) off += (total - k + 7) >> 3;
+    return off + (hw >> 3);
+}
+
+// (column chunk, row) of a work-group of a (chunks x rows) grid, the rows
+// handled by one XCD contiguous (xcd_block)
+__device__ __forceinline__ int2 xcd_rowgrid() {
+    const int gx = gridDim.x;
+    const int b = xcd_block(blockIdx.x + blockIdx.y * gx, gx * gridDim.y);
+    const int row = b / gx;
+    return make_int2(b - row * gx, row);
+}
+
 // One-pass reduction over the w-wide block of n (< 2^31) entries starting at
 // (i0, j0), spread over the whole grid, U entries per thread at a time:
 // load(i, j) reads an entry's values (in-range indices: past the end the last

@@ -68,17 +68,6 @@ __device__ __forceinline__ MGCtx lv_ctx(const LvArgs& a) {
     __syncthreads();                                                           \
     const MGCtx m{(a).mg, (a).code, s_d0_, (a).periodic, (a).ax, (a).ay}
 
-// Work-groups are dispatched round-robin over the 8 XCDs, each with its own
-// L2.  xcd_block maps the hardware index to a logical one such that the
-// work-groups of one XCD hold consecutive logical indices: neighbouring row
-// bands then share an L2 instead of being fetched again by another XCD.
-__device__ __forceinline__ int xcd_block(int hw, int total) {
-    const int x = hw & 7;
-    int off = 0;
-    for (int k = 0; k < x; ++k) off += (total - k + 7) >> 3;
-    return off + (hw >> 3);
-}
-
 // f(i, j) over the level's interior cells: a band of rows per work-group,
 // threads along the contiguous j direction
 template <typename F>

@@ -107,8 +107,9 @@ __global__ void k_momentum_tendencies(const double* __restrict__ u, const double
                                       const uint8_t* __restrict__ fl, Geo g, double nu0,
                                       const double* __restrict__ nuf, double gamma,
                                       double* __restrict__ hu, double* __restrict__ hv) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
+    const int2 rc_ = xcd_rowgrid();
+    const int j = rc_.x * blockDim.x + threadIdx.x;
+    const int i = rc_.y;
     if (j > g.ny + 1) return;
     const double idx2 = 1.0 / (g.dx * g.dx), idy2 = 1.0 / (g.dy * g.dy);
     const double idx = 1.0 / g.dx, idy = 1.0 / g.dy;
@@ -138,8 +139,9 @@ __global__ void k_momentum_tendencies(const double* __restrict__ u, const double
 // Fused predictor (solver.py:338-368, models.py:386-396, boundary.py:225-242)
 // ---------------------------------------------------------------------------
 __global__ void k_predictor(PredArgs a, Geo g) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
+    const int2 rc_ = xcd_rowgrid();
+    const int j = rc_.x * blockDim.x + threadIdx.x;
+    const int i = rc_.y;
     const int nx = g.nx, ny = g.ny;
     if (j > ny + 1) return;
     const DevState* st = a.st;
@@ -240,8 +242,9 @@ __global__ void k_predictor(PredArgs a, Geo g) {
 // Copy F = u, G = v with the outflow / periodic extrapolation
 // (solver.py:199-201, the predictor of the initial projection).
 __global__ void k_copy_predictor(PredArgs a, Geo g) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
+    const int2 rc_ = xcd_rowgrid();
+    const int j = rc_.x * blockDim.x + threadIdx.x;
+    const int i = rc_.y;
     const int nx = g.nx, ny = g.ny;
     if (j > ny + 1) return;
     if (i <= nx) {
@@ -426,8 +429,9 @@ __global__ void k_bc_domain(BCArgs a, Geo g) {
 // ---------------------------------------------------------------------------
 __global__ void k_obstacle_bc(double* __restrict__ u, double* __restrict__ v,
                               const uint8_t* __restrict__ fl, Geo g, const DevState* st) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
+    const int2 rc_ = xcd_rowgrid();
+    const int j = rc_.x * blockDim.x + threadIdx.x;
+    const int i = rc_.y;
     const int nx = g.nx, ny = g.ny, pu = g.pu, pv = g.pv, pc = g.pc;
     if (j > ny + 1) return;
     if (st && st->done) return;
@@ -778,8 +782,9 @@ __device__ __forceinline__ double yface_term(const ScalarArgs& a, const Geo& g, 
 // MODE 1: phi_out = phi_in + dt * rhs on fluid cells, phi_in elsewhere.
 template <bool CONST_D, int MODE>
 __global__ void k_scalar_transport(ScalarArgs a, Geo g) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
+    const int2 rc_ = xcd_rowgrid();
+    const int j = rc_.x * blockDim.x + threadIdx.x;
+    const int i = rc_.y;
     const int nx = g.nx, ny = g.ny, pc = g.pc;
     if (j > ny + 1) return;
     if (MODE == 1 && a.st->done) return;
@@ -806,8 +811,9 @@ __global__ void k_scalar_transport(ScalarArgs a, Geo g) {
 // phi_in still holds the previous values: ghost cells built from a solid edge
 // cell use its previous value, as the reference fills ghosts before solids.
 __global__ void k_fill_ghosts(ScalarArgs a, Geo g) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
+    const int2 rc_ = xcd_rowgrid();
+    const int j = rc_.x * blockDim.x + threadIdx.x;
+    const int i = rc_.y;
     const int nx = g.nx, ny = g.ny, pc = g.pc;
     if (j > ny + 1) return;
     if (a.st && a.st->done) return;
