@@ -99,8 +99,8 @@ LEGS = {"cg": ("cg", None, "previous"),
         "cg_learned_x0prev": ("cg_learned", {"mode": "net"}, "previous")}
 # the same workload with a turbulence model on the device (variable viscosity,
 # nu_t per step; SST also transports k and omega): the cost of the model
-TURB_LEGS = {"cg_learned_smagorinsky": ("cg_learned", "Smagorinsky"),
-             "cg_learned_sst": ("cg_learned", "kOmegaSST")}
+TURB_LEGS = {"cg_learned_smagorinsky": ("cg_learned_vcycle", "Smagorinsky"),
+             "cg_learned_sst": ("cg_learned_vcycle", "kOmegaSST")}
 
 
 USE_GRAPH = True   # learned legs: replay each iteration as a captured HIP graph

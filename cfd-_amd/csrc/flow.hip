@@ -63,8 +63,9 @@ __global__ void __launch_bounds__(256) k_flow_bulk(FlowArgs a, Geo g) {
 
 // u += du on the fluid faces; periodic: face 0 repeats face nx
 __global__ void __launch_bounds__(256) k_flow_shift(FlowArgs a, Geo g) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
+    const int2 rc_ = xcd_rowgrid();
+    const int j = rc_.x * blockDim.x + threadIdx.x;
+    const int i = rc_.y;
     if (j > g.ny + 1 || i > g.nx) return;
     if (a.st->done) return;
     const int c = ix(i, j, g.pu);

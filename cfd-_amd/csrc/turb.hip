@@ -72,8 +72,9 @@ __device__ __forceinline__ double strain2(const double* __restrict__ u,
 
 // one thread per inner cell
 __global__ void __launch_bounds__(256) k_turb_cells(TurbArgs a, Geo g) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int i = blockIdx.y + 1;
+    const int2 rc_ = xcd_rowgrid();
+    const int j = rc_.x * blockDim.x + threadIdx.x + 1;
+    const int i = rc_.y + 1;
     if (j > g.ny) return;
     if (a.st && a.st->done) return;
     const int c = ix(i, j, g.pc);
@@ -178,8 +179,9 @@ struct SstArgs {
 // use the k and omega of the start of the step.
 __global__ void __launch_bounds__(256) k_sst_update(SstArgs a, Geo g) {
     using namespace sst;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
+    const int2 rc_ = xcd_rowgrid();
+    const int j = rc_.x * blockDim.x + threadIdx.x;
+    const int i = rc_.y;
     const int nx = g.nx, ny = g.ny, pc = g.pc;
     if (j > ny + 1) return;
     if (a.k.st->done) return;
@@ -234,8 +236,9 @@ __global__ void __launch_bounds__(256) k_sst_wall(double* __restrict__ w,
                                                   const uint8_t* __restrict__ wadj,
                                                   const double* __restrict__ wwall, Geo g,
                                                   const DevState* st) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
+    const int2 rc_ = xcd_rowgrid();
+    const int j = rc_.x * blockDim.x + threadIdx.x;
+    const int i = rc_.y;
     if (j > g.ny + 1) return;
     if (st && st->done) return;
     const int c = ix(i, j, g.pc);

@@ -64,8 +64,9 @@ __device__ __forceinline__ void vof_normal(const double* a, const Geo& g, int i,
 }
 
 __global__ void __launch_bounds__(256) k_vof_kappa(VofArgs a, Geo g) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int i = blockIdx.y + 1;
+    const int2 rc_ = xcd_rowgrid();
+    const int j = rc_.x * blockDim.x + threadIdx.x + 1;
+    const int i = rc_.y + 1;
     if (j > g.ny) return;
     if (a.st->done) return;
     double nx11, ny11, nx01, ny01, nx10, ny10, nx00, ny00;
@@ -193,8 +194,9 @@ __global__ void __launch_bounds__(256) k_vof_nsub(VofArgs a, Geo g) {
 // one sub-step; skipped beyond the sub-step count (the host issues a fixed
 // number of launches)
 __global__ void __launch_bounds__(256) k_vof_advect(VofArgs a, Geo g) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
+    const int2 rc_ = xcd_rowgrid();
+    const int j = rc_.x * blockDim.x + threadIdx.x;
+    const int i = rc_.y;
     const int nx = g.nx, ny = g.ny, P = g.pc;
     if (j > ny + 1) return;
     if (a.st->done) return;
@@ -280,8 +282,9 @@ __global__ void __launch_bounds__(256) k_vof_ghosts(VofArgs a, Geo g) {
 // solid cells: mean of the fluid neighbours (0 without any); then rho and mu
 // of every cell from the final alpha (vof.py:194-210)
 __global__ void __launch_bounds__(256) k_vof_solids(VofArgs a, Geo g) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
+    const int2 rc_ = xcd_rowgrid();
+    const int j = rc_.x * blockDim.x + threadIdx.x;
+    const int i = rc_.y;
     if (j > g.ny + 1) return;
     if (a.st->done || (a.sub >= 0 && (double)a.sub >= a.st->vof_nsub)) return;
     const int P = g.pc;
