@@ -46,7 +46,14 @@ class LearnedConfig:
     ``cfd2d_amd/weights``; ``level`` None picks the level automatically.
     ``coarse_fused``: on a 32 x 32 level K, the levels of at most 64 x 64
     cells, the network and the level-K correction run in one work-group
-    launch (k_lv_coarse) instead of ~25 launches.
+    launch (k_lv_coarse) instead of ~25 launches; in mode "vcycle" the levels
+    of at most 64 x 64 cells down to the dense solve run in one LDS-resident
+    work-group (k_lv_vcoarse).  ``fused_legs`` (with ``coarse_fused``): the
+    levels above that coarse end take one launch per level and leg (tiles
+    staged in LDS) instead of two; ``fused_fine_up`` False keeps the fine
+    level's up-leg as two launches, whose dot partials (row bands) make the
+    solve bit-identical to the separate launches (tiles add them in another
+    order).
     ``x0``: "previous" starts each solve from the last solution (SciPy's
     warm start in the reference), "extrapolate" from the linear extrapolation
     of the last two solutions in time."""
@@ -60,6 +67,8 @@ class LearnedConfig:
     fallback: bool = True
     use_graph: bool = True
     coarse_fused: bool = True
+    fused_legs: bool = True
+    fused_fine_up: bool = True
     x0: str = "previous"
     harvest_cap: int = 0
     weights: str | None = None
@@ -150,7 +159,8 @@ class Attached:
         d.max_iter = int(cfg.max_iter)
         d.fallback = 1 if cfg.fallback else 0
         d.use_graph = 1 if cfg.use_graph else 0
-        d.coarse_fused = 1 if cfg.coarse_fused else 0
+        d.coarse_fused = ((1 | (2 if cfg.fused_legs else 0) | (0 if cfg.fused_fine_up else 4))
+                          if cfg.coarse_fused else 0)
         if cfg.x0 not in ("extrapolate", "previous"):
             raise ValueError(f"x0 must be 'extrapolate' or 'previous', got {cfg.x0!r}")
         # either setting asks for the extrapolated start (the solver's x0 or

@@ -154,6 +154,8 @@ struct cfd_ctx {
     int l_level = 0, l_mode = 0, l_cycles = 0, l_fallback = 1, l_graph = 1, l_maxiter = 200;
     int l_fused = 0, l_lc = 0;   // coarse levels l_lc .. l_level in one k_lv_coarse launch
     int l_vlds = 0;              // mode V-cycle: levels l_lc .. coarsest in LDS (k_lv_vcoarse)
+    int l_legs = 0;              // levels 0 .. l_lc - 1: one launch per level and leg (2: not
+                                 // the fine up-leg)
     VcArgs l_vargs;
     int l_x0 = 0, l_hist = 0;    // extrapolated initial iterate; x_prev holds the last solution
     double* xprev = nullptr;
@@ -715,6 +717,41 @@ LvArgs lv_args(cfd_ctx* c) {
     return a;
 }
 
+LegArgs leg_args(cfd_ctx* c, int l) {
+    const MGDev& mg = c->mg_host;
+    LegArgs g;
+    g.L = mg.lev[l];
+    g.C = mg.lev[l + 1];
+    // the down-leg's t in x, except on level 0 when its up-leg runs as
+    // k_lv_post + k_lv_smooth_dots (which read t)
+    g.tdown = (c->l_legs == 2 && l == 0) ? g.L.t : g.L.x;
+    g.tup = g.tdown;
+    g.code = c->code;
+    g.diag0 = c->diag0f;
+    g.done = &c->fcg->done;
+    g.first = &c->fcg->first;
+    g.r = c->r;
+    g.rold = c->rold;
+    g.part = c->l_part;
+    g.w = mg.omega;
+    g.wc = mg.wc;
+    g.ax = (float)(1.0 / (c->dx * c->dx));
+    g.ay = (float)(1.0 / (c->dy * c->dy));
+    g.periodic = c->periodic;
+    return g;
+}
+
+long leg_tiles_of(const MGLevelDev& L, int ti, int tj) {
+    return (long)((L.nx + ti - 1) / ti) * ((L.ny + tj - 1) / tj);
+}
+
+// 16 x 64 tiles when they give at least 512 work-groups
+bool leg_big(const MGLevelDev& L) { return leg_tiles_of(L, 16, 64) >= 512; }
+
+unsigned leg_grid(const MGLevelDev& L, bool big, long cap) {
+    return (unsigned)std::min(leg_tiles_of(L, big ? 16 : 8, big ? 64 : 32), cap);
+}
+
 FcgVec fcg_vec(cfd_ctx* c) {
     FcgVec v;
     v.x = c->x;
@@ -774,22 +811,75 @@ int learned_iteration(cfd_ctx* c) {
         LV_LAUNCH512(k_lv_smooth_dots, nb0, a, v);
         return 0;
     };
-    if (c->l_vlds) {   // the complete V-cycle with levels >= l_lc in one LDS work-group
-        const int Lc = c->l_lc;
-        for (int l = 0; l < Lc; ++l) TRY(down(l));
-        hipLaunchKernelGGL(k_lv_vcoarse, dim3(1), dim3(1024), 0, c->stream, c->l_vargs);
-        HIPC(hipGetLastError());
-        for (int l = Lc - 1; l >= 1; --l) TRY(up(l));
-        TRY(up0());
-        LV_LAUNCH512(k_fcg_dir, nb0, a, v, nb0);
+    // the flexible-CG steps after z: direction (adding nb_dots partials of r.z
+    // and r_old.z), update, recursive-residual test and true residual, end of
+    // the iteration
+    auto tail = [&](int nb_dots, int net_used) -> int {
+        LV_LAUNCH512(k_fcg_dir, nb0, a, v, nb_dots);
         LV_LAUNCH512(k_fcg_update, nb0, a, v, nb0);
         LV_LAUNCH512(k_fcg_true, nb0, a, v, nb0);
-        LV_LAUNCH(k_fcg_true_fin, 1, a, nb0, 0);
+        LV_LAUNCH(k_fcg_true_fin, 1, a, nb0, net_used);
         return 0;
+    };
+    // the legs of levels 0 .. Lc - 1 as one launch per level and leg (tiles
+    // of 16 x 64 cells where they give at least 512 work-groups, else 8 x 32);
+    // returns the work-group count of the level-0 up-leg (its dot partials)
+    auto legs_down = [&](int Lc) -> int {
+        for (int l = 0; l < Lc; ++l) {
+            const LegArgs g = leg_args(c, l);
+            const bool big = leg_big(g.L);
+            const unsigned nb = leg_grid(g.L, big, 2048);
+            void (*kern)(LegArgs) = l == 0 ? (big ? k_lv_down<true, 16, 64> : k_lv_down<true, 8, 32>)
+                                           : (big ? k_lv_down<false, 16, 64> : k_lv_down<false, 8, 32>);
+            LV_LAUNCH(kern, nb, g);
+        }
+        return 0;
+    };
+    auto legs_up = [&](int Lc, int* nb_dots) -> int {
+        if (c->l_legs == 2) {   // the fine level's up-leg as two launches
+            for (int l = Lc - 1; l >= 1; --l) {
+                const LegArgs g = leg_args(c, l);
+                const bool big = leg_big(g.L);
+                void (*kern)(LegArgs) = big ? k_lv_up<false, false, 16, 64> : k_lv_up<false, false, 8, 32>;
+                LV_LAUNCH(kern, leg_grid(g.L, big, 2048), g);
+            }
+            TRY(up0());
+            *nb_dots = nb0;
+            return 0;
+        }
+        for (int l = Lc - 1; l >= 0; --l) {
+            const LegArgs g = leg_args(c, l);
+            const bool big = leg_big(g.L);
+            const unsigned nb = leg_grid(g.L, big, l == 0 ? LV_MAXB : 2048);
+            if (l == 0) *nb_dots = (int)nb;
+            void (*kern)(LegArgs) =
+                l == 0 ? (big ? k_lv_up<true, true, 16, 64> : k_lv_up<true, true, 8, 32>)
+                       : (big ? k_lv_up<false, false, 16, 64> : k_lv_up<false, false, 8, 32>);
+            LV_LAUNCH(kern, nb, g);
+        }
+        return 0;
+    };
+    if (c->l_vlds) {   // the complete V-cycle with levels >= l_lc in one LDS work-group
+        const int Lc = c->l_lc;
+        if (c->l_legs) TRY(legs_down(Lc));
+        else
+            for (int l = 0; l < Lc; ++l) TRY(down(l));
+        hipLaunchKernelGGL(k_lv_vcoarse, dim3(1), dim3(1024), 0, c->stream, c->l_vargs);
+        HIPC(hipGetLastError());
+        if (c->l_legs) {
+            int nbd = 0;
+            TRY(legs_up(Lc, &nbd));
+            return tail(nbd, 0);
+        }
+        for (int l = Lc - 1; l >= 1; --l) TRY(up(l));
+        TRY(up0());
+        return tail(nb0, 0);
     }
     if (c->l_fused) {
         const int Lc = c->l_lc;
-        for (int l = 0; l < Lc; ++l) TRY(down(l));
+        if (c->l_legs) TRY(legs_down(Lc));
+        else
+            for (int l = 0; l < Lc; ++l) TRY(down(l));
         CoarseNet cn;
         std::memset(&cn, 0, sizeof(cn));
         if (c->l_mode == LEARNED_NET) {
@@ -803,13 +893,14 @@ int learned_iteration(cfd_ctx* c) {
         cn.one_a_one = c->l_1a1;
         hipLaunchKernelGGL(k_lv_coarse, dim3(1), dim3(1024), 0, c->stream, a, Lc, K, cn);
         HIPC(hipGetLastError());
+        if (c->l_legs) {
+            int nbd = 0;
+            TRY(legs_up(Lc, &nbd));
+            return tail(nbd, c->l_mode == LEARNED_NET ? 1 : 0);
+        }
         for (int l = Lc - 1; l >= 1; --l) TRY(up(l));
         TRY(up0());
-        LV_LAUNCH512(k_fcg_dir, nb0, a, v, nb0);
-        LV_LAUNCH512(k_fcg_update, nb0, a, v, nb0);
-        LV_LAUNCH512(k_fcg_true, nb0, a, v, nb0);
-        LV_LAUNCH(k_fcg_true_fin, 1, a, nb0, c->l_mode == LEARNED_NET ? 1 : 0);
-        return 0;
+        return tail(nb0, c->l_mode == LEARNED_NET ? 1 : 0);
     }
     if (c->l_mode == LEARNED_VCYCLE) {   // the complete V-cycle, dense coarsest solve
         for (int l = 0; l < NL - 1; ++l) TRY(down(l));
@@ -817,11 +908,7 @@ int learned_iteration(cfd_ctx* c) {
         for (int l = NL - 2; l >= 1; --l) TRY(up(l));
         if (NL >= 2) TRY(up0());
         else LV_LAUNCH512(k_fcg_dots, nb0, a, v);
-        LV_LAUNCH512(k_fcg_dir, nb0, a, v, nb0);
-        LV_LAUNCH512(k_fcg_update, nb0, a, v, nb0);
-        LV_LAUNCH512(k_fcg_true, nb0, a, v, nb0);
-        LV_LAUNCH(k_fcg_true_fin, 1, a, nb0, 0);
-        return 0;
+        return tail(nb0, 0);
     }
     for (int l = 0; l < K; ++l) TRY(down(l));
     LV_LAUNCH(k_lv_cmode, nbK, a, K);
@@ -869,11 +956,7 @@ int learned_iteration(cfd_ctx* c) {
     for (int l = K - 1; l >= 1; --l) TRY(up(l));
     if (K >= 1) TRY(up0());
     else LV_LAUNCH512(k_fcg_dots, nb0, a, v);
-    LV_LAUNCH512(k_fcg_dir, nb0, a, v, nb0);
-    LV_LAUNCH512(k_fcg_update, nb0, a, v, nb0);
-    LV_LAUNCH512(k_fcg_true, nb0, a, v, nb0);
-    LV_LAUNCH(k_fcg_true_fin, 1, a, nb0, c->l_mode == LEARNED_NET ? 1 : 0);
-    return 0;
+    return tail(nb0, c->l_mode == LEARNED_NET ? 1 : 0);
 }
 
 void learned_drop_graph(cfd_ctx* c) {
@@ -1374,6 +1457,7 @@ int cfd_set_learned(cfd_ctx* c, const cfd_learned_desc* d) {
         c->l_lc = lc;
         c->l_fused = 1;
     }
+    c->l_legs = ((d->coarse_fused & 2) && (c->l_vlds || c->l_fused)) ? ((d->coarse_fused & 4) ? 2 : 1) : 0;
     c->l_H = H;
     c->l_W = W;
     c->l_net = d->mode == LEARNED_NET ? d->net : nullptr;

@@ -543,6 +543,262 @@ __global__ void __launch_bounds__(256) k_lv_harvest(LvArgs a, int l, float* hin,
     hout[base + k] = vout;
 }
 
+// ---- the legs of the V-cycle above the coarse end, one launch per level and leg
+// A work-group takes tiles of TI x TJ cells of level l (TI, TJ even, so a
+// tile holds whole 2 x 2 aggregates) and stages them in LDS with a ring of two
+// cells (rows wrapped when periodic; cells outside the arrays count as
+// solid), so that the sweeps of a leg run back to back with barriers instead
+// of launches:
+//   down-leg (k_lv_down): x = w b / d (the first sweep from zero, as the
+//     producers of b store it), t = x + w (b - A x) / d, and the residual
+//     b - A t summed over each aggregate into b of level l + 1 (x = w b / d
+//     there);
+//   up-leg (k_lv_up): y = t + wc t_(l+1) (the prolongation), x = y + w (b -
+//     A y) / d, t = x + w (b - A x) / d; on level 0 with the partial dot
+//     products r.z and r_old.z of the flexible CG.
+// The values are those of k_lv_smooth / k_lv_restrict / k_lv_post /
+// k_lv_smooth_dots, expression for expression: a ring cell is recomputed from
+// the same inputs by the same FP32 arithmetic.  The down-leg's t goes to
+// `tdown` (the level's x array: no later kernel reads x there, k_lv_coarse and
+// the next down-leg start from b), where the up-leg reads it back: the up-leg
+// writes t, which the work-groups of neighbouring tiles read in their rings,
+// so it must not be the array it reads.
+struct LegArgs {
+    MGLevelDev L, C;          // level l and level l + 1
+    float* tdown;             // down-leg: t of level l goes here
+    const float* tup;         // up-leg: the down-leg's t of level l
+    const uint8_t* code;      // level 0: cell codes (pitch L.P)
+    const float* diag0;       // level 0: diagonal table (64 floats)
+    const int* done;          // the solve has finished
+    const int* first;         // the flexible CG (re)starts: no r_old.z
+    const double *r, *rold;   // level-0 up-leg: FP64 residuals (pitch L.P)
+    double* part;             // level-0 up-leg: dot partials, 2 per work-group
+    float w, wc, ax, ay;
+    int periodic;
+};
+
+namespace {
+
+// row of a ring cell: wrapped into 1..nx when periodic (mg_apply's iE / iW);
+// otherwise unchanged (0 and nx + 1 are the ghost rows)
+__device__ __forceinline__ int leg_row(int i, int nx, int periodic) {
+    if (!periodic) return i;
+    int r = (i - 1) % nx;
+    if (r < 0) r += nx;
+    return r + 1;
+}
+
+// (A v) at LDS cell k of a tile of row width W: mg_apply's expressions
+template <bool LEV0>
+__device__ __forceinline__ float leg_apply(const LegArgs& a, int k, int W, float d, const float* v,
+                                           const float* e, const float* n) {
+    if (LEV0) return d * v[k] - a.ax * (v[k + W] + v[k - W]) - a.ay * (v[k + 1] + v[k - 1]);
+    return d * v[k] - e[k] * v[k + W] - e[k - W] * v[k - W] - n[k] * v[k + 1] - n[k - 1] * v[k - 1];
+}
+
+// this work-group's tiles [t0, t1) of level L: contiguous runs of tiles
+// (row-major over the level) per XCD
+template <int TI, int TJ>
+__device__ __forceinline__ void leg_tiles(const MGLevelDev& L, int& ntj, int& t0, int& t1) {
+    ntj = (L.ny + TJ - 1) / TJ;
+    const long nt = (long)((L.nx + TI - 1) / TI) * ntj;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    t0 = (int)((nt * b) / gridDim.x);
+    t1 = (int)((nt * (b + 1)) / gridDim.x);
+}
+
+}  // namespace
+
+// The staging of a tile's LDS region (the tile and its two-cell ring), 256
+// threads: every global load of the thread's cells is issued before any is
+// used.  Per cell: b, the diagonal (level 0: from the code), the couplings
+// (levels >= 1) and, for the up-leg, the prolongated correction y = t + wc
+// t_(l+1) (k_lv_post's expression; 0 off the fluid).  Cells outside the arrays
+// count as solid.
+template <bool LEV0, bool UP, int TI, int TJ>
+__device__ __forceinline__ void leg_stage(const LegArgs& a, int ib, int jb, const float* s_d0,
+                                          float* sb, float* sd, float* se, float* sn, float* sv) {
+    constexpr int W = TJ + 4, N = (TI + 4) * W, LA = (N + 255) / 256;
+    const MGLevelDev& L = a.L;
+    const MGLevelDev& C = a.C;
+    float rb[LA], rd[LA], re[LA], rn[LA], rt[LA], rc[LA];
+    int rcode[LA];
+    bool ok[LA];
+#pragma unroll
+    for (int u = 0; u < LA; ++u) {
+        const int e = (int)threadIdx.x + u * 256;
+        const int r = e / W, q = e - r * W;
+        const int i = leg_row(ib + r, L.nx, a.periodic), j = jb + q;
+        ok[u] = e < N && i >= 0 && i <= L.nx + 1 && j >= 0 && j <= L.ny + 1;
+        const long c = ok[u] ? ix(i, j, L.P) : 0;
+        rb[u] = ok[u] ? L.b[c] : 0.0f;
+        if (LEV0) {
+            rcode[u] = ok[u] ? a.code[c] : 0;
+        } else {
+            rd[u] = ok[u] ? L.diag[c] : 0.0f;
+            re[u] = ok[u] ? L.cE[c] : 0.0f;
+            rn[u] = ok[u] ? L.cN[c] : 0.0f;
+        }
+        if (UP) {
+            rt[u] = ok[u] ? a.tup[c] : 0.0f;
+            rc[u] = ok[u] ? C.t[ix((i + 1) >> 1, (j + 1) >> 1, C.P)] : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < LA; ++u) {
+        const int e = (int)threadIdx.x + u * 256;
+        if (e >= N) continue;
+        const float d = LEV0 ? s_d0[rcode[u] & 63] : rd[u];
+        sb[e] = rb[u];
+        sd[e] = d;
+        if (!LEV0) {
+            se[e] = re[u];
+            sn[e] = rn[u];
+        }
+        if (UP) {
+            const float v = rt[u] + a.wc * rc[u];
+            sv[e] = d == 0.0f ? 0.0f : v;
+        } else {
+            sv[e] = d != 0.0f ? a.w * rb[u] / d : 0.0f;
+        }
+    }
+}
+
+// dst = src + w (b - A src) / d on the tile and its first ring (0 off the fluid)
+template <bool LEV0, int TI, int TJ>
+__device__ __forceinline__ void leg_sweep_ring(const LegArgs& a, const float* sb, const float* sd,
+                                               const float* se, const float* sn, const float* src,
+                                               float* dst) {
+    constexpr int W = TJ + 4, W1 = TJ + 2, N1 = (TI + 2) * W1, LS = (N1 + 255) / 256;
+#pragma unroll
+    for (int u = 0; u < LS; ++u) {
+        const int e = (int)threadIdx.x + u * 256;
+        if (e >= N1) continue;
+        const int r = e / W1 + 1, q = e - (r - 1) * W1 + 1;
+        const int k = r * W + q;
+        const float d = sd[k];
+        dst[k] = d != 0.0f ? src[k] + a.w * (sb[k] - leg_apply<LEV0>(a, k, W, d, src, se, sn)) / d
+                           : 0.0f;
+    }
+}
+
+template <bool LEV0, int TI, int TJ>
+__global__ void __launch_bounds__(256) k_lv_down(LegArgs a) {
+    if (*a.done) return;
+    constexpr int W = TJ + 4, N = (TI + 4) * W;
+    __shared__ float sb[N], sd[N], sx[N], st[N];
+    __shared__ float se[LEV0 ? 1 : N], sn[LEV0 ? 1 : N];
+    __shared__ float s_d0[64];
+    if (LEV0) {
+        if (threadIdx.x < 64) s_d0[threadIdx.x] = a.diag0[threadIdx.x];
+        __syncthreads();
+    }
+    const MGLevelDev& L = a.L;
+    const MGLevelDev& C = a.C;
+    const float w = a.w;
+    int ntj, t0, t1;
+    leg_tiles<TI, TJ>(L, ntj, t0, t1);
+    for (int tile = t0; tile < t1; ++tile) {
+        const int ib = (tile / ntj) * TI - 1;   // level row of LDS row 0
+        const int jb = (tile % ntj) * TJ - 1;
+        leg_stage<LEV0, false, TI, TJ>(a, ib, jb, s_d0, sb, sd, se, sn, sx);   // x = w b / d
+        __syncthreads();
+        leg_sweep_ring<LEV0, TI, TJ>(a, sb, sd, se, sn, sx, st);
+        __syncthreads();
+        // the residual restricted: one aggregate of level l + 1 per thread
+        const int ci0 = (tile / ntj) * (TI / 2) + 1, cj0 = (tile % ntj) * (TJ / 2) + 1;
+        constexpr int NC = (TI / 2) * (TJ / 2), LC = (NC + 255) / 256;
+#pragma unroll
+        for (int u = 0; u < LC; ++u) {
+            const int e = (int)threadIdx.x + u * 256;
+            const int cr = e / (TJ / 2), cs = e - cr * (TJ / 2);
+            const int ci = ci0 + cr, cj = cj0 + cs;
+            if (e >= NC || ci > C.nx || cj > C.ny) continue;
+            float bc = 0.0f;
+            for (int di = 0; di < 2; ++di)
+                for (int dj = 0; dj < 2; ++dj) {
+                    const int i = 2 * ci - 1 + di, j = 2 * cj - 1 + dj;
+                    if (i > L.nx || j > L.ny) continue;
+                    const int k = (2 + 2 * cr + di) * W + 2 + 2 * cs + dj;
+                    const float d = sd[k];
+                    if (d == 0.0f) continue;
+                    bc += sb[k] - leg_apply<LEV0>(a, k, W, d, st, se, sn);
+                }
+            const long cc = ix(ci, cj, C.P);
+            C.b[cc] = bc;
+            const float dc = C.diag[cc];
+            C.x[cc] = dc != 0.0f ? w * bc / dc : 0.0f;
+        }
+        constexpr int NI = TI * TJ, LI = (NI + 255) / 256;
+#pragma unroll
+        for (int u = 0; u < LI; ++u) {
+            const int e = (int)threadIdx.x + u * 256;
+            const int r = e / TJ, q = e - r * TJ;
+            const int i = ib + 2 + r, j = jb + 2 + q;
+            const int k = (r + 2) * W + q + 2;
+            if (e < NI && i <= L.nx && j <= L.ny && sd[k] != 0.0f) a.tdown[ix(i, j, L.P)] = st[k];
+        }
+        __syncthreads();   // the next tile overwrites the LDS arrays
+    }
+}
+
+template <bool LEV0, bool DOTS, int TI, int TJ>
+__global__ void __launch_bounds__(256) k_lv_up(LegArgs a) {
+    if (*a.done) return;
+    constexpr int W = TJ + 4, N = (TI + 4) * W;
+    __shared__ float sb[N], sd[N], sy[N], sx[N];
+    __shared__ float se[LEV0 ? 1 : N], sn[LEV0 ? 1 : N];
+    __shared__ float s_d0[64];
+    if (LEV0) {
+        if (threadIdx.x < 64) s_d0[threadIdx.x] = a.diag0[threadIdx.x];
+        __syncthreads();
+    }
+    const MGLevelDev& L = a.L;
+    const float w = a.w;
+    const bool first = DOTS ? *a.first != 0 : true;
+    double s2[2] = {0.0, 0.0};
+    int ntj, t0, t1;
+    leg_tiles<TI, TJ>(L, ntj, t0, t1);
+    for (int tile = t0; tile < t1; ++tile) {
+        const int ib = (tile / ntj) * TI - 1;
+        const int jb = (tile % ntj) * TJ - 1;
+        leg_stage<LEV0, true, TI, TJ>(a, ib, jb, s_d0, sb, sd, se, sn, sy);   // y
+        __syncthreads();
+        leg_sweep_ring<LEV0, TI, TJ>(a, sb, sd, se, sn, sy, sx);
+        __syncthreads();
+        constexpr int NI = TI * TJ, LI = (NI + 255) / 256;
+        double rr[LI], ro[LI];
+#pragma unroll
+        for (int u = 0; u < LI; ++u) {   // the residuals' loads first
+            const int e = (int)threadIdx.x + u * 256;
+            const int r = e / TJ, q = e - r * TJ;
+            const int i = ib + 2 + r, j = jb + 2 + q;
+            const bool in = e < NI && i <= L.nx && j <= L.ny;
+            const long c = in ? ix(i, j, L.P) : 0;
+            rr[u] = DOTS && in ? a.r[c] : 0.0;
+            ro[u] = DOTS && in && !first ? a.rold[c] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < LI; ++u) {
+            const int e = (int)threadIdx.x + u * 256;
+            const int r = e / TJ, q = e - r * TJ;
+            const int i = ib + 2 + r, j = jb + 2 + q;
+            const int k = (r + 2) * W + q + 2;
+            if (e >= NI || i > L.nx || j > L.ny) continue;
+            const float d = sd[k];
+            if (d == 0.0f) continue;
+            const float z = sx[k] + w * (sb[k] - leg_apply<LEV0>(a, k, W, d, sx, se, sn)) / d;
+            L.t[ix(i, j, L.P)] = z;
+            if (DOTS) {
+                s2[0] += rr[u] * (double)z;
+                if (!first) s2[1] += ro[u] * (double)z;
+            }
+        }
+        __syncthreads();
+    }
+    if (DOTS) lv_partial<2>(s2, a.part + LV_P_DOTS * LV_SLOT);
+}
+
 // ---- flexible CG steps on the fine level (z = lev[0].t) ----
 // beta = (r.z - r_old.z) / rho_old (0 at a (re)start) from the dot partials of
 // nb work-groups; d = z + beta d_old, q = A d (neighbours' d rebuilt from z and

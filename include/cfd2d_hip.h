@@ -303,9 +303,15 @@ typedef struct cfd_learned_desc {
     float* harvest_in;        /* device, harvest_cap x H x W floats (mode 2) */
     float* harvest_out;
     int64_t harvest_cap;
-    int32_t coarse_fused;     /* 1: levels with <= 64 x 64 cells, the network and the
+    int32_t coarse_fused;     /* bit 0: levels with <= 64 x 64 cells, the network and the
                                  level-K correction in one work-group launch (modes 0, 1
-                                 and 3 on a 32 x 32 level K; ignored otherwise) */
+                                 on a 32 x 32 level K; mode 3: those levels down to the
+                                 dense solve; ignored otherwise); bit 1 (with bit 0): the
+                                 levels above that coarse end in one launch per level
+                                 and leg (the same FP32 arithmetic; the partial dot
+                                 products of the fine up-leg in tile order); bit 2 (with
+                                 bit 1): the fine level's up-leg still as two launches
+                                 (row-band partials: bit-identical to bit 1 clear) */
     int32_t x0_extrapolate;   /* 1: start each solve of a time step from the linear
                                  extrapolation x + (dt/dt_old)(x - x_previous) of the last
                                  two solutions; 0: from the last solution (SciPy's x0) */

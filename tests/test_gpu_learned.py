@@ -253,7 +253,8 @@ def test_fused_coarse_end_matches_launch_sequence(gpu, mode):
     and the network's accumulation order differ in rounding.)"""
     from cfd2d_amd.learned import LearnedConfig
 
-    a = _run(_cavity(256, "cg_learned", LearnedConfig(mode=mode, coarse_fused=True)), 20)
+    a = _run(_cavity(256, "cg_learned", LearnedConfig(mode=mode, coarse_fused=True,
+                                                      fused_legs=False)), 20)
     b = _run(_cavity(256, "cg_learned", LearnedConfig(mode=mode, coarse_fused=False)), 20)
     ia, ib = np.mean(a.cg_iterations), np.mean(b.cg_iterations)
     print(f"{mode}: fused {ia:.2f} iterations, separate launches {ib:.2f}; "
@@ -282,10 +283,67 @@ def test_vcycle_fused_equals_launches(gpu):
     bit-identical fields."""
     from cfd2d_amd.learned import LearnedConfig
 
-    a = _run(_cavity(256, "cg_learned", LearnedConfig(mode="vcycle", coarse_fused=True)), 10)
+    a = _run(_cavity(256, "cg_learned", LearnedConfig(mode="vcycle", coarse_fused=True,
+                                                      fused_legs=False)), 10)
     b = _run(_cavity(256, "cg_learned", LearnedConfig(mode="vcycle", coarse_fused=False)), 10)
     assert a.cg_iterations == b.cg_iterations
     assert np.array_equal(a.u, b.u) and np.array_equal(a.p, b.p)
+
+
+def _legs_case(kind, learned):
+    from cfd2d_amd import cases
+
+    kw = dict(pressure_solver="cg_learned", pressure_tol=1e-6, learned=learned)
+    if kind == "cavity":        # 256^2: 8 x 32 tiles
+        return cases.lid_driven_cavity(re=1000, n=256, **kw).sim
+    if kind == "cavity2048":    # 16 x 64 tiles on levels 0, 1; several tiles per work-group
+        return cases.lid_driven_cavity(re=1000, n=2048, **kw).sim
+    if kind == "periodic":      # 258 x 64, periodic in x: wrapped rows, odd level 1 (129 x 32)
+        return cases.rayleigh_benard(ra=1e5, ny=64, **kw).sim
+    return cases.cylinder_flow(re=100, nx=220, ny=41, **kw).sim   # partial tiles, obstacle
+
+
+@pytest.mark.parametrize("mode", ["net", "none", "vcycle"])
+def test_fused_legs_are_the_launch_sequence(gpu, mode):
+    """The legs above the coarse end as one launch per level and leg
+    (k_lv_down / k_lv_up: tiles with a two-cell ring in LDS) compute the
+    separate sweep, restriction and prolongation launches' FP32 values: with
+    the fine level's up-leg (and so the dot partials) left as launches, the
+    solve is bit-identical."""
+    from cfd2d_amd.learned import LearnedConfig
+
+    a = _run(_cavity(256, "cg_learned", LearnedConfig(mode=mode, fused_fine_up=False)), 12)
+    b = _run(_cavity(256, "cg_learned", LearnedConfig(mode=mode, fused_legs=False)), 12)
+    assert a.cg_iterations == b.cg_iterations
+    assert np.array_equal(a.u, b.u) and np.array_equal(a.p, b.p)
+
+
+@pytest.mark.parametrize("kind", ["cavity", "cavity2048", "periodic", "cylinder"])
+def test_vcycle_fused_legs_match_launches(gpu, kind):
+    """The same on other grids (periodic rows wrapped in the tiles' rings, odd
+    level sizes, partial tiles, an obstacle): bit-identical with the fine
+    up-leg as launches; with it fused as well, the partial sums of r.z and
+    r_old.z come in tile order instead of row bands, so the iteration counts
+    agree and the fields within the solve tolerance (each solve stops at a
+    relative residual of 1e-6, wherever the rounding of the dot products
+    leaves it below that)."""
+    from cfd2d_amd.learned import LearnedConfig
+
+    steps = 4 if kind == "cavity2048" else 12
+    cfg = dict(mode="vcycle", x0="extrapolate")
+    a = _run(_legs_case(kind, LearnedConfig(**cfg)), steps)
+    e = _run(_legs_case(kind, LearnedConfig(**cfg, fused_fine_up=False)), steps)
+    b = _run(_legs_case(kind, LearnedConfig(**cfg, fused_legs=False)), steps)
+    print(f"{kind}: fused legs {a.cg_iterations}, separate launches {b.cg_iterations}; "
+          f"max|du| {_cmp(a.u, b.u):.1e} max|dp| {_cmp(a.p, b.p):.1e}")
+    assert e.cg_iterations == b.cg_iterations
+    assert np.array_equal(e.u, b.u) and np.array_equal(e.p, b.p)
+    assert max(a.cg_relres) <= 1e-6 and all(i == 0 for i in a.cg_info)
+    assert a.pressure.learned.stats()["fallbacks"] == 0
+    assert abs(sum(a.cg_iterations) - sum(b.cg_iterations)) <= 1
+    scale = float(np.abs(b.p).max()) + 1e-30
+    assert _cmp(a.u, b.u) < 1e-7 and _cmp(a.v, b.v) < 1e-7
+    assert _cmp(a.p, b.p) < 1e-6 * scale
 
 
 @pytest.mark.parametrize("n,re", [(2048, 1000.0), (1024, 5000.0)])
