@@ -78,7 +78,7 @@ class LearnedDesc(ctypes.Structure):
                 ("max_iter", _i32), ("fallback", _i32), ("use_graph", _i32),
                 ("sigma", ctypes.c_float), ("one_a_one", _d), ("harvest_in", _vp),
                 ("harvest_out", _vp), ("harvest_cap", _i64), ("coarse_fused", _i32),
-                ("x0_extrapolate", _i32)]
+                ("x0_extrapolate", _i32), ("sweeps", _i32)]
 
 
 class ProbeDesc(ctypes.Structure):
@@ -101,7 +101,7 @@ class VofDesc(ctypes.Structure):
 class LearnedStats(ctypes.Structure):
     _fields_ = [("harvested", _i64), ("solves", _i64), ("iterations", _i64), ("fallbacks", _i64),
                 ("last_iters", _i32), ("last_info", _i32), ("last_relres", _d),
-                ("net_off", _i32), ("pad", _i32)]
+                ("net_off", _i32), ("last_sweeps", _i32)]
 
 
 ENS_MAX_CLASSES = 8

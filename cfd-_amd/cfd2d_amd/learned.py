@@ -53,7 +53,12 @@ class LearnedConfig:
     staged in LDS) instead of two; ``fused_fine_up`` False keeps the fine
     level's up-leg as two launches, whose dot partials (row bands) make the
     solve bit-identical to the separate launches (tiles add them in another
-    order).
+    order).  ``sweeps``: damped-Jacobi sweeps per leg on those levels (2: the
+    multigrid solver's V-cycle; 3 or 4: a stronger preconditioner, with the
+    fused legs; "auto": 3 for a solve after one that took at least 4
+    iterations, back to 2 after 8 consecutive 3-sweep solves of at most 2 --
+    the extra sweep costs about 15 % of an iteration and pays where solves
+    take many).
     ``x0``: "previous" starts each solve from the last solution (SciPy's
     warm start in the reference), "extrapolate" from the linear extrapolation
     of the last two solutions in time."""
@@ -69,6 +74,7 @@ class LearnedConfig:
     coarse_fused: bool = True
     fused_legs: bool = True
     fused_fine_up: bool = True
+    sweeps: int | str = "auto"
     x0: str = "previous"
     harvest_cap: int = 0
     weights: str | None = None
@@ -166,6 +172,12 @@ class Attached:
         # either setting asks for the extrapolated start (the solver's x0 or
         # the pressure solver's, which applies to every method)
         d.x0_extrapolate = 1 if "extrapolate" in (cfg.x0, x0) else 0
+        if cfg.sweeps == "auto":
+            d.sweeps = -1
+        elif cfg.sweeps in (2, 3, 4):
+            d.sweeps = int(cfg.sweeps)
+        else:
+            raise ValueError(f"sweeps must be 2, 3, 4 or 'auto', got {cfg.sweeps!r}")
         d.sigma = self.sigma
         d.one_a_one = one_a_one(hierarchy, self.level)
         d.harvest_in = self.hin.ptr if self.hin else None

@@ -156,6 +156,9 @@ struct cfd_ctx {
     int l_vlds = 0;              // mode V-cycle: levels l_lc .. coarsest in LDS (k_lv_vcoarse)
     int l_legs = 0;              // levels 0 .. l_lc - 1: one launch per level and leg (2: not
                                  // the fine up-leg)
+    int l_sweeps = 2;            // smoothing sweeps per leg on those levels (this solve)
+    int l_sweeps_cfg = 2;        // 2, 3, 4, or -1: 2 or 3 by the last solves' iterations
+    int l_low_run = 0;           // consecutive 3-sweep solves of at most 2 iterations
     VcArgs l_vargs;
     int l_x0 = 0, l_hist = 0;    // extrapolated initial iterate; x_prev holds the last solution
     double* xprev = nullptr;
@@ -171,8 +174,8 @@ struct cfd_ctx {
     FcgState* fcg = nullptr;
     FcgState* fcg_host = nullptr;   // pinned mirror
     double* l_part = nullptr;
-    hipGraph_t l_graph_g = nullptr;
-    hipGraphExec_t l_graph_x = nullptr;
+    hipGraph_t l_graph_g[3] = {};    // one captured iteration per sweep count 2, 3, 4
+    hipGraphExec_t l_graph_x[3] = {};
     int l_prev_iters = 0;
     long long l_fallbacks = 0, l_solves = 0, l_iters_total = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -752,6 +755,30 @@ unsigned leg_grid(const MGLevelDev& L, bool big, long cap) {
     return (unsigned)std::min(leg_tiles_of(L, big ? 16 : 8, big ? 64 : 32), cap);
 }
 
+using LegKernel = void (*)(LegArgs);
+
+template <int NU>
+LegKernel leg_down_nu(bool lev0, bool big) {
+    return lev0 ? (big ? k_lv_down<true, NU, 16, 64> : k_lv_down<true, NU, 8, 32>)
+                : (big ? k_lv_down<false, NU, 16, 64> : k_lv_down<false, NU, 8, 32>);
+}
+
+// the fine level's up-leg with the dot partials
+template <int NU>
+LegKernel leg_up_nu(bool lev0, bool big) {
+    return lev0 ? (big ? k_lv_up<true, true, NU, 16, 64> : k_lv_up<true, true, NU, 8, 32>)
+                : (big ? k_lv_up<false, false, NU, 16, 64> : k_lv_up<false, false, NU, 8, 32>);
+}
+
+// sweeps per leg: 2, 3 or 4 (checked by cfd_set_learned)
+LegKernel leg_down_kernel(int nu, bool lev0, bool big) {
+    return nu == 4 ? leg_down_nu<4>(lev0, big) : nu == 3 ? leg_down_nu<3>(lev0, big) : leg_down_nu<2>(lev0, big);
+}
+
+LegKernel leg_up_kernel(int nu, bool lev0, bool big) {
+    return nu == 4 ? leg_up_nu<4>(lev0, big) : nu == 3 ? leg_up_nu<3>(lev0, big) : leg_up_nu<2>(lev0, big);
+}
+
 FcgVec fcg_vec(cfd_ctx* c) {
     FcgVec v;
     v.x = c->x;
@@ -824,14 +851,12 @@ int learned_iteration(cfd_ctx* c) {
     // the legs of levels 0 .. Lc - 1 as one launch per level and leg (tiles
     // of 16 x 64 cells where they give at least 512 work-groups, else 8 x 32);
     // returns the work-group count of the level-0 up-leg (its dot partials)
+    const int nu = c->l_sweeps;
     auto legs_down = [&](int Lc) -> int {
         for (int l = 0; l < Lc; ++l) {
             const LegArgs g = leg_args(c, l);
             const bool big = leg_big(g.L);
-            const unsigned nb = leg_grid(g.L, big, 2048);
-            void (*kern)(LegArgs) = l == 0 ? (big ? k_lv_down<true, 16, 64> : k_lv_down<true, 8, 32>)
-                                           : (big ? k_lv_down<false, 16, 64> : k_lv_down<false, 8, 32>);
-            LV_LAUNCH(kern, nb, g);
+            LV_LAUNCH(leg_down_kernel(nu, l == 0, big), leg_grid(g.L, big, 2048), g);
         }
         return 0;
     };
@@ -840,8 +865,7 @@ int learned_iteration(cfd_ctx* c) {
             for (int l = Lc - 1; l >= 1; --l) {
                 const LegArgs g = leg_args(c, l);
                 const bool big = leg_big(g.L);
-                void (*kern)(LegArgs) = big ? k_lv_up<false, false, 16, 64> : k_lv_up<false, false, 8, 32>;
-                LV_LAUNCH(kern, leg_grid(g.L, big, 2048), g);
+                LV_LAUNCH(leg_up_kernel(2, false, big), leg_grid(g.L, big, 2048), g);
             }
             TRY(up0());
             *nb_dots = nb0;
@@ -852,10 +876,7 @@ int learned_iteration(cfd_ctx* c) {
             const bool big = leg_big(g.L);
             const unsigned nb = leg_grid(g.L, big, l == 0 ? LV_MAXB : 2048);
             if (l == 0) *nb_dots = (int)nb;
-            void (*kern)(LegArgs) =
-                l == 0 ? (big ? k_lv_up<true, true, 16, 64> : k_lv_up<true, true, 8, 32>)
-                       : (big ? k_lv_up<false, false, 16, 64> : k_lv_up<false, false, 8, 32>);
-            LV_LAUNCH(kern, nb, g);
+            LV_LAUNCH(leg_up_kernel(nu, l == 0, big), nb, g);
         }
         return 0;
     };
@@ -960,13 +981,17 @@ int learned_iteration(cfd_ctx* c) {
 }
 
 void learned_drop_graph(cfd_ctx* c) {
-    if (c->l_graph_x) hipGraphExecDestroy(c->l_graph_x);
-    if (c->l_graph_g) hipGraphDestroy(c->l_graph_g);
-    c->l_graph_x = nullptr;
-    c->l_graph_g = nullptr;
+    for (int k = 0; k < 3; ++k) {
+        if (c->l_graph_x[k]) hipGraphExecDestroy(c->l_graph_x[k]);
+        if (c->l_graph_g[k]) hipGraphDestroy(c->l_graph_g[k]);
+        c->l_graph_x[k] = nullptr;
+        c->l_graph_g[k] = nullptr;
+    }
 }
 
+// the iteration with the current sweep count, captured once per count
 int learned_capture(cfd_ctx* c) {
+    const int k = c->l_sweeps - 2;
     HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
     const int rc = learned_iteration(c);
     hipGraph_t g = nullptr;
@@ -976,8 +1001,8 @@ int learned_capture(cfd_ctx* c) {
         return rc;
     }
     HIPC(e);
-    c->l_graph_g = g;
-    HIPC(hipGraphInstantiate(&c->l_graph_x, g, nullptr, nullptr, 0));
+    c->l_graph_g[k] = g;
+    HIPC(hipGraphInstantiate(&c->l_graph_x[k], g, nullptr, nullptr, 0));
     return 0;
 }
 
@@ -1001,22 +1026,41 @@ int learned_end(cfd_ctx* c, const ProjArgs& a) {
     LV_LAUNCH(k_lp_fin, 1, a, c->geo, c->l_part, nb);
     if (a.zero_p_after) {
         const long n = (long)(c->nx + 2) * c->P;
-        LV_LAUNCH(k_lp_pzero, (unsigned)((n + 255) / 256), c->p, n, (const DevState*)c->st);
+        LV_LAUNCH(k_lp_pzero, (unsigned)((n + 255) / 256), a, n);
     }
     return 0;
 }
 
 // A projection with the learned flexible CG: the start launches (right-hand
-// side, initial residual), iterations until the device reports the solve
-// finished (the host polls the state after the predicted number of
-// iterations, then after each one), then the end launches (pressure
-// write-back, correction); a solve that does not converge is continued by the
-// FP64 Jacobi-PCG of k_project when enabled.
+// side, initial residual), iterations, each batch followed by the end launches
+// (pressure write-back, correction), which act once the device reports the
+// solve finished; the host polls the state after each batch: first the
+// iteration count of the last solve, then one iteration at a time.  A solve
+// that does not converge is continued by the FP64 Jacobi-PCG of k_project when
+// enabled (the end launches then leave the fields alone).  With adaptive
+// sweeps, a solve after one of at least 4 iterations smooths with 3 sweeps per
+// leg, and after 8 consecutive 3-sweep solves of at most 2 iterations with 2
+// again.
 int run_learned(cfd_ctx* c, ProjArgs a) {
     const int maxiter_all = a.maxiter;
     a.fcg = c->fcg;
     a.maxiter = std::min(a.maxiter, c->l_maxiter);
-    if (c->l_graph && !c->l_graph_x) TRY(learned_capture(c));
+    if (c->l_sweeps_cfg < 0 && c->l_legs == 1) {
+        // rare switches: consecutive solves with different preconditioners
+        // leave uncorrelated errors in the two solutions the extrapolated
+        // start is formed from
+        c->l_low_run = (c->l_sweeps == 3 && c->l_prev_iters <= 2) ? c->l_low_run + 1 : 0;
+        if (c->l_sweeps == 2 && c->l_prev_iters >= 4) c->l_sweeps = 3;
+        else if (c->l_low_run >= 8) {
+            c->l_sweeps = 2;
+            c->l_low_run = 0;
+        }
+    }
+    hipGraphExec_t gx = nullptr;
+    if (c->l_graph) {
+        if (!c->l_graph_x[c->l_sweeps - 2]) TRY(learned_capture(c));
+        gx = c->l_graph_x[c->l_sweeps - 2];
+    }
     const bool prof = c->prof_on;
     if (prof) {   // the whole projection is timed as one unit
         if (c->prof_used + 2 > c->prof_ev.size()) {
@@ -1030,14 +1074,15 @@ int run_learned(cfd_ctx* c, ProjArgs a) {
     }
     TRY(x0_extrapolate(c, a));
     TRY(learned_start(c, a));
-    int batch = std::max(1, c->l_prev_iters - 1);
+    int batch = std::max(1, c->l_prev_iters);
     int launched = 0;
     for (;;) {
         for (int q = 0; q < batch; ++q) {
-            if (c->l_graph_x) HIPC(hipGraphLaunch(c->l_graph_x, c->stream));
+            if (gx) HIPC(hipGraphLaunch(gx, c->stream));
             else TRY(learned_iteration(c));
         }
         launched += batch;
+        TRY(learned_end(c, a));
         HIPC(hipMemcpyAsync(c->fcg_host, c->fcg, sizeof(FcgState), hipMemcpyDeviceToHost,
                             c->stream));
         HIPC(hipStreamSynchronize(c->stream));
@@ -1056,7 +1101,9 @@ int run_learned(cfd_ctx* c, ProjArgs a) {
         f.method = PROJ_PCG64;
         f.maxiter = maxiter_all;
         TRY(launch_coop(c, f, false));
-    } else {
+    } else if (h.info != 0) {
+        // a failed solve without the fallback: its result as the solver left it
+        a.end_always = 1;
         TRY(learned_end(c, a));
     }
     if (prof) {
@@ -1232,8 +1279,7 @@ int cfd_ctx_destroy(cfd_ctx* c) {
     if (!c) return 0;
     if (c->stream) hipStreamSynchronize(c->stream);
     for (hipEvent_t e : c->prof_ev) hipEventDestroy(e);
-    if (c->l_graph_x) hipGraphExecDestroy(c->l_graph_x);
-    if (c->l_graph_g) hipGraphDestroy(c->l_graph_g);
+    learned_drop_graph(c);
     if (c->fcg_host) hipHostFree(c->fcg_host);
     for (void* p : c->allocs) hipFree(p);
     for (void* p : c->probes.mem) hipFree(p);
@@ -1338,12 +1384,7 @@ int cfd_set_multigrid(cfd_ctx* c, const cfd_mg_desc* d) {
     c->mg_host = h;
     c->mg_ready = 1;
     c->l_ready = 0;   // a learned solver set up on the previous hierarchy is stale
-    if (c->l_graph_x) {
-        hipGraphExecDestroy(c->l_graph_x);
-        hipGraphDestroy(c->l_graph_g);
-        c->l_graph_x = nullptr;
-        c->l_graph_g = nullptr;
-    }
+    learned_drop_graph(c);
     return 0;
 }
 
@@ -1368,6 +1409,8 @@ int cfd_set_learned(cfd_ctx* c, const cfd_learned_desc* d) {
     if (d->harvest_cap < 0 || (d->harvest_cap > 0 && (!d->harvest_in || !d->harvest_out)))
         return set_err(CFD_ERR_ARG, "harvest buffers");
     if (d->max_iter < 1) return set_err(CFD_ERR_ARG, "max_iter %d", d->max_iter);
+    if (d->sweeps != 0 && d->sweeps != -1 && (d->sweeps < 2 || d->sweeps > 4))
+        return set_err(CFD_ERR_ARG, "smoothing sweeps %d (2, 3, 4 or -1)", d->sweeps);
     HIPC(hipStreamSynchronize(c->stream));
     if (!c->fcg) {
         TRY(dalloc(c, &c->fcg, 1));
@@ -1458,6 +1501,13 @@ int cfd_set_learned(cfd_ctx* c, const cfd_learned_desc* d) {
         c->l_fused = 1;
     }
     c->l_legs = ((d->coarse_fused & 2) && (c->l_vlds || c->l_fused)) ? ((d->coarse_fused & 4) ? 2 : 1) : 0;
+    c->l_sweeps_cfg = d->sweeps < 0 ? -1 : d->sweeps > 2 ? d->sweeps : 2;
+    c->l_sweeps = c->l_sweeps_cfg > 2 ? c->l_sweeps_cfg : 2;
+    if (c->l_sweeps > 2 && c->l_legs != 1) {
+        c->l_ready = 0;
+        return set_err(CFD_ERR_ARG, "more than 2 smoothing sweeps need the fused legs (coarse_fused "
+                                    "bits 0 and 1 without bit 2, a fused coarse end)");
+    }
     c->l_H = H;
     c->l_W = W;
     c->l_net = d->mode == LEARNED_NET ? d->net : nullptr;
@@ -1465,6 +1515,7 @@ int cfd_set_learned(cfd_ctx* c, const cfd_learned_desc* d) {
     c->l_hout = d->harvest_out;
     c->l_hcap = d->harvest_cap;
     c->l_prev_iters = 0;
+    c->l_low_run = 0;
     c->l_ready = 1;
     return 0;
 }
@@ -1500,6 +1551,7 @@ int cfd_learned_stats(cfd_ctx* c, cfd_learned_stats_t* out) {
     out->last_info = h.info;
     out->last_relres = h.bnorm > 0.0 ? sqrt(h.rr) / h.bnorm : 0.0;
     out->net_off = h.net_off;
+    out->last_sweeps = c->l_sweeps;
     return 0;
 }
 

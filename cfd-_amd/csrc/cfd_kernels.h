@@ -158,6 +158,7 @@ struct ProjArgs {
     double rtol;
     int maxiter;
     int zero_p_after;        // initial projection: p := 0 afterwards
+    int end_always;          // learned solver's end launches: act even when the solve failed
     int method;              // PROJ_PCG64, PROJ_MG, PROJ_LEARNED
     double reliable_delta;   // mixed: true-residual replacement when ||r|| drops by this factor
     struct FcgState* fcg;    // PROJ_LEARNED: state of the host-driven flexible CG

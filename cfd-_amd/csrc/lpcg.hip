@@ -545,22 +545,24 @@ __global__ void __launch_bounds__(256) k_lv_harvest(LvArgs a, int l, float* hin,
 
 // ---- the legs of the V-cycle above the coarse end, one launch per level and leg
 // A work-group takes tiles of TI x TJ cells of level l (TI, TJ even, so a
-// tile holds whole 2 x 2 aggregates) and stages them in LDS with a ring of two
+// tile holds whole 2 x 2 aggregates) and stages them in LDS with a ring of NU
 // cells (rows wrapped when periodic; cells outside the arrays count as
-// solid), so that the sweeps of a leg run back to back with barriers instead
-// of launches:
+// solid), so that the NU damped-Jacobi sweeps of a leg run back to back with
+// barriers instead of launches, each sweep on a ring one cell narrower:
 //   down-leg (k_lv_down): x = w b / d (the first sweep from zero, as the
-//     producers of b store it), t = x + w (b - A x) / d, and the residual
-//     b - A t summed over each aggregate into b of level l + 1 (x = w b / d
-//     there);
-//   up-leg (k_lv_up): y = t + wc t_(l+1) (the prolongation), x = y + w (b -
-//     A y) / d, t = x + w (b - A x) / d; on level 0 with the partial dot
-//     products r.z and r_old.z of the flexible CG.
-// The values are those of k_lv_smooth / k_lv_restrict / k_lv_post /
-// k_lv_smooth_dots, expression for expression: a ring cell is recomputed from
-// the same inputs by the same FP32 arithmetic.  The down-leg's t goes to
-// `tdown` (the level's x array: no later kernel reads x there, k_lv_coarse and
-// the next down-leg start from b), where the up-leg reads it back: the up-leg
+//     producers of b store it), NU - 1 sweeps t = x + w (b - A x) / d, and
+//     the residual b - A t summed over each aggregate into b of level l + 1
+//     (x = w b / d there);
+//   up-leg (k_lv_up): y = t + wc t_(l+1) (the prolongation of level l + 1's
+//     correction onto the down-leg's t), then NU sweeps; on level 0 with the
+//     partial dot products r.z and r_old.z of the flexible CG.
+// With NU = 2 the values are those of k_lv_smooth / k_lv_restrict / k_lv_post
+// / k_lv_smooth_dots (the V-cycle of the persistent multigrid solver),
+// expression for expression: a ring cell is recomputed from the same inputs by
+// the same FP32 arithmetic.  NU > 2 is a stronger preconditioner, still
+// symmetric (as many sweeps down as up).  The down-leg's t goes to `tdown`
+// (the level's x array: no later kernel reads x there, k_lv_coarse and the
+// next down-leg start from b), where the up-leg reads it back: the up-leg
 // writes t, which the work-groups of neighbouring tiles read in their rings,
 // so it must not be the array it reads.
 struct LegArgs {
@@ -607,18 +609,16 @@ __device__ __forceinline__ void leg_tiles(const MGLevelDev& L, int& ntj, int& t0
     t1 = (int)((nt * (b + 1)) / gridDim.x);
 }
 
-}  // namespace
-
-// The staging of a tile's LDS region (the tile and its two-cell ring), 256
+// The staging of a tile's LDS region (the tile and its ring of R cells), 256
 // threads: every global load of the thread's cells is issued before any is
 // used.  Per cell: b, the diagonal (level 0: from the code), the couplings
-// (levels >= 1) and, for the up-leg, the prolongated correction y = t + wc
-// t_(l+1) (k_lv_post's expression; 0 off the fluid).  Cells outside the arrays
-// count as solid.
-template <bool LEV0, bool UP, int TI, int TJ>
+// (levels >= 1) and either x = w b / d (down-leg) or the prolongated
+// correction y = t + wc t_(l+1) (up-leg; k_lv_post's expression, 0 off the
+// fluid).  (ib, jb): the level cell of LDS cell (0, 0).
+template <bool LEV0, bool UP, int TI, int TJ, int R>
 __device__ __forceinline__ void leg_stage(const LegArgs& a, int ib, int jb, const float* s_d0,
                                           float* sb, float* sd, float* se, float* sn, float* sv) {
-    constexpr int W = TJ + 4, N = (TI + 4) * W, LA = (N + 255) / 256;
+    constexpr int W = TJ + 2 * R, N = (TI + 2 * R) * W, LA = (N + 255) / 256;
     const MGLevelDev& L = a.L;
     const MGLevelDev& C = a.C;
     float rb[LA], rd[LA], re[LA], rn[LA], rt[LA], rc[LA];
@@ -664,28 +664,46 @@ __device__ __forceinline__ void leg_stage(const LegArgs& a, int ib, int jb, cons
     }
 }
 
-// dst = src + w (b - A src) / d on the tile and its first ring (0 off the fluid)
-template <bool LEV0, int TI, int TJ>
-__device__ __forceinline__ void leg_sweep_ring(const LegArgs& a, const float* sb, const float* sd,
-                                               const float* se, const float* sn, const float* src,
-                                               float* dst) {
-    constexpr int W = TJ + 4, W1 = TJ + 2, N1 = (TI + 2) * W1, LS = (N1 + 255) / 256;
+// dst = src + w (b - A src) / d on the tile and its ring of RHO cells (0 off
+// the fluid), in an LDS region with a ring of R cells
+template <bool LEV0, int TI, int TJ, int R, int RHO>
+__device__ __forceinline__ void leg_sweep(const LegArgs& a, const float* sb, const float* sd,
+                                          const float* se, const float* sn, const float* src,
+                                          float* dst) {
+    constexpr int W = TJ + 2 * R, W1 = TJ + 2 * RHO, N1 = (TI + 2 * RHO) * W1;
+    constexpr int LS = (N1 + 255) / 256, O = R - RHO;
 #pragma unroll
     for (int u = 0; u < LS; ++u) {
         const int e = (int)threadIdx.x + u * 256;
         if (e >= N1) continue;
-        const int r = e / W1 + 1, q = e - (r - 1) * W1 + 1;
-        const int k = r * W + q;
+        const int r = e / W1, q = e - r * W1;
+        const int k = (r + O) * W + q + O;
         const float d = sd[k];
         dst[k] = d != 0.0f ? src[k] + a.w * (sb[k] - leg_apply<LEV0>(a, k, W, d, src, se, sn)) / d
                            : 0.0f;
     }
 }
 
-template <bool LEV0, int TI, int TJ>
+// NS sweeps on the rings RHO, RHO - 1, .. alternating between p and q, each
+// followed by a barrier; the buffer that holds the last result
+template <bool LEV0, int TI, int TJ, int R, int RHO, int NS>
+__device__ __forceinline__ float* leg_sweeps(const LegArgs& a, const float* sb, const float* sd,
+                                             const float* se, const float* sn, float* p, float* q) {
+    if constexpr (NS == 0) {
+        return p;
+    } else {
+        leg_sweep<LEV0, TI, TJ, R, RHO>(a, sb, sd, se, sn, p, q);
+        __syncthreads();
+        return leg_sweeps<LEV0, TI, TJ, R, RHO - 1, NS - 1>(a, sb, sd, se, sn, q, p);
+    }
+}
+
+}  // namespace
+
+template <bool LEV0, int NU, int TI, int TJ>
 __global__ void __launch_bounds__(256) k_lv_down(LegArgs a) {
     if (*a.done) return;
-    constexpr int W = TJ + 4, N = (TI + 4) * W;
+    constexpr int R = NU, W = TJ + 2 * R, N = (TI + 2 * R) * W;
     __shared__ float sb[N], sd[N], sx[N], st[N];
     __shared__ float se[LEV0 ? 1 : N], sn[LEV0 ? 1 : N];
     __shared__ float s_d0[64];
@@ -699,12 +717,12 @@ __global__ void __launch_bounds__(256) k_lv_down(LegArgs a) {
     int ntj, t0, t1;
     leg_tiles<TI, TJ>(L, ntj, t0, t1);
     for (int tile = t0; tile < t1; ++tile) {
-        const int ib = (tile / ntj) * TI - 1;   // level row of LDS row 0
-        const int jb = (tile % ntj) * TJ - 1;
-        leg_stage<LEV0, false, TI, TJ>(a, ib, jb, s_d0, sb, sd, se, sn, sx);   // x = w b / d
+        const int ib = (tile / ntj) * TI + 1 - R;   // level row of LDS row 0
+        const int jb = (tile % ntj) * TJ + 1 - R;
+        leg_stage<LEV0, false, TI, TJ, R>(a, ib, jb, s_d0, sb, sd, se, sn, sx);   // x = w b / d
         __syncthreads();
-        leg_sweep_ring<LEV0, TI, TJ>(a, sb, sd, se, sn, sx, st);
-        __syncthreads();
+        // t on the tile and its first ring
+        const float* tt = leg_sweeps<LEV0, TI, TJ, R, R - 1, NU - 1>(a, sb, sd, se, sn, sx, st);
         // the residual restricted: one aggregate of level l + 1 per thread
         const int ci0 = (tile / ntj) * (TI / 2) + 1, cj0 = (tile % ntj) * (TJ / 2) + 1;
         constexpr int NC = (TI / 2) * (TJ / 2), LC = (NC + 255) / 256;
@@ -719,10 +737,10 @@ __global__ void __launch_bounds__(256) k_lv_down(LegArgs a) {
                 for (int dj = 0; dj < 2; ++dj) {
                     const int i = 2 * ci - 1 + di, j = 2 * cj - 1 + dj;
                     if (i > L.nx || j > L.ny) continue;
-                    const int k = (2 + 2 * cr + di) * W + 2 + 2 * cs + dj;
+                    const int k = (R + 2 * cr + di) * W + R + 2 * cs + dj;
                     const float d = sd[k];
                     if (d == 0.0f) continue;
-                    bc += sb[k] - leg_apply<LEV0>(a, k, W, d, st, se, sn);
+                    bc += sb[k] - leg_apply<LEV0>(a, k, W, d, tt, se, sn);
                 }
             const long cc = ix(ci, cj, C.P);
             C.b[cc] = bc;
@@ -734,18 +752,18 @@ __global__ void __launch_bounds__(256) k_lv_down(LegArgs a) {
         for (int u = 0; u < LI; ++u) {
             const int e = (int)threadIdx.x + u * 256;
             const int r = e / TJ, q = e - r * TJ;
-            const int i = ib + 2 + r, j = jb + 2 + q;
-            const int k = (r + 2) * W + q + 2;
-            if (e < NI && i <= L.nx && j <= L.ny && sd[k] != 0.0f) a.tdown[ix(i, j, L.P)] = st[k];
+            const int i = ib + R + r, j = jb + R + q;
+            const int k = (r + R) * W + q + R;
+            if (e < NI && i <= L.nx && j <= L.ny && sd[k] != 0.0f) a.tdown[ix(i, j, L.P)] = tt[k];
         }
         __syncthreads();   // the next tile overwrites the LDS arrays
     }
 }
 
-template <bool LEV0, bool DOTS, int TI, int TJ>
+template <bool LEV0, bool DOTS, int NU, int TI, int TJ>
 __global__ void __launch_bounds__(256) k_lv_up(LegArgs a) {
     if (*a.done) return;
-    constexpr int W = TJ + 4, N = (TI + 4) * W;
+    constexpr int R = NU, W = TJ + 2 * R, N = (TI + 2 * R) * W;
     __shared__ float sb[N], sd[N], sy[N], sx[N];
     __shared__ float se[LEV0 ? 1 : N], sn[LEV0 ? 1 : N];
     __shared__ float s_d0[64];
@@ -760,19 +778,19 @@ __global__ void __launch_bounds__(256) k_lv_up(LegArgs a) {
     int ntj, t0, t1;
     leg_tiles<TI, TJ>(L, ntj, t0, t1);
     for (int tile = t0; tile < t1; ++tile) {
-        const int ib = (tile / ntj) * TI - 1;
-        const int jb = (tile % ntj) * TJ - 1;
-        leg_stage<LEV0, true, TI, TJ>(a, ib, jb, s_d0, sb, sd, se, sn, sy);   // y
+        const int ib = (tile / ntj) * TI + 1 - R;
+        const int jb = (tile % ntj) * TJ + 1 - R;
+        leg_stage<LEV0, true, TI, TJ, R>(a, ib, jb, s_d0, sb, sd, se, sn, sy);   // y
         __syncthreads();
-        leg_sweep_ring<LEV0, TI, TJ>(a, sb, sd, se, sn, sy, sx);
-        __syncthreads();
+        // NU - 1 sweeps down to the first ring, the last one on the tile below
+        const float* xx = leg_sweeps<LEV0, TI, TJ, R, R - 1, NU - 1>(a, sb, sd, se, sn, sy, sx);
         constexpr int NI = TI * TJ, LI = (NI + 255) / 256;
         double rr[LI], ro[LI];
 #pragma unroll
         for (int u = 0; u < LI; ++u) {   // the residuals' loads first
             const int e = (int)threadIdx.x + u * 256;
             const int r = e / TJ, q = e - r * TJ;
-            const int i = ib + 2 + r, j = jb + 2 + q;
+            const int i = ib + R + r, j = jb + R + q;
             const bool in = e < NI && i <= L.nx && j <= L.ny;
             const long c = in ? ix(i, j, L.P) : 0;
             rr[u] = DOTS && in ? a.r[c] : 0.0;
@@ -782,12 +800,12 @@ __global__ void __launch_bounds__(256) k_lv_up(LegArgs a) {
         for (int u = 0; u < LI; ++u) {
             const int e = (int)threadIdx.x + u * 256;
             const int r = e / TJ, q = e - r * TJ;
-            const int i = ib + 2 + r, j = jb + 2 + q;
-            const int k = (r + 2) * W + q + 2;
+            const int i = ib + R + r, j = jb + R + q;
+            const int k = (r + R) * W + q + R;
             if (e >= NI || i > L.nx || j > L.ny) continue;
             const float d = sd[k];
             if (d == 0.0f) continue;
-            const float z = sx[k] + w * (sb[k] - leg_apply<LEV0>(a, k, W, d, sx, se, sn)) / d;
+            const float z = xx[k] + w * (sb[k] - leg_apply<LEV0>(a, k, W, d, xx, se, sn)) / d;
             L.t[ix(i, j, L.P)] = z;
             if (DOTS) {
                 s2[0] += rr[u] * (double)z;
@@ -1512,6 +1530,15 @@ __device__ __forceinline__ void lp_band(int nx, int b, int nb, int& i0, int& i1)
     i1 = 1 + (int)(((long)nx * (b + 1)) / nb);
 }
 
+// the end launches of a learned projection are enqueued after every batch of
+// iterations and act once: when the solve has finished successfully (or, with
+// end_always, after a failed one that is not continued by the fallback)
+__device__ __forceinline__ bool lp_end_idle(const ProjArgs& a) {
+    if (a.st->done) return true;
+    if (a.end_always) return false;
+    return !(a.fcg->done && a.fcg->info == 0);
+}
+
 __device__ __forceinline__ void lp_means(const ProjArgs& a, const double* slot, int nb,
                                          double (&mean)[CFD_MAX_COMP]) {
     double cs[CFD_MAX_COMP];
@@ -1639,7 +1666,7 @@ __global__ void __launch_bounds__(256) k_lp_init(ProjArgs a, LvArgs la, int nb) 
 // partial sums of x per Neumann component (x := 0 first on the fluid cells
 // when the right-hand side was zero: SciPy returns b itself)
 __global__ void __launch_bounds__(512) k_lp_xsum(ProjArgs a, Geo g, double* part) {
-    if (a.st->done) return;
+    if (lp_end_idle(a)) return;
     const bool zero = a.fcg->bnorm == 0.0;
     double cs[CFD_MAX_COMP];
 #pragma unroll
@@ -1661,7 +1688,7 @@ __global__ void __launch_bounds__(512) k_lp_xsum(ProjArgs a, Geo g, double* part
 // p = x - mean on the fluid cells, 0 elsewhere and on the ghost columns; the
 // ghost rows 0 and nx + 1 (periodic copies of cells) by the first and last band
 __global__ void __launch_bounds__(512) k_lp_pwrite(ProjArgs a, Geo g, LvArgs la, int nb) {
-    if (a.st->done) return;
+    if (lp_end_idle(a)) return;
     double mean[CFD_MAX_COMP];
     lp_means(a, la.part + LV_P_CM * LV_SLOT, nb, mean);
     const int nx = g.nx, ny = g.ny, pc = g.pc;
@@ -1689,7 +1716,7 @@ __global__ void __launch_bounds__(512) k_lp_pwrite(ProjArgs a, Geo g, LvArgs la,
 // u = F - dt grad p, v = G - dt grad p (boundary faces as in k_project), the
 // partial max |u_new - u_old| over the faces between fluid cells
 __global__ void __launch_bounds__(512) k_lp_correct(ProjArgs a, Geo g, double* part) {
-    if (!a.correct || a.st->done) return;
+    if (!a.correct || lp_end_idle(a)) return;
     const double dt = a.dt_override > 0.0 ? a.dt_override : a.st->dt;
     const int nx = g.nx, ny = g.ny, pc = g.pc;
     const int periodic = g.periodic;
@@ -1753,7 +1780,7 @@ __global__ void __launch_bounds__(512) k_lp_correct(ProjArgs a, Geo g, double* p
 // monitor, the periodic corner faces
 __global__ void __launch_bounds__(256) k_lp_fin(ProjArgs a, Geo g, double* part, int nb) {
     DevState* st = a.st;
-    if (st->done) return;
+    if (lp_end_idle(a)) return;
     double res = 0.0;
     if (a.correct)
         for (int k = threadIdx.x; k < nb; k += blockDim.x)
@@ -1777,8 +1804,9 @@ __global__ void __launch_bounds__(256) k_lp_fin(ProjArgs a, Geo g, double* part,
 }
 
 // p := 0 on the whole padded array (after the initial projection)
-__global__ void __launch_bounds__(256) k_lp_pzero(double* p, long n, const DevState* st) {
-    if (st->done) return;
+__global__ void __launch_bounds__(256) k_lp_pzero(ProjArgs a, long n) {
+    if (lp_end_idle(a)) return;
+    double* p = a.p;
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) p[k] = 0.0;
 }

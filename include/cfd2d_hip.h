@@ -315,13 +315,20 @@ typedef struct cfd_learned_desc {
     int32_t x0_extrapolate;   /* 1: start each solve of a time step from the linear
                                  extrapolation x + (dt/dt_old)(x - x_previous) of the last
                                  two solutions; 0: from the last solution (SciPy's x0) */
+    int32_t sweeps;           /* damped-Jacobi sweeps per leg on the levels above the fused
+                                 coarse end: 0 or 2 (the multigrid solver's V-cycle), 3 or
+                                 4 (a stronger preconditioner; needs coarse_fused bits 0
+                                 and 1 without bit 2), -1: 3 for a solve after one of at
+                                 least 4 iterations, 2 again after 8 consecutive 3-sweep
+                                 solves of at most 2 (2 throughout without those fused
+                                 legs) */
 } cfd_learned_desc;
 typedef struct cfd_learned_stats_t {
     int64_t harvested, solves, iterations, fallbacks;
     int32_t last_iters, last_info;
     double last_relres;
     int32_t net_off;           /* 1: the network failed its check and is switched off */
-    int32_t pad;
+    int32_t last_sweeps;       /* smoothing sweeps per leg of the last solve */
 } cfd_learned_stats_t;
 int cfd_set_learned(cfd_ctx* ctx, const cfd_learned_desc* desc);
 int cfd_learned_stats(cfd_ctx* ctx, cfd_learned_stats_t* out);
