@@ -1491,6 +1491,13 @@ int cfd_set_learned(cfd_ctx* c, const cfd_learned_desc* d) {
                 off += VC_ARR * L.sz;
                 v.total += L.sz;
             }
+            std::vector<float> img;
+            if (vc_build_image(v, img)) return set_err(CFD_ERR_HIP, "coarse-end image");
+            float* dimg = nullptr;
+            TRY(dalloc(c, &dimg, img.size()));
+            HIPC(hipMemcpy(dimg, img.data(), img.size() * sizeof(float), hipMemcpyHostToDevice));
+            v.image = reinterpret_cast<const float4*>(dimg);
+            v.image_n4 = (int)(img.size() / 4);
         }
     }
     if (!c->l_vlds && d->coarse_fused && d->mode != LEARNED_TEACHER && d->level >= 1 && L.nx == 32 &&

@@ -542,6 +542,13 @@ int cfd_ens_set_coarse(cfd_ens* e, int32_t mode) {
             off += VC_ARR * L.sz;
             v.total += L.sz;
         }
+        std::vector<float> img;
+        if (vc_build_image(v, img)) return set_err(CFD_ERR_HIP, "coarse-end image");
+        float* dimg = nullptr;
+        TRY(ealloc(e, &dimg, img.size()));
+        HIPC(hipMemcpy(dimg, img.data(), img.size() * sizeof(float), hipMemcpyHostToDevice));
+        v.image = reinterpret_cast<const float4*>(dimg);
+        v.image_n4 = (int)(img.size() / 4);
         e->vlc[k] = lc;
     }
     e->vcycle = 1;

@@ -4,8 +4,9 @@
 // products, ending on the FP64 true residual ||b - A x|| < rtol ||b|| (with a
 // restart from the true residual otherwise), as in the multigrid-preconditioned
 // branch of k_project.  Its iterations are ordinary kernel launches, captured
-// once into a HIP graph and replayed; every kernel returns at once after the
-// solve has finished.  Reductions: each work-group stores its partial sums,
+// once into a HIP graph and replayed; after the solve has finished every
+// kernel returns at once (the LDS coarse end only writes nothing back).
+// Reductions: each work-group stores its partial sums,
 // and the kernel that needs the total adds them in work-group order (so the
 // results are reproducible); one-work-group kernels take the decisions that
 // the host reads (converged, restart, breakdown).
@@ -1255,7 +1256,6 @@ __global__ void __launch_bounds__(1024) k_lv_coarse(LvArgs a, int Lc, int K, Coa
 constexpr int VC_POOL = 36864;           // LDS floats (144 KB)
 constexpr int VC_ARR = 6;                // b, x, t, diag, cE, cN per level
 constexpr int VC_MAXL = 8;               // levels held in LDS
-constexpr int VC_LOADS = VC_POOL / VC_ARR / 1024;   // coefficient entries per thread
 constexpr int VC_MAXC = 64;              // cells of the dense coarsest level
 
 // LDS floats needed for levels Lc .. nlev-1 (host and device)
@@ -1282,16 +1282,42 @@ struct VcLev {
 // arguments of k_lv_vcoarse, built on the host from the hierarchy (kernel
 // arguments: no descriptor loads on the device).  One work-group per instance
 // (the single case: one); an instance whose solve has finished (the int at
-// done + blockIdx.x * done_stride bytes) returns at once.
+// done + blockIdx.x * done_stride bytes) writes nothing back.
 struct VcArgs {
     const char* done;
     long done_stride;
     const float* cinv;       // dense inverse of the coarsest level (shared)
+    const float4* image;     // the pool as it starts: coefficients, zero b, x, t (shared)
+    int image_n4;            // its length in float4
     float omega, wc;
     int periodic, nl, nc;    // levels in LDS, cells of the coarsest level
     int total;               // sum of sz over the levels
     VcLev lev[VC_MAXL];
 };
+
+// The image of the pool at the start of k_lv_vcoarse, built on the host from
+// the levels' device arrays (pitch P): per level b, x, t zero and diag, cE, cN
+// at the tight pitch ny + 2 with their ghost rings, at the offsets of v.lev.
+inline int vc_build_image(const VcArgs& v, std::vector<float>& img) {
+    int n = 0;
+    for (int q = 0; q < v.nl; ++q) n = std::max(n, v.lev[q].off + VC_ARR * v.lev[q].sz);
+    img.assign((size_t)((n + 3) / 4) * 4, 0.0f);
+    std::vector<float> h;
+    for (int q = 0; q < v.nl; ++q) {
+        const VcLev& L = v.lev[q];
+        const size_t cnt = (size_t)(L.nx + 2) * L.P;
+        h.resize(cnt);
+        const float* src[3] = {L.gd, L.ge, L.gn};
+        for (int a = 0; a < 3; ++a) {
+            if (hipMemcpy(h.data(), src[a], cnt * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+                return -1;
+            float* dst = img.data() + L.off + (3 + a) * L.sz;
+            for (int i = 0; i <= L.nx + 1; ++i)
+                for (int j = 0; j <= L.ny + 1; ++j) dst[i * L.pw + j] = h[(size_t)i * L.P + j];
+        }
+    }
+    return 0;
+}
 
 namespace {
 
@@ -1334,32 +1360,71 @@ __device__ __forceinline__ float vc_apply(const VcView& L, int periodic, int i, 
            L.n[k] * v[k + 1] - L.n[k - 1] * v[k - 1];
 }
 
+// f over cells 0..n-1 of an LDS level by the whole work-group, U per thread
+// at a time: gather(q) (LDS loads and arithmetic, no stores) for all U before
+// scatter(q, value) stores any, so that the U cells' LDS loads are in flight
+// together instead of one cell's dependent chain after the other
+template <int U, typename G, typename S>
+__device__ __forceinline__ void vc_batch(int n, G gather, S scatter) {
+    const int bs = blockDim.x;
+    using T = decltype(gather(0));
+    for (int q0 = threadIdx.x; q0 < n; q0 += U * bs) {
+        T val[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) val[u] = gather(q0 + u * bs < n ? q0 + u * bs : q0);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (q0 + u * bs < n) scatter(val[u]);
+    }
+}
+
+// vc_batch with 4 cells per thread at a time on levels of at least 4 cells
+// per thread, else 1
+template <typename G, typename S>
+__device__ __forceinline__ void vc_cells(int n, G gather, S scatter) {
+    if (n >= 4 * (int)blockDim.x) vc_batch<4>(n, gather, scatter);
+    else vc_batch<1>(n, gather, scatter);
+}
+
+// a cell's LDS index, diagonal (0: not fluid) and new value
+struct VcCell {
+    int k;
+    float d, v;
+};
+
 // dst = src + w (b - A src) / d on the fluid cells of an LDS level
 __device__ __forceinline__ void vc_smooth(const VcView& L, int periodic, float w, const float* src,
                                           float* dst) {
-    const int n = L.nx * L.ny;
-    for (int q = threadIdx.x; q < n; q += blockDim.x) {
-        int i, j;
-        vc_ij(L, q, i, j);
-        const int k = i * L.pw + j;
-        const float d = L.d[k];
-        if (d == 0.0f) continue;
-        dst[k] = src[k] + w * (L.b[k] - vc_apply(L, periodic, i, j, d, src)) / d;
-    }
+    vc_cells(
+        L.nx * L.ny,
+        [&](int q) {
+            int i, j;
+            vc_ij(L, q, i, j);
+            VcCell o;
+            o.k = i * L.pw + j;
+            o.d = L.d[o.k];
+            o.v = src[o.k] + w * (L.b[o.k] - vc_apply(L, periodic, i, j, o.d, src)) / o.d;
+            return o;
+        },
+        [&](const VcCell& o) {
+            if (o.d != 0.0f) dst[o.k] = o.v;
+        });
     __syncthreads();
 }
 
 }  // namespace
 
 __global__ void __launch_bounds__(1024) k_lv_vcoarse(VcArgs a) {
-    __shared__ float pool[VC_POOL];
+    __shared__ __align__(16) float pool[VC_POOL];
     __shared__ float s_bc[VC_MAXC];
     const int nl = a.nl, nc = a.nc, per = a.periodic;
     const float w = a.omega, wc = a.wc;
     const int tid = threadIdx.x, bs = blockDim.x;
     // every global load of the launch in flight at once: the solve's state,
-    // this lane's row of the dense inverse (wave 0), the coefficients of every
-    // level (ghost ring included) and level Lc's right-hand side
+    // this lane's row of the dense inverse (wave 0), the pool's image and
+    // level Lc's right-hand side.  (An instance whose solve has finished
+    // still runs through the LDS work, which needs no global memory, so that
+    // no wave waits for that flag's load; it writes nothing back.)
     const long inst = blockIdx.x;
     const int done = *reinterpret_cast<const int*>(a.done + inst * a.done_stride);
     float crow[VC_MAXC];
@@ -1367,44 +1432,41 @@ __global__ void __launch_bounds__(1024) k_lv_vcoarse(VcArgs a) {
 #pragma unroll
         for (int q = 0; q < VC_MAXC; ++q) crow[q] = q < nc ? a.cinv[(long)tid * nc + q] : 0.0f;
     }
-    float rd[VC_LOADS], re[VC_LOADS], rn[VC_LOADS], rb[VC_LOADS];
-    int dst[VC_LOADS];
+    constexpr int NI = (VC_POOL / 4 + 1023) / 1024, NB = (64 * 64 + 1023) / 1024;
+    float4 img[NI];
 #pragma unroll
-    for (int u = 0; u < VC_LOADS; ++u) {
-        int e = tid + u * bs;
-        dst[u] = -1;
-        rd[u] = re[u] = rn[u] = rb[u] = 0.0f;
-        if (e >= a.total) continue;
-        int q = 0;
-        while (e >= a.lev[q].sz) e -= a.lev[q++].sz;
-        const VcLev& L = a.lev[q];
-        const int i = e / L.pw, j = e - i * L.pw;
-        const long g = ix(i, j, L.P);
-        rd[u] = L.gd[g];
-        re[u] = L.ge[g];
-        rn[u] = L.gn[g];
-        if (q == 0 && i >= 1 && i <= L.nx && j >= 1 && j <= L.ny) rb[u] = L.gb[inst * L.istride + g];
-        dst[u] = (q << 24) | e;   // level and position within its arrays
+    for (int u = 0; u < NI; ++u) {
+        const int k = tid + u * bs;
+        img[u] = k < a.image_n4 ? a.image[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    if (done) return;
-    // zero b, x, t of every level (ghost rings, solid cells) meanwhile
-    for (int q = 0; q < nl; ++q) {
-        const VcLev& L = a.lev[q];
-        for (int k = tid; k < 3 * L.sz; k += bs) pool[L.off + k] = 0.0f;
+    const VcLev& G = a.lev[0];
+    const VcView L0 = vc_view(pool, G);
+    float rb[NB];
+    int kb[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+        const int c = tid + u * bs;
+        kb[u] = -1;
+        rb[u] = 0.0f;
+        if (c >= L0.nx * L0.ny) continue;
+        int i, j;
+        vc_ij(L0, c, i, j);
+        rb[u] = G.gb[inst * G.istride + ix(i, j, G.P)];
+        kb[u] = i * L0.pw + j;
+    }
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+        const int k = tid + u * bs;
+        if (k < a.image_n4) reinterpret_cast<float4*>(pool)[k] = img[u];
     }
     __syncthreads();
+    // level Lc: b and the first sweep from zero, x = w b / d
 #pragma unroll
-    for (int u = 0; u < VC_LOADS; ++u) {
-        if (dst[u] < 0) continue;
-        const int q = dst[u] >> 24, e = dst[u] & 0xFFFFFF;
-        const VcView V = vc_view(pool, a.lev[q]);
-        V.d[e] = rd[u];
-        V.e[e] = re[u];
-        V.n[e] = rn[u];
-        if (q == 0) {   // level Lc: b and the first sweep from zero, x = w b / d
-            V.b[e] = rb[u];
-            V.x[e] = rd[u] != 0.0f ? w * rb[u] / rd[u] : 0.0f;
-        }
+    for (int u = 0; u < NB; ++u) {
+        if (kb[u] < 0) continue;
+        const float d = L0.d[kb[u]];
+        L0.b[kb[u]] = rb[u];
+        L0.x[kb[u]] = d != 0.0f ? w * rb[u] / d : 0.0f;
     }
     __syncthreads();
     // down-leg: second pre-smoothing sweep, residual, restriction
@@ -1412,38 +1474,38 @@ __global__ void __launch_bounds__(1024) k_lv_vcoarse(VcArgs a) {
         const VcView L = vc_view(pool, a.lev[q]);
         const VcView C = vc_view(pool, a.lev[q + 1]);
         vc_smooth(L, per, w, L.x, L.t);
-        // r = b - A t into the x slot: the pass reads t, and x is not read again
-        // before the up-leg rewrites it (vc_smooth's barrier ended its reads)
-        const int n = L.nx * L.ny;
-        for (int c = tid; c < n; c += bs) {
-            int i, j;
-            vc_ij(L, c, i, j);
-            const int k = i * L.pw + j;
-            const float d = L.d[k];
-            if (d != 0.0f) L.x[k] = L.b[k] - vc_apply(L, per, i, j, d, L.t);
-        }
-        __syncthreads();
-        const int ncl = C.nx * C.ny;
-        for (int c = tid; c < ncl; c += bs) {
-            int ci, cj;
-            vc_ij(C, c, ci, cj);
-            const int kc = ci * C.pw + cj;
-            float bc = 0.0f;
-            for (int di = 0; di < 2; ++di)
-                for (int dj = 0; dj < 2; ++dj) {
-                    const int i = 2 * ci - 1 + di, j = 2 * cj - 1 + dj;
-                    if (i > L.nx || j > L.ny) continue;
-                    const int k = i * L.pw + j;
-                    if (L.d[k] != 0.0f) bc += L.x[k];
+        // the residual b - A t of the aggregate's fluid cells, summed (the
+        // expression and order of k_lv_restrict)
+        const bool last = q + 1 == nl - 1;   // (the coarsest level's x is not used)
+        vc_cells(
+            C.nx * C.ny,
+            [&](int c) {
+                int ci, cj;
+                vc_ij(C, c, ci, cj);
+                VcCell o;
+                o.k = ci * C.pw + cj;
+                float bc = 0.0f;
+                for (int di = 0; di < 2; ++di)
+                    for (int dj = 0; dj < 2; ++dj) {
+                        const int i = 2 * ci - 1 + di, j = 2 * cj - 1 + dj;
+                        if (i > L.nx || j > L.ny) continue;
+                        const int k = i * L.pw + j;
+                        const float d = L.d[k];
+                        if (d != 0.0f) bc += L.b[k] - vc_apply(L, per, i, j, d, L.t);
+                    }
+                o.v = bc;
+                o.d = last ? (float)c : C.d[o.k];   // the last level: the compact index
+                return o;
+            },
+            [&](const VcCell& o) {
+                C.b[o.k] = o.v;
+                if (!last) {
+                    C.x[o.k] = o.d != 0.0f ? w * o.v / o.d : 0.0f;
+                } else {
+                    const int c = (int)o.d;
+                    if (c < VC_MAXC) s_bc[c] = o.v;   // the coarsest right-hand side, compact
                 }
-            C.b[kc] = bc;
-            if (q + 1 < nl - 1) {   // (the coarsest level's x is not used)
-                const float dc = C.d[kc];
-                C.x[kc] = dc != 0.0f ? w * bc / dc : 0.0f;
-            } else if (c < VC_MAXC) {
-                s_bc[c] = bc;       // the coarsest right-hand side, compact
-            }
-        }
+            });
         __syncthreads();
     }
     // coarsest level: t = cinv b, the sum in k_lv_dense's order
@@ -1464,26 +1526,24 @@ __global__ void __launch_bounds__(1024) k_lv_vcoarse(VcArgs a) {
     for (int q = nl - 2; q >= 0; --q) {
         const VcView L = vc_view(pool, a.lev[q]);
         const VcView C = vc_view(pool, a.lev[q + 1]);
-        const int n = L.nx * L.ny;
-        for (int c = tid; c < n; c += bs) {
-            int i, j;
-            vc_ij(L, c, i, j);
-            const int k = i * L.pw + j;
-            const float d = L.d[k];
-            if (d == 0.0f) continue;
-            auto corr = [&](int p, int r) -> float {
-                const int kk = p * L.pw + r;
-                const float v = L.t[kk] + wc * C.t[((p + 1) >> 1) * C.pw + ((r + 1) >> 1)];
-                return L.d[kk] == 0.0f ? 0.0f : v;
-            };
-            const int iE = (per && i == L.nx) ? 1 : i + 1;
-            const int iW = (per && i == 1) ? L.nx : i - 1;
-            const float y = corr(i, j);
-            const float Ay = d * y - L.e[k] * corr(iE, j) - L.e[iW * L.pw + j] * corr(iW, j) -
-                             L.n[k] * corr(i, j + 1) - L.n[k - 1] * corr(i, j - 1);
-            L.x[k] = y + w * (L.b[k] - Ay) / d;
-        }
+        // y = t + wc t_(l+1) in place on the fluid cells (0 stays on the
+        // others, as k_lv_post's corr has it), then the two sweeps
+        vc_cells(
+            L.nx * L.ny,
+            [&](int c) {
+                int i, j;
+                vc_ij(L, c, i, j);
+                VcCell o;
+                o.k = i * L.pw + j;
+                o.d = L.d[o.k];
+                o.v = L.t[o.k] + wc * C.t[((i + 1) >> 1) * C.pw + ((j + 1) >> 1)];
+                return o;
+            },
+            [&](const VcCell& o) {
+                if (o.d != 0.0f) L.t[o.k] = o.v;
+            });
         __syncthreads();
+        vc_smooth(L, per, w, L.t, L.x);
         vc_smooth(L, per, w, L.x, L.t);
     }
     // level Lc's t for the prolongation launch of level Lc - 1
@@ -1495,7 +1555,7 @@ __global__ void __launch_bounds__(1024) k_lv_vcoarse(VcArgs a) {
             int i, j;
             vc_ij(L, c, i, j);
             const int k = i * L.pw + j;
-            if (L.d[k] != 0.0f) G.gt[inst * G.istride + ix(i, j, G.P)] = L.t[k];
+            if (!done && L.d[k] != 0.0f) G.gt[inst * G.istride + ix(i, j, G.P)] = L.t[k];
         }
     }
 }
