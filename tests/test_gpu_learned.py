@@ -138,6 +138,47 @@ def test_config2_velocity_error_against_fp64_cg(gpu):
     assert eu < 1e-5 and ev < 1e-5 and ec < 1e-6
 
 
+def test_config2_full_run(gpu):
+    """The benchmark workload for all 500 steps of its configuration, on the
+    headline solver (flexible CG with the complete V-cycle, extrapolated
+    start): every solve ends on a true relative residual <= 1e-6 with no
+    fallback; the TVD scalar keeps the bounds of its initial field (van Leer
+    with diffusion and closed walls: no new extrema, up to 1e-5 of the range
+    for the divergence the solves leave); the velocities, the
+    pressure and the scalar after 500 steps agree with the multigrid FCG
+    solved to 1e-10 (FP64 vectors and dot products) to 1e-5, 1e-4 of the
+    pressure range and 1e-6."""
+    from bench import build_case
+
+    steps = 500
+    print(f"config C2 full run: 1024^2, {steps} steps, cg_learned_vcycle vs cg_mixed 1e-10",
+          flush=True)
+    a = build_case("cg_learned_vcycle").sim
+    c0 = a.scalars["c"].cell()
+    lo, hi = float(np.nanmin(c0)), float(np.nanmax(c0))
+    ref = build_case("cg_mixed").sim
+    ref.pressure.tol = 1e-10
+    a.sync_every = ref.sync_every = 100
+    _run(a, steps)
+    _run(ref, steps)
+    c = a.scalars["c"].cell()
+    eu, ev, ep = _cmp(a.u, ref.u), _cmp(a.v, ref.v), _cmp(a.p, ref.p)
+    ec = _cmp(a.scalars["c"].phi, ref.scalars["c"].phi)
+    prange = float(ref.p.max() - ref.p.min())
+    print(f"1024^2 after {steps} steps: iterations mean {np.mean(a.cg_iterations):.2f}, "
+          f"max|du| {eu:.2e} max|dv| {ev:.2e} max|dp| {ep:.2e} (p range {prange:.3g}) "
+          f"max|dc| {ec:.2e}; scalar in [{np.nanmin(c):.3e}, {np.nanmax(c):.6f}] "
+          f"(initial [{lo:.3e}, {hi:.6f}])")
+    assert a.n == steps and len(a.cg_relres) == steps
+    assert max(a.cg_relres) <= 1e-6 and all(i == 0 for i in a.cg_info)
+    tol = 1e-5 * (hi - lo)   # the solves leave a divergence of the order of rtol
+    assert np.nanmin(c) >= lo - tol and np.nanmax(c) <= hi + tol
+    assert eu < 1e-5 and ev < 1e-5 and ec < 1e-6
+    assert ep < 1e-4 * prange
+    a.close()
+    ref.close()
+
+
 def test_harvested_pairs_solve_the_coarse_system(gpu):
     """Teacher pairs (input x, target t): A_K t / D = x to the teacher's
     accuracy on the coarse level K (its Galerkin matrix from the hierarchy)."""

@@ -125,7 +125,7 @@ def test_config1_full_trajectory(gpu):
     assert np.allclose(sim.history.kinetic_energy, g["h_ke"], rtol=1e-6)
 
 
-def _config3_case(nx, ny, solver, seed=0):
+def _config3_case(nx, ny, solver, seed=0, learned=None):
     from cfd2d_amd import cases
 
     rng = np.random.default_rng(seed)
@@ -136,7 +136,8 @@ def _config3_case(nx, ny, solver, seed=0):
                  for x, y, r in zip(xs, ys, rs)]
     case = cases.custom_obstacles(obstacles, lx=4.0, ly=1.0, nx=nx, ny=ny, re=200, L_ref=1.0,
                                   inflow_profile="parabolic", pressure_solver=solver,
-                                  pressure_tol=1e-6 if solver == "cg_mixed" else 1e-10)
+                                  pressure_tol=1e-10 if solver == "direct" else 1e-6,
+                                  learned=learned)
     cases.add_scalar(case, "c", 1e-3, inflow_band=(0.25, 0.75), limiter="superbee")
     return case, (xs, ys, rs)
 
@@ -161,17 +162,23 @@ def test_config3_reduced_matches_oracle(gpu):
     assert max(sim.cg_relres) <= 1e-6
 
 
-def test_config3_full_size_invariants(gpu):
-    """2048x512 obstacle channel, 200 steps of the mixed solver: every solve
+@pytest.mark.parametrize("solver", ["cg_mixed", "cg_learned_vcycle"])
+def test_config3_full_size_invariants(gpu, solver):
+    """2048x512 obstacle channel, the configuration's full 1000 steps of the
+    mixed-precision solvers (the multigrid FCG, and the learned solver's FCG
+    with the complete V-cycle and the extrapolated start): every solve
     reaches the true residual, the scalar stays within [0, 1], inflow equals
     outflow."""
-    case, _ = _config3_case(2048, 512, "cg_mixed")
+    learned = {"mode": "vcycle", "x0": "extrapolate"} if solver == "cg_learned_vcycle" else None
+    case, _ = _config3_case(2048, 512, "cg_learned" if learned else solver, learned=learned)
     sim = case.sim
-    sim.sync_every = 50
-    for _ in range(200):
+    sim.sync_every = 100
+    print(f"config C3 full run: 2048x512 obstacle channel, 1000 steps, {solver}", flush=True)
+    for _ in range(1000):
         sim.step()
     sim._flush()
-    assert max(sim.cg_relres) <= 1e-6
+    assert sim.n == 1000 and len(sim.cg_relres) == 1000
+    assert max(sim.cg_relres) <= 1e-6 and all(i == 0 for i in sim.cg_info)
     c = sim.scalars["c"].cell()
     assert np.nanmin(c) >= -1e-9 and np.nanmax(c) <= 1 + 1e-9
     u = sim.u
