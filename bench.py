@@ -304,7 +304,8 @@ def run_ensemble(steps: int = 10, warmup: int = 3, net: bool = True,
                         "bytes_per_step": bytes_step},
            "predictor_shapes": {str(ins[c[0]].n): [ens.stats(c[0])["pred_tx"],
                                                     ens.stats(c[0])["pred_ty"]]
-                                for c in ens.classes}}
+                                for c in ens.classes},
+           "tuning": [ens.tuning(c) for c in range(len(ens.classes))]}
     ens.close()
     return out
 

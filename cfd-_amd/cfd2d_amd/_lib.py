@@ -129,6 +129,15 @@ class EnsStats(ctypes.Structure):
                 ("pred_ty", _i32), ("pred_us", ctypes.c_float)]
 
 
+class EnsTuning(ctypes.Structure):
+    _fields_ = [("pred_tx", _i32), ("pred_ty", _i32), ("nlev", _i32), ("vec_batch", _i32),
+                ("sweep_tx", _i32 * ENS_MAX_LEVELS), ("sweep_ty", _i32 * ENS_MAX_LEVELS),
+                ("pred_us", ctypes.c_float), ("vec_us", ctypes.c_float),
+                ("vec_default_us", ctypes.c_float), ("pad", ctypes.c_float),
+                ("sweep_us", ctypes.c_float * ENS_MAX_LEVELS),
+                ("sweep_default_us", ctypes.c_float * ENS_MAX_LEVELS)]
+
+
 class State(ctypes.Structure):
     _fields_ = [("t", _d), ("dt", _d), ("dt_old", _d), ("gamma", _d), ("flow_force", _d),
                 ("n", _i64), ("has_old", _i32), ("done", _i32), ("error", _i32), ("pad", _i32)]
@@ -246,6 +255,7 @@ def lib() -> ctypes.CDLL:
         "cfd_ens_step": ([_vp, _i32], _i32),
         "cfd_ens_download": ([_vp, _i32, _i32, _i32, _vp], _i32),
         "cfd_ens_stats": ([_vp, _i32, _i32, P(EnsStats)], _i32),
+        "cfd_ens_tuning": ([_vp, _i32, P(EnsTuning)], _i32),
         "cfd_unet_param_count": ([], _i64),
         "cfd_unet_create": ([_i32, _i32, P(_vp)], _i32),
         "cfd_unet_destroy": ([_vp], _i32),
@@ -287,7 +297,7 @@ def exported_symbols() -> list[str]:
                         "cfd_set_learned",
                         "cfd_learned_stats", "cfd_learned_reset", "cfd_ens_create",
                         "cfd_ens_destroy", "cfd_ens_set_net", "cfd_ens_set_coarse",
-                        "cfd_ens_step", "cfd_ens_download",
+                        "cfd_ens_step", "cfd_ens_download", "cfd_ens_tuning",
                         "cfd_ens_stats", "cfd_unet_param_count",
                         "cfd_unet_create", "cfd_unet_destroy", "cfd_unet_set_params",
                         "cfd_unet_get_params", "cfd_unet_get_grad", "cfd_unet_zero_grad",

@@ -15,8 +15,10 @@ solutions (``x0="extrapolate"``; "previous": from the last one).  With
 on the device (a step that needs more iterations than the graph holds
 continues in the next launch).  ``coarse="vcycle"`` replaces the network by the V-cycle carried on to the
 dense coarsest solve (levels of at most 64 x 64 cells in LDS, one work-group
-per instance).  The predictor's work-group shape is chosen per class by
-timing at setup (``autotune``).
+per instance).  With ``autotune`` the work-group shapes are chosen per class
+by timing at setup: the predictor's, the pointwise V-cycle kernels' per
+level, and the cells per thread of the FP64 vector kernels (``tuning``); no
+choice changes a computed value.
 """
 
 from __future__ import annotations
@@ -183,6 +185,23 @@ class Ensemble:
         s = _lib.EnsStats()
         _lib.check(_lib.lib().cfd_ens_stats(self._h, c, b, ctypes.byref(s)), "stats")
         return {name: getattr(s, name) for name, _t in _lib.EnsStats._fields_}
+
+    def tuning(self, cls: int) -> dict:
+        """What the autotuner chose for size class ``cls`` (index into
+        ``classes``): the predictor's work-group shape, the work-group shape of
+        the pointwise V-cycle kernels per level, the FP64 vector kernels' cells
+        per thread and batch, and the time per launch of each choice and of the
+        default shape."""
+        t = _lib.EnsTuning()
+        _lib.check(_lib.lib().cfd_ens_tuning(self._h, int(cls), ctypes.byref(t)), "tuning")
+        nl = t.nlev
+        return {"n": self.instances[self.classes[cls][0]].n,
+                "predictor": [t.pred_tx, t.pred_ty], "predictor_us": round(t.pred_us, 2),
+                "sweeps": [[t.sweep_tx[l], t.sweep_ty[l]] for l in range(nl)],
+                "sweeps_us": [round(t.sweep_us[l], 2) for l in range(nl)],
+                "sweeps_default_us": [round(t.sweep_default_us[l], 2) for l in range(nl)],
+                "vector_batch": t.vec_batch, "vector_us": round(t.vec_us, 2),
+                "vector_default_us": round(t.vec_default_us, 2)}
 
     def close(self) -> None:
         if self._h:

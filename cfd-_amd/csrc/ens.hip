@@ -466,11 +466,15 @@ __global__ void __launch_bounds__(256) k_e_init(const EnsClass* __restrict__ c) 
 }
 
 // ---- batched V-cycle and flexible CG -------------------------------------
-// one cell per thread of level l: work-group = (row, 256-cell chunk), y = instance
-#define ENS_CELL(L)                                                  \
-    const int cj_ = ((L).n + 255) >> 8;                              \
-    const int i = blockIdx.x / cj_ + 1;                              \
-    const int j = (blockIdx.x - (i - 1) * cj_) * 256 + threadIdx.x + 1; \
+// one cell per thread of level l: work-group = a tile of blockDim.y rows x
+// blockDim.x columns (the shape the autotuner picked for the class and level;
+// every shape gives the same values), blockIdx.x = tile (row-major over the
+// level), y = instance
+#define ENS_CELL(L)                                                           \
+    const int cj_ = ((L).n + (int)blockDim.x - 1) / (int)blockDim.x;          \
+    const int ti_ = blockIdx.x / cj_;                                         \
+    const int i = ti_ * (int)blockDim.y + (int)threadIdx.y + 1;               \
+    const int j = (blockIdx.x - ti_ * cj_) * (int)blockDim.x + (int)threadIdx.x + 1; \
     if (i > (L).n || j > (L).n) return
 
 __global__ void __launch_bounds__(256) k_e_smooth(const EnsClass* __restrict__ c, int l,
@@ -494,9 +498,10 @@ __global__ void __launch_bounds__(256) k_e_restrict(const EnsClass* __restrict__
     if (c->st[b].done) return;
     const EnsLevel& L = c->lev[l];
     const EnsLevel& C = c->lev[l + 1];
-    const int cj_ = (C.n + 255) >> 8;
-    const int ci = blockIdx.x / cj_ + 1;
-    const int cjj = (blockIdx.x - (ci - 1) * cj_) * 256 + threadIdx.x + 1;
+    const int cj_ = (C.n + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int ti_ = blockIdx.x / cj_;
+    const int ci = ti_ * (int)blockDim.y + (int)threadIdx.y + 1;
+    const int cjj = (blockIdx.x - ti_ * cj_) * (int)blockDim.x + (int)threadIdx.x + 1;
     if (ci > C.n || cjj > C.n) return;
     const long o = b * L.S, oc = b * C.S;
     float bc = 0.0f;
@@ -617,6 +622,7 @@ __global__ void __launch_bounds__(256) k_e_net_out(const EnsClass* __restrict__ 
 }
 
 // last sweep on the fine level with the partial dot products r.z, r_old.z
+template <int U>
 __global__ void __launch_bounds__(256) k_e_smooth_dots(const EnsClass* __restrict__ c) {
     const int b = blockIdx.y;
     const EnsState& st = c->st[b];
@@ -630,7 +636,7 @@ __global__ void __launch_bounds__(256) k_e_smooth_dots(const EnsClass* __restric
         float z;
         double r, ro;
     };
-    ens_cells_batch<8>(
+    ens_cells_batch<U>(
         n,
         [&](int i, int j) {
             const float d = lev_diag(c, 0, L, i, j);
@@ -652,6 +658,7 @@ __global__ void __launch_bounds__(256) k_e_smooth_dots(const EnsClass* __restric
 }
 
 // d = z + beta d_old, q = A d, partial d.q
+template <int U>
 __global__ void __launch_bounds__(256) k_e_dir(const EnsClass* __restrict__ c) {
     const int b = blockIdx.y;
     EnsState& st = c->st[b];
@@ -678,7 +685,7 @@ __global__ void __launch_bounds__(256) k_e_dir(const EnsClass* __restrict__ c) {
     // two variants keep the loads free of branches
     auto sweep = [&](auto first_c) {
         constexpr bool FIRST = decltype(first_c)::value;
-        ens_cells_batch<8>(
+        ens_cells_batch<U>(
             n,
             [&](int i, int j) {
                 auto dirv = [&](int p, int q) -> double {
@@ -704,6 +711,7 @@ __global__ void __launch_bounds__(256) k_e_dir(const EnsClass* __restrict__ c) {
 
 // alpha = rho / d.q; x += alpha d, r_old = r, r -= alpha q; next V-cycle input;
 // partial r.r
+template <int U>
 __global__ void __launch_bounds__(256) k_e_update(const EnsClass* __restrict__ c) {
     const int b = blockIdx.y;
     EnsState& st = c->st[b];
@@ -726,7 +734,7 @@ __global__ void __launch_bounds__(256) k_e_update(const EnsClass* __restrict__ c
     struct Up {
         double x, r, rn;
     };
-    ens_cells_batch<8>(
+    ens_cells_batch<U>(
         n,
         [&](int i, int j) {
             const long k = o + ix(i, j, P);
@@ -751,6 +759,7 @@ __global__ void __launch_bounds__(256) k_e_update(const EnsClass* __restrict__ c
 
 // recursive-residual test of the iteration (every band; band 0 records), and
 // the true residual r = b - A x once it passes or at maxiter; partial r.r
+template <int U>
 __global__ void __launch_bounds__(256) k_e_true(const EnsClass* __restrict__ c) {
     const int b = blockIdx.y;
     EnsState& st = c->st[b];
@@ -777,7 +786,7 @@ __global__ void __launch_bounds__(256) k_e_true(const EnsClass* __restrict__ c) 
     const EnsLevel& L0 = c->lev[0];
     const double* x = c->x + o;
     double s[1] = {0.0};
-    ens_cells_batch<8>(
+    ens_cells_batch<U>(
         n,
         [&](int i, int j) {
             const double d = c->axd * diag_count(i, j, n);

@@ -8,12 +8,20 @@
 // completed steps once per cfd_ens_step call and adapts `cap` to the
 // iterations the steps took.
 
+static_assert(ENS_MAXLEV == CFD_ENS_MAX_LEVELS, "level tables of the tuning record");
+
 struct cfd_ens {
     int ncls = 0;
     EnsClass h[CFD_ENS_MAX_CLASSES];
     EnsClass* d = nullptr;              // device copies
     int pred_tx[CFD_ENS_MAX_CLASSES], pred_ty[CFD_ENS_MAX_CLASSES];
     float pred_us[CFD_ENS_MAX_CLASSES];
+    // work-group shape of the pointwise V-cycle kernels per class and level,
+    // and the FP64 vector kernels' cells per thread and batch per class
+    int sw_tx[CFD_ENS_MAX_CLASSES][ENS_MAXLEV], sw_ty[CFD_ENS_MAX_CLASSES][ENS_MAXLEV];
+    float sw_us[CFD_ENS_MAX_CLASSES][ENS_MAXLEV], sw_us0[CFD_ENS_MAX_CLASSES][ENS_MAXLEV];
+    int vec_u[CFD_ENS_MAX_CLASSES];
+    float vec_us[CFD_ENS_MAX_CLASSES], vec_us0[CFD_ENS_MAX_CLASSES];
     int nimg = 0;
     float* img = nullptr;               // network input batch (nimg x 32 x 32)
     cfd_unet* net = nullptr;
@@ -55,7 +63,38 @@ int ealloc(cfd_ens* e, T** p, size_t n) {
         HIPC(hipGetLastError());                                                      \
     } while (0)
 
-unsigned cells1(const EnsLevel& L) { return (unsigned)(L.n * ((L.n + 255) / 256)); }
+// grid and work-group of the pointwise V-cycle kernels on level l of class k
+// (tiles of sw_ty rows x sw_tx columns, y = instance)
+dim3 sw_grid(const cfd_ens* e, int k, int l) {
+    const int n = e->h[k].lev[l].n, tx = e->sw_tx[k][l], ty = e->sw_ty[k][l];
+    return dim3((unsigned)(((n + ty - 1) / ty) * ((n + tx - 1) / tx)), (unsigned)e->h[k].B);
+}
+dim3 sw_block(const cfd_ens* e, int k, int l) { return dim3(e->sw_tx[k][l], e->sw_ty[k][l]); }
+
+// the FP64 vector kernels with the class's cells per thread and batch
+#define E_LAUNCH_VEC(kern, u, grid, block, ...)                       \
+    do {                                                              \
+        switch (u) {                                                  \
+        case 2: E_LAUNCH(kern<2>, grid, block, __VA_ARGS__); break;   \
+        case 4: E_LAUNCH(kern<4>, grid, block, __VA_ARGS__); break;   \
+        case 16: E_LAUNCH(kern<16>, grid, block, __VA_ARGS__); break; \
+        default: E_LAUNCH(kern<8>, grid, block, __VA_ARGS__); break;  \
+        }                                                             \
+    } while (0)
+
+// the fine level's last sweep with the dot products and the flexible CG's
+// vector kernels of class k
+int ens_fcg_tail(cfd_ens* e, int k) {
+    const EnsClass& c = e->h[k];
+    EnsClass* dc = e->d + k;
+    const int u = e->vec_u[k];
+    E_LAUNCH_VEC(k_e_smooth_dots, u, dim3(ENS_BANDS, c.B), dim3(256), dc);
+    E_LAUNCH_VEC(k_e_dir, u, dim3(ENS_BANDS, c.B), dim3(256), dc);
+    E_LAUNCH_VEC(k_e_update, u, dim3(ENS_BANDS, c.B), dim3(256), dc);
+    E_LAUNCH_VEC(k_e_true, u, dim3(ENS_BANDS, c.B), dim3(256), dc);
+    E_LAUNCH(k_e_true_fin, dim3(1), dim3(256), dc);
+    return 0;
+}
 
 int ens_pred(cfd_ens* e, int k, int tx, int ty) {
     const EnsClass& c = e->h[k];
@@ -72,20 +111,16 @@ int ens_iteration_vcycle(cfd_ens* e) {
         EnsClass* dc = e->d + k;
         const int Lc = e->vlc[k];
         for (int l = 0; l < Lc; ++l) {
-            E_LAUNCH(k_e_smooth, dim3(cells1(c.lev[l]), c.B), dim3(256), dc, l, 0);
-            E_LAUNCH(k_e_restrict, dim3(cells1(c.lev[l + 1]), c.B), dim3(256), dc, l);
+            E_LAUNCH(k_e_smooth, sw_grid(e, k, l), sw_block(e, k, l), dc, l, 0);
+            E_LAUNCH(k_e_restrict, sw_grid(e, k, l + 1), sw_block(e, k, l + 1), dc, l);
         }
         E_LAUNCH(k_lv_vcoarse, dim3(c.B), dim3(1024), e->vargs[k]);
         for (int l = Lc - 1; l >= 1; --l) {
-            E_LAUNCH(k_e_post, dim3(cells1(c.lev[l]), c.B), dim3(256), dc, l);
-            E_LAUNCH(k_e_smooth, dim3(cells1(c.lev[l]), c.B), dim3(256), dc, l, 0);
+            E_LAUNCH(k_e_post, sw_grid(e, k, l), sw_block(e, k, l), dc, l);
+            E_LAUNCH(k_e_smooth, sw_grid(e, k, l), sw_block(e, k, l), dc, l, 0);
         }
-        E_LAUNCH(k_e_post, dim3(cells1(c.lev[0]), c.B), dim3(256), dc, 0);
-        E_LAUNCH(k_e_smooth_dots, dim3(ENS_BANDS, c.B), dim3(256), dc);
-        E_LAUNCH(k_e_dir, dim3(ENS_BANDS, c.B), dim3(256), dc);
-        E_LAUNCH(k_e_update, dim3(ENS_BANDS, c.B), dim3(256), dc);
-        E_LAUNCH(k_e_true, dim3(ENS_BANDS, c.B), dim3(256), dc);
-        E_LAUNCH(k_e_true_fin, dim3(1), dim3(256), dc);
+        E_LAUNCH(k_e_post, sw_grid(e, k, 0), sw_block(e, k, 0), dc, 0);
+        TRY(ens_fcg_tail(e, k));
     }
     return 0;
 }
@@ -96,8 +131,8 @@ int ens_iteration(cfd_ens* e) {
         const EnsClass& c = e->h[k];
         EnsClass* dc = e->d + k;
         for (int l = 0; l < c.K; ++l) {
-            E_LAUNCH(k_e_smooth, dim3(cells1(c.lev[l]), c.B), dim3(256), dc, l, 0);
-            E_LAUNCH(k_e_restrict, dim3(cells1(c.lev[l + 1]), c.B), dim3(256), dc, l);
+            E_LAUNCH(k_e_smooth, sw_grid(e, k, l), sw_block(e, k, l), dc, l, 0);
+            E_LAUNCH(k_e_restrict, sw_grid(e, k, l + 1), sw_block(e, k, l + 1), dc, l);
         }
         E_LAUNCH(k_e_cmode, dim3(ENS_BANDS, c.B), dim3(256), dc);
         E_LAUNCH(k_e_net_in, dim3(4, c.B), dim3(256), dc);
@@ -113,20 +148,19 @@ int ens_iteration(cfd_ens* e) {
     for (int k = 0; k < e->ncls; ++k) {
         const EnsClass& c = e->h[k];
         EnsClass* dc = e->d + k;
-        const EnsLevel& LK = c.lev[c.K];
-        E_LAUNCH(k_e_net_out, dim3(cells1(LK), c.B), dim3(256), dc);
-        E_LAUNCH(k_e_smooth, dim3(cells1(LK), c.B), dim3(256), dc, c.K, 1);
-        E_LAUNCH(k_e_smooth, dim3(cells1(LK), c.B), dim3(256), dc, c.K, 0);
-        for (int l = c.K - 1; l >= 1; --l) {
-            E_LAUNCH(k_e_post, dim3(cells1(c.lev[l]), c.B), dim3(256), dc, l);
-            E_LAUNCH(k_e_smooth, dim3(cells1(c.lev[l]), c.B), dim3(256), dc, l, 0);
+        const int K = c.K;
+        // k_e_net_out's whole work-group computes the coarse statistics before
+        // any thread leaves: it keeps 256 x 1 work-groups
+        E_LAUNCH(k_e_net_out, dim3((unsigned)(c.lev[K].n * ((c.lev[K].n + 255) / 256)), c.B),
+                 dim3(256), dc);
+        E_LAUNCH(k_e_smooth, sw_grid(e, k, K), sw_block(e, k, K), dc, K, 1);
+        E_LAUNCH(k_e_smooth, sw_grid(e, k, K), sw_block(e, k, K), dc, K, 0);
+        for (int l = K - 1; l >= 1; --l) {
+            E_LAUNCH(k_e_post, sw_grid(e, k, l), sw_block(e, k, l), dc, l);
+            E_LAUNCH(k_e_smooth, sw_grid(e, k, l), sw_block(e, k, l), dc, l, 0);
         }
-        E_LAUNCH(k_e_post, dim3(cells1(c.lev[0]), c.B), dim3(256), dc, 0);
-        E_LAUNCH(k_e_smooth_dots, dim3(ENS_BANDS, c.B), dim3(256), dc);
-        E_LAUNCH(k_e_dir, dim3(ENS_BANDS, c.B), dim3(256), dc);
-        E_LAUNCH(k_e_update, dim3(ENS_BANDS, c.B), dim3(256), dc);
-        E_LAUNCH(k_e_true, dim3(ENS_BANDS, c.B), dim3(256), dc);
-        E_LAUNCH(k_e_true_fin, dim3(1), dim3(256), dc);
+        E_LAUNCH(k_e_post, sw_grid(e, k, 0), sw_block(e, k, 0), dc, 0);
+        TRY(ens_fcg_tail(e, k));
     }
     return 0;
 }
@@ -289,24 +323,42 @@ int ens_steps(cfd_ens* e, int count) {
     return ens_graph_ready(e);   // the next call starts without a capture
 }
 
-// Autotuner: the predictor's work-group shape per class, the fastest of the
-// candidates over 5 timed launches each (the fields are untouched: the
-// predictor only writes F, G and the tendency buffer of this parity).
+// Autotuner, run at creation before any step, on the class's own arrays:
+//   - the predictor's work-group shape per class (it writes only F, G and the
+//     tendency buffer of this parity, all zero from zero fields);
+//   - per class and level the work-group shape of the pointwise V-cycle
+//     kernels (smoothing, restriction, prolongation), timed on the level's
+//     sweep;
+//   - per class the cells per thread and batch of the FP64 vector kernels
+//     (the fine sweep with the dot products, direction, update, true
+//     residual), timed on the first two.
+// Each candidate is the fastest of 5 timed launches after one untimed; every
+// candidate computes the same values.  The timed kernels write the levels'
+// b, x, t, the directions, q, the partial sums and the solver state, which
+// are cleared afterwards: the class starts from its state at creation.
 int ens_autotune(cfd_ens* e) {
-    static const int cand[][2] = {{64, 4}, {128, 2}, {256, 1}, {64, 8}, {128, 4}, {32, 8}};
+    static const int pcand[][2] = {{64, 4}, {128, 2}, {256, 1}, {64, 8}, {128, 4}, {32, 8}};
+    static const int scand[][2] = {{256, 1}, {128, 2}, {64, 4}, {32, 8}, {16, 16}};
+    static const int ucand[] = {8, 2, 4, 16};   // the default first
     hipEvent_t a, b;
     HIPC(hipEventCreate(&a));
     HIPC(hipEventCreate(&b));
-    for (int k = 0; k < e->ncls; ++k) {
-        float best = 1e30f;
-        for (auto& cd : cand) {
-            TRY(ens_pred(e, k, cd[0], cd[1]));   // warm
-            HIPC(hipEventRecord(a, e->stream));
-            for (int r = 0; r < 5; ++r) TRY(ens_pred(e, k, cd[0], cd[1]));
-            HIPC(hipEventRecord(b, e->stream));
-            HIPC(hipEventSynchronize(b));
-            float ms = 0.f;
-            HIPC(hipEventElapsedTime(&ms, a, b));
+    auto timed = [&](auto launch, float* ms) -> int {
+        TRY(launch());   // warm
+        HIPC(hipEventRecord(a, e->stream));
+        for (int r = 0; r < 5; ++r) TRY(launch());
+        HIPC(hipEventRecord(b, e->stream));
+        HIPC(hipEventSynchronize(b));
+        HIPC(hipEventElapsedTime(ms, a, b));
+        return 0;
+    };
+    int rc = 0;
+    for (int k = 0; k < e->ncls && !rc; ++k) {
+        const EnsClass& c = e->h[k];
+        EnsClass* dc = e->d + k;
+        float best = 1e30f, ms = 0.f;
+        for (auto& cd : pcand) {
+            if ((rc = timed([&] { return ens_pred(e, k, cd[0], cd[1]); }, &ms))) break;
             if (ms < best) {
                 best = ms;
                 e->pred_tx[k] = cd[0];
@@ -314,9 +366,67 @@ int ens_autotune(cfd_ens* e) {
             }
         }
         e->pred_us[k] = 1e3f * best / 5;
+        for (int l = 0; l < c.nlev && !rc; ++l) {
+            best = 1e30f;
+            int btx = 256, bty = 1;
+            for (auto& cd : scand) {
+                e->sw_tx[k][l] = cd[0];
+                e->sw_ty[k][l] = cd[1];
+                if ((rc = timed([&] {
+                         E_LAUNCH(k_e_smooth, sw_grid(e, k, l), sw_block(e, k, l), dc, l, 0);
+                         return 0;
+                     }, &ms)))
+                    break;
+                if (cd[1] == 1) e->sw_us0[k][l] = 1e3f * ms / 5;   // the default 256 x 1
+                if (ms < best) {
+                    best = ms;
+                    btx = cd[0];
+                    bty = cd[1];
+                }
+            }
+            e->sw_tx[k][l] = btx;
+            e->sw_ty[k][l] = bty;
+            e->sw_us[k][l] = 1e3f * best / 5;
+        }
+        if (rc) break;
+        best = 1e30f;
+        int bu = 8;
+        for (int u : ucand) {
+            e->vec_u[k] = u;
+            if ((rc = timed([&] {
+                     E_LAUNCH_VEC(k_e_smooth_dots, e->vec_u[k], dim3(ENS_BANDS, c.B), dim3(256), dc);
+                     E_LAUNCH_VEC(k_e_dir, e->vec_u[k], dim3(ENS_BANDS, c.B), dim3(256), dc);
+                     return 0;
+                 }, &ms)))
+                break;
+            if (u == 8) e->vec_us0[k] = 1e3f * ms / 5;
+            if (ms < best) {
+                best = ms;
+                bu = u;
+            }
+        }
+        e->vec_u[k] = bu;
+        e->vec_us[k] = 1e3f * best / 5;
     }
     hipEventDestroy(a);
     hipEventDestroy(b);
+    if (rc) return rc;
+    // the class's state at creation
+    for (int k = 0; k < e->ncls; ++k) {
+        const EnsClass& c = e->h[k];
+        const size_t N = (size_t)c.S * c.B;
+        for (int l = 0; l < c.nlev; ++l) {
+            const size_t nl = (size_t)c.lev[l].S * c.B * sizeof(float);
+            HIPC(hipMemsetAsync(c.lev[l].b, 0, nl, e->stream));
+            HIPC(hipMemsetAsync(c.lev[l].x, 0, nl, e->stream));
+            HIPC(hipMemsetAsync(c.lev[l].t, 0, nl, e->stream));
+        }
+        for (double* p : {c.q, c.d0, c.d1}) HIPC(hipMemsetAsync(p, 0, N * sizeof(double), e->stream));
+        HIPC(hipMemsetAsync(c.part, 0, (size_t)ENS_NSLOT * c.B * ENS_BANDS * ENS_NV * sizeof(double),
+                            e->stream));
+        HIPC(hipMemsetAsync(c.st, 0, (size_t)c.B * sizeof(EnsState), e->stream));
+    }
+    HIPC(hipStreamSynchronize(e->stream));
     return 0;
 }
 
@@ -454,6 +564,13 @@ int cfd_ens_create(const cfd_ens_desc* d, cfd_ens** out) {
         e->h[k].ctl = e->ctl;
         e->pred_tx[k] = 256;
         e->pred_ty[k] = 1;
+        for (int l = 0; l < ENS_MAXLEV; ++l) {
+            e->sw_tx[k][l] = 256;
+            e->sw_ty[k][l] = 1;
+            e->sw_us[k][l] = e->sw_us0[k][l] = 0.f;
+        }
+        e->vec_u[k] = 8;
+        e->vec_us[k] = e->vec_us0[k] = 0.f;
     }
     if (hipMemcpy(e->d, e->h, e->ncls * sizeof(EnsClass), hipMemcpyHostToDevice) != hipSuccess)
         return fail(set_err(CFD_ERR_HIP, "class upload failed"));
@@ -608,6 +725,25 @@ int cfd_ens_stats(cfd_ens* e, int32_t cls, int32_t inst, cfd_ens_stats_t* out) {
     EnsCtl h;
     TRY(ens_read_ctl(e, &h));
     out->sweeps_total = h.iters;
+    return 0;
+}
+
+int cfd_ens_tuning(cfd_ens* e, int32_t cls, cfd_ens_tuning_t* out) {
+    if (!e || !out || cls < 0 || cls >= e->ncls) return set_err(CFD_ERR_ARG, "bad arguments");
+    std::memset(out, 0, sizeof(*out));
+    out->pred_tx = e->pred_tx[cls];
+    out->pred_ty = e->pred_ty[cls];
+    out->pred_us = e->pred_us[cls];
+    out->nlev = e->h[cls].nlev;
+    for (int l = 0; l < e->h[cls].nlev; ++l) {
+        out->sweep_tx[l] = e->sw_tx[cls][l];
+        out->sweep_ty[l] = e->sw_ty[cls][l];
+        out->sweep_us[l] = e->sw_us[cls][l];
+        out->sweep_default_us[l] = e->sw_us0[cls][l];
+    }
+    out->vec_batch = e->vec_u[cls];
+    out->vec_us = e->vec_us[cls];
+    out->vec_default_us = e->vec_us0[cls];
     return 0;
 }
 

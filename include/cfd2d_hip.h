@@ -503,7 +503,7 @@ typedef struct cfd_ens_desc {
     double tau, rtol;
     int32_t maxiter;
     float omega, wc;           /* damped-Jacobi weight, coarse-correction weight */
-    int32_t autotune;          /* 1: time the predictor's work-group shapes per class */
+    int32_t autotune;          /* 1: time work-group shapes per class (cfd_ens_tuning) */
     int32_t use_graph;         /* 1: replay each solver iteration as a captured HIP graph */
     int32_t x0_extrapolate;    /* 1: each solve starts from the linear extrapolation in time
                                   of the instance's last two solutions */
@@ -534,6 +534,20 @@ int cfd_ens_step(cfd_ens* ens, int32_t count);
 /* field 0 u, 1 v, 2 p, 3 T of one instance, reference layout, float */
 int cfd_ens_download(cfd_ens* ens, int32_t cls, int32_t inst, int32_t field, float* host);
 int cfd_ens_stats(cfd_ens* ens, int32_t cls, int32_t inst, cfd_ens_stats_t* out);
+/* What the autotuner chose for class `cls` (desc.autotune; the defaults
+   otherwise): the predictor's work-group shape; per level the work-group
+   shape (sweep_tx columns x sweep_ty rows) of the pointwise V-cycle kernels;
+   the cells per thread and batch of the FP64 vector kernels; the time per
+   launch of each choice and of the default (256 x 1, 8 cells).  The choices
+   change no computed value.  No counterpart in the reference, which has no
+   ensembles: it reports the tile shapes the autotuner picked. */
+typedef struct cfd_ens_tuning_t {
+    int32_t pred_tx, pred_ty, nlev, vec_batch;
+    int32_t sweep_tx[CFD_ENS_MAX_LEVELS], sweep_ty[CFD_ENS_MAX_LEVELS];
+    float pred_us, vec_us, vec_default_us, pad;
+    float sweep_us[CFD_ENS_MAX_LEVELS], sweep_default_us[CFD_ENS_MAX_LEVELS];
+} cfd_ens_tuning_t;
+int cfd_ens_tuning(cfd_ens* ens, int32_t cls, cfd_ens_tuning_t* out);
 
 #ifdef __cplusplus
 }

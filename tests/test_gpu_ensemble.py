@@ -63,6 +63,43 @@ def test_instance_independent_of_batch_and_graph(gpu):
     alone.close()
 
 
+@pytest.mark.parametrize("coarse", ["vcycle", "network"])
+def test_autotuned_shapes_change_no_value(gpu, coarse):
+    """The autotuner's choices (the predictor's and the V-cycle kernels'
+    work-group shapes, the FP64 vector kernels' cells per thread) leave every
+    field bit for bit as with the default shapes, and the tuning record holds
+    a valid choice for every class and level."""
+    from cfd2d_amd.ensemble import Ensemble
+
+    ins = _mixed() + [_heated_512()]
+    tuned = Ensemble(ins, coarse=coarse)
+    plain = Ensemble(ins, coarse=coarse, autotune=False)
+    tuned.step(6)
+    plain.step(6)
+    for c in range(len(tuned.classes)):
+        t = tuned.tuning(c)
+        print(t)
+        assert t["predictor"][0] * t["predictor"][1] in (256, 512)
+        assert all(tx * ty == 256 for tx, ty in t["sweeps"])
+        assert t["vector_batch"] in (2, 4, 8, 16)
+        assert t["vector_us"] <= t["vector_default_us"]
+        assert all(a <= b for a, b in zip(t["sweeps_us"], t["sweeps_default_us"]))
+        d = plain.tuning(c)
+        assert d["sweeps"] == [[256, 1]] * len(d["sweeps"]) and d["vector_batch"] == 8
+    for k in range(len(ins)):
+        for f in ("u", "v", "p"):
+            assert np.array_equal(tuned.field(k, f), plain.field(k, f)), (k, f)
+        assert tuned.stats(k)["iters"] == plain.stats(k)["iters"]
+    tuned.close()
+    plain.close()
+
+
+def _heated_512():
+    from cfd2d_amd.ensemble import Instance
+
+    return Instance("heated", 512, ra=1e6)
+
+
 def test_network_gain_in_ensemble(gpu):
     from cfd2d_amd.ensemble import Ensemble
 
@@ -92,6 +129,8 @@ def test_full_workload_256_instances(gpu):
     assert all(ens.stats(k)["info"] == 0 for k in range(0, 256, 17))
     tx = {ens.stats(c[0])["pred_tx"] * ens.stats(c[0])["pred_ty"] for c in ens.classes}
     assert tx and all(t in (256, 512) for t in tx)
+    for c in range(len(ens.classes)):
+        print(ens.tuning(c))
     print(f"256 instances, {ens.cells / 1e6:.2f} M cells: {5 * ens.cells / wall / 1e9:.3f} G "
           f"cell-updates/s")
     ens.close()

@@ -169,6 +169,7 @@ def test_ctypes_structs_match_the_c_header(tmp_path):
              "cfd_record": _lib.Record, "cfd_mg_desc": _lib.MGDesc,
              "cfd_learned_desc": _lib.LearnedDesc, "cfd_learned_stats_t": _lib.LearnedStats,
              "cfd_ens_desc": _lib.EnsDesc, "cfd_ens_stats_t": _lib.EnsStats,
+             "cfd_ens_tuning_t": _lib.EnsTuning,
              "cfd_probe_desc": _lib.ProbeDesc, "cfd_turb_desc": _lib.TurbDesc,
              "cfd_vof_desc": _lib.VofDesc}
     src = tmp_path / "sizes.c"
