@@ -782,13 +782,10 @@ __global__ void __launch_bounds__(256) k_lv_up(LegArgs a) {
         const int ib = (tile / ntj) * TI + 1 - R;
         const int jb = (tile % ntj) * TJ + 1 - R;
         leg_stage<LEV0, true, TI, TJ, R>(a, ib, jb, s_d0, sb, sd, se, sn, sy);   // y
-        __syncthreads();
-        // NU - 1 sweeps down to the first ring, the last one on the tile below
-        const float* xx = leg_sweeps<LEV0, TI, TJ, R, R - 1, NU - 1>(a, sb, sd, se, sn, sy, sx);
         constexpr int NI = TI * TJ, LI = (NI + 255) / 256;
         double rr[LI], ro[LI];
 #pragma unroll
-        for (int u = 0; u < LI; ++u) {   // the residuals' loads first
+        for (int u = 0; u < LI; ++u) {   // the residuals' loads, in flight during the sweeps
             const int e = (int)threadIdx.x + u * 256;
             const int r = e / TJ, q = e - r * TJ;
             const int i = ib + R + r, j = jb + R + q;
@@ -797,6 +794,9 @@ __global__ void __launch_bounds__(256) k_lv_up(LegArgs a) {
             rr[u] = DOTS && in ? a.r[c] : 0.0;
             ro[u] = DOTS && in && !first ? a.rold[c] : 0.0;
         }
+        __syncthreads();
+        // NU - 1 sweeps down to the first ring, the last one on the tile below
+        const float* xx = leg_sweeps<LEV0, TI, TJ, R, R - 1, NU - 1>(a, sb, sd, se, sn, sy, sx);
 #pragma unroll
         for (int u = 0; u < LI; ++u) {
             const int e = (int)threadIdx.x + u * 256;
@@ -1731,17 +1731,29 @@ __global__ void __launch_bounds__(512) k_lp_xsum(ProjArgs a, Geo g, double* part
     double cs[CFD_MAX_COMP];
 #pragma unroll
     for (int k = 0; k < CFD_MAX_COMP; ++k) cs[k] = 0.0;
-    lv_cells(g.nx, g.ny, [&](int i, int j) {
-        const long c = ix(i, j, g.pc);
-        double xv = a.x[c];
-        if (zero && (a.code[c] & 63)) {
-            a.x[c] = 0.0;
-            xv = 0.0;
-        }
-        const int cm = a.comp[c];
+    struct Xs {
+        double x;
+        int cm, fluid;
+    };
+    lv_cells_batch<4>(
+        g.nx, g.ny,
+        [&](int i, int j) {
+            const long c = ix(i, j, g.pc);
+            Xs o;
+            o.x = a.x[c];
+            o.cm = a.comp[c];
+            o.fluid = a.code[c] & 63;
+            return o;
+        },
+        [&](int i, int j, const Xs& o) {
+            double xv = o.x;
+            if (zero && o.fluid) {
+                a.x[ix(i, j, g.pc)] = 0.0;
+                xv = 0.0;
+            }
 #pragma unroll
-        for (int k = 0; k < CFD_MAX_COMP; ++k) cs[k] += (cm == k + 1) ? xv : 0.0;
-    });
+            for (int k = 0; k < CFD_MAX_COMP; ++k) cs[k] += (o.cm == k + 1) ? xv : 0.0;
+        });
     lv_partial<CFD_MAX_COMP>(cs, part + LV_P_CM * LV_SLOT);
 }
 
@@ -1752,19 +1764,24 @@ __global__ void __launch_bounds__(512) k_lp_pwrite(ProjArgs a, Geo g, LvArgs la,
     double mean[CFD_MAX_COMP];
     lp_means(a, la.part + LV_P_CM * LV_SLOT, nb, mean);
     const int nx = g.nx, ny = g.ny, pc = g.pc;
-    auto pval = [&](int i, int j) -> double {
+    auto pval = [&](int i, int j) -> double {   // loads and selects, no branches
         const long c = ix(i, j, pc);
-        if (!(a.code[c] & 63)) return 0.0;
+        const int fl = a.code[c] & 63;
         const int cm = a.comp[c];
         const double xv = a.x[c];
-        return cm ? xv - sel_comp(mean, cm) : xv;
+        const double v = cm ? xv - sel_comp(mean, cm) : xv;
+        return fl ? v : 0.0;
     };
     const int b = xcd_block(blockIdx.x, gridDim.x);
     int i0, i1;
     lp_band(nx, b, gridDim.x, i0, i1);
-    for (int i = i0; i < i1; ++i)
-        for (int j = threadIdx.x; j <= ny + 1; j += blockDim.x)
-            a.p[ix(i, j, pc)] = (j >= 1 && j <= ny) ? pval(i, j) : 0.0;
+    lv_cells_batch<4>(
+        nx, ny, [&](int i, int j) { return pval(i, j); },
+        [&](int i, int j, double v) { a.p[ix(i, j, pc)] = v; });
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {   // the ghost columns
+        a.p[ix(i, 0, pc)] = 0.0;
+        a.p[ix(i, ny + 1, pc)] = 0.0;
+    }
     if (b == 0)
         for (int j = threadIdx.x; j <= ny + 1; j += blockDim.x)
             a.p[ix(0, j, pc)] = (g.periodic && j >= 1 && j <= ny) ? pval(nx, j) : 0.0;
@@ -1774,7 +1791,10 @@ __global__ void __launch_bounds__(512) k_lp_pwrite(ProjArgs a, Geo g, LvArgs la,
 }
 
 // u = F - dt grad p, v = G - dt grad p (boundary faces as in k_project), the
-// partial max |u_new - u_old| over the faces between fluid cells
+// partial max |u_new - u_old| over the faces between fluid cells.  Thread
+// cells (i, j) of the band take the faces u(i, j) and v(i, j) (4 at a time,
+// their loads in flight together); the bottom faces v(i, 0) and the left
+// faces u(0, j) are the band's and the first band's extra work.
 __global__ void __launch_bounds__(512) k_lp_correct(ProjArgs a, Geo g, double* part) {
     if (!a.correct || lp_end_idle(a)) return;
     const double dt = a.dt_override > 0.0 ? a.dt_override : a.st->dt;
@@ -1786,42 +1806,53 @@ __global__ void __launch_bounds__(512) k_lp_correct(ProjArgs a, Geo g, double* p
     int i0, i1;
     lp_band(nx, b, gridDim.x, i0, i1);
     double res[1] = {0.0};
-    for (int i = i0; i < i1; ++i) {
-        for (int j = 1 + threadIdx.x; j <= ny; j += blockDim.x) {
-            const long cu = ix(i, j, g.pu);
-            const double F = a.F[cu];
-            double un;
-            bool uf = false;
+    struct Faces {
+        double un, vn, du, dv;
+        bool uf, vf, vset;
+    };
+    lv_cells_batch<4>(
+        nx, ny,
+        [&](int i, int j) {
+            Faces o;
+            const long cu = ix(i, j, g.pu), cv = ix(i, j, g.pv);
+            const double F = a.F[cu], G = a.G[cv];
+            o.uf = false;
             if (i < nx) {
-                uf = fluid(i, j) && fluid(i + 1, j);
-                un = uf ? F - dt * (p[ix(i + 1, j, pc)] - p[ix(i, j, pc)]) / g.dx : F;
+                o.uf = fluid(i, j) && fluid(i + 1, j);
+                o.un = o.uf ? F - dt * (p[ix(i + 1, j, pc)] - p[ix(i, j, pc)]) / g.dx : F;
             } else if (periodic) {
-                uf = fluid(nx, j) && fluid(1, j);
-                un = uf ? F - dt * (p[ix(1, j, pc)] - p[ix(nx, j, pc)]) / g.dx : F;
+                o.uf = fluid(nx, j) && fluid(1, j);
+                o.un = o.uf ? F - dt * (p[ix(1, j, pc)] - p[ix(nx, j, pc)]) / g.dx : F;
             } else {
-                un = (a.code[ix(nx, j, pc)] & 64) ? F + 2 * dt * p[ix(nx, j, pc)] / g.dx : F;
+                o.un = (a.code[ix(nx, j, pc)] & 64) ? F + 2 * dt * p[ix(nx, j, pc)] / g.dx : F;
             }
-            if (uf) res[0] = nanmax(res[0], fabs(un - a.u[cu]));
-            a.u[cu] = un;
-            if (periodic && i == nx) a.u[ix(0, j, g.pu)] = un;
-        }
-        for (int j = threadIdx.x; j <= ny; j += blockDim.x) {
-            const long cv = ix(i, j, g.pv);
+            o.du = o.uf ? fabs(o.un - a.u[cu]) : 0.0;
+            o.vf = false;
+            o.vset = true;
+            if (j < ny) {
+                o.vf = fluid(i, j) && fluid(i, j + 1);
+                o.vn = o.vf ? G - dt * (p[ix(i, j + 1, pc)] - p[ix(i, j, pc)]) / g.dy : G;
+            } else if (periodic) {
+                o.vset = false;   // solver.py:300-306 returns before the boundary faces
+                o.vn = G;
+            } else {
+                o.vn = (a.code[ix(i, ny, pc)] & 128) ? G + 2 * dt * p[ix(i, ny, pc)] / g.dy : G;
+            }
+            o.dv = o.vf ? fabs(o.vn - a.v[cv]) : 0.0;
+            return o;
+        },
+        [&](int i, int j, const Faces& o) {
+            if (o.uf) res[0] = nanmax(res[0], o.du);
+            a.u[ix(i, j, g.pu)] = o.un;
+            if (periodic && i == nx) a.u[ix(0, j, g.pu)] = o.un;
+            if (o.vf) res[0] = nanmax(res[0], o.dv);
+            if (o.vset) a.v[ix(i, j, g.pv)] = o.vn;
+        });
+    if (!periodic) {
+        for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {   // v(i, 0)
+            const long cv = ix(i, 0, g.pv);
             const double G = a.G[cv];
-            double vn;
-            bool vf = false;
-            if (j >= 1 && j < ny) {
-                vf = fluid(i, j) && fluid(i, j + 1);
-                vn = vf ? G - dt * (p[ix(i, j + 1, pc)] - p[ix(i, j, pc)]) / g.dy : G;
-            } else if (periodic) {
-                continue;   // solver.py:300-306 returns before the boundary faces
-            } else if (j == 0) {
-                vn = (a.code[ix(i, 1, pc)] & 128) ? G - 2 * dt * p[ix(i, 1, pc)] / g.dy : G;
-            } else {
-                vn = (a.code[ix(i, ny, pc)] & 128) ? G + 2 * dt * p[ix(i, ny, pc)] / g.dy : G;
-            }
-            if (vf) res[0] = nanmax(res[0], fabs(vn - a.v[cv]));
-            a.v[cv] = vn;
+            a.v[cv] = (a.code[ix(i, 1, pc)] & 128) ? G - 2 * dt * p[ix(i, 1, pc)] / g.dy : G;
         }
     }
     if (b == 0 && !periodic) {
