@@ -328,17 +328,19 @@ DEFAULT_CASES = ([(n, re) for n in (256, 512, 1024) for re in (100.0, 400.0, 100
                  + [(2048, 1000.0), (2048, 5000.0)])
 
 
-def train_default(out=None, steps: int = 150, seconds: float = 150.0, cases_list=None,
-                  side: int = TARGET_SIDE, seed: int = 0, log=None) -> Path:
+def train_default(out=None, steps: int = 650, seconds: float = 600.0, epochs: int = 250,
+                  cases_list=None, side: int = TARGET_SIDE, seed: int = 0, log=None) -> Path:
     """Harvest pairs from lid-driven cavities (sizes 256-2048, Re 100-5000,
-    pressure_tol 1e-6, `steps` steps each, teacher mode) and train the network
-    with the squared-error loss (lr 1e-3, batches of 8, augmentation by the
-    grid symmetries) for about `seconds`; writes the weights to `out`."""
+    pressure_tol 1e-6, `steps` steps each, teacher mode: about 50 000 pairs
+    with the defaults) and train the network with the squared-error loss (lr
+    1e-3 with cosine decay over `epochs` epochs of 2000 samples, batches of 8,
+    augmentation by the grid symmetries) for at most `seconds`; writes the
+    weights to `out`."""
     from . import cases as C
 
     xs, ts = [], []
     for n, re in (cases_list or DEFAULT_CASES):
-        cfg = LearnedConfig(mode="teacher", harvest_cap=4 * steps + 64, side=side,
+        cfg = LearnedConfig(mode="teacher", harvest_cap=5 * steps + 64, side=side,
                             teacher_cycles=10)
         sim = C.lid_driven_cavity(re=re, n=n, pressure_solver="cg_learned", pressure_tol=1e-6,
                                   learned=cfg).sim
@@ -352,7 +354,7 @@ def train_default(out=None, steps: int = 150, seconds: float = 150.0, cases_list
             log(f"cavity n={n} Re={re:g}: {x.shape[0]} pairs")
         sim.close()
     x, t = np.concatenate(xs), np.concatenate(ts)
-    params, sigma, hist = train(x, t, epochs=1000, batch=8, lr=1e-3, per_epoch=2000, seed=seed,
+    params, sigma, hist = train(x, t, epochs=epochs, batch=8, lr=1e-3, per_epoch=2000, seed=seed,
                                 loss="mse", log=log, time_limit=seconds)
     meta = {"pairs": int(x.shape[0]), "image": list(x.shape[1:]), "epochs": len(hist),
             "final_loss": float(hist[-1]), "side": side, "steps": steps}
