@@ -172,7 +172,9 @@ struct cfd_ctx {
     long long l_hcap = 0;
     float* diag0f = nullptr;
     FcgState* fcg = nullptr;
-    FcgState* fcg_host = nullptr;   // pinned mirror
+    LpPoll* poll_h = nullptr;       // pinned record of the solve's state (host address)
+    LpPoll* poll_d = nullptr;       // the same record as the device addresses it
+    unsigned poll_seq = 0;          // sequence number of the last end launches
     double* l_part = nullptr;
     hipGraph_t l_graph_g[3] = {};    // one captured iteration per sweep count 2, 3, 4
     hipGraphExec_t l_graph_x[3] = {};
@@ -1023,12 +1025,33 @@ int learned_end(cfd_ctx* c, const ProjArgs& a) {
     LV_LAUNCH512(k_lp_xsum, nb, a, c->geo, c->l_part);
     LV_LAUNCH512(k_lp_pwrite, nb, a, c->geo, la, nb);
     LV_LAUNCH512(k_lp_correct, nb, a, c->geo, c->l_part);
-    LV_LAUNCH(k_lp_fin, 1, a, c->geo, c->l_part, nb);
+    LV_LAUNCH(k_lp_fin, 1, a, c->geo, c->l_part, nb, c->poll_d, ++c->poll_seq);
     if (a.zero_p_after) {
         const long n = (long)(c->nx + 2) * c->P;
         LV_LAUNCH(k_lp_pzero, (unsigned)((n + 255) / 256), a, n);
     }
     return 0;
+}
+
+// Wait for the poll record of the last end launches (k_lp_fin stores the
+// solve's state, then the sequence number): the host spins on the pinned
+// record instead of copying the state and synchronising the stream.  Every
+// 1024 polls the stream is queried, so an error on the device is reported
+// and a drained stream without the record is an error, not a hang.
+int wait_poll(cfd_ctx* c) {
+    const unsigned want = c->poll_seq;
+    for (long spins = 1;; ++spins) {
+        if (__atomic_load_n(&c->poll_h->seq, __ATOMIC_ACQUIRE) == want) return 0;
+        if ((spins & 1023) == 0) {
+            const hipError_t e = hipStreamQuery(c->stream);
+            if (e == hipSuccess) {   // the record's stores precede the kernel's completion
+                if (__atomic_load_n(&c->poll_h->seq, __ATOMIC_ACQUIRE) == want) return 0;
+                return set_err(CFD_ERR_STATE, "the solve's state record did not arrive");
+            }
+            if (e != hipErrorNotReady) HIPC(e);
+        }
+        __builtin_ia32_pause();
+    }
 }
 
 // A projection with the learned flexible CG: the start launches (right-hand
@@ -1083,15 +1106,13 @@ int run_learned(cfd_ctx* c, ProjArgs a) {
         }
         launched += batch;
         TRY(learned_end(c, a));
-        HIPC(hipMemcpyAsync(c->fcg_host, c->fcg, sizeof(FcgState), hipMemcpyDeviceToHost,
-                            c->stream));
-        HIPC(hipStreamSynchronize(c->stream));
-        if (c->fcg_host->done) break;
+        TRY(wait_poll(c));
+        if (c->poll_h->s.done) break;
         if (launched > a.maxiter + 2)
             return set_err(CFD_ERR_STATE, "learned solver did not finish in %d iterations", launched);
         batch = 1;
     }
-    const FcgState h = *c->fcg_host;
+    const FcgState h = c->poll_h->s;
     c->l_solves += 1;
     c->l_iters_total += h.iters;
     if (h.info == 0) c->l_prev_iters = h.iters;
@@ -1280,7 +1301,7 @@ int cfd_ctx_destroy(cfd_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     for (hipEvent_t e : c->prof_ev) hipEventDestroy(e);
     learned_drop_graph(c);
-    if (c->fcg_host) hipHostFree(c->fcg_host);
+    if (c->poll_h) hipHostFree(c->poll_h);
     for (void* p : c->allocs) hipFree(p);
     for (void* p : c->probes.mem) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -1416,7 +1437,9 @@ int cfd_set_learned(cfd_ctx* c, const cfd_learned_desc* d) {
         TRY(dalloc(c, &c->fcg, 1));
         TRY(dalloc(c, &c->l_part, (size_t)LV_NSLOT * LV_SLOT));
         TRY(dalloc(c, &c->diag0f, 64));
-        HIPC(hipHostMalloc((void**)&c->fcg_host, sizeof(FcgState), 0));
+        HIPC(hipHostMalloc((void**)&c->poll_h, sizeof(LpPoll), 0));
+        std::memset(c->poll_h, 0, sizeof(LpPoll));
+        HIPC(hipHostGetDevicePointer((void**)&c->poll_d, c->poll_h, 0));
         std::vector<double> lut(128);
         HIPC(hipMemcpy(lut.data(), c->lut, 128 * sizeof(double), hipMemcpyDeviceToHost));
         float df[64];

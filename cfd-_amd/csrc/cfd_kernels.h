@@ -181,6 +181,16 @@ struct FcgState {
     unsigned ticket[4];
 };
 
+// The flexible CG's state as the host reads it after a batch of iterations:
+// k_lp_fin copies the state into this record in pinned host memory, then
+// stores the batch's sequence number; the host waits for that number instead
+// of copying the state and synchronising the stream.
+struct LpPoll {
+    FcgState s;
+    unsigned seq;
+    unsigned pad;
+};
+
 // Turbulent viscosity and diffusivities the time-step control reads
 // (solver.py:216-218, 249-251, models.py:204-208); nut == nullptr: laminar.
 struct TurbDt {

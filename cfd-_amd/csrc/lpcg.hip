@@ -1868,29 +1868,39 @@ __global__ void __launch_bounds__(512) k_lp_correct(ProjArgs a, Geo g, double* p
 }
 
 // one work-group: the solve's statistics into the step state, the residual
-// monitor, the periodic corner faces
-__global__ void __launch_bounds__(256) k_lp_fin(ProjArgs a, Geo g, double* part, int nb) {
+// monitor, the periodic corner faces; then (finished or not) the flexible
+// CG's state and the sequence number `seq` into the host's poll record
+__global__ void __launch_bounds__(256) k_lp_fin(ProjArgs a, Geo g, double* part, int nb,
+                                                LpPoll* poll, unsigned seq) {
     DevState* st = a.st;
-    if (lp_end_idle(a)) return;
-    double res = 0.0;
-    if (a.correct)
-        for (int k = threadIdx.x; k < nb; k += blockDim.x)
-            res = nanmax(res, part[LV_P_PQ * LV_SLOT + k]);
-    __shared__ double lds[16];
-    double m[1] = {res};
-    block_max<1>(m, lds);
-    if (threadIdx.x != 0) return;
-    const FcgState* fs = a.fcg;
-    st->cg_iters = fs->iters;
-    st->cg_info = fs->info;
-    st->cg_outer = fs->reps;
-    st->cg_bnorm = fs->bnorm;
-    st->cg_relres = fs->bnorm > 0.0 ? sqrt(fs->rr) / fs->bnorm : 0.0;
-    if (!a.correct) return;
-    if (a.dt_override <= 0.0) st->residual = m[0];
-    if (g.periodic) {
-        a.u[ix(0, 0, g.pu)] = a.u[ix(g.nx, 0, g.pu)];
-        a.u[ix(0, g.ny + 1, g.pu)] = a.u[ix(g.nx, g.ny + 1, g.pu)];
+    if (!lp_end_idle(a)) {
+        double res = 0.0;
+        if (a.correct)
+            for (int k = threadIdx.x; k < nb; k += blockDim.x)
+                res = nanmax(res, part[LV_P_PQ * LV_SLOT + k]);
+        __shared__ double lds[16];
+        double m[1] = {res};
+        block_max<1>(m, lds);
+        if (threadIdx.x == 0) {
+            const FcgState* fs = a.fcg;
+            st->cg_iters = fs->iters;
+            st->cg_info = fs->info;
+            st->cg_outer = fs->reps;
+            st->cg_bnorm = fs->bnorm;
+            st->cg_relres = fs->bnorm > 0.0 ? sqrt(fs->rr) / fs->bnorm : 0.0;
+            if (a.correct) {
+                if (a.dt_override <= 0.0) st->residual = m[0];
+                if (g.periodic) {
+                    a.u[ix(0, 0, g.pu)] = a.u[ix(g.nx, 0, g.pu)];
+                    a.u[ix(0, g.ny + 1, g.pu)] = a.u[ix(g.nx, g.ny + 1, g.pu)];
+                }
+            }
+        }
+    }
+    if (poll && threadIdx.x == 0) {
+        poll->s = *a.fcg;
+        __threadfence_system();
+        __hip_atomic_store(&poll->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
