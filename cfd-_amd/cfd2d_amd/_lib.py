@@ -269,6 +269,7 @@ def lib() -> ctypes.CDLL:
         "cfd_unet_set_loss": ([_vp, _i32, _i32, _i32, ctypes.c_float], _i32),
         "cfd_unet_set_batch": ([_vp, _i32], _i32),
         "cfd_mfma_layout_check": ([_vp, _vp, _vp], _i32),
+        "cfd_mfma_layout_check_k32": ([_vp, _vp, _vp], _i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -303,7 +304,7 @@ def exported_symbols() -> list[str]:
                         "cfd_unet_get_params", "cfd_unet_get_grad", "cfd_unet_zero_grad",
                         "cfd_unet_forward", "cfd_unet_loss_grad", "cfd_unet_adam",
                         "cfd_unet_set_loss", "cfd_unet_set_batch",
-                        "cfd_mfma_layout_check")]
+                        "cfd_mfma_layout_check", "cfd_mfma_layout_check_k32")]
 
 
 def check(rc: int, what: str = "") -> int:

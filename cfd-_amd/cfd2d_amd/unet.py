@@ -132,12 +132,18 @@ class UNet:
 
 
 def mfma_layout_check(A: np.ndarray, B: np.ndarray) -> np.ndarray:
-    """A @ B for 16 x 16 matrices through one bf16 matrix-core instruction."""
+    """A @ B through one bf16 matrix-core instruction: 16 x 16 operands on
+    v_mfma_f32_16x16x16_bf16, or A 16 x 32 and B 32 x 16 on
+    v_mfma_f32_16x16x32_bf16 fed with two 16-deep K steps as the
+    convolutions feed it."""
     a = np.ascontiguousarray(A, dtype=np.float32)
     b = np.ascontiguousarray(B, dtype=np.float32)
-    if a.shape != (16, 16) or b.shape != (16, 16):
-        raise ValueError("16 x 16 operands expected")
     d = np.empty((16, 16), dtype=np.float32)
-    _lib.check(_lib.lib().cfd_mfma_layout_check(_lib.ptr(a), _lib.ptr(b), _lib.ptr(d)),
-               "MFMA layout check")
+    if a.shape == (16, 16) and b.shape == (16, 16):
+        fn = _lib.lib().cfd_mfma_layout_check
+    elif a.shape == (16, 32) and b.shape == (32, 16):
+        fn = _lib.lib().cfd_mfma_layout_check_k32
+    else:
+        raise ValueError("operands 16 x 16 and 16 x 16, or 16 x 32 and 32 x 16 expected")
+    _lib.check(fn(_lib.ptr(a), _lib.ptr(b), _lib.ptr(d)), "MFMA layout check")
     return d

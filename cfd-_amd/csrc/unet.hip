@@ -1,14 +1,21 @@
 // Convolution building blocks of the learned pressure preconditioner (U-Net)
 // on gfx950: 3x3 "same" convolutions in NHWC with BF16 operands and FP32
-// accumulation on the matrix cores (v_mfma_f32_16x16x16_bf16), their
-// backward passes, pooling/upsampling, the loss and the Adam update.
+// accumulation on the matrix cores, their backward passes,
+// pooling/upsampling, the loss and the Adam update.
 //
 // MFMA tiling (one wavefront = one output tile of 16 pixels x 16 channels per
-// accumulator): for v_mfma_f32_16x16x16_bf16 lane l holds
+// accumulator).  A K step is 16 input channels of one filter tap; for it lane
+// l holds
 //   A (16 x 16, M x K): row l%16, k = 4*(l/16) .. +3
 //   B (16 x 16, K x N): col l%16, k = 4*(l/16) .. +3
 //   D (16 x 16, M x N): col l%16, rows 4*(l/16) .. +3
-// (checked on the device by cfd_mfma_layout_check against a host product).
+// (v_mfma_f32_16x16x16_bf16's layout).  Two K steps go to gfx950's double-K
+// instruction v_mfma_f32_16x16x32_bf16 as one: each lane passes its 4 values
+// of the first step followed by its 4 of the second, in A and in B alike, so
+// the instruction's sum over its 32 K positions is the sum of both steps'
+// products whatever order it pairs them in (mfma2).  Both layouts are checked
+// on the device against host products (cfd_mfma_layout_check,
+// cfd_mfma_layout_check_k32).
 // Here M = 16 consecutive pixels of one image row, K = 16 input channels of
 // one filter tap, N = 16 output channels.  Weights are stored [tap][Cout][Cin]
 // so the 4 K-values a lane needs are contiguous (one 8-byte load).
@@ -16,7 +23,18 @@
 #include <hip/hip_bf16.h>
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// acc + A0 B0 + A1 B1 for two K steps (lane fragments as for
+// v_mfma_f32_16x16x16_bf16) in one v_mfma_f32_16x16x32_bf16
+__device__ __forceinline__ f32x4 mfma2(bf16x4 a0, bf16x4 a1, bf16x4 b0, bf16x4 b1, f32x4 acc) {
+    const bf16x8 a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
+    const bf16x8 b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8v, a),
+                                                   __builtin_bit_cast(bf16x8v, b), acc, 0, 0, 0);
+}
 
 __device__ __forceinline__ unsigned short f2bf(float f) {
     // round to nearest even
@@ -40,6 +58,23 @@ __global__ void k_mfma_layout_check(const float* A, const float* B, float* D) {
     }
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    for (int j = 0; j < 4; ++j) D[(4 * (l / 16) + j) * 16 + (l % 16)] = acc[j];
+}
+
+// The same for D = A(16x32) * B(32x16) through mfma2: the two halves of K as
+// the two K steps
+__global__ void k_mfma_layout_check_k32(const float* A, const float* B, float* D) {
+    const int l = threadIdx.x;   // one wavefront
+    bf16x4 a0, a1, b0, b1;
+    for (int j = 0; j < 4; ++j) {
+        const int k = 4 * (l / 16) + j;
+        a0[j] = (short)f2bf(A[(l % 16) * 32 + k]);
+        a1[j] = (short)f2bf(A[(l % 16) * 32 + 16 + k]);
+        b0[j] = (short)f2bf(B[k * 16 + (l % 16)]);
+        b1[j] = (short)f2bf(B[(16 + k) * 16 + (l % 16)]);
+    }
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma2(a0, a1, b0, b1, acc);
     for (int j = 0; j < 4; ++j) D[(4 * (l / 16) + j) * 16 + (l % 16)] = acc[j];
 }
 
@@ -124,6 +159,7 @@ __global__ void __launch_bounds__(256) k_conv3x3_mfma(
                 Wt + ((long)t * Cout + (grp * G + g) * 16 + m) * Cin + k0 + kq);
     };
     constexpr int U = 8;
+    static_assert(U % 2 == 0, "steps in pairs");
     for (int s0 = wave * U; s0 < nsteps; s0 += 4 * U) {
         bf16x4 a[U], b[U][G];
 #pragma unroll
@@ -137,10 +173,10 @@ __global__ void __launch_bounds__(256) k_conv3x3_mfma(
             }
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u)
+        for (int u = 0; u < U; u += 2)
 #pragma unroll
             for (int g = 0; g < G; ++g)
-                acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], b[u][g], acc[g], 0, 0, 0);
+                acc[g] = mfma2(a[u], a[u + 1], b[u][g], b[u + 1][g], acc[g]);
     }
     if (wave > 0)
 #pragma unroll
@@ -206,33 +242,39 @@ __global__ void __launch_bounds__(256) k_conv3x3_wgrad(
     f32x4 acc[8];
 #pragma unroll
     for (int ib = 0; ib < 8; ++ib) acc[ib] = (f32x4){0.f, 0.f, 0.f, 0.f};
-    for (long p = p0; p < p1; p += 16) {
-        bf16x4 a;
-        long q[4];
+    // two K steps of 16 pixels per double-K instruction (pixels past p1 add 0)
+    for (long p = p0; p < p1; p += 32) {
+        bf16x4 a[2];
+        long q[2][4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const long pp = p + kq + j;
-            unsigned short ga = 0;
-            q[j] = -1;
-            if (pp < p1) {
-                ga = G[pp * Cout + cb * 16 + m];
-                const int hh = (int)(pp / Wd) + dh, ww = (int)(pp % Wd) + dw;
-                if (hh >= 0 && hh < H && ww >= 0 && ww < Wd) q[j] = (long)hh * Wd + ww;
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long pp = p + 16 * s + kq + j;
+                unsigned short ga = 0;
+                q[s][j] = -1;
+                if (pp < p1) {
+                    ga = G[pp * Cout + cb * 16 + m];
+                    const int hh = (int)(pp / Wd) + dh, ww = (int)(pp % Wd) + dw;
+                    if (hh >= 0 && hh < H && ww >= 0 && ww < Wd) q[s][j] = (long)hh * Wd + ww;
+                }
+                a[s][j] = (short)ga;
             }
-            a[j] = (short)ga;
-        }
 #pragma unroll
         for (int ib = 0; ib < 8; ++ib) {
             if (ib >= nci) break;
             const int ci = ib * 16 + m;
-            bf16x4 b;
+            bf16x4 b[2];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                unsigned short xb = 0;
-                if (q[j] >= 0) xb = ci < C1 ? X1[q[j] * C1 + ci] : X2[q[j] * C2 + ci - C1];
-                b[j] = (short)xb;
-            }
-            acc[ib] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[ib], 0, 0, 0);
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    unsigned short xb = 0;
+                    if (q[s][j] >= 0)
+                        xb = ci < C1 ? X1[q[s][j] * C1 + ci] : X2[q[s][j] * C2 + ci - C1];
+                    b[s][j] = (short)xb;
+                }
+            acc[ib] = mfma2(a[0], a[1], b[0], b[1], acc[ib]);
         }
     }
     // D rows = co (4*(lane/16)+j within the chunk), cols = ci (lane%16)
@@ -680,29 +722,36 @@ __device__ __attribute__((noinline)) void conv_img(const lds_u16* X1, const lds_
 #pragma unroll
             for (int u = 0; u < U; ++u) bn[u] = B[(s0 + U + u) * 64];
         }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            // the wave's K range starts at a multiple of 9 steps; tap and
-            // channel chunk follow from the absolute step
+        // the A fragment of K step u of the batch for tile q: the wave's K
+        // range starts at a multiple of 9 steps; tap and channel chunk follow
+        // from the absolute step
+        auto afrag = [&](int u, int q) -> bf16x4 {
             const int st = sbeg + s0 + u, t = st / nk, kc = st - t * nk;
-            const int dh = t / 3 - 1, dw = t % 3 - 1;
+            const int hh = ph[q] + t / 3 - 1, ww = pwc[q] + t % 3 - 1;
+            const bool in = hh >= 0 && hh < S && ww >= 0 && ww < S;
+            const int h2 = in ? hh : 0, w2 = in ? ww : 0;
+            bf16x4 a;
+            if (kc >= nk1)
+                a = *reinterpret_cast<const lds_bf16x4*>(X2 + (h2 * S + w2) * CS2 + (kc - nk1) * 16 + kq);
+            else if (UP)
+                a = *reinterpret_cast<const lds_bf16x4*>(
+                    X1 + ((h2 >> 1) * (S / 2) + (w2 >> 1)) * CS1 + kc * 16 + kq);
+            else
+                a = *reinterpret_cast<const lds_bf16x4*>(X1 + (h2 * S + w2) * CS1 + kc * 16 + kq);
+            return in ? a : (bf16x4){0, 0, 0, 0};
+        };
+        // steps in pairs on the double-K instruction, the odd one out paired
+        // with zeros: every product into an accumulator goes through the same
+        // instruction (with the single-K form after the double-K one on the
+        // same accumulator, this toolchain issued the dependent instruction
+        // too early: wrong sums, which the batched-forward test caught)
+        const bf16x4 z4 = {0, 0, 0, 0};
 #pragma unroll
-            for (int q = 0; q < T; ++q) {
-                const int hh = ph[q] + dh, ww = pwc[q] + dw;
-                const bool in = hh >= 0 && hh < S && ww >= 0 && ww < S;
-                const int h2 = in ? hh : 0, w2 = in ? ww : 0;
-                bf16x4 a;
-                if (kc >= nk1)
-                    a = *reinterpret_cast<const lds_bf16x4*>(X2 + (h2 * S + w2) * CS2 + (kc - nk1) * 16 + kq);
-                else if (UP)
-                    a = *reinterpret_cast<const lds_bf16x4*>(
-                        X1 + ((h2 >> 1) * (S / 2) + (w2 >> 1)) * CS1 + kc * 16 + kq);
-                else
-                    a = *reinterpret_cast<const lds_bf16x4*>(X1 + (h2 * S + w2) * CS1 + kc * 16 + kq);
-                if (!in) a = (bf16x4){0, 0, 0, 0};
-                acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bc[u], acc[q], 0, 0, 0);
-            }
-        }
+        for (int u = 0; u < U; u += 2)
+#pragma unroll
+            for (int q = 0; q < T; ++q)
+                acc[q] = u + 1 < U ? mfma2(afrag(u, q), afrag(u + 1, q), bc[u], bc[u + 1], acc[q])
+                                   : mfma2(afrag(u, q), z4, bc[u], z4, acc[q]);
         if (s0 + U < kst) {
 #pragma unroll
             for (int u = 0; u < U; ++u) bc[u] = bn[u];

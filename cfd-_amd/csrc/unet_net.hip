@@ -502,4 +502,21 @@ int cfd_mfma_layout_check(const float* A, const float* B, float* D) {
     return 0;
 }
 
+int cfd_mfma_layout_check_k32(const float* A, const float* B, float* D) {
+    if (!A || !B || !D) return set_err(CFD_ERR_ARG, "null argument");
+    float *a, *b, *d;
+    HIPC(hipMalloc(&a, 512 * sizeof(float)));
+    HIPC(hipMalloc(&b, 512 * sizeof(float)));
+    HIPC(hipMalloc(&d, 256 * sizeof(float)));
+    HIPC(hipMemcpy(a, A, 512 * sizeof(float), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(b, B, 512 * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_mfma_layout_check_k32, dim3(1), dim3(64), 0, 0, a, b, d);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpy(D, d, 256 * sizeof(float), hipMemcpyDeviceToHost));
+    hipFree(a);
+    hipFree(b);
+    hipFree(d);
+    return 0;
+}
+
 }  // extern "C"

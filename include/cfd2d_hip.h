@@ -467,6 +467,11 @@ int cfd_unet_adam(cfd_unet* net, float lr, float scale);
 /* D = A B for 16 x 16 row-major matrices, through one bf16 matrix-core
    instruction (checks the operand lane layout the convolutions rely on) */
 int cfd_mfma_layout_check(const float* A, const float* B, float* D);
+/* D = A B for A 16 x 32 and B 32 x 16 (row-major) through one
+   v_mfma_f32_16x16x32_bf16 fed with two 16-deep K steps in the lane layout
+   above (checks the pairing the convolutions rely on).  No counterpart in the
+   reference, which has no matrix-core code. */
+int cfd_mfma_layout_check_k32(const float* A, const float* B, float* D);
 
 /* ---- batched ensembles (no counterpart in the reference, which runs one case
    per process): many lid-driven / heated cavities on the unit square

@@ -32,6 +32,23 @@ def test_mfma_operand_layout(gpu):
     assert not np.allclose(D, A.T @ B, rtol=1e-2, atol=1e-2)
 
 
+def test_mfma_double_k_pairing(gpu):
+    """Two 16-deep K steps through one v_mfma_f32_16x16x32_bf16, each lane
+    passing its values of the first step and then of the second in A and B
+    alike (as the convolutions do): the product over all 32 K positions."""
+    from cfd2d_amd.unet import mfma_layout_check
+
+    rng = np.random.default_rng(1)
+    A = U.bf16(rng.standard_normal((16, 32)))
+    B = U.bf16(rng.standard_normal((32, 16)))
+    D = mfma_layout_check(A, B)
+    assert np.allclose(D, A @ B, rtol=1e-5, atol=1e-5)
+    # either half alone, or the halves crossed, would not pass
+    assert not np.allclose(D, A[:, :16] @ B[:16], rtol=1e-2, atol=1e-2)
+    crossed = A[:, :16] @ B[16:] + A[:, 16:] @ B[:16]
+    assert not np.allclose(D, crossed, rtol=1e-2, atol=1e-2)
+
+
 def _params(seed):
     p = U.init_params(seed).astype(np.float64)
     rng = np.random.default_rng(seed + 100)
