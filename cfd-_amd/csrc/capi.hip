@@ -719,6 +719,7 @@ LvArgs lv_args(cfd_ctx* c) {
     a.ay = (float)(1.0 / (c->dy * c->dy));
     a.fs = c->fcg;
     a.part = c->l_part;
+    a.store_x0 = (c->l_legs && (c->l_vlds || c->l_fused)) ? 0 : 1;
     return a;
 }
 

@@ -47,6 +47,8 @@ struct LvArgs {
     float ax, ay;
     FcgState* fs;
     double* part;            // LV_NSLOT slots of partial sums
+    int store_x0;            // the next V-cycle's level-0 x = w b / d is read (launch
+                             // sequence); the fused down-leg forms it from b itself
 };
 
 struct FcgVec {
@@ -703,15 +705,14 @@ __device__ __forceinline__ float* leg_sweeps(const LegArgs& a, const float* sb, 
 
 template <bool LEV0, int NU, int TI, int TJ>
 __global__ void __launch_bounds__(256) k_lv_down(LegArgs a) {
-    if (*a.done) return;
     constexpr int R = NU, W = TJ + 2 * R, N = (TI + 2 * R) * W;
     __shared__ float sb[N], sd[N], sx[N], st[N];
     __shared__ float se[LEV0 ? 1 : N], sn[LEV0 ? 1 : N];
     __shared__ float s_d0[64];
-    if (LEV0) {
-        if (threadIdx.x < 64) s_d0[threadIdx.x] = a.diag0[threadIdx.x];
-        __syncthreads();
-    }
+    // the diagonal table's load in flight with the flag's
+    if (LEV0 && threadIdx.x < 64) s_d0[threadIdx.x] = a.diag0[threadIdx.x];
+    if (*a.done) return;
+    if (LEV0) __syncthreads();
     const MGLevelDev& L = a.L;
     const MGLevelDev& C = a.C;
     const float w = a.w;
@@ -763,15 +764,14 @@ __global__ void __launch_bounds__(256) k_lv_down(LegArgs a) {
 
 template <bool LEV0, bool DOTS, int NU, int TI, int TJ>
 __global__ void __launch_bounds__(256) k_lv_up(LegArgs a) {
-    if (*a.done) return;
     constexpr int R = NU, W = TJ + 2 * R, N = (TI + 2 * R) * W;
     __shared__ float sb[N], sd[N], sy[N], sx[N];
     __shared__ float se[LEV0 ? 1 : N], sn[LEV0 ? 1 : N];
     __shared__ float s_d0[64];
-    if (LEV0) {
-        if (threadIdx.x < 64) s_d0[threadIdx.x] = a.diag0[threadIdx.x];
-        __syncthreads();
-    }
+    // the diagonal table's load in flight with the flag's
+    if (LEV0 && threadIdx.x < 64) s_d0[threadIdx.x] = a.diag0[threadIdx.x];
+    if (*a.done) return;
+    if (LEV0) __syncthreads();
     const MGLevelDev& L = a.L;
     const float w = a.w;
     const bool first = DOTS ? *a.first != 0 : true;
@@ -824,18 +824,20 @@ __global__ void __launch_bounds__(256) k_lv_up(LegArgs a) {
 // d_old), partial d.q
 __global__ void __launch_bounds__(512) k_fcg_dir(LvArgs a, FcgVec v, int nb) {
     FcgState* fs = a.fs;
-    if (fs->done) return;
+    // the flag, the table and the partials in flight together (the partial
+    // sums' barriers also publish the table)
+    const int done = fs->done;
+    __shared__ double s_lut[64];
+    if (threadIdx.x < 64) s_lut[threadIdx.x] = v.lut[threadIdx.x];
     double dots[2];
     lv_total<2>(a.part + LV_P_DOTS * LV_SLOT, nb, dots);
+    if (done) return;
     const bool first = fs->first;
     const double beta = first ? 0.0 : (dots[0] - dots[1]) / fs->rho_old;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         fs->rho = dots[0];
         fs->beta = beta;
     }
-    __shared__ double s_lut[64];
-    if (threadIdx.x < 64) s_lut[threadIdx.x] = v.lut[threadIdx.x];
-    __syncthreads();
     const float* z = a.mg->lev[0].t;
     const int ny = a.mg->lev[0].ny;
     const int k = fs->k;
@@ -883,9 +885,14 @@ __global__ void __launch_bounds__(512) k_fcg_dir(LvArgs a, FcgVec v, int nb) {
 // input b0 = r, x0 = w r / d; partial r.r
 __global__ void __launch_bounds__(512) k_fcg_update(LvArgs a, FcgVec v, int nb) {
     FcgState* fs = a.fs;
-    if (fs->done) return;
+    // the flag, the diagonal table and the partials in flight together (the
+    // partial sums' barriers also publish the table)
+    const int done = fs->done;
+    __shared__ float s_d0_[64];
+    if (threadIdx.x < 64) s_d0_[threadIdx.x] = a.diag0[threadIdx.x];
     double pq[1];
     lv_total<1>(a.part + LV_P_PQ * LV_SLOT, nb, pq);
+    if (done) return;
     const double alpha = fs->rho / pq[0];
     if (!(pq[0] > 0.0) || !(fabs(alpha) < 1e300)) {   // breakdown or non-finite
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -895,7 +902,7 @@ __global__ void __launch_bounds__(512) k_fcg_update(LvArgs a, FcgVec v, int nb) 
         return;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) fs->alpha = alpha;
-    LV_CTX_LDS(a);
+    const MGCtx m{a.mg, a.code, s_d0_, a.periodic, a.ax, a.ay};
     const MGLevelDev L0 = a.mg->lev[0];
     const float w = a.mg->omega;
     const int k = fs->k;
@@ -925,7 +932,7 @@ __global__ void __launch_bounds__(512) k_fcg_update(LvArgs a, FcgVec v, int nb) 
             s[0] += o.rn * o.rn;
             const float rf = (float)o.rn;
             L0.b[c] = rf;
-            L0.x[c] = w * rf / m.diag0[o.li];
+            if (a.store_x0) L0.x[c] = w * rf / m.diag0[o.li];
         });
     lv_partial<1>(s, a.part + LV_P_RR * LV_SLOT);
 }
@@ -936,9 +943,10 @@ __global__ void __launch_bounds__(512) k_fcg_update(LvArgs a, FcgVec v, int nb) 
 // with the next V-cycle input, and its partial r.r stored for k_fcg_true_fin.
 __global__ void __launch_bounds__(512) k_fcg_true(LvArgs a, FcgVec v, int nb) {
     FcgState* fs = a.fs;
-    if (fs->done) return;
+    const int done = fs->done;   // in flight with the partials
     double rr[1];
     lv_total<1>(a.part + LV_P_RR * LV_SLOT, nb, rr);
+    if (done) return;
     const int k = fs->k + 1;   // fs->k itself advances in k_fcg_true_fin
     const bool nan = !(rr[0] == rr[0]);
     const bool pending = !nan && (sqrt(rr[0]) < fs->atol || k >= fs->maxiter);
@@ -979,7 +987,7 @@ __global__ void __launch_bounds__(512) k_fcg_true(LvArgs a, FcgVec v, int nb) {
             s[0] += o.rt * o.rt;
             const float rf = (float)o.rt;
             L0.b[c] = rf;
-            L0.x[c] = w * rf / m.diag0[o.li];
+            if (a.store_x0) L0.x[c] = w * rf / m.diag0[o.li];
         });
     lv_partial<1>(s, a.part + LV_P_RR * LV_SLOT);
 }
@@ -1691,7 +1699,7 @@ __global__ void __launch_bounds__(512) k_lp_res(ProjArgs a, Geo g, LvArgs la, in
             s2[1] += o.rv * o.rv;
             const float rf = (float)o.rv;
             L0.b[c] = rf;
-            L0.x[c] = w * rf / s_diagf[o.li];
+            if (la.store_x0) L0.x[c] = w * rf / s_diagf[o.li];
         });
     lv_partial<2>(s2, la.part + LV_P_RR * LV_SLOT);
 }
