@@ -1496,7 +1496,6 @@ int cfd_set_learned(cfd_ctx* c, const cfd_learned_desc* d) {
             v.periodic = c->periodic;
             v.nl = mg.nlev - lc;
             v.nc = mg.ncoarse;
-            int off = 0;
             for (int q = 0; q < v.nl; ++q) {
                 const MGLevelDev& G = mg.lev[lc + q];
                 VcLev& L = v.lev[q];
@@ -1504,7 +1503,6 @@ int cfd_set_learned(cfd_ctx* c, const cfd_learned_desc* d) {
                 L.ny = G.ny;
                 L.pw = G.ny + 2;
                 L.sz = (G.nx + 2) * L.pw;
-                L.off = off;
                 L.P = G.P;
                 L.m = ((1ull << 32) + (unsigned long long)G.ny - 1) / (unsigned long long)G.ny;
                 L.gd = G.diag;
@@ -1512,16 +1510,15 @@ int cfd_set_learned(cfd_ctx* c, const cfd_learned_desc* d) {
                 L.gn = G.cN;
                 L.gb = G.b;
                 L.gt = G.t;
-                off += VC_ARR * L.sz;
                 v.total += L.sz;
             }
+            if (vc_assign(v) > VC_POOL) return set_err(CFD_ERR_STATE, "coarse-end pool");
             std::vector<float> img;
             if (vc_build_image(v, img)) return set_err(CFD_ERR_HIP, "coarse-end image");
             float* dimg = nullptr;
             TRY(dalloc(c, &dimg, img.size()));
             HIPC(hipMemcpy(dimg, img.data(), img.size() * sizeof(float), hipMemcpyHostToDevice));
             v.image = reinterpret_cast<const float4*>(dimg);
-            v.image_n4 = (int)(img.size() / 4);
         }
     }
     if (!c->l_vlds && d->coarse_fused && d->mode != LEARNED_TEACHER && d->level >= 1 && L.nx == 32 &&

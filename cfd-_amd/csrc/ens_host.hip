@@ -628,6 +628,7 @@ int cfd_ens_set_coarse(cfd_ens* e, int32_t mode) {
         while (lc < c.nlev - 1 && (long)c.lev[lc].n * c.lev[lc].n > 64L * 64) ++lc;
         long need = 0;
         for (int l = lc; l < c.nlev; ++l) need += (long)VC_ARR * (c.lev[l].n + 2) * (c.lev[l].n + 2);
+        need += 8;   // the float4 alignment of the pool's two parts (vc_assign)
         if (c.nlev - lc > VC_MAXL || c.ncoarse > VC_MAXC || need > VC_POOL)
             return set_err(CFD_ERR_LIMIT, "class %d: the coarse levels do not fit the LDS coarse end", k);
         VcArgs& v = e->vargs[k];
@@ -640,14 +641,12 @@ int cfd_ens_set_coarse(cfd_ens* e, int32_t mode) {
         v.periodic = 0;
         v.nl = c.nlev - lc;
         v.nc = c.ncoarse;
-        int off = 0;
         for (int q = 0; q < v.nl; ++q) {
             const EnsLevel& G = c.lev[lc + q];
             VcLev& L = v.lev[q];
             L.nx = L.ny = G.n;
             L.pw = G.n + 2;
             L.sz = (G.n + 2) * L.pw;
-            L.off = off;
             L.P = G.P;
             L.m = ((1ull << 32) + (unsigned long long)G.n - 1) / (unsigned long long)G.n;
             L.istride = G.S;
@@ -656,16 +655,15 @@ int cfd_ens_set_coarse(cfd_ens* e, int32_t mode) {
             L.gn = G.cN;
             L.gb = G.b;
             L.gt = G.t;
-            off += VC_ARR * L.sz;
             v.total += L.sz;
         }
+        if (vc_assign(v) > VC_POOL) return set_err(CFD_ERR_STATE, "coarse-end pool");
         std::vector<float> img;
         if (vc_build_image(v, img)) return set_err(CFD_ERR_HIP, "coarse-end image");
         float* dimg = nullptr;
         TRY(ealloc(e, &dimg, img.size()));
         HIPC(hipMemcpy(dimg, img.data(), img.size() * sizeof(float), hipMemcpyHostToDevice));
         v.image = reinterpret_cast<const float4*>(dimg);
-        v.image_n4 = (int)(img.size() / 4);
         e->vlc[k] = lc;
     }
     e->vcycle = 1;

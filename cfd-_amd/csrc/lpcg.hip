@@ -1266,21 +1266,23 @@ constexpr int VC_ARR = 6;                // b, x, t, diag, cE, cN per level
 constexpr int VC_MAXL = 8;               // levels held in LDS
 constexpr int VC_MAXC = 64;              // cells of the dense coarsest level
 
-// LDS floats needed for levels Lc .. nlev-1 (host and device)
+// LDS floats needed for levels Lc .. nlev-1 (host and device; with the
+// float4 alignment of the two parts of the pool, see vc_assign)
 __host__ __device__ inline long vc_need(const MGDev& mg, int Lc) {
     long s = 0;
     for (int l = Lc; l < mg.nlev; ++l)
         s += (long)VC_ARR * (mg.lev[l].nx + 2) * (mg.lev[l].ny + 2);
-    return s;
+    return s + 8;
 }
 
-// one LDS level: sizes, the offset of its six arrays in the pool, the
+// one LDS level: sizes, the offsets of its coefficients (diag, cE, cN) and
+// of its vectors (b, x, t) in the pool, the
 // multiplier of the exact division q / ny for q < 2^16 (m = ceil(2^32 / ny)),
 // and the global arrays it is loaded from / written back to (the right-hand
 // side and correction of instance blockIdx.x at gb / gt + blockIdx.x * istride;
 // the coefficients are shared)
 struct VcLev {
-    int nx, ny, pw, sz, off, P;
+    int nx, ny, pw, sz, coff, voff, P;
     unsigned long long m;
     long istride;
     const float *gd, *ge, *gn, *gb;
@@ -1295,21 +1297,42 @@ struct VcArgs {
     const char* done;
     long done_stride;
     const float* cinv;       // dense inverse of the coarsest level (shared)
-    const float4* image;     // the pool as it starts: coefficients, zero b, x, t (shared)
+    const float4* image;     // the coefficient part of the pool as it starts (shared)
     int image_n4;            // its length in float4
+    int zero_off4, zero_n4;  // the vector part (float4 offset, length): zero at the start
     float omega, wc;
     int periodic, nl, nc;    // levels in LDS, cells of the coarsest level
     int total;               // sum of sz over the levels
     VcLev lev[VC_MAXL];
 };
 
-// The image of the pool at the start of k_lv_vcoarse, built on the host from
-// the levels' device arrays (pitch P): per level b, x, t zero and diag, cE, cN
-// at the tight pitch ny + 2 with their ghost rings, at the offsets of v.lev.
+// The pool's layout for levels with nx, ny, pw, sz set: the coefficients of
+// every level first (the part the launch copies from the image), then from
+// a float4 boundary the vectors b, x, t of every level (the part the launch
+// zeroes).  Returns the floats used (at most vc_need).
+inline long vc_assign(VcArgs& v) {
+    long c = 0;
+    for (int q = 0; q < v.nl; ++q) {
+        v.lev[q].coff = (int)c;
+        c += 3L * v.lev[q].sz;
+    }
+    const long vb = (c + 3) / 4 * 4;
+    long x = vb;
+    for (int q = 0; q < v.nl; ++q) {
+        v.lev[q].voff = (int)x;
+        x += 3L * v.lev[q].sz;
+    }
+    v.image_n4 = (int)((c + 3) / 4);
+    v.zero_off4 = (int)(vb / 4);
+    v.zero_n4 = (int)((x - vb + 3) / 4);
+    return vb + 4L * v.zero_n4;
+}
+
+// The image of the coefficient part of the pool, built on the host from the
+// levels' device arrays (pitch P): per level diag, cE, cN at the tight pitch
+// ny + 2 with their ghost rings, at the offsets of v.lev (vc_assign).
 inline int vc_build_image(const VcArgs& v, std::vector<float>& img) {
-    int n = 0;
-    for (int q = 0; q < v.nl; ++q) n = std::max(n, v.lev[q].off + VC_ARR * v.lev[q].sz);
-    img.assign((size_t)((n + 3) / 4) * 4, 0.0f);
+    img.assign((size_t)v.image_n4 * 4, 0.0f);
     std::vector<float> h;
     for (int q = 0; q < v.nl; ++q) {
         const VcLev& L = v.lev[q];
@@ -1319,7 +1342,7 @@ inline int vc_build_image(const VcArgs& v, std::vector<float>& img) {
         for (int a = 0; a < 3; ++a) {
             if (hipMemcpy(h.data(), src[a], cnt * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
                 return -1;
-            float* dst = img.data() + L.off + (3 + a) * L.sz;
+            float* dst = img.data() + L.coff + a * L.sz;
             for (int i = 0; i <= L.nx + 1; ++i)
                 for (int j = 0; j <= L.ny + 1; ++j) dst[i * L.pw + j] = h[(size_t)i * L.P + j];
         }
@@ -1341,13 +1364,14 @@ __device__ __forceinline__ VcView vc_view(float* pool, const VcLev& L) {
     v.ny = L.ny;
     v.pw = L.pw;
     v.m = L.m;
-    float* p = pool + L.off;
+    float* p = pool + L.voff;
     v.b = p;
     v.x = p + L.sz;
     v.t = p + 2 * L.sz;
-    v.d = p + 3 * L.sz;
-    v.e = p + 4 * L.sz;
-    v.n = p + 5 * L.sz;
+    float* c = pool + L.coff;
+    v.d = c;
+    v.e = c + L.sz;
+    v.n = c + 2 * L.sz;
     return v;
 }
 
@@ -1440,7 +1464,8 @@ __global__ void __launch_bounds__(1024) k_lv_vcoarse(VcArgs a) {
 #pragma unroll
         for (int q = 0; q < VC_MAXC; ++q) crow[q] = q < nc ? a.cinv[(long)tid * nc + q] : 0.0f;
     }
-    constexpr int NI = (VC_POOL / 4 + 1023) / 1024, NB = (64 * 64 + 1023) / 1024;
+    // the coefficient part is at most half of the pool
+    constexpr int NI = (VC_POOL / 8 + 1 + 1023) / 1024, NZ = NI, NB = (64 * 64 + 1023) / 1024;
     float4 img[NI];
 #pragma unroll
     for (int u = 0; u < NI; ++u) {
@@ -1466,6 +1491,12 @@ __global__ void __launch_bounds__(1024) k_lv_vcoarse(VcArgs a) {
     for (int u = 0; u < NI; ++u) {
         const int k = tid + u * bs;
         if (k < a.image_n4) reinterpret_cast<float4*>(pool)[k] = img[u];
+    }
+#pragma unroll
+    for (int u = 0; u < NZ; ++u) {
+        const int k = tid + u * bs;
+        if (k < a.zero_n4)
+            reinterpret_cast<float4*>(pool)[a.zero_off4 + k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     __syncthreads();
     // level Lc: b and the first sweep from zero, x = w b / d
