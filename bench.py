@@ -60,8 +60,7 @@ HBM_PEAK_GBS = 8000.0
 # measured HBM traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a 256 MiB
 # copy) over the algorithmic bytes of the same projection kernels
 TRAFFIC_RATIO_MGCG = 1.10        # profiles/k_project_traffic_cavity1024.md
-TRAFFIC_RATIO_LEARNED = 1.20     # network leg, profiles/learned_projection_cavity1024.md
-TRAFFIC_RATIO_VCYCLE = 1.154     # steps 6-25, profiles/learned_vcycle_projection_cavity1024.md
+TRAFFIC_RATIO_VCYCLE = 0.997     # steps 6-25, profiles/learned_vcycle_projection_cavity1024.md
 WORKLOAD = ("lid-driven cavity Re=1000, 1024x1024 MAC grid, AB2, gamma auto, tau 0.5, "
             "passive van Leer scalar (D=1e-3, 8 Gaussian blobs, seed 0), "
             "pressure solve to relative residual 1e-6")
@@ -213,15 +212,25 @@ def roofline_for(leg: dict) -> dict:
     s = 8.0
     n_it = leg["cg_iters_mean"]
     if leg["solver"].startswith("cg_learned"):
-        # per learned-FCG iteration and fine cell: V-cycle sweeps on level 0
-        # (down sweep 13 B, restriction 11 B, prolongation sweep 14 B, last sweep
-        # with the r.z and r_old.z partials 29 B), FP64 direction + A d (29 B),
-        # FP64 update of x, r, r_old plus the next V-cycle input (73 B); coarse
-        # levels add a third of the level-0 float traffic with their
-        # coefficients (21 B); the U-Net on the 32x32 level is negligible.
-        # The extrapolated initial iterate reads and writes x and x_prev (32 B)
+        # per fine cell, what each kernel must read and write once (FP64 8 B,
+        # FP32 4 B, cell codes 1 B, component ids 4 B):
+        # - start: the extrapolated iterate (x and x_prev read and written,
+        #   32 B, with x0 "extrapolate"), the right-hand side (F, G, codes,
+        #   components read, b written: 29 B), the initial residual (b, x,
+        #   codes, components read; b, r and the V-cycle input written: 45 B);
+        # - end: the x sum (13 B), the pressure write-back (21 B), the velocity
+        #   correction (F, G, p, codes, u, v read; u, v written: 57 B) and the
+        #   last iteration's true residual (b, x, codes read; r and the V-cycle
+        #   input written: 32 B);
+        # - per iteration: the level-0 down-leg (b, codes read; t and the
+        #   level-1 right-hand side written: 12 B), the level-0 up-leg (t, b,
+        #   codes, the coarse correction, r and r_old read; z written: 30 B),
+        #   the direction (z, d_old, codes read; d and q written: 29 B), the
+        #   update (x, d, r, q, codes read; x, r_old, r and the V-cycle input
+        #   written: 61 B), levels 1-3 with their coefficients (47 B per
+        #   coarse cell: 15 B per fine cell) and the coarse end (< 1 B).
         x0_bytes = 32.0 if leg.get("x0") == "extrapolate" else 0.0
-        bytes_per_launch = n_cells * (8.0 * (3 + 7) + x0_bytes + n_it * (169.0 + 21.0))
+        bytes_per_launch = n_cells * (197.0 + x0_bytes + n_it * 148.0)
     elif leg["solver"] == "cg_mixed":
         # per flexible-CG iteration: FP64 outer vectors (r, r_old, x, p, q: 104 B/cell)
         # plus the FP32 V-cycle (fine level 50 B/cell, coarse levels 1/3 more
@@ -237,14 +246,9 @@ def roofline_for(leg: dict) -> dict:
     if leg["solver"] == "cg_learned_vcycle":
         traffic = round(TRAFFIC_RATIO_VCYCLE * bytes_per_launch)
         source = ("rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE summed over the projection kernels "
-                  "of steps 6-25 (the --warmup 5 --steps 20 window): 1216.3 MB per step at "
-                  "4.7 iterations, 1.154 times the algorithmic bytes; "
+                  "of steps 6-25 (the --warmup 5 --steps 20 window): 827.1 MB per step at "
+                  "3.8 iterations, 0.997 times the algorithmic bytes; "
                   "profiles/learned_vcycle_projection_cavity1024.md")
-    elif learned:
-        traffic = round(TRAFFIC_RATIO_LEARNED * bytes_per_launch)
-        source = ("rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE summed over the projection kernels "
-                  "of 10 steps (ratio 1.20 to algorithmic), "
-                  "profiles/learned_projection_cavity1024.md")
     elif leg["solver"] == "cg_mixed":
         traffic = round(TRAFFIC_RATIO_MGCG * bytes_per_launch)
         source = ("rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE of 6-iteration launches "
@@ -535,6 +539,9 @@ def main() -> None:
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device-to-device (a known-size kernel for "
                          "scaling rocprof FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--ensemble-only", choices=("graph", "direct"), default=None,
+                    help="run only the 256-cavity ensemble with the V-cycle coarse end (one graph "
+                         "launch per step, or direct launches for profilers) and print its JSON")
     ap.add_argument("--no-graph", action="store_true",
                     help="learned legs launch their iteration kernels directly instead of "
                          "replaying a captured graph (for profilers that do not follow graphs)")
@@ -552,6 +559,9 @@ def main() -> None:
               "single-GPU in this build", file=sys.stderr)
         sys.exit(2)
 
+    if args.ensemble_only:
+        print(json.dumps(run_ensemble(coarse="vcycle", use_graph=args.ensemble_only == "graph")))
+        return
     if args.copy_calibration:
         copy_calibration()
     legs = {}
