@@ -138,7 +138,7 @@ __global__ void k_momentum_tendencies(const double* __restrict__ u, const double
 // ---------------------------------------------------------------------------
 // Fused predictor (solver.py:338-368, models.py:386-396, boundary.py:225-242)
 // ---------------------------------------------------------------------------
-__global__ void k_predictor(PredArgs a, Geo g) {
+__global__ void __launch_bounds__(256) k_predictor(PredArgs a, Geo g) {
     const int2 rc_ = xcd_rowgrid();
     const int j = rc_.x * blockDim.x + threadIdx.x;
     const int i = rc_.y;
@@ -334,7 +334,7 @@ __device__ __forceinline__ double bc_normal_of(int kind, const BCNormalIn& e) {
     }
 }
 
-__global__ void k_bc_domain(BCArgs a, Geo g) {
+__global__ void __launch_bounds__(1024) k_bc_domain(BCArgs a, Geo g) {
     const int nx = g.nx, ny = g.ny, pu = g.pu, pv = g.pv;
     double* u = a.u;
     double* v = a.v;
@@ -427,7 +427,7 @@ __global__ void k_bc_domain(BCArgs a, Geo g) {
 // two fluid cells, which this kernel never writes, so all faces update in
 // parallel with the reference's result.
 // ---------------------------------------------------------------------------
-__global__ void k_obstacle_bc(double* __restrict__ u, double* __restrict__ v,
+__global__ void __launch_bounds__(256) k_obstacle_bc(double* __restrict__ u, double* __restrict__ v,
                               const uint8_t* __restrict__ fl, Geo g, const DevState* st) {
     const int2 rc_ = xcd_rowgrid();
     const int j = rc_.x * blockDim.x + threadIdx.x;
@@ -503,7 +503,7 @@ __device__ __forceinline__ unsigned long long dbits(double x) {
 // fluid cells (models.py:204-208, turbulence.py:314-315).  Each block leaves
 // max |u|, max |v|, max nu_t, -min nu_t (fluid), max D_k, max D_omega in
 // part[6 b ..]; block 0 takes the maxima and sets the step.
-__global__ void k_dt_gamma(const double* __restrict__ u, const double* __restrict__ v, Geo g,
+__global__ void __launch_bounds__(256) k_dt_gamma(const double* __restrict__ u, const double* __restrict__ v, Geo g,
                            StepParams prm, TurbDt td, DevState* st, double* part,
                            unsigned* flags, SpinGuard spin) {
     __shared__ double lds[5 * 16];
@@ -781,7 +781,7 @@ __device__ __forceinline__ double yface_term(const ScalarArgs& a, const Geo& g, 
 //         zero off the fluid interior).
 // MODE 1: phi_out = phi_in + dt * rhs on fluid cells, phi_in elsewhere.
 template <bool CONST_D, int MODE>
-__global__ void k_scalar_transport(ScalarArgs a, Geo g) {
+__global__ void __launch_bounds__(256) k_scalar_transport(ScalarArgs a, Geo g) {
     const int2 rc_ = xcd_rowgrid();
     const int j = rc_.x * blockDim.x + threadIdx.x;
     const int i = rc_.y;
@@ -810,7 +810,7 @@ __global__ void k_scalar_transport(ScalarArgs a, Geo g) {
 // Ghost and solid cells of a scalar after its update (models.py:139-183).
 // phi_in still holds the previous values: ghost cells built from a solid edge
 // cell use its previous value, as the reference fills ghosts before solids.
-__global__ void k_fill_ghosts(ScalarArgs a, Geo g) {
+__global__ void __launch_bounds__(256) k_fill_ghosts(ScalarArgs a, Geo g) {
     const int2 rc_ = xcd_rowgrid();
     const int j = rc_.x * blockDim.x + threadIdx.x;
     const int i = rc_.y;
@@ -886,7 +886,7 @@ static_assert(REC_NV + 4 < CFD_MAX_MON, "the record holds the sums, nu_t, force,
 // NS: the number of transported fields (a compile-time count keeps the
 // per-thread sums and the block reductions as small as the case needs)
 template <int NS>
-__global__ void k_record(RecordArgs a, Geo g) {
+__global__ void __launch_bounds__(256) k_record(RecordArgs a, Geo g) {
     constexpr int NV = 10 + 5 * NS;
     __shared__ double lds[NV * 16];
     const int nx = g.nx, ny = g.ny, pc = g.pc;
