@@ -330,8 +330,8 @@ BCArgs bc_args(cfd_ctx* c, bool check_done) {
 }
 
 int launch_bc(cfd_ctx* c, bool check_done) {
-    hipLaunchKernelGGL(k_bc_domain, dim3(1), dim3(1024), 0, c->stream, bc_args(c, check_done),
-                       c->geo);
+    hipLaunchKernelGGL(k_bc_domain, dim3(bc_domain_blocks(c->nx)), dim3(256), 0, c->stream,
+                       bc_args(c, check_done), c->geo);
     HIPC(hipGetLastError());
     if (c->has_solid || c->periodic) {
         hipLaunchKernelGGL(k_obstacle_bc, grid2d(c), dim3(BS2D), 0, c->stream, c->u, c->v, c->fl,
@@ -1855,7 +1855,7 @@ int cfd_step(cfd_ctx* c, const cfd_step_params* prm, int32_t count) {
             HIPC(hipMemcpyAsync(c->flow.v0, c->v, cells(c) * sizeof(double),
                                 hipMemcpyDeviceToDevice, c->stream));
         }
-        hipLaunchKernelGGL(k_dt_gamma, dim3(c->red_blocks), dim3(256), 0,
+        hipLaunchKernelGGL(k_dt_gamma, dim3(c->red_blocks), dim3(RED_THREADS), 0,
                            c->stream, c->u, c->v, c->geo, sp, td, c->st, c->part, c->flags,
                            spin_guard(c));
         HIPC(hipGetLastError());
@@ -1907,7 +1907,7 @@ int cfd_step(cfd_ctx* c, const cfd_step_params* prm, int32_t count) {
         ra.steady_tol = prm->steady_tol;
         ra.max_steps = prm->max_steps;
         {
-            const dim3 gr(c->red_blocks), bl(256);
+            const dim3 gr(c->red_blocks), bl(RED_THREADS);
             switch (ra.nscalar) {
             case 0: hipLaunchKernelGGL(k_record<0>, gr, bl, 0, c->stream, ra, c->geo); break;
             case 1: hipLaunchKernelGGL(k_record<1>, gr, bl, 0, c->stream, ra, c->geo); break;

@@ -208,6 +208,14 @@ __global__ void k_momentum_tendencies(const double* u, const double* v, const ui
                                       double* hv);
 __global__ void k_predictor(PredArgs a, Geo g);
 __global__ void k_copy_predictor(PredArgs a, Geo g);
+// domain boundary conditions: work-group 0 and bc_domain_blocks(nx) - 1 more of
+// 256 threads, BC_COLS bottom / top columns each
+constexpr int BC_COLS = 64;
+// threads per work-group of the one-pass reductions k_dt_gamma and k_record,
+// one work-group per CU (1024 measured slower: 18 -> 26 and 29 -> 45 us at
+// 1024^2, the block reductions over 16 waves and block 0's gather)
+constexpr int RED_THREADS = 256;
+inline int bc_domain_blocks(int nx) { return 1 + (nx > 2 ? (nx - 2 + BC_COLS - 1) / BC_COLS : 0); }
 __global__ void k_bc_domain(BCArgs a, Geo g);
 __global__ void k_obstacle_bc(double* u, double* v, const uint8_t* fl, Geo g, const DevState* st);
 __global__ void k_dt_gamma(const double* u, const double* v, Geo g, StepParams prm, TurbDt td,

