@@ -945,7 +945,9 @@ __global__ void __launch_bounds__(512) k_fcg_update(LvArgs a, FcgVec v, int nb) 
 // End of an iteration: every work-group adds the r.r partials of
 // k_fcg_update and applies the recursive-residual test (work-group 0 records
 // it); once it passes (or at maxiter) the true residual r = b - A x is formed
-// with the next V-cycle input, and its partial r.r stored for k_fcg_true_fin.
+// with the next V-cycle input, and its partial r.r stored for k_fcg_true_fin
+// in the d.q slot (free after k_fcg_update): the r.r slot is still being read
+// by work-groups that start later.
 __global__ void __launch_bounds__(512) k_fcg_true(LvArgs a, FcgVec v, int nb) {
     FcgState* fs = a.fs;
     const int done = fs->done;   // in flight with the partials
@@ -994,7 +996,7 @@ __global__ void __launch_bounds__(512) k_fcg_true(LvArgs a, FcgVec v, int nb) {
             L0.b[c] = rf;
             if (a.store_x0) L0.x[c] = w * rf / m.diag0[o.li];
         });
-    lv_partial<1>(s, a.part + LV_P_RR * LV_SLOT);
+    lv_partial<1>(s, a.part + LV_P_PQ * LV_SLOT);
 }
 
 // one work-group: advance the iteration count; after a true residual, finish
@@ -1016,7 +1018,7 @@ __global__ void __launch_bounds__(256) k_fcg_true_fin(LvArgs a, int nb, int net_
     __syncthreads();
     if (!fs->pending) return;
     double rr[1];
-    lv_total<1>(a.part + LV_P_RR * LV_SLOT, nb, rr);
+    lv_total<1>(a.part + LV_P_PQ * LV_SLOT, nb, rr);
     if (threadIdx.x == 0) {
         fs->rr = rr[0];
         fs->reps += 1;

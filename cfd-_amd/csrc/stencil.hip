@@ -794,9 +794,89 @@ __device__ __forceinline__ double yface_term(const ScalarArgs& a, const Geo& g, 
     return 0.0;
 }
 
+// The loads of one face between two interior cells L and R (rows along x or
+// columns along y), with the upwind cells UL = L - 1 (for a velocity >= 0)
+// and UR = R + 1 (< 0), -1 where there is none: all of them issued before any
+// is used, so that a face costs one memory round trip instead of the chain
+// flags -> velocity -> upwind cell.  Absent cells load the cell R instead.
+struct FaceIn {
+    double vel, pL, pR, pUL, pUR, dL, dR;
+    int UL, UR;
+    bool fL, fR, fUL, fUR;
+};
+
+template <bool CONST_D>
+__device__ __forceinline__ FaceIn face_load(const ScalarArgs& a, double vel, long cL, long cR,
+                                            long cUL, long cUR, int UL, int UR) {
+    FaceIn q;
+    q.vel = vel;
+    q.UL = UL;
+    q.UR = UR;
+    const long kUL = UL >= 0 ? cUL : cR, kUR = UR >= 0 ? cUR : cR;
+    q.fL = a.fl[cL] & 1;
+    q.fR = a.fl[cR] & 1;
+    q.fUL = a.fl[kUL] & 1;
+    q.fUR = a.fl[kUR] & 1;
+    q.pL = a.phi_in[cL];
+    q.pR = a.phi_in[cR];
+    q.pUL = a.phi_in[kUL];
+    q.pUR = a.phi_in[kUR];
+    q.dL = CONST_D ? a.D0 : a.diff[cL];
+    q.dR = CONST_D ? a.D0 : a.diff[cR];
+    return q;
+}
+
+// x-face f (1 <= f, not the closed right boundary): L = f, R = f + 1 (1 past
+// nx when periodic), the upwind rows as xface_term picks them
+template <bool CONST_D>
+__device__ __forceinline__ FaceIn face_load_x(const ScalarArgs& a, const Geo& g, int f, int j) {
+    const int nx = g.nx, pc = g.pc;
+    const int L = f, R = f < nx ? f + 1 : 1;
+    int UL = L - 1, UR = R + 1;
+    if (UL < 1) UL = g.periodic ? nx : -1;
+    if (UR > nx) UR = g.periodic ? 1 : -1;
+    return face_load<CONST_D>(a, a.u[ix(f, j, g.pu)], ix(L, j, pc), ix(R, j, pc),
+                              ix(UL >= 0 ? UL : R, j, pc), ix(UR >= 0 ? UR : R, j, pc), UL, UR);
+}
+
+// y-face f (1 <= f <= ny - 1): B = f, T = f + 1, the upwind columns as
+// yface_term picks them
+template <bool CONST_D>
+__device__ __forceinline__ FaceIn face_load_y(const ScalarArgs& a, const Geo& g, int i, int f) {
+    const int ny = g.ny, pc = g.pc;
+    const int B = f, T = f + 1;
+    const int UB = B - 1 >= 1 ? B - 1 : -1, UT = T + 1 <= ny ? T + 1 : -1;
+    return face_load<CONST_D>(a, a.v[ix(i, f, g.pv)], ix(i, B, pc), ix(i, T, pc),
+                              ix(i, UB >= 0 ? UB : T, pc), ix(i, UT >= 0 ? UT : T, pc), UB, UT);
+}
+
+// the signed contribution of a loaded face to the fluid cell on its L side
+// (left) or R side, h the spacing: xface_term's / yface_term's arithmetic
+template <bool CONST_D>
+__device__ __forceinline__ double face_term(const ScalarArgs& a, const FaceIn& q, bool left,
+                                            double h) {
+    if (q.fL && q.fR) {
+        const bool pos = q.vel >= 0.0;
+        const int U = pos ? q.UL : q.UR;
+        const bool has_u = U >= 1 && (pos ? q.fUL : q.fUR);
+        const double phu = has_u ? (pos ? q.pUL : q.pUR) : 0.0;
+        const double phf = face_value(phu, pos ? q.pL : q.pR, pos ? q.pR : q.pL, has_u, a.limiter);
+        const double df = q.dL + q.dR > 0.0 ? 2.0 * q.dL * q.dR / (q.dL + q.dR) : 0.0;
+        const double F = q.vel * phf - df * (q.pR - q.pL) / h;
+        return left ? -(F / h) : F / h;
+    }
+    if (a.obs_t == 1) {
+        const double dP = left ? q.dL : q.dR, pP = left ? q.pL : q.pR;
+        return dP * (a.obs_v - pP) / (0.5 * h) / h;
+    }
+    return 0.0;
+}
+
 // MODE 0: write the transport right-hand side into a.out (kernels.py output,
 //         zero off the fluid interior).
 // MODE 1: phi_out = phi_in + dt * rhs on fluid cells, phi_in elsewhere.
+// The faces between interior cells are loaded first (face_load), the
+// boundary faces by xface_term / yface_term.
 template <bool CONST_D, int MODE>
 __global__ void __launch_bounds__(256) k_scalar_transport(ScalarArgs a, Geo g) {
     const int2 rc_ = xcd_rowgrid();
@@ -806,15 +886,26 @@ __global__ void __launch_bounds__(256) k_scalar_transport(ScalarArgs a, Geo g) {
     if (j > ny + 1) return;
     if (MODE == 1 && a.st->done) return;
     const bool inner = i >= 1 && i <= nx && j >= 1 && j <= ny;
-    const bool fluid = inner && (a.fl[ix(i, j, pc)] & 1);
+    bool fluid = false;
     double rhs = 0.0;
-    if (fluid) {
-        const int fl_face = (g.periodic && i == 1) ? nx : i - 1;
-        double out = xface_term<CONST_D>(a, g, fl_face, j, i);
-        out += xface_term<CONST_D>(a, g, i, j, i);
-        out += yface_term<CONST_D>(a, g, i, j - 1, j);
-        out += yface_term<CONST_D>(a, g, i, j, j);
-        rhs = out;
+    if (inner) {
+        fluid = a.fl[ix(i, j, pc)] & 1;
+        // west, east, south, north (the reference's order)
+        const int fw = (g.periodic && i == 1) ? nx : i - 1;
+        const bool bw = fw == 0, be = i == nx && !g.periodic, bs = j == 1, bn = j == ny;
+        FaceIn q[4];
+        if (!bw) q[0] = face_load_x<CONST_D>(a, g, fw, j);
+        if (!be) q[1] = face_load_x<CONST_D>(a, g, i, j);
+        if (!bs) q[2] = face_load_y<CONST_D>(a, g, i, j - 1);
+        if (!bn) q[3] = face_load_y<CONST_D>(a, g, i, j);
+        if (fluid) {
+            double out = bw ? xface_term<CONST_D>(a, g, fw, j, i)
+                            : face_term<CONST_D>(a, q[0], fw == i, g.dx);   // (fw == i: nx = 1)
+            out += be ? xface_term<CONST_D>(a, g, i, j, i) : face_term<CONST_D>(a, q[1], true, g.dx);
+            out += bs ? yface_term<CONST_D>(a, g, i, j - 1, j) : face_term<CONST_D>(a, q[2], false, g.dy);
+            out += bn ? yface_term<CONST_D>(a, g, i, j, j) : face_term<CONST_D>(a, q[3], true, g.dy);
+            rhs = out;
+        }
     }
     if (MODE == 0) {
         a.out[ix(i, j, pc)] = rhs;
