@@ -53,7 +53,11 @@ class LearnedConfig:
     staged in LDS) instead of two; ``fused_fine_up`` False keeps the fine
     level's up-leg as two launches, whose dot partials (row bands) make the
     solve bit-identical to the separate launches (tiles add them in another
-    order).  ``sweeps``: damped-Jacobi sweeps per leg on those levels (2: the
+    order).  ``fused_pair`` (mode "vcycle", with the fused legs): the two
+    levels just above the LDS coarse end in one launch per leg, each
+    work-group recomputing the cells it needs from the other level (closed
+    domains whose level sizes halve exactly; the same fields bit for bit).
+    ``sweeps``: damped-Jacobi sweeps per leg on those levels (2: the
     multigrid solver's V-cycle; 3 or 4: a stronger preconditioner, with the
     fused legs; "auto": 3 for a solve after one that took at least 4
     iterations, back to 2 after 8 consecutive 3-sweep solves of at most 2 --
@@ -74,6 +78,7 @@ class LearnedConfig:
     coarse_fused: bool = True
     fused_legs: bool = True
     fused_fine_up: bool = True
+    fused_pair: bool = True
     sweeps: int | str = "auto"
     x0: str = "previous"
     harvest_cap: int = 0
@@ -165,7 +170,8 @@ class Attached:
         d.max_iter = int(cfg.max_iter)
         d.fallback = 1 if cfg.fallback else 0
         d.use_graph = 1 if cfg.use_graph else 0
-        d.coarse_fused = ((1 | (2 if cfg.fused_legs else 0) | (0 if cfg.fused_fine_up else 4))
+        d.coarse_fused = ((1 | (2 if cfg.fused_legs else 0) | (0 if cfg.fused_fine_up else 4)
+                           | (8 if cfg.fused_pair else 0))
                           if cfg.coarse_fused else 0)
         if cfg.x0 not in ("extrapolate", "previous"):
             raise ValueError(f"x0 must be 'extrapolate' or 'previous', got {cfg.x0!r}")

@@ -154,6 +154,7 @@ struct cfd_ctx {
     int l_level = 0, l_mode = 0, l_cycles = 0, l_fallback = 1, l_graph = 1, l_maxiter = 200;
     int l_fused = 0, l_lc = 0;   // coarse levels l_lc .. l_level in one k_lv_coarse launch
     int l_vlds = 0;              // mode V-cycle: levels l_lc .. coarsest in LDS (k_lv_vcoarse)
+    int l_pair = 0;              // levels l_lc - 2 and l_lc - 1 in one launch per leg
     int l_legs = 0;              // levels 0 .. l_lc - 1: one launch per level and leg (2: not
                                  // the fine up-leg)
     int l_sweeps = 2;            // smoothing sweeps per leg on those levels (this solve)
@@ -782,6 +783,21 @@ LegKernel leg_up_kernel(int nu, bool lev0, bool big) {
     return nu == 4 ? leg_up_nu<4>(lev0, big) : nu == 3 ? leg_up_nu<3>(lev0, big) : leg_up_nu<2>(lev0, big);
 }
 
+// two coarse levels per leg launch (8 x 32 tiles of the coarser one)
+using LegKernel2 = void (*)(LegArgs, LegArgs);
+constexpr int LEG2_DOWN_TI = 8, LEG2_DOWN_TJ = 16;   // the down-leg's M tiles
+LegKernel2 leg_down2_kernel(int nu) {
+    return nu == 3 ? k_lv_down2<3, LEG2_DOWN_TI, LEG2_DOWN_TJ> : k_lv_down2<2, LEG2_DOWN_TI, LEG2_DOWN_TJ>;
+}
+constexpr int LEG2_UP_TI = 8, LEG2_UP_TJ = 16;   // the up-leg's M tiles
+LegKernel2 leg_up2_kernel(int nu) {
+    return nu == 3 ? k_lv_up2<3, LEG2_UP_TI, LEG2_UP_TJ> : k_lv_up2<2, LEG2_UP_TI, LEG2_UP_TJ>;
+}
+unsigned leg_grid2(const MGLevelDev& M, bool up) {
+    return (unsigned)std::min(leg_tiles_of(M, up ? LEG2_UP_TI : LEG2_DOWN_TI, up ? LEG2_UP_TJ : LEG2_DOWN_TJ),
+                              2048L);
+}
+
 FcgVec fcg_vec(cfd_ctx* c) {
     FcgVec v;
     v.x = c->x;
@@ -855,11 +871,18 @@ int learned_iteration(cfd_ctx* c) {
     // of 16 x 64 cells where they give at least 512 work-groups, else 8 x 32);
     // returns the work-group count of the level-0 up-leg (its dot partials)
     const int nu = c->l_sweeps;
+    const bool pair = c->l_pair && c->l_vlds && nu <= 3;   // (4 sweeps: the LDS region spills)
     auto legs_down = [&](int Lc) -> int {
-        for (int l = 0; l < Lc; ++l) {
+        for (int l = 0; l < (pair ? Lc - 2 : Lc); ++l) {
             const LegArgs g = leg_args(c, l);
             const bool big = leg_big(g.L);
             LV_LAUNCH(leg_down_kernel(nu, l == 0, big), leg_grid(g.L, big, 2048), g);
+        }
+        if (pair) {
+            const LegArgs f = leg_args(c, Lc - 2), m = leg_args(c, Lc - 1);
+            hipLaunchKernelGGL(leg_down2_kernel(nu), dim3(leg_grid2(m.L, false)), dim3(LEG2_DOWN_BS),
+                               0, c->stream, f, m);
+            HIPC(hipGetLastError());
         }
         return 0;
     };
@@ -874,7 +897,13 @@ int learned_iteration(cfd_ctx* c) {
             *nb_dots = nb0;
             return 0;
         }
-        for (int l = Lc - 1; l >= 0; --l) {
+        if (pair) {
+            const LegArgs m = leg_args(c, Lc - 1), f = leg_args(c, Lc - 2);
+            hipLaunchKernelGGL(leg_up2_kernel(nu), dim3(leg_grid2(m.L, true)), dim3(LEG2_UP_BS),
+                               0, c->stream, m, f);
+            HIPC(hipGetLastError());
+        }
+        for (int l = pair ? Lc - 3 : Lc - 1; l >= 0; --l) {
             const LegArgs g = leg_args(c, l);
             const bool big = leg_big(g.L);
             const unsigned nb = leg_grid(g.L, big, l == 0 ? LV_MAXB : 2048);
@@ -1529,6 +1558,12 @@ int cfd_set_learned(cfd_ctx* c, const cfd_learned_desc* d) {
         c->l_fused = 1;
     }
     c->l_legs = ((d->coarse_fused & 2) && (c->l_vlds || c->l_fused)) ? ((d->coarse_fused & 4) ? 2 : 1) : 0;
+    c->l_pair = 0;
+    if ((d->coarse_fused & 8) && c->l_vlds && c->l_legs == 1 && c->l_lc >= 3 && !c->periodic) {
+        const MGLevelDev& F = mg.lev[c->l_lc - 2];
+        const MGLevelDev& M = mg.lev[c->l_lc - 1];
+        c->l_pair = F.nx == 2 * M.nx && F.ny == 2 * M.ny && !leg_big(F) && !leg_big(M) ? 1 : 0;
+    }
     c->l_sweeps_cfg = d->sweeps < 0 ? -1 : d->sweeps > 2 ? d->sweeps : 2;
     c->l_sweeps = c->l_sweeps_cfg > 2 ? c->l_sweeps_cfg : 2;
     if (c->l_sweeps > 2 && c->l_legs != 1) {

@@ -621,11 +621,11 @@ __device__ __forceinline__ void leg_tiles(const MGLevelDev& L, int& ntj, int& t0
 // first tile of a coarse level's launch) the solve's finished flag is tested
 // only once the tile's loads are in flight (false: finished, nothing is
 // staged).
-template <bool LEV0, bool UP, int TI, int TJ, int R>
+template <bool LEV0, bool UP, int TI, int TJ, int R, int BS = 256>
 __device__ __forceinline__ bool leg_stage(const LegArgs& a, int ib, int jb, const float* s_d0,
                                           float* sb, float* sd, float* se, float* sn, float* sv,
                                           const int* gate) {
-    constexpr int W = TJ + 2 * R, N = (TI + 2 * R) * W, LA = (N + 255) / 256;
+    constexpr int W = TJ + 2 * R, N = (TI + 2 * R) * W, LA = (N + BS - 1) / BS;
     const MGLevelDev& L = a.L;
     const MGLevelDev& C = a.C;
     float rb[LA], rd[LA], re[LA], rn[LA], rt[LA], rc[LA];
@@ -633,7 +633,7 @@ __device__ __forceinline__ bool leg_stage(const LegArgs& a, int ib, int jb, cons
     bool ok[LA];
 #pragma unroll
     for (int u = 0; u < LA; ++u) {
-        const int e = (int)threadIdx.x + u * 256;
+        const int e = (int)threadIdx.x + u * BS;
         const int r = e / W, q = e - r * W;
         const int i = leg_row(ib + r, L.nx, a.periodic), j = jb + q;
         ok[u] = e < N && i >= 0 && i <= L.nx + 1 && j >= 0 && j <= L.ny + 1;
@@ -654,7 +654,7 @@ __device__ __forceinline__ bool leg_stage(const LegArgs& a, int ib, int jb, cons
     if (gate && *gate) return false;
 #pragma unroll
     for (int u = 0; u < LA; ++u) {
-        const int e = (int)threadIdx.x + u * 256;
+        const int e = (int)threadIdx.x + u * BS;
         if (e >= N) continue;
         const float d = LEV0 ? s_d0[rcode[u] & 63] : rd[u];
         sb[e] = rb[u];
@@ -675,15 +675,15 @@ __device__ __forceinline__ bool leg_stage(const LegArgs& a, int ib, int jb, cons
 
 // dst = src + w (b - A src) / d on the tile and its ring of RHO cells (0 off
 // the fluid), in an LDS region with a ring of R cells
-template <bool LEV0, int TI, int TJ, int R, int RHO>
+template <bool LEV0, int TI, int TJ, int R, int RHO, int BS = 256>
 __device__ __forceinline__ void leg_sweep(const LegArgs& a, const float* sb, const float* sd,
                                           const float* se, const float* sn, const float* src,
                                           float* dst) {
     constexpr int W = TJ + 2 * R, W1 = TJ + 2 * RHO, N1 = (TI + 2 * RHO) * W1;
-    constexpr int LS = (N1 + 255) / 256, O = R - RHO;
+    constexpr int LS = (N1 + BS - 1) / BS, O = R - RHO;
 #pragma unroll
     for (int u = 0; u < LS; ++u) {
-        const int e = (int)threadIdx.x + u * 256;
+        const int e = (int)threadIdx.x + u * BS;
         if (e >= N1) continue;
         const int r = e / W1, q = e - r * W1;
         const int k = (r + O) * W + q + O;
@@ -695,15 +695,15 @@ __device__ __forceinline__ void leg_sweep(const LegArgs& a, const float* sb, con
 
 // NS sweeps on the rings RHO, RHO - 1, .. alternating between p and q, each
 // followed by a barrier; the buffer that holds the last result
-template <bool LEV0, int TI, int TJ, int R, int RHO, int NS>
+template <bool LEV0, int TI, int TJ, int R, int RHO, int NS, int BS = 256>
 __device__ __forceinline__ float* leg_sweeps(const LegArgs& a, const float* sb, const float* sd,
                                              const float* se, const float* sn, float* p, float* q) {
     if constexpr (NS == 0) {
         return p;
     } else {
-        leg_sweep<LEV0, TI, TJ, R, RHO>(a, sb, sd, se, sn, p, q);
+        leg_sweep<LEV0, TI, TJ, R, RHO, BS>(a, sb, sd, se, sn, p, q);
         __syncthreads();
-        return leg_sweeps<LEV0, TI, TJ, R, RHO - 1, NS - 1>(a, sb, sd, se, sn, q, p);
+        return leg_sweeps<LEV0, TI, TJ, R, RHO - 1, NS - 1, BS>(a, sb, sd, se, sn, q, p);
     }
 }
 
@@ -838,6 +838,226 @@ __global__ void __launch_bounds__(256, (LEV0 && TI == 16 && NU == 3) ? 4 : 1) k_
         __syncthreads();
     }
     if (DOTS) lv_partial<2>(s2, a.part + LV_P_DOTS * LV_SLOT);
+}
+
+// ---- two coarse levels per leg launch -------------------------------------
+// Levels l (F, not level 0) and l + 1 (M) whose legs k_lv_down / k_lv_up
+// would take one launch each, in one launch per leg: one TI x TJ tile of M per
+// work-group (LEG2_DOWN_BS / LEG2_UP_BS threads).
+// Every cell value a work-group needs from the other level is recomputed in
+// its LDS from the same inputs by the same FP32 expressions as the
+// single-level kernels (the fields are bit for bit those of the two
+// launches): the down-leg smooths F on the region under the M tile and its
+// ring of R cells (plus F's own ring), restricts F's residual into M's b
+// there, then runs M's leg; the up-leg runs M's leg on the tile and a ring of
+// (R + 1) / 2 cells, whose t the F cells of the tile under it and their ring
+// prolongate.  The second level's global loads are issued with the first's.
+// Closed (not periodic) domains with nx(F) = 2 nx(M), ny(F) = 2 ny(M) only
+// (the host checks).
+constexpr int LEG2_DOWN_BS = 512, LEG2_UP_BS = 1024;
+
+template <int NU, int TI, int TJ>
+__global__ void __launch_bounds__(LEG2_DOWN_BS) k_lv_down2(LegArgs f, LegArgs m) {
+    constexpr int R = NU, BS = LEG2_DOWN_BS;
+    // F: the region under the M tile and its ring R, plus F's ring R
+    constexpr int FI = 2 * (TI + 2 * R), FJ = 2 * (TJ + 2 * R);
+    constexpr int FW = FJ + 2 * R;
+    constexpr int MW = TJ + 2 * R, MN = (TI + 2 * R) * MW, LM = (MN + BS - 1) / BS;
+    __shared__ float fb[(FI + 2 * R) * FW], fd[(FI + 2 * R) * FW], fe[(FI + 2 * R) * FW],
+        fn[(FI + 2 * R) * FW], fx[(FI + 2 * R) * FW], ft[(FI + 2 * R) * FW];
+    __shared__ float mb[MN], md[MN], me[MN], mn[MN], mx[MN], mt[MN];
+    const MGLevelDev& F = f.L;
+    const MGLevelDev& M = m.L;
+    const MGLevelDev& C = m.C;
+    const float w = f.w;
+    int ntj, t0, t1;
+    leg_tiles<TI, TJ>(M, ntj, t0, t1);
+    for (int tile = t0; tile < t1; ++tile) {
+        const int ci0 = (tile / ntj) * TI + 1, cj0 = (tile % ntj) * TJ + 1;   // M tile origin
+        const int mib = ci0 - R, mjb = cj0 - R;                             // M region origin
+        // M's coefficients on its region (leg_stage's loads), in flight with F's stage
+        float pd[LM], pe[LM], pn[LM];
+#pragma unroll
+        for (int u = 0; u < LM; ++u) {
+            const int e = (int)threadIdx.x + u * BS;
+            const int r = e / MW, q = e - r * MW;
+            const int i = mib + r, j = mjb + q;
+            const bool ok = e < MN && i >= 0 && i <= M.nx + 1 && j >= 0 && j <= M.ny + 1;
+            const long c = ok ? ix(i, j, M.P) : 0;
+            pd[u] = ok ? M.diag[c] : 0.0f;
+            pe[u] = ok ? M.cE[c] : 0.0f;
+            pn[u] = ok ? M.cN[c] : 0.0f;
+        }
+        // F cell of F-LDS cell (0, 0): the first child row of M row mib, less F's ring
+        const int fib = 2 * mib - 1 - R, fjb = 2 * mjb - 1 - R;
+        if (!leg_stage<false, false, FI, FJ, R, BS>(f, fib, fjb, nullptr, fb, fd, fe, fn, fx,
+                                                     tile == t0 ? f.done : nullptr))
+            return;
+        __syncthreads();
+        const float* tf = leg_sweeps<false, FI, FJ, R, R - 1, NU - 1, BS>(f, fb, fd, fe, fn, fx, ft);
+        // F's residual restricted: M's b on the M region (k_lv_down's loop),
+        // M's first iterate x = w b / d (leg_stage's expression)
+#pragma unroll
+        for (int u = 0; u < LM; ++u) {
+            const int e = (int)threadIdx.x + u * BS;
+            if (e >= MN) continue;
+            const int cr = e / MW, cs = e - cr * MW;
+            const int ci = mib + cr, cj = mjb + cs;
+            float bc = 0.0f;
+            if (ci >= 1 && cj >= 1 && ci <= M.nx && cj <= M.ny) {
+                for (int di = 0; di < 2; ++di)
+                    for (int dj = 0; dj < 2; ++dj) {
+                        const int i = 2 * ci - 1 + di, j = 2 * cj - 1 + dj;
+                        if (i > F.nx || j > F.ny) continue;
+                        const int k = (R + 2 * cr + di) * FW + R + 2 * cs + dj;
+                        const float d = fd[k];
+                        if (d == 0.0f) continue;
+                        bc += fb[k] - leg_apply<false>(f, k, FW, d, tf, fe, fn);
+                    }
+            }
+            const float d = pd[u];
+            mb[e] = bc;
+            md[e] = d;
+            me[e] = pe[u];
+            mn[e] = pn[u];
+            mx[e] = d != 0.0f ? m.w * bc / d : 0.0f;
+        }
+        __syncthreads();
+        const float* tm = leg_sweeps<false, TI, TJ, R, R - 1, NU - 1, BS>(m, mb, md, me, mn, mx, mt);
+        // M's residual restricted into level l + 2 (k_lv_down's loop)
+        const int ki0 = (tile / ntj) * (TI / 2) + 1, kj0 = (tile % ntj) * (TJ / 2) + 1;
+        constexpr int NC = (TI / 2) * (TJ / 2), LC = (NC + BS - 1) / BS;
+#pragma unroll
+        for (int u = 0; u < LC; ++u) {
+            const int e = (int)threadIdx.x + u * BS;
+            const int cr = e / (TJ / 2), cs = e - cr * (TJ / 2);
+            const int ci = ki0 + cr, cj = kj0 + cs;
+            if (e >= NC || ci > C.nx || cj > C.ny) continue;
+            float bc = 0.0f;
+            for (int di = 0; di < 2; ++di)
+                for (int dj = 0; dj < 2; ++dj) {
+                    const int i = 2 * ci - 1 + di, j = 2 * cj - 1 + dj;
+                    if (i > M.nx || j > M.ny) continue;
+                    const int k = (R + 2 * cr + di) * MW + R + 2 * cs + dj;
+                    const float d = md[k];
+                    if (d == 0.0f) continue;
+                    bc += mb[k] - leg_apply<false>(m, k, MW, d, tm, me, mn);
+                }
+            const long cc = ix(ci, cj, C.P);
+            C.b[cc] = bc;
+            const float dc = C.diag[cc];
+            C.x[cc] = dc != 0.0f ? w * bc / dc : 0.0f;
+        }
+        // M's b (its up-leg stages it) and both levels' t under the tile
+#pragma unroll
+        for (int u = 0; u < (TI * TJ + BS - 1) / BS; ++u) {
+            const int e = (int)threadIdx.x + u * BS;
+            const int r = e / TJ, q = e - r * TJ;
+            const int i = ci0 + r, j = cj0 + q;
+            const int k = (r + R) * MW + q + R;
+            if (e >= TI * TJ || i > M.nx || j > M.ny) continue;
+            M.b[ix(i, j, M.P)] = mb[k];
+            if (md[k] != 0.0f) m.tdown[ix(i, j, M.P)] = tm[k];
+        }
+#pragma unroll
+        for (int u = 0; u < (4 * TI * TJ + BS - 1) / BS; ++u) {
+            const int e = (int)threadIdx.x + u * BS;
+            const int r = e / (2 * TJ), q = e - r * (2 * TJ);
+            const int i = 2 * ci0 - 1 + r, j = 2 * cj0 - 1 + q;
+            const int k = (R + 2 * R + r) * FW + R + 2 * R + q;
+            if (e >= 4 * TI * TJ || i > F.nx || j > F.ny) continue;
+            if (fd[k] != 0.0f) f.tdown[ix(i, j, F.P)] = tf[k];
+        }
+        __syncthreads();   // the next tile overwrites the LDS arrays
+    }
+}
+
+template <int NU, int TI, int TJ>
+__global__ void __launch_bounds__(LEG2_UP_BS) k_lv_up2(LegArgs m, LegArgs f) {
+    constexpr int R = NU, RP = (R + 1) / 2, BS = LEG2_UP_BS;
+    // M: the tile and a ring of RP cells as the "tile", with M's own ring R
+    constexpr int EI = TI + 2 * RP, EJ = TJ + 2 * RP;
+    constexpr int MW = EJ + 2 * R, MN = (EI + 2 * R) * MW;
+    // F: the 2 TI x 2 TJ cells under the M tile, with F's ring R
+    constexpr int FI = 2 * TI, FJ = 2 * TJ, FW = FJ + 2 * R, FN = (FI + 2 * R) * FW;
+    constexpr int LA = (FN + BS - 1) / BS;
+    __shared__ float mb[MN], md[MN], me[MN], mn[MN], my[MN], mx[MN];
+    __shared__ float fb[FN], fd[FN], fe[FN], fn[FN], fy[FN], fx[FN];
+    const MGLevelDev& F = f.L;
+    const MGLevelDev& M = m.L;
+    int ntj, t0, t1;
+    leg_tiles<TI, TJ>(M, ntj, t0, t1);
+    for (int tile = t0; tile < t1; ++tile) {
+        const int ci0 = (tile / ntj) * TI + 1, cj0 = (tile % ntj) * TJ + 1;
+        const int mib = ci0 - RP - R, mjb = cj0 - RP - R;
+        const int fib = 2 * ci0 - 1 - R, fjb = 2 * cj0 - 1 - R;
+        // F's stage loads (k_lv_up's), in flight with M's
+        float rb[LA], rd[LA], re[LA], rn[LA], rt[LA];
+        int kc[LA];
+        bool ok[LA];
+#pragma unroll
+        for (int u = 0; u < LA; ++u) {
+            const int e = (int)threadIdx.x + u * BS;
+            const int r = e / FW, q = e - r * FW;
+            const int i = fib + r, j = fjb + q;
+            ok[u] = e < FN && i >= 0 && i <= F.nx + 1 && j >= 0 && j <= F.ny + 1;
+            const long c = ok[u] ? ix(i, j, F.P) : 0;
+            rb[u] = ok[u] ? F.b[c] : 0.0f;
+            rd[u] = ok[u] ? F.diag[c] : 0.0f;
+            re[u] = ok[u] ? F.cE[c] : 0.0f;
+            rn[u] = ok[u] ? F.cN[c] : 0.0f;
+            rt[u] = ok[u] ? f.tup[c] : 0.0f;
+            // the parent's cell in M's LDS region (inside it for every F cell staged)
+            kc[u] = (((i + 1) >> 1) - mib) * MW + (((j + 1) >> 1) - mjb);
+        }
+        // M: its up-leg on the extended tile (k_lv_up's stage, sweeps, last sweep)
+        if (!leg_stage<false, true, EI, EJ, R, BS>(m, mib, mjb, nullptr, mb, md, me, mn, my,
+                                                    tile == t0 ? m.done : nullptr))
+            return;
+        __syncthreads();
+        const float* xm = leg_sweeps<false, EI, EJ, R, R - 1, NU - 1, BS>(m, mb, md, me, mn, my, mx);
+        float* tm = xm == my ? mx : my;   // the last sweep's result goes to the other buffer
+        constexpr int NE = EI * EJ;
+#pragma unroll
+        for (int u = 0; u < (NE + BS - 1) / BS; ++u) {
+            const int e = (int)threadIdx.x + u * BS;
+            if (e >= NE) continue;
+            const int r = e / EJ, q = e - r * EJ;
+            const int k = (r + R) * MW + q + R;
+            const float d = md[k];
+            tm[k] = d != 0.0f ? xm[k] + m.w * (mb[k] - leg_apply<false>(m, k, MW, d, xm, me, mn)) / d
+                              : 0.0f;
+        }
+        __syncthreads();
+        // F: k_lv_up's stage with the prolongated M t from LDS
+#pragma unroll
+        for (int u = 0; u < LA; ++u) {
+            const int e = (int)threadIdx.x + u * BS;
+            if (e >= FN) continue;
+            const float d = rd[u];
+            fb[e] = rb[u];
+            fd[e] = d;
+            fe[e] = re[u];
+            fn[e] = rn[u];
+            const float rc = ok[u] ? tm[kc[u]] : 0.0f;
+            const float v = rt[u] + f.wc * rc;
+            fy[e] = d == 0.0f ? 0.0f : v;
+        }
+        __syncthreads();
+        const float* xf = leg_sweeps<false, FI, FJ, R, R - 1, NU - 1, BS>(f, fb, fd, fe, fn, fy, fx);
+#pragma unroll
+        for (int u = 0; u < (FI * FJ + BS - 1) / BS; ++u) {
+            const int e = (int)threadIdx.x + u * BS;
+            const int r = e / FJ, q = e - r * FJ;
+            const int i = fib + R + r, j = fjb + R + q;
+            const int k = (r + R) * FW + q + R;
+            if (e >= FI * FJ || i > F.nx || j > F.ny) continue;
+            const float d = fd[k];
+            if (d == 0.0f) continue;
+            F.t[ix(i, j, F.P)] = xf[k] + f.w * (fb[k] - leg_apply<false>(f, k, FW, d, xf, fe, fn)) / d;
+        }
+        __syncthreads();
+    }
 }
 
 // ---- flexible CG steps on the fine level (z = lev[0].t) ----

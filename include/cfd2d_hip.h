@@ -311,7 +311,10 @@ typedef struct cfd_learned_desc {
                                  and leg (the same FP32 arithmetic; the partial dot
                                  products of the fine up-leg in tile order); bit 2 (with
                                  bit 1): the fine level's up-leg still as two launches
-                                 (row-band partials: bit-identical to bit 1 clear) */
+                                 (row-band partials: bit-identical to bit 1 clear); bit 3
+                                 (with bit 1, bit 2 clear, mode 3): the two levels just
+                                 above the coarse end in one launch per leg (closed
+                                 domains whose sizes halve exactly; bit-identical) */
     int32_t x0_extrapolate;   /* 1: start each solve of a time step from the linear
                                  extrapolation x + (dt/dt_old)(x - x_previous) of the last
                                  two solutions; 0: from the last solution (SciPy's x0) */

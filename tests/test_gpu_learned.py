@@ -318,6 +318,24 @@ def test_vcycle_mode_is_the_multigrid_preconditioner(gpu):
     assert _cmp(lv.u, mg.u) < 1e-9 and _cmp(lv.p, mg.p) < 1e-7
 
 
+@pytest.mark.parametrize("sweeps", [2, 3])
+def test_pair_legs_equal_single_level_legs(gpu, sweeps):
+    """The two levels above the LDS coarse end in one launch per leg
+    (k_lv_down2 / k_lv_up2, each work-group recomputing what it needs from the
+    other level) against one launch per level and leg: the same FP32
+    arithmetic, bit-identical fields and iteration counts (512^2: levels
+    256^2 and 128^2 paired)."""
+    from cfd2d_amd.learned import LearnedConfig
+
+    out = []
+    for pair in (True, False):
+        out.append(_run(_cavity(512, "cg_learned", LearnedConfig(mode="vcycle", sweeps=sweeps,
+                                                                 fused_pair=pair)), 10))
+    a, b = out
+    assert a.cg_iterations == b.cg_iterations
+    assert np.array_equal(a.u, b.u) and np.array_equal(a.p, b.p)
+
+
 def test_vcycle_fused_equals_launches(gpu):
     """The complete V-cycle's coarse end in one work-group (levels <= 64^2 to
     the dense solve) is the same FP32 arithmetic as the separate launches:
