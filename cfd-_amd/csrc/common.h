@@ -152,11 +152,26 @@ __device__ __forceinline__ void block_sum(double (&val)[NV], double* lds) {
         if (lane == 0) lds[q * nw + wid] = s;
     }
     __syncthreads();
+    if (NV > 2) {
+        // thread q adds value q's wave sums (the same order as below) and
+        // leaves the total in the first slot: with many values, every thread
+        // reading every wave's sums cost more LDS bandwidth than a barrier
+        if ((int)threadIdx.x < NV) {
+            const int q = threadIdx.x;
+            double s = 0.0;
+            for (int w = 0; w < nw; ++w) s += lds[q * nw + w];
+            lds[q * nw] = s;
+        }
+        __syncthreads();
 #pragma unroll
-    for (int q = 0; q < NV; ++q) {
-        double s = 0.0;
-        for (int w = 0; w < nw; ++w) s += lds[q * nw + w];
-        val[q] = s;
+        for (int q = 0; q < NV; ++q) val[q] = lds[q * nw];
+    } else {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            double s = 0.0;
+            for (int w = 0; w < nw; ++w) s += lds[q * nw + w];
+            val[q] = s;
+        }
     }
     __syncthreads();
 }
@@ -172,11 +187,23 @@ __device__ __forceinline__ void block_max(double (&val)[NV], double* lds) {
         if (lane == 0) lds[q * nw + wid] = s;
     }
     __syncthreads();
+    if (NV > 2) {
+        if ((int)threadIdx.x < NV) {   // as in block_sum
+            const int q = threadIdx.x;
+            double s = lds[q * nw];
+            for (int w = 1; w < nw; ++w) s = nanmax(s, lds[q * nw + w]);
+            lds[q * nw] = s;
+        }
+        __syncthreads();
 #pragma unroll
-    for (int q = 0; q < NV; ++q) {
-        double s = lds[q * nw];
-        for (int w = 1; w < nw; ++w) s = nanmax(s, lds[q * nw + w]);
-        val[q] = s;
+        for (int q = 0; q < NV; ++q) val[q] = lds[q * nw];
+    } else {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            double s = lds[q * nw];
+            for (int w = 1; w < nw; ++w) s = nanmax(s, lds[q * nw + w]);
+            val[q] = s;
+        }
     }
     __syncthreads();
 }
