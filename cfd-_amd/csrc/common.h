@@ -124,9 +124,39 @@ __device__ __forceinline__ bool gather_block0(unsigned* flags, const SpinGuard& 
 // ---------------------------------------------------------------------------
 // Wave / block reductions (deterministic: fixed shuffle tree, fixed wave order)
 // ---------------------------------------------------------------------------
+// the value of lane (lane ^ M): M = 1, 2 by DPP quad permutes, 4, 8, 16 by
+// ds_swizzle's bit-mask mode (no LDS access), 32 by a permute across the
+// halves
+template <int M>
+__device__ __forceinline__ double lane_xor(double v) {
+    if constexpr (M == 32) {
+        return __shfl_xor(v, 32, CFD_WAVE);
+    } else {
+        const long long b = __double_as_longlong(v);
+        int lo = (int)(b & 0xffffffffLL), hi = (int)(b >> 32);
+        if constexpr (M == 1 || M == 2) {
+            constexpr int ctrl = M == 1 ? 0xB1 : 0x4E;   // quad_perm [1,0,3,2] / [2,3,0,1]
+            lo = __builtin_amdgcn_mov_dpp(lo, ctrl, 0xF, 0xF, true);
+            hi = __builtin_amdgcn_mov_dpp(hi, ctrl, 0xF, 0xF, true);
+        } else {
+            constexpr int pat = 0x1F | (M << 10);   // and 0x1f, or 0, xor M
+            lo = __builtin_amdgcn_ds_swizzle(lo, pat);
+            hi = __builtin_amdgcn_ds_swizzle(hi, pat);
+        }
+        return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    }
+}
+
+// the butterfly over lane distances 32, 16, .., 1: every lane ends with the
+// same sum (each step adds two equal-shaped partial sums, so the pairing, not
+// the operand order, fixes the result)
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, CFD_WAVE);
+    v += lane_xor<32>(v);
+    v += lane_xor<16>(v);
+    v += lane_xor<8>(v);
+    v += lane_xor<4>(v);
+    v += lane_xor<2>(v);
+    v += lane_xor<1>(v);
     return v;
 }
 // max that propagates NaN (as numpy's max does), so a blown-up field is seen
@@ -134,8 +164,12 @@ __device__ __forceinline__ double nanmax(double a, double b) {
     return (a != a) ? a : ((b != b) ? b : fmax(a, b));
 }
 __device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o, CFD_WAVE));
+    v = nanmax(v, lane_xor<32>(v));
+    v = nanmax(v, lane_xor<16>(v));
+    v = nanmax(v, lane_xor<8>(v));
+    v = nanmax(v, lane_xor<4>(v));
+    v = nanmax(v, lane_xor<2>(v));
+    v = nanmax(v, lane_xor<1>(v));
     return v;
 }
 
