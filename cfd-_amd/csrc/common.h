@@ -125,15 +125,21 @@ __device__ __forceinline__ bool gather_block0(unsigned* flags, const SpinGuard& 
 // Wave / block reductions (deterministic: fixed shuffle tree, fixed wave order)
 // ---------------------------------------------------------------------------
 // the value of lane (lane ^ M): M = 1, 2 by DPP quad permutes, 4, 8, 16 by
-// ds_swizzle's bit-mask mode (no LDS access), 32 by a permute across the
-// halves
+// ds_swizzle's bit-mask mode (no LDS access), 32 by gfx950's swap of the
+// wave's halves (v_permlane32_swap: the first result holds the lower half in
+// both halves, the second the upper half)
 template <int M>
 __device__ __forceinline__ double lane_xor(double v) {
+    const long long b = __double_as_longlong(v);
+    int lo = (int)(b & 0xffffffffLL), hi = (int)(b >> 32);
     if constexpr (M == 32) {
-        return __shfl_xor(v, 32, CFD_WAVE);
+        const bool low = (threadIdx.x & (CFD_WAVE - 1)) < 32;
+        const auto rl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+        const auto rh = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+        lo = (int)(low ? rl[1] : rl[0]);
+        hi = (int)(low ? rh[1] : rh[0]);
+        return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
     } else {
-        const long long b = __double_as_longlong(v);
-        int lo = (int)(b & 0xffffffffLL), hi = (int)(b >> 32);
         if constexpr (M == 1 || M == 2) {
             constexpr int ctrl = M == 1 ? 0xB1 : 0x4E;   // quad_perm [1,0,3,2] / [2,3,0,1]
             lo = __builtin_amdgcn_mov_dpp(lo, ctrl, 0xF, 0xF, true);
