@@ -60,7 +60,7 @@ HBM_PEAK_GBS = 8000.0
 # measured HBM traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a 256 MiB
 # copy) over the algorithmic bytes of the same projection kernels
 TRAFFIC_RATIO_MGCG = 1.10        # profiles/k_project_traffic_cavity1024.md
-TRAFFIC_RATIO_VCYCLE = 1.063     # steps 6-25, profiles/learned_vcycle_projection_cavity1024.md
+TRAFFIC_RATIO_VCYCLE = 0.999     # steps 6-25, profiles/learned_vcycle_projection_cavity1024.md
 WORKLOAD = ("lid-driven cavity Re=1000, 1024x1024 MAC grid, AB2, gamma auto, tau 0.5, "
             "passive van Leer scalar (D=1e-3, 8 Gaussian blobs, seed 0), "
             "pressure solve to relative residual 1e-6")
@@ -246,8 +246,8 @@ def roofline_for(leg: dict) -> dict:
     if leg["solver"] == "cg_learned_vcycle":
         traffic = round(TRAFFIC_RATIO_VCYCLE * bytes_per_launch)
         source = ("rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE summed over the projection kernels "
-                  "of steps 6-25 (the --warmup 5 --steps 20 window): 882.3 MB per step at "
-                  "3.8 iterations, 1.063 times the algorithmic bytes; "
+                  "of steps 6-25 (the --warmup 5 --steps 20 window): 828.9 MB per step at "
+                  "3.8 iterations, 0.999 times the algorithmic bytes; "
                   "profiles/learned_vcycle_projection_cavity1024.md")
     elif leg["solver"] == "cg_mixed":
         traffic = round(TRAFFIC_RATIO_MGCG * bytes_per_launch)
