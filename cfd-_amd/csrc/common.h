@@ -133,7 +133,7 @@ __device__ __forceinline__ double lane_xor(double v) {
     const long long b = __double_as_longlong(v);
     int lo = (int)(b & 0xffffffffLL), hi = (int)(b >> 32);
     if constexpr (M == 32) {
-        const bool low = (threadIdx.x & (CFD_WAVE - 1)) < 32;
+        const bool low = __lane_id() < 32;   // the hardware lane
         const auto rl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
         const auto rh = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
         lo = (int)(low ? rl[1] : rl[0]);
