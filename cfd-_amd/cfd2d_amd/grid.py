@@ -72,21 +72,22 @@ class Grid:
         self.fluid[1:-1, 1:-1] &= ~mask
 
     def finalize(self) -> int:
-        """Turn fluid pockets without a path to the domain edge into solid;
-        returns the number of cells removed (grid.py:100-125)."""
+        """Make fluid pockets solid: interior fluid cells with no path of
+        face neighbours through fluid to a cell of the interior's outer ring.
+        A single closed fluid region is left as it is.  Returns the number of
+        cells made solid (behaviour of grid.py:100-125)."""
         inner = self.fluid[1:-1, 1:-1]
-        lab, count = ndimage.label(inner)
-        if count <= 1:
+        ring = np.ones_like(inner)
+        ring[1:-1, 1:-1] = False
+        # flood fill from the ring's fluid cells over face neighbours
+        reached = ndimage.binary_propagation(ring & inner, mask=inner)
+        pocket = inner & ~reached
+        if not pocket.any():
             return 0
-        edge = np.zeros_like(inner)
-        edge[0, :] = edge[-1, :] = edge[:, 0] = edge[:, -1] = True
-        keep = np.unique(lab[edge & inner])
-        keep = keep[keep > 0]
-        pocket = inner & ~np.isin(lab, keep)
-        n = int(pocket.sum())
-        if n:
-            self.fluid[1:-1, 1:-1] &= ~pocket
-        return n
+        if not reached.any() and ndimage.label(inner)[1] <= 1:
+            return 0
+        self.fluid[1:-1, 1:-1] = reached
+        return int(pocket.sum())
 
     def set_periodic_x(self) -> None:
         self.periodic_x = True

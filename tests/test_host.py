@@ -127,6 +127,74 @@ def test_grid_finalize_removes_enclosed_fluid():
     assert g.solid[20, 20]
 
 
+def _pockets_by_search(inner):
+    """Fluid cells with no face-neighbour path through fluid to the outer ring,
+    by breadth-first search from every ring fluid cell; with the count of
+    fluid regions (a second search per unvisited cell)."""
+    nx, ny = inner.shape
+    seen = np.zeros_like(inner)
+    queue = [(i, j) for i in range(nx) for j in range(ny)
+             if inner[i, j] and (i in (0, nx - 1) or j in (0, ny - 1))]
+    for c in queue:
+        seen[c] = True
+    while queue:
+        i, j = queue.pop()
+        for a, b in ((i + 1, j), (i - 1, j), (i, j + 1), (i, j - 1)):
+            if 0 <= a < nx and 0 <= b < ny and inner[a, b] and not seen[a, b]:
+                seen[a, b] = True
+                queue.append((a, b))
+    label = np.zeros(inner.shape, int)
+    regions = 0
+    for i in range(nx):
+        for j in range(ny):
+            if inner[i, j] and not label[i, j]:
+                regions += 1
+                label[i, j] = regions
+                stack = [(i, j)]
+                while stack:
+                    p, q = stack.pop()
+                    for a, b in ((p + 1, q), (p - 1, q), (p, q + 1), (p, q - 1)):
+                        if 0 <= a < nx and 0 <= b < ny and inner[a, b] and not label[a, b]:
+                            label[a, b] = regions
+                            stack.append((a, b))
+    return inner & ~seen, regions
+
+
+def test_grid_finalize_matches_search():
+    """finalize against a breadth-first search on random obstacle masks, plus
+    the closed-region cases: one enclosed region stays, two enclosed regions
+    both go, diagonal contact is no connection."""
+    from cfd2d_amd.grid import Grid
+
+    rng = np.random.default_rng(5)
+    masks = [rng.random((n, m)) < f for n, m, f in
+             [(8, 8, 0.3), (9, 12, 0.45), (12, 7, 0.55), (10, 10, 0.6)] for _ in range(25)]
+    one = np.ones((8, 8), bool)
+    one[2:6, 2:6] = False                      # a single closed region
+    two = one.copy()
+    two[:, 4] = True                           # two closed regions of 8 and 4 cells
+    diag = np.zeros((6, 6), bool)
+    diag[1:4, 1:4] = True
+    diag[2, 2] = False                         # pocket at (2, 2) ...
+    diag[1, 1] = diag[3, 3] = False            # ... touching fluid only diagonally
+    masks += [one, two, diag]
+    for mask in masks:
+        g = Grid(*mask.shape, 1.0, 1.0)
+        g.add_obstacle(mask)
+        inner = g.fluid_interior.copy()
+        pocket, regions = _pockets_by_search(inner)
+        if regions <= 1:
+            pocket[:] = False
+        assert g.finalize() == int(pocket.sum())
+        assert np.array_equal(g.fluid_interior, inner & ~pocket)
+    g = Grid(8, 8, 1.0, 1.0)
+    g.add_obstacle(one)
+    assert g.finalize() == 0
+    g = Grid(8, 8, 1.0, 1.0)
+    g.add_obstacle(two)
+    assert g.finalize() == 12 and not g.fluid_interior.any()
+
+
 def test_validation_errors():
     from cfd2d_amd.boundary import BoundaryCondition as BC
     from cfd2d_amd.grid import Grid
