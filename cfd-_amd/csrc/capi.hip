@@ -1053,8 +1053,8 @@ int learned_end(cfd_ctx* c, const ProjArgs& a) {
     const LvArgs la = lv_args(c);
     const int nb = std::min(c->nx, 512);
     LV_LAUNCH512(k_lp_xsum, nb, a, c->geo, c->l_part);
-    LV_LAUNCH512(k_lp_pwrite, nb, a, c->geo, la, nb);
-    LV_LAUNCH512(k_lp_correct, nb, a, c->geo, c->l_part);
+    if (a.correct) LV_LAUNCH512(k_lp_end<true>, nb, a, c->geo, la, nb);
+    else LV_LAUNCH512(k_lp_end<false>, nb, a, c->geo, la, nb);
     LV_LAUNCH(k_lp_fin, 1, a, c->geo, c->l_part, nb, c->poll_d, ++c->poll_seq);
     if (a.zero_p_after) {
         const long n = (long)(c->nx + 2) * c->P;

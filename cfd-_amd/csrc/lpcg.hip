@@ -2058,12 +2058,22 @@ __global__ void __launch_bounds__(512) k_lp_xsum(ProjArgs a, Geo g, double* part
 }
 
 // p = x - mean on the fluid cells, 0 elsewhere and on the ghost columns; the
-// ghost rows 0 and nx + 1 (periodic copies of cells) by the first and last band
-__global__ void __launch_bounds__(512) k_lp_pwrite(ProjArgs a, Geo g, LvArgs la, int nb) {
+// ghost rows 0 and nx + 1 (periodic copies of cells) by the first and last
+// band.  With CORRECT also u = F - dt grad p, v = G - dt grad p (boundary faces
+// as in k_project) and the partial max |u_new - u_old| over the faces between
+// fluid cells.  The pressure of the neighbouring cells is formed from x here
+// too (the same expression as the stored p), so that no work-group waits for
+// another band's p.  Thread cells (i, j) of the band take the faces u(i, j)
+// and v(i, j) (4 at a time, their loads in flight together); the bottom faces
+// v(i, 0) and the left faces u(0, j) are the band's and the first band's
+// extra work.
+template <bool CORRECT>
+__global__ void __launch_bounds__(512) k_lp_end(ProjArgs a, Geo g, LvArgs la, int nb) {
     if (lp_end_idle(a)) return;
     double mean[CFD_MAX_COMP];
     lp_means(a, la.part + LV_P_CM * LV_SLOT, nb, mean);
     const int nx = g.nx, ny = g.ny, pc = g.pc;
+    const int periodic = g.periodic;
     auto pval = [&](int i, int j) -> double {   // loads and selects, no branches
         const long c = ix(i, j, pc);
         const int fl = a.code[c] & 63;
@@ -2072,100 +2082,92 @@ __global__ void __launch_bounds__(512) k_lp_pwrite(ProjArgs a, Geo g, LvArgs la,
         const double v = cm ? xv - sel_comp(mean, cm) : xv;
         return fl ? v : 0.0;
     };
+    auto fluid = [&](int i, int j) -> bool { return a.code[ix(i, j, pc)] & 63; };
     const int b = xcd_block(blockIdx.x, gridDim.x);
     int i0, i1;
     lp_band(nx, b, gridDim.x, i0, i1);
-    lv_cells_batch<4>(
-        nx, ny, [&](int i, int j) { return pval(i, j); },
-        [&](int i, int j, double v) { a.p[ix(i, j, pc)] = v; });
+    if (!CORRECT) {
+        lv_cells_batch<4>(
+            nx, ny, [&](int i, int j) { return pval(i, j); },
+            [&](int i, int j, double v) { a.p[ix(i, j, pc)] = v; });
+    } else {
+        const double dt = a.dt_override > 0.0 ? a.dt_override : a.st->dt;
+        double res[1] = {0.0};
+        struct Faces {
+            double p, un, vn, du, dv;
+            bool uf, vf, vset;
+        };
+        lv_cells_batch<4>(
+            nx, ny,
+            [&](int i, int j) {
+                Faces o;
+                const long cu = ix(i, j, g.pu), cv = ix(i, j, g.pv);
+                const double F = a.F[cu], G = a.G[cv];
+                o.p = pval(i, j);
+                o.uf = false;
+                if (i < nx) {
+                    o.uf = fluid(i, j) && fluid(i + 1, j);
+                    o.un = o.uf ? F - dt * (pval(i + 1, j) - o.p) / g.dx : F;
+                } else if (periodic) {
+                    o.uf = fluid(nx, j) && fluid(1, j);
+                    o.un = o.uf ? F - dt * (pval(1, j) - o.p) / g.dx : F;
+                } else {
+                    o.un = (a.code[ix(nx, j, pc)] & 64) ? F + 2 * dt * o.p / g.dx : F;
+                }
+                o.du = o.uf ? fabs(o.un - a.u[cu]) : 0.0;
+                o.vf = false;
+                o.vset = true;
+                if (j < ny) {
+                    o.vf = fluid(i, j) && fluid(i, j + 1);
+                    o.vn = o.vf ? G - dt * (pval(i, j + 1) - o.p) / g.dy : G;
+                } else if (periodic) {
+                    o.vset = false;   // solver.py:300-306 returns before the boundary faces
+                    o.vn = G;
+                } else {
+                    o.vn = (a.code[ix(i, ny, pc)] & 128) ? G + 2 * dt * o.p / g.dy : G;
+                }
+                o.dv = o.vf ? fabs(o.vn - a.v[cv]) : 0.0;
+                return o;
+            },
+            [&](int i, int j, const Faces& o) {
+                a.p[ix(i, j, pc)] = o.p;
+                if (o.uf) res[0] = nanmax(res[0], o.du);
+                a.u[ix(i, j, g.pu)] = o.un;
+                if (periodic && i == nx) a.u[ix(0, j, g.pu)] = o.un;
+                if (o.vf) res[0] = nanmax(res[0], o.dv);
+                if (o.vset) a.v[ix(i, j, g.pv)] = o.vn;
+            });
+        if (!periodic) {
+            for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {   // v(i, 0)
+                const long cv = ix(i, 0, g.pv);
+                const double G = a.G[cv];
+                a.v[cv] = (a.code[ix(i, 1, pc)] & 128) ? G - 2 * dt * pval(i, 1) / g.dy : G;
+            }
+        }
+        if (b == 0 && !periodic) {
+            for (int j = 1 + threadIdx.x; j <= ny; j += blockDim.x) {
+                const long cu = ix(0, j, g.pu);
+                const double F = a.F[cu];
+                a.u[cu] = (a.code[ix(1, j, pc)] & 64) ? F - 2 * dt * pval(1, j) / g.dx : F;
+            }
+        }
+        __shared__ double lds[16];
+        block_max<1>(res, lds);
+        if (threadIdx.x == 0) la.part[LV_P_PQ * LV_SLOT + blockIdx.x] = res[0];
+    }
     for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {   // the ghost columns
         a.p[ix(i, 0, pc)] = 0.0;
         a.p[ix(i, ny + 1, pc)] = 0.0;
     }
     if (b == 0)
         for (int j = threadIdx.x; j <= ny + 1; j += blockDim.x)
-            a.p[ix(0, j, pc)] = (g.periodic && j >= 1 && j <= ny) ? pval(nx, j) : 0.0;
+            a.p[ix(0, j, pc)] = (periodic && j >= 1 && j <= ny) ? pval(nx, j) : 0.0;
     if (b == (int)gridDim.x - 1)
         for (int j = threadIdx.x; j <= ny + 1; j += blockDim.x)
-            a.p[ix(nx + 1, j, pc)] = (g.periodic && j >= 1 && j <= ny) ? pval(1, j) : 0.0;
+            a.p[ix(nx + 1, j, pc)] = (periodic && j >= 1 && j <= ny) ? pval(1, j) : 0.0;
 }
-
-// u = F - dt grad p, v = G - dt grad p (boundary faces as in k_project), the
-// partial max |u_new - u_old| over the faces between fluid cells.  Thread
-// cells (i, j) of the band take the faces u(i, j) and v(i, j) (4 at a time,
-// their loads in flight together); the bottom faces v(i, 0) and the left
-// faces u(0, j) are the band's and the first band's extra work.
-__global__ void __launch_bounds__(512) k_lp_correct(ProjArgs a, Geo g, double* part) {
-    if (!a.correct || lp_end_idle(a)) return;
-    const double dt = a.dt_override > 0.0 ? a.dt_override : a.st->dt;
-    const int nx = g.nx, ny = g.ny, pc = g.pc;
-    const int periodic = g.periodic;
-    const double* p = a.p;
-    auto fluid = [&](int i, int j) -> bool { return a.code[ix(i, j, pc)] & 63; };
-    const int b = xcd_block(blockIdx.x, gridDim.x);
-    int i0, i1;
-    lp_band(nx, b, gridDim.x, i0, i1);
-    double res[1] = {0.0};
-    struct Faces {
-        double un, vn, du, dv;
-        bool uf, vf, vset;
-    };
-    lv_cells_batch<4>(
-        nx, ny,
-        [&](int i, int j) {
-            Faces o;
-            const long cu = ix(i, j, g.pu), cv = ix(i, j, g.pv);
-            const double F = a.F[cu], G = a.G[cv];
-            o.uf = false;
-            if (i < nx) {
-                o.uf = fluid(i, j) && fluid(i + 1, j);
-                o.un = o.uf ? F - dt * (p[ix(i + 1, j, pc)] - p[ix(i, j, pc)]) / g.dx : F;
-            } else if (periodic) {
-                o.uf = fluid(nx, j) && fluid(1, j);
-                o.un = o.uf ? F - dt * (p[ix(1, j, pc)] - p[ix(nx, j, pc)]) / g.dx : F;
-            } else {
-                o.un = (a.code[ix(nx, j, pc)] & 64) ? F + 2 * dt * p[ix(nx, j, pc)] / g.dx : F;
-            }
-            o.du = o.uf ? fabs(o.un - a.u[cu]) : 0.0;
-            o.vf = false;
-            o.vset = true;
-            if (j < ny) {
-                o.vf = fluid(i, j) && fluid(i, j + 1);
-                o.vn = o.vf ? G - dt * (p[ix(i, j + 1, pc)] - p[ix(i, j, pc)]) / g.dy : G;
-            } else if (periodic) {
-                o.vset = false;   // solver.py:300-306 returns before the boundary faces
-                o.vn = G;
-            } else {
-                o.vn = (a.code[ix(i, ny, pc)] & 128) ? G + 2 * dt * p[ix(i, ny, pc)] / g.dy : G;
-            }
-            o.dv = o.vf ? fabs(o.vn - a.v[cv]) : 0.0;
-            return o;
-        },
-        [&](int i, int j, const Faces& o) {
-            if (o.uf) res[0] = nanmax(res[0], o.du);
-            a.u[ix(i, j, g.pu)] = o.un;
-            if (periodic && i == nx) a.u[ix(0, j, g.pu)] = o.un;
-            if (o.vf) res[0] = nanmax(res[0], o.dv);
-            if (o.vset) a.v[ix(i, j, g.pv)] = o.vn;
-        });
-    if (!periodic) {
-        for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {   // v(i, 0)
-            const long cv = ix(i, 0, g.pv);
-            const double G = a.G[cv];
-            a.v[cv] = (a.code[ix(i, 1, pc)] & 128) ? G - 2 * dt * p[ix(i, 1, pc)] / g.dy : G;
-        }
-    }
-    if (b == 0 && !periodic) {
-        for (int j = 1 + threadIdx.x; j <= ny; j += blockDim.x) {
-            const long cu = ix(0, j, g.pu);
-            const double F = a.F[cu];
-            a.u[cu] = (a.code[ix(1, j, pc)] & 64) ? F - 2 * dt * p[ix(1, j, pc)] / g.dx : F;
-        }
-    }
-    __shared__ double lds[16];
-    block_max<1>(res, lds);
-    if (threadIdx.x == 0) part[LV_P_PQ * LV_SLOT + blockIdx.x] = res[0];
-}
+template __global__ void k_lp_end<false>(ProjArgs, Geo, LvArgs, int);
+template __global__ void k_lp_end<true>(ProjArgs, Geo, LvArgs, int);
 
 // one work-group: the solve's statistics into the step state, the residual
 // monitor, the periodic corner faces; then (finished or not) the flexible
