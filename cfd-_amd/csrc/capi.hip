@@ -1043,7 +1043,13 @@ int learned_capture(cfd_ctx* c) {
 int learned_start(cfd_ctx* c, const ProjArgs& a) {
     const LvArgs la = lv_args(c);
     const int nb = std::min(c->nx, 512);
-    LV_LAUNCH512(k_lp_rhs, nb, a, c->geo, c->l_part);
+    // the extrapolated initial iterate (x0_extrapolate's) with the right-hand side
+    int x0mode = 0;
+    if (c->l_x0 && a.dt_override == 0.0) {
+        x0mode = c->l_hist ? 2 : 1;
+        c->l_hist = 1;
+    }
+    LV_LAUNCH512(k_lp_rhs, nb, a, c->geo, c->l_part, c->xprev, x0mode);
     LV_LAUNCH512(k_lp_res, nb, a, c->geo, la, nb);
     LV_LAUNCH(k_lp_init, 1, a, la, nb);
     return 0;
@@ -1125,7 +1131,6 @@ int run_learned(cfd_ctx* c, ProjArgs a) {
         }
         HIPC(hipEventRecord(c->prof_ev[c->prof_used], c->stream));
     }
-    TRY(x0_extrapolate(c, a));
     TRY(learned_start(c, a));
     int batch = std::max(1, c->l_prev_iters);
     int launched = 0;

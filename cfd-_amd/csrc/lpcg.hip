@@ -1860,12 +1860,13 @@ __global__ void __launch_bounds__(1024) k_lv_vcoarse(VcArgs a) {
     }
 }
 
-// initial iterate of a time step's learned solve: x <- x + theta (x - x_prev)
-// with theta = dt / dt_old, and x_prev <- the old x (hist = 0: the copy only)
+// initial iterate of a time step's solve: x <- x + theta (x - x_prev) with
+// theta = dt / dt_old, and x_prev <- the old x (hist = 0: the copy only); a
+// step after the end of the run (st->done) leaves both as they are
 __global__ void __launch_bounds__(256) k_x0_extrap(double* __restrict__ x, double* __restrict__ xprev,
                                                    long n, const DevState* st, int hist) {
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+    if (k >= n || st->done) return;
     const double xn = x[k];
     if (hist) {
         const double th = st->dt_old > 0.0 ? st->dt / st->dt_old : 1.0;
@@ -1911,17 +1912,21 @@ __device__ __forceinline__ void lp_means(const ProjArgs& a, const double* slot, 
 }  // namespace
 
 // b = div(F, G) / dt (or the given right-hand side) on the fluid cells, and
-// the partial sums per Neumann component
-__global__ void __launch_bounds__(512) k_lp_rhs(ProjArgs a, Geo g, double* part) {
+// the partial sums per Neumann component; with x0mode the initial iterate as
+// k_x0_extrap forms it (1: x_prev <- x, 2: also x <- x + theta (x - x_prev)),
+// on the interior cells (x and x_prev are zero on the ghost ring)
+__global__ void __launch_bounds__(512) k_lp_rhs(ProjArgs a, Geo g, double* part,
+                                                double* __restrict__ xprev, int x0mode) {
     const DevState* st = a.st;
     if (st->done) return;
     const double dt = a.dt_override > 0.0 ? a.dt_override : st->dt;
+    const double th = st->dt_old > 0.0 ? st->dt / st->dt_old : 1.0;
     double cs[CFD_MAX_COMP];
 #pragma unroll
     for (int k = 0; k < CFD_MAX_COMP; ++k) cs[k] = 0.0;
     struct Rhs {
         int fluid, cm;
-        double rhs;
+        double rhs, x, xp;
     };
     lv_cells_batch<4>(
         g.nx, g.ny,
@@ -1937,9 +1942,16 @@ __global__ void __launch_bounds__(512) k_lp_rhs(ProjArgs a, Geo g, double* part)
                                    (a.G[ix(i, j, g.pv)] - a.G[ix(i, j - 1, g.pv)]) / g.dy;
                 o.rhs = div / dt;
             }
+            o.x = x0mode ? a.x[c] : 0.0;
+            o.xp = x0mode == 2 ? xprev[c] : 0.0;
             return o;
         },
         [&](int i, int j, const Rhs& o) {
+            if (x0mode) {
+                const long c = ix(i, j, g.pc);
+                if (x0mode == 2) a.x[c] = o.x + th * (o.x - o.xp);
+                xprev[c] = o.x;
+            }
             if (!o.fluid) return;
             a.b[ix(i, j, g.pc)] = o.rhs;
 #pragma unroll

@@ -64,6 +64,31 @@ def test_checkpoint_restart_is_exact(gpu, tmp_path, solver, numerik, extra):
     assert b.sim.history.kinetic_energy == ref.sim.history.kinetic_energy
 
 
+@pytest.mark.parametrize("solver,numerik", [
+    ("cg", {"pressure_x0": "extrapolate", "pressure_tol": 1e-6}),
+    ("cg_learned", {"pressure_tol": 1e-6, "learned": {"mode": "vcycle", "x0": "extrapolate"}}),
+])
+def test_steps_after_the_end_change_nothing(gpu, tmp_path, solver, numerik):
+    """A run to t_end launched in chunks of many steps idles through the steps
+    after t_end; its checkpoint, the initial iterates of the extrapolated start
+    included, equals that of the same run launched one step at a time."""
+    from cfd2d_amd import cases, functions
+
+    print(f"idle steps: {solver}", flush=True)
+    files = []
+    for one_by_one in (False, True):
+        c = cases.build_case(_channel_spec(solver, numerik))
+        kw = {"callback": lambda s: None, "callback_every": 1} if one_by_one else {}
+        assert c.sim.run(1.0, verbose=False, **kw) == "t_end"
+        functions.save_checkpoint(c.sim, tmp_path / f"cp{int(one_by_one)}.npz")
+        files.append(np.load(tmp_path / f"cp{int(one_by_one)}.npz"))
+    a, b = files
+    assert 10 < int(a["n"]) < 200 and int(a["n"]) == int(b["n"])
+    assert "druck_xprev" in a.files and sorted(a.files) == sorted(b.files)
+    for k in a.files:
+        assert np.array_equal(a[k], b[k]), k
+
+
 def test_checkpoint_layout_matches_reference(gpu, tmp_path):
     from cfd2d_amd import cases, functions
 
