@@ -60,7 +60,7 @@ HBM_PEAK_GBS = 8000.0
 # measured HBM traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a 256 MiB
 # copy) over the algorithmic bytes of the same projection kernels
 TRAFFIC_RATIO_MGCG = 1.10        # profiles/k_project_traffic_cavity1024.md
-TRAFFIC_RATIO_VCYCLE = 0.999     # steps 6-25, profiles/learned_vcycle_projection_cavity1024.md
+TRAFFIC_RATIO_VCYCLE = 1.014     # steps 6-25, profiles/learned_vcycle_projection_cavity1024.md
 WORKLOAD = ("lid-driven cavity Re=1000, 1024x1024 MAC grid, AB2, gamma auto, tau 0.5, "
             "passive van Leer scalar (D=1e-3, 8 Gaussian blobs, seed 0), "
             "pressure solve to relative residual 1e-6")
@@ -213,15 +213,15 @@ def roofline_for(leg: dict) -> dict:
     n_it = leg["cg_iters_mean"]
     if leg["solver"].startswith("cg_learned"):
         # per fine cell, what each kernel must read and write once (FP64 8 B,
-        # FP32 4 B, cell codes 1 B, component ids 4 B):
+        # FP32 4 B, cell codes and component ids 1 B):
         # - start: the extrapolated iterate (x and x_prev read and written,
         #   32 B, with x0 "extrapolate"), the right-hand side (F, G, codes,
-        #   components read, b written: 29 B), the initial residual (b, x,
-        #   codes, components read; b, r and the V-cycle input written: 45 B);
-        # - end: the x sum (13 B), the pressure write-back (21 B), the velocity
-        #   correction (F, G, p, codes, u, v read; u, v written: 57 B) and the
-        #   last iteration's true residual (b, x, codes read; r and the V-cycle
-        #   input written: 32 B);
+        #   components read, b written: 26 B), the initial residual (b, x,
+        #   codes, components read; b, r and the V-cycle input written: 42 B);
+        # - end: the x sum (10 B), the pressure write-back with the velocity
+        #   correction (x, codes, components, F, G, u, v read; p, u, v written:
+        #   66 B) and the last iteration's true residual (b, x, codes read; r
+        #   and the V-cycle input written: 32 B);
         # - per iteration: the level-0 down-leg (b, codes read; t and the
         #   level-1 right-hand side written: 12 B), the level-0 up-leg (t, b,
         #   codes, the coarse correction, r and r_old read; z written: 30 B),
@@ -230,7 +230,7 @@ def roofline_for(leg: dict) -> dict:
         #   written: 61 B), levels 1-3 with their coefficients (47 B per
         #   coarse cell: 15 B per fine cell) and the coarse end (< 1 B).
         x0_bytes = 32.0 if leg.get("x0") == "extrapolate" else 0.0
-        bytes_per_launch = n_cells * (197.0 + x0_bytes + n_it * 148.0)
+        bytes_per_launch = n_cells * (176.0 + x0_bytes + n_it * 148.0)
     elif leg["solver"] == "cg_mixed":
         # per flexible-CG iteration: FP64 outer vectors (r, r_old, x, p, q: 104 B/cell)
         # plus the FP32 V-cycle (fine level 50 B/cell, coarse levels 1/3 more
@@ -246,8 +246,8 @@ def roofline_for(leg: dict) -> dict:
     if leg["solver"] == "cg_learned_vcycle":
         traffic = round(TRAFFIC_RATIO_VCYCLE * bytes_per_launch)
         source = ("rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE summed over the projection kernels "
-                  "of steps 6-25 (the --warmup 5 --steps 20 window): 828.9 MB per step at "
-                  "3.8 iterations, 0.999 times the algorithmic bytes; "
+                  "of steps 6-25 (the --warmup 5 --steps 20 window): 819.1 MB per step at "
+                  "3.8 iterations, 1.014 times the algorithmic bytes; "
                   "profiles/learned_vcycle_projection_cavity1024.md")
     elif leg["solver"] == "cg_mixed":
         traffic = round(TRAFFIC_RATIO_MGCG * bytes_per_launch)
@@ -258,8 +258,9 @@ def roofline_for(leg: dict) -> dict:
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "traffic_source": source,
-            "kernel": ("learned projection: the start and end launches (k_lp_*), the "
-                       "extrapolated initial iterate and the flexible-CG iteration kernels"
+            "kernel": ("learned projection: the start and end launches (k_lp_*, the "
+                       "extrapolated initial iterate in k_lp_rhs) and the flexible-CG "
+                       "iteration kernels"
                        if learned else "k_project"),
             "bytes_per_launch": bytes_per_launch,
             "launch_ms": round(leg["proj_ms_mean"], 4), "cg_iters_per_launch": round(n_it, 2)}
