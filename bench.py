@@ -556,8 +556,8 @@ def main() -> None:
         print(json.dumps(cpu_child(mode, arg, float(budget))))
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path is "
-              "single-GPU in this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the projection "
+              "step and its benchmark use a single GPU", file=sys.stderr)
         sys.exit(2)
 
     if args.ensemble_only:
